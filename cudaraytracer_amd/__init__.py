@@ -1,0 +1,259 @@
+"""MI355X wavefront path tracer: Python mirror of the reference's Scene / Renderer.
+
+The reference drives rendering as ``Scene scene(path); Renderer renderer(ui);
+renderer.RenderLoop();`` (Src/main.cu:26-33).  Here the same three steps are
+
+    scene = Scene("scene.json")            # Scene::Scene (Src/Core/scene.cpp:50-57)
+    r = Renderer(scene)                     # Renderer::Renderer (Src/Core/renderer.cuh:1094-1100)
+    r.render(spp)                           # RenderLoop frames, one sample per pixel each
+    img = r.read_accum(RT_ACCUM_MEAN)       # renderBuffers read-back
+
+Everything runs in the HIP library ``librtamd.so`` through its C ABI
+(include/rt.h); there is no CPU fallback: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librtamd.so")
+
+RT_OK = 0
+RT_ACCUM_DIRECT = 0
+RT_ACCUM_INDIRECT = 1
+RT_ACCUM_MEAN = 2
+STAGES = ("raygen", "extend", "shade", "shadow", "resolve")
+
+# C symbols declared in include/rt.h (checked by the test suite).
+EXPORTS = (
+    "rt_render_params_default", "rt_scene_load", "rt_scene_info_get", "rt_scene_destroy",
+    "rt_renderer_create", "rt_renderer_destroy", "rt_render", "rt_render_counted", "rt_synchronize",
+    "rt_reset", "rt_reset_stats", "rt_read_accum", "rt_read_rgba8", "rt_stats", "rt_last_error",
+    "rt_scene_get_triangles", "rt_scene_get_meshes", "rt_scene_get_bvh", "rt_trace_rays", "rt_device_math",
+)
+
+
+class RtError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"rt error {code}: {msg}")
+        self.code = code
+
+
+class SceneInfo(ctypes.Structure):
+    _fields_ = [
+        ("num_triangles", ctypes.c_int), ("num_meshes", ctypes.c_int), ("num_materials", ctypes.c_int),
+        ("num_lights", ctypes.c_int), ("num_bvh_nodes", ctypes.c_int), ("bvh_max_depth", ctypes.c_int),
+        ("bvh_fallback_splits", ctypes.c_int), ("width", ctypes.c_int), ("height", ctypes.c_int),
+        ("spp", ctypes.c_int), ("max_bounces", ctypes.c_int), ("rr", ctypes.c_float), ("denoise", ctypes.c_int),
+        ("seed", ctypes.c_uint32), ("rr_start_depth", ctypes.c_int),
+        ("bounds_lo", ctypes.c_float * 3), ("bounds_hi", ctypes.c_float * 3),
+        ("camera_position", ctypes.c_float * 3), ("camera_left_down", ctypes.c_float * 3),
+        ("camera_du", ctypes.c_float * 3), ("camera_dv", ctypes.c_float * 3),
+    ]
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [
+        ("max_bounces", ctypes.c_int), ("rr", ctypes.c_float), ("rr_start_depth", ctypes.c_int),
+        ("seed", ctypes.c_int64), ("batch_spp", ctypes.c_int), ("queue_budget_bytes", ctypes.c_int64),
+        ("flush_survivors", ctypes.c_int), ("profile", ctypes.c_int),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("extension_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
+        ("extension_node_visits", ctypes.c_uint64), ("extension_tri_tests", ctypes.c_uint64),
+        ("shadow_node_visits", ctypes.c_uint64), ("shadow_tri_tests", ctypes.c_uint64),
+        ("probe_rays", ctypes.c_uint64), ("stage_ms", ctypes.c_double * 5), ("stage_launches", ctypes.c_int * 5),
+        ("spp_done", ctypes.c_int), ("batch_spp", ctypes.c_int),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("stage_ms", "stage_launches")}
+        d["stage_ms"] = {s: self.stage_ms[i] for i, s in enumerate(STAGES)}
+        d["stage_launches"] = {s: self.stage_launches[i] for i, s in enumerate(STAGES)}
+        return d
+
+
+_LIB: Optional[ctypes.CDLL] = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load librtamd.so (built in-tree by ``python -m cudaraytracer_amd.build``)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing; build it with `python -m cudaraytracer_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, fp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_float)
+    L.rt_last_error.restype = ctypes.c_char_p
+    L.rt_render_params_default.argtypes = [ctypes.POINTER(RenderParams)]
+    L.rt_render_params_default.restype = None
+    L.rt_scene_load.argtypes = [ctypes.c_char_p, vp, ctypes.POINTER(vp)]
+    L.rt_scene_info_get.argtypes = [vp, ctypes.POINTER(SceneInfo)]
+    L.rt_scene_destroy.argtypes = [vp]
+    L.rt_scene_destroy.restype = None
+    L.rt_renderer_create.argtypes = [vp, ctypes.POINTER(RenderParams), ctypes.POINTER(vp)]
+    L.rt_renderer_destroy.argtypes = [vp]
+    L.rt_renderer_destroy.restype = None
+    for name in ("rt_render", "rt_render_counted"):
+        getattr(L, name).argtypes = [vp, ip]
+    for name in ("rt_synchronize", "rt_reset", "rt_reset_stats"):
+        getattr(L, name).argtypes = [vp]
+    L.rt_read_accum.argtypes = [vp, ip, fp]
+    L.rt_read_rgba8.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8)]
+    L.rt_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.rt_scene_get_triangles.argtypes = [vp, fp, fp, fp, ctypes.POINTER(ctypes.c_int32)]
+    L.rt_scene_get_meshes.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), fp]
+    L.rt_scene_get_bvh.argtypes = [vp, ip, vp, ctypes.POINTER(ctypes.c_int32)]
+    L.rt_trace_rays.argtypes = [vp, fp, ip, fp, ctypes.POINTER(ctypes.c_uint32)]
+    L.rt_device_math.argtypes = [ip, fp, fp, fp, ip]
+    _LIB = L
+    return L
+
+
+def _check(code: int) -> None:
+    if code != RT_OK:
+        raise RtError(code, lib().rt_last_error().decode(errors="replace"))
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+class Scene:
+    """A loaded scene: JSON + OBJ (+ environment map), SAH BVH built on the host."""
+
+    def __init__(self, json_path: str):
+        self._h = ctypes.c_void_p()
+        _check(lib().rt_scene_load(os.fsencode(json_path), None, ctypes.byref(self._h)))
+        self.path = json_path
+
+    def close(self) -> None:
+        if self._h:
+            lib().rt_scene_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> SceneInfo:
+        out = SceneInfo()
+        _check(lib().rt_scene_info_get(self._h, ctypes.byref(out)))
+        return out
+
+    def triangles(self):
+        n = self.info().num_triangles
+        pos = np.zeros((n, 9), np.float32)
+        nrm = np.zeros((n, 9), np.float32)
+        uv = np.zeros((n, 6), np.float32)
+        mesh = np.zeros(n, np.int32)
+        _check(lib().rt_scene_get_triangles(self._h, _fptr(pos), _fptr(nrm), _fptr(uv), _iptr(mesh)))
+        return pos, nrm, uv, mesh
+
+    def meshes(self):
+        m = self.info().num_meshes
+        mesh4 = np.zeros((m, 4), np.int32)
+        area = np.zeros(m, np.float32)
+        _check(lib().rt_scene_get_meshes(self._h, _iptr(mesh4), _fptr(area)))
+        return mesh4, area
+
+    def bvh(self, from_packed: bool = False):
+        info = self.info()
+        nodes = np.zeros((info.num_bvh_nodes, 8), np.int32)
+        order = np.zeros(info.num_triangles, np.int32)
+        _check(lib().rt_scene_get_bvh(self._h, int(from_packed), nodes.ctypes.data_as(ctypes.c_void_p), _iptr(order)))
+        return nodes, order
+
+    def trace(self, rays: np.ndarray):
+        """Closest hits on the GPU. rays: (n, 8) float32 = o.xyz, tmin, d.xyz, tmax."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        hits = np.zeros((n, 4), np.float32)
+        counts = np.zeros((n, 2), np.uint32)
+        _check(lib().rt_trace_rays(self._h, _fptr(rays), n, _fptr(hits),
+                                   counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        prim = hits[:, 3].copy().view(np.int32)
+        return hits[:, 0].copy(), hits[:, 1].copy(), hits[:, 2].copy(), prim, counts
+
+
+class Renderer:
+    """Wavefront renderer for one scene; owns its HIP stream and device buffers."""
+
+    def __init__(self, scene: Scene, max_bounces: int = -1, rr: float = -1.0, rr_start_depth: int = -1,
+                 seed: int = -1, batch_spp: int = 0, queue_budget_bytes: int = 0, flush_survivors: bool = False,
+                 profile: bool = False):
+        p = RenderParams()
+        lib().rt_render_params_default(ctypes.byref(p))
+        p.max_bounces, p.rr, p.rr_start_depth, p.seed = max_bounces, rr, rr_start_depth, seed
+        p.batch_spp, p.queue_budget_bytes = batch_spp, queue_budget_bytes
+        p.flush_survivors, p.profile = int(flush_survivors), int(profile)
+        self.scene = scene
+        self._h = ctypes.c_void_p()
+        _check(lib().rt_renderer_create(scene.handle, ctypes.byref(p), ctypes.byref(self._h)))
+        info = scene.info()
+        self.width, self.height = info.width, info.height
+
+    def close(self) -> None:
+        if self._h:
+            lib().rt_renderer_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, spp: int, count_traversal: bool = False) -> None:
+        fn = lib().rt_render_counted if count_traversal else lib().rt_render
+        _check(fn(self._h, int(spp)))
+
+    def synchronize(self) -> None:
+        _check(lib().rt_synchronize(self._h))
+
+    def reset(self) -> None:
+        _check(lib().rt_reset(self._h))
+
+    def reset_stats(self) -> None:
+        _check(lib().rt_reset_stats(self._h))
+
+    def read_accum(self, which: int = RT_ACCUM_MEAN) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        _check(lib().rt_read_accum(self._h, int(which), _fptr(out)))
+        return out
+
+    def read_rgba8(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        _check(lib().rt_read_rgba8(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return out
+
+    def stats(self) -> Stats:
+        s = Stats()
+        _check(lib().rt_stats(self._h, ctypes.byref(s)))
+        return s
+
+
+def device_math(op: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    out = np.zeros_like(a)
+    _check(lib().rt_device_math(int(op), _fptr(a), _fptr(b), _fptr(out), a.size))
+    return out

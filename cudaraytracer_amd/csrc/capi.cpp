@@ -1,0 +1,316 @@
+// extern "C" boundary: opaque handles, return codes, thread-local last error.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/rt.h"
+#include "json.h"
+#include "render.h"
+#include "scene.h"
+
+struct rt_scene {
+    rt::HostScene host;
+    std::unique_ptr<rt::GpuScene> gpu;  // created on first GPU use
+};
+
+struct rt_renderer {
+    rt_scene* scene = nullptr;
+    std::unique_ptr<rt::Renderer> r;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        g_err.clear();
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(RT_ERR_OUT_OF_MEMORY, "out of host memory");
+    } catch (const rt::JsonError& e) {
+        return fail(RT_ERR_SCENE, e.what());
+    } catch (const std::runtime_error& e) {
+        std::string m = e.what();
+        if (m.rfind("HIP error", 0) == 0) {
+            if (m.find("out of memory") != std::string::npos) return fail(RT_ERR_OUT_OF_MEMORY, m);
+            return fail(RT_ERR_DEVICE, m);
+        }
+        if (m.rfind("cannot open", 0) == 0) return fail(RT_ERR_IO, m);
+        return fail(RT_ERR_SCENE, m);
+    } catch (const std::exception& e) {
+        return fail(RT_ERR_INTERNAL, e.what());
+    }
+}
+
+void ensure_gpu(rt_scene* s) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw std::runtime_error("HIP error: no GPU device available");
+    if (!s->gpu) s->gpu.reset(new rt::GpuScene(s->host));
+}
+}  // namespace
+
+extern "C" {
+
+void rt_render_params_default(rt_render_params* p) {
+    if (!p) return;
+    p->max_bounces = -1;
+    p->rr = -1.f;
+    p->rr_start_depth = -1;
+    p->seed = -1;
+    p->batch_spp = 0;
+    p->queue_budget_bytes = 0;
+    p->flush_survivors = 0;
+    p->profile = 0;
+}
+
+const char* rt_last_error(void) { return g_err.c_str(); }
+
+int rt_scene_load(const char* json_path, const rt_load_opts* opts, rt_scene** out) {
+    if (!json_path || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
+    if (opts && opts->reserved != 0) return fail(RT_ERR_INVALID_ARG, "rt_load_opts.reserved must be 0");
+    *out = nullptr;
+    return guard([&]() {
+        std::unique_ptr<rt_scene> s(new rt_scene);
+        rt::load_scene(json_path, s->host);
+        *out = s.release();
+        return RT_OK;
+    });
+}
+
+int rt_scene_info_get(const rt_scene* s, rt_scene_info* o) {
+    if (!s || !o) return fail(RT_ERR_INVALID_ARG, "null argument");
+    const rt::HostScene& h = s->host;
+    std::memset(o, 0, sizeof(*o));
+    o->num_triangles = h.num_tris();
+    o->num_meshes = (int)h.meshes.size();
+    o->num_materials = (int)h.materials.size();
+    o->num_lights = (int)h.lights.size();
+    o->num_bvh_nodes = (int)h.bvh.nodes.size();
+    o->bvh_max_depth = h.bvh.max_depth;
+    o->bvh_fallback_splits = h.bvh.fallback_splits;
+    o->width = h.camera.width;
+    o->height = h.camera.height;
+    o->spp = h.spp;
+    o->max_bounces = h.max_bounces;
+    o->rr = h.rr;
+    o->denoise = h.denoise ? 1 : 0;
+    o->seed = h.seed;
+    o->rr_start_depth = h.rr_start_depth;
+    const rt::Box& b = h.bvh.bounds;
+    float lo[3] = {b.lo.x, b.lo.y, b.lo.z}, hi[3] = {b.hi.x, b.hi.y, b.hi.z};
+    std::memcpy(o->bounds_lo, lo, sizeof(lo));
+    std::memcpy(o->bounds_hi, hi, sizeof(hi));
+    const rt::CameraRec& c = h.camera;
+    float cp[3] = {c.position.x, c.position.y, c.position.z};
+    float ld[3] = {c.left_down.x, c.left_down.y, c.left_down.z};
+    float du[3] = {c.du.x, c.du.y, c.du.z};
+    float dv[3] = {c.dv.x, c.dv.y, c.dv.z};
+    std::memcpy(o->camera_position, cp, sizeof(cp));
+    std::memcpy(o->camera_left_down, ld, sizeof(ld));
+    std::memcpy(o->camera_du, du, sizeof(du));
+    std::memcpy(o->camera_dv, dv, sizeof(dv));
+    return RT_OK;
+}
+
+void rt_scene_destroy(rt_scene* s) { delete s; }
+
+int rt_renderer_create(rt_scene* s, const rt_render_params* p, rt_renderer** out) {
+    if (!s || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    return guard([&]() {
+        rt_render_params d;
+        rt_render_params_default(&d);
+        if (!p) p = &d;
+        rt::RenderSettings set;
+        const rt::HostScene& h = s->host;
+        set.max_bounces = p->max_bounces >= 0 ? p->max_bounces : h.max_bounces;
+        set.rr = p->rr >= 0.f ? p->rr : h.rr;
+        set.rr_start_depth = p->rr_start_depth >= 0 ? p->rr_start_depth : h.rr_start_depth;
+        set.seed = p->seed >= 0 ? (uint32_t)p->seed : h.seed;
+        if (p->batch_spp < 0) throw std::runtime_error("batch_spp must be >= 0");
+        set.batch_spp = p->batch_spp;
+        set.queue_budget_bytes = p->queue_budget_bytes > 0 ? (size_t)p->queue_budget_bytes : 0;
+        set.flush_survivors = p->flush_survivors;
+        set.profile = p->profile;
+        if (set.max_bounces > 255) throw std::runtime_error("max_bounces must be <= 255");
+        ensure_gpu(s);
+        std::unique_ptr<rt_renderer> r(new rt_renderer);
+        r->scene = s;
+        r->r.reset(new rt::Renderer(*s->gpu, s->host, set));
+        *out = r.release();
+        return RT_OK;
+    });
+}
+
+void rt_renderer_destroy(rt_renderer* r) { delete r; }
+
+int rt_render(rt_renderer* r, int spp) {
+    if (!r || spp < 0) return fail(RT_ERR_INVALID_ARG, "invalid argument");
+    return guard([&]() {
+        r->r->render(spp, false);
+        return RT_OK;
+    });
+}
+
+int rt_render_counted(rt_renderer* r, int spp) {
+    if (!r || spp < 0) return fail(RT_ERR_INVALID_ARG, "invalid argument");
+    return guard([&]() {
+        r->r->render(spp, true);
+        return RT_OK;
+    });
+}
+
+int rt_synchronize(rt_renderer* r) {
+    if (!r) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return guard([&]() {
+        r->r->synchronize();
+        return RT_OK;
+    });
+}
+
+int rt_reset(rt_renderer* r) {
+    if (!r) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return guard([&]() {
+        r->r->reset();
+        return RT_OK;
+    });
+}
+
+int rt_reset_stats(rt_renderer* r) {
+    if (!r) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return guard([&]() {
+        r->r->reset_stats();
+        return RT_OK;
+    });
+}
+
+int rt_read_accum(rt_renderer* r, int which, float* rgba) {
+    if (!r || !rgba || which < 0 || which > 2) return fail(RT_ERR_INVALID_ARG, "invalid argument");
+    return guard([&]() {
+        r->r->read_accum(which, rgba);
+        return RT_OK;
+    });
+}
+
+int rt_read_rgba8(rt_renderer* r, uint8_t* out) {
+    if (!r || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return guard([&]() {
+        int w = r->r->width(), h = r->r->height();
+        std::vector<float> img((size_t)w * h * 4);
+        r->r->read_accum(RT_ACCUM_MEAN, img.data());
+        for (int j = 0; j < h; ++j) {
+            int dst = h - 1 - j;  // top row first
+            for (int i = 0; i < w; ++i) {
+                const float* c = &img[((size_t)j * w + i) * 4];
+                uint8_t* o = &out[((size_t)dst * w + i) * 4];
+                for (int k = 0; k < 3; ++k) {
+                    float v = std::pow(c[k], 1.0f / 2.2f) * 255.0f;
+                    v = std::fmin(std::fmax(v, 0.0f), 255.0f);
+                    o[k] = (uint8_t)v;
+                }
+                o[3] = 255;
+            }
+        }
+        return RT_OK;
+    });
+}
+
+int rt_stats(rt_renderer* r, rt_stats_out* o) {
+    if (!r || !o) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return guard([&]() {
+        rt::RenderStats s;
+        r->r->stats(s);
+        o->extension_rays = s.ext_rays;
+        o->shadow_rays = s.shadow_rays;
+        o->extension_node_visits = s.ext_nodes;
+        o->extension_tri_tests = s.ext_tris;
+        o->shadow_node_visits = s.shadow_nodes;
+        o->shadow_tri_tests = s.shadow_tris;
+        o->probe_rays = s.probe_rays;
+        for (int k = 0; k < RT_NUM_STAGES; ++k) {
+            o->stage_ms[k] = s.stage_ms[k];
+            o->stage_launches[k] = s.stage_launches[k];
+        }
+        o->spp_done = s.spp_done;
+        o->batch_spp = s.batch_spp;
+        return RT_OK;
+    });
+}
+
+int rt_scene_get_triangles(const rt_scene* s, float* pos, float* nrm, float* uv, int32_t* mesh) {
+    if (!s) return fail(RT_ERR_INVALID_ARG, "null argument");
+    const rt::HostScene& h = s->host;
+    if (pos) std::memcpy(pos, h.pos.data(), h.pos.size() * 4);
+    if (nrm) std::memcpy(nrm, h.nrm.data(), h.nrm.size() * 4);
+    if (uv) std::memcpy(uv, h.uv.data(), h.uv.size() * 4);
+    if (mesh) std::memcpy(mesh, h.mesh.data(), h.mesh.size() * 4);
+    return RT_OK;
+}
+
+int rt_scene_get_meshes(const rt_scene* s, int32_t* mesh4, float* area) {
+    if (!s) return fail(RT_ERR_INVALID_ARG, "null argument");
+    const rt::HostScene& h = s->host;
+    for (size_t m = 0; m < h.meshes.size(); ++m) {
+        if (mesh4) {
+            mesh4[4 * m] = h.meshes[m].material;
+            mesh4[4 * m + 1] = h.meshes[m].light;
+            mesh4[4 * m + 2] = h.meshes[m].start;
+            mesh4[4 * m + 3] = h.meshes[m].count;
+        }
+        if (area) area[m] = h.meshes[m].area;
+    }
+    return RT_OK;
+}
+
+int rt_scene_get_bvh(const rt_scene* s, int from_packed, void* nodes_out, int32_t* order_out) {
+    if (!s) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return guard([&]() {
+        const rt::BvhBuild& b = s->host.bvh;
+        if (nodes_out) {
+            if (from_packed) {
+                std::vector<rt::LinearNode> un;
+                rt::unpack_bvh(b.packed, b.root_ref, b.bounds, un);
+                if (un.size() != b.nodes.size()) throw std::runtime_error("unpacked node count differs");
+                std::memcpy(nodes_out, un.data(), un.size() * sizeof(rt::LinearNode));
+            } else {
+                std::memcpy(nodes_out, b.nodes.data(), b.nodes.size() * sizeof(rt::LinearNode));
+            }
+        }
+        if (order_out) std::memcpy(order_out, b.order.data(), b.order.size() * 4);
+        return RT_OK;
+    });
+}
+
+int rt_trace_rays(rt_scene* s, const float* rays, int n, float* hits, uint32_t* counts) {
+    if (!s || !rays || !hits || n < 0) return fail(RT_ERR_INVALID_ARG, "invalid argument");
+    if (n == 0) return RT_OK;
+    return guard([&]() {
+        ensure_gpu(s);
+        rt::trace_batch(*s->gpu, rays, n, hits, counts);
+        return RT_OK;
+    });
+}
+
+int rt_device_math(int op, const float* a, const float* b, float* out, int n) {
+    if (!a || !b || !out || n < 0 || op < 0 || op > 9) return fail(RT_ERR_INVALID_ARG, "invalid argument");
+    if (n == 0) return RT_OK;
+    return guard([&]() {
+        int c = 0;
+        if (hipGetDeviceCount(&c) != hipSuccess || c <= 0) throw std::runtime_error("HIP error: no GPU device available");
+        rt::math_selftest(op, a, b, out, n);
+        return RT_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,657 @@
+// Scene-JSON and OBJ loading (reference semantics: Src/Core/scene.cpp:129-690,
+// Src/Utils/objloader.cpp:66-210), camera set-up and BVH build.
+#include "scene.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <unordered_map>
+
+#include "json.h"
+
+namespace rt {
+namespace {
+
+// 4x4 float matrix, row-major m[r][c].
+struct Mat4 {
+    float m[4][4];
+};
+Mat4 identity() {
+    Mat4 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) r.m[i][j] = i == j ? 1.f : 0.f;
+    return r;
+}
+// Row r, column c = ((a[r][0] b[0][c] + a[r][1] b[1][c]) + a[r][2] b[2][c]) + a[r][3] b[3][c].
+Mat4 mul(const Mat4& a, const Mat4& b) {
+    Mat4 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            r.m[i][j] = ((a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j]) + a.m[i][2] * b.m[2][j]) + a.m[i][3] * b.m[3][j];
+    return r;
+}
+// AngleToRadian: angle / 180 * PI with PI the double 3.1415926 (reference:
+// Src/Utils/mathutil.cu:4-7, Src/Core/global.h:7).
+float deg2rad(float a) { return (float)((double)(a / 180.f) * 3.1415926); }
+float rad2deg(float r) { return (float)((double)(r * 180.f) / 3.1415926); }
+
+Mat4 rotate_axis(int axis, float deg) {
+    float rad = deg2rad(deg);
+    float c = cosf(rad), s = sinf(rad);
+    Mat4 r = identity();
+    if (axis == 0) { r.m[1][1] = c; r.m[1][2] = -s; r.m[2][1] = s; r.m[2][2] = c; }
+    else if (axis == 1) { r.m[0][0] = c; r.m[0][2] = s; r.m[2][0] = -s; r.m[2][2] = c; }
+    else { r.m[0][0] = c; r.m[0][1] = -s; r.m[1][0] = s; r.m[1][1] = c; }
+    return r;
+}
+// General inverse by Gauss-Jordan elimination in double, rounded to float.
+Mat4 inverse(const Mat4& a) {
+    double m[4][8];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 8; ++j) m[i][j] = j < 4 ? (double)a.m[i][j] : (j - 4 == i ? 1.0 : 0.0);
+    for (int c = 0; c < 4; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (std::fabs(m[r][c]) > std::fabs(m[piv][c])) piv = r;
+        if (m[piv][c] == 0.0) throw std::runtime_error("object transform is singular");
+        if (piv != c)
+            for (int j = 0; j < 8; ++j) std::swap(m[c][j], m[piv][j]);
+        double d = m[c][c];
+        for (int j = 0; j < 8; ++j) m[c][j] /= d;
+        for (int r = 0; r < 4; ++r) {
+            if (r == c) continue;
+            double f = m[r][c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < 8; ++j) m[r][j] -= f * m[c][j];
+        }
+    }
+    Mat4 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) r.m[i][j] = (float)m[i][j + 4];
+    return r;
+}
+vec3 xform_point(const Mat4& t, vec3 p) {
+    float o[3];
+    for (int r = 0; r < 3; ++r)
+        o[r] = ((t.m[r][0] * p.x + t.m[r][1] * p.y) + t.m[r][2] * p.z) + t.m[r][3] * 1.f;
+    return v3(o[0], o[1], o[2]);
+}
+// transformInv.transpose() * (n, 1), then normalized (reference: Src/Utils/objloader.cpp:36-41).
+vec3 xform_normal(const Mat4& inv, vec3 n) {
+    float o[3];
+    for (int r = 0; r < 3; ++r)
+        o[r] = ((inv.m[0][r] * n.x + inv.m[1][r] * n.y) + inv.m[2][r] * n.z) + inv.m[3][r] * 1.f;
+    return normalize(v3(o[0], o[1], o[2]));
+}
+
+std::string read_file(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f.is_open()) throw std::runtime_error("cannot open file: " + path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    return ss.str();
+}
+std::string dir_of(const std::string& p) {
+    size_t k = p.find_last_of('/');
+    return k == std::string::npos ? std::string() : p.substr(0, k + 1);
+}
+// GetFilenameFromPath: basename without the last extension (reference: Src/Utils/stringutil.cpp:18-33).
+std::string stem_of(const std::string& p) {
+    size_t k = p.find_last_of('/');
+    std::string f = k == std::string::npos ? p : p.substr(k + 1);
+    size_t d = f.find_last_of('.');
+    if (d != std::string::npos && d > 0) return f.substr(0, d);
+    return f;
+}
+std::string resolve_path(const std::string& base_dir, const std::string& p) {
+    if (p.empty() || p[0] == '/') return p;
+    return base_dir + p;
+}
+
+// ---------------------------------------------------------------- OBJ ----
+struct ObjIdx {
+    int v = -1, vt = -1, vn = -1;
+};
+struct LoadedMesh {
+    std::string name;
+    std::string material;
+    std::vector<float> pos, nrm, uv;  // 9, 9, 6 per triangle
+    int count = 0;
+};
+
+int fix_index(long idx, size_t n) {
+    if (idx > 0) return (int)(idx - 1);
+    if (idx < 0) return (int)((long)n + idx);
+    return -1;
+}
+
+// Triangulated OBJ import, one mesh per (shape, material) pair, face normals
+// when a face lacks per-vertex normals (reference: Src/Utils/objloader.cpp:66-210).
+std::vector<LoadedMesh> load_obj(const std::string& path, const Mat4& xf, const Mat4& xf_inv) {
+    std::string text = read_file(path);
+    std::vector<float> V, VN, VT;
+    struct Face {
+        ObjIdx a, b, c;
+        int shape;
+        std::string mtl;
+    };
+    std::vector<Face> faces;
+    std::vector<std::string> shape_names;
+    int shape = 0;
+    bool shape_has_faces = false;
+    shape_names.push_back("");
+    std::string mtl;
+    std::istringstream in(text);
+    std::string line;
+    while (std::getline(in, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        size_t s = line.find_first_not_of(" \t");
+        if (s == std::string::npos || line[s] == '#') continue;
+        const char* c = line.c_str() + s;
+        if (c[0] == 'v' && (c[1] == ' ' || c[1] == '\t')) {
+            float x = 0, y = 0, z = 0;
+            char* e;
+            x = std::strtof(c + 2, &e);
+            y = std::strtof(e, &e);
+            z = std::strtof(e, &e);
+            V.push_back(x); V.push_back(y); V.push_back(z);
+        } else if (c[0] == 'v' && c[1] == 'n') {
+            char* e;
+            float x = std::strtof(c + 2, &e);
+            float y = std::strtof(e, &e);
+            float z = std::strtof(e, &e);
+            VN.push_back(x); VN.push_back(y); VN.push_back(z);
+        } else if (c[0] == 'v' && c[1] == 't') {
+            char* e;
+            float x = std::strtof(c + 2, &e);
+            float y = std::strtof(e, &e);
+            VT.push_back(x); VT.push_back(y);
+        } else if (c[0] == 'f' && (c[1] == ' ' || c[1] == '\t')) {
+            std::istringstream fs(std::string(c + 2));
+            std::string tok;
+            std::vector<ObjIdx> poly;
+            while (fs >> tok) {
+                ObjIdx ix;
+                const char* t = tok.c_str();
+                char* e;
+                long a = std::strtol(t, &e, 10);
+                ix.v = fix_index(a, V.size() / 3);
+                if (*e == '/') {
+                    ++e;
+                    if (*e != '/') {
+                        long b = std::strtol(e, &e, 10);
+                        ix.vt = fix_index(b, VT.size() / 2);
+                    }
+                    if (*e == '/') {
+                        ++e;
+                        long n = std::strtol(e, &e, 10);
+                        ix.vn = fix_index(n, VN.size() / 3);
+                    }
+                }
+                poly.push_back(ix);
+            }
+            // Fan triangulation of polygons.
+            for (size_t k = 1; k + 1 < poly.size(); ++k) {
+                faces.push_back(Face{poly[0], poly[k], poly[k + 1], shape, mtl});
+                shape_has_faces = true;
+            }
+        } else if ((c[0] == 'o' || c[0] == 'g') && (c[1] == ' ' || c[1] == '\t' || c[1] == 0)) {
+            std::string nm = c[1] ? std::string(c + 2) : std::string();
+            size_t a = nm.find_first_not_of(" \t"), b = nm.find_last_not_of(" \t");
+            nm = a == std::string::npos ? std::string() : nm.substr(a, b - a + 1);
+            if (shape_has_faces) {
+                shape_names.push_back(nm);
+                ++shape;
+                shape_has_faces = false;
+            } else {
+                shape_names[shape] = nm;
+            }
+        } else if (std::strncmp(c, "usemtl", 6) == 0) {
+            std::string nm(c + 6);
+            size_t a = nm.find_first_not_of(" \t"), b = nm.find_last_not_of(" \t");
+            mtl = a == std::string::npos ? std::string() : nm.substr(a, b - a + 1);
+        }
+    }
+
+    std::vector<LoadedMesh> meshes;
+    std::map<std::pair<int, std::string>, size_t> lookup;
+    auto vtx = [&](const ObjIdx& ix, vec3& p, vec3& n, float& u, float& v, bool& has_n) {
+        if (ix.v < 0 || (size_t)ix.v * 3 + 2 >= V.size() + 0 || (size_t)(3 * ix.v + 2) >= V.size())
+            throw std::runtime_error("OBJ face references a missing vertex in " + path);
+        p = xform_point(xf, v3(V[3 * ix.v], V[3 * ix.v + 1], V[3 * ix.v + 2]));
+        has_n = ix.vn >= 0 && (size_t)(3 * ix.vn + 2) < VN.size();
+        n = has_n ? xform_normal(xf_inv, v3(VN[3 * ix.vn], VN[3 * ix.vn + 1], VN[3 * ix.vn + 2])) : v3(0.f, 0.f, 0.f);
+        if (ix.vt >= 0 && (size_t)(2 * ix.vt + 1) < VT.size()) { u = VT[2 * ix.vt]; v = VT[2 * ix.vt + 1]; }
+        else { u = 0.f; v = 0.f; }
+    };
+    for (const Face& f : faces) {
+        auto key = std::make_pair(f.shape, f.mtl);
+        size_t mi;
+        auto it = lookup.find(key);
+        if (it == lookup.end()) {
+            LoadedMesh m;
+            const std::string& sn = shape_names[f.shape];
+            std::string base = sn.empty() ? stem_of(path) + "_shape_" + std::to_string(f.shape) : sn;
+            m.name = f.mtl.empty() ? base : base + "::" + f.mtl;
+            m.material = f.mtl;
+            mi = meshes.size();
+            lookup[key] = mi;
+            meshes.push_back(std::move(m));
+        } else {
+            mi = it->second;
+        }
+        vec3 p[3], n[3];
+        float u[3], v[3];
+        bool hn[3];
+        vtx(f.a, p[0], n[0], u[0], v[0], hn[0]);
+        vtx(f.b, p[1], n[1], u[1], v[1], hn[1]);
+        vtx(f.c, p[2], n[2], u[2], v[2], hn[2]);
+        if (!(hn[0] && hn[1] && hn[2])) {
+            vec3 fn = normalize(cross(p[1] - p[0], p[2] - p[0]));
+            n[0] = n[1] = n[2] = fn;
+        }
+        LoadedMesh& m = meshes[mi];
+        for (int k = 0; k < 3; ++k) {
+            m.pos.push_back(p[k].x); m.pos.push_back(p[k].y); m.pos.push_back(p[k].z);
+            m.nrm.push_back(n[k].x); m.nrm.push_back(n[k].y); m.nrm.push_back(n[k].z);
+            m.uv.push_back(u[k]); m.uv.push_back(v[k]);
+        }
+        m.count++;
+    }
+    return meshes;
+}
+
+// ---------------------------------------------------------------- env ----
+// Environment maps: Radiance RGBE (.hdr) or portable float map (.pfm).
+// Stored with row 0 = top of the image (the orientation stb_image returns).
+void load_env(const std::string& path, EnvMap& env) {
+    std::string data = read_file(path);
+    if (data.size() >= 2 && data[0] == 'P' && (data[1] == 'F' || data[1] == 'f')) {
+        bool color = data[1] == 'F';
+        size_t p = 2;
+        auto next_tok = [&]() {
+            while (p < data.size() && std::isspace((unsigned char)data[p])) ++p;
+            size_t b = p;
+            while (p < data.size() && !std::isspace((unsigned char)data[p])) ++p;
+            return data.substr(b, p - b);
+        };
+        int w = std::atoi(next_tok().c_str());
+        int h = std::atoi(next_tok().c_str());
+        double scale = std::atof(next_tok().c_str());
+        ++p;  // single whitespace after the scale
+        bool little = scale < 0;
+        int ch = color ? 3 : 1;
+        size_t need = (size_t)w * h * ch * 4;
+        if (w <= 0 || h <= 0 || data.size() < p + need) throw std::runtime_error("truncated PFM: " + path);
+        env.width = w;
+        env.height = h;
+        env.rgb.assign((size_t)w * h * 3, 0.f);
+        for (int y = 0; y < h; ++y) {
+            int row = h - 1 - y;  // PFM stores the bottom row first
+            for (int x = 0; x < w; ++x)
+                for (int c = 0; c < 3; ++c) {
+                    int sc = ch == 3 ? c : 0;
+                    uint32_t bits;
+                    std::memcpy(&bits, data.data() + p + (((size_t)y * w + x) * ch + sc) * 4, 4);
+                    if (!little) bits = __builtin_bswap32(bits);
+                    float f;
+                    std::memcpy(&f, &bits, 4);
+                    env.rgb[((size_t)row * w + x) * 3 + c] = f;
+                }
+        }
+        return;
+    }
+    // Radiance RGBE with new-style RLE scanlines.
+    size_t p = 0;
+    auto getline_ = [&]() {
+        size_t b = p;
+        while (p < data.size() && data[p] != '\n') ++p;
+        std::string l = data.substr(b, p - b);
+        if (p < data.size()) ++p;
+        return l;
+    };
+    std::string l = getline_();
+    if (l.rfind("#?", 0) != 0) throw std::runtime_error("unsupported environment map format: " + path);
+    while (p < data.size()) {
+        l = getline_();
+        if (l.empty()) break;
+    }
+    l = getline_();
+    int w = 0, h = 0;
+    char ya[3] = {0}, xa[3] = {0};
+    if (std::sscanf(l.c_str(), "%2s %d %2s %d", ya, &h, xa, &w) != 4 || std::strcmp(ya, "-Y") != 0 || std::strcmp(xa, "+X") != 0)
+        throw std::runtime_error("unsupported RGBE orientation in " + path);
+    env.width = w;
+    env.height = h;
+    env.rgb.assign((size_t)w * h * 3, 0.f);
+    std::vector<uint8_t> scan((size_t)w * 4);
+    const uint8_t* d = (const uint8_t*)data.data();
+    for (int y = 0; y < h; ++y) {
+        if (p + 4 > data.size()) throw std::runtime_error("truncated RGBE: " + path);
+        if (w >= 8 && w < 32768 && d[p] == 2 && d[p + 1] == 2 && ((d[p + 2] << 8) | d[p + 3]) == w) {
+            p += 4;
+            for (int c = 0; c < 4; ++c) {
+                int x = 0;
+                while (x < w) {
+                    if (p >= data.size()) throw std::runtime_error("truncated RGBE: " + path);
+                    int cnt = d[p++];
+                    if (cnt > 128) {
+                        cnt -= 128;
+                        if (x + cnt > w || p >= data.size()) throw std::runtime_error("bad RGBE run: " + path);
+                        uint8_t val = d[p++];
+                        for (int k = 0; k < cnt; ++k) scan[(size_t)(x++) * 4 + c] = val;
+                    } else {
+                        if (cnt == 0 || x + cnt > w || p + cnt > data.size()) throw std::runtime_error("bad RGBE run: " + path);
+                        for (int k = 0; k < cnt; ++k) scan[(size_t)(x++) * 4 + c] = d[p++];
+                    }
+                }
+            }
+        } else {
+            if (p + (size_t)w * 4 > data.size()) throw std::runtime_error("truncated RGBE: " + path);
+            std::memcpy(scan.data(), d + p, (size_t)w * 4);
+            p += (size_t)w * 4;
+        }
+        for (int x = 0; x < w; ++x) {
+            const uint8_t* px = &scan[(size_t)x * 4];
+            float f = px[3] ? std::ldexp(1.0f, (int)px[3] - (128 + 8)) : 0.f;
+            for (int c = 0; c < 3; ++c) env.rgb[((size_t)y * w + x) * 3 + c] = px[3] ? ((float)px[c] + 0.5f) * f : 0.f;
+        }
+    }
+}
+
+vec3 vec_of(const Json& a, const char* what) {
+    if (!a.is_array() || a.size() < 3) throw JsonError(std::string("expected a 3-vector for '") + what + "'");
+    return v3(a[0].as_float(what), a[1].as_float(what), a[2].as_float(what));
+}
+
+}  // namespace
+
+void finalize_triangles(HostScene& sc) {
+    int n = sc.num_tris();
+    sc.gnormal.resize(3 * (size_t)n);
+    sc.area.resize(n);
+    for (int t = 0; t < n; ++t) {
+        const float* p = &sc.pos[9 * (size_t)t];
+        vec3 a = v3(p[0], p[1], p[2]), b = v3(p[3], p[4], p[5]), c = v3(p[6], p[7], p[8]);
+        vec3 nn = cross(b - a, c - a);
+        sc.area[t] = 0.5f * length(nn);
+        nn = normalize(nn);
+        sc.gnormal[3 * t] = nn.x;
+        sc.gnormal[3 * t + 1] = nn.y;
+        sc.gnormal[3 * t + 2] = nn.z;
+    }
+    for (auto& m : sc.meshes) {
+        m.area = 0.f;
+        for (int t = m.start; t < m.start + m.count; ++t) m.area += sc.area[t];
+    }
+}
+
+void setup_camera(HostScene& sc) {
+    CameraRec& c = sc.camera;
+    Box b = sc.bvh.bounds;
+    vec3 center = (b.lo + b.hi) * 0.5f;
+    float diag = length(b.hi - b.lo);
+    c.target = center;
+    c.distance = ((diag * 0.5f) / sinf(deg2rad(c.fovy) / 2.f)) * 0.6f;
+    vec3 dir = normalize(c.direction);
+    float th = acosf(dir.y / length(dir));
+    float ph = atan2f(dir.x, dir.z);
+    c.theta = rad2deg(th);
+    c.phi = rad2deg(ph);
+    // Camera::ComputeCameraParam (reference: Src/Core/camera.h:58-84).
+    float theta = deg2rad(c.theta), phi = deg2rad(c.phi);
+    float x = (c.distance * sinf(theta)) * sinf(phi);
+    float y = c.distance * cosf(theta);
+    float z = (c.distance * sinf(theta)) * cosf(phi);
+    c.position = c.target + v3(x, y, z);
+    c.lookat = c.target;
+    c.up = v3(0.f, 1.f, 0.f);
+    c.w = -normalize(c.lookat - c.position);
+    c.u = normalize(cross(normalize(c.up), c.w));
+    c.v = normalize(cross(c.w, c.u));
+    c.focal = length(c.lookat - c.position);
+    float ratio = (1.f * (float)c.width) / (float)c.height;
+    float h_ = (2.f * c.focal) * tanf(deg2rad(c.fovy / 2.f));
+    float w_ = h_ * ratio;
+    c.du = (w_ * c.u) / (float)c.width;
+    c.dv = (h_ * c.v) / (float)c.height;
+    c.left_down = (c.lookat - (float)(c.width / 2) * c.du) - (float)(c.height / 2) * c.dv;
+}
+
+void load_scene(const std::string& json_path, HostScene& sc) {
+    sc = HostScene();
+    sc.path = json_path;
+    std::string base = dir_of(json_path);
+    Json root = Json::parse(read_file(json_path));
+
+    // ---- camera (reference: Src/Core/scene.cpp:187-252)
+    const Json& cam = root["camera"];
+    if (!cam.contains("type")) throw std::runtime_error("camera missing 'type' field");
+    if (cam["type"].as_string("camera.type") != "pinhole")
+        throw std::runtime_error("unsupported camera type " + cam["type"].as_string("camera.type"));
+    if (cam.contains("resolution")) {
+        sc.camera.width = cam["resolution"][0].as_int("resolution");
+        sc.camera.height = cam["resolution"][1].as_int("resolution");
+    }
+    if (sc.camera.width <= 0 || sc.camera.height <= 0) throw std::runtime_error("resolution must be positive");
+    if (cam.contains("fov")) sc.camera.fovy = cam["fov"].as_float("fov");
+    if (cam.contains("max_bounces")) sc.max_bounces = cam["max_bounces"].as_int("max_bounces");
+    if (cam.contains("rr")) sc.rr = cam["rr"].as_float("rr");
+    if (cam.contains("spp")) sc.spp = cam["spp"].as_int("spp");
+    if (cam.contains("direction")) sc.camera.direction = vec_of(cam["direction"], "direction");
+    if (cam.contains("denoise")) sc.denoise = cam["denoise"].as_bool("denoise");
+    // Additive keys (not in the reference schema).
+    if (cam.contains("seed")) sc.seed = (uint32_t)cam["seed"].as_number("seed");
+    if (cam.contains("rr_start_depth")) sc.rr_start_depth = cam["rr_start_depth"].as_int("rr_start_depth");
+
+    // ---- textures (reference: Src/Core/scene.cpp:618-644); LDR textures are not sampled yet.
+    const Json& texj = root["textures"];
+    if (!texj.is_array()) throw std::runtime_error("'textures' should be an array");
+    std::map<std::string, int> tex_map;
+    for (size_t i = 0; i < texj.size(); ++i) {
+        if (!texj[i].contains("path") || !texj[i].contains("name")) {
+            sc.warnings.push_back("texture missing 'path' or 'name'");
+            continue;
+        }
+        tex_map[texj[i]["name"].as_string("texture.name")] = (int)i;
+    }
+
+    // ---- materials (reference: Src/Core/scene.cpp:254-396)
+    const Json& matj = root["materials"];
+    if (!matj.is_array()) throw std::runtime_error("'materials' should be an array");
+    std::unordered_map<std::string, int> mat_map;
+    sc.materials.resize(matj.size());
+    sc.material_names.resize(matj.size());
+    for (size_t i = 0; i < matj.size(); ++i) {
+        const Json& m = matj[i];
+        if (!m.contains("name") || !m.contains("type")) {
+            sc.warnings.push_back("material missing 'name' or 'type'");
+            continue;
+        }
+        MaterialRec r;
+        std::string name = m["name"].as_string("material.name");
+        std::string type = m["type"].as_string("material.type");
+        if (type == "diffuse") r.type = kDiffuse;
+        else if (type == "light") r.type = kLight;
+        else if (type == "pbr") r.type = kPbr;
+        else if (type == "subsurface") r.type = kSubsurface;
+        else if (type == "dielectric" || type == "glass") r.type = kDielectric;
+        else {
+            sc.warnings.push_back("unsupported material type " + type);
+            continue;
+        }
+        auto set3 = [&](float* dst, const char* key, float d0, float d1, float d2) {
+            if (m.contains(key)) {
+                vec3 v = vec_of(m[key], key);
+                dst[0] = v.x; dst[1] = v.y; dst[2] = v.z;
+            } else {
+                dst[0] = d0; dst[1] = d1; dst[2] = d2;
+            }
+        };
+        if (r.type == kDiffuse) {
+            set3(r.basecolor, "basecolor", 0.f, 0.f, 0.f);
+        } else if (r.type == kLight) {
+            set3(r.ke, "emission", 0.f, 0.f, 0.f);
+        } else if (r.type == kPbr) {
+            set3(r.basecolor, "basecolor", 0.f, 0.f, 0.f);
+            r.metallic = m.contains("metallic") ? m["metallic"].as_float("metallic") : 0.0f;
+            r.roughness = m.contains("roughness") ? m["roughness"].as_float("roughness") : 0.5f;
+        } else if (r.type == kSubsurface) {
+            set3(r.basecolor, "basecolor", 0.f, 0.f, 0.f);
+            set3(r.scatter, "scatter_distance", 1.f, 1.f, 1.f);
+            if (m.contains("ior")) r.ior = m["ior"].as_float("ior");
+            if (m.contains("roughness")) r.roughness = m["roughness"].as_float("roughness");
+            r.metallic = 0.0f;
+        } else if (r.type == kDielectric) {
+            set3(r.basecolor, "basecolor", 1.f, 1.f, 1.f);
+            r.ior = m.contains("ior") ? m["ior"].as_float("ior") : 1.5f;
+            r.transmission = m.contains("transmission") ? m["transmission"].as_float("transmission") : 1.0f;
+            r.roughness = m.contains("roughness") ? m["roughness"].as_float("roughness") : 0.0f;
+            r.metallic = 0.0f;
+        }
+        if (m.contains("diffuse_texture")) {
+            std::string tn = m["diffuse_texture"].as_string("diffuse_texture");
+            auto it = tex_map.find(tn);
+            if (it != tex_map.end()) r.texture = it->second;
+            else sc.warnings.push_back("texture '" + tn + "' not found for material '" + name + "'");
+        }
+        r.valid = 1;
+        mat_map[name] = (int)i;
+        sc.materials[i] = r;
+        sc.material_names[i] = name;
+    }
+
+    // ---- objects (reference: Src/Core/scene.cpp:398-537)
+    const Json& objj = root["objects"];
+    if (!objj.is_array()) throw std::runtime_error("'objects' should be an array");
+    std::unordered_map<std::string, int> mesh_map;
+    for (size_t oi = 0; oi < objj.size(); ++oi) {
+        const Json& o = objj[oi];
+        if (!o.contains("name") || !o.contains("path")) {
+            sc.warnings.push_back("object missing 'name' or 'path'");
+            continue;
+        }
+        std::string oname = o["name"].as_string("object.name");
+        std::string opath = resolve_path(base, o["path"].as_string("object.path"));
+        Mat4 tr = identity(), rot = identity(), scl = identity();
+        if (o.contains("translate")) {
+            vec3 t = vec_of(o["translate"], "translate");
+            tr.m[0][3] = t.x; tr.m[1][3] = t.y; tr.m[2][3] = t.z;
+        }
+        if (o.contains("rotate")) {
+            const Json& ro = o["rotate"];
+            for (size_t k = 0; k < ro.size(); ++k) {
+                const Json& r = ro[k];
+                if (r.contains("rotateX")) rot = mul(rotate_axis(0, r["rotateX"].as_float("rotateX")), rot);
+                else if (r.contains("rotateY")) rot = mul(rotate_axis(1, r["rotateY"].as_float("rotateY")), rot);
+                else if (r.contains("rotateZ")) rot = mul(rotate_axis(2, r["rotateZ"].as_float("rotateZ")), rot);
+            }
+        }
+        if (o.contains("scale")) {
+            vec3 s = vec_of(o["scale"], "scale");
+            scl.m[0][0] = s.x; scl.m[1][1] = s.y; scl.m[2][2] = s.z;
+        }
+        Mat4 xf = mul(mul(tr, rot), scl);
+        Mat4 xf_inv = inverse(xf);
+        std::vector<LoadedMesh> loaded = load_obj(opath, xf, xf_inv);
+        if (loaded.empty()) {
+            sc.warnings.push_back("object '" + oname + "' produced no meshes");
+            continue;
+        }
+        bool expanded = loaded.size() > 1;
+        for (LoadedMesh& lm : loaded) {
+            if (lm.count == 0) continue;
+            // ResolveMaterialName (reference: Src/Core/scene.cpp:12-36).
+            std::string mname;
+            if (o.contains("material")) mname = o["material"].as_string("material");
+            else if (o.contains("material_overrides") && o["material_overrides"].is_object() && !lm.material.empty() &&
+                     o["material_overrides"].contains(lm.material))
+                mname = o["material_overrides"][lm.material].as_string("material_overrides");
+            else if (!lm.material.empty() && mat_map.count(lm.material)) mname = lm.material;
+            else if (o.contains("default_material")) mname = o["default_material"].as_string("default_material");
+            if (mname.empty() || !mat_map.count(mname)) {
+                sc.warnings.push_back("no valid material binding for mesh '" + lm.name + "' of object '" + oname + "'");
+                continue;
+            }
+            MeshRec mr;
+            mr.name = expanded ? oname + "::" + lm.name : oname;
+            mr.material = mat_map[mname];
+            mr.start = sc.num_tris();
+            mr.count = lm.count;
+            int mid = (int)sc.meshes.size();
+            sc.pos.insert(sc.pos.end(), lm.pos.begin(), lm.pos.end());
+            sc.nrm.insert(sc.nrm.end(), lm.nrm.begin(), lm.nrm.end());
+            sc.uv.insert(sc.uv.end(), lm.uv.begin(), lm.uv.end());
+            sc.mesh.insert(sc.mesh.end(), (size_t)lm.count, mid);
+            mesh_map[mr.name] = mid;
+            sc.meshes.push_back(mr);
+        }
+    }
+    finalize_triangles(sc);
+
+    // ---- lights (reference: Src/Core/scene.cpp:539-616)
+    const Json& lj = root["lights"];
+    if (!lj.is_array()) throw std::runtime_error("'lights' should be an array");
+    sc.lights.resize(lj.size());
+    for (size_t i = 0; i < lj.size(); ++i) {
+        const Json& l = lj[i];
+        LightRec rec;
+        if (!l.contains("type")) {
+            sc.warnings.push_back("light missing 'type'");
+            sc.lights[i] = rec;
+            continue;
+        }
+        std::string type = l["type"].as_string("light.type");
+        if (type == "area") {
+            rec.type = kAreaLight;
+            if (l.contains("ke")) {
+                vec3 e = vec_of(l["ke"], "ke");
+                rec.emission[0] = e.x; rec.emission[1] = e.y; rec.emission[2] = e.z;
+            }
+            if (l.contains("material")) {
+                std::string mn = l["material"].as_string("light.material");
+                auto it = mat_map.find(mn);
+                if (it != mat_map.end()) {
+                    const MaterialRec& mr = sc.materials[it->second];
+                    rec.emission[0] = mr.ke[0]; rec.emission[1] = mr.ke[1]; rec.emission[2] = mr.ke[2];
+                } else {
+                    sc.warnings.push_back("material '" + mn + "' not found for light");
+                }
+            }
+            std::string lm = l["lightMeshName"].as_string("lightMeshName");
+            auto it = mesh_map.find(lm);
+            if (it == mesh_map.end()) throw std::runtime_error("light mesh '" + lm + "' not found");
+            rec.mesh = it->second;
+            sc.meshes[rec.mesh].light = (int)i;
+            rec.tri_count = sc.meshes[rec.mesh].count;
+            rec.area = sc.meshes[rec.mesh].area;
+        } else if (type == "env") {
+            rec.type = kEnvLight;
+            if (!l.contains("path")) throw std::runtime_error("env light missing 'path'");
+            load_env(resolve_path(base, l["path"].as_string("light.path")), sc.env);
+            sc.env_light = (int)i;
+        } else {
+            throw std::runtime_error("unsupported light type " + type);
+        }
+        sc.lights[i] = rec;
+    }
+    // Light triangles are packed contiguously per area light.
+    int cursor = 0;
+    for (auto& l : sc.lights) {
+        if (l.type == kAreaLight && l.mesh >= 0) {
+            l.tri_start = cursor;
+            cursor += l.tri_count;
+        }
+    }
+
+    if (sc.num_tris() == 0) throw std::runtime_error("scene has no triangles");
+    build_bvh(sc.pos.data(), sc.num_tris(), sc.bvh);
+    setup_camera(sc);
+}
+
+}  // namespace rt

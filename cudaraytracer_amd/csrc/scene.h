@@ -1,0 +1,102 @@
+// Host-side scene: scene-JSON + OBJ loading, camera set-up and BVH build.
+#pragma once
+
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "bvh.h"
+#include "rt_math.h"
+
+namespace rt {
+
+// Material kinds in the reference's enum order (reference: Src/Core/material.h:6-12).
+enum MaterialType : int32_t { kLight = 0, kDiffuse = 1, kPbr = 2, kDielectric = 3, kSubsurface = 4 };
+enum LightType : int32_t { kAreaLight = 0, kPointLight = 1, kEnvLight = 2 };
+
+// Device material record (64 bytes).  Defaults are the reference's Material
+// member initialisers (reference: Src/Core/material.h:57-71).
+struct MaterialRec {
+    int32_t type = kDiffuse;
+    float basecolor[3] = {0.f, 0.f, 0.f};
+    float ke[3] = {0.f, 0.f, 0.f};
+    float roughness = 0.5f;
+    float metallic = 0.0f;
+    float transmission = 1.0f;
+    float ior = 1.0f;
+    float scatter[3] = {0.f, 0.f, 0.f};
+    int32_t texture = -1;
+    int32_t valid = 0;
+};
+static_assert(sizeof(MaterialRec) == 64, "material record is 64 bytes");
+
+// Device light record (32 bytes).
+struct LightRec {
+    int32_t type = kAreaLight;
+    int32_t mesh = -1;
+    float emission[3] = {0.f, 0.f, 0.f};
+    int32_t tri_start = 0;  // first triangle in the light-triangle array
+    int32_t tri_count = 0;
+    float area = 0.f;       // mesh area (sum of triangle areas, scene order)
+};
+static_assert(sizeof(LightRec) == 32, "light record is 32 bytes");
+
+struct MeshRec {
+    std::string name;
+    int32_t material = -1;
+    int32_t light = -1;
+    int32_t start = 0;
+    int32_t count = 0;
+    float area = 0.f;
+};
+
+// Pinhole orbit camera (reference: Src/Core/camera.h:11-84; Src/Core/scene.cpp:646-667).
+struct CameraRec {
+    int width = 800, height = 600;
+    float fovy = 45.f;
+    vec3 direction = {0.f, 0.f, -1.f};
+    vec3 target, position, lookat, up, u, v, w, du, dv, left_down;
+    float distance = 0.f, theta = 0.f, phi = 0.f, focal = 0.f;
+};
+
+struct EnvMap {
+    int width = 0, height = 0;
+    std::vector<float> rgb;  // width*height*3, row 0 = top
+};
+
+struct HostScene {
+    std::string path;
+    // Scene-order triangle soup.
+    std::vector<float> pos;      // 9 per triangle
+    std::vector<float> nrm;      // 9 per triangle (per-vertex shading normals)
+    std::vector<float> uv;       // 6 per triangle
+    std::vector<int32_t> mesh;   // mesh index per triangle
+    std::vector<float> gnormal;  // 3 per triangle, normalised geometric normal
+    std::vector<float> area;     // per triangle
+    std::vector<MeshRec> meshes;
+    std::vector<MaterialRec> materials;
+    std::vector<std::string> material_names;
+    std::vector<LightRec> lights;
+    int env_light = -1;
+    EnvMap env;
+    CameraRec camera;
+    int max_bounces = 5;
+    float rr = 0.8f;
+    int spp = 16;
+    bool denoise = false;
+    uint32_t seed = 1234u;      // reference: Src/Core/scene.cpp:167
+    int rr_start_depth = 0;     // 0: roulette from the first bounce, as the reference
+    BvhBuild bvh;
+    std::vector<std::string> warnings;
+
+    int num_tris() const { return (int)mesh.size(); }
+};
+
+// Parse a scene JSON (+ OBJ files) and build the BVH.  Throws std::runtime_error.
+void load_scene(const std::string& json_path, HostScene& sc);
+// Triangle constructor math: geometric normal and area (reference: Src/Core/triangle.cu:3-10).
+void finalize_triangles(HostScene& sc);
+// Camera parameters from the scene bounds (reference: Src/Core/scene.cpp:646-667, camera.h:58-84).
+void setup_camera(HostScene& sc);
+
+}  // namespace rt

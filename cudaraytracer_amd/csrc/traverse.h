@@ -1,0 +1,170 @@
+// Closest-hit BVH traversal for gfx950: 64-byte child-pair nodes, a per-lane
+// stack column in LDS, nearest child first.  The visiting order, the slab test
+// and the triangle test are the reference's (Src/Core/bvh.cu:253-306,
+// Src/Core/bounds.h:60-78, Src/Core/triangle.cu:102-133) so hits are
+// bit-identical to a host run of the same FP32 code.
+#pragma once
+
+#include "bvh.h"
+#include "device.h"
+#include "rt_math.h"
+
+namespace rt {
+
+struct RayInv {
+    vec3 o, d, inv;
+    bool nx, ny, nz;
+};
+
+RT_HD RayInv make_ray(vec3 o, vec3 d) {
+    RayInv r;
+    r.o = o;
+    r.d = d;
+    r.inv = v3(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    r.nx = r.inv.x < 0;
+    r.ny = r.inv.y < 0;
+    r.nz = r.inv.z < 0;
+    return r;
+}
+
+// Slab test with the reference's 1e-2 slack: (double)x + 1e-2 >= 0 holds
+// exactly when x >= -0.01f for every float x, so the test stays in FP32.
+RT_HD bool box_hit(const RayInv& r, float lx, float ly, float lz, float hx, float hy, float hz, float tmin,
+                   float tmax, float& t) {
+    float ex = (lx - r.o.x) * r.inv.x, xx = (hx - r.o.x) * r.inv.x;
+    float ey = (ly - r.o.y) * r.inv.y, xy = (hy - r.o.y) * r.inv.y;
+    float ez = (lz - r.o.z) * r.inv.z, xz = (hz - r.o.z) * r.inv.z;
+    float ent_x = r.nx ? xx : ex, ext_x = r.nx ? ex : xx;
+    float ent_y = r.ny ? xy : ey, ext_y = r.ny ? ey : xy;
+    float ent_z = r.nz ? xz : ez, ext_z = r.nz ? ez : xz;
+    float exit = fminf(ext_x, fminf(ext_y, ext_z));
+    float enter = fmaxf(ent_x, fmaxf(ent_y, ent_z));
+    t = enter;
+    return (exit - enter >= -0.01f) && (exit >= -0.01f) && (enter <= tmax) && (exit >= tmin);
+}
+
+// Moller-Trumbore with back-face culling and the triangle's own [0, 1e8]
+// range.  Returns true and updates (t, b1, b2) on a closer hit.
+RT_HD bool tri_hit(const RayInv& r, float4 a, float4 e1f, float4 e2f, float& hit_t, float& hb1, float& hb2) {
+    vec3 v0 = v3(a.x, a.y, a.z), e1 = v3(e1f.x, e1f.y, e1f.z), e2 = v3(e2f.x, e2f.y, e2f.z);
+    vec3 s = r.o - v0;
+    vec3 s1 = cross(r.d, e2);
+    vec3 s2 = cross(s, e1);
+    float det = dot(e1, s1);
+    if (det <= 0.f) return false;
+    float inv = 1.f / det;
+    float t = inv * dot(s2, e2);
+    if (t >= hit_t) return false;
+    float b1 = inv * dot(s1, s);
+    if (b1 < 0.f || b1 > 1.f) return false;
+    float b2 = inv * dot(s2, r.d);
+    if (b2 < 0.f || b1 + b2 > 1.f) return false;
+    if (t < 0.f || t > 1e8f) return false;
+    hit_t = t;
+    hb1 = b1;
+    hb2 = b2;
+    return true;
+}
+
+struct Hit {
+    float t, b1, b2;
+    int prim;
+};
+
+// Traversal counters for the algorithmic-bytes model (interior nodes whose
+// child pair is tested, triangle tests).
+struct TravCount {
+    uint32_t nodes, tris;
+};
+
+// `stk` points at this lane's stack column; entries are `stride` words apart.
+template <int STACK, bool COUNT>
+__device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r, float tmin, float tmax,
+                                             uint32_t* stk, int stride, TravCount& cnt) {
+    Hit h;
+    h.t = tmax;
+    h.b1 = 0.f;
+    h.b2 = 0.f;
+    h.prim = -1;
+    uint32_t ref = sc.root_ref;
+    int sp = 0;
+    while (true) {
+        if (ref & kLeafBit) {
+            uint32_t first = (ref & ~kLeafBit) >> 3;
+            uint32_t n = ref & 7u;
+            for (uint32_t k = 0; k < n; ++k) {
+                const float4* tp = sc.tri_isect + 3 * (size_t)(first + k);
+                float4 a = tp[0], e1 = tp[1], e2 = tp[2];
+                if (COUNT) cnt.tris++;
+                if (tri_hit(r, a, e1, e2, h.t, h.b1, h.b2)) h.prim = (int)(first + k);
+            }
+        } else {
+            const float4* np = reinterpret_cast<const float4*>(sc.nodes + ref);
+            float4 w0 = np[0], w1 = np[1], w2 = np[2];
+            int4 w3 = reinterpret_cast<const int4*>(np)[3];
+            if (COUNT) cnt.nodes++;
+            float tr, tl;
+            bool hr = box_hit(r, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w, tmin, h.t, tr);
+            bool hl = box_hit(r, w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, tmin, h.t, tl);
+            uint32_t rl = (uint32_t)w3.x, rr = (uint32_t)w3.y;
+            if (hr && hl) {
+                // Nearer child next; ties visit the left child first.
+                if (tr < tl) {
+                    stk[sp * stride] = rl;
+                    ref = rr;
+                } else {
+                    stk[sp * stride] = rr;
+                    ref = rl;
+                }
+                ++sp;
+                continue;
+            } else if (hr) {
+                ref = rr;
+                continue;
+            } else if (hl) {
+                ref = rl;
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        ref = stk[sp * stride];
+    }
+    return h;
+}
+
+// Shading data at a hit, reconstructed from barycentrics the way the
+// reference's triangle test records it (Src/Core/triangle.cu:121-131).
+struct SurfaceHit {
+    vec3 p;       // b0 v0 + b1 v1 + b2 v2
+    vec3 ns;      // normalised interpolated shading normal
+    vec3 ng;      // stored geometric normal
+    float u, v;
+    int mesh;
+};
+
+RT_HD SurfaceHit surface_at(const float4* attr, int prim, float b1, float b2) {
+    const float4* a = attr + kAttrStride * (size_t)prim;
+    float4 a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5], a6 = a[6];
+    float b0 = (1.f - b1) - b2;
+    SurfaceHit s;
+    vec3 v0 = v3(a0.x, a0.y, a0.z), v1 = v3(a1.x, a1.y, a1.z), v2 = v3(a2.x, a2.y, a2.z);
+    s.p = (b0 * v0 + b1 * v1) + b2 * v2;
+    vec3 n0 = v3(a3.x, a3.y, a3.z), n1 = v3(a4.x, a4.y, a4.z), n2 = v3(a5.x, a5.y, a5.z);
+    s.ns = normalize((b0 * n0 + b1 * n1) + b2 * n2);
+    s.ng = v3(a1.w, a2.w, a3.w);
+    s.u = (b0 * a4.w + b1 * a6.x) + b2 * a6.z;
+    s.v = (b0 * a5.w + b1 * a6.y) + b2 * a6.w;
+    s.mesh = f_as_i(a0.w);
+    return s;
+}
+
+RT_HD vec3 position_at(const float4* attr, int prim, float b1, float b2) {
+    const float4* a = attr + kAttrStride * (size_t)prim;
+    float4 a0 = a[0], a1 = a[1], a2 = a[2];
+    float b0 = (1.f - b1) - b2;
+    vec3 v0 = v3(a0.x, a0.y, a0.z), v1 = v3(a1.x, a1.y, a1.z), v2 = v3(a2.x, a2.y, a2.z);
+    return (b0 * v0 + b1 * v1) + b2 * v2;
+}
+
+}  // namespace rt

@@ -1,0 +1,150 @@
+/*
+ * C ABI of the MI355X wavefront path tracer.
+ *
+ * The reference has no headless API: its only caller is main()
+ * (Src/main.cu:18-45), which constructs Scene(path) (Src/Core/scene.cpp:50-57),
+ * then Renderer(ui) (Src/Core/renderer.cuh:1094-1100) and runs RenderLoop()
+ * (Src/Core/renderer.cuh:1106-1307); results live in the device accumulators
+ * renderBuffers[].{direct,indirect}LightingBuffer (renderer.cuh:56-59).  The
+ * entry points below replace that surface one to one:
+ *
+ *   rt_scene_load        <- Scene::Scene (scene.cpp:50-57): JSON + OBJ, SAH BVH
+ *   rt_renderer_create   <- Renderer::Renderer / CudaInit (renderer.cuh:1094-1100, 1309-1355)
+ *   rt_render            <- RenderLoop frames without denoising (renderer.cuh:1144-1193),
+ *                           one reference frame = one sample per pixel
+ *   rt_read_accum        <- renderBuffers[cur].direct/indirectLightingBuffer read-back
+ *   rt_read_rgba8        <- GatherKernel colour mode (renderer.cuh:956-993)
+ *   rt_stats             <- DenoiseTimingStats / stage timers (ui.h:27-37), plus ray counts
+ *   rt_last_error        <- std::runtime_error / printed CUDA errors (scene.cpp:51-53, global.h:19-27)
+ *
+ * Conventions: functions return RT_OK (0) or a negative RT_ERR_* code and set
+ * a thread-local message readable with rt_last_error().  Output buffers are
+ * caller-owned host memory; device memory belongs to the library.  Images are
+ * RGBA float32, width*height*4, row 0 = the BOTTOM row of the image (the
+ * reference's accumulation order); rt_read_rgba8 flips to top-first like the
+ * reference's display gather.  A handle may be used by one thread at a time;
+ * distinct handles are independent and each renderer owns one HIP stream.
+ */
+#ifndef RT_AMD_H
+#define RT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RT_OK 0
+#define RT_ERR_INVALID_ARG (-1)
+#define RT_ERR_IO (-2)
+#define RT_ERR_SCENE (-3)
+#define RT_ERR_DEVICE (-4)
+#define RT_ERR_OUT_OF_MEMORY (-5)
+#define RT_ERR_INTERNAL (-6)
+
+#define RT_ACCUM_DIRECT 0
+#define RT_ACCUM_INDIRECT 1
+#define RT_ACCUM_MEAN 2 /* (direct + indirect) / spp */
+
+#define RT_STAGE_RAYGEN 0
+#define RT_STAGE_EXTEND 1
+#define RT_STAGE_SHADE 2
+#define RT_STAGE_SHADOW 3
+#define RT_STAGE_RESOLVE 4
+#define RT_NUM_STAGES 5
+
+typedef struct rt_scene rt_scene;
+typedef struct rt_renderer rt_renderer;
+
+typedef struct rt_load_opts {
+    int reserved; /* must be 0 */
+} rt_load_opts;
+
+typedef struct rt_scene_info {
+    int num_triangles;
+    int num_meshes;
+    int num_materials;
+    int num_lights;
+    int num_bvh_nodes;      /* reference-layout node count (interior + leaves) */
+    int bvh_max_depth;
+    int bvh_fallback_splits; /* splits where the reference build would not terminate */
+    int width, height, spp, max_bounces;
+    float rr;
+    int denoise;
+    uint32_t seed;
+    int rr_start_depth;
+    float bounds_lo[3], bounds_hi[3];
+    float camera_position[3], camera_left_down[3], camera_du[3], camera_dv[3];
+} rt_scene_info;
+
+typedef struct rt_render_params {
+    int max_bounces;        /* < 0: scene value */
+    float rr;               /* < 0: scene value (continue probability) */
+    int rr_start_depth;     /* < 0: scene value; 0 = roulette from the first bounce (reference) */
+    int64_t seed;           /* < 0: scene value */
+    int batch_spp;          /* samples per pass; 0 = automatic */
+    int64_t queue_budget_bytes; /* 0 = 8 GiB */
+    int flush_survivors;    /* 0 = reference: paths alive after max_bounces contribute nothing */
+    int profile;            /* 1 = per-stage HIP event timing in rt_stats */
+} rt_render_params;
+
+typedef struct rt_stats_out {
+    uint64_t extension_rays;
+    uint64_t shadow_rays;
+    uint64_t extension_node_visits; /* interior nodes whose child pair was tested (counting renders only) */
+    uint64_t extension_tri_tests;
+    uint64_t shadow_node_visits;
+    uint64_t shadow_tri_tests;
+    uint64_t probe_rays;
+    double stage_ms[RT_NUM_STAGES];
+    int stage_launches[RT_NUM_STAGES];
+    int spp_done;
+    int batch_spp;
+} rt_stats_out;
+
+/* Fill *p with "use the scene's values" defaults. */
+void rt_render_params_default(rt_render_params* p);
+
+/* Scene::Scene (Src/Core/scene.cpp:50-57). */
+int rt_scene_load(const char* json_path, const rt_load_opts* opts, rt_scene** out);
+int rt_scene_info_get(const rt_scene* s, rt_scene_info* out);
+void rt_scene_destroy(rt_scene* s);
+
+/* Renderer::Renderer (Src/Core/renderer.cuh:1094-1100). */
+int rt_renderer_create(rt_scene* s, const rt_render_params* p, rt_renderer** out);
+void rt_renderer_destroy(rt_renderer* r);
+/* Accumulate spp more samples per pixel (RenderLoop, renderer.cuh:1144-1193). Asynchronous. */
+int rt_render(rt_renderer* r, int spp);
+/* Same with traversal counters (node visits / triangle tests) enabled. */
+int rt_render_counted(rt_renderer* r, int spp);
+int rt_synchronize(rt_renderer* r);
+/* Clear the accumulators (framebufferReset, renderer.cuh:1121-1124). */
+int rt_reset(rt_renderer* r);
+int rt_reset_stats(rt_renderer* r);
+/* Read back an accumulator: RT_ACCUM_DIRECT, RT_ACCUM_INDIRECT or RT_ACCUM_MEAN. */
+int rt_read_accum(rt_renderer* r, int which, float* rgba_out);
+/* Gamma-2.2 8-bit colour, top row first (GatherKernel colour mode). */
+int rt_read_rgba8(rt_renderer* r, uint8_t* rgba_out);
+int rt_stats(rt_renderer* r, rt_stats_out* out);
+const char* rt_last_error(void);
+
+/* ---- test and inspection hooks (no reference counterpart) ---- */
+/* Scene-order triangle soup: pos 9, normals 9, uv 6 floats and mesh id per triangle (any pointer may be NULL). */
+int rt_scene_get_triangles(const rt_scene* s, float* pos, float* nrm, float* uv, int32_t* mesh);
+/* Per-mesh (material, light, start, count) and area. */
+int rt_scene_get_meshes(const rt_scene* s, int32_t* mesh4, float* area);
+/* Reference-layout BVH: 8 words per node as in LinearBVHNode (Src/Core/bvh.h:25-33) and the
+   ordered-triangle permutation.  from_packed = 1 rebuilds the array from the device layout. */
+int rt_scene_get_bvh(const rt_scene* s, int from_packed, void* nodes_out, int32_t* order_out);
+/* Closest hit on the GPU for n rays given as 8 floats (o.xyz, tmin, d.xyz, tmax);
+   hits_out: (t, b1, b2, prim-as-int-bits) per ray; counts_out: (node visits, tri tests). */
+int rt_trace_rays(rt_scene* s, const float* rays, int n, float* hits_out, uint32_t* counts_out);
+/* Device arithmetic check (op: 0 div, 1 sqrt, 2 sin, 3 cos, 4 acos, 5 exp, 6 log, 7 atan2, 8 philox, 9 rcp). */
+int rt_device_math(int op, const float* a, const float* b, float* out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RT_AMD_H */
