@@ -1,0 +1,168 @@
+"""Benchmark: wavefront path tracing throughput on one MI355X.
+
+Workload (one step): the Cornell shell + 4 icospheres scene (20,516 triangles,
+GGX metallic-roughness, one IOR-1.5 dielectric, area light with MIS), 512x512,
+64 samples per pixel, max depth 12, roulette after depth 3 -- rendered in full.
+Inputs (BVH, triangles, queues) are resident in HBM before timing starts.
+
+Prints one JSON line.  `value` is Mrays/s over the timed steps (extension +
+shadow rays / GPU wall time).  `roofline` is for the kernel with the largest
+share of GPU time, from per-launch HIP events recorded on the renderer's own
+stream and the algorithmic-bytes model.  `cpu_baseline` times
+the CPU oracle (OpenMP, all host cores) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes (per unit): ray record 32 B + hit record 16 B per extension
+# ray; 32 B + 4 B per shadow ray; 64 B per interior node whose child pair is
+# tested; 36 B of vertex data per triangle test.
+EXT_RAY_BYTES, SHADOW_RAY_BYTES, NODE_BYTES, TRI_BYTES = 48, 36, 64, 36
+# Shade: 84 B read + 68 B written per path-bounce + 44 B per emitted shadow ray.
+SHADE_PATH_BYTES, SHADE_SHADOW_BYTES = 152, 44
+
+
+def _stage_roofline(name, bytes_per_launch, ms_per_launch, traffic=None):
+    achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "bytes_per_launch": int(bytes_per_launch), "ms_per_launch": round(ms_per_launch, 5)}
+
+
+def cpu_baseline(scene_path: str, spp: int):
+    from oracle import rtref
+
+    rtref.build()
+    a = rtref.parse_scene(scene_path)
+    rs = rtref.RefScene(a)
+    p = rs.params()
+    rs.render(1, p, pixels=np.arange(0, a.width * a.height, 64, dtype=np.int32))  # warm-up
+    t0 = time.perf_counter()
+    _, _, st = rs.render(spp, p)
+    dt = time.perf_counter() - t0
+    rays = int(st[0] + st[1])
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{a.width}x{a.height} x {spp} of {a.spp} spp of the same scene ({dt:.1f} s, {rays} rays)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-spp", type=int, default=16, help="samples per pixel for the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-json", default=None, help="HBM bytes per launch measured with rocprofv3 --pmc")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU is design-only)"}))
+        return 2
+
+    import cudaraytracer_amd as rt
+    from cudaraytracer_amd import scenes
+
+    tmp = tempfile.mkdtemp(prefix="rtbench_")
+    spec = scenes.cornell_spheres()
+    path = scenes.write_scene(spec, tmp, "cornell_spheres")
+    scene = rt.Scene(path)
+    info = scene.info()
+    spp = info.spp
+    r = rt.Renderer(scene)
+    print(f"[bench] scene loaded: {info.num_triangles} tris, {info.width}x{info.height}, {spp} spp, "
+          f"depth {info.max_bounces}, batch {r.stats().batch_spp}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        r.reset()
+        r.render(spp)
+    r.synchronize()
+    r.reset_stats()
+    # Timed region: K full renders, synchronised on both sides.
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.reset()
+        r.render(spp)
+    r.synchronize()
+    elapsed = time.perf_counter() - t0
+    st = r.stats()
+    rays = st.extension_rays + st.shadow_rays
+    mrays = rays / elapsed / 1e6
+    ms_per_step = elapsed * 1e3 / args.steps
+    img = r.read_accum(rt.RT_ACCUM_MEAN)
+    assert np.isfinite(img).all()
+    print(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, {mrays:.1f} Mrays/s", file=sys.stderr, flush=True)
+
+    # Per-kernel timing with HIP events on the renderer's stream, plus traversal counts.
+    rp = rt.Renderer(scene, profile=True)
+    rp.render(spp)
+    rp.synchronize()
+    rp.reset()
+    rp.reset_stats()
+    prof_steps = max(1, min(3, args.steps))
+    for _ in range(prof_steps):
+        rp.reset()
+        rp.render(spp)
+    sp = rp.stats()
+    rc = rt.Renderer(scene)
+    rc.render(spp, count_traversal=True)
+    sc = rc.stats()
+    traffic = None
+    if args.traffic_json and os.path.exists(args.traffic_json):
+        traffic = json.load(open(args.traffic_json))
+    ext_bytes = (sc.extension_rays * EXT_RAY_BYTES + sc.extension_node_visits * NODE_BYTES
+                 + sc.extension_tri_tests * TRI_BYTES)
+    sh_bytes = (sc.shadow_rays * SHADOW_RAY_BYTES + sc.shadow_node_visits * NODE_BYTES
+                + sc.shadow_tri_tests * TRI_BYTES)
+    shade_bytes = sc.extension_rays * SHADE_PATH_BYTES + sc.shadow_rays * SHADE_SHADOW_BYTES
+    stages = {}
+    for k, (name, nbytes) in enumerate([("k_extend", ext_bytes), ("k_shade", shade_bytes), ("k_shadow", sh_bytes)]):
+        idx = {"k_extend": 1, "k_shade": 2, "k_shadow": 3}[name]
+        launches = sp.stage_launches[idx] / prof_steps
+        ms = sp.stage_ms[idx] / prof_steps
+        t = traffic.get(name) if traffic else None
+        stages[name] = _stage_roofline(name, nbytes / launches, ms / launches, t)
+        stages[name]["ms_per_step"] = round(ms, 3)
+    dominant = max(stages.values(), key=lambda s: s["ms_per_step"])
+
+    out = {
+        "metric": "Mrays/s & % HBM roofline; served jobs/s + p50/p99 latency vs plain",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (procedural scene generated in the reference's scene-JSON + OBJ format)",
+        "config": {"workload": "cornell_spheres: Cornell shell + 4 icospheres (subdiv 4), 20516 tris, GGX + "
+                               "dielectric + MIS, 512x512, 64 spp, depth 12, RR after depth 3",
+                   "triangles": info.num_triangles, "width": info.width, "height": info.height, "spp": spp,
+                   "max_depth": info.max_bounces, "batch_spp": st.batch_spp,
+                   "rays_per_step": int(rays // args.steps)},
+        "roofline": {k: dominant[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
+        "roofline_kernel": dominant["kernel"],
+        "stages": stages,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(path, args.cpu_spp)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,101 @@
+/*
+ * rtref -- CPU oracle for the wavefront path tracer.  TEST INFRASTRUCTURE
+ * ONLY: it is loaded by tests/, by __graft_entry__.smoke() as the checker and
+ * by bench.py's cpu_baseline leg, never by the library.
+ *
+ * A plain-C restatement of the reference's hot path, written in the
+ * reference's own structure (recursive SAH build over 32-byte LinearBVHNode
+ * records, node-by-node stack traversal, one sequential loop per path), with
+ * every function citing the reference file:line it follows.  Built with
+ * -ffp-contract=off so its FP32 results are comparable bit for bit with the
+ * GPU kernels.  Randomness is the counter-based Philox4x32-10 stream
+ * (cuRAND's XORWOW is NVIDIA's and is not reproduced).
+ */
+#ifndef RTREF_H
+#define RTREF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Material record: 16 floats. */
+#define REF_MAT_STRIDE 16
+/* [0] type (0 light, 1 diffuse, 2 pbr, 3 dielectric, 4 subsurface)
+   [1..3] basecolor  [4..6] emission  [7] roughness  [8] metallic
+   [9] transmission  [10] ior  [11..13] scatter distance  [14..15] unused */
+
+typedef struct ref_scene_desc {
+    int num_tris;
+    const float* pos;      /* 9 per triangle, scene order */
+    const float* nrm;      /* 9 per triangle */
+    const float* uv;       /* 6 per triangle */
+    const int32_t* mesh;   /* mesh per triangle */
+    int num_meshes;
+    const int32_t* mesh_material;
+    const int32_t* mesh_light;
+    int num_materials;
+    const float* materials;
+    int num_lights;
+    const int32_t* light_type;  /* 0 area, 2 env */
+    const int32_t* light_mesh;
+    const float* light_emission; /* 3 per light */
+    int env_w, env_h;
+    const float* env_rgb;        /* env_w*env_h*3, row 0 = top */
+    int width, height;
+    float fovy;
+    float direction[3];
+} ref_scene_desc;
+
+typedef struct ref_params {
+    int max_bounces;
+    float rr;
+    int rr_start_depth;
+    uint32_t seed;
+    int flush_survivors;
+} ref_params;
+
+typedef struct ref_scene ref_scene;
+
+ref_scene* ref_scene_create(const ref_scene_desc* d);
+void ref_scene_destroy(ref_scene* s);
+
+int ref_bvh_num_nodes(const ref_scene* s);
+int ref_bvh_max_depth(const ref_scene* s);
+void ref_bvh_nodes(const ref_scene* s, int32_t* out8);   /* 8 words per node */
+void ref_bvh_order(const ref_scene* s, int32_t* out);
+void ref_camera(const ref_scene* s, float* out12);       /* position, left_down, du, dv */
+
+/* Closest hit for rays (o.xyz, tmin, d.xyz, tmax): hits (t, b1, b2, ordered prim as int bits),
+   counts (interior nodes whose children were tested, triangle tests). */
+void ref_trace(const ref_scene* s, const float* rays8, int n, float* hits4, uint32_t* counts2);
+
+/* Accumulate samples [sample0, sample0+spp) for the listed pixels (all when pixels == NULL),
+   in sample order; acc_* are npix*4 floats.  stats[0..1] += extension / shadow rays.
+   Uses OpenMP threads over pixels. */
+void ref_render(const ref_scene* s, const ref_params* p, int sample0, int spp, const int32_t* pixels, int npix,
+                float* acc_direct, float* acc_indirect, uint64_t* stats);
+
+/* Same estimator in double precision (FP64 twin) for converged-image comparisons. */
+void ref_render_f64(const ref_scene* s, const ref_params* p, int sample0, int spp, const int32_t* pixels, int npix,
+                    double* acc_direct, double* acc_indirect);
+
+/* Scalar pieces exposed for known-answer and closed-form tests. */
+void ref_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float ref_math(int op, float a, float b); /* 2 sin 3 cos 4 acos 5 exp 6 log 7 atan2 */
+/* BSDF helpers: type from material; eval returns rgb into out3, pdf returned. */
+float ref_bsdf_eval(const float* mat, const float* wi, const float* n, const float* wo, float* out3);
+float ref_bsdf_sample(const float* mat, const float* wi, const float* n, const float* u3, float* wo_out);
+float ref_fresnel_dielectric(float cos_i, float eta_i, float eta_t);
+float ref_ggx_d(const float* mat, const float* wi, const float* n, const float* wo);
+float ref_burley(const float* mat, float r, int ch);
+void ref_env_tables(const ref_scene* s, float* row_cdf, float* marg_cdf);
+void ref_env_sample(const ref_scene* s, float rv, float ru, float* uv_out);
+float ref_env_pdf_uv(const ref_scene* s, float u, float v);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

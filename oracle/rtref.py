@@ -1,0 +1,480 @@
+"""ctypes front-end of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Used by tests/, by ``__graft_entry__.smoke()`` as the checker and by
+``bench.py``'s cpu_baseline leg.  The scene is read here independently of the
+library's loader: JSON semantics follow the reference's Scene::Parse*
+(Src/Core/scene.cpp:187-616) and OBJ import follows OBJLoader::LoadObject
+(Src/Utils/objloader.cpp:66-210) for triangulated OBJ files.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+import subprocess
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "librtref.so")
+MAT_STRIDE = 16
+MAT_TYPES = {"light": 0, "diffuse": 1, "pbr": 2, "dielectric": 3, "glass": 3, "subsurface": 4}
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [
+        ("num_tris", ctypes.c_int), ("pos", ctypes.c_void_p), ("nrm", ctypes.c_void_p), ("uv", ctypes.c_void_p),
+        ("mesh", ctypes.c_void_p), ("num_meshes", ctypes.c_int), ("mesh_material", ctypes.c_void_p),
+        ("mesh_light", ctypes.c_void_p), ("num_materials", ctypes.c_int), ("materials", ctypes.c_void_p),
+        ("num_lights", ctypes.c_int), ("light_type", ctypes.c_void_p), ("light_mesh", ctypes.c_void_p),
+        ("light_emission", ctypes.c_void_p), ("env_w", ctypes.c_int), ("env_h", ctypes.c_int),
+        ("env_rgb", ctypes.c_void_p), ("width", ctypes.c_int), ("height", ctypes.c_int), ("fovy", ctypes.c_float),
+        ("direction", ctypes.c_float * 3),
+    ]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("max_bounces", ctypes.c_int), ("rr", ctypes.c_float), ("rr_start_depth", ctypes.c_int),
+                ("seed", ctypes.c_uint32), ("flush_survivors", ctypes.c_int)]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.ref_scene_create.argtypes = [ctypes.POINTER(Desc)]
+        L.ref_scene_create.restype = vp
+        L.ref_scene_destroy.argtypes = [vp]
+        L.ref_bvh_num_nodes.argtypes = [vp]
+        L.ref_bvh_max_depth.argtypes = [vp]
+        L.ref_bvh_nodes.argtypes = [vp, vp]
+        L.ref_bvh_order.argtypes = [vp, vp]
+        L.ref_camera.argtypes = [vp, vp]
+        L.ref_trace.argtypes = [vp, vp, ctypes.c_int, vp, vp]
+        L.ref_render.argtypes = [vp, ctypes.POINTER(Params), ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp]
+        L.ref_render_f64.argtypes = [vp, ctypes.POINTER(Params), ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp]
+        L.ref_philox.argtypes = [vp, vp, vp]
+        L.ref_math.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float]
+        L.ref_math.restype = ctypes.c_float
+        L.ref_bsdf_eval.argtypes = [vp, vp, vp, vp, vp]
+        L.ref_bsdf_eval.restype = ctypes.c_float
+        L.ref_bsdf_sample.argtypes = [vp, vp, vp, vp, vp]
+        L.ref_bsdf_sample.restype = ctypes.c_float
+        L.ref_fresnel_dielectric.argtypes = [ctypes.c_float] * 3
+        L.ref_fresnel_dielectric.restype = ctypes.c_float
+        L.ref_ggx_d.argtypes = [vp, vp, vp, vp]
+        L.ref_ggx_d.restype = ctypes.c_float
+        L.ref_burley.argtypes = [vp, ctypes.c_float, ctypes.c_int]
+        L.ref_burley.restype = ctypes.c_float
+        L.ref_env_tables.argtypes = [vp, vp, vp]
+        L.ref_env_sample.argtypes = [vp, ctypes.c_float, ctypes.c_float, vp]
+        L.ref_env_pdf_uv.argtypes = [vp, ctypes.c_float, ctypes.c_float]
+        L.ref_env_pdf_uv.restype = ctypes.c_float
+        _LIB = L
+    return _LIB
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ------------------------------------------------------------------ scene parse
+def _f32(x) -> np.float32:
+    return np.float32(x)
+
+
+def _deg2rad(a: np.float32) -> np.float32:
+    return np.float32(float(np.float32(a) / np.float32(180.0)) * 3.1415926)
+
+
+def _mat4_mul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    r = np.zeros((4, 4), np.float32)
+    for i in range(4):
+        for j in range(4):
+            r[i, j] = ((a[i, 0] * b[0, j] + a[i, 1] * b[1, j]) + a[i, 2] * b[2, j]) + a[i, 3] * b[3, j]
+    return r
+
+
+def _xform_points(m: np.ndarray, p: np.ndarray) -> np.ndarray:
+    out = np.empty_like(p)
+    for r in range(3):
+        out[:, r] = ((m[r, 0] * p[:, 0] + m[r, 1] * p[:, 1]) + m[r, 2] * p[:, 2]) + m[r, 3] * np.float32(1)
+    return out
+
+
+def _dot3(a, b):
+    return a[..., 0] * b[..., 0] + (a[..., 1] * b[..., 1] + a[..., 2] * b[..., 2])
+
+
+def _normalize(v: np.ndarray) -> np.ndarray:
+    z = _dot3(v, v)
+    s = np.sqrt(z)
+    out = v.copy()
+    m = z > 0
+    out[m] = v[m] / s[m][:, None]
+    return out
+
+
+def _cross(a, b):
+    return np.stack([a[..., 1] * b[..., 2] - a[..., 2] * b[..., 1],
+                     a[..., 2] * b[..., 0] - a[..., 0] * b[..., 2],
+                     a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]], axis=-1)
+
+
+def _load_obj(path: str, xf: np.ndarray, xf_inv: np.ndarray):
+    """Triangulated OBJ: one mesh per (shape, usemtl) pair, face normals when a
+    face lacks per-vertex normals."""
+    V, VN, VT, faces = [], [], [], []
+    shape, has_faces, names, mtl = 0, False, [""], ""
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if not t or t[0].startswith("#"):
+                continue
+            if t[0] == "v":
+                V.append([float(np.float32(x)) for x in t[1:4]])
+            elif t[0] == "vn":
+                VN.append([float(np.float32(x)) for x in t[1:4]])
+            elif t[0] == "vt":
+                VT.append([float(np.float32(x)) for x in t[1:3]])
+            elif t[0] == "f":
+                poly = []
+                for tok in t[1:]:
+                    parts = tok.split("/")
+                    fix = lambda s, n: (int(s) - 1 if int(s) > 0 else n + int(s)) if s else -1
+                    vi = fix(parts[0], len(V))
+                    ti = fix(parts[1], len(VT)) if len(parts) > 1 else -1
+                    ni = fix(parts[2], len(VN)) if len(parts) > 2 else -1
+                    poly.append((vi, ti, ni))
+                for k in range(1, len(poly) - 1):
+                    faces.append((poly[0], poly[k], poly[k + 1], shape, mtl))
+                    has_faces = True
+            elif t[0] in ("o", "g"):
+                nm = " ".join(t[1:])
+                if has_faces:
+                    names.append(nm)
+                    shape += 1
+                    has_faces = False
+                else:
+                    names[shape] = nm
+            elif t[0] == "usemtl":
+                mtl = " ".join(t[1:])
+    V = np.array(V, np.float32).reshape(-1, 3)
+    VN = np.array(VN, np.float32).reshape(-1, 3)
+    VT = np.array(VT, np.float32).reshape(-1, 2)
+    meshes = {}
+    order = []
+    stem = os.path.splitext(os.path.basename(path))[0]
+    for (a, b, c, sh, mt) in faces:
+        key = (sh, mt)
+        if key not in meshes:
+            base = names[sh] if names[sh] else f"{stem}_shape_{sh}"
+            meshes[key] = {"name": base + ("::" + mt if mt else ""), "material": mt, "pos": [], "nrm": [], "uv": []}
+            order.append(key)
+        m = meshes[key]
+        p = _xform_points(xf, V[[a[0], b[0], c[0]]])
+        has_n = all(x[2] >= 0 for x in (a, b, c))
+        if has_n:
+            n = VN[[a[2], b[2], c[2]]]
+            n2 = np.empty_like(n)
+            for r in range(3):
+                n2[:, r] = ((xf_inv[0, r] * n[:, 0] + xf_inv[1, r] * n[:, 1]) + xf_inv[2, r] * n[:, 2]) + xf_inv[3, r] * np.float32(1)
+            n = _normalize(n2)
+        else:
+            fn = _normalize(_cross(p[1] - p[0], p[2] - p[0])[None, :])[0]
+            n = np.stack([fn, fn, fn])
+        uv = np.zeros((3, 2), np.float32)
+        for k, x in enumerate((a, b, c)):
+            if x[1] >= 0:
+                uv[k] = VT[x[1]]
+        m["pos"].append(p.reshape(9))
+        m["nrm"].append(n.reshape(9))
+        m["uv"].append(uv.reshape(6))
+    return [meshes[k] for k in order]
+
+
+def _read_env(path: str):
+    with open(path, "rb") as f:
+        data = f.read()
+    if data[:2] in (b"PF", b"Pf"):
+        color = data[:2] == b"PF"
+        toks, p = [], 2
+        while len(toks) < 3:
+            while data[p:p + 1].isspace():
+                p += 1
+            b = p
+            while not data[p:p + 1].isspace():
+                p += 1
+            toks.append(data[b:p])
+        w, h, scale = int(toks[0]), int(toks[1]), float(toks[2])
+        p += 1
+        ch = 3 if color else 1
+        arr = np.frombuffer(data, dtype="<f4" if scale < 0 else ">f4", count=w * h * ch, offset=p)
+        arr = arr.reshape(h, w, ch)[::-1].astype(np.float32)
+        if ch == 1:
+            arr = np.repeat(arr, 3, axis=2)
+        return w, h, np.ascontiguousarray(arr)
+    raise ValueError(f"the oracle reads PFM environment maps only: {path}")
+
+
+@dataclass
+class SceneArrays:
+    pos: np.ndarray
+    nrm: np.ndarray
+    uv: np.ndarray
+    mesh: np.ndarray
+    mesh_material: np.ndarray
+    mesh_light: np.ndarray
+    materials: np.ndarray
+    light_type: np.ndarray
+    light_mesh: np.ndarray
+    light_emission: np.ndarray
+    env_w: int
+    env_h: int
+    env_rgb: Optional[np.ndarray]
+    width: int
+    height: int
+    fovy: float
+    direction: tuple
+    max_bounces: int
+    rr: float
+    spp: int
+    seed: int
+    rr_start_depth: int
+    num_lights: int = 0
+
+
+def parse_scene(json_path: str) -> SceneArrays:
+    base = os.path.dirname(os.path.abspath(json_path))
+    with open(json_path) as f:
+        doc = json.load(f)
+    cam = doc["camera"]
+    W, H = (cam["resolution"] if "resolution" in cam else (800, 600))
+    mats_json = doc["materials"]
+    materials = np.zeros((max(1, len(mats_json)), MAT_STRIDE), np.float32)
+    materials[:, 0] = 1
+    materials[:, 7] = 0.5
+    materials[:, 9] = 1.0
+    materials[:, 10] = 1.0
+    name_to_mat = {}
+    for i, m in enumerate(mats_json):
+        if "name" not in m or "type" not in m or m["type"] not in MAT_TYPES:
+            continue
+        t = MAT_TYPES[m["type"]]
+        r = materials[i]
+        r[0] = t
+        if t == 1:
+            r[1:4] = m.get("basecolor", [0, 0, 0])
+        elif t == 0:
+            r[4:7] = m.get("emission", [0, 0, 0])
+        elif t == 2:
+            r[1:4] = m.get("basecolor", [0, 0, 0])
+            r[8] = m.get("metallic", 0.0)
+            r[7] = m.get("roughness", 0.5)
+        elif t == 4:
+            r[1:4] = m.get("basecolor", [0, 0, 0])
+            r[11:14] = m.get("scatter_distance", [1, 1, 1])
+            if "ior" in m:
+                r[10] = m["ior"]
+            if "roughness" in m:
+                r[7] = m["roughness"]
+            r[8] = 0
+        elif t == 3:
+            r[1:4] = m.get("basecolor", [1, 1, 1])
+            r[10] = m.get("ior", 1.5)
+            r[9] = m.get("transmission", 1.0)
+            r[7] = m.get("roughness", 0.0)
+            r[8] = 0
+        name_to_mat[m["name"]] = i
+    pos, nrm, uv, mesh, mmat, mnames = [], [], [], [], [], []
+    for o in doc["objects"]:
+        if "name" not in o or "path" not in o:
+            continue
+        tr = np.eye(4, dtype=np.float32)
+        rot = np.eye(4, dtype=np.float32)
+        scl = np.eye(4, dtype=np.float32)
+        if "translate" in o:
+            tr[0:3, 3] = o["translate"]
+        for r in o.get("rotate", []):
+            for ax, key in enumerate(("rotateX", "rotateY", "rotateZ")):
+                if key in r:
+                    rad = _deg2rad(np.float32(r[key]))
+                    c, s = np.float32(np.cos(rad)), np.float32(np.sin(rad))
+                    R = np.eye(4, dtype=np.float32)
+                    i0, i1 = [(1, 2), (0, 2), (0, 1)][ax]
+                    R[i0, i0], R[i1, i1] = c, c
+                    if ax == 1:
+                        R[0, 2], R[2, 0] = s, -s
+                    else:
+                        R[i0, i1], R[i1, i0] = -s, s
+                    rot = _mat4_mul(R, rot)
+                    break
+        if "scale" in o:
+            scl[0, 0], scl[1, 1], scl[2, 2] = o["scale"]
+        xf = _mat4_mul(_mat4_mul(tr, rot), scl)
+        xf_inv = np.linalg.inv(xf.astype(np.float64)).astype(np.float32)
+        path = o["path"] if os.path.isabs(o["path"]) else os.path.join(base, o["path"])
+        loaded = _load_obj(path, xf, xf_inv)
+        expanded = len(loaded) > 1
+        for lm in loaded:
+            if "material" in o:
+                mn = o["material"]
+            elif isinstance(o.get("material_overrides"), dict) and lm["material"] in o["material_overrides"]:
+                mn = o["material_overrides"][lm["material"]]
+            elif lm["material"] and lm["material"] in name_to_mat:
+                mn = lm["material"]
+            else:
+                mn = o.get("default_material", "")
+            if mn not in name_to_mat:
+                continue
+            mid = len(mmat)
+            mmat.append(name_to_mat[mn])
+            mnames.append(o["name"] + "::" + lm["name"] if expanded else o["name"])
+            pos += lm["pos"]
+            nrm += lm["nrm"]
+            uv += lm["uv"]
+            mesh += [mid] * len(lm["pos"])
+    mesh_light = np.full(max(1, len(mmat)), -1, np.int32)
+    lj = doc["lights"]
+    lt = np.zeros(max(1, len(lj)), np.int32)
+    lmsh = np.full(max(1, len(lj)), -1, np.int32)
+    lem = np.zeros((max(1, len(lj)), 3), np.float32)
+    env = (0, 0, None)
+    for i, L in enumerate(lj):
+        if L.get("type") == "area":
+            if "ke" in L:
+                lem[i] = L["ke"]
+            if "material" in L and L["material"] in name_to_mat:
+                lem[i] = materials[name_to_mat[L["material"]], 4:7]
+            m = mnames.index(L["lightMeshName"])
+            lmsh[i] = m
+            mesh_light[m] = i
+        elif L.get("type") == "env":
+            lt[i] = 2
+            p = L["path"] if os.path.isabs(L["path"]) else os.path.join(base, L["path"])
+            env = _read_env(p)
+    return SceneArrays(
+        pos=np.ascontiguousarray(np.array(pos, np.float32).reshape(-1, 9)),
+        nrm=np.ascontiguousarray(np.array(nrm, np.float32).reshape(-1, 9)),
+        uv=np.ascontiguousarray(np.array(uv, np.float32).reshape(-1, 6)),
+        mesh=np.array(mesh, np.int32), mesh_material=np.array(mmat if mmat else [0], np.int32),
+        mesh_light=mesh_light, materials=materials, light_type=lt, light_mesh=lmsh, light_emission=lem,
+        env_w=env[0], env_h=env[1], env_rgb=env[2], width=int(W), height=int(H), fovy=float(cam.get("fov", 45.0)),
+        direction=tuple(cam.get("direction", [0, 0, -1])), max_bounces=int(cam.get("max_bounces", 5)),
+        rr=float(cam.get("rr", 0.8)), spp=int(cam.get("spp", 16)), seed=int(cam.get("seed", 1234)),
+        rr_start_depth=int(cam.get("rr_start_depth", 0)), num_lights=len(lj),
+    )
+
+
+class RefScene:
+    """The oracle's scene: BVH, camera and env tables built in C from SceneArrays."""
+
+    def __init__(self, a: SceneArrays):
+        self.a = a
+        d = Desc()
+        d.num_tris = a.pos.shape[0]
+        d.pos, d.nrm, d.uv, d.mesh = _p(a.pos), _p(a.nrm), _p(a.uv), _p(a.mesh)
+        d.num_meshes = int(a.mesh_material.shape[0])
+        d.mesh_material, d.mesh_light = _p(a.mesh_material), _p(a.mesh_light)
+        d.num_materials = int(a.materials.shape[0])
+        d.materials = _p(a.materials)
+        d.num_lights = int(a.num_lights)
+        d.light_type, d.light_mesh, d.light_emission = _p(a.light_type), _p(a.light_mesh), _p(a.light_emission)
+        d.env_w, d.env_h = a.env_w, a.env_h
+        d.env_rgb = _p(a.env_rgb) if a.env_rgb is not None else None
+        d.width, d.height, d.fovy = a.width, a.height, a.fovy
+        d.direction[:] = [np.float32(x) for x in a.direction]
+        self._desc = d
+        self._h = lib().ref_scene_create(ctypes.byref(d))
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().ref_scene_destroy(self._h)
+        except Exception:
+            pass
+
+    def params(self, max_bounces=None, rr=None, rr_start_depth=None, seed=None, flush_survivors=False) -> Params:
+        a = self.a
+        p = Params()
+        p.max_bounces = a.max_bounces if max_bounces is None else max_bounces
+        p.rr = a.rr if rr is None else rr
+        p.rr_start_depth = a.rr_start_depth if rr_start_depth is None else rr_start_depth
+        p.seed = a.seed if seed is None else seed
+        p.flush_survivors = int(flush_survivors)
+        return p
+
+    def bvh(self):
+        n = lib().ref_bvh_num_nodes(self._h)
+        nodes = np.zeros((n, 8), np.int32)
+        order = np.zeros(self.a.pos.shape[0], np.int32)
+        lib().ref_bvh_nodes(self._h, _p(nodes))
+        lib().ref_bvh_order(self._h, _p(order))
+        return nodes, order
+
+    def max_depth(self) -> int:
+        return lib().ref_bvh_max_depth(self._h)
+
+    def camera(self) -> np.ndarray:
+        out = np.zeros(12, np.float32)
+        lib().ref_camera(self._h, _p(out))
+        return out.reshape(4, 3)
+
+    def trace(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        hits = np.zeros((n, 4), np.float32)
+        counts = np.zeros((n, 2), np.uint32)
+        lib().ref_trace(self._h, _p(rays), n, _p(hits), _p(counts))
+        return hits[:, 0].copy(), hits[:, 1].copy(), hits[:, 2].copy(), hits[:, 3].copy().view(np.int32), counts
+
+    def render(self, spp: int, params: Params, sample0: int = 0, pixels: Optional[np.ndarray] = None):
+        """Returns (direct, indirect) sums, (npix, 4) float32, and [ext, shadow] ray counts."""
+        npix = self.a.width * self.a.height if pixels is None else len(pixels)
+        d = np.zeros((npix, 4), np.float32)
+        i = np.zeros((npix, 4), np.float32)
+        st = np.zeros(2, np.uint64)
+        px = None if pixels is None else np.ascontiguousarray(pixels, np.int32)
+        lib().ref_render(self._h, ctypes.byref(params), sample0, spp, _p(px) if px is not None else None, npix,
+                         _p(d), _p(i), _p(st))
+        return d, i, st
+
+    def render_f64(self, spp: int, params: Params, sample0: int = 0, pixels: Optional[np.ndarray] = None):
+        npix = self.a.width * self.a.height if pixels is None else len(pixels)
+        d = np.zeros((npix, 4), np.float64)
+        i = np.zeros((npix, 4), np.float64)
+        px = None if pixels is None else np.ascontiguousarray(pixels, np.int32)
+        lib().ref_render_f64(self._h, ctypes.byref(params), sample0, spp, _p(px) if px is not None else None, npix,
+                             _p(d), _p(i))
+        return d, i
+
+    def env_tables(self):
+        row = np.zeros((self.a.env_h, self.a.env_w), np.float32)
+        marg = np.zeros(self.a.env_h, np.float32)
+        lib().ref_env_tables(self._h, _p(row), _p(marg))
+        return row, marg
+
+
+def philox(ctr, key) -> np.ndarray:
+    c = np.array(ctr, np.uint32)
+    k = np.array(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().ref_philox(_p(c), _p(k), _p(out))
+    return out
+
+
+def math(op: int, a: float, b: float = 0.0) -> float:
+    return float(lib().ref_math(op, a, b))

@@ -1,0 +1,44 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP library)")
+
+
+@pytest.fixture(scope="session")
+def scene_dir(tmp_path_factory):
+    """Generated benchmark scenes (reference scene-JSON + OBJ format)."""
+    from cudaraytracer_amd import scenes
+
+    d = tmp_path_factory.mktemp("scenes")
+    paths = {}
+    for name, fn in scenes.WORKLOADS.items():
+        paths[name] = scenes.write_scene(fn(), str(d / name), name)
+    # Reduced-resolution copy of the spheres scene for full-image parity runs.
+    paths["cornell_spheres_128"] = scenes.write_scene(scenes.cornell_spheres(width=128, height=128, spp=4),
+                                                      str(d / "cornell_spheres_128"), "cornell_spheres_128")
+    return paths
+
+
+@pytest.fixture(scope="session")
+def oracle_lib():
+    from oracle import rtref
+
+    rtref.build()
+    return rtref
+
+
+def gpu_available() -> bool:
+    try:
+        import torch  # noqa: F401  (device discovery only)
+
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False

@@ -1,0 +1,175 @@
+"""GPU parity: the HIP kernels against the CPU oracle on the same inputs.
+
+Integer / index results (hit primitive, traversal counts) and the FP32
+radiance sums are required to be bit-identical: both sides run the same FP32
+operation sequence with contraction off and the same counter-based RNG."""
+import numpy as np
+import pytest
+
+import cudaraytracer_amd as rt
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.int32)
+
+
+@pytest.mark.parametrize("op,lo,hi", [(2, -20.0, 20.0), (3, -20.0, 20.0), (4, -1.0, 1.0), (5, -90.0, 90.0),
+                                      (6, 1e-30, 1e30), (7, -5.0, 5.0)])
+def test_device_transcendentals_bit_exact(oracle_lib, op, lo, hi):
+    rng = np.random.default_rng(op)
+    if op == 6:
+        a = np.exp(rng.uniform(np.log(lo), np.log(hi), 20000)).astype(np.float32)
+    else:
+        a = rng.uniform(lo, hi, 20000).astype(np.float32)
+    b = rng.uniform(-5, 5, 20000).astype(np.float32)
+    got = rt.device_math(op, a, b)
+    want = np.array([oracle_lib.math(op, float(x), float(y)) for x, y in zip(a, b)], np.float32)
+    assert np.array_equal(_bits(got), _bits(want))
+
+
+def test_device_ieee_ops_bit_exact():
+    rng = np.random.default_rng(0)
+    a = (rng.standard_normal(1 << 18) * 1e3).astype(np.float32)
+    b = (rng.standard_normal(1 << 18) + 0.01).astype(np.float32)
+    assert np.array_equal(_bits(rt.device_math(0, a, b)), _bits(a / b))
+    assert np.array_equal(_bits(rt.device_math(1, np.abs(a), b)), _bits(np.sqrt(np.abs(a))))
+    assert np.array_equal(_bits(rt.device_math(9, b, b)), _bits(np.float32(1) / b))
+
+
+def test_device_philox_matches_oracle(oracle_lib):
+    rng = np.random.default_rng(5)
+    n = 4096
+    a = rng.integers(0, 2**31, n).astype(np.int32).view(np.float32)
+    b = rng.integers(0, 2**31, n).astype(np.int32).view(np.float32)
+    got = rt.device_math(8, a, b).view(np.uint32)
+    for i in range(0, n, 97):
+        c = int(a[i:i + 1].view(np.uint32)[0])
+        k = int(b[i:i + 1].view(np.uint32)[0])
+        want = oracle_lib.philox([c, c ^ 0x9E3779B9, i, 7], [k, 0x5EEDC0DE])
+        assert got[i] == want[i & 3]
+
+
+def _camera_and_random_rays(scene_arrays, rs, n_random=20000, seed=3):
+    cam = rs.camera()
+    pos, ld, du, dv = cam
+    w, h = scene_arrays.width, scene_arrays.height
+    rng = np.random.default_rng(seed)
+    i = rng.integers(0, w, n_random)
+    j = rng.integers(0, h, n_random)
+    p = ld[None] + (i[:, None] + rng.random((n_random, 1))) * du[None] + (j[:, None] + rng.random((n_random, 1))) * dv[None]
+    d = p - pos[None]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    cam_rays = np.concatenate([np.repeat(pos[None], n_random, 0), np.zeros((n_random, 1)), d,
+                               np.full((n_random, 1), 1e8)], axis=1)
+    lo, hi = scene_arrays.pos.reshape(-1, 3).min(0), scene_arrays.pos.reshape(-1, 3).max(0)
+    o = rng.uniform(lo, hi, (n_random, 3))
+    d2 = rng.standard_normal((n_random, 3))
+    d2 /= np.linalg.norm(d2, axis=1, keepdims=True)
+    d2[:16] = np.eye(3)[np.arange(16) % 3] * np.where(np.arange(16) % 2, 1, -1)[:, None]  # axis-parallel
+    tmin = rng.choice([0.0, 1e-6, 1e-4], (n_random, 1))
+    tmax = rng.choice([1e8, 0.5, 2.0], (n_random, 1))
+    inner = np.concatenate([o, tmin, d2, tmax], axis=1)
+    return np.concatenate([cam_rays, inner]).astype(np.float32)
+
+
+@pytest.mark.parametrize("name", ["cornell_small", "cornell_spheres"])
+def test_traversal_bit_exact(scene_dir, oracle_lib, name):
+    s = rt.Scene(scene_dir[name])
+    a = oracle_lib.parse_scene(scene_dir[name])
+    rs = oracle_lib.RefScene(a)
+    rays = _camera_and_random_rays(a, rs)
+    gt, gb1, gb2, gp, gc = s.trace(rays)
+    rt_, rb1, rb2, rp, rc = rs.trace(rays)
+    assert np.array_equal(gp, rp)
+    hit = rp >= 0
+    assert hit.mean() > 0.3
+    assert np.array_equal(_bits(gt), _bits(rt_))
+    assert np.array_equal(_bits(gb1[hit]), _bits(rb1[hit])) and np.array_equal(_bits(gb2[hit]), _bits(rb2[hit]))
+    assert np.array_equal(gc, rc)  # same nodes tested and triangles tested per ray
+
+
+def _gpu_render(path, spp, **kw):
+    s = rt.Scene(path)
+    r = rt.Renderer(s, **kw)
+    r.render(spp)
+    d = r.read_accum(rt.RT_ACCUM_DIRECT).reshape(-1, 4)
+    i = r.read_accum(rt.RT_ACCUM_INDIRECT).reshape(-1, 4)
+    st = r.stats()
+    return d, i, st, r
+
+
+@pytest.mark.parametrize("name,spp", [("cornell_small", 4), ("cornell_spheres_128", 4)])
+def test_render_bit_exact_full_image(scene_dir, oracle_lib, name, spp):
+    d, i, st, _ = _gpu_render(scene_dir[name], spp)
+    a = oracle_lib.parse_scene(scene_dir[name])
+    rs = oracle_lib.RefScene(a)
+    rd, ri, rst = rs.render(spp, rs.params())
+    assert np.array_equal(_bits(d[:, :3]), _bits(rd[:, :3]))
+    assert np.array_equal(_bits(i[:, :3]), _bits(ri[:, :3]))
+    assert st.extension_rays == rst[0]
+    assert st.shadow_rays == rst[1]
+
+
+def test_render_reference_semantics_variants(scene_dir, oracle_lib):
+    """Roulette from bounce 0 (the reference's rule) and flushed survivors."""
+    path = scene_dir["cornell_spheres_128"]
+    a = oracle_lib.parse_scene(path)
+    rs = oracle_lib.RefScene(a)
+    for kw in ({"rr_start_depth": 0}, {"flush_survivors": True, "max_bounces": 3}):
+        d, i, _, _ = _gpu_render(path, 2, **kw)
+        rd, ri, _ = rs.render(2, rs.params(**kw))
+        assert np.array_equal(_bits(d[:, :3]), _bits(rd[:, :3]))
+        assert np.array_equal(_bits(i[:, :3]), _bits(ri[:, :3]))
+
+
+def test_batching_does_not_change_results(scene_dir):
+    path = scene_dir["cornell_spheres_128"]
+    d1, i1, _, _ = _gpu_render(path, 6, batch_spp=1)
+    d6, i6, _, _ = _gpu_render(path, 6, batch_spp=6)
+    d4, i4, _, _ = _gpu_render(path, 6, batch_spp=4)
+    assert np.array_equal(d1, d6) and np.array_equal(i1, i6)
+    assert np.array_equal(d4, d6) and np.array_equal(i4, i6)
+    s = rt.Scene(path)
+    r = rt.Renderer(s)
+    r.render(2)
+    r.render(4)
+    assert np.array_equal(r.read_accum(rt.RT_ACCUM_DIRECT).reshape(-1, 4), d6)
+
+
+def test_full_size_workload_matches_oracle_on_sampled_pixels(scene_dir, oracle_lib):
+    """512x512, 64 spp, depth 12: every sampled pixel's sums are bit-identical."""
+    path = scene_dir["cornell_spheres"]
+    d, i, st, r = _gpu_render(path, 64)
+    img = r.read_accum(rt.RT_ACCUM_MEAN)
+    assert np.isfinite(img).all()
+    a = oracle_lib.parse_scene(path)
+    rs = oracle_lib.RefScene(a)
+    pix = np.random.default_rng(9).choice(a.width * a.height, 600, replace=False).astype(np.int32)
+    rd, ri, _ = rs.render(64, rs.params(), pixels=pix)
+    assert np.array_equal(_bits(d[pix, :3]), _bits(rd[:, :3]))
+    assert np.array_equal(_bits(i[pix, :3]), _bits(ri[:, :3]))
+    assert st.spp_done == 64
+
+
+def test_rgba8_readback(scene_dir):
+    _, _, _, r = _gpu_render(scene_dir["cornell_small"], 4)
+    mean = r.read_accum(rt.RT_ACCUM_MEAN)
+    img8 = r.read_rgba8()
+    want = np.clip(np.power(mean[..., :3], np.float32(1 / 2.2)) * 255.0, 0, 255).astype(np.uint8)[::-1]
+    assert np.abs(img8[..., :3].astype(int) - want.astype(int)).max() <= 1
+    assert (img8[..., 3] == 255).all()
+
+
+def test_traversal_counters(scene_dir):
+    s = rt.Scene(scene_dir["cornell_spheres_128"])
+    r = rt.Renderer(s)
+    r.render(2, count_traversal=True)
+    st = r.stats()
+    assert st.extension_node_visits > st.extension_rays
+    assert st.extension_tri_tests > 0 and st.shadow_node_visits > 0
+    r.reset_stats()
+    r.render(2)
+    st2 = r.stats()
+    assert st2.extension_node_visits == 0 and st2.extension_rays == st.extension_rays
