@@ -33,8 +33,11 @@ LIGHT_RADIANCE = (17.0, 12.0, 4.0)
 class MeshSpec:
     name: str
     material: str
-    pos: np.ndarray                    # (n, 3, 3) float32
+    pos: Optional[np.ndarray] = None   # (n, 3, 3) float32 triangle soup
     nrm: Optional[np.ndarray] = None   # (n, 3, 3) float32 per-vertex normals or None
+    verts: Optional[np.ndarray] = None  # indexed form: (m, 3) float32 shared vertices
+    vnorm: Optional[np.ndarray] = None  # (m, 3) float32 shared normals
+    faces: Optional[np.ndarray] = None  # (n, 3) int vertex indices
 
 
 @dataclass
@@ -202,6 +205,9 @@ def _fmt(x: float) -> str:
 
 
 def write_obj(path: str, mesh: MeshSpec) -> None:
+    if mesh.faces is not None:
+        _write_indexed_obj(path, mesh)
+        return
     n = mesh.pos.shape[0]
     lines = [f"o {mesh.name}"]
     for t in range(n):
@@ -224,6 +230,126 @@ def write_obj(path: str, mesh: MeshSpec) -> None:
         f.write("\n".join(lines) + "\n")
 
 
+def _write_indexed_obj(path: str, mesh: MeshSpec) -> None:
+    with open(path, "w") as f:
+        f.write(f"o {mesh.name}\n")
+        np.savetxt(f, mesh.verts.astype(np.float32), fmt="v %.9g %.9g %.9g")
+        if mesh.vnorm is not None:
+            np.savetxt(f, mesh.vnorm.astype(np.float32), fmt="vn %.9g %.9g %.9g")
+            fi = mesh.faces.astype(np.int64) + 1
+            np.savetxt(f, np.repeat(fi, 2, axis=1), fmt="f %d//%d %d//%d %d//%d")
+        else:
+            np.savetxt(f, mesh.faces.astype(np.int64) + 1, fmt="f %d %d %d")
+
+
+def write_pfm(path: str, rgb: np.ndarray) -> None:
+    """Little-endian colour PFM; rgb is (h, w, 3) with row 0 = top of the image."""
+    h, w, _ = rgb.shape
+    with open(path, "wb") as f:
+        f.write(f"PF\n{w} {h}\n-1.0\n".encode())
+        f.write(np.ascontiguousarray(rgb[::-1].astype("<f4")).tobytes())
+
+
+def sky_env(width: int = 2048, height: int = 1024, sun_dir=(0.35, 0.6, -0.72), sun_peak: float = 1e4,
+            sun_radius_deg: float = 0.5) -> np.ndarray:
+    """Procedural HDR sky: zenith-to-horizon gradient, dark ground, Gaussian sun
+    lobe of the given peak radiance whose 1-sigma radius is `sun_radius_deg`.
+    Pixel (x, y) looks along theta = (y + 0.5) / h * pi from +y, phi = (x + 0.5) / w * 2 pi
+    with direction (sin theta sin phi, cos theta, sin theta cos phi)."""
+    y = (np.arange(height) + 0.5) / height * np.pi
+    x = (np.arange(width) + 0.5) / width * 2 * np.pi
+    th, ph = np.meshgrid(y, x, indexing="ij")
+    d = np.stack([np.sin(th) * np.sin(ph), np.cos(th), np.sin(th) * np.cos(ph)], axis=-1)
+    up = np.clip(d[..., 1], 0, 1)[..., None]
+    zenith = np.array([0.25, 0.45, 1.0])
+    horizon = np.array([1.0, 0.95, 0.9])
+    sky = horizon * (1 - up) ** 3 + zenith * (1 - (1 - up) ** 3)
+    ground = np.array([0.12, 0.1, 0.08])
+    rgb = np.where(d[..., 1:2] >= 0, sky, ground)
+    s = np.array(sun_dir, np.float64)
+    s /= np.linalg.norm(s)
+    ang = np.arccos(np.clip((d * s).sum(-1), -1, 1))
+    lobe = sun_peak * np.exp(-0.5 * (ang / np.radians(sun_radius_deg)) ** 2)
+    rgb = rgb + lobe[..., None] * np.array([1.0, 0.95, 0.85])
+    return rgb.astype(np.float32)
+
+
+def _fbm(n: int, seed: int, octaves: int = 6) -> np.ndarray:
+    """Fractional Brownian motion heights on an n x n grid (value noise, smoothstep)."""
+    rng = np.random.default_rng(seed)
+    u = np.linspace(0.0, 1.0, n)
+    h = np.zeros((n, n))
+    amp, freq = 1.0, 4
+    for _ in range(octaves):
+        lat = rng.standard_normal((freq + 2, freq + 2))
+        g = u * freq
+        i0 = np.floor(g).astype(int)
+        t = g - i0
+        t = t * t * (3 - 2 * t)
+        a = lat[i0][:, i0]
+        b = lat[i0 + 1][:, i0]
+        c = lat[i0][:, i0 + 1]
+        d = lat[i0 + 1][:, i0 + 1]
+        tx, ty = t[None, :], t[:, None]
+        h += amp * ((a * (1 - ty) + b * ty) * (1 - tx) + (c * (1 - ty) + d * ty) * tx)
+        amp *= 0.5
+        freq *= 2
+    return h
+
+
+def terrain_mesh(grid: int, size: float, height: float, seed: int) -> MeshSpec:
+    """(grid-1)^2 * 2 triangles over [-size/2, size/2]^2, smooth normals, faces up."""
+    h = _fbm(grid, seed) * height
+    u = np.linspace(-size / 2, size / 2, grid)
+    X, Z = np.meshgrid(u, u)  # row index = z
+    verts = np.stack([X, h, Z], axis=-1).reshape(-1, 3)
+    gz, gx = np.gradient(h, u, u)
+    nrm = np.stack([-gx, np.ones_like(h), -gz], axis=-1).reshape(-1, 3)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    idx = np.arange(grid * grid).reshape(grid, grid)
+    a, b, c, d = idx[:-1, :-1], idx[:-1, 1:], idx[1:, :-1], idx[1:, 1:]
+    # (v1-v0) x (v2-v0) points +y for (a, c, b) and (b, c, d) with x along columns, z along rows.
+    faces = np.concatenate([np.stack([a, c, b], -1).reshape(-1, 3), np.stack([b, c, d], -1).reshape(-1, 3)])
+    return MeshSpec("terrain", "terrain", verts=verts.astype(np.float32), vnorm=nrm.astype(np.float32), faces=faces)
+
+
+def burley_from_coefficients(sigma_a, sigma_s):
+    """Map volume coefficients (1/m) to the reference's Burley parameters:
+    basecolor = single-scattering albedo, scatter_distance = mean free path / s(A)
+    with Christensen & Burley's (2015) fit s(A) = 1.85 - A + 7 |A - 0.8|^3."""
+    sa, ss = np.asarray(sigma_a, float), np.asarray(sigma_s, float)
+    albedo = ss / (sa + ss)
+    s = 1.85 - albedo + 7 * np.abs(albedo - 0.8) ** 3
+    return albedo, (1.0 / (sa + ss)) / s
+
+
+def terrain_env(seed: int = 3, grid: int = 1001, size: float = 100.0, env_w: int = 2048, env_h: int = 1024,
+                width: int = 1920, height: int = 1080, spp: int = 32, depth: int = 16,
+                sphere_subdiv: int = 4) -> SceneSpec:
+    rng = np.random.default_rng(seed)
+    terrain = terrain_mesh(grid, size, height=size * 0.06, seed=seed)
+    V, F = icosphere(sphere_subdiv)
+    r = size * 0.04
+    # Rest the sphere on the terrain near the centre.
+    c = grid // 2
+    y0 = float(terrain.verts.reshape(grid, grid, 3)[c, c, 1])
+    sphere = MeshSpec("sss_ball", "sss", pos=(V[F] * r + np.array([0.0, y0 + r * 0.9, 0.0])).astype(np.float32),
+                      nrm=V[F].astype(np.float32))
+    sig_a = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 3))
+    sig_s = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 3))
+    albedo, dist = burley_from_coefficients(sig_a, sig_s)
+    mats = [
+        {"name": "terrain", "type": "diffuse", "basecolor": [0.35, 0.3, 0.22]},
+        {"name": "sss", "type": "subsurface", "basecolor": [float(x) for x in albedo],
+         "scatter_distance": [float(x) for x in dist], "ior": 1.4, "roughness": 0.3},
+    ]
+    spec = SceneSpec(camera=dict(_camera(width, height, spp, depth), direction=[0.35, 0.45, -1.0]),
+                     materials=mats, meshes=[terrain, sphere],
+                     lights=[{"name": "sky", "type": "env", "path": "sky.pfm"}])
+    spec.env = sky_env(env_w, env_h)
+    return spec
+
+
 def write_scene(spec: SceneSpec, out_dir: str, name: str = "scene") -> str:
     """Write <out_dir>/<name>.json and one OBJ per mesh; returns the JSON path."""
     os.makedirs(out_dir, exist_ok=True)
@@ -233,6 +359,11 @@ def write_scene(spec: SceneSpec, out_dir: str, name: str = "scene") -> str:
         write_obj(os.path.join(out_dir, fn), m)
         objects.append({"name": m.name, "path": fn, "translate": [0, 0, 0], "scale": [1, 1, 1],
                         "rotate": [{"rotateX": 0}], "material": m.material})
+    env = getattr(spec, "env", None)
+    if env is not None:
+        for L in spec.lights:
+            if L.get("type") == "env":
+                write_pfm(os.path.join(out_dir, L["path"]), env)
     doc = {"camera": spec.camera, "textures": spec.textures, "materials": spec.materials, "objects": objects,
            "lights": spec.lights}
     path = os.path.join(out_dir, f"{name}.json")
@@ -244,4 +375,8 @@ def write_scene(spec: SceneSpec, out_dir: str, name: str = "scene") -> str:
 WORKLOADS = {
     "cornell_small": cornell_small,
     "cornell_spheres": cornell_spheres,
+}
+# Large scenes are generated on demand (the full terrain is 2M triangles).
+LARGE_WORKLOADS = {
+    "terrain_env": terrain_env,
 }

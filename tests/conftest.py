@@ -24,6 +24,10 @@ def scene_dir(tmp_path_factory):
     # Reduced-resolution copy of the spheres scene for full-image parity runs.
     paths["cornell_spheres_128"] = scenes.write_scene(scenes.cornell_spheres(width=128, height=128, spp=4),
                                                       str(d / "cornell_spheres_128"), "cornell_spheres_128")
+    # Reduced terrain + subsurface sphere + HDR environment light.
+    paths["terrain_small"] = scenes.write_scene(
+        scenes.terrain_env(grid=101, env_w=256, env_h=128, width=160, height=90, spp=4),
+        str(d / "terrain_small"), "terrain_small")
     return paths
 
 

@@ -74,7 +74,7 @@ def _camera_and_random_rays(scene_arrays, rs, n_random=20000, seed=3):
     return np.concatenate([cam_rays, inner]).astype(np.float32)
 
 
-@pytest.mark.parametrize("name", ["cornell_small", "cornell_spheres"])
+@pytest.mark.parametrize("name", ["cornell_small", "cornell_spheres", "terrain_small"])
 def test_traversal_bit_exact(scene_dir, oracle_lib, name):
     s = rt.Scene(scene_dir[name])
     a = oracle_lib.parse_scene(scene_dir[name])
@@ -84,7 +84,7 @@ def test_traversal_bit_exact(scene_dir, oracle_lib, name):
     rt_, rb1, rb2, rp, rc = rs.trace(rays)
     assert np.array_equal(gp, rp)
     hit = rp >= 0
-    assert hit.mean() > 0.3
+    assert hit.mean() > 0.15
     assert np.array_equal(_bits(gt), _bits(rt_))
     assert np.array_equal(_bits(gb1[hit]), _bits(rb1[hit])) and np.array_equal(_bits(gb2[hit]), _bits(rb2[hit]))
     assert np.array_equal(gc, rc)  # same nodes tested and triangles tested per ray
@@ -100,7 +100,7 @@ def _gpu_render(path, spp, **kw):
     return d, i, st, r
 
 
-@pytest.mark.parametrize("name,spp", [("cornell_small", 4), ("cornell_spheres_128", 4)])
+@pytest.mark.parametrize("name,spp", [("cornell_small", 4), ("cornell_spheres_128", 4), ("terrain_small", 4)])
 def test_render_bit_exact_full_image(scene_dir, oracle_lib, name, spp):
     d, i, st, _ = _gpu_render(scene_dir[name], spp)
     a = oracle_lib.parse_scene(scene_dir[name])
@@ -169,6 +169,7 @@ def test_traversal_counters(scene_dir):
     st = r.stats()
     assert st.extension_node_visits > st.extension_rays
     assert st.extension_tri_tests > 0 and st.shadow_node_visits > 0
+    r.reset()
     r.reset_stats()
     r.render(2)
     st2 = r.stats()
