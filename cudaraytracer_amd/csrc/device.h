@@ -14,8 +14,7 @@ struct DevScene {
     const PackedNode* nodes;      // interior nodes, 64 B each
     uint32_t root_ref;
     const float4* tri_isect;      // 3 x float4 per triangle (BVH order): v0, e1 = v1-v0, e2 = v2-v0
-    const float4* tri_attr;       // 7 x float4 per triangle (BVH order), see kAttr* below
-    const int2* mesh_info;        // (material, light) per mesh
+    const float4* tri_attr;       // 8 x float4 per triangle (BVH order), see below
     const MaterialRec* materials;
     const LightRec* lights;
     int num_lights;
@@ -30,12 +29,25 @@ struct DevScene {
     const float* env_pdf;         // env_h * env_w, pdf in uv-space
 };
 
-// tri_attr layout, 7 float4 per triangle:
-//   [0] v0.xyz, mesh id (int bits)     [1] v1.xyz, gn.x
+// tri_attr layout, 8 float4 = one 128-byte line per triangle:
+//   [0] v0.xyz, -                      [1] v1.xyz, gn.x
 //   [2] v2.xyz, gn.y                   [3] n0.xyz, gn.z
 //   [4] n1.xyz, uv0.x                  [5] n2.xyz, uv0.y
-//   [6] uv1.xy, uv2.xy
-constexpr int kAttrStride = 7;
+//   [6] uv1.xy, uv2.xy                 [7] mesh, material, light, shading class (int bits)
+// Material and light ids ride in the triangle's line so shading needs no
+// mesh-table lookup between the hit and the material.
+constexpr int kAttrStride = 8;
+
+// Shading classes: extend bins every path by the class of what it hit, and
+// one specialised shade kernel per class runs without cross-material divergence.
+enum ShadeClass : int {
+    kClsTerminal = 0,   // miss (environment) or light hit: the path ends
+    kClsDiffuse = 1,
+    kClsPbr = 2,
+    kClsDielectric = 3,
+    kClsSubsurface = 4,
+    kNumClasses = 5,
+};
 
 // Path flags kept in the .w lane of the direction word.
 constexpr uint32_t kFlagFirst = 1u;
@@ -54,17 +66,26 @@ struct FrameParams {
     int bounce;           // current bounce (set per launch)
 };
 
-// Counter slots (int32) in device memory.  No memset is used: kernels zero
-// the slots the next kernel needs, so the bounce loop can be graph-captured.
-enum CounterSlot : int {
-    kCntQueue0 = 0,
+// Work-queue counters.  Every queue is split into kSlices slices; each slice
+// has its own counter of every kind on its own 256-byte line, because one hot
+// atomic address serialises the whole chip (about 11 ns per atomic on MI355X).
+// Counter (kind, slice) lives at int offset (kind * kSlices + slice) * kCounterStride.
+// No memset is used: kernels zero the counters the next kernel needs, so the
+// bounce loop can be graph-captured.
+constexpr int kSlices = 64;
+constexpr int kCounterStride = 64;
+enum CounterKind : int {
+    kCntQueue0 = 0,     // live paths in queue 0 / 1 (ping-pong)
     kCntQueue1 = 1,
-    kCntShadow = 2,
-    kFetchExtend = 3,
-    kFetchShade = 4,
-    kFetchShadow = 5,
-    kNumCounters = 8,
+    kCntShadow = 2,     // shadow records
+    kFetchExtend = 3,   // fetch cursors
+    kFetchShadow = 4,
+    kCntClass0 = 5,     // kNumClasses per-class list lengths
+    kFetchClass0 = 10,  // kNumClasses per-class fetch cursors
+    kNumCounterKinds = 16,
 };
+constexpr size_t kCounterInts = (size_t)kNumCounterKinds * kSlices * kCounterStride;
+
 // Statistics slots (uint64).
 enum StatSlot : int {
     kStatExtRays = 0,

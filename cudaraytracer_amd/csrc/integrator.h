@@ -155,8 +155,10 @@ __device__ __forceinline__ void sss_probe(const DevScene& sc, vec3 hit_p, vec3 n
     }
 }
 
-// One bounce of one path.  `hit` holds (t, b1, b2, prim) from the extend stage.
-template <bool HAS_SSS, int STACK>
+// One bounce of one path.  `hit` holds (t, b1, b2, prim) from the extend
+// stage; CLS is the shading class of what was hit, fixed at compile time so
+// each class kernel carries only its own material code.
+template <int CLS, int STACK>
 __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FrameParams& fp, PathState& ps,
                                               float4 hit, uint32_t pixel, uint32_t sample, uint32_t* stk,
                                               int stride) {
@@ -169,7 +171,7 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
     const bool first = (ps.flags & kFlagFirst) != 0;
     const bool specular = (ps.flags & kFlagSpecular) != 0;
 
-    if (prim < 0) {
+    if (CLS == kClsTerminal && prim < 0) {
         // Miss: environment radiance without MIS, then the path ends
         // (reference: Src/Core/renderer.cuh:142-159).
         if (sc.env_light >= 0) {
@@ -180,19 +182,12 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
         return out;
     }
 
-    Rng rng;
-    rng.init(fp.seed, pixel, sample, (uint32_t)fp.bounce, kStreamPath);
-
     SurfaceHit sh = surface_at(sc.tri_attr, prim, hit.y, hit.z);
-    int2 mi = sc.mesh_info[sh.mesh];
-    DevMaterial mat = load_material(sc.materials, mi.x);
+    DevMaterial mat = load_material(sc.materials, sh.material);
     vec3 hit_p = sh.p;
-    vec3 geom_n = normalize(sh.ng);
-    vec3 hit_n = (double)norm2(sh.ns) > kEpsD ? normalize(sh.ns) : geom_n;
-    if (dot(hit_n, geom_n) < 0.f) hit_n = -hit_n;
-    vec3 wo = -ps.d;
 
-    if (mat.type == kLight) {
+    if (CLS == kClsTerminal) {
+        // Light hit (renderer.cuh:192-216): the path ends.
         if (first) {
             out.emit = mat.ke;
             out.has_emit = true;
@@ -200,11 +195,11 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
             out.emit = cmul(ps.w, mat.ke);
             out.has_emit = true;
         } else {
-            // Light-hit MIS against the BSDF pdf of the previous bounce; the
-            // light-selection probability is not applied here.
+            // MIS against the BSDF pdf of the previous bounce; the light-selection
+            // probability is not applied here (renderer.cuh:205-208).
             float mis = 1.f;
-            if (mi.y >= 0 && mi.y < sc.num_lights) {
-                float pdf_light = light_pdf_sa(sc, mi.y, ps.o, hit_p, sh.ng);
+            if (sh.light >= 0 && sh.light < sc.num_lights) {
+                float pdf_light = light_pdf_sa(sc, sh.light, ps.o, hit_p, sh.ng);
                 mis = mis_weight(ps.pdf, pdf_light);
             }
             out.emit = cmul(ps.w, mat.ke) * mis;
@@ -213,7 +208,15 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
         return out;
     }
 
-    if (mat.type == kDielectric) {
+    Rng rng;
+    rng.init(fp.seed, pixel, sample, (uint32_t)fp.bounce, kStreamPath);
+    vec3 geom_n = normalize(sh.ng);
+    vec3 hit_n = (double)norm2(sh.ns) > kEpsD ? normalize(sh.ns) : geom_n;
+    if (dot(hit_n, geom_n) < 0.f) hit_n = -hit_n;
+    vec3 wo = -ps.d;
+
+    if (CLS == kClsDielectric) {
+        // Delta lobe (renderer.cuh:218-240).
         vec3 wi, weight;
         float pdf = 1.0f;
         float u = rng.next();
@@ -232,7 +235,7 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
         return out;
     }
 
-    if (mat.type != kSubsurface || !HAS_SSS) {
+    if (CLS == kClsDiffuse || CLS == kClsPbr) {
         // Next-event estimation (renderer.cuh:242-273).
         int lid;
         float p_sel;
@@ -242,10 +245,10 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
             vec3 dir_wi = normalize(lp - hit_p);
             float pdf_light = light_pdf_sa(sc, lid, hit_p, lp, ln) * p_sel;
             float cos_t = dot(hit_n, dir_wi);
-            vec3 f = eval_bsdf(mat, wo, hit_n, dir_wi);
+            vec3 f = CLS == kClsPbr ? eval_pbr(mat, wo, hit_n, dir_wi) : eval_diffuse(mat, wo, hit_n, dir_wi);
             vec3 le = light_emission(sc, lid, dir_wi);
             vec3 dir_l = (cmul(le, f) * cos_t) / pdf_light;
-            float pdf_b = pdf_bsdf(mat, wo, hit_n, dir_wi);
+            float pdf_b = CLS == kClsPbr ? pdf_pbr(mat, wo, hit_n, dir_wi) : pdf_diffuse(hit_n, dir_wi);
             float mis = mis_weight(pdf_light, pdf_b);
             vec3 c = mis * cmul(ps.w, dir_l);
             // A zero contribution leaves the path sums unchanged, so no ray is needed.
@@ -256,21 +259,22 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
             }
         }
         // BSDF sample (renderer.cuh:274-289).
-        vec3 wi;
+        vec3 wi, f;
         float pdf;
-        if (mat.type == kPbr) {
+        if (CLS == kClsPbr) {
             float r1 = rng.next();
             float r2 = rng.next();
             float ch = rng.next();
             wi = sample_pbr_dir(mat, wo, hit_n, r1, r2, ch);
             pdf = pdf_pbr(mat, wo, hit_n, wi);
+            f = eval_pbr(mat, wo, hit_n, wi);
         } else {
             float t1 = rng.next();
             float t2 = rng.next();
             wi = sample_diffuse_dir(hit_n, t1, t2);
             pdf = pdf_diffuse(hit_n, wi);
+            f = eval_diffuse(mat, wo, hit_n, wi);
         }
-        vec3 f = eval_bsdf(mat, wo, hit_n, wi);
         ps.pdf = pdf;
         ps.o = hit_p;
         ps.d = wi;
@@ -280,9 +284,11 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
         } else {
             return out;
         }
-    } else {
-        // Subsurface: Fresnel-chosen specular reflection or diffusion-profile
-        // transmission with an exit-point NEE (renderer.cuh:290-404).
+    }
+
+    if (CLS == kClsSubsurface) {
+        // Fresnel-chosen specular reflection or diffusion-profile transmission
+        // with an exit-point NEE (renderer.cuh:290-404).
         float fin = fresnel_dielectric(dot(hit_n, wo), 1.0f, mat.ior);
         vec3 f_in = v3(fin, fin, fin);
         float reflect_prob = clampf(0.02f, 0.98f, fin);
@@ -319,7 +325,6 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
         }
         // Burley radius (material.cu:436-456), angle, probe (material.cu:415-424).
         float dist = 0.f;
-        float prof_pdf = 0.f;
         int ch = (int)(rng.next() * 3.f);
         ch = clampi(0, 2, ch);
         float dch = ch == 0 ? mat.scatter.x : (ch == 1 ? mat.scatter.y : mat.scatter.z);
@@ -332,9 +337,7 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
                 r1 = (r1 - 0.5f) * 2.f;
                 dist = (-3.f * dch) * fm::logf_(1.f - r1);
             }
-            prof_pdf = burley_pdf(mat, dist);
         }
-        (void)prof_pdf;
         float angle = (float)((double)(rng.next() * 2.f) * kPiD);
         vec3 out_p, out_n;
         sss_probe<STACK>(sc, hit_p, geom_n, dist, angle, out_p, out_n, stk, stride);
@@ -344,9 +347,6 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
         vec3 profile = v3(0.f, 0.f, 0.f);
         if (!(sd < 1e-6f)) profile = cmul(mat.basecolor, v3(burley(mat, sd, 0), burley(mat, sd, 1), burley(mat, sd, 2))) / kPiF;
         ps.w = cmul(cmul(ps.w, v3(1.f, 1.f, 1.f) - f_in), profile) / (transmit_prob * spatial_pdf);
-        vec3 wo_sub = hit_p - out_p;
-        if ((double)length(wo_sub) < kEpsD) wo_sub = wo;
-        else wo_sub = normalize(wo_sub);
 
         int lid;
         float p_sel;
@@ -379,7 +379,6 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
         float cos_out = dot(out_n, od);
         float fout = fresnel_dielectric(dot(out_n, od), 1.0f, mat.ior);
         float ft = 1 - fout;
-        (void)wo_sub;
         if ((double)od_pdf < kEpsD || cos_out < 0.f) return out;
         ps.o = out_p + out_n * 1e-4f;
         ps.d = od;

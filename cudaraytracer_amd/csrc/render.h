@@ -43,7 +43,7 @@ public:
     GpuScene& operator=(const GpuScene&) = delete;
     const DevScene& dev() const;
     int stack() const;
-    bool has_sss() const;
+    bool has_class(int shading_class) const;
     struct Impl;
 
 private:

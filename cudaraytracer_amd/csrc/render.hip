@@ -32,10 +32,12 @@ namespace rt {
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 
-__device__ __forceinline__ int wave_fetch(int* ctr) {
-    int base = 0;
-    if ((threadIdx.x & (kWave - 1)) == 0) base = atomicAdd(ctr, kWave);
-    return __shfl(base, 0);
+// Counter (kind, slice).
+__device__ __forceinline__ int* ctr_at(int* counters, int kind, int slice) {
+    return counters + ((size_t)kind * kSlices + slice) * kCounterStride;
+}
+__device__ __forceinline__ int load_relaxed(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t lanemask_lt() {
     int lane = threadIdx.x & (kWave - 1);
@@ -57,16 +59,61 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* p, uint32_t v) 
     if ((threadIdx.x & (kWave - 1)) == 0 && s) atomicAdd(p, (unsigned long long)s);
 }
 
+// Persistent-wave work distribution over the slices of one list: a wave starts
+// on its own slice and moves on (stealing) when a slice is drained.  Each call
+// to next() yields a 64-item chunk (slice, first item) or false when every
+// slice is drained.  All lanes of the wave see the same values.
+struct SliceCursor {
+    int* counters;
+    int cnt_kind, fetch_kind;
+    int slice, visited, count;
+    __device__ void init(int* c, int ck, int fk) {
+        counters = c;
+        cnt_kind = ck;
+        fetch_kind = fk;
+        int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+        slice = wave % kSlices;
+        visited = 0;
+        count = *ctr_at(counters, cnt_kind, slice);
+    }
+    __device__ bool next(int& base) {
+        while (visited < kSlices) {
+            int* f = ctr_at(counters, fetch_kind, slice);
+            int b = count;
+            // Drained slices are skipped with a plain load; only live ones take an atomic.
+            // Lane 0 decides and broadcasts, so the wave stays uniform.
+            if ((threadIdx.x & (kWave - 1)) == 0 && load_relaxed(f) < count) b = atomicAdd(f, kWave);
+            b = __shfl(b, 0);
+            if (b < count) {
+                base = b;
+                return true;
+            }
+            ++visited;
+            slice = slice + 1 == kSlices ? 0 : slice + 1;
+            count = *ctr_at(counters, cnt_kind, slice);
+        }
+        return false;
+    }
+};
+
 // ------------------------------------------------------------------ raygen
 // Jittered pinhole ray per (pixel, sample) (reference: Src/Core/camera.cu:27-35,
 // Src/Core/renderer.cuh:96-115).  Path p covers pixel p % wh of sample
-// sample0 + p / wh; row 0 of the image is the bottom row.
+// sample0 + p / wh; row 0 of the image is the bottom row.  Path p sits at
+// queue index p, in slice p / region.
 __global__ __launch_bounds__(kBlock) void k_raygen(FrameParams fp, float4* qo, float4* qd, float4* qw,
-                                                    float4* slot_d, float4* slot_i, int* counters, int qslot) {
+                                                    float4* slot_d, float4* slot_i, int* counters, int qslot,
+                                                    int region) {
     int stride = gridDim.x * blockDim.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        counters[qslot] = fp.n_paths;
-        counters[kFetchExtend] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kSlices) {
+        int sl = threadIdx.x;
+        int n = fp.n_paths - sl * region;
+        *ctr_at(counters, qslot, sl) = n < 0 ? 0 : (n > region ? region : n);
+        *ctr_at(counters, kFetchExtend, sl) = 0;
+        for (int c = 0; c < kNumClasses; ++c) {
+            *ctr_at(counters, kCntClass0 + c, sl) = 0;
+            *ctr_at(counters, kFetchClass0 + c, sl) = 0;
+        }
     }
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < fp.n_paths; p += stride) {
         uint32_t pixel = (uint32_t)(p % fp.wh);
@@ -87,30 +134,43 @@ __global__ __launch_bounds__(kBlock) void k_raygen(FrameParams fp, float4* qo, f
 }
 
 // ------------------------------------------------------------------ extend
+// Closest hit for every path of the current queue, then the path's queue
+// index is appended to its slice's list for the shading class of what it hit.
 template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, const float4* __restrict__ qo,
                                                     const float4* __restrict__ qd, float4* __restrict__ hits,
-                                                    int* counters, int qslot, unsigned long long* stats) {
+                                                    int* __restrict__ lists, int cap, int region, int* counters,
+                                                    int qslot, unsigned long long* stats) {
     __shared__ uint32_t stk[STACK * kBlock];
-    const int count = counters[qslot];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        counters[qslot ^ 1] = 0;
-        counters[kCntShadow] = 0;
-        counters[kFetchShade] = 0;
-        counters[kFetchShadow] = 0;
-        if (count > 0) atomicAdd(&stats[kStatExtRays], (unsigned long long)count);
+    if (blockIdx.x == 0 && threadIdx.x < kSlices) {
+        int sl = threadIdx.x;
+        *ctr_at(counters, qslot ^ 1, sl) = 0;
+        *ctr_at(counters, kCntShadow, sl) = 0;
+        *ctr_at(counters, kFetchShadow, sl) = 0;
+        int n = *ctr_at(counters, qslot, sl);
+        if (n > 0) atomicAdd(&stats[kStatExtRays], (unsigned long long)n);
     }
     uint32_t* col = stk + threadIdx.x;
-    while (true) {
-        int base = wave_fetch(&counters[kFetchExtend]);
-        if (base >= count) break;
-        int i = base + (int)(threadIdx.x & (kWave - 1));
+    SliceCursor cur;
+    cur.init(counters, qslot, kFetchExtend);
+    int base;
+    while (cur.next(base)) {
+        int j = base + (int)(threadIdx.x & (kWave - 1));
+        int i = cur.slice * region + j;
         TravCount c{0, 0};
-        if (i < count) {
+        int cls = -1;
+        if (j < cur.count) {
             float4 o = qo[i], d = qd[i];
             RayInv r = make_ray(v3(o.x, o.y, o.z), v3(d.x, d.y, d.z));
             Hit h = trace_closest<STACK, COUNT>(sc, r, 0.f, 1e8f, col, kBlock, c);
             hits[i] = make_float4(h.t, h.b1, h.b2, i_as_f(h.prim));
+            cls = shade_class_of(sc.tri_attr, h.prim);
+        }
+#pragma unroll
+        for (int k = 0; k < kNumClasses; ++k) {
+            if (__ballot(cls == k) == 0ull) continue;
+            int pos = wave_append(cls == k, ctr_at(counters, kCntClass0 + k, cur.slice));
+            if (cls == k) lists[(size_t)k * cap + (size_t)cur.slice * region + pos] = i;
         }
         if (COUNT) {
             wave_add_u64(&stats[kStatExtNodes], c.nodes);
@@ -120,27 +180,31 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, const float4* __
 }
 
 // ------------------------------------------------------------------ shade
-template <bool HAS_SSS, int STACK>
+// One kernel per shading class; each reads the queue indices binned by extend
+// and appends survivors / shadow records to the slice they came from.
+template <int CLS, int STACK>
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameParams fp, const float4* __restrict__ qo,
                                                    const float4* __restrict__ qd, const float4* __restrict__ qw,
-                                                   const float4* __restrict__ hits, float4* __restrict__ no,
-                                                   float4* __restrict__ nd, float4* __restrict__ nw,
-                                                   float4* __restrict__ shadow, float4* slot_d, float4* slot_i,
-                                                   int* counters, int qslot) {
-    __shared__ uint32_t stk[HAS_SSS ? STACK * kBlock : 1];
-    const int count = counters[qslot];
-    if (blockIdx.x == 0 && threadIdx.x == 0) counters[kFetchExtend] = 0;
-    uint32_t* col = stk + (HAS_SSS ? threadIdx.x : 0);
-    while (true) {
-        int base = wave_fetch(&counters[kFetchShade]);
-        if (base >= count) break;
-        int i = base + (int)(threadIdx.x & (kWave - 1));
+                                                   const float4* __restrict__ hits, const int* __restrict__ list,
+                                                   float4* __restrict__ no, float4* __restrict__ nd,
+                                                   float4* __restrict__ nw, float4* __restrict__ shadow,
+                                                   float4* slot_d, float4* slot_i, int region, int* counters,
+                                                   int qslot) {
+    __shared__ uint32_t stk[CLS == kClsSubsurface ? STACK * kBlock : 1];
+    if (blockIdx.x == 0 && threadIdx.x < kSlices) *ctr_at(counters, kFetchExtend, threadIdx.x) = 0;
+    uint32_t* col = stk + (CLS == kClsSubsurface ? threadIdx.x : 0);
+    SliceCursor cur;
+    cur.init(counters, kCntClass0 + CLS, kFetchClass0 + CLS);
+    int base;
+    while (cur.next(base)) {
+        int j = base + (int)(threadIdx.x & (kWave - 1));
         ShadeOut so;
         so.alive = false;
         so.shadow = false;
         PathState ps;
         int pid = 0;
-        if (i < count) {
+        if (j < cur.count) {
+            int i = list[(size_t)cur.slice * region + j];
             float4 o = qo[i], d = qd[i], w = qw[i];
             ps.o = v3(o.x, o.y, o.z);
             ps.pdf = o.w;
@@ -150,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameParams fp, c
             pid = f_as_i(w.w);
             uint32_t pixel = (uint32_t)(pid % fp.wh);
             uint32_t sample = fp.sample0 + (uint32_t)(pid / fp.wh);
-            so = shade_path<HAS_SSS, STACK>(sc, fp, ps, hits[i], pixel, sample, col, kBlock);
+            so = shade_path<CLS, STACK>(sc, fp, ps, hits[i], pixel, sample, col, kBlock);
             if (so.has_emit) {
                 float4* s = so.add_direct ? slot_d : slot_i;
                 float4 a = s[pid];
@@ -160,19 +224,24 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameParams fp, c
                 s[pid] = a;
             }
         }
-        int qpos = wave_append(so.alive, &counters[qslot ^ 1]);
-        if (so.alive) {
-            no[qpos] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.pdf);
-            nd[qpos] = make_float4(ps.d.x, ps.d.y, ps.d.z, i_as_f((int)ps.flags));
-            nw[qpos] = make_float4(ps.w.x, ps.w.y, ps.w.z, i_as_f(pid));
+        if (CLS != kClsTerminal) {
+            int qpos = wave_append(so.alive, ctr_at(counters, qslot ^ 1, cur.slice));
+            if (so.alive) {
+                size_t o = (size_t)cur.slice * region + qpos;
+                no[o] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.pdf);
+                nd[o] = make_float4(ps.d.x, ps.d.y, ps.d.z, i_as_f((int)ps.flags));
+                nw[o] = make_float4(ps.w.x, ps.w.y, ps.w.z, i_as_f(pid));
+            }
         }
-        int spos = wave_append(so.shadow, &counters[kCntShadow]);
-        if (so.shadow) {
-            float4* r = shadow + 4 * (size_t)spos;
-            r[0] = make_float4(so.rec.lp.x, so.rec.lp.y, so.rec.lp.z, so.rec.sdist);
-            r[1] = make_float4(so.rec.dir.x, so.rec.dir.y, so.rec.dir.z, i_as_f(pid));
-            r[2] = make_float4(so.rec.hp.x, so.rec.hp.y, so.rec.hp.z, i_as_f(so.add_direct ? 0 : 1));
-            r[3] = make_float4(so.rec.contrib.x, so.rec.contrib.y, so.rec.contrib.z, 0.f);
+        if (CLS == kClsDiffuse || CLS == kClsPbr || CLS == kClsSubsurface) {
+            int spos = wave_append(so.shadow, ctr_at(counters, kCntShadow, cur.slice));
+            if (so.shadow) {
+                float4* r = shadow + 4 * ((size_t)cur.slice * region + spos);
+                r[0] = make_float4(so.rec.lp.x, so.rec.lp.y, so.rec.lp.z, so.rec.sdist);
+                r[1] = make_float4(so.rec.dir.x, so.rec.dir.y, so.rec.dir.z, i_as_f(pid));
+                r[2] = make_float4(so.rec.hp.x, so.rec.hp.y, so.rec.hp.z, i_as_f(so.add_direct ? 0 : 1));
+                r[3] = make_float4(so.rec.contrib.x, so.rec.contrib.y, so.rec.contrib.z, 0.f);
+            }
         }
     }
 }
@@ -182,19 +251,28 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameParams fp, c
 // then the reference's 0.1 distance tolerances (Src/Core/light.cu:5-27).
 template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, const float4* __restrict__ shadow, float4* slot_d,
-                                                    float4* slot_i, int* counters, unsigned long long* stats) {
+                                                    float4* slot_i, int region, int* counters,
+                                                    unsigned long long* stats) {
     __shared__ uint32_t stk[STACK * kBlock];
-    const int count = counters[kCntShadow];
-    if (blockIdx.x == 0 && threadIdx.x == 0 && count > 0)
-        atomicAdd(&stats[kStatShadowRays], (unsigned long long)count);
+    if (blockIdx.x == 0 && threadIdx.x < kSlices) {
+        int sl = threadIdx.x;
+        int n = *ctr_at(counters, kCntShadow, sl);
+        if (n > 0) atomicAdd(&stats[kStatShadowRays], (unsigned long long)n);
+        // The class lists of this bounce are consumed: reset them for the next extend.
+        for (int c = 0; c < kNumClasses; ++c) {
+            *ctr_at(counters, kCntClass0 + c, sl) = 0;
+            *ctr_at(counters, kFetchClass0 + c, sl) = 0;
+        }
+    }
     uint32_t* col = stk + threadIdx.x;
-    while (true) {
-        int base = wave_fetch(&counters[kFetchShadow]);
-        if (base >= count) break;
-        int i = base + (int)(threadIdx.x & (kWave - 1));
+    SliceCursor cur;
+    cur.init(counters, kCntShadow, kFetchShadow);
+    int base;
+    while (cur.next(base)) {
+        int j = base + (int)(threadIdx.x & (kWave - 1));
         TravCount c{0, 0};
-        if (i < count) {
-            const float4* r = shadow + 4 * (size_t)i;
+        if (j < cur.count) {
+            const float4* r = shadow + 4 * ((size_t)cur.slice * region + j);
             float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
             ShadowRec rec;
             rec.lp = v3(r0.x, r0.y, r0.z);
@@ -227,12 +305,14 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, const float4* __
 // Paths still alive after the last bounce are dropped without contributing
 // (reference: Src/Core/renderer.cuh:1166-1194 never accumulates them).
 __global__ __launch_bounds__(kBlock) void k_lost(const float4* __restrict__ qw, float4* slot_d, float4* slot_i,
-                                                  const int* counters, int qslot) {
-    const int count = counters[qslot];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-        int pid = f_as_i(qw[i].w);
-        slot_d[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
-        slot_i[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+                                                  int* counters, int qslot, int region) {
+    for (int sl = 0; sl < kSlices; ++sl) {
+        const int count = *ctr_at(counters, qslot, sl);
+        for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += gridDim.x * blockDim.x) {
+            int pid = f_as_i(qw[(size_t)sl * region + j].w);
+            slot_d[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+            slot_i[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
     }
 }
 
@@ -331,10 +411,10 @@ struct DevBuf {
 };
 
 struct GpuScene::Impl {
-    DevBuf nodes, tri_isect, tri_attr, mesh_info, materials, lights, light_tris, env_rgb, env_row, env_marg;
+    DevBuf nodes, tri_isect, tri_attr, materials, lights, light_tris, env_rgb, env_row, env_marg;
     DevScene dev{};
     int stack = 32;
-    bool has_sss = false;
+    bool has_class[kNumClasses] = {true, false, false, false, false};
     int device = 0;
 };
 
@@ -355,6 +435,15 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
     if (nodes.empty()) nodes.resize(1);
     upload(I.nodes, nodes.data(), nodes.size() * sizeof(PackedNode));
     std::vector<float4> isect(3 * (size_t)n), attr(kAttrStride * (size_t)n);
+    auto class_of = [&](int material) -> int {
+        switch (sc.materials[material].type) {
+            case kLight: return kClsTerminal;
+            case kPbr: return kClsPbr;
+            case kDielectric: return kClsDielectric;
+            case kSubsurface: return kClsSubsurface;
+            default: return kClsDiffuse;
+        }
+    };
     for (int k = 0; k < n; ++k) {
         int t = bvh.order[k];
         const float* p = &sc.pos[9 * (size_t)t];
@@ -367,23 +456,22 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
         isect[3 * (size_t)k + 1] = make_float4(e1.x, e1.y, e1.z, 0.f);
         isect[3 * (size_t)k + 2] = make_float4(e2.x, e2.y, e2.z, 0.f);
         float4* a = &attr[kAttrStride * (size_t)k];
-        a[0] = make_float4(p[0], p[1], p[2], i_as_f(sc.mesh[t]));
+        const MeshRec& mr = sc.meshes[sc.mesh[t]];
+        int cls = class_of(mr.material);
+        I.has_class[cls] = true;
+        a[0] = make_float4(p[0], p[1], p[2], 0.f);
         a[1] = make_float4(p[3], p[4], p[5], g[0]);
         a[2] = make_float4(p[6], p[7], p[8], g[1]);
         a[3] = make_float4(nn[0], nn[1], nn[2], g[2]);
         a[4] = make_float4(nn[3], nn[4], nn[5], uv[0]);
         a[5] = make_float4(nn[6], nn[7], nn[8], uv[1]);
         a[6] = make_float4(uv[2], uv[3], uv[4], uv[5]);
+        a[7] = make_float4(i_as_f(sc.mesh[t]), i_as_f(mr.material), i_as_f(mr.light), i_as_f(cls));
     }
     upload(I.tri_isect, isect.data(), isect.size() * sizeof(float4));
     upload(I.tri_attr, attr.data(), attr.size() * sizeof(float4));
-    std::vector<int2> mi(sc.meshes.size());
-    for (size_t m = 0; m < sc.meshes.size(); ++m) mi[m] = make_int2(sc.meshes[m].material, sc.meshes[m].light);
-    if (mi.empty()) mi.resize(1);
-    upload(I.mesh_info, mi.data(), mi.size() * sizeof(int2));
     std::vector<MaterialRec> mats = sc.materials;
     if (mats.empty()) mats.resize(1);
-    for (auto& m : mats) I.has_sss = I.has_sss || (m.valid && m.type == kSubsurface);
     upload(I.materials, mats.data(), mats.size() * sizeof(MaterialRec));
     std::vector<LightRec> lights = sc.lights;
     std::vector<float4> ltris;
@@ -409,7 +497,6 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
     d.root_ref = bvh.root_ref;
     d.tri_isect = I.tri_isect.as<float4>();
     d.tri_attr = I.tri_attr.as<float4>();
-    d.mesh_info = I.mesh_info.as<int2>();
     d.materials = I.materials.as<MaterialRec>();
     d.lights = I.lights.as<LightRec>();
     d.num_lights = (int)sc.lights.size();
@@ -438,7 +525,7 @@ GpuScene::~GpuScene() { delete impl_; }
 
 const DevScene& GpuScene::dev() const { return impl_->dev; }
 int GpuScene::stack() const { return impl_->stack; }
-bool GpuScene::has_sss() const { return impl_->has_sss; }
+bool GpuScene::has_class(int c) const { return impl_->has_class[c]; }
 
 // Row-conditional and marginal CDFs weighted by sin(theta) (reference:
 // Src/Core/texture.cu:38-80).
@@ -481,8 +568,10 @@ struct Renderer::Impl {
     int width = 0, height = 0, wh = 0;
     int batch = 1;
     size_t cap = 0;  // paths per pass
-    DevBuf qo[2], qd[2], qw[2], hits, shadow, slot_d, slot_i, acc_d, acc_i, counters, stats;
-    int grid_persist_ext = 0, grid_persist_shade = 0, grid_persist_shadow = 0, grid_linear = 0;
+    int region = 0;  // queue entries per slice
+    DevBuf qo[2], qd[2], qw[2], hits, lists, shadow, slot_d, slot_i, acc_d, acc_i, counters, stats;
+    int grid_persist_ext = 0, grid_persist_shadow = 0, grid_linear = 0;
+    int grid_shade[kNumClasses] = {0};
     int spp_done = 0;
     // timing
     bool profile = false;
@@ -515,7 +604,7 @@ Renderer::Renderer(const GpuScene& scene, const HostScene& host, const RenderSet
     HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     // Paths per pass: all requested samples of every pixel when they fit the
     // queue budget (about 200 bytes of queue state per path).
-    size_t per_path = 3 * 16 * 2 + 16 + 64 + 32;
+    size_t per_path = 3 * 16 * 2 + 16 + 4 * kNumClasses + 64 + 32;
     size_t budget = s.queue_budget_bytes ? s.queue_budget_bytes : ((size_t)8 << 30);
     int bmax = (int)std::max<size_t>(1, budget / (per_path * (size_t)I.wh));
     int want = s.batch_spp > 0 ? s.batch_spp : 64;
@@ -528,28 +617,32 @@ Renderer::Renderer(const GpuScene& scene, const HostScene& host, const RenderSet
         I.qw[k].alloc(I.cap * 16);
     }
     I.hits.alloc(I.cap * 16);
+    I.lists.alloc(I.cap * 4 * kNumClasses);
     I.shadow.alloc(I.cap * 64);
     I.slot_d.alloc(I.cap * 16);
     I.slot_i.alloc(I.cap * 16);
     I.acc_d.alloc((size_t)I.wh * 16);
     I.acc_i.alloc((size_t)I.wh * 16);
-    I.counters.alloc(kNumCounters * sizeof(int));
+    I.region = (int)((I.cap + kSlices - 1) / kSlices);
+    I.counters.alloc(kCounterInts * sizeof(int));
     I.stats.alloc(kNumStats * sizeof(unsigned long long));
-    HIP_OK(hipMemset(I.counters.p, 0, kNumCounters * sizeof(int)));
+    HIP_OK(hipMemset(I.counters.p, 0, kCounterInts * sizeof(int)));
     HIP_OK(hipMemset(I.stats.p, 0, kNumStats * sizeof(unsigned long long)));
     HIP_OK(hipMemset(I.acc_d.p, 0, (size_t)I.wh * 16));
     HIP_OK(hipMemset(I.acc_i.p, 0, (size_t)I.wh * 16));
     if (scene.stack() == 32) {
         I.grid_persist_ext = persistent_grid(k_extend<32, false>, cus);
         I.grid_persist_shadow = persistent_grid(k_shadow<32, false>, cus);
-        I.grid_persist_shade = scene.has_sss() ? persistent_grid(k_shade<true, 32>, cus)
-                                               : persistent_grid(k_shade<false, 32>, cus);
+        I.grid_shade[kClsSubsurface] = persistent_grid(k_shade<kClsSubsurface, 32>, cus);
     } else {
         I.grid_persist_ext = persistent_grid(k_extend<64, false>, cus);
         I.grid_persist_shadow = persistent_grid(k_shadow<64, false>, cus);
-        I.grid_persist_shade = scene.has_sss() ? persistent_grid(k_shade<true, 64>, cus)
-                                               : persistent_grid(k_shade<false, 64>, cus);
+        I.grid_shade[kClsSubsurface] = persistent_grid(k_shade<kClsSubsurface, 64>, cus);
     }
+    I.grid_shade[kClsTerminal] = persistent_grid(k_shade<kClsTerminal, 32>, cus);
+    I.grid_shade[kClsDiffuse] = persistent_grid(k_shade<kClsDiffuse, 32>, cus);
+    I.grid_shade[kClsPbr] = persistent_grid(k_shade<kClsPbr, 32>, cus);
+    I.grid_shade[kClsDielectric] = persistent_grid(k_shade<kClsDielectric, 32>, cus);
     I.grid_linear = cus * 8;
     HIP_OK(hipDeviceSynchronize());
 }
@@ -570,15 +663,25 @@ int Renderer::height() const { return impl_->height; }
 int Renderer::batch() const { return impl_->batch; }
 int Renderer::spp_done() const { return impl_->spp_done; }
 
-template <int STACK, bool SSS>
+template <int CLS, int STACK>
+static void launch_shade(Renderer::Impl& I, const DevScene& sc, const FrameParams& fp, int q) {
+    int* ctr = I.counters.as<int>();
+    k_shade<CLS, STACK><<<I.grid_shade[CLS], kBlock, 0, I.stream>>>(
+        sc, fp, I.qo[q].as<float4>(), I.qd[q].as<float4>(), I.qw[q].as<float4>(), I.hits.as<float4>(),
+        I.lists.as<int>() + (size_t)CLS * I.cap, I.qo[q ^ 1].as<float4>(), I.qd[q ^ 1].as<float4>(),
+        I.qw[q ^ 1].as<float4>(), I.shadow.as<float4>(), I.slot_d.as<float4>(), I.slot_i.as<float4>(), I.region, ctr,
+        q);
+    HIP_OK(hipGetLastError());
+}
+
+template <int STACK>
 static void run_pass(Renderer::Impl& I, FrameParams fp, bool count) {
     const DevScene& sc = I.scene->dev();
-    auto tick = [&](int stage) -> hipEvent_t {
+    auto tick = [&]() -> hipEvent_t {
         if (!I.profile) return nullptr;
         hipEvent_t e;
         HIP_OK(hipEventCreate(&e));
         HIP_OK(hipEventRecord(e, I.stream));
-        (void)stage;
         return e;
     };
     auto tock = [&](int stage, hipEvent_t b) {
@@ -590,45 +693,49 @@ static void run_pass(Renderer::Impl& I, FrameParams fp, bool count) {
     };
     int* ctr = I.counters.as<int>();
     unsigned long long* st = I.stats.as<unsigned long long>();
+    const int cap = (int)I.cap;
     int q = 0;
-    hipEvent_t b = tick(kStageRaygen);
+    hipEvent_t b = tick();
     int grid_rg = (int)std::min<size_t>((fp.n_paths + kBlock - 1) / kBlock, (size_t)I.grid_linear * 4);
     k_raygen<<<grid_rg, kBlock, 0, I.stream>>>(fp, I.qo[q].as<float4>(), I.qd[q].as<float4>(), I.qw[q].as<float4>(),
-                                               I.slot_d.as<float4>(), I.slot_i.as<float4>(), ctr, q);
+                                               I.slot_d.as<float4>(), I.slot_i.as<float4>(), ctr, q, I.region);
     HIP_OK(hipGetLastError());
     tock(kStageRaygen, b);
+    const GpuScene& gs = *I.scene;
     for (int bounce = 0; bounce < fp.max_bounces; ++bounce) {
         fp.bounce = bounce;
-        b = tick(kStageExtend);
+        b = tick();
         if (count)
-            k_extend<STACK, true><<<I.grid_persist_ext, kBlock, 0, I.stream>>>(sc, I.qo[q].as<float4>(), I.qd[q].as<float4>(),
-                                                                               I.hits.as<float4>(), ctr, q, st);
+            k_extend<STACK, true><<<I.grid_persist_ext, kBlock, 0, I.stream>>>(
+                sc, I.qo[q].as<float4>(), I.qd[q].as<float4>(), I.hits.as<float4>(), I.lists.as<int>(), cap, I.region, ctr, q, st);
         else
-            k_extend<STACK, false><<<I.grid_persist_ext, kBlock, 0, I.stream>>>(sc, I.qo[q].as<float4>(), I.qd[q].as<float4>(),
-                                                                                I.hits.as<float4>(), ctr, q, st);
+            k_extend<STACK, false><<<I.grid_persist_ext, kBlock, 0, I.stream>>>(
+                sc, I.qo[q].as<float4>(), I.qd[q].as<float4>(), I.hits.as<float4>(), I.lists.as<int>(), cap, I.region, ctr, q, st);
         HIP_OK(hipGetLastError());
         tock(kStageExtend, b);
-        b = tick(kStageShade);
-        k_shade<SSS, STACK><<<I.grid_persist_shade, kBlock, 0, I.stream>>>(
-            sc, fp, I.qo[q].as<float4>(), I.qd[q].as<float4>(), I.qw[q].as<float4>(), I.hits.as<float4>(),
-            I.qo[q ^ 1].as<float4>(), I.qd[q ^ 1].as<float4>(), I.qw[q ^ 1].as<float4>(), I.shadow.as<float4>(),
-            I.slot_d.as<float4>(), I.slot_i.as<float4>(), ctr, q);
-        HIP_OK(hipGetLastError());
+        b = tick();
+        // Classes absent from the scene get no launch; their lists stay empty.
+        launch_shade<kClsTerminal, STACK>(I, sc, fp, q);
+        if (gs.has_class(kClsDiffuse)) launch_shade<kClsDiffuse, STACK>(I, sc, fp, q);
+        if (gs.has_class(kClsPbr)) launch_shade<kClsPbr, STACK>(I, sc, fp, q);
+        if (gs.has_class(kClsDielectric)) launch_shade<kClsDielectric, STACK>(I, sc, fp, q);
+        if (gs.has_class(kClsSubsurface)) launch_shade<kClsSubsurface, STACK>(I, sc, fp, q);
         tock(kStageShade, b);
-        b = tick(kStageShadow);
+        b = tick();
         if (count)
-            k_shadow<STACK, true><<<I.grid_persist_shadow, kBlock, 0, I.stream>>>(sc, I.shadow.as<float4>(), I.slot_d.as<float4>(),
-                                                                                  I.slot_i.as<float4>(), ctr, st);
+            k_shadow<STACK, true><<<I.grid_persist_shadow, kBlock, 0, I.stream>>>(
+                sc, I.shadow.as<float4>(), I.slot_d.as<float4>(), I.slot_i.as<float4>(), I.region, ctr, st);
         else
-            k_shadow<STACK, false><<<I.grid_persist_shadow, kBlock, 0, I.stream>>>(sc, I.shadow.as<float4>(), I.slot_d.as<float4>(),
-                                                                                   I.slot_i.as<float4>(), ctr, st);
+            k_shadow<STACK, false><<<I.grid_persist_shadow, kBlock, 0, I.stream>>>(
+                sc, I.shadow.as<float4>(), I.slot_d.as<float4>(), I.slot_i.as<float4>(), I.region, ctr, st);
         HIP_OK(hipGetLastError());
         tock(kStageShadow, b);
         q ^= 1;
     }
-    b = tick(kStageResolve);
+    b = tick();
     if (!I.set.flush_survivors)
-        k_lost<<<I.grid_linear, kBlock, 0, I.stream>>>(I.qw[q].as<float4>(), I.slot_d.as<float4>(), I.slot_i.as<float4>(), ctr, q);
+        k_lost<<<I.grid_linear, kBlock, 0, I.stream>>>(I.qw[q].as<float4>(), I.slot_d.as<float4>(), I.slot_i.as<float4>(), ctr,
+                                                       q, I.region);
     int k = fp.n_paths / I.wh;
     int grid_res = (int)std::min<size_t>((I.wh + kBlock - 1) / kBlock, (size_t)I.grid_linear * 4);
     k_resolve<<<grid_res, kBlock, 0, I.stream>>>(I.slot_d.as<float4>(), I.slot_i.as<float4>(), I.wh, k,
@@ -659,12 +766,8 @@ void Renderer::render(int spp, bool count) {
         fp.sample0 = (uint32_t)I.spp_done;
         fp.n_paths = I.wh * k;
         fp.bounce = 0;
-        int st = I.scene->stack();
-        bool sss = I.scene->has_sss();
-        if (st == 32 && !sss) run_pass<32, false>(I, fp, count);
-        else if (st == 32 && sss) run_pass<32, true>(I, fp, count);
-        else if (!sss) run_pass<64, false>(I, fp, count);
-        else run_pass<64, true>(I, fp, count);
+        if (I.scene->stack() == 32) run_pass<32>(I, fp, count);
+        else run_pass<64>(I, fp, count);
         I.spp_done += k;
         remaining -= k;
     }

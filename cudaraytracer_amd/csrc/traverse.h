@@ -140,12 +140,12 @@ struct SurfaceHit {
     vec3 ns;      // normalised interpolated shading normal
     vec3 ng;      // stored geometric normal
     float u, v;
-    int mesh;
+    int mesh, material, light;
 };
 
 RT_HD SurfaceHit surface_at(const float4* attr, int prim, float b1, float b2) {
     const float4* a = attr + kAttrStride * (size_t)prim;
-    float4 a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5], a6 = a[6];
+    float4 a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5], a6 = a[6], a7 = a[7];
     float b0 = (1.f - b1) - b2;
     SurfaceHit s;
     vec3 v0 = v3(a0.x, a0.y, a0.z), v1 = v3(a1.x, a1.y, a1.z), v2 = v3(a2.x, a2.y, a2.z);
@@ -155,8 +155,14 @@ RT_HD SurfaceHit surface_at(const float4* attr, int prim, float b1, float b2) {
     s.ng = v3(a1.w, a2.w, a3.w);
     s.u = (b0 * a4.w + b1 * a6.x) + b2 * a6.z;
     s.v = (b0 * a5.w + b1 * a6.y) + b2 * a6.w;
-    s.mesh = f_as_i(a0.w);
+    s.mesh = f_as_i(a7.x);
+    s.material = f_as_i(a7.y);
+    s.light = f_as_i(a7.z);
     return s;
+}
+
+RT_HD int shade_class_of(const float4* attr, int prim) {
+    return prim < 0 ? (int)kClsTerminal : f_as_i(attr[kAttrStride * (size_t)prim + 7].w);
 }
 
 RT_HD vec3 position_at(const float4* attr, int prim, float b1, float b2) {
