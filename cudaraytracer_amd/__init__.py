@@ -8,6 +8,13 @@ renderer.RenderLoop();`` (Src/main.cu:26-33).  Here the same three steps are
     r.render(spp)                           # RenderLoop frames, one sample per pixel each
     img = r.read_accum(RT_ACCUM_MEAN)       # renderBuffers read-back
 
+and, for the real-time denoised loop (RenderLoop with denoise on,
+Src/Core/renderer.cuh:1141-1262),
+
+    r.set_camera(Camera(...))               # orbit-camera move (Src/Utils/ui.cpp:313-348)
+    r.render_frame(spp=1)                   # G-buffer, path trace, SVGF
+    img = r.read_frame(RT_FRAME_DENOISED)
+
 Everything runs in the HIP library ``librtamd.so`` through its C ABI
 (include/rt.h); there is no CPU fallback: a missing library raises.
 """
@@ -26,7 +33,28 @@ RT_OK = 0
 RT_ACCUM_DIRECT = 0
 RT_ACCUM_INDIRECT = 1
 RT_ACCUM_MEAN = 2
-STAGES = ("raygen", "extend", "shade", "shadow", "resolve")
+STAGES = ("raygen", "extend", "shade", "shadow", "resolve", "gbuffer", "temporal", "variance", "atrous")
+
+# Frame buffers (rt_read_frame): id -> (numpy dtype, channels).
+RT_FRAME_DENOISED = 0
+RT_FRAME_DIRECT = 1
+RT_FRAME_INDIRECT = 2
+RT_FRAME_HISTORY_DIRECT = 3
+RT_FRAME_HISTORY_INDIRECT = 4
+RT_FRAME_MOMENTS_DIRECT = 5
+RT_FRAME_MOMENTS_INDIRECT = 6
+RT_FRAME_HISTORY_LENGTH = 7
+RT_FRAME_VARIANCE_DIRECT = 8
+RT_FRAME_VARIANCE_INDIRECT = 9
+RT_FRAME_G_POSITION_TRI = 10
+RT_FRAME_G_NORMAL_MAT = 11
+RT_FRAME_G_BARY_MESH = 12
+RT_FRAME_G_MOTION_DEPTH = 13
+FRAME_LAYOUT = {
+    RT_FRAME_MOMENTS_DIRECT: (np.float32, 2), RT_FRAME_MOMENTS_INDIRECT: (np.float32, 2),
+    RT_FRAME_HISTORY_LENGTH: (np.int32, 1),
+    RT_FRAME_G_NORMAL_MAT: (np.uint16, 4), RT_FRAME_G_BARY_MESH: (np.uint16, 4),
+}
 
 # C symbols declared in include/rt.h (checked by the test suite).
 EXPORTS = (
@@ -34,6 +62,8 @@ EXPORTS = (
     "rt_renderer_create", "rt_renderer_destroy", "rt_render", "rt_render_counted", "rt_synchronize",
     "rt_reset", "rt_reset_stats", "rt_read_accum", "rt_read_rgba8", "rt_stats", "rt_last_error",
     "rt_scene_get_triangles", "rt_scene_get_meshes", "rt_scene_get_bvh", "rt_trace_rays", "rt_device_math",
+    "rt_denoise_params_default", "rt_get_camera", "rt_set_camera", "rt_render_frame", "rt_reset_history",
+    "rt_read_frame",
 )
 
 
@@ -69,7 +99,8 @@ class Stats(ctypes.Structure):
         ("extension_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
         ("extension_node_visits", ctypes.c_uint64), ("extension_tri_tests", ctypes.c_uint64),
         ("shadow_node_visits", ctypes.c_uint64), ("shadow_tri_tests", ctypes.c_uint64),
-        ("probe_rays", ctypes.c_uint64), ("stage_ms", ctypes.c_double * 5), ("stage_launches", ctypes.c_int * 5),
+        ("probe_rays", ctypes.c_uint64), ("stage_ms", ctypes.c_double * len(STAGES)),
+        ("stage_launches", ctypes.c_int * len(STAGES)),
         ("spp_done", ctypes.c_int), ("batch_spp", ctypes.c_int),
     ]
 
@@ -78,6 +109,31 @@ class Stats(ctypes.Structure):
         d["stage_ms"] = {s: self.stage_ms[i] for i, s in enumerate(STAGES)}
         d["stage_launches"] = {s: self.stage_launches[i] for i, s in enumerate(STAGES)}
         return d
+
+
+class Camera(ctypes.Structure):
+    """Orbit camera: target, distance, polar angle theta from +y, azimuth phi (degrees)."""
+    _fields_ = [("target", ctypes.c_float * 3), ("distance", ctypes.c_float),
+                ("theta_deg", ctypes.c_float), ("phi_deg", ctypes.c_float)]
+
+    def __repr__(self) -> str:
+        return (f"Camera(target={list(self.target)}, distance={self.distance}, "
+                f"theta_deg={self.theta_deg}, phi_deg={self.phi_deg})")
+
+
+class DenoiseParams(ctypes.Structure):
+    """DenoiseParam (Src/Utils/ui.h:14-25); defaults from rt_denoise_params_default."""
+    _fields_ = [("max_history", ctypes.c_int), ("sigma_light", ctypes.c_float), ("sigma_normal", ctypes.c_float),
+                ("sigma_depth", ctypes.c_float), ("reprojection_depth", ctypes.c_float),
+                ("atrous_iterations", ctypes.c_int)]
+
+    @classmethod
+    def default(cls, **kw) -> "DenoiseParams":
+        p = cls()
+        lib().rt_denoise_params_default(ctypes.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
 
 
 _LIB: Optional[ctypes.CDLL] = None
@@ -114,6 +170,13 @@ def lib() -> ctypes.CDLL:
     L.rt_scene_get_bvh.argtypes = [vp, ip, vp, ctypes.POINTER(ctypes.c_int32)]
     L.rt_trace_rays.argtypes = [vp, fp, ip, fp, ctypes.POINTER(ctypes.c_uint32)]
     L.rt_device_math.argtypes = [ip, fp, fp, fp, ip]
+    L.rt_denoise_params_default.argtypes = [ctypes.POINTER(DenoiseParams)]
+    L.rt_denoise_params_default.restype = None
+    L.rt_get_camera.argtypes = [vp, ctypes.POINTER(Camera)]
+    L.rt_set_camera.argtypes = [vp, ctypes.POINTER(Camera)]
+    L.rt_render_frame.argtypes = [vp, ip, ctypes.POINTER(DenoiseParams)]
+    L.rt_reset_history.argtypes = [vp]
+    L.rt_read_frame.argtypes = [vp, ip, vp, ctypes.c_size_t]
     _LIB = L
     return L
 
@@ -249,6 +312,27 @@ class Renderer:
         s = Stats()
         _check(lib().rt_stats(self._h, ctypes.byref(s)))
         return s
+
+    # ---- real-time frames
+    def camera(self) -> Camera:
+        c = Camera()
+        _check(lib().rt_get_camera(self._h, ctypes.byref(c)))
+        return c
+
+    def set_camera(self, cam: Camera) -> None:
+        _check(lib().rt_set_camera(self._h, ctypes.byref(cam)))
+
+    def render_frame(self, spp: int = 1, params: Optional[DenoiseParams] = None) -> None:
+        _check(lib().rt_render_frame(self._h, int(spp), ctypes.byref(params) if params is not None else None))
+
+    def reset_history(self) -> None:
+        _check(lib().rt_reset_history(self._h))
+
+    def read_frame(self, which: int = RT_FRAME_DENOISED) -> np.ndarray:
+        dt, ch = FRAME_LAYOUT.get(which, (np.float32, 4))
+        out = np.zeros((self.height, self.width, ch), dt)
+        _check(lib().rt_read_frame(self._h, int(which), out.ctypes.data_as(ctypes.c_void_p), out.nbytes))
+        return out
 
 
 def device_math(op: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:

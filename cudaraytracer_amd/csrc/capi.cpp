@@ -38,6 +38,8 @@ int guard(F&& f) {
         return f();
     } catch (const std::bad_alloc&) {
         return fail(RT_ERR_OUT_OF_MEMORY, "out of host memory");
+    } catch (const std::invalid_argument& e) {
+        return fail(RT_ERR_INVALID_ARG, e.what());
     } catch (const rt::JsonError& e) {
         return fail(RT_ERR_SCENE, e.what());
     } catch (const std::runtime_error& e) {
@@ -72,6 +74,17 @@ void rt_render_params_default(rt_render_params* p) {
     p->queue_budget_bytes = 0;
     p->flush_survivors = 0;
     p->profile = 0;
+}
+
+void rt_denoise_params_default(rt_denoise_params* p) {
+    if (!p) return;
+    rt::SvgfParams d;
+    p->max_history = d.max_history;
+    p->sigma_light = d.sigma_light;
+    p->sigma_normal = d.sigma_normal;
+    p->sigma_depth = d.sigma_depth;
+    p->reprojection_depth = d.reproj_depth;
+    p->atrous_iterations = d.atrous_iters;
 }
 
 const char* rt_last_error(void) { return g_err.c_str(); }
@@ -248,6 +261,73 @@ int rt_stats(rt_renderer* r, rt_stats_out* o) {
     });
 }
 
+int rt_get_camera(rt_renderer* r, rt_camera* out) {
+    if (!r || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
+    const rt::CameraRec& c = r->r->camera();
+    out->target[0] = c.target.x;
+    out->target[1] = c.target.y;
+    out->target[2] = c.target.z;
+    out->distance = c.distance;
+    out->theta_deg = c.theta;
+    out->phi_deg = c.phi;
+    return RT_OK;
+}
+
+int rt_set_camera(rt_renderer* r, const rt_camera* cam) {
+    if (!r || !cam) return fail(RT_ERR_INVALID_ARG, "null argument");
+    float vals[6] = {cam->target[0], cam->target[1], cam->target[2], cam->distance, cam->theta_deg, cam->phi_deg};
+    for (float v : vals)
+        if (!std::isfinite(v)) return fail(RT_ERR_INVALID_ARG, "camera parameters must be finite");
+    if (!(cam->distance > 0.f)) return fail(RT_ERR_INVALID_ARG, "camera distance must be positive");
+    return guard([&]() {
+        rt::CameraRec c = r->r->camera();
+        c.target = rt::v3(cam->target[0], cam->target[1], cam->target[2]);
+        c.distance = cam->distance;
+        c.theta = cam->theta_deg;
+        c.phi = cam->phi_deg;
+        rt::compute_camera(c);
+        r->r->set_camera(c);
+        return RT_OK;
+    });
+}
+
+int rt_render_frame(rt_renderer* r, int spp, const rt_denoise_params* p) {
+    if (!r || spp < 1) return fail(RT_ERR_INVALID_ARG, "invalid argument");
+    rt::SvgfParams sp;
+    if (p) {
+        if (p->max_history < 1) return fail(RT_ERR_INVALID_ARG, "max_history must be >= 1");
+        sp.max_history = p->max_history;
+        sp.sigma_light = p->sigma_light;
+        sp.sigma_normal = p->sigma_normal;
+        sp.sigma_depth = p->sigma_depth;
+        sp.reproj_depth = p->reprojection_depth;
+        sp.atrous_iters = p->atrous_iterations;
+    }
+    return guard([&]() {
+        r->r->render_frame(spp, sp);
+        return RT_OK;
+    });
+}
+
+int rt_reset_history(rt_renderer* r) {
+    if (!r) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return guard([&]() {
+        r->r->reset_history();
+        return RT_OK;
+    });
+}
+
+int rt_read_frame(rt_renderer* r, int which, void* out, size_t bytes) {
+    if (!r || !out || which < 0 || which >= rt::kNumFrameBuffers) return fail(RT_ERR_INVALID_ARG, "invalid argument");
+    size_t need = rt::frame_buffer_bytes(which, r->r->width(), r->r->height());
+    if (bytes != need)
+        return fail(RT_ERR_INVALID_ARG, "buffer size " + std::to_string(bytes) + " != " + std::to_string(need));
+    return guard([&]() {
+        r->r->read_frame(which, out);
+        return RT_OK;
+    });
+}
+
 int rt_scene_get_triangles(const rt_scene* s, float* pos, float* nrm, float* uv, int32_t* mesh) {
     if (!s) return fail(RT_ERR_INVALID_ARG, "null argument");
     const rt::HostScene& h = s->host;
@@ -303,7 +383,7 @@ int rt_trace_rays(rt_scene* s, const float* rays, int n, float* hits, uint32_t* 
 }
 
 int rt_device_math(int op, const float* a, const float* b, float* out, int n) {
-    if (!a || !b || !out || n < 0 || op < 0 || op > 9) return fail(RT_ERR_INVALID_ARG, "invalid argument");
+    if (!a || !b || !out || n < 0 || op < 0 || op > 10) return fail(RT_ERR_INVALID_ARG, "invalid argument");
     if (n == 0) return RT_OK;
     return guard([&]() {
         int c = 0;

@@ -30,7 +30,7 @@ struct DevScene {
 };
 
 // tri_attr layout, 8 float4 = one 128-byte line per triangle:
-//   [0] v0.xyz, -                      [1] v1.xyz, gn.x
+//   [0] v0.xyz, index within its mesh  [1] v1.xyz, gn.x
 //   [2] v2.xyz, gn.y                   [3] n0.xyz, gn.z
 //   [4] n1.xyz, uv0.x                  [5] n2.xyz, uv0.y
 //   [6] uv1.xy, uv2.xy                 [7] mesh, material, light, shading class (int bits)

@@ -6,10 +6,42 @@
 
 #include "device.h"
 #include "scene.h"
+#include "svgf.h"
 
 namespace rt {
 
-enum Stage : int { kStageRaygen = 0, kStageExtend = 1, kStageShade = 2, kStageShadow = 3, kStageResolve = 4, kNumStages = 5 };
+enum Stage : int {
+    kStageRaygen = 0,
+    kStageExtend = 1,
+    kStageShade = 2,
+    kStageShadow = 3,
+    kStageResolve = 4,
+    kStageGBuffer = 5,
+    kStageTemporal = 6,
+    kStageVariance = 7,
+    kStageAtrous = 8,
+    kNumStages = 9
+};
+
+// Buffers of the last real-time frame, for read-back.
+enum FrameBuffer : int {
+    kFrameDenoised = 0,          // rgba float: filtered direct + indirect, alpha 1
+    kFrameDirect = 1,            // rgba float: filtered direct, .w variance
+    kFrameIndirect = 2,
+    kFrameHistoryDirect = 3,     // temporal output (the next frame's history), .w variance
+    kFrameHistoryIndirect = 4,
+    kFrameMomentsDirect = 5,     // float2 (mean luminance, mean squared luminance)
+    kFrameMomentsIndirect = 6,
+    kFrameHistoryLength = 7,     // int32
+    kFrameVarianceDirect = 8,    // variance-pass output, .w variance
+    kFrameVarianceIndirect = 9,
+    kFrameGPosTri = 10,          // float4
+    kFrameGNrmMat = 11,          // uint16 x4
+    kFrameGBaryMesh = 12,        // uint16 x4
+    kFrameGMotionDepth = 13,     // float4
+    kNumFrameBuffers = 14
+};
+size_t frame_buffer_bytes(int which, int width, int height);
 
 struct RenderSettings {
     int max_bounces = 5;
@@ -58,6 +90,21 @@ public:
     Renderer& operator=(const Renderer&) = delete;
     // Enqueue `spp` more samples per pixel on the renderer's stream.
     void render(int spp, bool count_traversal = false);
+    // Camera of the following passes and frames; clears the accumulation
+    // (a camera move resets the frame buffer: reference Src/Utils/ui.cpp:313-348).
+    void set_camera(const CameraRec& c);
+    const CameraRec& camera() const;
+    // One real-time frame (reference: renderer.cuh:1141-1262 with denoising on):
+    // G-buffer against the previous frame's camera, `spp` fresh samples
+    // (sample indices frame * spp ...), temporal blend, variance estimate and
+    // the a-trous iterations (skipped on the first frame after a history reset).
+    void render_frame(int spp, const SvgfParams& p);
+    // Forget the temporal history (hardResetDenoiseHistory, renderer.cuh:1125-1139).
+    void reset_history();
+    int frames() const;
+    // Copy one of the last frame's buffers (FrameBuffer) into host memory of
+    // frame_buffer_bytes(which, width, height) bytes.
+    void read_frame(int which, void* out);
     void synchronize();
     void reset();
     void reset_stats();

@@ -360,7 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch(DevScene sc, const float
 }
 
 // Device math self-test: op 0 div, 1 sqrt, 2 sin, 3 cos, 4 acos, 5 exp, 6 log,
-// 7 atan2, 8 philox (a = counter lane bits), 9 inverse.
+// 7 atan2, 8 philox (a = counter lane bits), 9 inverse, 10 pow.
 __global__ void k_math(int op, const float* a, const float* b, float* out, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -382,9 +382,58 @@ __global__ void k_math(int op, const float* a, const float* b, float* out, int n
             break;
         }
         case 9: r = 1.f / x; break;
+        case 10: r = fm::powf_(x, y); break;
         default: break;
     }
     out[i] = r;
+}
+
+// ---------------------------------------------------------------- frames
+// Per-pixel frame kernels work on 16x16 tiles (four 16x4 waves), so the
+// neighbourhood taps of the filters stay within a few cache lines per wave.
+__device__ __forceinline__ bool tile_pixel(int w, int h, int& i, int& j) {
+    int tiles_x = (w + 15) >> 4;
+    int t = blockIdx.x;
+    i = (t % tiles_x) * 16 + (threadIdx.x & 15);
+    j = (t / tiles_x) * 16 + (threadIdx.x >> 4);
+    return i < w && j < h;
+}
+
+template <int STACK>
+__global__ __launch_bounds__(kBlock) void k_gbuffer(DevScene sc, GBufferParams g, GBufferView out) {
+    __shared__ uint32_t stk[STACK * kBlock];
+    int i, j;
+    if (!tile_pixel(g.width, g.height, i, j)) return;
+    gbuffer_pixel<STACK>(sc, g, i, j, stk + threadIdx.x, kBlock, out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_temporal(int w, int h, int first, SvgfParams p, TemporalIO io,
+                                                      GBufferView cur, GBufferView prev) {
+    int i, j;
+    if (!tile_pixel(w, h, i, j)) return;
+    temporal_pixel(i, j, w, h, first != 0, p, io, cur, prev);
+}
+
+__global__ __launch_bounds__(kBlock) void k_variance(int w, int h, SvgfParams p, FilterIO io, GBufferView g) {
+    int i, j;
+    if (!tile_pixel(w, h, i, j)) return;
+    variance_pixel(i, j, w, h, p, io, g);
+}
+
+__global__ __launch_bounds__(kBlock) void k_atrous(int w, int h, int step, SvgfParams p, FilterIO io, GBufferView g) {
+    int i, j;
+    if (!tile_pixel(w, h, i, j)) return;
+    atrous_pixel(i, j, w, h, step, p, io, g);
+}
+
+// Denoised colour = filtered direct + indirect (GatherKernel's denoise branch,
+// reference: renderer.cuh:970-978, before the display gamma).
+__global__ __launch_bounds__(kBlock) void k_combine(const float4* __restrict__ d, const float4* __restrict__ ind,
+                                                     int n, float4* out) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        float4 a = d[k], b = ind[k];
+        out[k] = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, 1.f);
+    }
 }
 
 // ======================================================================
@@ -459,7 +508,7 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
         const MeshRec& mr = sc.meshes[sc.mesh[t]];
         int cls = class_of(mr.material);
         I.has_class[cls] = true;
-        a[0] = make_float4(p[0], p[1], p[2], 0.f);
+        a[0] = make_float4(p[0], p[1], p[2], i_as_f(t - mr.start));
         a[1] = make_float4(p[3], p[4], p[5], g[0]);
         a[2] = make_float4(p[6], p[7], p[8], g[1]);
         a[3] = make_float4(nn[0], nn[1], nn[2], g[2]);
@@ -573,6 +622,18 @@ struct Renderer::Impl {
     int grid_persist_ext = 0, grid_persist_shadow = 0, grid_linear = 0;
     int grid_shade[kNumClasses] = {0};
     int spp_done = 0;
+    uint32_t sample_base = 0;  // sample index of the first sample in the accumulators
+    CameraRec cam;             // camera of the following passes
+    // Real-time frames: two G-buffers / histories (this frame, previous frame).
+    bool frame_alloc = false;
+    DevBuf g_pos[2], g_nrm[2], g_bary[2], g_md[2];
+    DevBuf h_d[2], h_i[2], m_d[2], m_i[2], h_len[2];
+    DevBuf v_d, v_i, f_d[2], f_i[2], denoised;
+    CameraRec last_frame_cam;
+    int fcur = 0;     // buffer set the next frame writes
+    int flast = -1;   // buffer set of the last frame (-1: none yet)
+    int frames = 0;   // frames since the last history reset
+    int out_sel = -1; // -1: the variance pass output is final, else f_*[out_sel]
     // timing
     bool profile = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
@@ -596,6 +657,7 @@ Renderer::Renderer(const GpuScene& scene, const HostScene& host, const RenderSet
     I.width = host.camera.width;
     I.height = host.camera.height;
     I.wh = I.width * I.height;
+    I.cam = host.camera;
     I.profile = s.profile != 0;
     HIP_OK(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
     int dev = 0;
@@ -663,6 +725,22 @@ int Renderer::height() const { return impl_->height; }
 int Renderer::batch() const { return impl_->batch; }
 int Renderer::spp_done() const { return impl_->spp_done; }
 
+// Stage timing with HIP events on the renderer's stream (profiled renderers only).
+static hipEvent_t stage_begin(Renderer::Impl& I) {
+    if (!I.profile) return nullptr;
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    HIP_OK(hipEventRecord(e, I.stream));
+    return e;
+}
+static void stage_end(Renderer::Impl& I, int stage, hipEvent_t b) {
+    if (!I.profile) return;
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    HIP_OK(hipEventRecord(e, I.stream));
+    I.ev.push_back({stage, {b, e}});
+}
+
 template <int CLS, int STACK>
 static void launch_shade(Renderer::Impl& I, const DevScene& sc, const FrameParams& fp, int q) {
     int* ctr = I.counters.as<int>();
@@ -677,20 +755,8 @@ static void launch_shade(Renderer::Impl& I, const DevScene& sc, const FrameParam
 template <int STACK>
 static void run_pass(Renderer::Impl& I, FrameParams fp, bool count) {
     const DevScene& sc = I.scene->dev();
-    auto tick = [&]() -> hipEvent_t {
-        if (!I.profile) return nullptr;
-        hipEvent_t e;
-        HIP_OK(hipEventCreate(&e));
-        HIP_OK(hipEventRecord(e, I.stream));
-        return e;
-    };
-    auto tock = [&](int stage, hipEvent_t b) {
-        if (!I.profile) return;
-        hipEvent_t e;
-        HIP_OK(hipEventCreate(&e));
-        HIP_OK(hipEventRecord(e, I.stream));
-        I.ev.push_back({stage, {b, e}});
-    };
+    auto tick = [&]() { return stage_begin(I); };
+    auto tock = [&](int stage, hipEvent_t e) { stage_end(I, stage, e); };
     int* ctr = I.counters.as<int>();
     unsigned long long* st = I.stats.as<unsigned long long>();
     const int cap = (int)I.cap;
@@ -746,16 +812,14 @@ static void run_pass(Renderer::Impl& I, FrameParams fp, bool count) {
 
 void Renderer::render(int spp, bool count) {
     Impl& I = *impl_;
-    const HostScene& h = *I.host;
-    const CameraRec& c = h.camera;
     int remaining = spp;
     while (remaining > 0) {
         int k = std::min(remaining, I.batch);
         FrameParams fp;
-        fp.cam_pos = c.position;
-        fp.left_down = c.left_down;
-        fp.du = c.du;
-        fp.dv = c.dv;
+        fp.cam_pos = I.cam.position;
+        fp.left_down = I.cam.left_down;
+        fp.du = I.cam.du;
+        fp.dv = I.cam.dv;
         fp.width = I.width;
         fp.height = I.height;
         fp.wh = I.wh;
@@ -763,7 +827,7 @@ void Renderer::render(int spp, bool count) {
         fp.rr = I.set.rr;
         fp.rr_start = I.set.rr_start_depth;
         fp.max_bounces = I.set.max_bounces;
-        fp.sample0 = (uint32_t)I.spp_done;
+        fp.sample0 = I.sample_base + (uint32_t)I.spp_done;
         fp.n_paths = I.wh * k;
         fp.bounce = 0;
         if (I.scene->stack() == 32) run_pass<32>(I, fp, count);
@@ -775,6 +839,199 @@ void Renderer::render(int spp, bool count) {
 
 void Renderer::synchronize() { HIP_OK(hipStreamSynchronize(impl_->stream)); }
 
+// ------------------------------------------------------------------ frames
+void Renderer::set_camera(const CameraRec& c) {
+    Impl& I = *impl_;
+    if (c.width != I.width || c.height != I.height) throw std::invalid_argument("camera resolution differs from the renderer's");
+    I.cam = c;
+    reset();
+}
+
+const CameraRec& Renderer::camera() const { return impl_->cam; }
+int Renderer::frames() const { return impl_->frames; }
+
+void Renderer::reset_history() {
+    Impl& I = *impl_;
+    HIP_OK(hipStreamSynchronize(I.stream));
+    I.frames = 0;
+}
+
+size_t frame_buffer_bytes(int which, int width, int height) {
+    size_t wh = (size_t)width * height;
+    switch (which) {
+        case kFrameMomentsDirect:
+        case kFrameMomentsIndirect:
+        case kFrameGNrmMat:
+        case kFrameGBaryMesh: return wh * 8;
+        case kFrameHistoryLength: return wh * 4;
+        default: return (which >= 0 && which < kNumFrameBuffers) ? wh * 16 : 0;
+    }
+}
+
+static void ensure_frame_buffers(Renderer::Impl& I) {
+    if (I.frame_alloc) return;
+    size_t wh = (size_t)I.wh;
+    for (int k = 0; k < 2; ++k) {
+        I.g_pos[k].alloc(wh * 16);
+        I.g_nrm[k].alloc(wh * 8);
+        I.g_bary[k].alloc(wh * 8);
+        I.g_md[k].alloc(wh * 16);
+        I.h_d[k].alloc(wh * 16);
+        I.h_i[k].alloc(wh * 16);
+        I.m_d[k].alloc(wh * 8);
+        I.m_i[k].alloc(wh * 8);
+        I.h_len[k].alloc(wh * 4);
+        I.f_d[k].alloc(wh * 16);
+        I.f_i[k].alloc(wh * 16);
+    }
+    I.v_d.alloc(wh * 16);
+    I.v_i.alloc(wh * 16);
+    I.denoised.alloc(wh * 16);
+    I.frame_alloc = true;
+}
+
+static GBufferView gview(Renderer::Impl& I, int k) {
+    GBufferView v;
+    v.pos_tri = I.g_pos[k].as<float4>();
+    v.nrm_mat = I.g_nrm[k].as<ushort4>();
+    v.bary_mesh = I.g_bary[k].as<ushort4>();
+    v.motion_depth = I.g_md[k].as<float4>();
+    return v;
+}
+
+// Matrices and planes of the G-buffer pass (reference: renderer.cuh:1463-1474):
+// near 0.1, far = scene diagonal + distance from the camera to the scene centre,
+// the previous view with the current projection.
+static GBufferParams gbuffer_params(const HostScene& h, const CameraRec& cur, const CameraRec& prev) {
+    GBufferParams g;
+    g.cam_pos = cur.position;
+    g.left_down = cur.left_down;
+    g.du = cur.du;
+    g.dv = cur.dv;
+    const Box& b = h.bvh.bounds;
+    vec3 center = (b.lo + b.hi) * 0.5f;
+    g.near_plane = 0.1f;
+    g.far_plane = length(b.hi - b.lo) + length(cur.position - center);
+    camera_view_projection(cur, g.near_plane, g.far_plane, g.vp);
+    camera_view_projection(prev, g.near_plane, g.far_plane, g.prev_vp);
+    g.width = cur.width;
+    g.height = cur.height;
+    return g;
+}
+
+void Renderer::render_frame(int spp, const SvgfParams& p) {
+    Impl& I = *impl_;
+    if (spp < 1) throw std::invalid_argument("spp must be positive");
+    ensure_frame_buffers(I);
+    const int c = I.fcur, pv = c ^ 1;
+    const bool first = I.frames == 0;
+    const int w = I.width, h = I.height;
+    const int tiles = ((w + 15) / 16) * ((h + 15) / 16);
+    const DevScene& sc = I.scene->dev();
+
+    hipEvent_t e = stage_begin(I);
+    GBufferParams g = gbuffer_params(*I.host, I.cam, first ? I.cam : I.last_frame_cam);
+    if (I.scene->stack() == 32) k_gbuffer<32><<<tiles, kBlock, 0, I.stream>>>(sc, g, gview(I, c));
+    else k_gbuffer<64><<<tiles, kBlock, 0, I.stream>>>(sc, g, gview(I, c));
+    HIP_OK(hipGetLastError());
+    stage_end(I, kStageGBuffer, e);
+
+    // Fresh samples of this frame only (the reference resets its frame buffer
+    // every denoised frame: renderer.cuh:1297-1302, 1121-1124).
+    k_zero4<<<I.grid_linear, kBlock, 0, I.stream>>>(I.acc_d.as<float4>(), (size_t)I.wh);
+    k_zero4<<<I.grid_linear, kBlock, 0, I.stream>>>(I.acc_i.as<float4>(), (size_t)I.wh);
+    HIP_OK(hipGetLastError());
+    I.spp_done = 0;
+    I.sample_base = (uint32_t)I.frames * (uint32_t)spp;
+    render(spp);
+
+    e = stage_begin(I);
+    TemporalIO tio;
+    tio.acc_d = I.acc_d.as<float4>();
+    tio.acc_i = I.acc_i.as<float4>();
+    tio.inv_spp = 1.f / (float)spp;
+    tio.cur_d = I.h_d[c].as<float4>();
+    tio.cur_i = I.h_i[c].as<float4>();
+    tio.hist_d = I.h_d[pv].as<float4>();
+    tio.hist_i = I.h_i[pv].as<float4>();
+    tio.mom_prev_d = I.m_d[pv].as<float2>();
+    tio.mom_prev_i = I.m_i[pv].as<float2>();
+    tio.len_prev = I.h_len[pv].as<int>();
+    tio.mom_d = I.m_d[c].as<float2>();
+    tio.mom_i = I.m_i[c].as<float2>();
+    tio.len = I.h_len[c].as<int>();
+    k_temporal<<<tiles, kBlock, 0, I.stream>>>(w, h, first ? 1 : 0, p, tio, gview(I, c), gview(I, pv));
+    HIP_OK(hipGetLastError());
+    stage_end(I, kStageTemporal, e);
+
+    e = stage_begin(I);
+    FilterIO fio;
+    fio.in_d = I.h_d[c].as<float4>();
+    fio.in_i = I.h_i[c].as<float4>();
+    fio.out_d = I.v_d.as<float4>();
+    fio.out_i = I.v_i.as<float4>();
+    fio.mom_d = I.m_d[c].as<float2>();
+    fio.mom_i = I.m_i[c].as<float2>();
+    fio.len = I.h_len[c].as<int>();
+    k_variance<<<tiles, kBlock, 0, I.stream>>>(w, h, p, fio, gview(I, c));
+    HIP_OK(hipGetLastError());
+    stage_end(I, kStageVariance, e);
+
+    I.out_sel = -1;
+    if (!first) {
+        // a-trous passes with step 1, 2, 4, ... (reference: renderer.cuh:1236-1250).
+        int passes = std::max(1, std::min(8, p.atrous_iters));
+        e = stage_begin(I);
+        for (int k = 0; k < passes; ++k) {
+            int o = k & 1;
+            fio.in_d = k == 0 ? I.v_d.as<float4>() : I.f_d[o ^ 1].as<float4>();
+            fio.in_i = k == 0 ? I.v_i.as<float4>() : I.f_i[o ^ 1].as<float4>();
+            fio.out_d = I.f_d[o].as<float4>();
+            fio.out_i = I.f_i[o].as<float4>();
+            k_atrous<<<tiles, kBlock, 0, I.stream>>>(w, h, 1 << k, p, fio, gview(I, c));
+            HIP_OK(hipGetLastError());
+            I.out_sel = o;
+        }
+        stage_end(I, kStageAtrous, e);
+    }
+    const float4* od = I.out_sel < 0 ? I.v_d.as<float4>() : I.f_d[I.out_sel].as<float4>();
+    const float4* oi = I.out_sel < 0 ? I.v_i.as<float4>() : I.f_i[I.out_sel].as<float4>();
+    int grid_c = (int)std::min<size_t>((I.wh + kBlock - 1) / kBlock, (size_t)I.grid_linear * 4);
+    k_combine<<<grid_c, kBlock, 0, I.stream>>>(od, oi, I.wh, I.denoised.as<float4>());
+    HIP_OK(hipGetLastError());
+
+    I.last_frame_cam = I.cam;
+    I.flast = c;
+    I.fcur = pv;
+    I.frames++;
+}
+
+void Renderer::read_frame(int which, void* out) {
+    Impl& I = *impl_;
+    if (which < 0 || which >= kNumFrameBuffers) throw std::invalid_argument("unknown frame buffer");
+    if (I.flast < 0) throw std::invalid_argument("no frame has been rendered");
+    HIP_OK(hipStreamSynchronize(I.stream));
+    const int c = I.flast;
+    const DevBuf* src = nullptr;
+    switch (which) {
+        case kFrameDenoised: src = &I.denoised; break;
+        case kFrameDirect: src = I.out_sel < 0 ? &I.v_d : &I.f_d[I.out_sel]; break;
+        case kFrameIndirect: src = I.out_sel < 0 ? &I.v_i : &I.f_i[I.out_sel]; break;
+        case kFrameHistoryDirect: src = &I.h_d[c]; break;
+        case kFrameHistoryIndirect: src = &I.h_i[c]; break;
+        case kFrameMomentsDirect: src = &I.m_d[c]; break;
+        case kFrameMomentsIndirect: src = &I.m_i[c]; break;
+        case kFrameHistoryLength: src = &I.h_len[c]; break;
+        case kFrameVarianceDirect: src = &I.v_d; break;
+        case kFrameVarianceIndirect: src = &I.v_i; break;
+        case kFrameGPosTri: src = &I.g_pos[c]; break;
+        case kFrameGNrmMat: src = &I.g_nrm[c]; break;
+        case kFrameGBaryMesh: src = &I.g_bary[c]; break;
+        case kFrameGMotionDepth: src = &I.g_md[c]; break;
+    }
+    HIP_OK(hipMemcpy(out, src->p, frame_buffer_bytes(which, I.width, I.height), hipMemcpyDeviceToHost));
+}
+
 void Renderer::reset() {
     Impl& I = *impl_;
     HIP_OK(hipStreamSynchronize(I.stream));
@@ -783,6 +1040,7 @@ void Renderer::reset() {
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(I.stream));
     I.spp_done = 0;
+    I.sample_base = 0;
 }
 
 void Renderer::reset_stats() {

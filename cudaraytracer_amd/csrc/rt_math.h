@@ -248,6 +248,11 @@ RT_HD float pow5(float x) {
     float x2 = x * x;
     return (x2 * x2) * x;
 }
+// x^y for x >= 0 as exp(y log x); 0^y = 0 for y > 0 and 1 otherwise.
+RT_HD float powf_(float x, float y) {
+    if (x == 0.0f) return y > 0.0f ? 0.0f : 1.0f;
+    return expf_(y * logf_(x));
+}
 }  // namespace fm
 
 // ---------------------------------------------------------------------------

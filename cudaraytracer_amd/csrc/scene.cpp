@@ -391,18 +391,7 @@ void finalize_triangles(HostScene& sc) {
     }
 }
 
-void setup_camera(HostScene& sc) {
-    CameraRec& c = sc.camera;
-    Box b = sc.bvh.bounds;
-    vec3 center = (b.lo + b.hi) * 0.5f;
-    float diag = length(b.hi - b.lo);
-    c.target = center;
-    c.distance = ((diag * 0.5f) / sinf(deg2rad(c.fovy) / 2.f)) * 0.6f;
-    vec3 dir = normalize(c.direction);
-    float th = acosf(dir.y / length(dir));
-    float ph = atan2f(dir.x, dir.z);
-    c.theta = rad2deg(th);
-    c.phi = rad2deg(ph);
+void compute_camera(CameraRec& c) {
     // Camera::ComputeCameraParam (reference: Src/Core/camera.h:58-84).
     float theta = deg2rad(c.theta), phi = deg2rad(c.phi);
     float x = (c.distance * sinf(theta)) * sinf(phi);
@@ -421,6 +410,50 @@ void setup_camera(HostScene& sc) {
     c.du = (w_ * c.u) / (float)c.width;
     c.dv = (h_ * c.v) / (float)c.height;
     c.left_down = (c.lookat - (float)(c.width / 2) * c.du) - (float)(c.height / 2) * c.dv;
+}
+
+void setup_camera(HostScene& sc) {
+    CameraRec& c = sc.camera;
+    Box b = sc.bvh.bounds;
+    vec3 center = (b.lo + b.hi) * 0.5f;
+    float diag = length(b.hi - b.lo);
+    // Scene::InitCameraParam (reference: Src/Core/scene.cpp:646-667).
+    c.target = center;
+    c.distance = ((diag * 0.5f) / sinf(deg2rad(c.fovy) / 2.f)) * 0.6f;
+    vec3 dir = normalize(c.direction);
+    float th = acosf(dir.y / length(dir));
+    float ph = atan2f(dir.x, dir.z);
+    c.theta = rad2deg(th);
+    c.phi = rad2deg(ph);
+    compute_camera(c);
+}
+
+void camera_view_projection(const CameraRec& c, float near_plane, float far_plane, float vp[16]) {
+    // LookAt and Perspective (reference: Src/Utils/mathutil.cu:163-194), then
+    // projection * view as the G-buffer vertex shader composes them.
+    vec3 f = normalize(c.lookat - c.position);
+    vec3 s = normalize(cross(f, c.up));
+    vec3 u = cross(s, f);
+    Mat4 view = identity();
+    view.m[0][0] = s.x; view.m[0][1] = s.y; view.m[0][2] = s.z;
+    view.m[1][0] = u.x; view.m[1][1] = u.y; view.m[1][2] = u.z;
+    view.m[2][0] = -f.x; view.m[2][1] = -f.y; view.m[2][2] = -f.z;
+    view.m[0][3] = -dot(s, c.position);
+    view.m[1][3] = -dot(u, c.position);
+    view.m[2][3] = dot(f, c.position);
+    float fy = 1.0f / tanf(deg2rad(c.fovy) / 2.0f);
+    float aspect = (1.f * (float)c.width) / (float)c.height;
+    Mat4 proj;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) proj.m[i][j] = 0.f;
+    proj.m[0][0] = fy / aspect;
+    proj.m[1][1] = fy;
+    proj.m[2][2] = (far_plane + near_plane) / (near_plane - far_plane);
+    proj.m[2][3] = ((2.f * far_plane) * near_plane) / (near_plane - far_plane);
+    proj.m[3][2] = -1.0f;
+    Mat4 m = mul(proj, view);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) vp[4 * i + j] = m.m[i][j];
 }
 
 void load_scene(const std::string& json_path, HostScene& sc) {

@@ -98,5 +98,9 @@ void load_scene(const std::string& json_path, HostScene& sc);
 void finalize_triangles(HostScene& sc);
 // Camera parameters from the scene bounds (reference: Src/Core/scene.cpp:646-667, camera.h:58-84).
 void setup_camera(HostScene& sc);
+// Pinhole basis and pixel steps from the orbit parameters (target, distance, theta, phi).
+void compute_camera(CameraRec& c);
+// Row-major projection * view of the G-buffer pass (reference: renderer.cuh:1463-1474).
+void camera_view_projection(const CameraRec& c, float near_plane, float far_plane, float vp[16]);
 
 }  // namespace rt

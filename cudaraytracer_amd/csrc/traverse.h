@@ -43,15 +43,18 @@ RT_HD bool box_hit(const RayInv& r, float lx, float ly, float lz, float hx, floa
     return (exit - enter >= -0.01f) && (exit >= -0.01f) && (enter <= tmax) && (exit >= tmin);
 }
 
-// Moller-Trumbore with back-face culling and the triangle's own [0, 1e8]
-// range.  Returns true and updates (t, b1, b2) on a closer hit.
+// Moller-Trumbore with the triangle's own [0, 1e8] range.  Returns true and
+// updates (t, b1, b2) on a closer hit.  CULL rejects back faces as the path
+// tracer does (reference: Src/Core/triangle.cu:102-133); without it both
+// faces count, as in the rasterised G-buffer (no GL_CULL_FACE).
+template <bool CULL = true>
 RT_HD bool tri_hit(const RayInv& r, float4 a, float4 e1f, float4 e2f, float& hit_t, float& hb1, float& hb2) {
     vec3 v0 = v3(a.x, a.y, a.z), e1 = v3(e1f.x, e1f.y, e1f.z), e2 = v3(e2f.x, e2f.y, e2f.z);
     vec3 s = r.o - v0;
     vec3 s1 = cross(r.d, e2);
     vec3 s2 = cross(s, e1);
     float det = dot(e1, s1);
-    if (det <= 0.f) return false;
+    if (CULL ? det <= 0.f : det == 0.f) return false;
     float inv = 1.f / det;
     float t = inv * dot(s2, e2);
     if (t >= hit_t) return false;
@@ -78,7 +81,7 @@ struct TravCount {
 };
 
 // `stk` points at this lane's stack column; entries are `stride` words apart.
-template <int STACK, bool COUNT>
+template <int STACK, bool COUNT, bool CULL = true>
 __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r, float tmin, float tmax,
                                              uint32_t* stk, int stride, TravCount& cnt) {
     Hit h;
@@ -96,7 +99,7 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r
                 const float4* tp = sc.tri_isect + 3 * (size_t)(first + k);
                 float4 a = tp[0], e1 = tp[1], e2 = tp[2];
                 if (COUNT) cnt.tris++;
-                if (tri_hit(r, a, e1, e2, h.t, h.b1, h.b2)) h.prim = (int)(first + k);
+                if (tri_hit<CULL>(r, a, e1, e2, h.t, h.b1, h.b2)) h.prim = (int)(first + k);
             }
         } else {
             const float4* np = reinterpret_cast<const float4*>(sc.nodes + ref);

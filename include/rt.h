@@ -15,6 +15,13 @@
  *   rt_read_accum        <- renderBuffers[cur].direct/indirectLightingBuffer read-back
  *   rt_read_rgba8        <- GatherKernel colour mode (renderer.cuh:956-993)
  *   rt_stats             <- DenoiseTimingStats / stage timers (ui.h:27-37), plus ray counts
+ *   rt_set_camera        <- orbit-camera update (Src/Utils/ui.cpp:313-348 -> camera.h:58-84)
+ *   rt_render_frame      <- RenderLoop frame with denoising on (renderer.cuh:1141-1262):
+ *                           G-buffer (renderer.cuh:1446-1503), path trace, temporal,
+ *                           variance and a-trous kernels (renderer.cuh:442-954)
+ *   rt_read_frame        <- GatherKernel denoise branch (renderer.cuh:970-978) and the
+ *                           G-buffer / history buffers the SVGF kernels read
+ *   rt_reset_history     <- hardResetDenoiseHistory (renderer.cuh:1125-1139)
  *   rt_last_error        <- std::runtime_error / printed CUDA errors (scene.cpp:51-53, global.h:19-27)
  *
  * Conventions: functions return RT_OK (0) or a negative RT_ERR_* code and set
@@ -52,7 +59,27 @@ extern "C" {
 #define RT_STAGE_SHADE 2
 #define RT_STAGE_SHADOW 3
 #define RT_STAGE_RESOLVE 4
-#define RT_NUM_STAGES 5
+#define RT_STAGE_GBUFFER 5
+#define RT_STAGE_TEMPORAL 6
+#define RT_STAGE_VARIANCE 7
+#define RT_STAGE_ATROUS 8
+#define RT_NUM_STAGES 9
+
+/* Buffers of the last rt_render_frame, row 0 = bottom. */
+#define RT_FRAME_DENOISED 0          /* float4: filtered direct + indirect, alpha 1 */
+#define RT_FRAME_DIRECT 1            /* float4: filtered direct, .w variance */
+#define RT_FRAME_INDIRECT 2
+#define RT_FRAME_HISTORY_DIRECT 3    /* float4: temporal blend (next frame's history), .w variance */
+#define RT_FRAME_HISTORY_INDIRECT 4
+#define RT_FRAME_MOMENTS_DIRECT 5    /* float2: luminance moments */
+#define RT_FRAME_MOMENTS_INDIRECT 6
+#define RT_FRAME_HISTORY_LENGTH 7    /* int32 */
+#define RT_FRAME_VARIANCE_DIRECT 8   /* float4: variance-pass output */
+#define RT_FRAME_VARIANCE_INDIRECT 9
+#define RT_FRAME_G_POSITION_TRI 10   /* float4: world position, triangle index within its mesh */
+#define RT_FRAME_G_NORMAL_MAT 11     /* uint16x4: normal*0.5+0.5 in 1/65535 units, material id */
+#define RT_FRAME_G_BARY_MESH 12      /* uint16x4: barycentrics in 1/65535 units, mesh id + 1 (0: background) */
+#define RT_FRAME_G_MOTION_DEPTH 13   /* float4: NDC motion (current - previous), linear depth, fwidth(depth) */
 
 typedef struct rt_scene rt_scene;
 typedef struct rt_renderer rt_renderer;
@@ -103,8 +130,28 @@ typedef struct rt_stats_out {
     int batch_spp;
 } rt_stats_out;
 
+/* Orbit camera (CameraParam, Src/Core/camera.h:11-84): the basis follows from these. */
+typedef struct rt_camera {
+    float target[3];
+    float distance;
+    float theta_deg; /* polar angle from +y */
+    float phi_deg;   /* azimuth about +y, from +z towards +x */
+} rt_camera;
+
+/* DenoiseParam (Src/Utils/ui.h:14-25). */
+typedef struct rt_denoise_params {
+    int max_history;          /* 24 */
+    float sigma_light;        /* 8 */
+    float sigma_normal;       /* 256 */
+    float sigma_depth;        /* 1 */
+    float reprojection_depth; /* 0.1 */
+    int atrous_iterations;    /* 5 (clamped to 1..8) */
+} rt_denoise_params;
+
 /* Fill *p with "use the scene's values" defaults. */
 void rt_render_params_default(rt_render_params* p);
+/* Fill *p with the reference's DenoiseParam defaults. */
+void rt_denoise_params_default(rt_denoise_params* p);
 
 /* Scene::Scene (Src/Core/scene.cpp:50-57). */
 int rt_scene_load(const char* json_path, const rt_load_opts* opts, rt_scene** out);
@@ -129,6 +176,21 @@ int rt_read_rgba8(rt_renderer* r, uint8_t* rgba_out);
 int rt_stats(rt_renderer* r, rt_stats_out* out);
 const char* rt_last_error(void);
 
+/* Current orbit camera of the renderer. */
+int rt_get_camera(rt_renderer* r, rt_camera* out);
+/* Move the camera; clears the accumulators (a camera move resets the frame buffer). */
+int rt_set_camera(rt_renderer* r, const rt_camera* cam);
+/* One real-time frame: G-buffer against the previous frame's camera, spp fresh samples
+   per pixel (sample indices frame*spp ...), temporal reprojection, variance estimate and
+   a-trous filtering (skipped on the first frame after a history reset).  p NULL: defaults.
+   Asynchronous; rt_read_accum afterwards returns this frame's raw sums. */
+int rt_render_frame(rt_renderer* r, int spp, const rt_denoise_params* p);
+/* Forget the temporal history; the next frame is a first frame. */
+int rt_reset_history(rt_renderer* r);
+/* Copy an RT_FRAME_* buffer of the last frame; bytes must be its exact size
+   (width*height times 16 for float4, 8 for float2 / uint16x4, 4 for int32). */
+int rt_read_frame(rt_renderer* r, int which, void* out, size_t bytes);
+
 /* ---- test and inspection hooks (no reference counterpart) ---- */
 /* Scene-order triangle soup: pos 9, normals 9, uv 6 floats and mesh id per triangle (any pointer may be NULL). */
 int rt_scene_get_triangles(const rt_scene* s, float* pos, float* nrm, float* uv, int32_t* mesh);
@@ -140,7 +202,7 @@ int rt_scene_get_bvh(const rt_scene* s, int from_packed, void* nodes_out, int32_
 /* Closest hit on the GPU for n rays given as 8 floats (o.xyz, tmin, d.xyz, tmax);
    hits_out: (t, b1, b2, prim-as-int-bits) per ray; counts_out: (node visits, tri tests). */
 int rt_trace_rays(rt_scene* s, const float* rays, int n, float* hits_out, uint32_t* counts_out);
-/* Device arithmetic check (op: 0 div, 1 sqrt, 2 sin, 3 cos, 4 acos, 5 exp, 6 log, 7 atan2, 8 philox, 9 rcp). */
+/* Device arithmetic check (op: 0 div, 1 sqrt, 2 sin, 3 cos, 4 acos, 5 exp, 6 log, 7 atan2, 8 philox, 9 rcp, 10 pow). */
 int rt_device_math(int op, const float* a, const float* b, float* out, int n);
 
 #ifdef __cplusplus

@@ -179,8 +179,16 @@ static float c_atan2(float y, float x) {
     return w + c_atan(y / x);
 }
 
+/* x^y for x >= 0 as exp(y log x), 0^y = 0 for y > 0 (stands in for powf in the
+   SVGF normal weight, renderer.cuh:755, 919). */
+static float c_pow(float x, float y) {
+    if (x == 0.0f) return y > 0.0f ? 0.0f : 1.0f;
+    return c_exp(y * c_log(x));
+}
+
 float ref_math(int op, float a, float b) {
     switch (op) {
+        case 10: return c_pow(a, b);
         case 2: return c_sin(a);
         case 3: return c_cos(a);
         case 4: return c_acos(a);
@@ -305,6 +313,8 @@ struct ref_scene {
     float fovy;
     V3 dir;
     V3 cam_pos, cam_ld, cam_du, cam_dv;
+    V3 cam_lookat, cam_up;
+    ref_orbit orbit;  /* target, distance, theta, phi (degrees) of the current camera */
 };
 
 static float smin(float a, float b) { return (b < a) ? b : a; } /* std::min */
@@ -487,7 +497,7 @@ typedef struct {
 
 /* Triangle::IsIntersect (triangle.cu:102-133): double det, 1.f/det rounded
    through double, own [0, 1e8] range. */
-static int tri_isect(const Ray* r, const Tri* tr, float* hit_t, float* hb1, float* hb2) {
+static int tri_isect(const Ray* r, const Tri* tr, float* hit_t, float* hb1, float* hb2, int cull) {
     V3 e1 = vsub(tr->p[1], tr->p[0]);
     V3 e2 = vsub(tr->p[2], tr->p[0]);
     V3 s = vsub(r->o, tr->p[0]);
@@ -495,7 +505,7 @@ static int tri_isect(const Ray* r, const Tri* tr, float* hit_t, float* hb1, floa
     V3 s2 = vcross(s, e1);
     double det = vdot(e1, s1);
     float inv, t, b1, b2;
-    if (det == 0 || det < 0) return 0;
+    if (det == 0 || (cull && det < 0)) return 0; /* cull = 0: both faces, as the rasterised G-buffer */
     inv = (float)(1.0 / det);
     t = inv * vdot(s2, e2);
     if (t >= *hit_t) return 0;
@@ -511,7 +521,8 @@ static int tri_isect(const Ray* r, const Tri* tr, float* hit_t, float* hb1, floa
 }
 
 /* BVH::IsIntersect (bvh.cu:253-306). */
-static Hit bvh_trace(const ref_scene* s, const Ray* r, float tmin, float tmax, uint32_t* nv, uint32_t* nt) {
+static Hit bvh_trace_c(const ref_scene* s, const Ray* r, float tmin, float tmax, uint32_t* nv, uint32_t* nt,
+                       int cull) {
     Hit h;
     int stack[64], sp = 0, cur = 0;
     h.t = tmax;
@@ -523,7 +534,7 @@ static Hit bvh_trace(const ref_scene* s, const Ray* r, float tmin, float tmax, u
             int k;
             for (k = 0; k < n->second; ++k) {
                 if (nt) (*nt)++;
-                if (tri_isect(r, &s->ordered[n->tri_index + k], &h.t, &h.b1, &h.b2)) h.prim = n->tri_index + k;
+                if (tri_isect(r, &s->ordered[n->tri_index + k], &h.t, &h.b1, &h.b2, cull)) h.prim = n->tri_index + k;
             }
         } else {
             int first = n->second, second = cur + 1;
@@ -545,6 +556,10 @@ static Hit bvh_trace(const ref_scene* s, const Ray* r, float tmin, float tmax, u
         cur = stack[--sp];
     }
     return h;
+}
+
+static Hit bvh_trace(const ref_scene* s, const Ray* r, float tmin, float tmax, uint32_t* nv, uint32_t* nt) {
+    return bvh_trace_c(s, r, tmin, tmax, nv, nt, 1);
 }
 
 void ref_trace(const ref_scene* s, const float* rays8, int n, float* hits4, uint32_t* counts2) {
@@ -570,29 +585,59 @@ void ref_trace(const ref_scene* s, const float* rays8, int n, float* hits4, uint
 static float deg2rad(float a) { return (float)((double)(a / 180.f) * PI_D); }  /* mathutil.cu:4-7 */
 static float rad2deg(float r) { return (float)((double)(r * 180.f) / PI_D); }  /* mathutil.cu:9-12 */
 
-static void setup_camera(ref_scene* s) { /* scene.cpp:646-667, camera.h:58-84 */
+typedef struct {
+    V3 pos, lookat, up, ld, du, dv;
+} CamBasis;
+
+/* Camera::ComputeCameraParam (camera.h:58-84): orbit -> pinhole basis. */
+static CamBasis orbit_basis(const ref_scene* s, const ref_orbit* o) {
+    CamBasis c;
+    V3 target = v3(o->target[0], o->target[1], o->target[2]);
+    float dist = o->distance, theta = deg2rad(o->theta_deg), phi = deg2rad(o->phi_deg);
+    float x = (dist * sinf(theta)) * sinf(phi), y = dist * cosf(theta), z = (dist * sinf(theta)) * cosf(phi);
+    V3 w, u, v;
+    float focal, ratio, h_, w_;
+    c.pos = vadd(target, v3(x, y, z));
+    c.lookat = target;
+    c.up = v3(0.f, 1.f, 0.f);
+    w = vneg(vnormalize(vsub(c.lookat, c.pos)));
+    u = vnormalize(vcross(vnormalize(c.up), w));
+    v = vnormalize(vcross(w, u));
+    focal = vlen(vsub(c.lookat, c.pos));
+    ratio = (1.f * (float)s->W) / (float)s->H;
+    h_ = (2.f * focal) * tanf(deg2rad(s->fovy / 2.f));
+    w_ = h_ * ratio;
+    c.du = vdivs(vscale(u, w_), (float)s->W);
+    c.dv = vdivs(vscale(v, h_), (float)s->H);
+    c.ld = vsub(vsub(c.lookat, vscale(c.du, (float)(s->W / 2))), vscale(c.dv, (float)(s->H / 2)));
+    return c;
+}
+
+static void apply_orbit(ref_scene* s, const ref_orbit* o) {
+    CamBasis c = orbit_basis(s, o);
+    s->orbit = *o;
+    s->cam_pos = c.pos;
+    s->cam_ld = c.ld;
+    s->cam_du = c.du;
+    s->cam_dv = c.dv;
+    s->cam_lookat = c.lookat;
+    s->cam_up = c.up;
+}
+
+static void setup_camera(ref_scene* s) { /* scene.cpp:646-667 */
     V3 lo = v3(s->bounds.lo[0], s->bounds.lo[1], s->bounds.lo[2]);
     V3 hi = v3(s->bounds.hi[0], s->bounds.hi[1], s->bounds.hi[2]);
     V3 target = vscale(vadd(lo, hi), 0.5f);
     float diag = vlen(vsub(hi, lo));
-    float dist = ((diag * 0.5f) / sinf(deg2rad(s->fovy) / 2.f)) * 0.6f;
     V3 dir = vnormalize(s->dir);
-    float th = acosf(dir.y / vlen(dir)), ph = atan2f(dir.x, dir.z);
-    float theta = deg2rad(rad2deg(th)), phi = deg2rad(rad2deg(ph));
-    float x = (dist * sinf(theta)) * sinf(phi), y = dist * cosf(theta), z = (dist * sinf(theta)) * cosf(phi);
-    V3 pos = vadd(target, v3(x, y, z));
-    V3 up = v3(0.f, 1.f, 0.f);
-    V3 w = vneg(vnormalize(vsub(target, pos)));
-    V3 u = vnormalize(vcross(vnormalize(up), w));
-    V3 v = vnormalize(vcross(w, u));
-    float focal = vlen(vsub(target, pos));
-    float ratio = (1.f * (float)s->W) / (float)s->H;
-    float h_ = (2.f * focal) * tanf(deg2rad(s->fovy / 2.f));
-    float w_ = h_ * ratio;
-    s->cam_pos = pos;
-    s->cam_du = vdivs(vscale(u, w_), (float)s->W);
-    s->cam_dv = vdivs(vscale(v, h_), (float)s->H);
-    s->cam_ld = vsub(vsub(target, vscale(s->cam_du, (float)(s->W / 2))), vscale(s->cam_dv, (float)(s->H / 2)));
+    ref_orbit o;
+    o.target[0] = target.x;
+    o.target[1] = target.y;
+    o.target[2] = target.z;
+    o.distance = ((diag * 0.5f) / sinf(deg2rad(s->fovy) / 2.f)) * 0.6f;
+    o.theta_deg = rad2deg(acosf(dir.y / vlen(dir)));
+    o.phi_deg = rad2deg(atan2f(dir.x, dir.z));
+    apply_orbit(s, &o);
 }
 
 static void setup_env(ref_scene* s) { /* texture.cu:38-80 */
@@ -866,3 +911,5 @@ void ref_env_sample(const ref_scene* s, float rv, float ru, float* uv_out) {
 }
 
 float ref_env_pdf_uv(const ref_scene* s, float u, float v) { return env_pdf_uv_f(s, u, v); }
+
+#include "rtref_svgf.inc"

@@ -58,6 +58,27 @@ typedef struct ref_params {
 
 typedef struct ref_scene ref_scene;
 
+/* Orbit camera (camera.h:11-84): target, distance, polar angle from +y, azimuth (degrees). */
+typedef struct ref_orbit {
+    float target[3];
+    float distance, theta_deg, phi_deg;
+} ref_orbit;
+
+/* G-buffer planes, width*height each, row 0 = bottom (Shader/gbuffer.fs:2-5). */
+typedef struct ref_gbuf {
+    float* pos_tri;       /* 4 floats: world position, triangle index within its mesh */
+    uint16_t* nrm_mat;    /* 4: normal*0.5+0.5 in 1/65535 units, material id */
+    uint16_t* bary_mesh;  /* 4: barycentrics in 1/65535 units, mesh id + 1 (0 = background) */
+    float* motion_depth;  /* 4: NDC motion (current - previous), linear depth, fwidth(depth) */
+} ref_gbuf;
+
+/* DenoiseParam (Src/Utils/ui.h:14-25). */
+typedef struct ref_svgf_params {
+    int max_history;
+    float sigma_light, sigma_normal, sigma_depth, reproj_depth;
+    int atrous_iters;
+} ref_svgf_params;
+
 ref_scene* ref_scene_create(const ref_scene_desc* d);
 void ref_scene_destroy(ref_scene* s);
 
@@ -81,9 +102,30 @@ void ref_render(const ref_scene* s, const ref_params* p, int sample0, int spp, c
 void ref_render_f64(const ref_scene* s, const ref_params* p, int sample0, int spp, const int32_t* pixels, int npix,
                     double* acc_direct, double* acc_indirect);
 
+/* Current orbit of the path-tracing camera; moving it changes the camera of ref_render. */
+void ref_get_orbit(const ref_scene* s, ref_orbit* out);
+void ref_set_orbit(ref_scene* s, const ref_orbit* o);
+/* 4x4 row-major projection * view of the G-buffer pass for an orbit (mathutil.cu:163-194). */
+void ref_view_projection(const ref_scene* s, const ref_orbit* o, float near_plane, float far_plane, float* out16);
+/* Primary-visibility G-buffer of the current camera at pixel centres; motion against `prev`
+   (renderer.cuh:1446-1503 with Shader/gbuffer.*).  Both faces count. */
+void ref_gbuffer(const ref_scene* s, const ref_orbit* prev, const ref_gbuf* out);
+/* SVGF passes over width*height pixels (renderer.cuh:442-954).  Colours are 4 floats per
+   pixel (.w variance), moments 2 floats, history lengths int32. */
+void ref_svgf_temporal(int w, int h, int first, const ref_svgf_params* p, float inv_spp, const float* acc_d,
+                       const float* acc_i, const float* hist_prev_d, const float* hist_prev_i,
+                       const float* mom_prev_d, const float* mom_prev_i, const int32_t* len_prev,
+                       const ref_gbuf* cur, const ref_gbuf* prev, float* hist_d, float* hist_i, float* mom_d,
+                       float* mom_i, int32_t* len);
+void ref_svgf_variance(int w, int h, const ref_svgf_params* p, const float* in_d, const float* in_i,
+                       const float* mom_d, const float* mom_i, const int32_t* len, const ref_gbuf* g,
+                       float* out_d, float* out_i);
+void ref_svgf_atrous(int w, int h, int step, const ref_svgf_params* p, const float* in_d, const float* in_i,
+                     const int32_t* len, const ref_gbuf* g, float* out_d, float* out_i);
+
 /* Scalar pieces exposed for known-answer and closed-form tests. */
 void ref_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
-float ref_math(int op, float a, float b); /* 2 sin 3 cos 4 acos 5 exp 6 log 7 atan2 */
+float ref_math(int op, float a, float b); /* 2 sin 3 cos 4 acos 5 exp 6 log 7 atan2 10 pow */
 /* BSDF helpers: type from material; eval returns rgb into out3, pdf returned. */
 float ref_bsdf_eval(const float* mat, const float* wi, const float* n, const float* wo, float* out3);
 float ref_bsdf_sample(const float* mat, const float* wi, const float* n, const float* u3, float* wo_out);

@@ -45,6 +45,29 @@ class Params(ctypes.Structure):
                 ("seed", ctypes.c_uint32), ("flush_survivors", ctypes.c_int)]
 
 
+class Orbit(ctypes.Structure):
+    _fields_ = [("target", ctypes.c_float * 3), ("distance", ctypes.c_float), ("theta_deg", ctypes.c_float),
+                ("phi_deg", ctypes.c_float)]
+
+
+class GBufPtrs(ctypes.Structure):
+    _fields_ = [("pos_tri", ctypes.c_void_p), ("nrm_mat", ctypes.c_void_p), ("bary_mesh", ctypes.c_void_p),
+                ("motion_depth", ctypes.c_void_p)]
+
+
+class SvgfParams(ctypes.Structure):
+    """DenoiseParam (Src/Utils/ui.h:14-25) with its defaults."""
+    _fields_ = [("max_history", ctypes.c_int), ("sigma_light", ctypes.c_float), ("sigma_normal", ctypes.c_float),
+                ("sigma_depth", ctypes.c_float), ("reproj_depth", ctypes.c_float), ("atrous_iters", ctypes.c_int)]
+
+    @classmethod
+    def default(cls, **kw) -> "SvgfParams":
+        p = cls(24, 8.0, 256.0, 1.0, 0.1, 5)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
 _LIB = None
 
 
@@ -83,6 +106,16 @@ def lib():
         L.ref_env_sample.argtypes = [vp, ctypes.c_float, ctypes.c_float, vp]
         L.ref_env_pdf_uv.argtypes = [vp, ctypes.c_float, ctypes.c_float]
         L.ref_env_pdf_uv.restype = ctypes.c_float
+        L.ref_get_orbit.argtypes = [vp, ctypes.POINTER(Orbit)]
+        L.ref_set_orbit.argtypes = [vp, ctypes.POINTER(Orbit)]
+        L.ref_view_projection.argtypes = [vp, ctypes.POINTER(Orbit), ctypes.c_float, ctypes.c_float, vp]
+        L.ref_gbuffer.argtypes = [vp, ctypes.POINTER(Orbit), ctypes.POINTER(GBufPtrs)]
+        L.ref_svgf_temporal.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SvgfParams),
+                                        ctypes.c_float] + [vp] * 7 + [ctypes.POINTER(GBufPtrs)] * 2 + [vp] * 5
+        L.ref_svgf_variance.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(SvgfParams)] + [vp] * 5 + \
+            [ctypes.POINTER(GBufPtrs), vp, vp]
+        L.ref_svgf_atrous.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SvgfParams)] + \
+            [vp] * 3 + [ctypes.POINTER(GBufPtrs), vp, vp]
         _LIB = L
     return _LIB
 
@@ -461,6 +494,24 @@ class RefScene:
                              _p(d), _p(i))
         return d, i
 
+    def orbit(self) -> Orbit:
+        o = Orbit()
+        lib().ref_get_orbit(self._h, ctypes.byref(o))
+        return o
+
+    def set_orbit(self, o: Orbit) -> None:
+        lib().ref_set_orbit(self._h, ctypes.byref(o))
+
+    def view_projection(self, o: Orbit, near: float, far: float) -> np.ndarray:
+        out = np.zeros(16, np.float32)
+        lib().ref_view_projection(self._h, ctypes.byref(o), near, far, _p(out))
+        return out.reshape(4, 4)
+
+    def gbuffer(self, prev: Optional[Orbit] = None) -> "GBuffer":
+        g = GBuffer.empty(self.a.width, self.a.height)
+        lib().ref_gbuffer(self._h, ctypes.byref(prev if prev is not None else self.orbit()), ctypes.byref(g.ptrs()))
+        return g
+
     def env_tables(self):
         row = np.zeros((self.a.env_h, self.a.env_w), np.float32)
         marg = np.zeros(self.a.env_h, np.float32)
@@ -478,3 +529,120 @@ def philox(ctr, key) -> np.ndarray:
 
 def math(op: int, a: float, b: float = 0.0) -> float:
     return float(lib().ref_math(op, a, b))
+
+
+# ------------------------------------------------------------------ SVGF
+@dataclass
+class GBuffer:
+    """G-buffer planes, (height, width, 4) each, row 0 = bottom."""
+    pos_tri: np.ndarray
+    nrm_mat: np.ndarray
+    bary_mesh: np.ndarray
+    motion_depth: np.ndarray
+
+    @classmethod
+    def empty(cls, w: int, h: int) -> "GBuffer":
+        return cls(np.zeros((h, w, 4), np.float32), np.zeros((h, w, 4), np.uint16), np.zeros((h, w, 4), np.uint16),
+                   np.zeros((h, w, 4), np.float32))
+
+    def ptrs(self) -> GBufPtrs:
+        for a in (self.pos_tri, self.nrm_mat, self.bary_mesh, self.motion_depth):
+            assert a.flags.c_contiguous
+        return GBufPtrs(_p(self.pos_tri), _p(self.nrm_mat), _p(self.bary_mesh), _p(self.motion_depth))
+
+
+def _c(a: np.ndarray, dt) -> np.ndarray:
+    a = np.ascontiguousarray(a, dt)
+    return a
+
+
+def svgf_temporal(first: bool, p: SvgfParams, inv_spp, acc_d, acc_i, hist_prev_d, hist_prev_i, mom_prev_d,
+                  mom_prev_i, len_prev, cur: GBuffer, prev: GBuffer):
+    """TemporalDenoising (renderer.cuh:524-677): returns (hist_d, hist_i, mom_d, mom_i, len)."""
+    h, w = acc_d.shape[:2]
+    acc_d, acc_i = _c(acc_d, np.float32), _c(acc_i, np.float32)
+    hpd, hpi = _c(hist_prev_d, np.float32), _c(hist_prev_i, np.float32)
+    mpd, mpi = _c(mom_prev_d, np.float32), _c(mom_prev_i, np.float32)
+    lp = _c(len_prev, np.int32)
+    hd, hi = np.zeros((h, w, 4), np.float32), np.zeros((h, w, 4), np.float32)
+    md, mi = np.zeros((h, w, 2), np.float32), np.zeros((h, w, 2), np.float32)
+    ln = np.zeros((h, w), np.int32)
+    cp, pp = cur.ptrs(), prev.ptrs()
+    lib().ref_svgf_temporal(w, h, int(first), ctypes.byref(p), float(np.float32(inv_spp)), _p(acc_d), _p(acc_i),
+                            _p(hpd), _p(hpi), _p(mpd), _p(mpi), _p(lp), ctypes.byref(cp), ctypes.byref(pp),
+                            _p(hd), _p(hi), _p(md), _p(mi), _p(ln))
+    return hd, hi, md, mi, ln
+
+
+def svgf_variance(p: SvgfParams, in_d, in_i, mom_d, mom_i, length, g: GBuffer):
+    """ComputeVarianceKernel (renderer.cuh:689-805): returns (out_d, out_i)."""
+    h, w = in_d.shape[:2]
+    in_d, in_i = _c(in_d, np.float32), _c(in_i, np.float32)
+    mom_d, mom_i, length = _c(mom_d, np.float32), _c(mom_i, np.float32), _c(length, np.int32)
+    od, oi = np.zeros((h, w, 4), np.float32), np.zeros((h, w, 4), np.float32)
+    gp = g.ptrs()
+    lib().ref_svgf_variance(w, h, ctypes.byref(p), _p(in_d), _p(in_i), _p(mom_d), _p(mom_i), _p(length),
+                            ctypes.byref(gp), _p(od), _p(oi))
+    return od, oi
+
+
+def svgf_atrous(step: int, p: SvgfParams, in_d, in_i, length, g: GBuffer):
+    """AtrousKernel (renderer.cuh:807-954): returns (out_d, out_i)."""
+    h, w = in_d.shape[:2]
+    in_d, in_i, length = _c(in_d, np.float32), _c(in_i, np.float32), _c(length, np.int32)
+    od, oi = np.zeros((h, w, 4), np.float32), np.zeros((h, w, 4), np.float32)
+    gp = g.ptrs()
+    lib().ref_svgf_atrous(w, h, int(step), ctypes.byref(p), _p(in_d), _p(in_i), _p(length), ctypes.byref(gp),
+                          _p(od), _p(oi))
+    return od, oi
+
+
+class RefFrames:
+    """The oracle's real-time loop (RenderLoop with denoising, renderer.cuh:1141-1302):
+    per frame the G-buffer against the previous frame's camera, `spp` fresh samples
+    (sample indices frame*spp ...), temporal blend, variance, and the a-trous passes
+    (none on the first frame after a history reset)."""
+
+    def __init__(self, scene: RefScene, params: Params, svgf: Optional[SvgfParams] = None):
+        self.s, self.params = scene, params
+        self.p = svgf if svgf is not None else SvgfParams.default()
+        w, h = scene.a.width, scene.a.height
+        self.w, self.h = w, h
+        self.frames = 0
+        self.last_orbit: Optional[Orbit] = None
+        z4 = lambda: np.zeros((h, w, 4), np.float32)
+        self.hist = (z4(), z4())
+        self.mom = (np.zeros((h, w, 2), np.float32), np.zeros((h, w, 2), np.float32))
+        self.length = np.zeros((h, w), np.int32)
+        self.g: Optional[GBuffer] = None
+
+    def reset_history(self) -> None:
+        self.frames = 0
+
+    def frame(self, spp: int = 1, orbit: Optional[Orbit] = None) -> dict:
+        s, w, h = self.s, self.w, self.h
+        if orbit is not None:
+            s.set_orbit(orbit)
+        cur = s.orbit()
+        first = self.frames == 0
+        prev_orbit = cur if first else self.last_orbit
+        g = s.gbuffer(prev_orbit)
+        d, i, _ = s.render(spp, self.params, sample0=self.frames * spp)
+        acc_d, acc_i = d.reshape(h, w, 4), i.reshape(h, w, 4)
+        inv = np.float32(1.0) / np.float32(spp)
+        prev_g = g if self.g is None else self.g
+        hd, hi, md, mi, ln = svgf_temporal(first, self.p, inv, acc_d, acc_i, self.hist[0], self.hist[1],
+                                           self.mom[0], self.mom[1], self.length, g, prev_g)
+        vd, vi = svgf_variance(self.p, hd, hi, md, mi, ln, g)
+        od, oi = vd, vi
+        if not first:
+            for k in range(max(1, min(8, self.p.atrous_iters))):
+                od, oi = svgf_atrous(1 << k, self.p, od, oi, ln, g)
+        den = np.empty((h, w, 4), np.float32)
+        den[..., :3] = od[..., :3] + oi[..., :3]
+        den[..., 3] = 1.0
+        self.hist, self.mom, self.length, self.g = (hd, hi), (md, mi), ln, g
+        self.last_orbit = cur
+        self.frames += 1
+        return {"acc_d": acc_d, "acc_i": acc_i, "gbuffer": g, "hist_d": hd, "hist_i": hi, "mom_d": md,
+                "mom_i": mi, "length": ln, "var_d": vd, "var_i": vi, "out_d": od, "out_i": oi, "denoised": den}

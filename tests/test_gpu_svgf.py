@@ -1,0 +1,157 @@
+"""GPU parity of the real-time frame loop (G-buffer + SVGF) against the oracle.
+
+Every buffer of every frame -- the four G-buffer planes, the temporal history,
+moments and history length, the variance-pass output, the a-trous output and
+the denoised colour -- must be bit-identical to the oracle's frame loop
+(oracle/rtref.py RefFrames) on the same scene and camera path: both sides run
+the same FP32 operation sequence with contraction off."""
+import numpy as np
+import pytest
+
+import cudaraytracer_amd as rt
+from oracle import rtref
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(got, want, what):
+    got = np.ascontiguousarray(got)
+    want = np.ascontiguousarray(want).reshape(got.shape).astype(got.dtype)
+    if got.dtype == np.float32:
+        bad = got.view(np.int32) != want.view(np.int32)
+    else:
+        bad = got != want
+    if bad.any():
+        idx = np.argwhere(bad)[:5]
+        raise AssertionError(f"{what}: {bad.sum()} of {bad.size} differ, first at {idx.tolist()}: "
+                             f"got {got[tuple(idx[0])]} want {want[tuple(idx[0])]}")
+
+
+def _check_frame(r, ref, label):
+    g = ref["gbuffer"]
+    _same(r.read_frame(rt.RT_FRAME_G_POSITION_TRI), g.pos_tri, f"{label} g.pos_tri")
+    _same(r.read_frame(rt.RT_FRAME_G_NORMAL_MAT), g.nrm_mat, f"{label} g.nrm_mat")
+    _same(r.read_frame(rt.RT_FRAME_G_BARY_MESH), g.bary_mesh, f"{label} g.bary_mesh")
+    _same(r.read_frame(rt.RT_FRAME_G_MOTION_DEPTH), g.motion_depth, f"{label} g.motion_depth")
+    _same(r.read_accum(rt.RT_ACCUM_DIRECT)[..., :3], ref["acc_d"][..., :3], f"{label} acc direct")
+    _same(r.read_accum(rt.RT_ACCUM_INDIRECT)[..., :3], ref["acc_i"][..., :3], f"{label} acc indirect")
+    _same(r.read_frame(rt.RT_FRAME_HISTORY_DIRECT), ref["hist_d"], f"{label} history direct")
+    _same(r.read_frame(rt.RT_FRAME_HISTORY_INDIRECT), ref["hist_i"], f"{label} history indirect")
+    _same(r.read_frame(rt.RT_FRAME_MOMENTS_DIRECT), ref["mom_d"], f"{label} moments direct")
+    _same(r.read_frame(rt.RT_FRAME_MOMENTS_INDIRECT), ref["mom_i"], f"{label} moments indirect")
+    _same(r.read_frame(rt.RT_FRAME_HISTORY_LENGTH)[..., 0], ref["length"], f"{label} history length")
+    _same(r.read_frame(rt.RT_FRAME_VARIANCE_DIRECT), ref["var_d"], f"{label} variance direct")
+    _same(r.read_frame(rt.RT_FRAME_VARIANCE_INDIRECT), ref["var_i"], f"{label} variance indirect")
+    _same(r.read_frame(rt.RT_FRAME_DIRECT), ref["out_d"], f"{label} filtered direct")
+    _same(r.read_frame(rt.RT_FRAME_INDIRECT), ref["out_i"], f"{label} filtered indirect")
+    _same(r.read_frame(rt.RT_FRAME_DENOISED), ref["denoised"], f"{label} denoised")
+
+
+def _orbit_to_camera(o):
+    return rt.Camera((o.target[0], o.target[1], o.target[2]), o.distance, o.theta_deg, o.phi_deg)
+
+
+def _moves(o0):
+    """Static, orbit, zoom + tilt, pan, static."""
+    t = tuple(o0.target)
+    return [
+        None,
+        None,
+        rtref.Orbit(t, o0.distance, o0.theta_deg, o0.phi_deg + 2.0),
+        rtref.Orbit(t, o0.distance * 0.97, o0.theta_deg - 1.5, o0.phi_deg + 2.0),
+        rtref.Orbit((t[0] + 0.05, t[1], t[2]), o0.distance * 0.97, o0.theta_deg - 1.5, o0.phi_deg + 2.0),
+        None,
+    ]
+
+
+def _run(scene_path, spp, atrous_iters=5):
+    a = rtref.parse_scene(scene_path)
+    ref_scene = rtref.RefScene(a)
+    sp = rtref.SvgfParams.default(atrous_iters=atrous_iters)
+    frames = rtref.RefFrames(ref_scene, ref_scene.params(), sp)
+    scene = rt.Scene(scene_path)
+    r = rt.Renderer(scene)
+    cam = r.camera()
+    o0 = ref_scene.orbit()
+    assert list(cam.target) == list(o0.target)
+    assert (cam.distance, cam.theta_deg, cam.phi_deg) == (o0.distance, o0.theta_deg, o0.phi_deg)
+    dp = rt.DenoiseParams.default(atrous_iterations=atrous_iters)
+    for f, o in enumerate(_moves(o0)):
+        print(f"frame {f} orbit {None if o is None else (list(o.target), o.distance, o.theta_deg, o.phi_deg)}",
+              flush=True)
+        if o is not None:
+            r.set_camera(_orbit_to_camera(o))
+        r.render_frame(spp, dp)
+        ref = frames.frame(spp, o)
+        _check_frame(r, ref, f"frame {f}")
+    return r, frames
+
+
+def test_svgf_frames_bit_exact_cornell(scene_dir, oracle_lib):
+    r, frames = _run(scene_dir["cornell_spheres_rt"], spp=1)
+    ln = r.read_frame(rt.RT_FRAME_HISTORY_LENGTH)[..., 0]
+    assert ln.max() >= 4  # the static tail grew real histories
+    # A history reset makes the next frame a first frame again.
+    r.reset_history()
+    frames.reset_history()
+    r.render_frame(1)
+    ref = frames.frame(1)
+    _check_frame(r, ref, "after reset")
+    assert (r.read_frame(rt.RT_FRAME_HISTORY_LENGTH) == 1).all()
+
+
+def test_svgf_frames_bit_exact_terrain_env_sss(scene_dir, oracle_lib):
+    _run(scene_dir["terrain_small"], spp=2, atrous_iters=3)
+
+
+def test_frame_api_errors_and_accumulation(scene_dir, oracle_lib):
+    scene = rt.Scene(scene_dir["cornell_spheres_rt"])
+    r = rt.Renderer(scene)
+    with pytest.raises(rt.RtError) as e:
+        r.read_frame(rt.RT_FRAME_DENOISED)
+    assert e.value.code == -1
+    with pytest.raises(rt.RtError):
+        r.render_frame(0)
+    cam = r.camera()
+    bad = rt.Camera(tuple(cam.target), 0.0, cam.theta_deg, cam.phi_deg)
+    with pytest.raises(rt.RtError):
+        r.set_camera(bad)
+    r.render_frame(1)
+    small = np.zeros(4, np.float32)
+    code = rt.lib().rt_read_frame(r._h, rt.RT_FRAME_DENOISED, small.ctypes.data, small.nbytes)
+    assert code == -1
+    # Moving the camera clears the progressive accumulation; rt_render afterwards
+    # is the plain accumulator again (samples from index 0).
+    cam2 = rt.Camera(tuple(cam.target), cam.distance, cam.theta_deg, cam.phi_deg + 3.0)
+    r.set_camera(cam2)
+    assert r.stats().spp_done == 0
+    r.render(3)
+    a = rtref.parse_scene(scene_dir["cornell_spheres_rt"])
+    s = rtref.RefScene(a)
+    s.set_orbit(rtref.Orbit(tuple(cam.target), cam.distance, cam.theta_deg, cam.phi_deg + 3.0))
+    d, i, _ = s.render(3, s.params())
+    _same(r.read_accum(rt.RT_ACCUM_DIRECT)[..., :3], d.reshape(r.height, r.width, 4)[..., :3],
+          "accum after camera move (direct)")
+    _same(r.read_accum(rt.RT_ACCUM_INDIRECT)[..., :3], i.reshape(r.height, r.width, 4)[..., :3],
+          "accum after camera move (indirect)")
+
+
+def test_svgf_full_size_invariants(scene_dir):
+    """The denoised loop at the full benchmark resolution: finite, non-negative
+    output, history lengths within [1, max_history], first frame all ones."""
+    scene = rt.Scene(scene_dir["cornell_spheres"])
+    r = rt.Renderer(scene)
+    r.render_frame(1)
+    assert (r.read_frame(rt.RT_FRAME_HISTORY_LENGTH) == 1).all()
+    cam = r.camera()
+    for k in range(1, 6):
+        r.set_camera(rt.Camera(tuple(cam.target), cam.distance, cam.theta_deg, cam.phi_deg + 0.5 * k))
+        r.render_frame(1)
+    den = r.read_frame(rt.RT_FRAME_DENOISED)
+    assert np.isfinite(den).all() and (den[..., :3] >= 0).all()
+    ln = r.read_frame(rt.RT_FRAME_HISTORY_LENGTH)
+    assert ln.min() >= 1 and ln.max() <= 24 and ln.max() >= 5
+    # Filtering lowers the pixel-to-pixel noise of the 1-spp frame.
+    raw = r.read_accum(rt.RT_ACCUM_MEAN)[..., :3]
+    hit = r.read_frame(rt.RT_FRAME_G_BARY_MESH)[..., 3] > 0
+    assert np.abs(np.diff(den[..., 0], axis=1))[hit[:, 1:]].mean() < np.abs(np.diff(raw[..., 0], axis=1))[hit[:, 1:]].mean()

@@ -49,6 +49,25 @@ def test_transcendentals_vs_libm(oracle_lib, op, fn, lo, hi, maxulp):
     assert worst <= maxulp, worst
 
 
+def test_pow_vs_libm(oracle_lib):
+    """pow(x, y) = exp(y log x) for the SVGF normal weight: relative error bounded by
+    the rounding of y log x, |y log x| * 2^-24, plus a few ulp of exp."""
+    rng = np.random.default_rng(10)
+    xs = rng.uniform(0.0, 1.0, 4000).astype(np.float32)
+    ys = rng.uniform(0.0, 300.0, 4000).astype(np.float32)
+    for x, y in zip(xs, ys):
+        want = math.pow(float(x), float(y))
+        if want < 1e-30:
+            continue
+        got = oracle_lib.math(10, float(x), float(y))
+        tol = 3e-7 + abs(float(y) * math.log(float(x))) * 1.2e-7
+        assert abs(got - want) <= tol * want, (x, y, got, want)
+    assert oracle_lib.math(10, 0.0, 2.0) == 0.0
+    assert oracle_lib.math(10, 0.0, 0.0) == 1.0
+    assert oracle_lib.math(10, 1.0, 256.0) == 1.0
+    assert oracle_lib.math(10, 0.5, 1.0) == 0.5
+
+
 def test_acos_and_atan2_vs_libm(oracle_lib):
     rng = np.random.default_rng(7)
     for x in rng.uniform(-1, 1, 4000).astype(np.float32):
