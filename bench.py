@@ -41,6 +41,71 @@ def _stage_roofline(name, bytes_per_launch, ms_per_launch, traffic=None):
             "bytes_per_launch": int(bytes_per_launch), "ms_per_launch": round(ms_per_launch, 5)}
 
 
+# SVGF, per pixel, each pixel's data read once: temporal reads this frame's sums
+# (32), last frame's history, moments and length (52) and both G-buffers
+# (2 x 48) and writes history, moments and length (52); variance reads colour
+# (32), moments (16), length (4) and the G-buffer planes it uses (32) and writes
+# 32; one a-trous pass reads colour (32), length (4) and G-buffer (32) and
+# writes 32.  The G-buffer kernel writes 48 B per pixel (its traversal is not
+# counted).
+TEMPORAL_PX_BYTES, VARIANCE_PX_BYTES, ATROUS_PX_BYTES, GBUFFER_PX_BYTES = 232, 116, 100, 48
+
+
+def frames_bench(frames: int = 10, width: int = 1280, height: int = 720):
+    """Real-time loop on the terrain + subsurface + HDR environment scene
+    (2 M triangles): `frames` frames of an orbiting camera at 1 spp with SVGF
+    (temporal, variance, 5 a-trous passes).  Returns per-stage ms per frame."""
+    import cudaraytracer_amd as rt
+    from cudaraytracer_amd import scenes
+
+    tmp = tempfile.mkdtemp(prefix="rtframes_")
+    path = scenes.write_scene(scenes.terrain_env(width=width, height=height, spp=1), tmp, "terrain_frames")
+    scene = rt.Scene(path)
+    info = scene.info()
+
+    # Wall-clock frame latency (synchronised per frame), then a profiled pass for stage times.
+    r = rt.Renderer(scene)
+    base = r.camera()
+    r.render_frame(1)
+    r.synchronize()
+    lat = []
+    for k in range(1, frames + 1):
+        r.set_camera(rt.Camera(tuple(base.target), base.distance, base.theta_deg, base.phi_deg + 0.5 * k))
+        t0 = time.perf_counter()
+        r.render_frame(1)
+        r.synchronize()
+        lat.append((time.perf_counter() - t0) * 1e3)
+    den = r.read_frame(rt.RT_FRAME_DENOISED)
+    assert np.isfinite(den).all()
+    rp = rt.Renderer(scene, profile=True)
+    rp.render_frame(1)
+    rp.synchronize()
+    rp.reset_stats()
+    for k in range(1, frames + 1):
+        rp.set_camera(rt.Camera(tuple(base.target), base.distance, base.theta_deg, base.phi_deg + 0.5 * k))
+        rp.render_frame(1)
+    st = rp.stats()
+    ms = {name: st.stage_ms[i] / frames for i, name in enumerate(rt.STAGES)}
+    px = width * height
+    atrous_launch_ms = st.stage_ms[8] / frames / 5
+    ach = ATROUS_PX_BYTES * px / (atrous_launch_ms * 1e-3) / 1e9
+    per_stage_gbs = {
+        "temporal": TEMPORAL_PX_BYTES * px / (ms["temporal"] * 1e-3) / 1e9,
+        "variance": VARIANCE_PX_BYTES * px / (ms["variance"] * 1e-3) / 1e9,
+        "atrous": ach,
+    }
+    return {
+        "workload": f"terrain_env {info.num_triangles} tris + SSS sphere + 2048x1024 env, {width}x{height}, "
+                    f"{frames}-frame orbit, 1 spp/frame, SVGF 5 a-trous passes",
+        "frame_ms_p50": round(float(np.percentile(lat, 50)), 3),
+        "frame_ms_max": round(float(np.max(lat)), 3),
+        "stage_ms_per_frame": {k: round(v, 4) for k, v in ms.items()},
+        "svgf_gbs": {k: round(v, 1) for k, v in per_stage_gbs.items()},
+        "atrous_roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(ach / HBM_PEAK_GBS, 4)},
+    }
+
+
 def cpu_baseline(scene_path: str, spp: int):
     from oracle import rtref
 
@@ -71,6 +136,7 @@ def main() -> int:
     ap.add_argument("--cpu-spp", type=int, default=64, help="samples per pixel for the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=None, help="HBM bytes per launch measured with rocprofv3 --pmc")
+    ap.add_argument("--no-frames", action="store_true", help="skip the real-time SVGF frame-loop measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU is design-only)"}))
@@ -163,7 +229,11 @@ def main() -> int:
         "roofline_kernel": dominant["kernel"],
         "stages": stages,
     }
+    if not args.no_frames:
+        print("[bench] real-time frame loop (SVGF)", file=sys.stderr, flush=True)
+        out["frames"] = frames_bench()
     if not args.no_cpu_baseline:
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(path, args.cpu_spp)
     print(json.dumps(out))
     return 0

@@ -903,6 +903,20 @@ float ref_burley(const float* mat, float r, int ch) {
     return burley_f(&m, r, ch);
 }
 
+float ref_sss_dist(float scatter_distance, float u) { return sss_dist_f(scatter_distance, u); }
+
+void ref_sss_probe(const ref_scene* s, const float* p, const float* n, float dist, float angle, float* out_p,
+                   float* out_n) {
+    Vec_f op, on;
+    probe_f(s, from_f(v3p(p)), from_f(v3p(n)), dist, angle, &op, &on);
+    out_p[0] = op.x;
+    out_p[1] = op.y;
+    out_p[2] = op.z;
+    out_n[0] = on.x;
+    out_n[1] = on.y;
+    out_n[2] = on.z;
+}
+
 void ref_env_sample(const ref_scene* s, float rv, float ru, float* uv_out) {
     float u, v;
     env_sample_uv_f(s, rv, ru, &u, &v);

@@ -132,6 +132,11 @@ float ref_bsdf_sample(const float* mat, const float* wi, const float* n, const f
 float ref_fresnel_dielectric(float cos_i, float eta_i, float eta_t);
 float ref_ggx_d(const float* mat, const float* wi, const float* n, const float* wo);
 float ref_burley(const float* mat, float r, int ch);
+/* SSS: exit distance from one uniform for scatter distance d; the two-sided probe
+   from p (normal n) at (dist, angle) returning the exit point and its normal. */
+float ref_sss_dist(float scatter_distance, float u);
+void ref_sss_probe(const ref_scene* s, const float* p, const float* n, float dist, float angle, float* out_p,
+                   float* out_n);
 void ref_env_tables(const ref_scene* s, float* row_cdf, float* marg_cdf);
 void ref_env_sample(const ref_scene* s, float rv, float ru, float* uv_out);
 float ref_env_pdf_uv(const ref_scene* s, float u, float v);

@@ -102,6 +102,9 @@ def lib():
         L.ref_ggx_d.restype = ctypes.c_float
         L.ref_burley.argtypes = [vp, ctypes.c_float, ctypes.c_int]
         L.ref_burley.restype = ctypes.c_float
+        L.ref_sss_dist.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.ref_sss_dist.restype = ctypes.c_float
+        L.ref_sss_probe.argtypes = [vp, vp, vp, ctypes.c_float, ctypes.c_float, vp, vp]
         L.ref_env_tables.argtypes = [vp, vp, vp]
         L.ref_env_sample.argtypes = [vp, ctypes.c_float, ctypes.c_float, vp]
         L.ref_env_pdf_uv.argtypes = [vp, ctypes.c_float, ctypes.c_float]
@@ -511,6 +514,22 @@ class RefScene:
         g = GBuffer.empty(self.a.width, self.a.height)
         lib().ref_gbuffer(self._h, ctypes.byref(prev if prev is not None else self.orbit()), ctypes.byref(g.ptrs()))
         return g
+
+    def sss_probe(self, p, n, dist: float, angle: float):
+        pp = np.ascontiguousarray(p, np.float32)
+        nn = np.ascontiguousarray(n, np.float32)
+        op = np.zeros(3, np.float32)
+        on = np.zeros(3, np.float32)
+        lib().ref_sss_probe(self._h, _p(pp), _p(nn), dist, angle, _p(op), _p(on))
+        return op, on
+
+    def env_sample(self, rv: float, ru: float):
+        uv = np.zeros(2, np.float32)
+        lib().ref_env_sample(self._h, rv, ru, _p(uv))
+        return uv
+
+    def env_pdf_uv(self, u: float, v: float) -> float:
+        return float(lib().ref_env_pdf_uv(self._h, u, v))
 
     def env_tables(self):
         row = np.zeros((self.a.env_h, self.a.env_w), np.float32)
