@@ -63,8 +63,11 @@ EXPORTS = (
     "rt_reset", "rt_reset_stats", "rt_read_accum", "rt_read_rgba8", "rt_stats", "rt_last_error",
     "rt_scene_get_triangles", "rt_scene_get_meshes", "rt_scene_get_bvh", "rt_trace_rays", "rt_device_math",
     "rt_denoise_params_default", "rt_get_camera", "rt_set_camera", "rt_render_frame", "rt_reset_history",
-    "rt_read_frame",
+    "rt_read_frame", "rt_pool_create", "rt_pool_destroy", "rt_pool_add_scene", "rt_pool_submit", "rt_pool_step",
+    "rt_pool_job_status", "rt_pool_read_accum", "rt_pool_read_frame", "rt_pool_release", "rt_pool_stats",
+    "rt_pool_reset_stats", "rt_pool_serve",
 )
+RT_ERR_BUSY = -7
 
 
 class RtError(RuntimeError):
@@ -136,6 +139,34 @@ class DenoiseParams(ctypes.Structure):
         return p
 
 
+class PoolParams(ctypes.Structure):
+    _fields_ = [("queue_budget_bytes", ctypes.c_int64), ("hbm_budget_bytes", ctypes.c_int64), ("profile", ctypes.c_int)]
+
+
+class JobDescC(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_int), ("spp", ctypes.c_int), ("frames", ctypes.c_int),
+                ("cameras", ctypes.POINTER(Camera)), ("render", ctypes.POINTER(RenderParams)),
+                ("denoise", ctypes.POINTER(DenoiseParams))]
+
+
+class JobStatus(ctypes.Structure):
+    _fields_ = [("done", ctypes.c_int), ("spp_done", ctypes.c_int), ("frames_done", ctypes.c_int),
+                ("submit_s", ctypes.c_double), ("start_s", ctypes.c_double), ("done_s", ctypes.c_double)]
+
+
+class ServeRequestC(ctypes.Structure):
+    _fields_ = [("arrival_s", ctypes.c_double), ("job", JobDescC)]
+
+
+class ServeReport(ctypes.Structure):
+    _fields_ = [("jobs", ctypes.c_int), ("frames", ctypes.c_int), ("makespan_s", ctypes.c_double),
+                ("jobs_per_s", ctypes.c_double), ("frame_p50_ms", ctypes.c_double), ("frame_p99_ms", ctypes.c_double),
+                ("job_p50_ms", ctypes.c_double), ("job_p99_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _LIB: Optional[ctypes.CDLL] = None
 
 
@@ -177,6 +208,20 @@ def lib() -> ctypes.CDLL:
     L.rt_render_frame.argtypes = [vp, ip, ctypes.POINTER(DenoiseParams)]
     L.rt_reset_history.argtypes = [vp]
     L.rt_read_frame.argtypes = [vp, ip, vp, ctypes.c_size_t]
+    L.rt_pool_create.argtypes = [ctypes.POINTER(PoolParams), ctypes.POINTER(vp)]
+    L.rt_pool_destroy.argtypes = [vp]
+    L.rt_pool_destroy.restype = None
+    L.rt_pool_add_scene.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int)]
+    L.rt_pool_submit.argtypes = [vp, ctypes.POINTER(JobDescC), ctypes.POINTER(ctypes.c_int)]
+    L.rt_pool_step.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.rt_pool_job_status.argtypes = [vp, ip, ctypes.POINTER(JobStatus)]
+    L.rt_pool_read_accum.argtypes = [vp, ip, ip, fp]
+    L.rt_pool_read_frame.argtypes = [vp, ip, ip, vp, ctypes.c_size_t]
+    L.rt_pool_release.argtypes = [vp, ip]
+    L.rt_pool_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.rt_pool_reset_stats.argtypes = [vp]
+    L.rt_pool_serve.argtypes = [vp, ctypes.POINTER(ServeRequestC), ip, ip, ctypes.POINTER(ServeReport),
+                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     _LIB = L
     return L
 
@@ -333,6 +378,131 @@ class Renderer:
         out = np.zeros((self.height, self.width, ch), dt)
         _check(lib().rt_read_frame(self._h, int(which), out.ctypes.data_as(ctypes.c_void_p), out.nbytes))
         return out
+
+
+class Job:
+    """A render job for a Pool: an accumulate job (frames = 0, `spp` samples in
+    total) or a real-time job (`frames` > 0 frames of `spp` samples each, one
+    camera per frame, SVGF-denoised)."""
+
+    def __init__(self, scene: int, spp: int = 1, frames: int = 0, cameras=None,
+                 render: Optional[dict] = None, denoise: Optional[DenoiseParams] = None):
+        self.scene, self.spp, self.frames = scene, spp, frames
+        self.cameras = list(cameras) if cameras is not None else None
+        self.render, self.denoise = render, denoise
+
+    def to_c(self, keep: list) -> JobDescC:
+        d = JobDescC()
+        d.scene, d.spp, d.frames = self.scene, self.spp, self.frames
+        if self.cameras:
+            arr = (Camera * len(self.cameras))(*self.cameras)
+            keep.append(arr)
+            d.cameras = ctypes.cast(arr, ctypes.POINTER(Camera))
+        if self.render is not None:
+            rp = RenderParams()
+            lib().rt_render_params_default(ctypes.byref(rp))
+            for k, v in self.render.items():
+                setattr(rp, k, v)
+            keep.append(rp)
+            d.render = ctypes.pointer(rp)
+        if self.denoise is not None:
+            keep.append(self.denoise)
+            d.denoise = ctypes.pointer(self.denoise)
+        return d
+
+
+class Pool:
+    """Many jobs over resident scenes, served by one wavefront engine."""
+
+    def __init__(self, queue_budget_bytes: int = 0, hbm_budget_bytes: int = 0, profile: bool = False):
+        p = PoolParams(queue_budget_bytes, hbm_budget_bytes, int(profile))
+        self._h = ctypes.c_void_p()
+        _check(lib().rt_pool_create(ctypes.byref(p), ctypes.byref(self._h)))
+        self._scenes = []
+
+    def close(self) -> None:
+        if self._h:
+            lib().rt_pool_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_scene(self, scene: Scene) -> int:
+        slot = ctypes.c_int()
+        _check(lib().rt_pool_add_scene(self._h, scene.handle, ctypes.byref(slot)))
+        self._scenes.append(scene)  # the scene must outlive the pool
+        return slot.value
+
+    def submit(self, job: Job) -> int:
+        """Job id; raises RtError(RT_ERR_BUSY) when the HBM budget cannot admit it now."""
+        keep: list = []
+        d = job.to_c(keep)
+        out = ctypes.c_int()
+        _check(lib().rt_pool_submit(self._h, ctypes.byref(d), ctypes.byref(out)))
+        return out.value
+
+    def step(self) -> int:
+        n = ctypes.c_int()
+        _check(lib().rt_pool_step(self._h, ctypes.byref(n)))
+        return n.value
+
+    def run(self) -> None:
+        while self.step() > 0:
+            pass
+
+    def status(self, job: int) -> JobStatus:
+        st = JobStatus()
+        _check(lib().rt_pool_job_status(self._h, job, ctypes.byref(st)))
+        return st
+
+    def read_accum(self, job: int, which: int, width: int, height: int) -> np.ndarray:
+        out = np.zeros((height, width, 4), np.float32)
+        _check(lib().rt_pool_read_accum(self._h, job, which, _fptr(out)))
+        return out
+
+    def read_frame(self, job: int, which: int, width: int, height: int) -> np.ndarray:
+        dt, ch = FRAME_LAYOUT.get(which, (np.float32, 4))
+        out = np.zeros((height, width, ch), dt)
+        _check(lib().rt_pool_read_frame(self._h, job, which, out.ctypes.data_as(ctypes.c_void_p), out.nbytes))
+        return out
+
+    def release(self, job: int) -> None:
+        _check(lib().rt_pool_release(self._h, job))
+
+    def stats(self) -> Stats:
+        s = Stats()
+        _check(lib().rt_pool_stats(self._h, ctypes.byref(s)))
+        return s
+
+    def reset_stats(self) -> None:
+        _check(lib().rt_pool_reset_stats(self._h))
+
+    def serve(self, requests, batched: bool = True):
+        """requests: list of (arrival_s, Job).  Returns (report dict, per-request frame
+        latencies in ms, per-request job latencies in ms)."""
+        keep: list = []
+        n = len(requests)
+        arr = (ServeRequestC * n)()
+        nframes = []
+        for k, (t, job) in enumerate(requests):
+            arr[k].arrival_s = t
+            arr[k].job = job.to_c(keep)
+            nframes.append(max(1, job.frames))
+        fl = np.zeros(sum(nframes), np.float64)
+        jl = np.zeros(n, np.float64)
+        rep = ServeReport()
+        _check(lib().rt_pool_serve(self._h, arr, n, int(batched), ctypes.byref(rep),
+                                   fl.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   jl.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        per_req, o = [], 0
+        for c in nframes:
+            per_req.append(fl[o:o + c].copy())
+            o += c
+        return rep.as_dict(), per_req, jl
 
 
 def device_math(op: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:

@@ -1,6 +1,7 @@
 // extern "C" boundary: opaque handles, return codes, thread-local last error.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -21,6 +22,10 @@ struct rt_scene {
 struct rt_renderer {
     rt_scene* scene = nullptr;
     std::unique_ptr<rt::Renderer> r;
+};
+
+struct rt_pool {
+    std::unique_ptr<rt::Pool> p;
 };
 
 namespace {
@@ -59,6 +64,63 @@ void ensure_gpu(rt_scene* s) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw std::runtime_error("HIP error: no GPU device available");
     if (!s->gpu) s->gpu.reset(new rt::GpuScene(s->host));
+}
+
+// Render settings from the caller's overrides (negative = the scene's value).
+rt::RenderSettings settings_from(const rt_render_params* p, const rt::HostScene& h) {
+    rt_render_params d;
+    rt_render_params_default(&d);
+    if (!p) p = &d;
+    rt::RenderSettings set;
+    set.max_bounces = p->max_bounces >= 0 ? p->max_bounces : h.max_bounces;
+    set.rr = p->rr >= 0.f ? p->rr : h.rr;
+    set.rr_start_depth = p->rr_start_depth >= 0 ? p->rr_start_depth : h.rr_start_depth;
+    set.seed = p->seed >= 0 ? (uint32_t)p->seed : h.seed;
+    if (p->batch_spp < 0) throw std::invalid_argument("batch_spp must be >= 0");
+    set.batch_spp = p->batch_spp;
+    set.queue_budget_bytes = p->queue_budget_bytes > 0 ? (size_t)p->queue_budget_bytes : 0;
+    set.flush_survivors = p->flush_survivors;
+    set.profile = p->profile;
+    if (set.max_bounces > 255) throw std::invalid_argument("max_bounces must be <= 255");
+    return set;
+}
+
+// Orbit camera on top of a base camera (resolution, fov).
+rt::CameraRec camera_from(const rt_camera* cam, const rt::CameraRec& base) {
+    float vals[6] = {cam->target[0], cam->target[1], cam->target[2], cam->distance, cam->theta_deg, cam->phi_deg};
+    for (float v : vals)
+        if (!std::isfinite(v)) throw std::invalid_argument("camera parameters must be finite");
+    if (!(cam->distance > 0.f)) throw std::invalid_argument("camera distance must be positive");
+    rt::CameraRec c = base;
+    c.target = rt::v3(cam->target[0], cam->target[1], cam->target[2]);
+    c.distance = cam->distance;
+    c.theta = cam->theta_deg;
+    c.phi = cam->phi_deg;
+    rt::compute_camera(c);
+    return c;
+}
+
+rt::SvgfParams svgf_from(const rt_denoise_params* p) {
+    rt::SvgfParams sp;
+    if (p) {
+        if (p->max_history < 1) throw std::invalid_argument("max_history must be >= 1");
+        sp.max_history = p->max_history;
+        sp.sigma_light = p->sigma_light;
+        sp.sigma_normal = p->sigma_normal;
+        sp.sigma_depth = p->sigma_depth;
+        sp.reproj_depth = p->reprojection_depth;
+        sp.atrous_iters = p->atrous_iterations;
+    }
+    return sp;
+}
+
+double percentile(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    size_t k = (size_t)std::ceil(q * (double)v.size());
+    if (k < 1) k = 1;
+    if (k > v.size()) k = v.size();
+    return v[k - 1];  // nearest rank
 }
 }  // namespace
 
@@ -142,21 +204,7 @@ int rt_renderer_create(rt_scene* s, const rt_render_params* p, rt_renderer** out
     if (!s || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     return guard([&]() {
-        rt_render_params d;
-        rt_render_params_default(&d);
-        if (!p) p = &d;
-        rt::RenderSettings set;
-        const rt::HostScene& h = s->host;
-        set.max_bounces = p->max_bounces >= 0 ? p->max_bounces : h.max_bounces;
-        set.rr = p->rr >= 0.f ? p->rr : h.rr;
-        set.rr_start_depth = p->rr_start_depth >= 0 ? p->rr_start_depth : h.rr_start_depth;
-        set.seed = p->seed >= 0 ? (uint32_t)p->seed : h.seed;
-        if (p->batch_spp < 0) throw std::runtime_error("batch_spp must be >= 0");
-        set.batch_spp = p->batch_spp;
-        set.queue_budget_bytes = p->queue_budget_bytes > 0 ? (size_t)p->queue_budget_bytes : 0;
-        set.flush_survivors = p->flush_survivors;
-        set.profile = p->profile;
-        if (set.max_bounces > 255) throw std::runtime_error("max_bounces must be <= 255");
+        rt::RenderSettings set = settings_from(p, s->host);
         ensure_gpu(s);
         std::unique_ptr<rt_renderer> r(new rt_renderer);
         r->scene = s;
@@ -275,36 +323,16 @@ int rt_get_camera(rt_renderer* r, rt_camera* out) {
 
 int rt_set_camera(rt_renderer* r, const rt_camera* cam) {
     if (!r || !cam) return fail(RT_ERR_INVALID_ARG, "null argument");
-    float vals[6] = {cam->target[0], cam->target[1], cam->target[2], cam->distance, cam->theta_deg, cam->phi_deg};
-    for (float v : vals)
-        if (!std::isfinite(v)) return fail(RT_ERR_INVALID_ARG, "camera parameters must be finite");
-    if (!(cam->distance > 0.f)) return fail(RT_ERR_INVALID_ARG, "camera distance must be positive");
     return guard([&]() {
-        rt::CameraRec c = r->r->camera();
-        c.target = rt::v3(cam->target[0], cam->target[1], cam->target[2]);
-        c.distance = cam->distance;
-        c.theta = cam->theta_deg;
-        c.phi = cam->phi_deg;
-        rt::compute_camera(c);
-        r->r->set_camera(c);
+        r->r->set_camera(camera_from(cam, r->r->camera()));
         return RT_OK;
     });
 }
 
 int rt_render_frame(rt_renderer* r, int spp, const rt_denoise_params* p) {
     if (!r || spp < 1) return fail(RT_ERR_INVALID_ARG, "invalid argument");
-    rt::SvgfParams sp;
-    if (p) {
-        if (p->max_history < 1) return fail(RT_ERR_INVALID_ARG, "max_history must be >= 1");
-        sp.max_history = p->max_history;
-        sp.sigma_light = p->sigma_light;
-        sp.sigma_normal = p->sigma_normal;
-        sp.sigma_depth = p->sigma_depth;
-        sp.reproj_depth = p->reprojection_depth;
-        sp.atrous_iters = p->atrous_iterations;
-    }
     return guard([&]() {
-        r->r->render_frame(spp, sp);
+        r->r->render_frame(spp, svgf_from(p));
         return RT_OK;
     });
 }
@@ -324,6 +352,173 @@ int rt_read_frame(rt_renderer* r, int which, void* out, size_t bytes) {
         return fail(RT_ERR_INVALID_ARG, "buffer size " + std::to_string(bytes) + " != " + std::to_string(need));
     return guard([&]() {
         r->r->read_frame(which, out);
+        return RT_OK;
+    });
+}
+
+// ------------------------------------------------------------------ pools
+namespace {
+rt::JobDesc job_from(rt_pool* p, const rt_job_desc* d) {
+    const rt::HostScene& h = p->p->scene_host(d->scene);
+    rt::JobDesc j;
+    j.scene = d->scene;
+    j.set = settings_from(d->render, h);
+    j.spp = d->spp;
+    j.frames = d->frames;
+    j.svgf = svgf_from(d->denoise);
+    int ncam = d->frames > 0 ? d->frames : (d->cameras ? 1 : 0);
+    if (d->frames > 0 && !d->cameras) throw std::invalid_argument("a real-time job needs one camera per frame");
+    for (int k = 0; k < ncam; ++k) j.cams.push_back(camera_from(&d->cameras[k], h.camera));
+    return j;
+}
+}  // namespace
+
+int rt_pool_create(const rt_pool_params* p, rt_pool** out) {
+    if (!out) return fail(RT_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (p && (p->queue_budget_bytes < 0 || p->hbm_budget_bytes < 0)) return fail(RT_ERR_INVALID_ARG, "negative budget");
+    return guard([&]() {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw std::runtime_error("HIP error: no GPU device available");
+        rt::PoolSettings s;
+        if (p) {
+            s.queue_budget_bytes = (size_t)p->queue_budget_bytes;
+            s.hbm_budget_bytes = (size_t)p->hbm_budget_bytes;
+            s.profile = p->profile;
+        }
+        std::unique_ptr<rt_pool> pool(new rt_pool);
+        pool->p.reset(new rt::Pool(s));
+        *out = pool.release();
+        return RT_OK;
+    });
+}
+
+void rt_pool_destroy(rt_pool* p) { delete p; }
+
+int rt_pool_add_scene(rt_pool* p, rt_scene* s, int* slot_out) {
+    if (!p || !s || !slot_out) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return guard([&]() {
+        ensure_gpu(s);
+        *slot_out = p->p->add_scene(s->host, *s->gpu);
+        return RT_OK;
+    });
+}
+
+int rt_pool_submit(rt_pool* p, const rt_job_desc* d, int* job_out) {
+    if (!p || !d || !job_out) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return guard([&]() {
+        int id = p->p->submit(job_from(p, d));
+        if (id < 0) return fail(RT_ERR_BUSY, "the job's state does not fit the pool's HBM budget now");
+        *job_out = id;
+        return RT_OK;
+    });
+}
+
+int rt_pool_step(rt_pool* p, int* served_out) {
+    if (!p) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return guard([&]() {
+        int n = p->p->step();
+        if (served_out) *served_out = n;
+        return RT_OK;
+    });
+}
+
+int rt_pool_job_status(rt_pool* p, int job, rt_job_status* out) {
+    if (!p || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return guard([&]() {
+        const rt::JobStatus& st = p->p->status(job);
+        out->done = st.done;
+        out->spp_done = st.spp_done;
+        out->frames_done = st.frames_done;
+        out->submit_s = st.submit_s;
+        out->start_s = st.start_s;
+        out->done_s = st.done_s;
+        return RT_OK;
+    });
+}
+
+int rt_pool_read_accum(rt_pool* p, int job, int which, float* rgba) {
+    if (!p || !rgba || which < 0 || which > 2) return fail(RT_ERR_INVALID_ARG, "invalid argument");
+    return guard([&]() {
+        p->p->read_accum(job, which, rgba);
+        return RT_OK;
+    });
+}
+
+int rt_pool_read_frame(rt_pool* p, int job, int which, void* out, size_t bytes) {
+    if (!p || !out || which < 0 || which >= rt::kNumFrameBuffers) return fail(RT_ERR_INVALID_ARG, "invalid argument");
+    return guard([&]() {
+        int w = 0, h = 0;
+        p->p->job_size(job, w, h);
+        size_t need = rt::frame_buffer_bytes(which, w, h);
+        if (bytes != need)
+            throw std::invalid_argument("buffer size " + std::to_string(bytes) + " != " + std::to_string(need));
+        p->p->read_frame(job, which, out);
+        return RT_OK;
+    });
+}
+
+int rt_pool_release(rt_pool* p, int job) {
+    if (!p) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return guard([&]() {
+        p->p->release(job);
+        return RT_OK;
+    });
+}
+
+int rt_pool_stats(rt_pool* p, rt_stats_out* o) {
+    if (!p || !o) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return guard([&]() {
+        rt::RenderStats s;
+        p->p->stats(s);
+        std::memset(o, 0, sizeof(*o));
+        o->extension_rays = s.ext_rays;
+        o->shadow_rays = s.shadow_rays;
+        o->extension_node_visits = s.ext_nodes;
+        o->extension_tri_tests = s.ext_tris;
+        o->shadow_node_visits = s.shadow_nodes;
+        o->shadow_tri_tests = s.shadow_tris;
+        o->probe_rays = s.probe_rays;
+        for (int k = 0; k < RT_NUM_STAGES; ++k) {
+            o->stage_ms[k] = s.stage_ms[k];
+            o->stage_launches[k] = s.stage_launches[k];
+        }
+        return RT_OK;
+    });
+}
+
+int rt_pool_reset_stats(rt_pool* p) {
+    if (!p) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return guard([&]() {
+        p->p->reset_stats();
+        return RT_OK;
+    });
+}
+
+int rt_pool_serve(rt_pool* p, const rt_serve_request* reqs, int n, int batched, rt_serve_report* out,
+                  double* frame_latency_ms, double* job_latency_ms) {
+    if (!p || (!reqs && n > 0) || n < 0 || !out) return fail(RT_ERR_INVALID_ARG, "invalid argument");
+    return guard([&]() {
+        std::vector<rt::ServeRequest> rq((size_t)n);
+        for (int k = 0; k < n; ++k) {
+            if (!(reqs[k].arrival_s >= 0.0)) throw std::invalid_argument("arrival times must be >= 0");
+            rq[k].arrival_s = reqs[k].arrival_s;
+            rq[k].job = job_from(p, &reqs[k].job);
+        }
+        rt::ServeReport rep = rt::serve(*p->p, rq, batched != 0);
+        std::vector<double> frames;
+        for (const auto& f : rep.frame_ms) frames.insert(frames.end(), f.begin(), f.end());
+        std::memset(out, 0, sizeof(*out));
+        out->jobs = rep.jobs;
+        out->frames = rep.frames;
+        out->makespan_s = rep.makespan_s;
+        out->jobs_per_s = rep.makespan_s > 0 ? rep.jobs / rep.makespan_s : 0.0;
+        out->frame_p50_ms = percentile(frames, 0.50);
+        out->frame_p99_ms = percentile(frames, 0.99);
+        out->job_p50_ms = percentile(rep.job_ms, 0.50);
+        out->job_p99_ms = percentile(rep.job_ms, 0.99);
+        if (frame_latency_ms) std::copy(frames.begin(), frames.end(), frame_latency_ms);
+        if (job_latency_ms) std::copy(rep.job_ms.begin(), rep.job_ms.end(), job_latency_ms);
         return RT_OK;
     });
 }

@@ -66,6 +66,35 @@ struct FrameParams {
     int bounce;           // current bounce (set per launch)
 };
 
+// A render pass carries paths of one or more jobs.  The job's index in the
+// pass table rides in the path flags word above the flag bits and in the
+// shadow record, so every stage finds its scene and frame constants without a
+// search; one job's paths are contiguous in the pass (path index p,
+// local index p - path_base).
+constexpr int kJobShift = 8;
+constexpr int kMaxPassJobs = 256;
+
+struct JobDev {
+    DevScene sc;
+    FrameParams fp;        // fp.n_paths = paths of this job in the pass, fp.sample0 its first sample
+    int path_base;         // first pass path index of the job
+    int pix_base;          // first pass pixel index of the job (resolve)
+    int flush_survivors;
+    int pad;
+    float4* acc_d;         // the job's accumulators, W*H each
+    float4* acc_i;
+};
+
+struct PassHeader {
+    int n_jobs;
+    int n_paths;
+    int n_pixels;
+    int pad;
+};
+// Device pass table: header, then the jobs, 256-byte aligned.
+constexpr size_t kPassHeaderBytes = 256;
+constexpr size_t kPassTableBytes = kPassHeaderBytes + sizeof(JobDev) * kMaxPassJobs;
+
 // Work-queue counters.  Every queue is split into kSlices slices; each slice
 // has its own counter of every kind on its own 256-byte line, because one hot
 // atomic address serialises the whole chip (about 11 ns per atomic on MI355X).

@@ -1,0 +1,153 @@
+// Wavefront engine internals shared by the single-scene renderer and the
+// multi-job pool: device buffers, stage timing, render passes over one or
+// many jobs, and the per-view G-buffer / SVGF state.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "render.h"
+
+namespace rt {
+
+#define HIP_OK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess)                                                                       \
+            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " + \
+                                     #x);                                                           \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    void alloc(size_t b) {
+        release();
+        if (b == 0) return;
+        HIP_OK(hipMalloc(&p, b));
+        bytes = b;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T* as() const { return reinterpret_cast<T*>(p); }
+    ~DevBuf() { release(); }
+};
+
+// Device bytes per path of an engine's queues: two ping-pong SoA queues
+// (3 x float4 each), the hit record, one class-list entry per class, the
+// shadow record and the per-sample direct/indirect sums.
+constexpr size_t kQueueBytesPerPath = 2 * 3 * 16 + 16 + 4 * kNumClasses + 64 + 32;
+// Device bytes per pixel of a FrameState: two G-buffers (48 B each), two
+// histories (2 x float4), moments (2 x float2) and lengths (int) per frame
+// parity, the variance output, the a-trous ping-pong pair and the denoised colour.
+constexpr size_t kFrameStateBytesPerPixel = 2 * 48 + 2 * 32 + 2 * 16 + 2 * 4 + 32 + 2 * 32 + 16;
+
+// Per-stage GPU time from HIP events recorded on the work's own stream.
+struct Profiler {
+    bool on = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+    double ms[kNumStages] = {0};
+    int launches[kNumStages] = {0};
+    hipEvent_t begin(hipStream_t s);
+    void end(int stage, hipEvent_t b, hipStream_t s);
+    void collect();  // after the stream has been synchronised
+    void reset();
+    ~Profiler();
+};
+
+// One job's share of a render pass: `spp` samples from `fp.sample0` of every
+// pixel, summed in sample order into the job's accumulators.
+struct PassItem {
+    const GpuScene* scene = nullptr;
+    FrameParams fp{};
+    int spp = 0;
+    int flush_survivors = 0;
+    float4* acc_d = nullptr;
+    float4* acc_i = nullptr;
+};
+
+// Path queues, counters and the pass driver.  A pass holds at most
+// capacity() paths and kMaxPassJobs jobs.
+class Engine {
+public:
+    Engine(size_t capacity_paths, bool profile);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+    size_t capacity() const { return cap_; }
+    hipStream_t stream() const { return stream_; }
+    // Enqueue one pass on the engine's stream (asynchronous).
+    void run(const std::vector<PassItem>& items, bool count);
+    void read_stats(unsigned long long out[kNumStats]);
+    void reset_stats();
+    Profiler prof;
+
+private:
+    template <int STACK, bool POOLED>
+    void run_t(int n_items, int max_bounces, const bool* classes, int table, bool count);
+    size_t cap_ = 0;
+    int region_ = 0;
+    hipStream_t stream_ = nullptr;
+    DevBuf qo_[2], qd_[2], qw_[2], hits_, lists_, shadow_, slot_d_, slot_i_, counters_, stats_;
+    // Pass tables: a ring of device tables filled from pinned host copies.
+    static constexpr int kRing = 8;
+    DevBuf tables_;
+    void* host_tables_ = nullptr;
+    hipEvent_t ring_ev_[kRing] = {};
+    int ring_next_ = 0;
+    // Persistent grids per [pooled][64-entry stack] kernel variant.
+    int grid_ext_[2][2] = {}, grid_shadow_[2][2] = {}, grid_shade_[2][2][kNumClasses] = {};
+    int grid_linear_ = 0;
+    template <bool POOLED, int STACK>
+    void init_grids(int cus);
+};
+
+// G-buffers and SVGF state of one width x height view (reference: the
+// G-buffer pair renderer.cuh:1356-1503 and the denoise buffers 1309-1355).
+class FrameState {
+public:
+    FrameState(int width, int height);
+    // Start a frame seen from `cam`: G-buffer against the previous frame's
+    // camera, and cleared accumulators for the frame's samples.
+    void begin(const GpuScene& gs, const HostScene& host, const CameraRec& cam, float4* acc_d, float4* acc_i,
+               Profiler& prof, hipStream_t s);
+    // Finish it: temporal blend of the frame's `spp` samples, variance, a-trous, combine.
+    void end(int spp, const SvgfParams& p, const float4* acc_d, const float4* acc_i, Profiler& prof,
+             hipStream_t s);
+    void reset_history() { frames_ = 0; }
+    int frames() const { return frames_; }
+    // Sample index of the next frame's first sample.
+    uint32_t next_sample0(int spp) const { return (uint32_t)frames_ * (uint32_t)spp; }
+    void read(int which, void* out, hipStream_t s);
+    size_t device_bytes() const;
+
+private:
+    int w_, h_, wh_;
+    DevBuf g_pos_[2], g_nrm_[2], g_bary_[2], g_md_[2];
+    DevBuf h_d_[2], h_i_[2], m_d_[2], m_i_[2], h_len_[2];
+    DevBuf v_d_, v_i_, f_d_[2], f_i_[2], denoised_;
+    CameraRec last_cam_;
+    int fcur_ = 0, flast_ = -1, frames_ = 0, out_sel_ = -1;
+    bool started_ = false;
+};
+
+// Read accumulators (which: 0 direct sums, 1 indirect sums, 2 (direct + indirect) / spp).
+void read_accum_buffers(const float4* acc_d, const float4* acc_i, int wh, int spp_done, int which, float* out,
+                        hipStream_t s);
+
+// FrameParams of a camera for a job's pass.
+FrameParams frame_params(const CameraRec& cam, const RenderSettings& set, uint32_t sample0, int spp);
+
+}  // namespace rt

@@ -1,0 +1,368 @@
+// Multi-job scheduling over one wavefront engine, and the arrival-driven
+// serving loop (continuous batching vs. the plain one-job-at-a-time loop).
+//
+// The reference renders one scene per process in an interactive loop
+// (Src/main.cu:26-33, Src/Core/renderer.cuh:1106-1307); a pool keeps many
+// scenes resident and carries the next work of many jobs in every pass.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <thread>
+
+#include "engine.h"
+
+namespace rt {
+
+namespace {
+using Clock = std::chrono::steady_clock;
+}
+
+struct Pool::Impl {
+    PoolSettings set;
+    std::unique_ptr<Engine> eng;
+    struct SceneSlot {
+        const HostScene* host;
+        const GpuScene* gpu;
+    };
+    std::vector<SceneSlot> scenes;
+    struct Job {
+        int id = 0;
+        JobDesc d;
+        JobStatus st;
+        int wh = 0;
+        DevBuf acc_d, acc_i;
+        std::unique_ptr<FrameState> fs;
+        size_t bytes = 0;
+    };
+    std::map<int, std::unique_ptr<Job>> jobs;
+    std::deque<int> active;  // submission order
+    int next_id = 0;
+    size_t scene_bytes = 0, job_bytes = 0, hbm_budget = 0;
+    Clock::time_point t0;
+
+    Job& job(int id) const {
+        auto it = jobs.find(id);
+        if (it == jobs.end()) throw std::invalid_argument("unknown job id");
+        return *it->second;
+    }
+};
+
+Pool::Pool(const PoolSettings& s) : impl_(new Impl) {
+    Impl& I = *impl_;
+    I.set = s;
+    size_t qb = s.queue_budget_bytes ? s.queue_budget_bytes : ((size_t)8 << 30);
+    I.eng.reset(new Engine(std::max<size_t>(1, qb / kQueueBytesPerPath), s.profile != 0));
+    I.hbm_budget = s.hbm_budget_bytes ? s.hbm_budget_bytes : ((size_t)64 << 30);
+    I.t0 = Clock::now();
+}
+
+Pool::~Pool() {
+    if (impl_) {
+        if (impl_->eng) (void)hipStreamSynchronize(impl_->eng->stream());
+        delete impl_;
+    }
+}
+
+double Pool::now() const { return std::chrono::duration<double>(Clock::now() - impl_->t0).count(); }
+size_t Pool::capacity() const { return impl_->eng->capacity(); }
+size_t Pool::bytes_in_use() const { return impl_->scene_bytes + impl_->job_bytes; }
+bool Pool::idle() const { return impl_->active.empty(); }
+
+int Pool::add_scene(const HostScene& host, const GpuScene& gpu) {
+    Impl& I = *impl_;
+    I.scenes.push_back({&host, &gpu});
+    I.scene_bytes += gpu.device_bytes();
+    return (int)I.scenes.size() - 1;
+}
+
+const HostScene& Pool::scene_host(int slot) const {
+    if (slot < 0 || slot >= (int)impl_->scenes.size()) throw std::invalid_argument("unknown scene slot");
+    return *impl_->scenes[slot].host;
+}
+
+const GpuScene& Pool::scene_gpu(int slot) const {
+    if (slot < 0 || slot >= (int)impl_->scenes.size()) throw std::invalid_argument("unknown scene slot");
+    return *impl_->scenes[slot].gpu;
+}
+
+size_t Pool::check(const JobDesc& d) const {
+    const Impl& I = *impl_;
+    if (d.scene < 0 || d.scene >= (int)I.scenes.size()) throw std::invalid_argument("unknown scene slot");
+    if (d.spp < 1) throw std::invalid_argument("spp must be positive");
+    if (d.frames < 0) throw std::invalid_argument("frames must be >= 0");
+    if (d.frames > 0 && (int)d.cams.size() != d.frames) throw std::invalid_argument("one camera per frame is required");
+    if (d.frames == 0 && d.cams.size() > 1) throw std::invalid_argument("an accumulate job takes at most one camera");
+    if (d.set.max_bounces < 0 || d.set.max_bounces > 255) throw std::invalid_argument("max_bounces out of range");
+    const HostScene& h = *I.scenes[d.scene].host;
+    const CameraRec& c0 = d.cams.empty() ? h.camera : d.cams[0];
+    for (const auto& c : d.cams)
+        if (c.width != c0.width || c.height != c0.height) throw std::invalid_argument("cameras of a job differ in size");
+    const size_t wh = (size_t)c0.width * c0.height;
+    if (wh * (size_t)(d.frames > 0 ? d.spp : 1) > I.eng->capacity())
+        throw std::invalid_argument("one frame of the job exceeds the pool's path capacity");
+    size_t bytes = wh * 32 + (d.frames > 0 ? wh * kFrameStateBytesPerPixel : 0);
+    if (I.scene_bytes + bytes > I.hbm_budget) throw std::invalid_argument("the job alone exceeds the HBM budget");
+    return bytes;
+}
+
+int Pool::submit(const JobDesc& d) {
+    Impl& I = *impl_;
+    const size_t bytes = check(d);
+    if (I.scene_bytes + I.job_bytes + bytes > I.hbm_budget) return -1;
+    const HostScene& h = *I.scenes[d.scene].host;
+    const CameraRec& c0 = d.cams.empty() ? h.camera : d.cams[0];
+    const size_t wh = (size_t)c0.width * c0.height;
+    std::unique_ptr<Impl::Job> j(new Impl::Job);
+    j->id = I.next_id++;
+    j->d = d;
+    if (j->d.cams.empty()) j->d.cams.push_back(h.camera);
+    j->wh = (int)wh;
+    j->bytes = bytes;
+    j->acc_d.alloc(wh * 16);
+    j->acc_i.alloc(wh * 16);
+    HIP_OK(hipMemsetAsync(j->acc_d.p, 0, wh * 16, I.eng->stream()));
+    HIP_OK(hipMemsetAsync(j->acc_i.p, 0, wh * 16, I.eng->stream()));
+    if (d.frames > 0) j->fs.reset(new FrameState(c0.width, c0.height));
+    j->st.submit_s = now();
+    I.job_bytes += bytes;
+    int id = j->id;
+    I.active.push_back(id);
+    I.jobs[id] = std::move(j);
+    return id;
+}
+
+int Pool::step() {
+    Impl& I = *impl_;
+    if (I.active.empty()) return 0;
+    hipStream_t s = I.eng->stream();
+    const size_t cap = I.eng->capacity();
+    std::vector<PassItem> items;
+    std::vector<Impl::Job*> served;
+    std::vector<int> spp_of;
+    size_t used = 0;
+    for (int id : I.active) {
+        if ((int)items.size() == kMaxPassJobs) break;
+        Impl::Job& j = I.job(id);
+        const Impl::SceneSlot& sc = I.scenes[j.d.scene];
+        int k;
+        if (j.d.frames > 0) {
+            k = j.d.spp;
+            if (used + (size_t)j.wh * k > cap) continue;
+        } else {
+            int remaining = j.d.spp - j.st.spp_done;
+            size_t fit = (cap - used) / (size_t)j.wh;
+            k = (int)std::min<size_t>((size_t)remaining, fit);
+            if (k < 1) continue;
+        }
+        PassItem it;
+        it.scene = sc.gpu;
+        const CameraRec& cam = j.d.frames > 0 ? j.d.cams[j.st.frames_done] : j.d.cams[0];
+        uint32_t sample0 = j.d.frames > 0 ? j.fs->next_sample0(k) : (uint32_t)j.st.spp_done;
+        it.fp = frame_params(cam, j.d.set, sample0, k);
+        it.spp = k;
+        it.flush_survivors = j.d.set.flush_survivors;
+        it.acc_d = j.acc_d.as<float4>();
+        it.acc_i = j.acc_i.as<float4>();
+        items.push_back(it);
+        served.push_back(&j);
+        spp_of.push_back(k);
+        used += (size_t)j.wh * k;
+    }
+    if (items.empty()) throw std::runtime_error("no active job fits the pool's path capacity");
+    const double t_start = now();
+    for (Impl::Job* j : served) {
+        if (j->st.start_s < 0) j->st.start_s = t_start;
+        if (j->d.frames > 0)
+            j->fs->begin(*I.scenes[j->d.scene].gpu, *I.scenes[j->d.scene].host, j->d.cams[j->st.frames_done],
+                         j->acc_d.as<float4>(), j->acc_i.as<float4>(), I.eng->prof, s);
+    }
+    I.eng->run(items, false);
+    for (size_t k = 0; k < served.size(); ++k) {
+        Impl::Job* j = served[k];
+        if (j->d.frames > 0)
+            j->fs->end(spp_of[k], j->d.svgf, j->acc_d.as<float4>(), j->acc_i.as<float4>(), I.eng->prof, s);
+    }
+    HIP_OK(hipStreamSynchronize(s));
+    const double t_done = now();
+    for (size_t k = 0; k < served.size(); ++k) {
+        Impl::Job* j = served[k];
+        if (j->d.frames > 0) {
+            j->st.frames_done++;
+            j->st.spp_done = spp_of[k];
+            j->st.frame_done_s.push_back(t_done);
+            if (j->st.frames_done == j->d.frames) j->st.done = 1;
+        } else {
+            j->st.spp_done += spp_of[k];
+            if (j->st.spp_done == j->d.spp) j->st.done = 1;
+        }
+        if (j->st.done) {
+            j->st.done_s = t_done;
+            I.active.erase(std::find(I.active.begin(), I.active.end(), j->id));
+        }
+    }
+    return (int)served.size();
+}
+
+const JobStatus& Pool::status(int job) const { return impl_->job(job).st; }
+
+void Pool::job_size(int job, int& width, int& height) const {
+    const Impl::Job& j = impl_->job(job);
+    width = j.d.cams[0].width;
+    height = j.d.cams[0].height;
+}
+
+void Pool::read_accum(int job, int which, float* out) {
+    Impl::Job& j = impl_->job(job);
+    read_accum_buffers(j.acc_d.as<float4>(), j.acc_i.as<float4>(), j.wh, j.st.spp_done, which, out,
+                       impl_->eng->stream());
+}
+
+void Pool::read_frame(int job, int which, void* out) {
+    Impl::Job& j = impl_->job(job);
+    if (!j.fs) throw std::invalid_argument("not a real-time job");
+    j.fs->read(which, out, impl_->eng->stream());
+}
+
+void Pool::release(int job) {
+    Impl& I = *impl_;
+    Impl::Job& j = I.job(job);
+    HIP_OK(hipStreamSynchronize(I.eng->stream()));
+    auto it = std::find(I.active.begin(), I.active.end(), job);
+    if (it != I.active.end()) I.active.erase(it);
+    I.job_bytes -= j.bytes;
+    I.jobs.erase(job);
+}
+
+void Pool::stats(RenderStats& out) {
+    Impl& I = *impl_;
+    unsigned long long s[kNumStats];
+    I.eng->read_stats(s);
+    std::memset(&out, 0, sizeof(out));
+    out.ext_rays = s[kStatExtRays];
+    out.shadow_rays = s[kStatShadowRays];
+    out.ext_nodes = s[kStatExtNodes];
+    out.ext_tris = s[kStatExtTris];
+    out.shadow_nodes = s[kStatShadowNodes];
+    out.shadow_tris = s[kStatShadowTris];
+    out.probe_rays = s[kStatProbeRays];
+    for (int k = 0; k < kNumStages; ++k) {
+        out.stage_ms[k] = I.eng->prof.ms[k];
+        out.stage_launches[k] = I.eng->prof.launches[k];
+    }
+}
+
+void Pool::reset_stats() { impl_->eng->reset_stats(); }
+
+// ------------------------------------------------------------------ serving
+namespace {
+
+void wait_until(const Pool& pool, double t) {
+    double dt = t - pool.now();
+    if (dt > 0) std::this_thread::sleep_for(std::chrono::duration<double>(dt));
+}
+
+// done: completion times of the request's frames (one for an accumulate job),
+// on the same clock as its arrival.
+void record(ServeReport& rep, size_t req, double arrival, const std::vector<double>& done) {
+    double prev = arrival;
+    std::vector<double>& f = rep.frame_ms[req];
+    for (double t : done) {
+        f.push_back((t - prev) * 1e3);
+        prev = t;
+    }
+    rep.job_ms[req] = (done.back() - arrival) * 1e3;
+    rep.frames += (int)done.size();
+    rep.jobs += 1;
+}
+
+}  // namespace
+
+ServeReport serve(Pool& pool, const std::vector<ServeRequest>& reqs, bool batched) {
+    ServeReport rep;
+    if (reqs.empty()) return rep;
+    rep.frame_ms.resize(reqs.size());
+    rep.job_ms.assign(reqs.size(), 0.0);
+    for (const auto& rq : reqs) pool.check(rq.job);
+    std::vector<size_t> order(reqs.size());
+    for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
+    const double base = pool.now();
+    double last_done = 0;
+    if (batched) {
+        std::deque<size_t> waiting;  // arrived, not yet admitted
+        std::map<int, size_t> running;
+        size_t next = 0;
+        while (next < order.size() || !waiting.empty() || !running.empty()) {
+            const double t = pool.now() - base;
+            while (next < order.size() && reqs[order[next]].arrival_s <= t) waiting.push_back(order[next++]);
+            while (!waiting.empty()) {
+                int id = pool.submit(reqs[waiting.front()].job);
+                if (id < 0) break;  // over the HBM budget: admitted when a running job finishes
+                running[id] = waiting.front();
+                waiting.pop_front();
+            }
+            if (running.empty()) {
+                if (!waiting.empty()) throw std::runtime_error("a request never fits the HBM budget");
+                wait_until(pool, base + reqs[order[next]].arrival_s);
+                continue;
+            }
+            pool.step();
+            for (auto it = running.begin(); it != running.end();) {
+                const JobStatus& st = pool.status(it->first);
+                if (!st.done) {
+                    ++it;
+                    continue;
+                }
+                const ServeRequest& rq = reqs[it->second];
+                std::vector<double> done;
+                if (rq.job.frames > 0) {
+                    for (double x : st.frame_done_s) done.push_back(x - base);
+                } else {
+                    done.push_back(st.done_s - base);
+                }
+                record(rep, it->second, rq.arrival_s, done);
+                last_done = std::max(last_done, done.back());
+                pool.release(it->first);
+                it = running.erase(it);
+            }
+        }
+    } else {
+        // Plain: one job at a time in arrival order, each with a fresh
+        // single-scene renderer whose queues are sized for its own samples.
+        for (size_t k : order) {
+            const ServeRequest& rq = reqs[k];
+            wait_until(pool, base + rq.arrival_s);
+            const JobDesc& d = rq.job;
+            RenderSettings set = d.set;
+            set.batch_spp = d.frames > 0 ? d.spp : std::min(d.spp, 64);
+            std::vector<double> done;
+            {
+                Renderer r(pool.scene_gpu(d.scene), pool.scene_host(d.scene), set);
+                if (d.frames > 0) {
+                    for (int f = 0; f < d.frames; ++f) {
+                        r.set_camera(d.cams[f]);
+                        r.render_frame(d.spp, d.svgf);
+                        r.synchronize();
+                        done.push_back(pool.now() - base);
+                    }
+                } else {
+                    if (!d.cams.empty()) r.set_camera(d.cams[0]);
+                    r.render(d.spp);
+                    r.synchronize();
+                    done.push_back(pool.now() - base);
+                }
+            }
+            record(rep, k, rq.arrival_s, done);
+            last_done = std::max(last_done, done.back());
+        }
+    }
+    rep.makespan_s = last_done - reqs[order.front()].arrival_s;
+    return rep;
+}
+
+}  // namespace rt

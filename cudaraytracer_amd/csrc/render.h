@@ -76,6 +76,7 @@ public:
     const DevScene& dev() const;
     int stack() const;
     bool has_class(int shading_class) const;
+    size_t device_bytes() const;
     struct Impl;
 
 private:
@@ -120,6 +121,86 @@ public:
 private:
     Impl* impl_;
 };
+
+// ------------------------------------------------------------------ pool
+// Many render jobs over resident scenes, served by one wavefront engine: every
+// pass carries the next work of as many jobs as its path capacity holds (all
+// remaining samples of accumulate jobs that fit, one frame of each real-time
+// job), so uneven jobs share each persistent stage launch.
+struct PoolSettings {
+    size_t queue_budget_bytes = 0;  // path queues; 0: 8 GiB
+    size_t hbm_budget_bytes = 0;    // resident scenes + job state; 0: 64 GiB
+    int profile = 0;
+};
+
+struct JobDesc {
+    int scene = 0;                  // slot from add_scene
+    RenderSettings set;
+    int spp = 1;                    // accumulate: total samples; real-time: samples per frame
+    int frames = 0;                 // 0: accumulate job; > 0: real-time job with SVGF
+    SvgfParams svgf;
+    std::vector<CameraRec> cams;    // real-time: one per frame; accumulate: at most one
+};
+
+struct JobStatus {
+    int done = 0;
+    int spp_done = 0;
+    int frames_done = 0;
+    double submit_s = 0, start_s = -1, done_s = -1;   // seconds on the pool's clock
+    std::vector<double> frame_done_s;
+};
+
+class Pool {
+public:
+    explicit Pool(const PoolSettings& s);
+    ~Pool();
+    Pool(const Pool&) = delete;
+    Pool& operator=(const Pool&) = delete;
+    // The scene objects must outlive the pool.
+    int add_scene(const HostScene& host, const GpuScene& gpu);
+    // Validate a job; returns the device bytes of its state.
+    size_t check(const JobDesc& d) const;
+    // Job id, or -1 when its state does not fit the HBM budget now.
+    int submit(const JobDesc& d);
+    const HostScene& scene_host(int slot) const;
+    const GpuScene& scene_gpu(int slot) const;
+    // One pass over the active jobs (synchronous); returns the number of jobs served.
+    int step();
+    bool idle() const;
+    const JobStatus& status(int job) const;
+    void job_size(int job, int& width, int& height) const;
+    void read_accum(int job, int which, float* out);
+    void read_frame(int job, int which, void* out);
+    void release(int job);
+    double now() const;  // seconds since the pool was created
+    size_t capacity() const;
+    size_t bytes_in_use() const;
+    void stats(RenderStats& out);
+    void reset_stats();
+    struct Impl;
+
+private:
+    Impl* impl_;
+};
+
+// Arrival-driven serving on a pool's scenes: each request is submitted at its
+// arrival time (or later, when its state fits the HBM budget).  `batched`
+// runs the pool's continuous batching; otherwise the plain loop renders the
+// requests one at a time, in arrival order, each with its own fresh
+// single-scene Renderer.  Frame latency = completion - (arrival for the first
+// frame, completion of the previous frame otherwise); job latency =
+// completion of the last frame - arrival.
+struct ServeRequest {
+    double arrival_s = 0;
+    JobDesc job;
+};
+struct ServeReport {
+    int jobs = 0, frames = 0;
+    double makespan_s = 0;                        // first arrival to last completion
+    std::vector<std::vector<double>> frame_ms;    // per request, per frame
+    std::vector<double> job_ms;                   // per request
+};
+ServeReport serve(Pool& pool, const std::vector<ServeRequest>& reqs, bool batched);
 
 // Closest hit for a batch of rays laid out as (o.xyz, tmin, d.xyz, tmax):
 // out_hits = (t, b1, b2, prim bits) per ray, out_counts = (nodes, tris) per ray.

@@ -15,19 +15,13 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
+#include "engine.h"
 #include "integrator.h"
 #include "traverse.h"
 
 namespace rt {
-
-#define HIP_OK(x)                                                                                   \
-    do {                                                                                            \
-        hipError_t e_ = (x);                                                                        \
-        if (e_ != hipSuccess)                                                                       \
-            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " + \
-                                     #x);                                                           \
-    } while (0)
 
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
@@ -96,18 +90,52 @@ struct SliceCursor {
     }
 };
 
+// ------------------------------------------------------------------ jobs
+// POOLED kernels serve passes of many jobs: the job index is read from the
+// path's flags word / shadow tag and selects the scene and frame constants.
+// Single-job passes instantiate POOLED = false, where the job is 0 and every
+// job field is a uniform (scalar) load.
+template <bool POOLED>
+__device__ __forceinline__ const JobDev& pass_job(const JobDev* __restrict__ jobs, int job) {
+    return jobs[POOLED ? job : 0];
+}
+constexpr uint32_t kFlagMask = (1u << kJobShift) - 1u;
+
+// Job owning pass path p (paths of a job are contiguous, bases ascending).
+__device__ __forceinline__ int job_of_path(const JobDev* __restrict__ jobs, int n_jobs, int p) {
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].path_base <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+__device__ __forceinline__ int job_of_pixel(const JobDev* __restrict__ jobs, int n_jobs, int g) {
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].pix_base <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // ------------------------------------------------------------------ raygen
 // Jittered pinhole ray per (pixel, sample) (reference: Src/Core/camera.cu:27-35,
-// Src/Core/renderer.cuh:96-115).  Path p covers pixel p % wh of sample
-// sample0 + p / wh; row 0 of the image is the bottom row.  Path p sits at
-// queue index p, in slice p / region.
-__global__ __launch_bounds__(kBlock) void k_raygen(FrameParams fp, float4* qo, float4* qd, float4* qw,
-                                                    float4* slot_d, float4* slot_i, int* counters, int qslot,
-                                                    int region) {
+// Src/Core/renderer.cuh:96-115).  Local path l of a job covers pixel l % wh of
+// sample sample0 + l / wh; row 0 of the image is the bottom row.  Pass path p
+// sits at queue index p, in slice p / region.
+template <bool POOLED>
+__global__ __launch_bounds__(kBlock) void k_raygen(const PassHeader* __restrict__ hdr, const JobDev* __restrict__ jobs,
+                                                    float4* qo, float4* qd, float4* qw, float4* slot_d, float4* slot_i,
+                                                    int* counters, int qslot, int region) {
+    const int n_paths = hdr->n_paths;
+    const int n_jobs = POOLED ? hdr->n_jobs : 1;
     int stride = gridDim.x * blockDim.x;
     if (blockIdx.x == 0 && threadIdx.x < kSlices) {
         int sl = threadIdx.x;
-        int n = fp.n_paths - sl * region;
+        int n = n_paths - sl * region;
         *ctr_at(counters, qslot, sl) = n < 0 ? 0 : (n > region ? region : n);
         *ctr_at(counters, kFetchExtend, sl) = 0;
         for (int c = 0; c < kNumClasses; ++c) {
@@ -115,18 +143,23 @@ __global__ __launch_bounds__(kBlock) void k_raygen(FrameParams fp, float4* qo, f
             *ctr_at(counters, kFetchClass0 + c, sl) = 0;
         }
     }
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < fp.n_paths; p += stride) {
-        uint32_t pixel = (uint32_t)(p % fp.wh);
-        uint32_t sample = fp.sample0 + (uint32_t)(p / fp.wh);
-        int i = (int)(pixel % (uint32_t)fp.width), j = (int)(pixel / (uint32_t)fp.width);
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_paths; p += stride) {
+        const int job = POOLED ? job_of_path(jobs, n_jobs, p) : 0;
+        const JobDev& J = pass_job<POOLED>(jobs, job);
+        const int wh = J.fp.wh, width = J.fp.width;
+        const int local = p - J.path_base;
+        uint32_t pixel = (uint32_t)(local % wh);
+        uint32_t sample = J.fp.sample0 + (uint32_t)(local / wh);
+        int i = (int)(pixel % (uint32_t)width), j = (int)(pixel / (uint32_t)width);
         Rng rng;
-        rng.init(fp.seed, pixel, sample, 0u, kStreamCamera);
+        rng.init(J.fp.seed, pixel, sample, 0u, kStreamCamera);
         float x0 = rng.next();
         float x1 = rng.next();
-        vec3 pp = (fp.left_down + ((float)i + x0) * fp.du) + ((float)j + x1) * fp.dv;
-        vec3 d = normalize(pp - fp.cam_pos);
-        qo[p] = make_float4(fp.cam_pos.x, fp.cam_pos.y, fp.cam_pos.z, 0.f);
-        qd[p] = make_float4(d.x, d.y, d.z, i_as_f((int)kFlagFirst));
+        const vec3 cam = J.fp.cam_pos;
+        vec3 pp = (J.fp.left_down + ((float)i + x0) * J.fp.du) + ((float)j + x1) * J.fp.dv;
+        vec3 d = normalize(pp - cam);
+        qo[p] = make_float4(cam.x, cam.y, cam.z, 0.f);
+        qd[p] = make_float4(d.x, d.y, d.z, i_as_f((int)(kFlagFirst | ((uint32_t)job << kJobShift))));
         qw[p] = make_float4(1.f, 1.f, 1.f, i_as_f(p));
         slot_d[p] = make_float4(0.f, 0.f, 0.f, 0.f);
         slot_i[p] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -136,8 +169,8 @@ __global__ __launch_bounds__(kBlock) void k_raygen(FrameParams fp, float4* qo, f
 // ------------------------------------------------------------------ extend
 // Closest hit for every path of the current queue, then the path's queue
 // index is appended to its slice's list for the shading class of what it hit.
-template <int STACK, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, const float4* __restrict__ qo,
+template <int STACK, bool COUNT, bool POOLED>
+__global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jobs, const float4* __restrict__ qo,
                                                     const float4* __restrict__ qd, float4* __restrict__ hits,
                                                     int* __restrict__ lists, int cap, int region, int* counters,
                                                     int qslot, unsigned long long* stats) {
@@ -161,6 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, const float4* __
         int cls = -1;
         if (j < cur.count) {
             float4 o = qo[i], d = qd[i];
+            const DevScene& sc = pass_job<POOLED>(jobs, (int)((uint32_t)f_as_i(d.w) >> kJobShift)).sc;
             RayInv r = make_ray(v3(o.x, o.y, o.z), v3(d.x, d.y, d.z));
             Hit h = trace_closest<STACK, COUNT>(sc, r, 0.f, 1e8f, col, kBlock, c);
             hits[i] = make_float4(h.t, h.b1, h.b2, i_as_f(h.prim));
@@ -181,15 +215,18 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, const float4* __
 
 // ------------------------------------------------------------------ shade
 // One kernel per shading class; each reads the queue indices binned by extend
-// and appends survivors / shadow records to the slice they came from.
-template <int CLS, int STACK>
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameParams fp, const float4* __restrict__ qo,
-                                                   const float4* __restrict__ qd, const float4* __restrict__ qw,
-                                                   const float4* __restrict__ hits, const int* __restrict__ list,
-                                                   float4* __restrict__ no, float4* __restrict__ nd,
-                                                   float4* __restrict__ nw, float4* __restrict__ shadow,
-                                                   float4* slot_d, float4* slot_i, int region, int* counters,
-                                                   int qslot) {
+// and appends survivors / shadow records to the slice they came from.  At a
+// job's last bounce its survivors are dropped together with everything they
+// gathered (reference: Src/Core/renderer.cuh:1166-1194 never accumulates a
+// path still alive after maxBounces), unless the job flushes survivors.
+template <int CLS, int STACK, bool POOLED>
+__global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ jobs, int bounce,
+                                                   const float4* __restrict__ qo, const float4* __restrict__ qd,
+                                                   const float4* __restrict__ qw, const float4* __restrict__ hits,
+                                                   const int* __restrict__ list, float4* __restrict__ no,
+                                                   float4* __restrict__ nd, float4* __restrict__ nw,
+                                                   float4* __restrict__ shadow, float4* slot_d, float4* slot_i,
+                                                   int region, int* counters, int qslot) {
     __shared__ uint32_t stk[CLS == kClsSubsurface ? STACK * kBlock : 1];
     if (blockIdx.x == 0 && threadIdx.x < kSlices) *ctr_at(counters, kFetchExtend, threadIdx.x) = 0;
     uint32_t* col = stk + (CLS == kClsSubsurface ? threadIdx.x : 0);
@@ -201,20 +238,28 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameParams fp, c
         ShadeOut so;
         so.alive = false;
         so.shadow = false;
+        so.add_direct = false;
         PathState ps;
         int pid = 0;
+        uint32_t jobbits = 0;
         if (j < cur.count) {
             int i = list[(size_t)cur.slice * region + j];
             float4 o = qo[i], d = qd[i], w = qw[i];
+            const uint32_t flags = (uint32_t)f_as_i(d.w);
+            jobbits = flags & ~kFlagMask;
+            const JobDev& J = pass_job<POOLED>(jobs, (int)(flags >> kJobShift));
+            FrameParams fp = J.fp;
+            fp.bounce = bounce;
             ps.o = v3(o.x, o.y, o.z);
             ps.pdf = o.w;
             ps.d = v3(d.x, d.y, d.z);
-            ps.flags = (uint32_t)f_as_i(d.w);
+            ps.flags = flags & kFlagMask;
             ps.w = v3(w.x, w.y, w.z);
             pid = f_as_i(w.w);
-            uint32_t pixel = (uint32_t)(pid % fp.wh);
-            uint32_t sample = fp.sample0 + (uint32_t)(pid / fp.wh);
-            so = shade_path<CLS, STACK>(sc, fp, ps, hits[i], pixel, sample, col, kBlock);
+            const int local = pid - J.path_base;
+            uint32_t pixel = (uint32_t)(local % fp.wh);
+            uint32_t sample = fp.sample0 + (uint32_t)(local / fp.wh);
+            so = shade_path<CLS, STACK>(J.sc, fp, ps, hits[i], pixel, sample, col, kBlock);
             if (so.has_emit) {
                 float4* s = so.add_direct ? slot_d : slot_i;
                 float4 a = s[pid];
@@ -223,23 +268,33 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameParams fp, c
                 a.z += so.emit.z;
                 s[pid] = a;
             }
+            if (CLS != kClsTerminal && so.alive && bounce + 1 >= fp.max_bounces) {
+                so.alive = false;
+                if (!J.flush_survivors) {
+                    slot_d[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    slot_i[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    so.shadow = false;
+                }
+            }
         }
         if (CLS != kClsTerminal) {
             int qpos = wave_append(so.alive, ctr_at(counters, qslot ^ 1, cur.slice));
             if (so.alive) {
                 size_t o = (size_t)cur.slice * region + qpos;
                 no[o] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.pdf);
-                nd[o] = make_float4(ps.d.x, ps.d.y, ps.d.z, i_as_f((int)ps.flags));
+                nd[o] = make_float4(ps.d.x, ps.d.y, ps.d.z, i_as_f((int)((ps.flags & kFlagMask) | jobbits)));
                 nw[o] = make_float4(ps.w.x, ps.w.y, ps.w.z, i_as_f(pid));
             }
         }
         if (CLS == kClsDiffuse || CLS == kClsPbr || CLS == kClsSubsurface) {
             int spos = wave_append(so.shadow, ctr_at(counters, kCntShadow, cur.slice));
             if (so.shadow) {
+                // tag = job << 1 | (sum: 0 direct, 1 indirect)
+                int tag = (int)((jobbits >> kJobShift) << 1) | (so.add_direct ? 0 : 1);
                 float4* r = shadow + 4 * ((size_t)cur.slice * region + spos);
                 r[0] = make_float4(so.rec.lp.x, so.rec.lp.y, so.rec.lp.z, so.rec.sdist);
                 r[1] = make_float4(so.rec.dir.x, so.rec.dir.y, so.rec.dir.z, i_as_f(pid));
-                r[2] = make_float4(so.rec.hp.x, so.rec.hp.y, so.rec.hp.z, i_as_f(so.add_direct ? 0 : 1));
+                r[2] = make_float4(so.rec.hp.x, so.rec.hp.y, so.rec.hp.z, i_as_f(tag));
                 r[3] = make_float4(so.rec.contrib.x, so.rec.contrib.y, so.rec.contrib.z, 0.f);
             }
         }
@@ -249,9 +304,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameParams fp, c
 // ------------------------------------------------------------------ shadow
 // Closest hit from the light point towards the shading point with tMin 1e-6,
 // then the reference's 0.1 distance tolerances (Src/Core/light.cu:5-27).
-template <int STACK, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, const float4* __restrict__ shadow, float4* slot_d,
-                                                    float4* slot_i, int region, int* counters,
+template <int STACK, bool COUNT, bool POOLED>
+__global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jobs, const float4* __restrict__ shadow,
+                                                    float4* slot_d, float4* slot_i, int region, int* counters,
                                                     unsigned long long* stats) {
     __shared__ uint32_t stk[STACK * kBlock];
     if (blockIdx.x == 0 && threadIdx.x < kSlices) {
@@ -274,6 +329,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, const float4* __
         if (j < cur.count) {
             const float4* r = shadow + 4 * ((size_t)cur.slice * region + j);
             float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+            const int tag = f_as_i(r2.w);
+            const DevScene& sc = pass_job<POOLED>(jobs, tag >> 1).sc;
             ShadowRec rec;
             rec.lp = v3(r0.x, r0.y, r0.z);
             rec.sdist = r0.w;
@@ -287,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, const float4* __
             if (hit) tp = position_at(sc.tri_attr, h.prim, h.b1, h.b2);
             if (shadow_visible(rec, hit, tp)) {
                 int pid = f_as_i(r1.w);
-                float4* s = f_as_i(r2.w) == 0 ? slot_d : slot_i;
+                float4* s = (tag & 1) == 0 ? slot_d : slot_i;
                 float4 a = s[pid];
                 a.x += r3.x;
                 a.y += r3.y;
@@ -302,35 +359,30 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, const float4* __
     }
 }
 
-// Paths still alive after the last bounce are dropped without contributing
-// (reference: Src/Core/renderer.cuh:1166-1194 never accumulates them).
-__global__ __launch_bounds__(kBlock) void k_lost(const float4* __restrict__ qw, float4* slot_d, float4* slot_i,
-                                                  int* counters, int qslot, int region) {
-    for (int sl = 0; sl < kSlices; ++sl) {
-        const int count = *ctr_at(counters, qslot, sl);
-        for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < count; j += gridDim.x * blockDim.x) {
-            int pid = f_as_i(qw[(size_t)sl * region + j].w);
-            slot_d[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
-            slot_i[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-    }
-}
-
 // Per-pixel accumulation in sample order, so the sum is the same as adding
-// one sample per frame (reference: Src/Core/renderer.cuh:117-122).
-__global__ __launch_bounds__(kBlock) void k_resolve(const float4* __restrict__ slot_d,
-                                                     const float4* __restrict__ slot_i, int wh, int k,
-                                                     float4* acc_d, float4* acc_i) {
-    for (int px = blockIdx.x * blockDim.x + threadIdx.x; px < wh; px += gridDim.x * blockDim.x) {
-        float4 a = acc_d[px], b = acc_i[px];
+// one sample per frame (reference: Src/Core/renderer.cuh:117-122).  Pass
+// pixel g is pixel g - pix_base of its job; the job's k samples sit at pass
+// paths path_base + s * wh + pixel.
+template <bool POOLED>
+__global__ __launch_bounds__(kBlock) void k_resolve(const PassHeader* __restrict__ hdr,
+                                                     const JobDev* __restrict__ jobs,
+                                                     const float4* __restrict__ slot_d,
+                                                     const float4* __restrict__ slot_i) {
+    const int n_pix = hdr->n_pixels;
+    const int n_jobs = POOLED ? hdr->n_jobs : 1;
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n_pix; g += gridDim.x * blockDim.x) {
+        const JobDev& J = pass_job<POOLED>(jobs, POOLED ? job_of_pixel(jobs, n_jobs, g) : 0);
+        const int wh = J.fp.wh, px = g - J.pix_base, k = J.fp.n_paths / wh;
+        float4 a = J.acc_d[px], b = J.acc_i[px];
+        const size_t base = (size_t)J.path_base + px;
         for (int s = 0; s < k; ++s) {
-            float4 x = slot_d[(size_t)s * wh + px];
-            float4 y = slot_i[(size_t)s * wh + px];
+            float4 x = slot_d[base + (size_t)s * wh];
+            float4 y = slot_i[base + (size_t)s * wh];
             a.x += x.x; a.y += x.y; a.z += x.z;
             b.x += y.x; b.y += y.y; b.z += y.z;
         }
-        acc_d[px] = a;
-        acc_i[px] = b;
+        J.acc_d[px] = a;
+        J.acc_i[px] = b;
     }
 }
 
@@ -439,25 +491,6 @@ __global__ __launch_bounds__(kBlock) void k_combine(const float4* __restrict__ d
 // ======================================================================
 // Host side
 // ======================================================================
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    void alloc(size_t b) {
-        release();
-        if (b == 0) return;
-        HIP_OK(hipMalloc(&p, b));
-        bytes = b;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <typename T>
-    T* as() const { return reinterpret_cast<T*>(p); }
-    ~DevBuf() { release(); }
-};
 
 struct GpuScene::Impl {
     DevBuf nodes, tri_isect, tri_attr, materials, lights, light_tris, env_rgb, env_row, env_marg;
@@ -575,6 +608,11 @@ GpuScene::~GpuScene() { delete impl_; }
 const DevScene& GpuScene::dev() const { return impl_->dev; }
 int GpuScene::stack() const { return impl_->stack; }
 bool GpuScene::has_class(int c) const { return impl_->has_class[c]; }
+size_t GpuScene::device_bytes() const {
+    const Impl& I = *impl_;
+    return I.nodes.bytes + I.tri_isect.bytes + I.tri_attr.bytes + I.materials.bytes + I.lights.bytes +
+           I.light_tris.bytes + I.env_rgb.bytes + I.env_row.bytes + I.env_marg.bytes;
+}
 
 // Row-conditional and marginal CDFs weighted by sin(theta) (reference:
 // Src/Core/texture.cu:38-80).
@@ -608,39 +646,55 @@ void build_env_tables(const EnvMap& env, EnvTables& t) {
     for (int i = 0; i < h; ++i) t.marg_cdf[i] = total > 0.0f ? t.marg_cdf[i] / total : 0.0f;
 }
 
-// ------------------------------------------------------------------ renderer
-struct Renderer::Impl {
-    const GpuScene* scene = nullptr;
-    const HostScene* host = nullptr;
-    RenderSettings set;
-    hipStream_t stream = nullptr;
-    int width = 0, height = 0, wh = 0;
-    int batch = 1;
-    size_t cap = 0;  // paths per pass
-    int region = 0;  // queue entries per slice
-    DevBuf qo[2], qd[2], qw[2], hits, lists, shadow, slot_d, slot_i, acc_d, acc_i, counters, stats;
-    int grid_persist_ext = 0, grid_persist_shadow = 0, grid_linear = 0;
-    int grid_shade[kNumClasses] = {0};
-    int spp_done = 0;
-    uint32_t sample_base = 0;  // sample index of the first sample in the accumulators
-    CameraRec cam;             // camera of the following passes
-    // Real-time frames: two G-buffers / histories (this frame, previous frame).
-    bool frame_alloc = false;
-    DevBuf g_pos[2], g_nrm[2], g_bary[2], g_md[2];
-    DevBuf h_d[2], h_i[2], m_d[2], m_i[2], h_len[2];
-    DevBuf v_d, v_i, f_d[2], f_i[2], denoised;
-    CameraRec last_frame_cam;
-    int fcur = 0;     // buffer set the next frame writes
-    int flast = -1;   // buffer set of the last frame (-1: none yet)
-    int frames = 0;   // frames since the last history reset
-    int out_sel = -1; // -1: the variance pass output is final, else f_*[out_sel]
-    // timing
-    bool profile = false;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
-    double stage_ms[kNumStages] = {0};
-    int stage_launches[kNumStages] = {0};
-};
+// ------------------------------------------------------------------ profiler
+hipEvent_t Profiler::begin(hipStream_t s) {
+    if (!on) return nullptr;
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    HIP_OK(hipEventRecord(e, s));
+    return e;
+}
 
+void Profiler::end(int stage, hipEvent_t b, hipStream_t s) {
+    if (!on) return;
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    HIP_OK(hipEventRecord(e, s));
+    ev.push_back({stage, {b, e}});
+}
+
+void Profiler::collect() {
+    for (auto& e : ev) {
+        float t = 0.f;
+        HIP_OK(hipEventElapsedTime(&t, e.second.first, e.second.second));
+        ms[e.first] += t;
+        launches[e.first] += 1;
+        (void)hipEventDestroy(e.second.first);
+        (void)hipEventDestroy(e.second.second);
+    }
+    ev.clear();
+}
+
+void Profiler::reset() {
+    for (auto& e : ev) {
+        (void)hipEventDestroy(e.second.first);
+        (void)hipEventDestroy(e.second.second);
+    }
+    ev.clear();
+    for (int s = 0; s < kNumStages; ++s) {
+        ms[s] = 0;
+        launches[s] = 0;
+    }
+}
+
+Profiler::~Profiler() {
+    for (auto& e : ev) {
+        (void)hipEventDestroy(e.second.first);
+        (void)hipEventDestroy(e.second.second);
+    }
+}
+
+// ------------------------------------------------------------------ engine
 template <typename K>
 static int persistent_grid(K kernel, int cus) {
     int per_cu = 0;
@@ -649,213 +703,213 @@ static int persistent_grid(K kernel, int cus) {
     return per_cu * cus;
 }
 
-Renderer::Renderer(const GpuScene& scene, const HostScene& host, const RenderSettings& s) : impl_(new Impl) {
-    Impl& I = *impl_;
-    I.scene = &scene;
-    I.host = &host;
-    I.set = s;
-    I.width = host.camera.width;
-    I.height = host.camera.height;
-    I.wh = I.width * I.height;
-    I.cam = host.camera;
-    I.profile = s.profile != 0;
-    HIP_OK(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
-    int dev = 0;
+Engine::Engine(size_t capacity_paths, bool profile) {
+    prof.on = profile;
+    if (capacity_paths < 1) capacity_paths = 1;
+    if (capacity_paths > (size_t)0x7fffffff) capacity_paths = 0x7fffffff;
+    cap_ = capacity_paths;
+    HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    int dev = 0, cus = 0;
     HIP_OK(hipGetDevice(&dev));
-    int cus = 0;
     HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    // Paths per pass: all requested samples of every pixel when they fit the
-    // queue budget (about 200 bytes of queue state per path).
-    size_t per_path = 3 * 16 * 2 + 16 + 4 * kNumClasses + 64 + 32;
-    size_t budget = s.queue_budget_bytes ? s.queue_budget_bytes : ((size_t)8 << 30);
-    int bmax = (int)std::max<size_t>(1, budget / (per_path * (size_t)I.wh));
-    int want = s.batch_spp > 0 ? s.batch_spp : 64;
-    I.batch = std::max(1, std::min(want, bmax));
-    if ((size_t)I.wh * I.batch > (size_t)0x7fffffff) I.batch = std::max(1, (int)(0x7fffffff / I.wh));
-    I.cap = (size_t)I.wh * I.batch;
     for (int k = 0; k < 2; ++k) {
-        I.qo[k].alloc(I.cap * 16);
-        I.qd[k].alloc(I.cap * 16);
-        I.qw[k].alloc(I.cap * 16);
+        qo_[k].alloc(cap_ * 16);
+        qd_[k].alloc(cap_ * 16);
+        qw_[k].alloc(cap_ * 16);
     }
-    I.hits.alloc(I.cap * 16);
-    I.lists.alloc(I.cap * 4 * kNumClasses);
-    I.shadow.alloc(I.cap * 64);
-    I.slot_d.alloc(I.cap * 16);
-    I.slot_i.alloc(I.cap * 16);
-    I.acc_d.alloc((size_t)I.wh * 16);
-    I.acc_i.alloc((size_t)I.wh * 16);
-    I.region = (int)((I.cap + kSlices - 1) / kSlices);
-    I.counters.alloc(kCounterInts * sizeof(int));
-    I.stats.alloc(kNumStats * sizeof(unsigned long long));
-    HIP_OK(hipMemset(I.counters.p, 0, kCounterInts * sizeof(int)));
-    HIP_OK(hipMemset(I.stats.p, 0, kNumStats * sizeof(unsigned long long)));
-    HIP_OK(hipMemset(I.acc_d.p, 0, (size_t)I.wh * 16));
-    HIP_OK(hipMemset(I.acc_i.p, 0, (size_t)I.wh * 16));
-    if (scene.stack() == 32) {
-        I.grid_persist_ext = persistent_grid(k_extend<32, false>, cus);
-        I.grid_persist_shadow = persistent_grid(k_shadow<32, false>, cus);
-        I.grid_shade[kClsSubsurface] = persistent_grid(k_shade<kClsSubsurface, 32>, cus);
-    } else {
-        I.grid_persist_ext = persistent_grid(k_extend<64, false>, cus);
-        I.grid_persist_shadow = persistent_grid(k_shadow<64, false>, cus);
-        I.grid_shade[kClsSubsurface] = persistent_grid(k_shade<kClsSubsurface, 64>, cus);
-    }
-    I.grid_shade[kClsTerminal] = persistent_grid(k_shade<kClsTerminal, 32>, cus);
-    I.grid_shade[kClsDiffuse] = persistent_grid(k_shade<kClsDiffuse, 32>, cus);
-    I.grid_shade[kClsPbr] = persistent_grid(k_shade<kClsPbr, 32>, cus);
-    I.grid_shade[kClsDielectric] = persistent_grid(k_shade<kClsDielectric, 32>, cus);
-    I.grid_linear = cus * 8;
+    hits_.alloc(cap_ * 16);
+    lists_.alloc(cap_ * 4 * kNumClasses);
+    shadow_.alloc(cap_ * 64);
+    slot_d_.alloc(cap_ * 16);
+    slot_i_.alloc(cap_ * 16);
+    region_ = (int)((cap_ + kSlices - 1) / kSlices);
+    counters_.alloc(kCounterInts * sizeof(int));
+    stats_.alloc(kNumStats * sizeof(unsigned long long));
+    tables_.alloc(kPassTableBytes * kRing);
+    HIP_OK(hipHostMalloc(&host_tables_, kPassTableBytes * kRing, hipHostMallocDefault));
+    for (int k = 0; k < kRing; ++k) HIP_OK(hipEventCreateWithFlags(&ring_ev_[k], hipEventDisableTiming));
+    HIP_OK(hipMemset(counters_.p, 0, kCounterInts * sizeof(int)));
+    HIP_OK(hipMemset(stats_.p, 0, kNumStats * sizeof(unsigned long long)));
+    init_grids<false, 32>(cus);
+    init_grids<false, 64>(cus);
+    init_grids<true, 32>(cus);
+    init_grids<true, 64>(cus);
+    grid_linear_ = cus * 8;
     HIP_OK(hipDeviceSynchronize());
 }
 
-Renderer::~Renderer() {
-    if (impl_) {
-        for (auto& e : impl_->ev) {
-            (void)hipEventDestroy(e.second.first);
-            (void)hipEventDestroy(e.second.second);
-        }
-        if (impl_->stream) (void)hipStreamDestroy(impl_->stream);
-        delete impl_;
+template <bool POOLED, int STACK>
+void Engine::init_grids(int cus) {
+    const int p = POOLED ? 1 : 0, s = STACK == 64 ? 1 : 0;
+    grid_ext_[p][s] = persistent_grid(k_extend<STACK, false, POOLED>, cus);
+    grid_shadow_[p][s] = persistent_grid(k_shadow<STACK, false, POOLED>, cus);
+    grid_shade_[p][s][kClsTerminal] = persistent_grid(k_shade<kClsTerminal, STACK, POOLED>, cus);
+    grid_shade_[p][s][kClsDiffuse] = persistent_grid(k_shade<kClsDiffuse, STACK, POOLED>, cus);
+    grid_shade_[p][s][kClsPbr] = persistent_grid(k_shade<kClsPbr, STACK, POOLED>, cus);
+    grid_shade_[p][s][kClsDielectric] = persistent_grid(k_shade<kClsDielectric, STACK, POOLED>, cus);
+    grid_shade_[p][s][kClsSubsurface] = persistent_grid(k_shade<kClsSubsurface, STACK, POOLED>, cus);
+}
+
+Engine::~Engine() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (int k = 0; k < kRing; ++k)
+        if (ring_ev_[k]) (void)hipEventDestroy(ring_ev_[k]);
+    if (host_tables_) (void)hipHostFree(host_tables_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::read_stats(unsigned long long out[kNumStats]) {
+    HIP_OK(hipStreamSynchronize(stream_));
+    HIP_OK(hipMemcpy(out, stats_.p, sizeof(unsigned long long) * kNumStats, hipMemcpyDeviceToHost));
+    prof.collect();
+}
+
+void Engine::reset_stats() {
+    HIP_OK(hipStreamSynchronize(stream_));
+    k_zero_u64<<<1, 64, 0, stream_>>>(stats_.as<unsigned long long>(), kNumStats);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(stream_));
+    prof.reset();
+}
+
+void Engine::run(const std::vector<PassItem>& items, bool count) {
+    const int n = (int)items.size();
+    if (n == 0) return;
+    if (n > kMaxPassJobs) throw std::invalid_argument("too many jobs in one pass");
+    // Fill the next ring slot: wait until the pass that used it last has consumed it.
+    const int slot = ring_next_;
+    ring_next_ = (ring_next_ + 1) % kRing;
+    HIP_OK(hipEventSynchronize(ring_ev_[slot]));
+    char* host = static_cast<char*>(host_tables_) + kPassTableBytes * slot;
+    PassHeader* hdr = reinterpret_cast<PassHeader*>(host);
+    JobDev* jobs = reinterpret_cast<JobDev*>(host + kPassHeaderBytes);
+    size_t paths = 0, pixels = 0;
+    int max_bounces = 0, stack = 32;
+    bool classes[kNumClasses] = {true, false, false, false, false};
+    for (int k = 0; k < n; ++k) {
+        const PassItem& it = items[k];
+        if (it.spp < 1) throw std::invalid_argument("pass item without samples");
+        JobDev& J = jobs[k];
+        J.sc = it.scene->dev();
+        J.fp = it.fp;
+        J.fp.n_paths = it.fp.wh * it.spp;
+        J.fp.bounce = 0;
+        J.path_base = (int)paths;
+        J.pix_base = (int)pixels;
+        J.flush_survivors = it.flush_survivors;
+        J.pad = 0;
+        J.acc_d = it.acc_d;
+        J.acc_i = it.acc_i;
+        paths += (size_t)J.fp.n_paths;
+        pixels += (size_t)it.fp.wh;
+        max_bounces = std::max(max_bounces, it.fp.max_bounces);
+        stack = std::max(stack, it.scene->stack());
+        for (int c = 1; c < kNumClasses; ++c) classes[c] = classes[c] || it.scene->has_class(c);
+    }
+    if (paths > cap_) throw std::invalid_argument("pass exceeds the engine's path capacity");
+    if (pixels > (size_t)0x7fffffff) throw std::invalid_argument("pass has too many pixels");
+    hdr->n_jobs = n;
+    hdr->n_paths = (int)paths;
+    hdr->n_pixels = (int)pixels;
+    hdr->pad = 0;
+    char* dev = tables_.as<char>() + kPassTableBytes * slot;
+    HIP_OK(hipMemcpyAsync(dev, host, kPassHeaderBytes + sizeof(JobDev) * n, hipMemcpyHostToDevice, stream_));
+    HIP_OK(hipEventRecord(ring_ev_[slot], stream_));
+    if (n == 1) {
+        if (stack == 32) run_t<32, false>(n, max_bounces, classes, slot, count);
+        else run_t<64, false>(n, max_bounces, classes, slot, count);
+    } else {
+        if (stack == 32) run_t<32, true>(n, max_bounces, classes, slot, count);
+        else run_t<64, true>(n, max_bounces, classes, slot, count);
     }
 }
 
-int Renderer::width() const { return impl_->width; }
-int Renderer::height() const { return impl_->height; }
-int Renderer::batch() const { return impl_->batch; }
-int Renderer::spp_done() const { return impl_->spp_done; }
-
-// Stage timing with HIP events on the renderer's stream (profiled renderers only).
-static hipEvent_t stage_begin(Renderer::Impl& I) {
-    if (!I.profile) return nullptr;
-    hipEvent_t e;
-    HIP_OK(hipEventCreate(&e));
-    HIP_OK(hipEventRecord(e, I.stream));
-    return e;
-}
-static void stage_end(Renderer::Impl& I, int stage, hipEvent_t b) {
-    if (!I.profile) return;
-    hipEvent_t e;
-    HIP_OK(hipEventCreate(&e));
-    HIP_OK(hipEventRecord(e, I.stream));
-    I.ev.push_back({stage, {b, e}});
-}
-
-template <int CLS, int STACK>
-static void launch_shade(Renderer::Impl& I, const DevScene& sc, const FrameParams& fp, int q) {
-    int* ctr = I.counters.as<int>();
-    k_shade<CLS, STACK><<<I.grid_shade[CLS], kBlock, 0, I.stream>>>(
-        sc, fp, I.qo[q].as<float4>(), I.qd[q].as<float4>(), I.qw[q].as<float4>(), I.hits.as<float4>(),
-        I.lists.as<int>() + (size_t)CLS * I.cap, I.qo[q ^ 1].as<float4>(), I.qd[q ^ 1].as<float4>(),
-        I.qw[q ^ 1].as<float4>(), I.shadow.as<float4>(), I.slot_d.as<float4>(), I.slot_i.as<float4>(), I.region, ctr,
-        q);
-    HIP_OK(hipGetLastError());
-}
-
-template <int STACK>
-static void run_pass(Renderer::Impl& I, FrameParams fp, bool count) {
-    const DevScene& sc = I.scene->dev();
-    auto tick = [&]() { return stage_begin(I); };
-    auto tock = [&](int stage, hipEvent_t e) { stage_end(I, stage, e); };
-    int* ctr = I.counters.as<int>();
-    unsigned long long* st = I.stats.as<unsigned long long>();
-    const int cap = (int)I.cap;
+template <int STACK, bool POOLED>
+void Engine::run_t(int n_items, int max_bounces, const bool* classes, int table, bool count) {
+    (void)n_items;
+    const char* dev = tables_.as<char>() + kPassTableBytes * table;
+    const PassHeader* hdr = reinterpret_cast<const PassHeader*>(dev);
+    const JobDev* jobs = reinterpret_cast<const JobDev*>(dev + kPassHeaderBytes);
+    int* ctr = counters_.as<int>();
+    unsigned long long* st = stats_.as<unsigned long long>();
+    const int cap = (int)cap_;
+    const int pi = POOLED ? 1 : 0, si = STACK == 64 ? 1 : 0;
+    const int gext = grid_ext_[pi][si];
+    const int gsh = grid_shadow_[pi][si];
+    const int* gshade = grid_shade_[pi][si];
     int q = 0;
-    hipEvent_t b = tick();
-    int grid_rg = (int)std::min<size_t>((fp.n_paths + kBlock - 1) / kBlock, (size_t)I.grid_linear * 4);
-    k_raygen<<<grid_rg, kBlock, 0, I.stream>>>(fp, I.qo[q].as<float4>(), I.qd[q].as<float4>(), I.qw[q].as<float4>(),
-                                               I.slot_d.as<float4>(), I.slot_i.as<float4>(), ctr, q, I.region);
+    hipEvent_t b = prof.begin(stream_);
+    k_raygen<POOLED><<<grid_linear_ * 4, kBlock, 0, stream_>>>(hdr, jobs, qo_[q].as<float4>(), qd_[q].as<float4>(),
+                                                               qw_[q].as<float4>(), slot_d_.as<float4>(),
+                                                               slot_i_.as<float4>(), ctr, q, region_);
     HIP_OK(hipGetLastError());
-    tock(kStageRaygen, b);
-    const GpuScene& gs = *I.scene;
-    for (int bounce = 0; bounce < fp.max_bounces; ++bounce) {
-        fp.bounce = bounce;
-        b = tick();
+    prof.end(kStageRaygen, b, stream_);
+    for (int bounce = 0; bounce < max_bounces; ++bounce) {
+        b = prof.begin(stream_);
         if (count)
-            k_extend<STACK, true><<<I.grid_persist_ext, kBlock, 0, I.stream>>>(
-                sc, I.qo[q].as<float4>(), I.qd[q].as<float4>(), I.hits.as<float4>(), I.lists.as<int>(), cap, I.region, ctr, q, st);
+            k_extend<STACK, true, POOLED><<<gext, kBlock, 0, stream_>>>(
+                jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), hits_.as<float4>(), lists_.as<int>(), cap, region_,
+                ctr, q, st);
         else
-            k_extend<STACK, false><<<I.grid_persist_ext, kBlock, 0, I.stream>>>(
-                sc, I.qo[q].as<float4>(), I.qd[q].as<float4>(), I.hits.as<float4>(), I.lists.as<int>(), cap, I.region, ctr, q, st);
+            k_extend<STACK, false, POOLED><<<gext, kBlock, 0, stream_>>>(
+                jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), hits_.as<float4>(), lists_.as<int>(), cap, region_,
+                ctr, q, st);
         HIP_OK(hipGetLastError());
-        tock(kStageExtend, b);
-        b = tick();
-        // Classes absent from the scene get no launch; their lists stay empty.
-        launch_shade<kClsTerminal, STACK>(I, sc, fp, q);
-        if (gs.has_class(kClsDiffuse)) launch_shade<kClsDiffuse, STACK>(I, sc, fp, q);
-        if (gs.has_class(kClsPbr)) launch_shade<kClsPbr, STACK>(I, sc, fp, q);
-        if (gs.has_class(kClsDielectric)) launch_shade<kClsDielectric, STACK>(I, sc, fp, q);
-        if (gs.has_class(kClsSubsurface)) launch_shade<kClsSubsurface, STACK>(I, sc, fp, q);
-        tock(kStageShade, b);
-        b = tick();
+        prof.end(kStageExtend, b, stream_);
+        b = prof.begin(stream_);
+        // Classes absent from every job of the pass get no launch; their lists stay empty.
+        auto shade = [&](auto cls_tag, int grid) {
+            constexpr int CLS = decltype(cls_tag)::value;
+            k_shade<CLS, STACK, POOLED><<<grid, kBlock, 0, stream_>>>(
+                jobs, bounce, qo_[q].as<float4>(), qd_[q].as<float4>(), qw_[q].as<float4>(), hits_.as<float4>(),
+                lists_.as<int>() + (size_t)CLS * cap_, qo_[q ^ 1].as<float4>(), qd_[q ^ 1].as<float4>(),
+                qw_[q ^ 1].as<float4>(), shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), region_,
+                ctr, q);
+            HIP_OK(hipGetLastError());
+        };
+        shade(std::integral_constant<int, kClsTerminal>(), gshade[kClsTerminal]);
+        if (classes[kClsDiffuse]) shade(std::integral_constant<int, kClsDiffuse>(), gshade[kClsDiffuse]);
+        if (classes[kClsPbr]) shade(std::integral_constant<int, kClsPbr>(), gshade[kClsPbr]);
+        if (classes[kClsDielectric]) shade(std::integral_constant<int, kClsDielectric>(), gshade[kClsDielectric]);
+        if (classes[kClsSubsurface]) shade(std::integral_constant<int, kClsSubsurface>(), gshade[kClsSubsurface]);
+        prof.end(kStageShade, b, stream_);
+        b = prof.begin(stream_);
         if (count)
-            k_shadow<STACK, true><<<I.grid_persist_shadow, kBlock, 0, I.stream>>>(
-                sc, I.shadow.as<float4>(), I.slot_d.as<float4>(), I.slot_i.as<float4>(), I.region, ctr, st);
+            k_shadow<STACK, true, POOLED><<<gsh, kBlock, 0, stream_>>>(
+                jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), region_, ctr, st);
         else
-            k_shadow<STACK, false><<<I.grid_persist_shadow, kBlock, 0, I.stream>>>(
-                sc, I.shadow.as<float4>(), I.slot_d.as<float4>(), I.slot_i.as<float4>(), I.region, ctr, st);
+            k_shadow<STACK, false, POOLED><<<gsh, kBlock, 0, stream_>>>(
+                jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), region_, ctr, st);
         HIP_OK(hipGetLastError());
-        tock(kStageShadow, b);
+        prof.end(kStageShadow, b, stream_);
         q ^= 1;
     }
-    b = tick();
-    if (!I.set.flush_survivors)
-        k_lost<<<I.grid_linear, kBlock, 0, I.stream>>>(I.qw[q].as<float4>(), I.slot_d.as<float4>(), I.slot_i.as<float4>(), ctr,
-                                                       q, I.region);
-    int k = fp.n_paths / I.wh;
-    int grid_res = (int)std::min<size_t>((I.wh + kBlock - 1) / kBlock, (size_t)I.grid_linear * 4);
-    k_resolve<<<grid_res, kBlock, 0, I.stream>>>(I.slot_d.as<float4>(), I.slot_i.as<float4>(), I.wh, k,
-                                                 I.acc_d.as<float4>(), I.acc_i.as<float4>());
+    b = prof.begin(stream_);
+    k_resolve<POOLED><<<grid_linear_ * 4, kBlock, 0, stream_>>>(hdr, jobs, slot_d_.as<float4>(), slot_i_.as<float4>());
     HIP_OK(hipGetLastError());
-    tock(kStageResolve, b);
+    prof.end(kStageResolve, b, stream_);
 }
 
-void Renderer::render(int spp, bool count) {
-    Impl& I = *impl_;
-    int remaining = spp;
-    while (remaining > 0) {
-        int k = std::min(remaining, I.batch);
-        FrameParams fp;
-        fp.cam_pos = I.cam.position;
-        fp.left_down = I.cam.left_down;
-        fp.du = I.cam.du;
-        fp.dv = I.cam.dv;
-        fp.width = I.width;
-        fp.height = I.height;
-        fp.wh = I.wh;
-        fp.seed = I.set.seed;
-        fp.rr = I.set.rr;
-        fp.rr_start = I.set.rr_start_depth;
-        fp.max_bounces = I.set.max_bounces;
-        fp.sample0 = I.sample_base + (uint32_t)I.spp_done;
-        fp.n_paths = I.wh * k;
-        fp.bounce = 0;
-        if (I.scene->stack() == 32) run_pass<32>(I, fp, count);
-        else run_pass<64>(I, fp, count);
-        I.spp_done += k;
-        remaining -= k;
-    }
+FrameParams frame_params(const CameraRec& cam, const RenderSettings& set, uint32_t sample0, int spp) {
+    FrameParams fp{};
+    fp.cam_pos = cam.position;
+    fp.left_down = cam.left_down;
+    fp.du = cam.du;
+    fp.dv = cam.dv;
+    fp.width = cam.width;
+    fp.height = cam.height;
+    fp.wh = cam.width * cam.height;
+    fp.seed = set.seed;
+    fp.rr = set.rr;
+    fp.rr_start = set.rr_start_depth;
+    fp.max_bounces = set.max_bounces;
+    fp.sample0 = sample0;
+    fp.n_paths = fp.wh * spp;
+    fp.bounce = 0;
+    return fp;
 }
 
-void Renderer::synchronize() { HIP_OK(hipStreamSynchronize(impl_->stream)); }
-
-// ------------------------------------------------------------------ frames
-void Renderer::set_camera(const CameraRec& c) {
-    Impl& I = *impl_;
-    if (c.width != I.width || c.height != I.height) throw std::invalid_argument("camera resolution differs from the renderer's");
-    I.cam = c;
-    reset();
-}
-
-const CameraRec& Renderer::camera() const { return impl_->cam; }
-int Renderer::frames() const { return impl_->frames; }
-
-void Renderer::reset_history() {
-    Impl& I = *impl_;
-    HIP_OK(hipStreamSynchronize(I.stream));
-    I.frames = 0;
-}
-
+// ------------------------------------------------------------------ frame state
 size_t frame_buffer_bytes(int which, int width, int height) {
     size_t wh = (size_t)width * height;
     switch (which) {
@@ -868,34 +922,34 @@ size_t frame_buffer_bytes(int which, int width, int height) {
     }
 }
 
-static void ensure_frame_buffers(Renderer::Impl& I) {
-    if (I.frame_alloc) return;
-    size_t wh = (size_t)I.wh;
+FrameState::FrameState(int width, int height) : w_(width), h_(height), wh_(width * height) {
+    size_t wh = (size_t)wh_;
     for (int k = 0; k < 2; ++k) {
-        I.g_pos[k].alloc(wh * 16);
-        I.g_nrm[k].alloc(wh * 8);
-        I.g_bary[k].alloc(wh * 8);
-        I.g_md[k].alloc(wh * 16);
-        I.h_d[k].alloc(wh * 16);
-        I.h_i[k].alloc(wh * 16);
-        I.m_d[k].alloc(wh * 8);
-        I.m_i[k].alloc(wh * 8);
-        I.h_len[k].alloc(wh * 4);
-        I.f_d[k].alloc(wh * 16);
-        I.f_i[k].alloc(wh * 16);
+        g_pos_[k].alloc(wh * 16);
+        g_nrm_[k].alloc(wh * 8);
+        g_bary_[k].alloc(wh * 8);
+        g_md_[k].alloc(wh * 16);
+        h_d_[k].alloc(wh * 16);
+        h_i_[k].alloc(wh * 16);
+        m_d_[k].alloc(wh * 8);
+        m_i_[k].alloc(wh * 8);
+        h_len_[k].alloc(wh * 4);
+        f_d_[k].alloc(wh * 16);
+        f_i_[k].alloc(wh * 16);
     }
-    I.v_d.alloc(wh * 16);
-    I.v_i.alloc(wh * 16);
-    I.denoised.alloc(wh * 16);
-    I.frame_alloc = true;
+    v_d_.alloc(wh * 16);
+    v_i_.alloc(wh * 16);
+    denoised_.alloc(wh * 16);
 }
 
-static GBufferView gview(Renderer::Impl& I, int k) {
+size_t FrameState::device_bytes() const { return (size_t)wh_ * kFrameStateBytesPerPixel; }
+
+static GBufferView gview(DevBuf* pos, DevBuf* nrm, DevBuf* bary, DevBuf* md, int k) {
     GBufferView v;
-    v.pos_tri = I.g_pos[k].as<float4>();
-    v.nrm_mat = I.g_nrm[k].as<ushort4>();
-    v.bary_mesh = I.g_bary[k].as<ushort4>();
-    v.motion_depth = I.g_md[k].as<float4>();
+    v.pos_tri = pos[k].as<float4>();
+    v.nrm_mat = nrm[k].as<ushort4>();
+    v.bary_mesh = bary[k].as<ushort4>();
+    v.motion_depth = md[k].as<float4>();
     return v;
 }
 
@@ -919,160 +973,210 @@ static GBufferParams gbuffer_params(const HostScene& h, const CameraRec& cur, co
     return g;
 }
 
-void Renderer::render_frame(int spp, const SvgfParams& p) {
-    Impl& I = *impl_;
-    if (spp < 1) throw std::invalid_argument("spp must be positive");
-    ensure_frame_buffers(I);
-    const int c = I.fcur, pv = c ^ 1;
-    const bool first = I.frames == 0;
-    const int w = I.width, h = I.height;
-    const int tiles = ((w + 15) / 16) * ((h + 15) / 16);
-    const DevScene& sc = I.scene->dev();
-
-    hipEvent_t e = stage_begin(I);
-    GBufferParams g = gbuffer_params(*I.host, I.cam, first ? I.cam : I.last_frame_cam);
-    if (I.scene->stack() == 32) k_gbuffer<32><<<tiles, kBlock, 0, I.stream>>>(sc, g, gview(I, c));
-    else k_gbuffer<64><<<tiles, kBlock, 0, I.stream>>>(sc, g, gview(I, c));
+void FrameState::begin(const GpuScene& gs, const HostScene& host, const CameraRec& cam, float4* acc_d,
+                       float4* acc_i, Profiler& prof, hipStream_t s) {
+    if (cam.width != w_ || cam.height != h_) throw std::invalid_argument("camera resolution differs from the view's");
+    const int c = fcur_;
+    const bool first = frames_ == 0;
+    const int tiles = ((w_ + 15) / 16) * ((h_ + 15) / 16);
+    hipEvent_t e = prof.begin(s);
+    GBufferParams g = gbuffer_params(host, cam, first ? cam : last_cam_);
+    GBufferView v = gview(g_pos_, g_nrm_, g_bary_, g_md_, c);
+    if (gs.stack() == 32) k_gbuffer<32><<<tiles, kBlock, 0, s>>>(gs.dev(), g, v);
+    else k_gbuffer<64><<<tiles, kBlock, 0, s>>>(gs.dev(), g, v);
     HIP_OK(hipGetLastError());
-    stage_end(I, kStageGBuffer, e);
-
+    prof.end(kStageGBuffer, e, s);
     // Fresh samples of this frame only (the reference resets its frame buffer
     // every denoised frame: renderer.cuh:1297-1302, 1121-1124).
-    k_zero4<<<I.grid_linear, kBlock, 0, I.stream>>>(I.acc_d.as<float4>(), (size_t)I.wh);
-    k_zero4<<<I.grid_linear, kBlock, 0, I.stream>>>(I.acc_i.as<float4>(), (size_t)I.wh);
+    int grid = std::max(1, std::min((wh_ + kBlock - 1) / kBlock, 2048));
+    k_zero4<<<grid, kBlock, 0, s>>>(acc_d, (size_t)wh_);
+    k_zero4<<<grid, kBlock, 0, s>>>(acc_i, (size_t)wh_);
     HIP_OK(hipGetLastError());
-    I.spp_done = 0;
-    I.sample_base = (uint32_t)I.frames * (uint32_t)spp;
-    render(spp);
+    last_cam_ = cam;
+    started_ = true;
+}
 
-    e = stage_begin(I);
+void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const float4* acc_i, Profiler& prof,
+                     hipStream_t s) {
+    if (!started_) throw std::logic_error("frame end without begin");
+    started_ = false;
+    const int c = fcur_, pv = c ^ 1;
+    const bool first = frames_ == 0;
+    const int tiles = ((w_ + 15) / 16) * ((h_ + 15) / 16);
+    GBufferView gc = gview(g_pos_, g_nrm_, g_bary_, g_md_, c), gp = gview(g_pos_, g_nrm_, g_bary_, g_md_, pv);
+    hipEvent_t e = prof.begin(s);
     TemporalIO tio;
-    tio.acc_d = I.acc_d.as<float4>();
-    tio.acc_i = I.acc_i.as<float4>();
+    tio.acc_d = acc_d;
+    tio.acc_i = acc_i;
     tio.inv_spp = 1.f / (float)spp;
-    tio.cur_d = I.h_d[c].as<float4>();
-    tio.cur_i = I.h_i[c].as<float4>();
-    tio.hist_d = I.h_d[pv].as<float4>();
-    tio.hist_i = I.h_i[pv].as<float4>();
-    tio.mom_prev_d = I.m_d[pv].as<float2>();
-    tio.mom_prev_i = I.m_i[pv].as<float2>();
-    tio.len_prev = I.h_len[pv].as<int>();
-    tio.mom_d = I.m_d[c].as<float2>();
-    tio.mom_i = I.m_i[c].as<float2>();
-    tio.len = I.h_len[c].as<int>();
-    k_temporal<<<tiles, kBlock, 0, I.stream>>>(w, h, first ? 1 : 0, p, tio, gview(I, c), gview(I, pv));
+    tio.cur_d = h_d_[c].as<float4>();
+    tio.cur_i = h_i_[c].as<float4>();
+    tio.hist_d = h_d_[pv].as<float4>();
+    tio.hist_i = h_i_[pv].as<float4>();
+    tio.mom_prev_d = m_d_[pv].as<float2>();
+    tio.mom_prev_i = m_i_[pv].as<float2>();
+    tio.len_prev = h_len_[pv].as<int>();
+    tio.mom_d = m_d_[c].as<float2>();
+    tio.mom_i = m_i_[c].as<float2>();
+    tio.len = h_len_[c].as<int>();
+    k_temporal<<<tiles, kBlock, 0, s>>>(w_, h_, first ? 1 : 0, p, tio, gc, gp);
     HIP_OK(hipGetLastError());
-    stage_end(I, kStageTemporal, e);
+    prof.end(kStageTemporal, e, s);
 
-    e = stage_begin(I);
+    e = prof.begin(s);
     FilterIO fio;
-    fio.in_d = I.h_d[c].as<float4>();
-    fio.in_i = I.h_i[c].as<float4>();
-    fio.out_d = I.v_d.as<float4>();
-    fio.out_i = I.v_i.as<float4>();
-    fio.mom_d = I.m_d[c].as<float2>();
-    fio.mom_i = I.m_i[c].as<float2>();
-    fio.len = I.h_len[c].as<int>();
-    k_variance<<<tiles, kBlock, 0, I.stream>>>(w, h, p, fio, gview(I, c));
+    fio.in_d = h_d_[c].as<float4>();
+    fio.in_i = h_i_[c].as<float4>();
+    fio.out_d = v_d_.as<float4>();
+    fio.out_i = v_i_.as<float4>();
+    fio.mom_d = m_d_[c].as<float2>();
+    fio.mom_i = m_i_[c].as<float2>();
+    fio.len = h_len_[c].as<int>();
+    k_variance<<<tiles, kBlock, 0, s>>>(w_, h_, p, fio, gc);
     HIP_OK(hipGetLastError());
-    stage_end(I, kStageVariance, e);
+    prof.end(kStageVariance, e, s);
 
-    I.out_sel = -1;
+    out_sel_ = -1;
     if (!first) {
         // a-trous passes with step 1, 2, 4, ... (reference: renderer.cuh:1236-1250).
         int passes = std::max(1, std::min(8, p.atrous_iters));
-        e = stage_begin(I);
+        e = prof.begin(s);
         for (int k = 0; k < passes; ++k) {
             int o = k & 1;
-            fio.in_d = k == 0 ? I.v_d.as<float4>() : I.f_d[o ^ 1].as<float4>();
-            fio.in_i = k == 0 ? I.v_i.as<float4>() : I.f_i[o ^ 1].as<float4>();
-            fio.out_d = I.f_d[o].as<float4>();
-            fio.out_i = I.f_i[o].as<float4>();
-            k_atrous<<<tiles, kBlock, 0, I.stream>>>(w, h, 1 << k, p, fio, gview(I, c));
+            fio.in_d = k == 0 ? v_d_.as<float4>() : f_d_[o ^ 1].as<float4>();
+            fio.in_i = k == 0 ? v_i_.as<float4>() : f_i_[o ^ 1].as<float4>();
+            fio.out_d = f_d_[o].as<float4>();
+            fio.out_i = f_i_[o].as<float4>();
+            k_atrous<<<tiles, kBlock, 0, s>>>(w_, h_, 1 << k, p, fio, gc);
             HIP_OK(hipGetLastError());
-            I.out_sel = o;
+            out_sel_ = o;
         }
-        stage_end(I, kStageAtrous, e);
+        prof.end(kStageAtrous, e, s);
     }
-    const float4* od = I.out_sel < 0 ? I.v_d.as<float4>() : I.f_d[I.out_sel].as<float4>();
-    const float4* oi = I.out_sel < 0 ? I.v_i.as<float4>() : I.f_i[I.out_sel].as<float4>();
-    int grid_c = (int)std::min<size_t>((I.wh + kBlock - 1) / kBlock, (size_t)I.grid_linear * 4);
-    k_combine<<<grid_c, kBlock, 0, I.stream>>>(od, oi, I.wh, I.denoised.as<float4>());
+    const float4* od = out_sel_ < 0 ? v_d_.as<float4>() : f_d_[out_sel_].as<float4>();
+    const float4* oi = out_sel_ < 0 ? v_i_.as<float4>() : f_i_[out_sel_].as<float4>();
+    int grid = std::max(1, std::min((wh_ + kBlock - 1) / kBlock, 2048));
+    k_combine<<<grid, kBlock, 0, s>>>(od, oi, wh_, denoised_.as<float4>());
     HIP_OK(hipGetLastError());
-
-    I.last_frame_cam = I.cam;
-    I.flast = c;
-    I.fcur = pv;
-    I.frames++;
+    flast_ = c;
+    fcur_ = pv;
+    frames_++;
 }
 
-void Renderer::read_frame(int which, void* out) {
-    Impl& I = *impl_;
+void FrameState::read(int which, void* out, hipStream_t s) {
     if (which < 0 || which >= kNumFrameBuffers) throw std::invalid_argument("unknown frame buffer");
-    if (I.flast < 0) throw std::invalid_argument("no frame has been rendered");
-    HIP_OK(hipStreamSynchronize(I.stream));
-    const int c = I.flast;
+    if (flast_ < 0) throw std::invalid_argument("no frame has been rendered");
+    HIP_OK(hipStreamSynchronize(s));
+    const int c = flast_;
     const DevBuf* src = nullptr;
     switch (which) {
-        case kFrameDenoised: src = &I.denoised; break;
-        case kFrameDirect: src = I.out_sel < 0 ? &I.v_d : &I.f_d[I.out_sel]; break;
-        case kFrameIndirect: src = I.out_sel < 0 ? &I.v_i : &I.f_i[I.out_sel]; break;
-        case kFrameHistoryDirect: src = &I.h_d[c]; break;
-        case kFrameHistoryIndirect: src = &I.h_i[c]; break;
-        case kFrameMomentsDirect: src = &I.m_d[c]; break;
-        case kFrameMomentsIndirect: src = &I.m_i[c]; break;
-        case kFrameHistoryLength: src = &I.h_len[c]; break;
-        case kFrameVarianceDirect: src = &I.v_d; break;
-        case kFrameVarianceIndirect: src = &I.v_i; break;
-        case kFrameGPosTri: src = &I.g_pos[c]; break;
-        case kFrameGNrmMat: src = &I.g_nrm[c]; break;
-        case kFrameGBaryMesh: src = &I.g_bary[c]; break;
-        case kFrameGMotionDepth: src = &I.g_md[c]; break;
+        case kFrameDenoised: src = &denoised_; break;
+        case kFrameDirect: src = out_sel_ < 0 ? &v_d_ : &f_d_[out_sel_]; break;
+        case kFrameIndirect: src = out_sel_ < 0 ? &v_i_ : &f_i_[out_sel_]; break;
+        case kFrameHistoryDirect: src = &h_d_[c]; break;
+        case kFrameHistoryIndirect: src = &h_i_[c]; break;
+        case kFrameMomentsDirect: src = &m_d_[c]; break;
+        case kFrameMomentsIndirect: src = &m_i_[c]; break;
+        case kFrameHistoryLength: src = &h_len_[c]; break;
+        case kFrameVarianceDirect: src = &v_d_; break;
+        case kFrameVarianceIndirect: src = &v_i_; break;
+        case kFrameGPosTri: src = &g_pos_[c]; break;
+        case kFrameGNrmMat: src = &g_nrm_[c]; break;
+        case kFrameGBaryMesh: src = &g_bary_[c]; break;
+        case kFrameGMotionDepth: src = &g_md_[c]; break;
     }
-    HIP_OK(hipMemcpy(out, src->p, frame_buffer_bytes(which, I.width, I.height), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(out, src->p, frame_buffer_bytes(which, w_, h_), hipMemcpyDeviceToHost));
 }
+
+// ------------------------------------------------------------------ renderer
+struct Renderer::Impl {
+    const GpuScene* scene = nullptr;
+    const HostScene* host = nullptr;
+    RenderSettings set;
+    int width = 0, height = 0, wh = 0;
+    int batch = 1;
+    std::unique_ptr<Engine> eng;
+    DevBuf acc_d, acc_i;
+    int spp_done = 0;
+    uint32_t sample_base = 0;  // sample index of the first sample in the accumulators
+    CameraRec cam;             // camera of the following passes
+    std::unique_ptr<FrameState> frames;
+};
+
+Renderer::Renderer(const GpuScene& scene, const HostScene& host, const RenderSettings& s) : impl_(new Impl) {
+    Impl& I = *impl_;
+    I.scene = &scene;
+    I.host = &host;
+    I.set = s;
+    I.width = host.camera.width;
+    I.height = host.camera.height;
+    I.wh = I.width * I.height;
+    I.cam = host.camera;
+    // Paths per pass: all requested samples of every pixel when they fit the
+    // queue budget.
+    size_t budget = s.queue_budget_bytes ? s.queue_budget_bytes : ((size_t)8 << 30);
+    int bmax = (int)std::max<size_t>(1, budget / (kQueueBytesPerPath * (size_t)I.wh));
+    int want = s.batch_spp > 0 ? s.batch_spp : 64;
+    I.batch = std::max(1, std::min(want, bmax));
+    if ((size_t)I.wh * I.batch > (size_t)0x7fffffff) I.batch = std::max(1, (int)(0x7fffffff / I.wh));
+    I.eng.reset(new Engine((size_t)I.wh * I.batch, s.profile != 0));
+    I.acc_d.alloc((size_t)I.wh * 16);
+    I.acc_i.alloc((size_t)I.wh * 16);
+    HIP_OK(hipMemset(I.acc_d.p, 0, (size_t)I.wh * 16));
+    HIP_OK(hipMemset(I.acc_i.p, 0, (size_t)I.wh * 16));
+    HIP_OK(hipDeviceSynchronize());
+}
+
+Renderer::~Renderer() {
+    if (impl_) {
+        if (impl_->eng) (void)hipStreamSynchronize(impl_->eng->stream());
+        delete impl_;
+    }
+}
+
+int Renderer::width() const { return impl_->width; }
+int Renderer::height() const { return impl_->height; }
+int Renderer::batch() const { return impl_->batch; }
+int Renderer::spp_done() const { return impl_->spp_done; }
+
+void Renderer::render(int spp, bool count) {
+    Impl& I = *impl_;
+    int remaining = spp;
+    while (remaining > 0) {
+        int k = std::min(remaining, I.batch);
+        PassItem it;
+        it.scene = I.scene;
+        it.fp = frame_params(I.cam, I.set, I.sample_base + (uint32_t)I.spp_done, k);
+        it.spp = k;
+        it.flush_survivors = I.set.flush_survivors;
+        it.acc_d = I.acc_d.as<float4>();
+        it.acc_i = I.acc_i.as<float4>();
+        I.eng->run({it}, count);
+        I.spp_done += k;
+        remaining -= k;
+    }
+}
+
+void Renderer::synchronize() { HIP_OK(hipStreamSynchronize(impl_->eng->stream())); }
 
 void Renderer::reset() {
     Impl& I = *impl_;
-    HIP_OK(hipStreamSynchronize(I.stream));
-    k_zero4<<<I.grid_linear, kBlock, 0, I.stream>>>(I.acc_d.as<float4>(), (size_t)I.wh);
-    k_zero4<<<I.grid_linear, kBlock, 0, I.stream>>>(I.acc_i.as<float4>(), (size_t)I.wh);
+    hipStream_t s = I.eng->stream();
+    int grid = std::max(1, std::min((I.wh + kBlock - 1) / kBlock, 2048));
+    k_zero4<<<grid, kBlock, 0, s>>>(I.acc_d.as<float4>(), (size_t)I.wh);
+    k_zero4<<<grid, kBlock, 0, s>>>(I.acc_i.as<float4>(), (size_t)I.wh);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipStreamSynchronize(I.stream));
+    HIP_OK(hipStreamSynchronize(s));
     I.spp_done = 0;
     I.sample_base = 0;
 }
 
-void Renderer::reset_stats() {
-    Impl& I = *impl_;
-    HIP_OK(hipStreamSynchronize(I.stream));
-    k_zero_u64<<<1, 64, 0, I.stream>>>(I.stats.as<unsigned long long>(), kNumStats);
-    HIP_OK(hipStreamSynchronize(I.stream));
-    for (auto& e : I.ev) {
-        (void)hipEventDestroy(e.second.first);
-        (void)hipEventDestroy(e.second.second);
-    }
-    I.ev.clear();
-    for (int s = 0; s < kNumStages; ++s) {
-        I.stage_ms[s] = 0;
-        I.stage_launches[s] = 0;
-    }
-}
+void Renderer::reset_stats() { impl_->eng->reset_stats(); }
 
 void Renderer::stats(RenderStats& out) {
     Impl& I = *impl_;
-    HIP_OK(hipStreamSynchronize(I.stream));
     unsigned long long s[kNumStats];
-    HIP_OK(hipMemcpy(s, I.stats.p, sizeof(s), hipMemcpyDeviceToHost));
-    for (auto& e : I.ev) {
-        float ms = 0.f;
-        HIP_OK(hipEventElapsedTime(&ms, e.second.first, e.second.second));
-        I.stage_ms[e.first] += ms;
-        I.stage_launches[e.first] += 1;
-        (void)hipEventDestroy(e.second.first);
-        (void)hipEventDestroy(e.second.second);
-    }
-    I.ev.clear();
+    I.eng->read_stats(s);
     out.ext_rays = s[kStatExtRays];
     out.shadow_rays = s[kStatShadowRays];
     out.ext_nodes = s[kStatExtNodes];
@@ -1081,29 +1185,72 @@ void Renderer::stats(RenderStats& out) {
     out.shadow_tris = s[kStatShadowTris];
     out.probe_rays = s[kStatProbeRays];
     for (int k = 0; k < kNumStages; ++k) {
-        out.stage_ms[k] = I.stage_ms[k];
-        out.stage_launches[k] = I.stage_launches[k];
+        out.stage_ms[k] = I.eng->prof.ms[k];
+        out.stage_launches[k] = I.eng->prof.launches[k];
     }
     out.spp_done = I.spp_done;
     out.batch_spp = I.batch;
 }
 
+void read_accum_buffers(const float4* acc_d, const float4* acc_i, int wh, int spp_done, int which, float* out,
+                        hipStream_t s) {
+    HIP_OK(hipStreamSynchronize(s));
+    size_t n = (size_t)wh * 4;
+    if (which == 0) {
+        HIP_OK(hipMemcpy(out, acc_d, n * 4, hipMemcpyDeviceToHost));
+        return;
+    }
+    if (which == 1) {
+        HIP_OK(hipMemcpy(out, acc_i, n * 4, hipMemcpyDeviceToHost));
+        return;
+    }
+    std::vector<float> d(n), ind(n);
+    HIP_OK(hipMemcpy(d.data(), acc_d, n * 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(ind.data(), acc_i, n * 4, hipMemcpyDeviceToHost));
+    int spp = spp_done > 0 ? spp_done : 1;
+    for (size_t k = 0; k < (size_t)wh; ++k) {
+        for (int c = 0; c < 3; ++c) out[4 * k + c] = (d[4 * k + c] + ind[4 * k + c]) / (float)spp;
+        out[4 * k + 3] = 1.f;
+    }
+}
+
 void Renderer::read_accum(int which, float* out) {
     Impl& I = *impl_;
-    HIP_OK(hipStreamSynchronize(I.stream));
-    std::vector<float> d((size_t)I.wh * 4), ind((size_t)I.wh * 4);
-    HIP_OK(hipMemcpy(d.data(), I.acc_d.p, d.size() * 4, hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(ind.data(), I.acc_i.p, ind.size() * 4, hipMemcpyDeviceToHost));
-    size_t n = (size_t)I.wh * 4;
-    if (which == 0) std::memcpy(out, d.data(), n * 4);
-    else if (which == 1) std::memcpy(out, ind.data(), n * 4);
-    else {
-        int spp = I.spp_done > 0 ? I.spp_done : 1;
-        for (size_t k = 0; k < (size_t)I.wh; ++k) {
-            for (int c = 0; c < 3; ++c) out[4 * k + c] = (d[4 * k + c] + ind[4 * k + c]) / (float)spp;
-            out[4 * k + 3] = 1.f;
-        }
-    }
+    read_accum_buffers(I.acc_d.as<float4>(), I.acc_i.as<float4>(), I.wh, I.spp_done, which, out, I.eng->stream());
+}
+
+void Renderer::set_camera(const CameraRec& c) {
+    Impl& I = *impl_;
+    if (c.width != I.width || c.height != I.height) throw std::invalid_argument("camera resolution differs from the renderer's");
+    I.cam = c;
+    reset();
+}
+
+const CameraRec& Renderer::camera() const { return impl_->cam; }
+int Renderer::frames() const { return impl_->frames ? impl_->frames->frames() : 0; }
+
+void Renderer::reset_history() {
+    Impl& I = *impl_;
+    HIP_OK(hipStreamSynchronize(I.eng->stream()));
+    if (I.frames) I.frames->reset_history();
+}
+
+void Renderer::render_frame(int spp, const SvgfParams& p) {
+    Impl& I = *impl_;
+    if (spp < 1) throw std::invalid_argument("spp must be positive");
+    if (!I.frames) I.frames.reset(new FrameState(I.width, I.height));
+    hipStream_t s = I.eng->stream();
+    I.frames->begin(*I.scene, *I.host, I.cam, I.acc_d.as<float4>(), I.acc_i.as<float4>(), I.eng->prof, s);
+    I.spp_done = 0;
+    I.sample_base = I.frames->next_sample0(spp);
+    render(spp);
+    I.frames->end(spp, p, I.acc_d.as<float4>(), I.acc_i.as<float4>(), I.eng->prof, s);
+}
+
+void Renderer::read_frame(int which, void* out) {
+    Impl& I = *impl_;
+    if (!I.frames) throw std::invalid_argument("no frame has been rendered");
+    I.frames->read(which, out, I.eng->stream());
 }
 
 // ------------------------------------------------------------------ tools

@@ -22,6 +22,10 @@
  *   rt_read_frame        <- GatherKernel denoise branch (renderer.cuh:970-978) and the
  *                           G-buffer / history buffers the SVGF kernels read
  *   rt_reset_history     <- hardResetDenoiseHistory (renderer.cuh:1125-1139)
+ *   rt_pool_*            <- no counterpart: the reference renders one scene per process
+ *                           (main.cu:26-33).  A pool keeps many scenes resident and
+ *                           runs many jobs through one wavefront engine (batching), and
+ *                           rt_pool_serve replays an arrival stream against it (serving).
  *   rt_last_error        <- std::runtime_error / printed CUDA errors (scene.cpp:51-53, global.h:19-27)
  *
  * Conventions: functions return RT_OK (0) or a negative RT_ERR_* code and set
@@ -49,6 +53,7 @@ extern "C" {
 #define RT_ERR_DEVICE (-4)
 #define RT_ERR_OUT_OF_MEMORY (-5)
 #define RT_ERR_INTERNAL (-6)
+#define RT_ERR_BUSY (-7) /* the pool's HBM budget cannot admit the job now */
 
 #define RT_ACCUM_DIRECT 0
 #define RT_ACCUM_INDIRECT 1
@@ -190,6 +195,66 @@ int rt_reset_history(rt_renderer* r);
 /* Copy an RT_FRAME_* buffer of the last frame; bytes must be its exact size
    (width*height times 16 for float4, 8 for float2 / uint16x4, 4 for int32). */
 int rt_read_frame(rt_renderer* r, int which, void* out, size_t bytes);
+
+/* ---- pools: many jobs over resident scenes, one wavefront engine ---- */
+typedef struct rt_pool rt_pool;
+
+typedef struct rt_pool_params {
+    int64_t queue_budget_bytes; /* path queues of the shared engine; 0 = 8 GiB */
+    int64_t hbm_budget_bytes;   /* resident scenes + job state admitted at once; 0 = 64 GiB */
+    int profile;                /* 1 = per-stage HIP event timing in rt_pool_stats */
+} rt_pool_params;
+
+typedef struct rt_job_desc {
+    int scene;                        /* slot from rt_pool_add_scene */
+    int spp;                          /* accumulate: total samples; real-time: samples per frame */
+    int frames;                       /* 0 = accumulate job; > 0 = real-time job with SVGF */
+    const rt_camera* cameras;         /* real-time: `frames` cameras; accumulate: NULL or one */
+    const rt_render_params* render;   /* NULL: the scene's values */
+    const rt_denoise_params* denoise; /* NULL: defaults */
+} rt_job_desc;
+
+typedef struct rt_job_status {
+    int done;
+    int spp_done;     /* accumulate: samples so far; real-time: samples of the last frame */
+    int frames_done;
+    double submit_s, start_s, done_s; /* seconds on the pool's clock; -1 = not yet */
+} rt_job_status;
+
+typedef struct rt_serve_request {
+    double arrival_s; /* seconds after the start of the run */
+    rt_job_desc job;
+} rt_serve_request;
+
+typedef struct rt_serve_report {
+    int jobs, frames;
+    double makespan_s;   /* first arrival to last completion */
+    double jobs_per_s;   /* jobs / makespan */
+    double frame_p50_ms, frame_p99_ms, job_p50_ms, job_p99_ms;
+} rt_serve_report;
+
+int rt_pool_create(const rt_pool_params* p, rt_pool** out);
+void rt_pool_destroy(rt_pool* p);
+/* The scene must outlive the pool. */
+int rt_pool_add_scene(rt_pool* p, rt_scene* s, int* slot_out);
+/* Admit a job; RT_ERR_BUSY when its state does not fit the HBM budget now. */
+int rt_pool_submit(rt_pool* p, const rt_job_desc* d, int* job_out);
+/* One pass over the active jobs (synchronous): each accumulate job that fits gets as many
+   samples as fit, each real-time job one frame.  served_out: jobs in the pass (0 = idle). */
+int rt_pool_step(rt_pool* p, int* served_out);
+int rt_pool_job_status(rt_pool* p, int job, rt_job_status* out);
+int rt_pool_read_accum(rt_pool* p, int job, int which, float* rgba_out);
+int rt_pool_read_frame(rt_pool* p, int job, int which, void* out, size_t bytes);
+int rt_pool_release(rt_pool* p, int job);
+int rt_pool_stats(rt_pool* p, rt_stats_out* out);
+int rt_pool_reset_stats(rt_pool* p);
+/* Replay an arrival stream: batched = 1 serves it with the pool's continuous batching,
+   0 with the plain loop (one job at a time, a fresh single-scene renderer each).  Frame
+   latency = completion - (arrival for a job's first frame, the previous frame's completion
+   otherwise); job latency = last completion - arrival.  Optional outputs: frame latencies
+   in request order then frame order, job latencies in request order. */
+int rt_pool_serve(rt_pool* p, const rt_serve_request* reqs, int n, int batched, rt_serve_report* out,
+                  double* frame_latency_ms, double* job_latency_ms);
 
 /* ---- test and inspection hooks (no reference counterpart) ---- */
 /* Scene-order triangle soup: pos 9, normals 9, uv 6 floats and mesh id per triangle (any pointer may be NULL). */
