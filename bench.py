@@ -106,6 +106,135 @@ def frames_bench(frames: int = 10, width: int = 1280, height: int = 720):
     }
 
 
+def _write_scene_job(job):
+    """Worker: build one generated scene and write it (JSON + OBJ + PFM)."""
+    kind, kwargs, out_dir, name = job
+    from cudaraytracer_amd import scenes
+
+    spec = getattr(scenes, kind)(**kwargs)
+    return scenes.write_scene(spec, out_dir, name)
+
+
+def _generate(jobs):
+    """Write generated scenes in parallel worker processes (spawned: no GPU state)."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+
+    workers = max(1, min(16, os.cpu_count() or 1, len(jobs)))
+    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_write_scene_job, jobs))
+
+
+def _load(paths):
+    """Load scenes on host threads (the C++ loader and BVH build run without the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import cudaraytracer_amd as rt
+
+    rt.lib()
+    with ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1))) as ex:
+        return list(ex.map(rt.Scene, paths))
+
+
+def batched_bench(n: int = 64, seed: int = 4, queue_gib: int = 16):
+    """The uneven batch: n seeded scenes (triangles log-uniform 1k-500k, square
+    resolution from {128..1024}, spp 1..64, depth 12), all submitted at t = 0.
+    Batched: the pool's continuous batching, every stage launch shared by all
+    jobs with work left.  Plain: one job at a time, a fresh single-scene
+    renderer each.  Images are bit-identical between the two (same RNG keys)."""
+    import cudaraytracer_amd as rt
+
+    rng = np.random.default_rng(seed)
+    jobs, meta = [], []
+    tmp = tempfile.mkdtemp(prefix="rtbatch_")
+    for k in range(n):
+        tris = int(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
+        width = int(rng.choice([128, 256, 512, 1024]))
+        spp = int(rng.integers(1, 65))
+        name = f"batch{k:02d}"
+        jobs.append(("batch_scene", {"seed": 1000 + k, "target_tris": tris, "width": width, "spp": spp},
+                     os.path.join(tmp, name), name))
+        meta.append((width, spp))
+    t0 = time.perf_counter()
+    paths = _generate(jobs)
+    t1 = time.perf_counter()
+    sc = _load(paths)
+    t2 = time.perf_counter()
+    pool = rt.Pool(queue_budget_bytes=queue_gib << 30)
+    slots = [pool.add_scene(s) for s in sc]
+    reqs = [(0.0, rt.Job(slot, spp=spp)) for slot, (_, spp) in zip(slots, meta)]
+    tris_total = sum(s.info().num_triangles for s in sc)
+    paths_total = sum(w * w * spp for w, spp in meta)
+    # Warm-up: one small job through each mode (kernel loading, allocator).
+    pool.serve([reqs[0]], batched=True)
+    pool.serve([reqs[0]], batched=False)
+    pool.reset_stats()
+    rep_b, _, job_b = pool.serve(reqs, batched=True)
+    st = pool.stats()
+    rays = st.extension_rays + st.shadow_rays
+    rep_p, _, job_p = pool.serve(reqs, batched=False)
+    return {
+        "workload": f"{n} seeded scenes: Cornell shell + GGX/dielectric/diffuse icospheres + heightfield patch, "
+                    "triangles log-uniform 1k-500k, resolution from {128,256,512,1024}^2, spp 1..64, depth 12",
+        "triangles_total": int(tris_total),
+        "paths_total": int(paths_total),
+        "scene_write_s": round(t1 - t0, 2),
+        "scene_load_s": round(t2 - t1, 2),
+        "batched": {"jobs_per_s": round(rep_b["jobs_per_s"], 3), "makespan_s": round(rep_b["makespan_s"], 4),
+                    "job_p50_ms": round(rep_b["job_p50_ms"], 2), "job_p99_ms": round(rep_b["job_p99_ms"], 2),
+                    "mrays_s": round(rays / rep_b["makespan_s"] / 1e6, 1)},
+        "plain": {"jobs_per_s": round(rep_p["jobs_per_s"], 3), "makespan_s": round(rep_p["makespan_s"], 4),
+                  "job_p50_ms": round(rep_p["job_p50_ms"], 2), "job_p99_ms": round(rep_p["job_p99_ms"], 2),
+                  "mrays_s": round(rays / rep_p["makespan_s"] / 1e6, 1)},
+        "speedup": round(rep_b["jobs_per_s"] / rep_p["jobs_per_s"], 3),
+        "queue_gib": queue_gib,
+    }
+
+
+def serving_bench(rates=(5.0, 20.0, 100.0), n_jobs: int = 30, frames: int = 10, width: int = 1280,
+                  height: int = 720, seed: int = 5, queue_gib: int = 16):
+    """Poisson stream of real-time jobs on terrain + SSS + env scenes (2 M,
+    1 M and 0.5 M triangles): each job a `frames`-frame orbit at width x height,
+    1 spp per frame, SVGF with 5 a-trous passes.  For each mean rate the same
+    arrival sequence is served with continuous batching and with the plain
+    loop; p50/p99 of frame and job latency and jobs/s are reported."""
+    import cudaraytracer_amd as rt
+
+    tmp = tempfile.mkdtemp(prefix="rtserve_")
+    specs = [("terrain_env", {"seed": sd, "grid": g, "width": width, "height": height, "spp": 1},
+              os.path.join(tmp, f"terrain{sd}"), f"terrain{sd}") for sd, g in ((3, 1001), (5, 709), (7, 501))]
+    sc = _load(_generate(specs))
+    pool = rt.Pool(queue_budget_bytes=queue_gib << 30)
+    slots = [pool.add_scene(s) for s in sc]
+    base_cams = [rt.Renderer(s, batch_spp=1).camera() for s in sc]
+    out = {"workload": f"Poisson arrivals of {n_jobs} jobs per rate; each a {frames}-frame orbit at "
+                       f"{width}x{height}, 1 spp/frame, SVGF (5 a-trous); scenes: terrain_env 2.0M/1.0M/0.5M "
+                       "triangles + SSS sphere + 2048x1024 env", "rates": []}
+    warm = rt.Job(slots[2], spp=1, frames=2, cameras=[base_cams[2]] * 2)
+    pool.serve([(0.0, warm)], batched=True)
+    pool.serve([(0.0, warm)], batched=False)
+    for rate in rates:
+        rng = np.random.default_rng(seed + int(rate))
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+        arrivals -= arrivals[0]
+        reqs = []
+        for k, t in enumerate(arrivals):
+            s = int(rng.integers(0, len(sc)))
+            c = base_cams[s]
+            phi0 = float(rng.uniform(-20, 20))
+            cams = [rt.Camera(tuple(c.target), c.distance, c.theta_deg, c.phi_deg + phi0 + 0.5 * f)
+                    for f in range(frames)]
+            reqs.append((float(t), rt.Job(slots[s], spp=1, frames=frames, cameras=cams)))
+        row = {"rate_jobs_s": rate}
+        for mode in ("batched", "plain"):
+            rep, _, _ = pool.serve(reqs, batched=(mode == "batched"))
+            row[mode] = {k: round(v, 3) if isinstance(v, float) else v for k, v in rep.items()}
+            print(f"[bench] serving rate {rate}: {mode} {row[mode]}", file=sys.stderr, flush=True)
+        row["jobs_per_s_ratio"] = round(row["batched"]["jobs_per_s"] / row["plain"]["jobs_per_s"], 3)
+        out["rates"].append(row)
+    return out
+
+
 def cpu_baseline(scene_path: str, spp: int):
     from oracle import rtref
 
@@ -137,6 +266,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=None, help="HBM bytes per launch measured with rocprofv3 --pmc")
     ap.add_argument("--no-frames", action="store_true", help="skip the real-time SVGF frame-loop measurement")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched-vs-plain measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the Poisson serving measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU is design-only)"}))
@@ -232,6 +363,13 @@ def main() -> int:
     if not args.no_frames:
         print("[bench] real-time frame loop (SVGF)", file=sys.stderr, flush=True)
         out["frames"] = frames_bench()
+    if not args.no_batched:
+        print("[bench] batched: 64 uneven scenes, pool vs one at a time", file=sys.stderr, flush=True)
+        out["batched"] = batched_bench()
+        print(f"[bench] batched: {out['batched']}", file=sys.stderr, flush=True)
+    if not args.no_serving:
+        print("[bench] serving: Poisson stream of real-time jobs", file=sys.stderr, flush=True)
+        out["serving"] = serving_bench()
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(path, args.cpu_spp)

@@ -231,15 +231,17 @@ def write_obj(path: str, mesh: MeshSpec) -> None:
 
 
 def _write_indexed_obj(path: str, mesh: MeshSpec) -> None:
+    # float32 values go through Python floats (exact) and %.9g (round-trips float32).
     with open(path, "w") as f:
         f.write(f"o {mesh.name}\n")
-        np.savetxt(f, mesh.verts.astype(np.float32), fmt="v %.9g %.9g %.9g")
+        f.write("\n".join(["v %.9g %.9g %.9g" % tuple(r) for r in mesh.verts.astype(np.float32).tolist()]) + "\n")
+        fi = (mesh.faces.astype(np.int64) + 1).tolist()
         if mesh.vnorm is not None:
-            np.savetxt(f, mesh.vnorm.astype(np.float32), fmt="vn %.9g %.9g %.9g")
-            fi = mesh.faces.astype(np.int64) + 1
-            np.savetxt(f, np.repeat(fi, 2, axis=1), fmt="f %d//%d %d//%d %d//%d")
+            f.write("\n".join(["vn %.9g %.9g %.9g" % tuple(r) for r in mesh.vnorm.astype(np.float32).tolist()])
+                    + "\n")
+            f.write("\n".join(["f %d//%d %d//%d %d//%d" % (a, a, b, b, c, c) for a, b, c in fi]) + "\n")
         else:
-            np.savetxt(f, mesh.faces.astype(np.int64) + 1, fmt="f %d %d %d")
+            f.write("\n".join(["f %d %d %d" % (a, b, c) for a, b, c in fi]) + "\n")
 
 
 def write_pfm(path: str, rgb: np.ndarray) -> None:
@@ -348,6 +350,71 @@ def terrain_env(seed: int = 3, grid: int = 1001, size: float = 100.0, env_w: int
                      lights=[{"name": "sky", "type": "env", "path": "sky.pfm"}])
     spec.env = sky_env(env_w, env_h)
     return spec
+
+
+def batch_scene(seed: int, target_tris: int, width: int, spp: int, depth: int = 12) -> SceneSpec:
+    """One scene of the uneven batch: the Cornell shell, up to four icospheres
+    (subdivision 4 or 3) with the materials of the spheres scene (GGX
+    metallic-roughness, IOR-1.5 dielectric, Lambertian), and the remaining
+    triangle budget as a heightfield patch on the floor."""
+    rng = np.random.default_rng(seed)
+    meshes = cornell_shell()
+    mats = _base_materials()
+    budget = max(0, int(target_tris) - 36)
+    boxes = [((-0.75, -0.6), (-0.15, 0.0)), ((0.1, 0.1), (0.7, 0.7))]
+    placed = []
+    kinds = ["pbr", "dielectric", "diffuse", "pbr"]
+    for k, kind in enumerate(kinds):
+        sub = 4 if budget >= 2 * 5120 else (3 if budget >= 2 * 1280 else None)
+        if sub is None:
+            break
+        V, F = icosphere(sub)
+        r = float(rng.uniform(0.12, 0.22))
+        for _ in range(1000):
+            cx, cz = rng.uniform(-0.95 + r, 0.95 - r, size=2)
+            ok = all(not (lo[0] - r < cx < hi[0] + r and lo[1] - r < cz < hi[1] + r) for lo, hi in boxes)
+            ok = ok and all((cx - px) ** 2 + (cz - pz) ** 2 > (r + pr + 0.02) ** 2 for px, pz, pr in placed)
+            if ok:
+                break
+        else:
+            break
+        placed.append((cx, cz, r))
+        name = f"sphere{k}"
+        if kind == "pbr":
+            mats.append({"name": f"mat_{name}", "type": "pbr",
+                         "basecolor": [float(x) for x in rng.uniform(0.2, 0.95, size=3)],
+                         "metallic": float(rng.integers(0, 2)), "roughness": float(rng.uniform(0.05, 0.8))})
+        elif kind == "dielectric":
+            mats.append({"name": f"mat_{name}", "type": "dielectric", "basecolor": [1.0, 1.0, 1.0], "ior": 1.5})
+        else:
+            mats.append({"name": f"mat_{name}", "type": "diffuse",
+                         "basecolor": [float(x) for x in rng.uniform(0.2, 0.9, size=3)]})
+        center = np.array([cx, r + 0.06, cz])
+        meshes.append(MeshSpec(name, f"mat_{name}", (V[F] * r + center).astype(np.float32), V[F].astype(np.float32)))
+        budget -= len(F)
+    if budget >= 2:
+        g = int(np.sqrt(budget / 2.0)) + 1
+        patch = terrain_mesh(g, 1.9, 0.05, seed)
+        v = patch.verts.copy()
+        v[:, 1] -= v[:, 1].min() - 0.002
+        mats.append({"name": "patch", "type": "diffuse", "basecolor": [float(x) for x in rng.uniform(0.3, 0.8, 3)]})
+        meshes.append(MeshSpec("patch", "patch", verts=v, vnorm=patch.vnorm, faces=patch.faces))
+    return SceneSpec(camera=_camera(width, width, spp, depth), materials=mats, meshes=meshes,
+                     lights=[{"name": "light0", "type": "area", "lightMeshName": "light", "material": "light"}])
+
+
+def batch_mix(n: int = 64, seed: int = 4, lo_tris: float = 1e3, hi_tris: float = 5e5):
+    """The uneven batch: n seeded scenes with triangle counts log-uniform in
+    [lo_tris, hi_tris], square resolutions from {128, 256, 512, 1024} and spp
+    uniform on 1..64.  Returns [(name, SceneSpec)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        tris = float(np.exp(rng.uniform(np.log(lo_tris), np.log(hi_tris))))
+        width = int(rng.choice([128, 256, 512, 1024]))
+        spp = int(rng.integers(1, 65))
+        out.append((f"batch{k:02d}", batch_scene(1000 + k, int(tris), width, spp)))
+    return out
 
 
 def write_scene(spec: SceneSpec, out_dir: str, name: str = "scene") -> str:
