@@ -940,6 +940,8 @@ FrameState::FrameState(int width, int height) : w_(width), h_(height), wh_(width
     v_d_.alloc(wh * 16);
     v_i_.alloc(wh * 16);
     denoised_.alloc(wh * 16);
+    guide_nd_.alloc(wh * 16);
+    guide_kl_.alloc(wh * 8);
 }
 
 size_t FrameState::device_bytes() const { return (size_t)wh_ * kFrameStateBytesPerPixel; }
@@ -1019,6 +1021,8 @@ void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const fl
     tio.mom_d = m_d_[c].as<float2>();
     tio.mom_i = m_i_[c].as<float2>();
     tio.len = h_len_[c].as<int>();
+    tio.guide_nd = guide_nd_.as<float4>();
+    tio.guide_kl = guide_kl_.as<int2>();
     k_temporal<<<tiles, kBlock, 0, s>>>(w_, h_, first ? 1 : 0, p, tio, gc, gp);
     HIP_OK(hipGetLastError());
     prof.end(kStageTemporal, e, s);
@@ -1032,6 +1036,8 @@ void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const fl
     fio.mom_d = m_d_[c].as<float2>();
     fio.mom_i = m_i_[c].as<float2>();
     fio.len = h_len_[c].as<int>();
+    fio.guide_nd = guide_nd_.as<float4>();
+    fio.guide_kl = guide_kl_.as<int2>();
     k_variance<<<tiles, kBlock, 0, s>>>(w_, h_, p, fio, gc);
     HIP_OK(hipGetLastError());
     prof.end(kStageVariance, e, s);

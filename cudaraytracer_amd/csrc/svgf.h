@@ -129,7 +129,16 @@ struct TemporalIO {
     const int* len_prev;
     float2 *mom_d, *mom_i;
     int* len;
+    // Filter guide of this frame, one record per pixel, written here once so the
+    // variance and a-trous taps need no normal decode: (decoded normal, linear
+    // depth) and (mesh << 16 | material, history length).
+    float4* guide_nd;
+    int2* guide_kl;
 };
+
+RT_HD int2 guide_key_len(ushort4 nrm_mat, ushort4 bary_mesh, int len) {
+    return make_int2((int)(((uint32_t)bary_mesh.w << 16) | (uint32_t)nrm_mat.w), len);
+}
 
 // IsValidTemporalReprojectionTap (reference: renderer.cuh:460-497).
 RT_HD bool temporal_tap_ok(int tx, int ty, int w, int h, int mesh, int mat, int tri, vec3 n, float depth, float dz,
@@ -191,6 +200,12 @@ RT_HD void temporal_pixel(int i, int j, int w, int h, bool first, const SvgfPara
         io.cur_d[idx] = cd;
         io.cur_i[idx] = ci;
         io.len[idx] = 1;
+        {
+            ushort4 gn0 = cur.nrm_mat[idx];
+            vec3 n0 = decode_normal(gn0);
+            io.guide_nd[idx] = make_float4(n0.x, n0.y, n0.z, cur.motion_depth[idx].z);
+            io.guide_kl[idx] = guide_key_len(gn0, cur.bary_mesh[idx], 1);
+        }
         return;
     }
     float cx = (float)i + 0.5f, cy = (float)j + 0.5f;
@@ -291,6 +306,8 @@ RT_HD void temporal_pixel(int i, int j, int w, int h, bool first, const SvgfPara
     }
     io.cur_d[idx] = cd;
     io.cur_i[idx] = ci;
+    io.guide_nd[idx] = make_float4(n.x, n.y, n.z, depth);
+    io.guide_kl[idx] = guide_key_len(gn, gb, valid ? hl : 1);
 }
 
 // ---------------------------------------------------------------- variance
@@ -299,10 +316,15 @@ struct FilterIO {
     float4 *out_d, *out_i;
     const float2 *mom_d, *mom_i;
     const int* len;
+    const float4* guide_nd;  // decoded normal, linear depth
+    const int2* guide_kl;    // mesh << 16 | material, history length
 };
 
 // ComputeVarianceKernel (reference: renderer.cuh:689-805): a 7x7 bilateral
-// estimate while the history is shorter than 4 frames, else a copy.
+// estimate while the history is shorter than 4 frames, else a copy.  Taps
+// read the per-pixel guide (the same decoded normals and depths the
+// reference's G-buffer fetches give); denominators that depend only on the
+// centre are formed once.
 RT_HD void variance_pixel(int i, int j, int w, int h, const SvgfParams& p, const FilterIO& io, const GBufferView& g) {
     size_t idx = (size_t)j * w + i;
     int hl = io.len[idx];
@@ -313,36 +335,32 @@ RT_HD void variance_pixel(int i, int j, int w, int h, const SvgfParams& p, const
         return;
     }
     float cld = luminance(v3(cd4.x, cd4.y, cd4.z)), cli = luminance(v3(ci4.x, ci4.y, ci4.z));
-    float4 gm = g.motion_depth[idx];
-    float cdepth = gm.z;
-    float dz = fmaxf(gm.w, 1e-4f);
-    ushort4 gn = g.nrm_mat[idx];
-    ushort4 gb = g.bary_mesh[idx];
-    int cmat = gn.w, cmesh = gb.w;
-    vec3 cn = decode_normal(gn);
+    const float4 cg = io.guide_nd[idx];
+    const int ckey = io.guide_kl[idx].x;
+    float cdepth = cg.w;
+    float dz = fmaxf(g.motion_depth[idx].w, 1e-4f);
+    vec3 cn = v3(cg.x, cg.y, cg.z);
     vec3 fd = v3(0.f, 0.f, 0.f), fi = v3(0.f, 0.f, 0.f);
     float fmd0 = 0.f, fmd1 = 0.f, fmi0 = 0.f, fmi1 = 0.f, twd = 0.f, twi = 0.f;
     float sigma_depth = p.sigma_depth * fmaxf(dz, 1e-4f);
+    const float wind_den = p.sigma_light / sqrtf(cli);
     for (int kj = -3; kj <= 3; ++kj)
         for (int ki = -3; ki <= 3; ++ki) {
             int ni = i + ki, nj = j + kj;
             if (!(ni >= 0 && ni < w && nj >= 0 && nj < h)) continue;
             size_t n = (size_t)nj * w + ni;
+            if (io.guide_kl[n].x != ckey) continue;
+            float4 ng = io.guide_nd[n];
             float4 nd4 = io.in_d[n], ni4 = io.in_i[n];
-            float ndepth = g.motion_depth[n].z;
-            ushort4 nn = g.nrm_mat[n];
-            ushort4 nb = g.bary_mesh[n];
-            if (cmat != (int)nn.w || cmesh != (int)nb.w) continue;
-            vec3 nrm = decode_normal(nn);
             vec3 nd = v3(nd4.x, nd4.y, nd4.z), nic = v3(ni4.x, ni4.y, ni4.z);
             float nld = luminance(nd), nli = luminance(nic);
             float2 nmd = io.mom_d[n], nmi = io.mom_i[n];
             float dist = sqrtf((float)(ki * ki + kj * kj));
-            float wz = -fabsf(cdepth - ndepth) / (sigma_depth * dist + 1e-4f);
-            float wn = fm::powf_(fmaxf(0.f, dot(cn, nrm)), p.sigma_normal);
+            float wz = -fabsf(cdepth - ng.w) / (sigma_depth * dist + 1e-4f);
+            float wn = fm::powf_(fmaxf(0.f, dot(cn, v3(ng.x, ng.y, ng.z))), p.sigma_normal);
             float dref = fmaxf(fmaxf(cld, nld), 1e-4f);
             float wdir = -fabsf(cld - nld) / (p.sigma_light / sqrtf(dref));
-            float wind = -fabsf(cli - nli) / (p.sigma_light / sqrtf(cli));
+            float wind = -fabsf(cli - nli) / wind_den;
             float dw = fm::expf_(wz + wdir) * wn;
             float iw = fm::expf_(wz + wind) * wn;
             fd = fd + nd * dw;
@@ -373,20 +391,19 @@ RT_HD void variance_pixel(int i, int j, int w, int h, const SvgfParams& p, const
 // ---------------------------------------------------------------- a-trous
 // AtrousKernel (reference: renderer.cuh:807-954): 3x3-filtered variance,
 // history bootstrap, 5x5 B3 taps at `step`, variance propagated with w^2.
+// Taps read the per-pixel guide; centre-only denominators are formed once.
 RT_HD void atrous_pixel(int i, int j, int w, int h, int step, const SvgfParams& p, const FilterIO& io,
                         const GBufferView& g) {
     size_t idx = (size_t)j * w + i;
-    int chl = io.len[idx] > 1 ? io.len[idx] : 1;
+    const int2 ckl = io.guide_kl[idx];
+    int chl = ckl.y > 1 ? ckl.y : 1;
     float4 cd4 = io.in_d[idx], ci4 = io.in_i[idx];
     vec3 cd = v3(cd4.x, cd4.y, cd4.z), ci = v3(ci4.x, ci4.y, ci4.z);
     float cld = luminance(cd), cli = luminance(ci);
-    float4 gm = g.motion_depth[idx];
-    float cdepth = gm.z;
-    float dz = fmaxf(gm.w, 1e-4f);
-    ushort4 gn = g.nrm_mat[idx];
-    ushort4 gb = g.bary_mesh[idx];
-    int cmat = gn.w, cmesh = gb.w;
-    vec3 cn = decode_normal(gn);
+    const float4 cg = io.guide_nd[idx];
+    float cdepth = cg.w;
+    float dz = fmaxf(g.motion_depth[idx].w, 1e-4f);
+    vec3 cn = v3(cg.x, cg.y, cg.z);
     float fvd = 0.f, fvi = 0.f, fvw = 0.f;
     for (int vj = -1; vj <= 1; ++vj)
         for (int vi = -1; vi <= 1; ++vi) {
@@ -407,6 +424,8 @@ RT_HD void atrous_pixel(int i, int j, int w, int h, int step, const SvgfParams& 
     float sld = p.sigma_light * (1.f + 2.0f * boot);
     float sli = p.sigma_light * (1.f + 1.5f * boot);
     float sn = fmaxf(8.f, p.sigma_normal * (1.f - 0.85f * boot));
+    const float sld_sdd = sld * sdd;
+    const float wind_den = (sli * sdi) / sqrtf(fmaxf(cli, 1e-4f));
     vec3 fd = v3(0.f, 0.f, 0.f), fi = v3(0.f, 0.f, 0.f);
     float twd = 0.f, twi = 0.f, vard = 0.f, vari = 0.f;
     for (int kj = -2; kj <= 2; ++kj)
@@ -414,23 +433,20 @@ RT_HD void atrous_pixel(int i, int j, int w, int h, int step, const SvgfParams& 
             int ni = i + ki * step, nj = j + kj * step;
             if (!(ni >= 0 && ni < w && nj >= 0 && nj < h)) continue;
             size_t n = (size_t)nj * w + ni;
-            int nhl = io.len[n] > 1 ? io.len[n] : 1;
-            float ndepth = g.motion_depth[n].z;
-            ushort4 nn = g.nrm_mat[n];
-            ushort4 nb = g.bary_mesh[n];
-            if (cmat != (int)nn.w || cmesh != (int)nb.w) continue;
-            vec3 nrm = decode_normal(nn);
+            const int2 nkl = io.guide_kl[n];
+            if (nkl.x != ckl.x) continue;
+            int nhl = nkl.y > 1 ? nkl.y : 1;
+            float4 ng = io.guide_nd[n];
             float4 nd4 = io.in_d[n], ni4 = io.in_i[n];
             vec3 nd = v3(nd4.x, nd4.y, nd4.z), nic = v3(ni4.x, ni4.y, ni4.z);
             float nvd = fmaxf(0.0f, nd4.w), nvi = fmaxf(0.0f, ni4.w);
             float dist = sqrtf((float)(ki * ki + kj * kj)) * (float)step;
-            float wz = -fabsf(cdepth - ndepth) / (sigma_depth * dist + 1e-6f);
-            float wn = fm::powf_(fmaxf(0.f, dot(cn, nrm)), sn);
+            float wz = -fabsf(cdepth - ng.w) / (sigma_depth * dist + 1e-6f);
+            float wn = fm::powf_(fmaxf(0.f, dot(cn, v3(ng.x, ng.y, ng.z))), sn);
             float ld = luminance(nd), li = luminance(nic);
             float dref = fmaxf(fmaxf(cld, ld), 1e-4f);
-            float iref = fmaxf(cli, 1e-4f);
-            float wdir = -fabsf(cld - ld) / ((sld * sdd) / sqrtf(dref));
-            float wind = -fabsf(cli - li) / ((sli * sdi) / sqrtf(iref));
+            float wdir = -fabsf(cld - ld) / (sld_sdd / sqrtf(dref));
+            float wind = -fabsf(cli - li) / wind_den;
             float conf = boot > 0.f ? (0.5f + 0.5f * fminf((float)nhl / 4.f, 1.f)) : 1.f;
             float kw = (ki == 0 ? 0.375f : (ki == 1 || ki == -1 ? 0.25f : 0.0625f)) *
                        (kj == 0 ? 0.375f : (kj == 1 || kj == -1 ? 0.25f : 0.0625f));
