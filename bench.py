@@ -51,10 +51,11 @@ def _stage_roofline(name, bytes_per_launch, ms_per_launch, traffic=None):
 TEMPORAL_PX_BYTES, VARIANCE_PX_BYTES, ATROUS_PX_BYTES, GBUFFER_PX_BYTES = 232, 116, 100, 48
 
 
-def frames_bench(frames: int = 10, width: int = 1280, height: int = 720):
+def frames_bench(frames: int = 10, width: int = 1280, height: int = 720, env_alias: bool = True):
     """Real-time loop on the terrain + subsurface + HDR environment scene
     (2 M triangles): `frames` frames of an orbiting camera at 1 spp with SVGF
-    (temporal, variance, 5 a-trous passes).  Returns per-stage ms per frame."""
+    (temporal, variance, 5 a-trous passes), the environment sampled through
+    the alias tables.  Returns per-stage ms per frame."""
     import cudaraytracer_amd as rt
     from cudaraytracer_amd import scenes
 
@@ -64,7 +65,7 @@ def frames_bench(frames: int = 10, width: int = 1280, height: int = 720):
     info = scene.info()
 
     # Wall-clock frame latency (synchronised per frame), then a profiled pass for stage times.
-    r = rt.Renderer(scene)
+    r = rt.Renderer(scene, env_alias=env_alias)
     base = r.camera()
     r.render_frame(1)
     r.synchronize()
@@ -77,7 +78,7 @@ def frames_bench(frames: int = 10, width: int = 1280, height: int = 720):
         lat.append((time.perf_counter() - t0) * 1e3)
     den = r.read_frame(rt.RT_FRAME_DENOISED)
     assert np.isfinite(den).all()
-    rp = rt.Renderer(scene, profile=True)
+    rp = rt.Renderer(scene, profile=True, env_alias=env_alias)
     rp.render_frame(1)
     rp.synchronize()
     rp.reset_stats()
@@ -96,7 +97,8 @@ def frames_bench(frames: int = 10, width: int = 1280, height: int = 720):
     }
     return {
         "workload": f"terrain_env {info.num_triangles} tris + SSS sphere + 2048x1024 env, {width}x{height}, "
-                    f"{frames}-frame orbit, 1 spp/frame, SVGF 5 a-trous passes",
+                    f"{frames}-frame orbit, 1 spp/frame, SVGF 5 a-trous passes, env sampling: "
+                    f"{'alias tables' if env_alias else 'CDF search'}",
         "frame_ms_p50": round(float(np.percentile(lat, 50)), 3),
         "frame_ms_max": round(float(np.max(lat)), 3),
         "stage_ms_per_frame": {k: round(v, 4) for k, v in ms.items()},
@@ -209,7 +211,7 @@ def serving_bench(rates=(5.0, 20.0, 100.0), n_jobs: int = 30, frames: int = 10, 
     base_cams = [rt.Renderer(s, batch_spp=1).camera() for s in sc]
     out = {"workload": f"Poisson arrivals of {n_jobs} jobs per rate; each a {frames}-frame orbit at "
                        f"{width}x{height}, 1 spp/frame, SVGF (5 a-trous); scenes: terrain_env 2.0M/1.0M/0.5M "
-                       "triangles + SSS sphere + 2048x1024 env", "rates": []}
+                       "triangles + SSS sphere + 2048x1024 env sampled by alias tables", "rates": []}
     warm = rt.Job(slots[2], spp=1, frames=2, cameras=[base_cams[2]] * 2)
     pool.serve([(0.0, warm)], batched=True)
     pool.serve([(0.0, warm)], batched=False)
@@ -224,7 +226,7 @@ def serving_bench(rates=(5.0, 20.0, 100.0), n_jobs: int = 30, frames: int = 10, 
             phi0 = float(rng.uniform(-20, 20))
             cams = [rt.Camera(tuple(c.target), c.distance, c.theta_deg, c.phi_deg + phi0 + 0.5 * f)
                     for f in range(frames)]
-            reqs.append((float(t), rt.Job(slots[s], spp=1, frames=frames, cameras=cams)))
+            reqs.append((float(t), rt.Job(slots[s], spp=1, frames=frames, cameras=cams, render={"env_sampling": 1})))
         row = {"rate_jobs_s": rate}
         for mode in ("batched", "plain"):
             rep, _, _ = pool.serve(reqs, batched=(mode == "batched"))

@@ -81,6 +81,8 @@ rt::RenderSettings settings_from(const rt_render_params* p, const rt::HostScene&
     set.queue_budget_bytes = p->queue_budget_bytes > 0 ? (size_t)p->queue_budget_bytes : 0;
     set.flush_survivors = p->flush_survivors;
     set.profile = p->profile;
+    if (p->env_sampling != 0 && p->env_sampling != 1) throw std::invalid_argument("env_sampling must be 0 or 1");
+    set.env_alias = p->env_sampling;
     if (set.max_bounces > 255) throw std::invalid_argument("max_bounces must be <= 255");
     return set;
 }
@@ -136,6 +138,7 @@ void rt_render_params_default(rt_render_params* p) {
     p->queue_budget_bytes = 0;
     p->flush_survivors = 0;
     p->profile = 0;
+    p->env_sampling = 0;
 }
 
 void rt_denoise_params_default(rt_denoise_params* p) {

@@ -27,6 +27,12 @@ struct DevScene {
     const float* env_row_cdf;     // env_h * env_w, per-row conditional CDF
     const float* env_marg_cdf;    // env_h, marginal CDF over rows
     const float* env_pdf;         // env_h * env_w, pdf in uv-space
+    // Alias tables over the same texel distribution (Walker/Vose): per row a
+    // conditional table of env_w entries, and a marginal table of env_h
+    // entries; entry = (acceptance threshold, alias index bits).
+    const float2* env_alias_row;
+    const float2* env_alias_marg;
+    int env_alias;                // per job: 1 = sample the environment with the alias tables
 };
 
 // tri_attr layout, 8 float4 = one 128-byte line per triangle:

@@ -75,6 +75,7 @@ struct PassItem {
     FrameParams fp{};
     int spp = 0;
     int flush_survivors = 0;
+    int env_alias = 0;
     float4* acc_d = nullptr;
     float4* acc_i = nullptr;
 };

@@ -1,5 +1,6 @@
-// Environment light: emission lookup and CDF importance sampling
-// (reference: Src/Core/texture.cu:17-166, Src/Core/light.cu:91-155).
+// Environment light: emission lookup and importance sampling by CDF binary
+// search (reference: Src/Core/texture.cu:17-166, Src/Core/light.cu:91-155) or,
+// as an opt-in fast mode, by alias tables over the same texel distribution.
 #pragma once
 
 #include "device.h"
@@ -80,11 +81,32 @@ RT_HD void env_sample_uv(const DevScene& sc, float rv, float ru, float& u, float
     v = ((float)vi + 0.5f) / (float)sc.env_h;
 }
 
+// O(1) pick from an alias table of n entries: cell floor(u n), accepted when
+// the fractional part is below the cell's threshold, else its alias.
+RT_HD int alias_pick(const float2* t, int n, float u) {
+    float x = u * (float)n;
+    int i = (int)x;
+    if (i > n - 1) i = n - 1;
+    float f = x - (float)i;
+    float2 e = t[i];
+    return f < e.x ? i : f_as_i(e.y);
+}
+
+// Alias-table sampling of the same texel distribution as env_sample_uv (row
+// from the marginal table, column from the row's table), at the texel centre.
+RT_HD void env_sample_uv_alias(const DevScene& sc, float rv, float ru, float& u, float& v) {
+    int vi = alias_pick(sc.env_alias_marg, sc.env_h, rv);
+    int ui = alias_pick(sc.env_alias_row + (size_t)vi * sc.env_w, sc.env_w, ru);
+    u = ((float)ui + 0.5f) / (float)sc.env_w;
+    v = ((float)vi + 0.5f) / (float)sc.env_h;
+}
+
 // Light::SampleEnvLight: a point on a sphere of radius 2*diag around the scene
 // centre, polar angle acos(1 - 2v), normal towards the centre.
 RT_HD void env_sample_point(const DevScene& sc, float rv, float ru, vec3& pos, vec3& nrm) {
     float u, v;
-    env_sample_uv(sc, rv, ru, u, v);
+    if (sc.env_alias) env_sample_uv_alias(sc, rv, ru, u, v);
+    else env_sample_uv(sc, rv, ru, u, v);
     float theta = (float)((double)(u * 2.f) * kPiD);
     float phi = fm::acosf_(1.f - 2.f * v);
     float x = fm::sinf_(phi) * fm::sinf_(theta);

@@ -165,6 +165,7 @@ int Pool::step() {
         it.fp = frame_params(cam, j.d.set, sample0, k);
         it.spp = k;
         it.flush_survivors = j.d.set.flush_survivors;
+        it.env_alias = j.d.set.env_alias;
         it.acc_d = j.acc_d.as<float4>();
         it.acc_i = j.acc_i.as<float4>();
         items.push_back(it);

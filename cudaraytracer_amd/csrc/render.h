@@ -51,6 +51,7 @@ struct RenderSettings {
     int batch_spp = 0;               // samples per pass (0: as many as the queue budget allows, up to 64)
     size_t queue_budget_bytes = 0;   // 0: 8 GiB
     int flush_survivors = 0;         // 1: paths alive after the last bounce keep their radiance
+    int env_alias = 0;               // 1: environment sampled with alias tables (not the reference's CDF search)
     int profile = 0;                 // per-stage HIP event timing
 };
 
@@ -63,7 +64,12 @@ struct RenderStats {
 
 struct EnvTables {
     std::vector<float> row_cdf, marg_cdf;
+    // Alias tables: (threshold, alias index bits) per entry.
+    std::vector<float> alias_row;   // 2 * env_w * env_h
+    std::vector<float> alias_marg;  // 2 * env_h
 };
+// Walker/Vose alias table of n weights: threshold and alias per entry.
+void build_alias_table(const double* w, int n, float* prob, int32_t* alias);
 void build_env_tables(const EnvMap& env, EnvTables& t);
 
 // Scene arrays resident in HBM.

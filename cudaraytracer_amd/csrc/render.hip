@@ -493,7 +493,8 @@ __global__ __launch_bounds__(kBlock) void k_combine(const float4* __restrict__ d
 // ======================================================================
 
 struct GpuScene::Impl {
-    DevBuf nodes, tri_isect, tri_attr, materials, lights, light_tris, env_rgb, env_row, env_marg;
+    DevBuf nodes, tri_isect, tri_attr, materials, lights, light_tris, env_rgb, env_row, env_marg, env_alias_row,
+        env_alias_marg;
     DevScene dev{};
     int stack = 32;
     bool has_class[kNumClasses] = {true, false, false, false, false};
@@ -589,6 +590,8 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
     d.env_rgb = nullptr;
     d.env_w = d.env_h = 0;
     d.env_row_cdf = d.env_marg_cdf = d.env_pdf = nullptr;
+    d.env_alias_row = d.env_alias_marg = nullptr;
+    d.env_alias = 0;
     if (sc.env_light >= 0) {
         EnvTables t;
         build_env_tables(sc.env, t);
@@ -600,6 +603,10 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
         d.env_h = sc.env.height;
         d.env_row_cdf = I.env_row.as<float>();
         d.env_marg_cdf = I.env_marg.as<float>();
+        upload(I.env_alias_row, t.alias_row.data(), t.alias_row.size() * sizeof(float));
+        upload(I.env_alias_marg, t.alias_marg.data(), t.alias_marg.size() * sizeof(float));
+        d.env_alias_row = I.env_alias_row.as<float2>();
+        d.env_alias_marg = I.env_alias_marg.as<float2>();
     }
 }
 
@@ -611,11 +618,53 @@ bool GpuScene::has_class(int c) const { return impl_->has_class[c]; }
 size_t GpuScene::device_bytes() const {
     const Impl& I = *impl_;
     return I.nodes.bytes + I.tri_isect.bytes + I.tri_attr.bytes + I.materials.bytes + I.lights.bytes +
-           I.light_tris.bytes + I.env_rgb.bytes + I.env_row.bytes + I.env_marg.bytes;
+           I.light_tris.bytes + I.env_rgb.bytes + I.env_row.bytes + I.env_marg.bytes + I.env_alias_row.bytes +
+           I.env_alias_marg.bytes;
 }
 
 // Row-conditional and marginal CDFs weighted by sin(theta) (reference:
 // Src/Core/texture.cu:38-80).
+void build_alias_table(const double* w, int n, float* prob, int32_t* alias) {
+    double total = 0.0;
+    for (int i = 0; i < n; ++i) total += w[i];
+    if (!(total > 0.0)) {
+        for (int i = 0; i < n; ++i) {
+            prob[i] = 1.f;
+            alias[i] = i;
+        }
+        return;
+    }
+    std::vector<double> q(n);
+    std::vector<int> small, large;
+    small.reserve(n);
+    large.reserve(n);
+    for (int i = 0; i < n; ++i) {
+        q[i] = (w[i] * (double)n) / total;
+        if (q[i] < 1.0) small.push_back(i);
+        else large.push_back(i);
+    }
+    while (!small.empty() && !large.empty()) {
+        int a = small.back(), b = large.back();
+        small.pop_back();
+        large.pop_back();
+        prob[a] = (float)q[a];
+        alias[a] = b;
+        q[b] = (q[b] + q[a]) - 1.0;
+        if (q[b] < 1.0) small.push_back(b);
+        else large.push_back(b);
+    }
+    while (!large.empty()) {
+        prob[large.back()] = 1.f;
+        alias[large.back()] = large.back();
+        large.pop_back();
+    }
+    while (!small.empty()) {
+        prob[small.back()] = 1.f;
+        alias[small.back()] = small.back();
+        small.pop_back();
+    }
+}
+
 void build_env_tables(const EnvMap& env, EnvTables& t) {
     int w = env.width, h = env.height;
     t.row_cdf.assign((size_t)w * h, 0.f);
@@ -639,6 +688,35 @@ void build_env_tables(const EnvMap& env, EnvTables& t) {
         for (int x = 0; x < w; ++x) {
             size_t idx = (size_t)y * w + x;
             t.row_cdf[idx] = rs > 0.0f ? t.row_cdf[idx] / rs : 0.0f;
+        }
+    }
+    // Alias tables over the same weights: texel luminance * sin(theta) per
+    // row, the row totals for the marginal.
+    {
+        t.alias_row.assign(2 * (size_t)w * h, 0.f);
+        t.alias_marg.assign(2 * (size_t)h, 0.f);
+        std::vector<double> wt(std::max(w, h));
+        std::vector<float> pr(std::max(w, h));
+        std::vector<int32_t> al(std::max(w, h));
+        for (int y = 0; y < h; ++y) {
+            float theta = (float)((double)(((float)y + 0.5f) / (float)h) * 3.1415926);
+            float st = sinf(theta);
+            for (int x = 0; x < w; ++x) {
+                const float* c = &env.rgb[3 * ((size_t)y * w + x)];
+                float lum = (0.2126f * c[0] + 0.7152f * c[1]) + 0.0722f * c[2];
+                wt[x] = (double)(lum * st);
+            }
+            build_alias_table(wt.data(), w, pr.data(), al.data());
+            for (int x = 0; x < w; ++x) {
+                t.alias_row[2 * ((size_t)y * w + x)] = pr[x];
+                std::memcpy(&t.alias_row[2 * ((size_t)y * w + x) + 1], &al[x], 4);
+            }
+        }
+        for (int y = 0; y < h; ++y) wt[y] = (double)t.marg_cdf[y];
+        build_alias_table(wt.data(), h, pr.data(), al.data());
+        for (int y = 0; y < h; ++y) {
+            t.alias_marg[2 * (size_t)y] = pr[y];
+            std::memcpy(&t.alias_marg[2 * (size_t)y + 1], &al[y], 4);
         }
     }
     for (int i = 1; i < h; ++i) t.marg_cdf[i] += t.marg_cdf[i - 1];
@@ -791,6 +869,7 @@ void Engine::run(const std::vector<PassItem>& items, bool count) {
         if (it.spp < 1) throw std::invalid_argument("pass item without samples");
         JobDev& J = jobs[k];
         J.sc = it.scene->dev();
+        J.sc.env_alias = it.env_alias && J.sc.env_alias_row ? 1 : 0;
         J.fp = it.fp;
         J.fp.n_paths = it.fp.wh * it.spp;
         J.fp.bounce = 0;
@@ -1155,6 +1234,7 @@ void Renderer::render(int spp, bool count) {
         it.fp = frame_params(I.cam, I.set, I.sample_base + (uint32_t)I.spp_done, k);
         it.spp = k;
         it.flush_survivors = I.set.flush_survivors;
+        it.env_alias = I.set.env_alias;
         it.acc_d = I.acc_d.as<float4>();
         it.acc_i = I.acc_i.as<float4>();
         I.eng->run({it}, count);

@@ -119,6 +119,7 @@ typedef struct rt_render_params {
     int64_t queue_budget_bytes; /* 0 = 8 GiB */
     int flush_survivors;    /* 0 = reference: paths alive after max_bounces contribute nothing */
     int profile;            /* 1 = per-stage HIP event timing in rt_stats */
+    int env_sampling;       /* 0 = CDF binary search (reference, texture.cu:112-153); 1 = alias tables */
 } rt_render_params;
 
 typedef struct rt_stats_out {

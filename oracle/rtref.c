@@ -308,6 +308,8 @@ struct ref_scene {
     float* env;       /* rgb */
     float* env_row;   /* row-conditional CDF */
     float* env_marg;  /* marginal CDF */
+    float* env_alias_row;   /* (threshold, alias bits) per texel, row tables */
+    float* env_alias_marg;  /* (threshold, alias bits) per row */
     Box bounds;
     int W, H;
     float fovy;
@@ -640,6 +642,84 @@ static void setup_camera(ref_scene* s) { /* scene.cpp:646-667 */
     apply_orbit(s, &o);
 }
 
+/* Walker's alias method with Vose's two worklists: scaled weights below 1
+   are paired, last in first out, with one at or above 1, which donates the
+   remainder and returns to the list its new weight falls in; leftovers get
+   threshold 1. */
+void ref_alias_build(const double* w, int n, float* prob, int32_t* alias) {
+    double total = 0.0, *q;
+    int *small, *large, ns = 0, nl = 0, i;
+    for (i = 0; i < n; ++i) total += w[i];
+    if (!(total > 0.0)) {
+        for (i = 0; i < n; ++i) {
+            prob[i] = 1.f;
+            alias[i] = i;
+        }
+        return;
+    }
+    q = (double*)malloc(sizeof(double) * (size_t)n);
+    small = (int*)malloc(sizeof(int) * (size_t)n);
+    large = (int*)malloc(sizeof(int) * (size_t)n);
+    for (i = 0; i < n; ++i) {
+        q[i] = (w[i] * (double)n) / total;
+        if (q[i] < 1.0) small[ns++] = i;
+        else large[nl++] = i;
+    }
+    while (ns > 0 && nl > 0) {
+        int a = small[--ns], b = large[--nl];
+        prob[a] = (float)q[a];
+        alias[a] = b;
+        q[b] = (q[b] + q[a]) - 1.0;
+        if (q[b] < 1.0) small[ns++] = b;
+        else large[nl++] = b;
+    }
+    while (nl > 0) {
+        int b = large[--nl];
+        prob[b] = 1.f;
+        alias[b] = b;
+    }
+    while (ns > 0) {
+        int a = small[--ns];
+        prob[a] = 1.f;
+        alias[a] = a;
+    }
+    free(q);
+    free(small);
+    free(large);
+}
+
+static void setup_env_alias(ref_scene* s, const float* row_totals) {
+    int w = s->env_w, h = s->env_h, x, y, n = w > h ? w : h;
+    double* wt = (double*)malloc(sizeof(double) * (size_t)n);
+    float* pr = (float*)malloc(sizeof(float) * (size_t)n);
+    int32_t* al = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    s->env_alias_row = (float*)calloc(2 * (size_t)w * h, sizeof(float));
+    s->env_alias_marg = (float*)calloc(2 * (size_t)h, sizeof(float));
+    for (y = 0; y < h; ++y) {
+        float theta = (float)((double)(((float)y + 0.5f) / (float)h) * PI_D);
+        float st = sinf(theta);
+        for (x = 0; x < w; ++x) {
+            const float* c = &s->env[3 * ((size_t)y * w + x)];
+            float lum = (0.2126f * c[0] + 0.7152f * c[1]) + 0.0722f * c[2];
+            wt[x] = (double)(lum * st);
+        }
+        ref_alias_build(wt, w, pr, al);
+        for (x = 0; x < w; ++x) {
+            s->env_alias_row[2 * ((size_t)y * w + x)] = pr[x];
+            memcpy(&s->env_alias_row[2 * ((size_t)y * w + x) + 1], &al[x], 4);
+        }
+    }
+    for (y = 0; y < h; ++y) wt[y] = (double)row_totals[y];
+    ref_alias_build(wt, h, pr, al);
+    for (y = 0; y < h; ++y) {
+        s->env_alias_marg[2 * (size_t)y] = pr[y];
+        memcpy(&s->env_alias_marg[2 * (size_t)y + 1], &al[y], 4);
+    }
+    free(wt);
+    free(pr);
+    free(al);
+}
+
 static void setup_env(ref_scene* s) { /* texture.cu:38-80 */
     int w = s->env_w, h = s->env_h, x, y;
     s->env_row = (float*)calloc((size_t)w * h, sizeof(float));
@@ -665,6 +745,7 @@ static void setup_env(ref_scene* s) { /* texture.cu:38-80 */
             s->env_row[idx] = rs > 0.0f ? s->env_row[idx] / rs : 0.0f;
         }
     }
+    setup_env_alias(s, s->env_marg); /* row totals, before the prefix sum */
     for (y = 1; y < h; ++y) s->env_marg[y] += s->env_marg[y - 1];
     {
         float total = h > 0 ? s->env_marg[h - 1] : 0.f;
@@ -768,7 +849,7 @@ void ref_scene_destroy(ref_scene* s) {
     free(s->tris); free(s->ordered); free(s->order); free(s->nodes);
     free(s->mesh_material); free(s->mesh_light); free(s->mesh_area); free(s->mesh_start); free(s->mesh_count);
     free(s->mats); free(s->light_type); free(s->light_mesh); free(s->light_em);
-    free(s->env); free(s->env_row); free(s->env_marg);
+    free(s->env); free(s->env_row); free(s->env_marg); free(s->env_alias_row); free(s->env_alias_marg);
     free(s);
 }
 
@@ -925,5 +1006,18 @@ void ref_env_sample(const ref_scene* s, float rv, float ru, float* uv_out) {
 }
 
 float ref_env_pdf_uv(const ref_scene* s, float u, float v) { return env_pdf_uv_f(s, u, v); }
+
+void ref_env_alias_tables(const ref_scene* s, float* row2, float* marg2) {
+    if (s->env_light < 0) return;
+    memcpy(row2, s->env_alias_row, sizeof(float) * 2 * (size_t)s->env_w * s->env_h);
+    memcpy(marg2, s->env_alias_marg, sizeof(float) * 2 * (size_t)s->env_h);
+}
+
+void ref_env_sample_alias(const ref_scene* s, float rv, float ru, float* uv_out) {
+    float u, v;
+    env_sample_uv_alias_f(s, rv, ru, &u, &v);
+    uv_out[0] = u;
+    uv_out[1] = v;
+}
 
 #include "rtref_svgf.inc"

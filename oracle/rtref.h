@@ -54,6 +54,7 @@ typedef struct ref_params {
     int rr_start_depth;
     uint32_t seed;
     int flush_survivors;
+    int env_alias; /* 1: environment sampled with alias tables instead of the CDF search */
 } ref_params;
 
 typedef struct ref_scene ref_scene;
@@ -138,6 +139,11 @@ float ref_sss_dist(float scatter_distance, float u);
 void ref_sss_probe(const ref_scene* s, const float* p, const float* n, float dist, float angle, float* out_p,
                    float* out_n);
 void ref_env_tables(const ref_scene* s, float* row_cdf, float* marg_cdf);
+/* Alias tables (threshold, alias index bits) per row entry (2*w*h floats) and per row (2*h). */
+void ref_env_alias_tables(const ref_scene* s, float* row2, float* marg2);
+void ref_env_sample_alias(const ref_scene* s, float rv, float ru, float* uv_out);
+/* Walker/Vose table of n double weights. */
+void ref_alias_build(const double* w, int n, float* prob, int32_t* alias);
 void ref_env_sample(const ref_scene* s, float rv, float ru, float* uv_out);
 float ref_env_pdf_uv(const ref_scene* s, float u, float v);
 

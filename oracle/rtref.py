@@ -42,7 +42,7 @@ class Desc(ctypes.Structure):
 
 class Params(ctypes.Structure):
     _fields_ = [("max_bounces", ctypes.c_int), ("rr", ctypes.c_float), ("rr_start_depth", ctypes.c_int),
-                ("seed", ctypes.c_uint32), ("flush_survivors", ctypes.c_int)]
+                ("seed", ctypes.c_uint32), ("flush_survivors", ctypes.c_int), ("env_alias", ctypes.c_int)]
 
 
 class Orbit(ctypes.Structure):
@@ -106,6 +106,9 @@ def lib():
         L.ref_sss_dist.restype = ctypes.c_float
         L.ref_sss_probe.argtypes = [vp, vp, vp, ctypes.c_float, ctypes.c_float, vp, vp]
         L.ref_env_tables.argtypes = [vp, vp, vp]
+        L.ref_env_alias_tables.argtypes = [vp, vp, vp]
+        L.ref_env_sample_alias.argtypes = [vp, ctypes.c_float, ctypes.c_float, vp]
+        L.ref_alias_build.argtypes = [vp, ctypes.c_int, vp, vp]
         L.ref_env_sample.argtypes = [vp, ctypes.c_float, ctypes.c_float, vp]
         L.ref_env_pdf_uv.argtypes = [vp, ctypes.c_float, ctypes.c_float]
         L.ref_env_pdf_uv.restype = ctypes.c_float
@@ -443,7 +446,8 @@ class RefScene:
         except Exception:
             pass
 
-    def params(self, max_bounces=None, rr=None, rr_start_depth=None, seed=None, flush_survivors=False) -> Params:
+    def params(self, max_bounces=None, rr=None, rr_start_depth=None, seed=None, flush_survivors=False,
+               env_alias=False) -> Params:
         a = self.a
         p = Params()
         p.max_bounces = a.max_bounces if max_bounces is None else max_bounces
@@ -451,6 +455,7 @@ class RefScene:
         p.rr_start_depth = a.rr_start_depth if rr_start_depth is None else rr_start_depth
         p.seed = a.seed if seed is None else seed
         p.flush_survivors = int(flush_survivors)
+        p.env_alias = int(env_alias)
         return p
 
     def bvh(self):
@@ -530,6 +535,19 @@ class RefScene:
 
     def env_pdf_uv(self, u: float, v: float) -> float:
         return float(lib().ref_env_pdf_uv(self._h, u, v))
+
+    def env_sample_alias(self, rv: float, ru: float):
+        uv = np.zeros(2, np.float32)
+        lib().ref_env_sample_alias(self._h, rv, ru, _p(uv))
+        return uv
+
+    def env_alias_tables(self):
+        """(row thresholds (h, w), row aliases (h, w), marginal thresholds (h,), marginal aliases (h,))."""
+        h, w = self.a.env_h, self.a.env_w
+        row = np.zeros((h, w, 2), np.float32)
+        marg = np.zeros((h, 2), np.float32)
+        lib().ref_env_alias_tables(self._h, _p(row), _p(marg))
+        return row[..., 0], row[..., 1].copy().view(np.int32), marg[:, 0], marg[:, 1].copy().view(np.int32)
 
     def env_tables(self):
         row = np.zeros((self.a.env_h, self.a.env_w), np.float32)
@@ -665,3 +683,12 @@ class RefFrames:
         self.frames += 1
         return {"acc_d": acc_d, "acc_i": acc_i, "gbuffer": g, "hist_d": hd, "hist_i": hi, "mom_d": md,
                 "mom_i": mi, "length": ln, "var_d": vd, "var_i": vi, "out_d": od, "out_i": oi, "denoised": den}
+
+
+def alias_build(weights):
+    """Walker/Vose alias table of the weights: (thresholds float32, aliases int32)."""
+    w = np.ascontiguousarray(weights, np.float64)
+    prob = np.zeros(len(w), np.float32)
+    alias = np.zeros(len(w), np.int32)
+    lib().ref_alias_build(_p(w), len(w), _p(prob), _p(alias))
+    return prob, alias

@@ -126,6 +126,24 @@ def test_render_reference_semantics_variants(scene_dir, oracle_lib):
         assert np.array_equal(_bits(i[:, :3]), _bits(ri[:, :3]))
 
 
+def test_env_alias_sampling_bit_exact(scene_dir, oracle_lib):
+    """The opt-in alias-table environment sampler against the oracle's, and
+    against the CDF sampler in expectation (same texel distribution and pdf)."""
+    path = scene_dir["terrain_small"]
+    a = oracle_lib.parse_scene(path)
+    rs = oracle_lib.RefScene(a)
+    d, i, st, _ = _gpu_render(path, 4, env_alias=True)
+    rd, ri, rst = rs.render(4, rs.params(env_alias=True))
+    assert np.array_equal(_bits(d[:, :3]), _bits(rd[:, :3]))
+    assert np.array_equal(_bits(i[:, :3]), _bits(ri[:, :3]))
+    assert st.shadow_rays == rst[1]
+    # Same estimator, different samples: the means over the image agree.
+    dc, ic, _, _ = _gpu_render(path, 64)
+    da, ia, _, _ = _gpu_render(path, 64, env_alias=True)
+    mc, ma = (dc + ic)[:, :3].mean(), (da + ia)[:, :3].mean()
+    assert abs(mc - ma) < 0.05 * mc
+
+
 def test_batching_does_not_change_results(scene_dir):
     path = scene_dir["cornell_spheres_128"]
     d1, i1, _, _ = _gpu_render(path, 6, batch_spp=1)

@@ -8,6 +8,9 @@ Src/Core/light.cu:91-155) and the subsurface branch's sampling pieces
   its texel centre; sweeping one uniform reproduces the row (marginal) and
   column (conditional) probabilities L sin(theta) / sum;
 * the uv-space pdf equals the texel probability times W*H and sums to one;
+* the alias tables (the opt-in fast sampler) hold exactly the same texel
+  probabilities (closed form over thresholds and aliases), and sampling them
+  reproduces the marginal;
 * the exit-distance sampler is the 50/50 mixture of Exp(d) and Exp(3d);
 * the two-sided probe lands on a plane at exactly the requested distance and
   on a sphere's surface, returning the surface's normal.
@@ -92,6 +95,64 @@ def test_env_pdf_is_texel_probability_times_area(env_base):
     pdf = np.array([[rs.env_pdf_uv((x + 0.5) / w, (y + 0.5) / h) for x in range(w)] for y in range(h)])
     np.testing.assert_allclose(pdf / (w * h), p, rtol=2e-5)
     assert abs(pdf.sum() / (w * h) - 1) < 1e-5
+
+
+def _alias_probs(prob, alias):
+    """Probability of each entry under an alias table (closed form)."""
+    n = len(prob)
+    out = prob.astype(np.float64).copy()
+    np.add.at(out, alias, 1.0 - prob.astype(np.float64))
+    return out / n
+
+
+def test_alias_build_closed_forms(oracle_lib):
+    prob, alias = rtref.alias_build([1.0, 3.0])
+    assert prob[0] == 0.5 and alias[0] == 1 and prob[1] == 1.0
+    prob, alias = rtref.alias_build(np.ones(7))
+    assert (prob == 1).all() and (alias == np.arange(7)).all()
+    prob, alias = rtref.alias_build([0.0, 0.0, 5.0, 0.0])
+    np.testing.assert_allclose(_alias_probs(prob, alias), [0, 0, 1, 0])
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 5, 64, 1000):
+        w = rng.random(n) ** 3
+        w[rng.random(n) < 0.2] = 0.0
+        if w.sum() == 0:
+            w[0] = 1.0
+        prob, alias = rtref.alias_build(w)
+        np.testing.assert_allclose(_alias_probs(prob, alias), w / w.sum(), atol=2e-7)
+
+
+def test_env_alias_tables_reproduce_texel_probabilities(env_base):
+    h, w = 16, 32
+    rng = np.random.default_rng(12)
+    env = (rng.random((h, w, 3)) ** 4 * 50).astype(np.float32)
+    env[3] = 0.0  # a black row is never picked
+    rs = _with_env(env_base, env)
+    p = _texel_prob(env)
+    rp, ra, mp, ma = rs.env_alias_tables()
+    np.testing.assert_allclose(_alias_probs(mp, ma), p.sum(1), atol=5e-7)
+    for y in range(h):
+        if p[y].sum() > 0:
+            np.testing.assert_allclose(_alias_probs(rp[y], ra[y]), p[y] / p[y].sum(), atol=5e-7)
+    # Sweeping one uniform reproduces the marginal; the texel centre is returned.
+    n = 4096
+    us = (np.arange(n) + 0.5) / n
+    rows = np.array([int(round(rs.env_sample_alias(float(r), 0.5)[1] * h - 0.5)) for r in us])
+    np.testing.assert_allclose(np.bincount(rows, minlength=h) / n, p.sum(1), atol=2.5 / n)
+    assert (rows != 3).all()
+    uv = rs.env_sample_alias(0.37, 0.81)
+    assert (uv * [w, h] - 0.5 == np.round(uv * [w, h] - 0.5)).all()
+
+
+def test_env_alias_single_bright_texel(env_base):
+    h, w, x0, y0 = 16, 32, 9, 12
+    env = np.zeros((h, w, 3), np.float32)
+    env[y0, x0] = (1.0, 2.0, 3.0)
+    rs = _with_env(env_base, env)
+    rng = np.random.default_rng(13)
+    for rv, ru in rng.random((200, 2)).astype(np.float32) * np.float32(0.999) + np.float32(1e-3):
+        u, v = rs.env_sample_alias(float(rv), float(ru))
+        assert u == np.float32((x0 + 0.5) / w) and v == np.float32((y0 + 0.5) / h)
 
 
 @pytest.mark.parametrize("d", [0.05, 1.0, 7.5])
