@@ -83,6 +83,7 @@ rt::RenderSettings settings_from(const rt_render_params* p, const rt::HostScene&
     set.profile = p->profile;
     if (p->env_sampling != 0 && p->env_sampling != 1) throw std::invalid_argument("env_sampling must be 0 or 1");
     set.env_alias = p->env_sampling;
+    set.use_graphs = p->disable_graphs ? 0 : 1;
     if (set.max_bounces > 255) throw std::invalid_argument("max_bounces must be <= 255");
     return set;
 }
@@ -139,6 +140,7 @@ void rt_render_params_default(rt_render_params* p) {
     p->flush_survivors = 0;
     p->profile = 0;
     p->env_sampling = 0;
+    p->disable_graphs = 0;
 }
 
 void rt_denoise_params_default(rt_denoise_params* p) {

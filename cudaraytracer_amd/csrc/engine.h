@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -95,15 +96,20 @@ public:
     void read_stats(unsigned long long out[kNumStats]);
     void reset_stats();
     Profiler prof;
+    // Replay passes as captured HIP graphs (off under profiling or counting).
+    void set_use_graphs(bool on) { use_graphs_ = on; }
 
 private:
     template <int STACK, bool POOLED>
-    void run_t(int n_items, int max_bounces, const bool* classes, int table, bool count);
+    void run_t(int max_bounces, const bool* classes, bool count);
+    bool use_graphs_ = true;
+    std::map<uint64_t, hipGraphExec_t> graphs_;
     size_t cap_ = 0;
     int region_ = 0;
     hipStream_t stream_ = nullptr;
     DevBuf qo_[2], qd_[2], qw_[2], hits_, lists_, shadow_, slot_d_, slot_i_, counters_, stats_;
-    // Pass tables: a ring of device tables filled from pinned host copies.
+    // Pass tables: one device table, filled by stream-ordered copies from a
+    // ring of pinned host tables.
     static constexpr int kRing = 8;
     DevBuf tables_;
     void* host_tables_ = nullptr;
@@ -129,6 +135,16 @@ public:
     void end(int spp, const SvgfParams& p, const float4* acc_d, const float4* acc_i, Profiler& prof,
              hipStream_t s);
     void reset_history() { frames_ = 0; }
+    // Make the state reusable for a new job: no history, nothing readable.
+    void recycle() {
+        frames_ = 0;
+        fcur_ = 0;
+        flast_ = -1;
+        out_sel_ = -1;
+        started_ = false;
+    }
+    int width() const { return w_; }
+    int height() const { return h_; }
     int frames() const { return frames_; }
     // Sample index of the next frame's first sample.
     uint32_t next_sample0(int spp) const { return (uint32_t)frames_ * (uint32_t)spp; }

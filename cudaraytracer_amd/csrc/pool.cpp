@@ -34,10 +34,20 @@ struct Pool::Impl {
         JobDesc d;
         JobStatus st;
         int wh = 0;
-        DevBuf acc_d, acc_i;
+        std::unique_ptr<DevBuf> acc_d, acc_i;
         std::unique_ptr<FrameState> fs;
         size_t bytes = 0;
     };
+    // Device state of released jobs, reused by later jobs of the same size
+    // (admission then needs no allocation).
+    std::multimap<int, std::unique_ptr<DevBuf>> spare_acc;                       // by W*H
+    std::multimap<std::pair<int, int>, std::unique_ptr<FrameState>> spare_fs;    // by (W, H)
+    size_t spare_bytes = 0;
+    void free_spares() {
+        spare_acc.clear();
+        spare_fs.clear();
+        spare_bytes = 0;
+    }
     std::map<int, std::unique_ptr<Job>> jobs;
     std::deque<int> active;  // submission order
     int next_id = 0;
@@ -112,21 +122,51 @@ size_t Pool::check(const JobDesc& d) const {
 int Pool::submit(const JobDesc& d) {
     Impl& I = *impl_;
     const size_t bytes = check(d);
-    if (I.scene_bytes + I.job_bytes + bytes > I.hbm_budget) return -1;
     const HostScene& h = *I.scenes[d.scene].host;
     const CameraRec& c0 = d.cams.empty() ? h.camera : d.cams[0];
     const size_t wh = (size_t)c0.width * c0.height;
+    // Bytes this job takes from spare state of the same size, and bytes it allocates.
+    size_t from_spare = 0;
+    if (I.spare_acc.count((int)wh) >= 2) from_spare += wh * 32;
+    if (d.frames > 0 && I.spare_fs.count({c0.width, c0.height}) > 0) from_spare += wh * kFrameStateBytesPerPixel;
+    if (I.scene_bytes + I.job_bytes + I.spare_bytes + (bytes - from_spare) > I.hbm_budget) {
+        I.free_spares();  // spare state of other sizes gives way first
+        from_spare = 0;
+    }
+    if (I.scene_bytes + I.job_bytes + I.spare_bytes + (bytes - from_spare) > I.hbm_budget) return -1;
     std::unique_ptr<Impl::Job> j(new Impl::Job);
     j->id = I.next_id++;
     j->d = d;
     if (j->d.cams.empty()) j->d.cams.push_back(h.camera);
     j->wh = (int)wh;
     j->bytes = bytes;
-    j->acc_d.alloc(wh * 16);
-    j->acc_i.alloc(wh * 16);
-    HIP_OK(hipMemsetAsync(j->acc_d.p, 0, wh * 16, I.eng->stream()));
-    HIP_OK(hipMemsetAsync(j->acc_i.p, 0, wh * 16, I.eng->stream()));
-    if (d.frames > 0) j->fs.reset(new FrameState(c0.width, c0.height));
+    auto take_acc = [&]() {
+        auto it = I.spare_acc.find((int)wh);
+        if (it != I.spare_acc.end()) {
+            std::unique_ptr<DevBuf> b = std::move(it->second);
+            I.spare_acc.erase(it);
+            I.spare_bytes -= wh * 16;
+            return b;
+        }
+        std::unique_ptr<DevBuf> b(new DevBuf);
+        b->alloc(wh * 16);
+        return b;
+    };
+    j->acc_d = take_acc();
+    j->acc_i = take_acc();
+    HIP_OK(hipMemsetAsync(j->acc_d->p, 0, wh * 16, I.eng->stream()));
+    HIP_OK(hipMemsetAsync(j->acc_i->p, 0, wh * 16, I.eng->stream()));
+    if (d.frames > 0) {
+        auto it = I.spare_fs.find({c0.width, c0.height});
+        if (it != I.spare_fs.end()) {
+            j->fs = std::move(it->second);
+            I.spare_fs.erase(it);
+            I.spare_bytes -= wh * kFrameStateBytesPerPixel;
+            j->fs->recycle();
+        } else {
+            j->fs.reset(new FrameState(c0.width, c0.height));
+        }
+    }
     j->st.submit_s = now();
     I.job_bytes += bytes;
     int id = j->id;
@@ -166,8 +206,8 @@ int Pool::step() {
         it.spp = k;
         it.flush_survivors = j.d.set.flush_survivors;
         it.env_alias = j.d.set.env_alias;
-        it.acc_d = j.acc_d.as<float4>();
-        it.acc_i = j.acc_i.as<float4>();
+        it.acc_d = j.acc_d->as<float4>();
+        it.acc_i = j.acc_i->as<float4>();
         items.push_back(it);
         served.push_back(&j);
         spp_of.push_back(k);
@@ -179,13 +219,13 @@ int Pool::step() {
         if (j->st.start_s < 0) j->st.start_s = t_start;
         if (j->d.frames > 0)
             j->fs->begin(*I.scenes[j->d.scene].gpu, *I.scenes[j->d.scene].host, j->d.cams[j->st.frames_done],
-                         j->acc_d.as<float4>(), j->acc_i.as<float4>(), I.eng->prof, s);
+                         j->acc_d->as<float4>(), j->acc_i->as<float4>(), I.eng->prof, s);
     }
     I.eng->run(items, false);
     for (size_t k = 0; k < served.size(); ++k) {
         Impl::Job* j = served[k];
         if (j->d.frames > 0)
-            j->fs->end(spp_of[k], j->d.svgf, j->acc_d.as<float4>(), j->acc_i.as<float4>(), I.eng->prof, s);
+            j->fs->end(spp_of[k], j->d.svgf, j->acc_d->as<float4>(), j->acc_i->as<float4>(), I.eng->prof, s);
     }
     HIP_OK(hipStreamSynchronize(s));
     const double t_done = now();
@@ -218,7 +258,7 @@ void Pool::job_size(int job, int& width, int& height) const {
 
 void Pool::read_accum(int job, int which, float* out) {
     Impl::Job& j = impl_->job(job);
-    read_accum_buffers(j.acc_d.as<float4>(), j.acc_i.as<float4>(), j.wh, j.st.spp_done, which, out,
+    read_accum_buffers(j.acc_d->as<float4>(), j.acc_i->as<float4>(), j.wh, j.st.spp_done, which, out,
                        impl_->eng->stream());
 }
 
@@ -235,6 +275,14 @@ void Pool::release(int job) {
     auto it = std::find(I.active.begin(), I.active.end(), job);
     if (it != I.active.end()) I.active.erase(it);
     I.job_bytes -= j.bytes;
+    I.spare_bytes += (size_t)j.wh * 32;
+    I.spare_acc.emplace(j.wh, std::move(j.acc_d));
+    I.spare_acc.emplace(j.wh, std::move(j.acc_i));
+    if (j.fs) {
+        I.spare_bytes += (size_t)j.wh * kFrameStateBytesPerPixel;
+        std::pair<int, int> key(j.fs->width(), j.fs->height());
+        I.spare_fs.emplace(key, std::move(j.fs));
+    }
     I.jobs.erase(job);
 }
 

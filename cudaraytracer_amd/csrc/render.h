@@ -52,6 +52,7 @@ struct RenderSettings {
     size_t queue_budget_bytes = 0;   // 0: 8 GiB
     int flush_survivors = 0;         // 1: paths alive after the last bounce keep their radiance
     int env_alias = 0;               // 1: environment sampled with alias tables (not the reference's CDF search)
+    int use_graphs = 1;              // replay passes as captured HIP graphs
     int profile = 0;                 // per-stage HIP event timing
 };
 

@@ -803,7 +803,7 @@ Engine::Engine(size_t capacity_paths, bool profile) {
     region_ = (int)((cap_ + kSlices - 1) / kSlices);
     counters_.alloc(kCounterInts * sizeof(int));
     stats_.alloc(kNumStats * sizeof(unsigned long long));
-    tables_.alloc(kPassTableBytes * kRing);
+    tables_.alloc(kPassTableBytes);
     HIP_OK(hipHostMalloc(&host_tables_, kPassTableBytes * kRing, hipHostMallocDefault));
     for (int k = 0; k < kRing; ++k) HIP_OK(hipEventCreateWithFlags(&ring_ev_[k], hipEventDisableTiming));
     HIP_OK(hipMemset(counters_.p, 0, kCounterInts * sizeof(int)));
@@ -832,6 +832,7 @@ Engine::~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (int k = 0; k < kRing; ++k)
         if (ring_ev_[k]) (void)hipEventDestroy(ring_ev_[k]);
+    for (auto& g : graphs_) (void)hipGraphExecDestroy(g.second);
     if (host_tables_) (void)hipHostFree(host_tables_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -891,22 +892,58 @@ void Engine::run(const std::vector<PassItem>& items, bool count) {
     hdr->n_paths = (int)paths;
     hdr->n_pixels = (int)pixels;
     hdr->pad = 0;
-    char* dev = tables_.as<char>() + kPassTableBytes * slot;
-    HIP_OK(hipMemcpyAsync(dev, host, kPassHeaderBytes + sizeof(JobDev) * n, hipMemcpyHostToDevice, stream_));
+    // One device table: stream order puts this copy after the previous pass's
+    // kernels, so every pass (and every replay of a captured pass) reads the
+    // same address; the host copy stays untouched until the copy has run.
+    HIP_OK(hipMemcpyAsync(tables_.p, host, kPassHeaderBytes + sizeof(JobDev) * n, hipMemcpyHostToDevice, stream_));
     HIP_OK(hipEventRecord(ring_ev_[slot], stream_));
-    if (n == 1) {
-        if (stack == 32) run_t<32, false>(n, max_bounces, classes, slot, count);
-        else run_t<64, false>(n, max_bounces, classes, slot, count);
-    } else {
-        if (stack == 32) run_t<32, true>(n, max_bounces, classes, slot, count);
-        else run_t<64, true>(n, max_bounces, classes, slot, count);
+    const bool pooled = n > 1;
+    const bool graph = use_graphs_ && !prof.on && !count;
+    uint64_t key = 0;
+    if (graph) {
+        int mask = 0;
+        for (int c = 0; c < kNumClasses; ++c) mask |= classes[c] ? (1 << c) : 0;
+        key = ((uint64_t)max_bounces << 16) | ((uint64_t)mask << 8) | ((uint64_t)(stack == 64) << 1) | (pooled ? 1 : 0);
+        auto it = graphs_.find(key);
+        if (it != graphs_.end()) {
+            HIP_OK(hipGraphLaunch(it->second, stream_));
+            return;
+        }
+        HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    }
+    try {
+        if (!pooled) {
+            if (stack == 32) run_t<32, false>(max_bounces, classes, count);
+            else run_t<64, false>(max_bounces, classes, count);
+        } else {
+            if (stack == 32) run_t<32, true>(max_bounces, classes, count);
+            else run_t<64, true>(max_bounces, classes, count);
+        }
+    } catch (...) {
+        if (graph) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+        }
+        throw;
+    }
+    if (graph) {
+        // The whole pass (raygen, bounce loop, resolve) becomes one graph,
+        // replayed for every later pass of the same shape.
+        hipGraph_t g = nullptr;
+        HIP_OK(hipStreamEndCapture(stream_, &g));
+        hipGraphExec_t exec = nullptr;
+        hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_OK(e);
+        graphs_[key] = exec;
+        HIP_OK(hipGraphLaunch(exec, stream_));
     }
 }
 
 template <int STACK, bool POOLED>
-void Engine::run_t(int n_items, int max_bounces, const bool* classes, int table, bool count) {
-    (void)n_items;
-    const char* dev = tables_.as<char>() + kPassTableBytes * table;
+void Engine::run_t(int max_bounces, const bool* classes, bool count) {
+    const char* dev = tables_.as<char>();
     const PassHeader* hdr = reinterpret_cast<const PassHeader*>(dev);
     const JobDev* jobs = reinterpret_cast<const JobDev*>(dev + kPassHeaderBytes);
     int* ctr = counters_.as<int>();
@@ -1205,6 +1242,7 @@ Renderer::Renderer(const GpuScene& scene, const HostScene& host, const RenderSet
     I.batch = std::max(1, std::min(want, bmax));
     if ((size_t)I.wh * I.batch > (size_t)0x7fffffff) I.batch = std::max(1, (int)(0x7fffffff / I.wh));
     I.eng.reset(new Engine((size_t)I.wh * I.batch, s.profile != 0));
+    I.eng->set_use_graphs(s.use_graphs != 0);
     I.acc_d.alloc((size_t)I.wh * 16);
     I.acc_i.alloc((size_t)I.wh * 16);
     HIP_OK(hipMemset(I.acc_d.p, 0, (size_t)I.wh * 16));

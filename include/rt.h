@@ -120,6 +120,7 @@ typedef struct rt_render_params {
     int flush_survivors;    /* 0 = reference: paths alive after max_bounces contribute nothing */
     int profile;            /* 1 = per-stage HIP event timing in rt_stats */
     int env_sampling;       /* 0 = CDF binary search (reference, texture.cu:112-153); 1 = alias tables */
+    int disable_graphs;     /* 1 = launch every kernel of a pass; 0 = replay passes as captured HIP graphs */
 } rt_render_params;
 
 typedef struct rt_stats_out {

@@ -144,6 +144,25 @@ def test_env_alias_sampling_bit_exact(scene_dir, oracle_lib):
     assert abs(mc - ma) < 0.05 * mc
 
 
+def test_graph_replays_match_eager_launches(scene_dir):
+    """Every pass after the first of a shape is a replay of the captured graph;
+    replays interleaved with other eager work and synchronisations must give
+    the eager launches' results, pass after pass."""
+    path = scene_dir["cornell_spheres_128"]
+    s = rt.Scene(path)
+    eager = rt.Renderer(s, batch_spp=1, graphs=False)
+    graph = rt.Renderer(s, batch_spp=1)
+    other = rt.Renderer(rt.Scene(scene_dir["cornell_small"]), batch_spp=2)
+    for k in range(5):
+        eager.render(1)
+        graph.render(1)
+        other.render(2)  # unrelated eager work between the replays
+        other.synchronize()
+        for which in (rt.RT_ACCUM_DIRECT, rt.RT_ACCUM_INDIRECT):
+            assert np.array_equal(_bits(graph.read_accum(which)), _bits(eager.read_accum(which))), (k, which)
+    assert graph.stats().extension_rays == eager.stats().extension_rays
+
+
 def test_batching_does_not_change_results(scene_dir):
     path = scene_dir["cornell_spheres_128"]
     d1, i1, _, _ = _gpu_render(path, 6, batch_spp=1)
