@@ -141,7 +141,8 @@ class DenoiseParams(ctypes.Structure):
 
 
 class PoolParams(ctypes.Structure):
-    _fields_ = [("queue_budget_bytes", ctypes.c_int64), ("hbm_budget_bytes", ctypes.c_int64), ("profile", ctypes.c_int)]
+    _fields_ = [("queue_budget_bytes", ctypes.c_int64), ("hbm_budget_bytes", ctypes.c_int64), ("profile", ctypes.c_int),
+                ("cohort_ticks", ctypes.c_int)]
 
 
 class JobDescC(ctypes.Structure):
@@ -417,8 +418,9 @@ class Job:
 class Pool:
     """Many jobs over resident scenes, served by one wavefront engine."""
 
-    def __init__(self, queue_budget_bytes: int = 0, hbm_budget_bytes: int = 0, profile: bool = False):
-        p = PoolParams(queue_budget_bytes, hbm_budget_bytes, int(profile))
+    def __init__(self, queue_budget_bytes: int = 0, hbm_budget_bytes: int = 0, profile: bool = False,
+                 pipelined: bool = False):
+        p = PoolParams(queue_budget_bytes, hbm_budget_bytes, int(profile), int(pipelined))
         self._h = ctypes.c_void_p()
         _check(lib().rt_pool_create(ctypes.byref(p), ctypes.byref(self._h)))
         self._scenes = []

@@ -390,6 +390,7 @@ int rt_pool_create(const rt_pool_params* p, rt_pool** out) {
             s.queue_budget_bytes = (size_t)p->queue_budget_bytes;
             s.hbm_budget_bytes = (size_t)p->hbm_budget_bytes;
             s.profile = p->profile;
+            s.pipeline = p->cohort_ticks ? 1 : 0;
         }
         std::unique_ptr<rt_pool> pool(new rt_pool);
         pool->p.reset(new rt::Pool(s));

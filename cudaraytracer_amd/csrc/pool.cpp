@@ -37,6 +37,10 @@ struct Pool::Impl {
         std::unique_ptr<DevBuf> acc_d, acc_i;
         std::unique_ptr<FrameState> fs;
         size_t bytes = 0;
+        // Pipelined real-time jobs: cohort (which ticks trace it) and phase
+        // (0 = frame not begun, 1 = G-buffer done, ready to trace, 2 = traced,
+        // SVGF pending).
+        int cohort = 0, phase = 0;
     };
     // Device state of released jobs, reused by later jobs of the same size
     // (admission then needs no allocation).
@@ -50,6 +54,9 @@ struct Pool::Impl {
     }
     std::map<int, std::unique_ptr<Job>> jobs;
     std::deque<int> active;  // submission order
+    hipStream_t side = nullptr;  // frame work (G-buffer, SVGF) beside the engine's pass
+    long tick = 0;
+    int cohort_size[2] = {0, 0};
     int next_id = 0;
     size_t scene_bytes = 0, job_bytes = 0, hbm_budget = 0;
     Clock::time_point t0;
@@ -67,13 +74,17 @@ Pool::Pool(const PoolSettings& s) : impl_(new Impl) {
     size_t qb = s.queue_budget_bytes ? s.queue_budget_bytes : ((size_t)8 << 30);
     I.eng.reset(new Engine(std::max<size_t>(1, qb / kQueueBytesPerPath), s.profile != 0));
     I.hbm_budget = s.hbm_budget_bytes ? s.hbm_budget_bytes : ((size_t)64 << 30);
+    HIP_OK(hipStreamCreateWithFlags(&I.side, hipStreamNonBlocking));
     I.t0 = Clock::now();
 }
 
 Pool::~Pool() {
     if (impl_) {
         if (impl_->eng) (void)hipStreamSynchronize(impl_->eng->stream());
+        if (impl_->side) (void)hipStreamSynchronize(impl_->side);
+        hipStream_t side = impl_->side;
         delete impl_;
+        if (side) (void)hipStreamDestroy(side);
     }
 }
 
@@ -167,6 +178,13 @@ int Pool::submit(const JobDesc& d) {
             j->fs.reset(new FrameState(c0.width, c0.height));
         }
     }
+    if (d.frames > 0) {
+        // The cohort whose frame work runs in the next tick begins soonest.
+        int first = (int)((I.tick & 1) ^ 1);
+        j->cohort = I.cohort_size[first] <= I.cohort_size[first ^ 1] ? first : first ^ 1;
+        I.cohort_size[j->cohort]++;
+        j->phase = 0;
+    }
     j->st.submit_s = now();
     I.job_bytes += bytes;
     int id = j->id;
@@ -175,7 +193,139 @@ int Pool::submit(const JobDesc& d) {
     return id;
 }
 
+namespace {
+PassItem pass_item(const GpuScene* gpu, const JobDesc& d, const CameraRec& cam, uint32_t sample0, int k,
+                   float4* acc_d, float4* acc_i) {
+    PassItem it;
+    it.scene = gpu;
+    it.fp = frame_params(cam, d.set, sample0, k);
+    it.spp = k;
+    it.flush_survivors = d.set.flush_survivors;
+    it.env_alias = d.set.env_alias;
+    it.acc_d = acc_d;
+    it.acc_i = acc_i;
+    return it;
+}
+}  // namespace
+
 int Pool::step() {
+    Impl& I = *impl_;
+    if (I.active.empty()) return 0;
+    if (I.set.pipeline) {
+        for (int guard = 0; guard < 4; ++guard) {
+            int n = step_pipelined();
+            if (n > 0) return n;
+        }
+        throw std::runtime_error("no active job fits the pool's path capacity");
+    }
+    return step_serial();
+}
+
+// Two cohorts of real-time jobs alternate: in a tick the engine's stream
+// traces one cohort's frames (and every accumulate job's next samples) while
+// the side stream finishes the other cohort's previous frames (temporal,
+// variance, a-trous) and starts their next ones (G-buffer, cleared
+// accumulators).  Path tracing (latency-bound traversal) and the filters
+// (ALU-bound) overlap on the device; each job still sees its frames in order.
+int Pool::step_pipelined() {
+    Impl& I = *impl_;
+    hipStream_t s = I.eng->stream();
+    const int c = (int)(I.tick & 1), o = c ^ 1;
+    const size_t cap = I.eng->capacity();
+    std::vector<PassItem> items;
+    std::vector<Impl::Job*> traced;
+    std::vector<int> spp_of;
+    std::vector<Impl::Job*> side_jobs;
+    size_t used = 0;
+    const double t_start = now();
+    for (int id : I.active) {
+        Impl::Job& j = I.job(id);
+        if (j.d.frames > 0 && j.cohort == o) {
+            if (j.phase == 2 || j.phase == 0) side_jobs.push_back(&j);
+            continue;
+        }
+        if (j.d.frames > 0 && j.phase == 0) {
+            side_jobs.push_back(&j);  // a cohort-c job not begun yet: begin now, trace on its next turn
+            continue;
+        }
+        if ((int)items.size() == kMaxPassJobs) continue;
+        if (j.d.frames > 0 && j.phase != 1) continue;
+        const Impl::SceneSlot& sc = I.scenes[j.d.scene];
+        int k;
+        uint32_t sample0;
+        const CameraRec* cam;
+        if (j.d.frames > 0) {
+            k = j.d.spp;
+            if (used + (size_t)j.wh * k > cap) continue;
+            sample0 = j.fs->next_sample0(k);
+            cam = &j.d.cams[j.st.frames_done];
+        } else {
+            int remaining = j.d.spp - j.st.spp_done;
+            size_t fit = (cap - used) / (size_t)j.wh;
+            k = (int)std::min<size_t>((size_t)remaining, fit);
+            if (k < 1) continue;
+            sample0 = (uint32_t)j.st.spp_done;
+            cam = &j.d.cams[0];
+        }
+        items.push_back(pass_item(sc.gpu, j.d, *cam, sample0, k, j.acc_d->as<float4>(), j.acc_i->as<float4>()));
+        traced.push_back(&j);
+        spp_of.push_back(k);
+        used += (size_t)j.wh * k;
+    }
+    // Side stream: finish traced frames of cohort o, begin the next ones.
+    for (Impl::Job* j : side_jobs) {
+        const Impl::SceneSlot& sc = I.scenes[j->d.scene];
+        if (j->phase == 2) {
+            j->fs->end(j->d.spp, j->d.svgf, j->acc_d->as<float4>(), j->acc_i->as<float4>(), I.eng->prof, I.side);
+            if (j->st.frames_done + 1 == j->d.frames) continue;
+            j->fs->begin(*sc.gpu, *sc.host, j->d.cams[j->st.frames_done + 1], j->acc_d->as<float4>(),
+                         j->acc_i->as<float4>(), I.eng->prof, I.side);
+        } else {
+            if (j->st.start_s < 0) j->st.start_s = t_start;
+            j->fs->begin(*sc.gpu, *sc.host, j->d.cams[j->st.frames_done], j->acc_d->as<float4>(),
+                         j->acc_i->as<float4>(), I.eng->prof, I.side);
+        }
+    }
+    if (!items.empty()) I.eng->run(items, false);
+    HIP_OK(hipStreamSynchronize(I.side));
+    HIP_OK(hipStreamSynchronize(s));
+    const double t_done = now();
+    I.tick++;
+    int served = 0;
+    for (Impl::Job* j : side_jobs) {
+        if (j->phase == 2) {
+            j->st.frames_done++;
+            j->st.spp_done = j->d.spp;
+            j->st.frame_done_s.push_back(t_done);
+            if (j->st.frames_done == j->d.frames) {
+                j->st.done = 1;
+                j->st.done_s = t_done;
+                I.cohort_size[j->cohort]--;
+                I.active.erase(std::find(I.active.begin(), I.active.end(), j->id));
+            }
+        }
+        j->phase = 1;
+        ++served;
+    }
+    for (size_t k = 0; k < traced.size(); ++k) {
+        Impl::Job* j = traced[k];
+        if (j->st.start_s < 0) j->st.start_s = t_start;
+        if (j->d.frames > 0) {
+            j->phase = 2;
+        } else {
+            j->st.spp_done += spp_of[k];
+            if (j->st.spp_done == j->d.spp) {
+                j->st.done = 1;
+                j->st.done_s = t_done;
+                I.active.erase(std::find(I.active.begin(), I.active.end(), j->id));
+            }
+        }
+        ++served;
+    }
+    return served;
+}
+
+int Pool::step_serial() {
     Impl& I = *impl_;
     if (I.active.empty()) return 0;
     hipStream_t s = I.eng->stream();
@@ -272,8 +422,10 @@ void Pool::release(int job) {
     Impl& I = *impl_;
     Impl::Job& j = I.job(job);
     HIP_OK(hipStreamSynchronize(I.eng->stream()));
+    HIP_OK(hipStreamSynchronize(I.side));
     auto it = std::find(I.active.begin(), I.active.end(), job);
     if (it != I.active.end()) I.active.erase(it);
+    if (j.fs && !j.st.done) I.cohort_size[j.cohort]--;
     I.job_bytes -= j.bytes;
     I.spare_bytes += (size_t)j.wh * 32;
     I.spare_acc.emplace(j.wh, std::move(j.acc_d));

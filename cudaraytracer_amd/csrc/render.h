@@ -138,6 +138,7 @@ struct PoolSettings {
     size_t queue_budget_bytes = 0;  // path queues; 0: 8 GiB
     size_t hbm_budget_bytes = 0;    // resident scenes + job state; 0: 64 GiB
     int profile = 0;
+    int pipeline = 0;               // 1: two cohorts, one traced while the other is filtered
 };
 
 struct JobDesc {
@@ -171,7 +172,7 @@ public:
     int submit(const JobDesc& d);
     const HostScene& scene_host(int slot) const;
     const GpuScene& scene_gpu(int slot) const;
-    // One pass over the active jobs (synchronous); returns the number of jobs served.
+    // One tick over the active jobs (synchronous); returns the number of jobs served.
     int step();
     bool idle() const;
     const JobStatus& status(int job) const;
@@ -187,6 +188,8 @@ public:
     struct Impl;
 
 private:
+    int step_serial();
+    int step_pipelined();
     Impl* impl_;
 };
 

@@ -205,6 +205,9 @@ typedef struct rt_pool_params {
     int64_t queue_budget_bytes; /* path queues of the shared engine; 0 = 8 GiB */
     int64_t hbm_budget_bytes;   /* resident scenes + job state admitted at once; 0 = 64 GiB */
     int profile;                /* 1 = per-stage HIP event timing in rt_pool_stats */
+    int cohort_ticks;           /* 0 = a tick traces and filters every real-time job on one stream;
+                                   1 = two cohorts alternate, one traced while the other is filtered
+                                   on a second stream (a real-time job advances every second tick) */
 } rt_pool_params;
 
 typedef struct rt_job_desc {
@@ -241,8 +244,9 @@ void rt_pool_destroy(rt_pool* p);
 int rt_pool_add_scene(rt_pool* p, rt_scene* s, int* slot_out);
 /* Admit a job; RT_ERR_BUSY when its state does not fit the HBM budget now. */
 int rt_pool_submit(rt_pool* p, const rt_job_desc* d, int* job_out);
-/* One pass over the active jobs (synchronous): each accumulate job that fits gets as many
-   samples as fit, each real-time job one frame.  served_out: jobs in the pass (0 = idle). */
+/* One tick over the active jobs (synchronous): each accumulate job that fits gets as many
+   samples as fit; real-time jobs are traced or filtered (see serial_ticks).  served_out: jobs
+   that progressed (0 = idle). */
 int rt_pool_step(rt_pool* p, int* served_out);
 int rt_pool_job_status(rt_pool* p, int job, rt_job_status* out);
 int rt_pool_read_accum(rt_pool* p, int job, int which, float* rgba_out);

@@ -66,7 +66,8 @@ FRAME_BUFFERS = [rt.RT_FRAME_DENOISED, rt.RT_FRAME_HISTORY_DIRECT, rt.RT_FRAME_H
                  rt.RT_FRAME_DIRECT, rt.RT_FRAME_INDIRECT, rt.RT_FRAME_G_MOTION_DEPTH, rt.RT_FRAME_G_NORMAL_MAT]
 
 
-def test_pool_realtime_jobs_match_alone(scene_dir):
+@pytest.mark.parametrize("pipelined", [True, False])
+def test_pool_realtime_jobs_match_alone(scene_dir, pipelined):
     names = ["cornell_spheres_rt", "terrain_small"]
     nframes = 4
     scenes = {n: rt.Scene(scene_dir[n]) for n in names}
@@ -81,21 +82,28 @@ def test_pool_realtime_jobs_match_alone(scene_dir):
             r.render_frame(1)
             frames.append({w: r.read_frame(w) for w in FRAME_BUFFERS})
         alone[n] = frames
-    # In the mix: both real-time jobs plus an accumulate job share every pass.
-    pool = rt.Pool()
+    # In the mix: both real-time jobs plus an accumulate job share the passes.
+    pool = rt.Pool(pipelined=pipelined)
     slots = {n: pool.add_scene(scenes[n]) for n in names}
     extra = rt.Scene(scene_dir["cornell_small"])
     jobs = {n: pool.submit(rt.Job(slots[n], spp=1, frames=nframes, cameras=paths[n])) for n in names}
     acc_job = pool.submit(rt.Job(pool.add_scene(extra), spp=6))
-    for f in range(nframes):
-        served = pool.step()
-        assert served >= len(names)
+    seen = {n: 0 for n in names}
+    for _ in range(4 * nframes + 4):
+        assert pool.step() > 0
         for n in names:
             info = scenes[n].info()
-            assert pool.status(jobs[n]).frames_done == f + 1
-            for w in FRAME_BUFFERS:
-                got = pool.read_frame(jobs[n], w, info.width, info.height)
-                _bits_equal(got, alone[n][f][w], f"{n} frame {f} buffer {w}")
+            done = pool.status(jobs[n]).frames_done
+            assert done <= seen[n] + 1
+            if done == seen[n] + 1:  # a frame completed in this tick: check it
+                f = seen[n]
+                for w in FRAME_BUFFERS:
+                    got = pool.read_frame(jobs[n], w, info.width, info.height)
+                    _bits_equal(got, alone[n][f][w], f"{n} frame {f} buffer {w}")
+                seen[n] = done
+        if all(v == nframes for v in seen.values()):
+            break
+    assert all(v == nframes for v in seen.values())
     pool.run()
     assert pool.status(acc_job).done == 1
     d_alone, _, _ = _alone_accum(extra, 6)
