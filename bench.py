@@ -108,6 +108,57 @@ def frames_bench(frames: int = 10, width: int = 1280, height: int = 720, env_ali
     }
 
 
+def c3_bench(env_alias: bool = True):
+    """The large-mesh workload: 2 M-triangle fBm terrain + random-walk-parameter
+    SSS sphere under a 2048x1024 HDR sky (alias-table sampled), 1920x1080,
+    32 spp, depth 16, one full render; Mrays/s and the extend kernel's
+    algorithmic HBM rate from counted traversal."""
+    import cudaraytracer_amd as rt
+
+    tmp = tempfile.mkdtemp(prefix="rtc3_")
+    path = _generate([("terrain_env", {}, tmp, "terrain_c3")])[0]
+    scene = rt.Scene(path)
+    info = scene.info()
+    spp = info.spp
+    r = rt.Renderer(scene, env_alias=env_alias)
+    r.render(2)
+    r.synchronize()
+    r.reset()
+    r.reset_stats()
+    t0 = time.perf_counter()
+    r.render(spp)
+    r.synchronize()
+    dt = time.perf_counter() - t0
+    st = r.stats()
+    rays = st.extension_rays + st.shadow_rays
+    rp = rt.Renderer(scene, profile=True, env_alias=env_alias)
+    rp.render(spp)
+    sp = rp.stats()
+    rc = rt.Renderer(scene, env_alias=env_alias)
+    rc.render(spp, count_traversal=True)
+    sc = rc.stats()
+    ext_bytes = (sc.extension_rays * EXT_RAY_BYTES + sc.extension_node_visits * NODE_BYTES
+                 + sc.extension_tri_tests * TRI_BYTES)
+    sh_bytes = (sc.shadow_rays * SHADOW_RAY_BYTES + sc.shadow_node_visits * NODE_BYTES
+                + sc.shadow_tri_tests * TRI_BYTES)
+    ext = _stage_roofline("k_extend", ext_bytes / sp.stage_launches[1], sp.stage_ms[1] / sp.stage_launches[1])
+    sh = _stage_roofline("k_shadow", sh_bytes / sp.stage_launches[3], sp.stage_ms[3] / sp.stage_launches[3])
+    return {
+        "workload": f"terrain_env {info.num_triangles} tris + SSS sphere + 2048x1024 env "
+                    f"({'alias tables' if env_alias else 'CDF search'}), {info.width}x{info.height}, {spp} spp, "
+                    f"depth {info.max_bounces}",
+        "mrays_s": round(rays / dt / 1e6, 1),
+        "ms": round(dt * 1e3, 2),
+        "rays": int(rays),
+        "batch_spp": st.batch_spp,
+        "nodes_per_ext_ray": round(sc.extension_node_visits / max(1, sc.extension_rays), 2),
+        "tris_per_ext_ray": round(sc.extension_tri_tests / max(1, sc.extension_rays), 2),
+        "stage_ms": {n: round(sp.stage_ms[i], 3) for i, n in enumerate(rt.STAGES) if sp.stage_launches[i]},
+        "extend_roofline": {k: ext[k] for k in ("bound", "achieved", "peak", "unit", "frac")},
+        "shadow_roofline": {k: sh[k] for k in ("bound", "achieved", "peak", "unit", "frac")},
+    }
+
+
 def _write_scene_job(job):
     """Worker: build one generated scene and write it (JSON + OBJ + PFM)."""
     kind, kwargs, out_dir, name = job
@@ -167,8 +218,11 @@ def batched_bench(n: int = 64, seed: int = 4, queue_gib: int = 16):
     reqs = [(0.0, rt.Job(slot, spp=spp)) for slot, (_, spp) in zip(slots, meta)]
     tris_total = sum(s.info().num_triangles for s in sc)
     paths_total = sum(w * w * spp for w, spp in meta)
-    # Warm-up: one small job through each mode (kernel loading, allocator).
-    pool.serve([reqs[0]], batched=True)
+    # Autotune the pool on a sample of the mix (every 8th job, at most 4 spp),
+    # which also warms up kernels and graphs; then one job through the plain loop.
+    sample = [rt.Job(slot, spp=min(spp, 4)) for slot, (_, spp) in list(zip(slots, meta))[::8]]
+    tune = pool.autotune(sample)
+    print(f"[bench] batched autotune: {tune}", file=sys.stderr, flush=True)
     pool.serve([reqs[0]], batched=False)
     pool.reset_stats()
     rep_b, _, job_b = pool.serve(reqs, batched=True)
@@ -190,6 +244,7 @@ def batched_bench(n: int = 64, seed: int = 4, queue_gib: int = 16):
                   "mrays_s": round(rays / rep_p["makespan_s"] / 1e6, 1)},
         "speedup": round(rep_b["jobs_per_s"] / rep_p["jobs_per_s"], 3),
         "queue_gib": queue_gib,
+        "autotune": tune,
     }
 
 
@@ -212,8 +267,14 @@ def serving_bench(rates=(5.0, 20.0, 100.0), n_jobs: int = 30, frames: int = 10, 
     out = {"workload": f"Poisson arrivals of {n_jobs} jobs per rate; each a {frames}-frame orbit at "
                        f"{width}x{height}, 1 spp/frame, SVGF (5 a-trous); scenes: terrain_env 2.0M/1.0M/0.5M "
                        "triangles + SSS sphere + 2048x1024 env sampled by alias tables", "rates": []}
+    sample = []
+    for k in range(6):
+        c = base_cams[k % len(sc)]
+        cams = [rt.Camera(tuple(c.target), c.distance, c.theta_deg, c.phi_deg + 0.5 * f) for f in range(3)]
+        sample.append(rt.Job(slots[k % len(sc)], spp=1, frames=3, cameras=cams, render={"env_sampling": 1}))
+    out["autotune"] = pool.autotune(sample)
+    print(f"[bench] serving autotune: {out['autotune']}", file=sys.stderr, flush=True)
     warm = rt.Job(slots[2], spp=1, frames=2, cameras=[base_cams[2]] * 2)
-    pool.serve([(0.0, warm)], batched=True)
     pool.serve([(0.0, warm)], batched=False)
     for rate in rates:
         rng = np.random.default_rng(seed + int(rate))
@@ -268,6 +329,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=None, help="HBM bytes per launch measured with rocprofv3 --pmc")
     ap.add_argument("--no-frames", action="store_true", help="skip the real-time SVGF frame-loop measurement")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 2 M-triangle terrain render")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched-vs-plain measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the Poisson serving measurement")
     args = ap.parse_args()
@@ -365,6 +427,10 @@ def main() -> int:
     if not args.no_frames:
         print("[bench] real-time frame loop (SVGF)", file=sys.stderr, flush=True)
         out["frames"] = frames_bench()
+    if not args.no_c3:
+        print("[bench] C3: 2 M-triangle terrain, 1920x1080, 32 spp", file=sys.stderr, flush=True)
+        out["c3"] = c3_bench()
+        print(f"[bench] c3: {out['c3']}", file=sys.stderr, flush=True)
     if not args.no_batched:
         print("[bench] batched: 64 uneven scenes, pool vs one at a time", file=sys.stderr, flush=True)
         out["batched"] = batched_bench()

@@ -65,7 +65,7 @@ EXPORTS = (
     "rt_denoise_params_default", "rt_get_camera", "rt_set_camera", "rt_render_frame", "rt_reset_history",
     "rt_read_frame", "rt_pool_create", "rt_pool_destroy", "rt_pool_add_scene", "rt_pool_submit", "rt_pool_step",
     "rt_pool_job_status", "rt_pool_read_accum", "rt_pool_read_frame", "rt_pool_release", "rt_pool_stats",
-    "rt_pool_reset_stats", "rt_pool_serve",
+    "rt_pool_reset_stats", "rt_pool_serve", "rt_pool_set_tuning", "rt_pool_get_tuning", "rt_pool_autotune",
 )
 RT_ERR_BUSY = -7
 
@@ -156,6 +156,22 @@ class JobStatus(ctypes.Structure):
                 ("submit_s", ctypes.c_double), ("start_s", ctypes.c_double), ("done_s", ctypes.c_double)]
 
 
+class PoolTuning(ctypes.Structure):
+    _fields_ = [("grid_scale", ctypes.c_float), ("pass_paths", ctypes.c_int64), ("tile_width", ctypes.c_int)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("best", PoolTuning), ("baseline_ms", ctypes.c_double), ("best_ms", ctypes.c_double),
+                ("runs", ctypes.c_int)]
+
+    def as_dict(self) -> dict:
+        return {"best": self.best.as_dict(), "baseline_ms": self.baseline_ms, "best_ms": self.best_ms,
+                "runs": self.runs}
+
+
 class ServeRequestC(ctypes.Structure):
     _fields_ = [("arrival_s", ctypes.c_double), ("job", JobDescC)]
 
@@ -222,6 +238,9 @@ def lib() -> ctypes.CDLL:
     L.rt_pool_release.argtypes = [vp, ip]
     L.rt_pool_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     L.rt_pool_reset_stats.argtypes = [vp]
+    L.rt_pool_set_tuning.argtypes = [vp, ctypes.POINTER(PoolTuning)]
+    L.rt_pool_get_tuning.argtypes = [vp, ctypes.POINTER(PoolTuning)]
+    L.rt_pool_autotune.argtypes = [vp, ctypes.POINTER(JobDescC), ip, ctypes.POINTER(TuneReport)]
     L.rt_pool_serve.argtypes = [vp, ctypes.POINTER(ServeRequestC), ip, ip, ctypes.POINTER(ServeReport),
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     _LIB = L
@@ -485,6 +504,22 @@ class Pool:
 
     def reset_stats(self) -> None:
         _check(lib().rt_pool_reset_stats(self._h))
+
+    def tuning(self) -> PoolTuning:
+        t = PoolTuning()
+        _check(lib().rt_pool_get_tuning(self._h, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, grid_scale: float = 1.0, pass_paths: int = 0, tile_width: int = 16) -> None:
+        t = PoolTuning(grid_scale, pass_paths, tile_width)
+        _check(lib().rt_pool_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, sample_jobs) -> dict:
+        keep: list = []
+        arr = (JobDescC * len(sample_jobs))(*[j.to_c(keep) for j in sample_jobs])
+        rep = TuneReport()
+        _check(lib().rt_pool_autotune(self._h, arr, len(sample_jobs), ctypes.byref(rep)))
+        return rep.as_dict()
 
     def serve(self, requests, batched: bool = True):
         """requests: list of (arrival_s, Job).  Returns (report dict, per-request frame

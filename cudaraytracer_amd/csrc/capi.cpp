@@ -501,6 +501,52 @@ int rt_pool_reset_stats(rt_pool* p) {
     });
 }
 
+namespace {
+int tile_shift_of(int width) {
+    for (int s = 3; s <= 7; ++s)
+        if ((1 << s) == width) return s;
+    throw std::invalid_argument("tile_width must be 8, 16, 32, 64 or 128");
+}
+void tuning_out(const rt::Pool::Tuning& t, rt_pool_tuning* o) {
+    o->grid_scale = t.grid_scale;
+    o->pass_paths = (int64_t)t.pass_paths;
+    o->tile_width = 1 << t.tile_shift;
+}
+}  // namespace
+
+int rt_pool_set_tuning(rt_pool* p, const rt_pool_tuning* t) {
+    if (!p || !t) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return guard([&]() {
+        if (t->pass_paths < 0) throw std::invalid_argument("pass_paths must be >= 0");
+        rt::Pool::Tuning tu;
+        tu.grid_scale = t->grid_scale;
+        tu.pass_paths = (size_t)t->pass_paths;
+        tu.tile_shift = tile_shift_of(t->tile_width);
+        p->p->set_tuning(tu);
+        return RT_OK;
+    });
+}
+
+int rt_pool_get_tuning(rt_pool* p, rt_pool_tuning* out) {
+    if (!p || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
+    tuning_out(p->p->tuning(), out);
+    return RT_OK;
+}
+
+int rt_pool_autotune(rt_pool* p, const rt_job_desc* sample, int n, rt_tune_report* out) {
+    if (!p || !sample || n < 1 || !out) return fail(RT_ERR_INVALID_ARG, "invalid argument");
+    return guard([&]() {
+        std::vector<rt::JobDesc> jobs;
+        for (int k = 0; k < n; ++k) jobs.push_back(job_from(p, &sample[k]));
+        rt::Pool::TuneReport r = p->p->autotune(jobs);
+        tuning_out(r.best, &out->best);
+        out->baseline_ms = r.baseline_ms;
+        out->best_ms = r.best_ms;
+        out->runs = r.runs;
+        return RT_OK;
+    });
+}
+
 int rt_pool_serve(rt_pool* p, const rt_serve_request* reqs, int n, int batched, rt_serve_report* out,
                   double* frame_latency_ms, double* job_latency_ms) {
     if (!p || (!reqs && n > 0) || n < 0 || !out) return fail(RT_ERR_INVALID_ARG, "invalid argument");

@@ -98,11 +98,15 @@ public:
     Profiler prof;
     // Replay passes as captured HIP graphs (off under profiling or counting).
     void set_use_graphs(bool on) { use_graphs_ = on; }
+    // Persistent stage grids = occupancy x CUs x scale (a tuning knob).
+    void set_grid_scale(float scale);
+    float grid_scale() const { return grid_scale_; }
 
 private:
     template <int STACK, bool POOLED>
     void run_t(int max_bounces, const bool* classes, bool count);
     bool use_graphs_ = true;
+    float grid_scale_ = 1.f;
     std::map<uint64_t, hipGraphExec_t> graphs_;
     size_t cap_ = 0;
     int region_ = 0;
@@ -145,6 +149,11 @@ public:
     }
     int width() const { return w_; }
     int height() const { return h_; }
+    // Tile shape of the per-pixel kernels: 2^shift x (256 >> shift), shift in 3..7.
+    void set_tile_shift(int shift) {
+        if (shift < 3 || shift > 7) throw std::invalid_argument("tile shift must be in 3..7");
+        tile_shift_ = shift;
+    }
     int frames() const { return frames_; }
     // Sample index of the next frame's first sample.
     uint32_t next_sample0(int spp) const { return (uint32_t)frames_ * (uint32_t)spp; }
@@ -159,6 +168,7 @@ private:
     DevBuf guide_nd_, guide_kl_;  // this frame's filter guide
     CameraRec last_cam_;
     int fcur_ = 0, flast_ = -1, frames_ = 0, out_sel_ = -1;
+    int tile_shift_ = 4;
     bool started_ = false;
 };
 

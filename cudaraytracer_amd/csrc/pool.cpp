@@ -55,6 +55,7 @@ struct Pool::Impl {
     std::map<int, std::unique_ptr<Job>> jobs;
     std::deque<int> active;  // submission order
     hipStream_t side = nullptr;  // frame work (G-buffer, SVGF) beside the engine's pass
+    Pool::Tuning tune;
     long tick = 0;
     int cohort_size[2] = {0, 0};
     int next_id = 0;
@@ -177,6 +178,7 @@ int Pool::submit(const JobDesc& d) {
         } else {
             j->fs.reset(new FrameState(c0.width, c0.height));
         }
+        j->fs->set_tile_shift(I.tune.tile_shift);
     }
     if (d.frames > 0) {
         // The cohort whose frame work runs in the next tick begins soonest.
@@ -194,6 +196,20 @@ int Pool::submit(const JobDesc& d) {
 }
 
 namespace {
+// Paths per pass: the tuned limit, but never below the largest single unit of
+// work of an active job (one frame, or one sample of every pixel), so every
+// job can be scheduled.
+size_t pass_limit(Pool::Impl& I) {
+    size_t cap = I.eng->capacity();
+    if (!I.tune.pass_paths || I.tune.pass_paths >= cap) return cap;
+    size_t unit = 0;
+    for (int id : I.active) {
+        const Pool::Impl::Job& j = I.job(id);
+        unit = std::max(unit, (size_t)j.wh * (size_t)(j.d.frames > 0 ? j.d.spp : 1));
+    }
+    return std::min(cap, std::max(I.tune.pass_paths, unit));
+}
+
 PassItem pass_item(const GpuScene* gpu, const JobDesc& d, const CameraRec& cam, uint32_t sample0, int k,
                    float4* acc_d, float4* acc_i) {
     PassItem it;
@@ -231,7 +247,7 @@ int Pool::step_pipelined() {
     Impl& I = *impl_;
     hipStream_t s = I.eng->stream();
     const int c = (int)(I.tick & 1), o = c ^ 1;
-    const size_t cap = I.eng->capacity();
+    const size_t cap = pass_limit(I);
     std::vector<PassItem> items;
     std::vector<Impl::Job*> traced;
     std::vector<int> spp_of;
@@ -329,7 +345,7 @@ int Pool::step_serial() {
     Impl& I = *impl_;
     if (I.active.empty()) return 0;
     hipStream_t s = I.eng->stream();
-    const size_t cap = I.eng->capacity();
+    const size_t cap = pass_limit(I);
     std::vector<PassItem> items;
     std::vector<Impl::Job*> served;
     std::vector<int> spp_of;
@@ -457,6 +473,84 @@ void Pool::stats(RenderStats& out) {
 }
 
 void Pool::reset_stats() { impl_->eng->reset_stats(); }
+
+void Pool::set_tuning(const Tuning& t) {
+    Impl& I = *impl_;
+    if (t.tile_shift < 3 || t.tile_shift > 7) throw std::invalid_argument("tile shift must be in 3..7");
+    I.eng->set_grid_scale(t.grid_scale);
+    I.tune = t;
+    for (auto& kv : I.jobs)
+        if (kv.second->fs) kv.second->fs->set_tile_shift(t.tile_shift);
+}
+
+Pool::Tuning Pool::tuning() const { return impl_->tune; }
+
+Pool::TuneReport Pool::autotune(const std::vector<JobDesc>& sample) {
+    Impl& I = *impl_;
+    if (!I.active.empty()) throw std::invalid_argument("autotune needs an idle pool");
+    if (sample.empty()) throw std::invalid_argument("autotune needs sample jobs");
+    size_t sample_paths = 0;
+    bool frames = false;
+    for (const auto& d : sample) {
+        check(d);
+        const CameraRec& c = d.cams.empty() ? I.scenes[d.scene].host->camera : d.cams[0];
+        sample_paths += (size_t)c.width * c.height * d.spp * (size_t)std::max(1, d.frames);
+        frames = frames || d.frames > 0;
+    }
+    TuneReport rep;
+    auto run = [&](const Tuning& t) {
+        set_tuning(t);
+        double best = 1e30;
+        for (int rep_i = 0; rep_i < 2; ++rep_i) {
+            HIP_OK(hipDeviceSynchronize());
+            const double t0 = now();
+            std::vector<int> ids;
+            for (const auto& d : sample) {
+                int id = submit(d);
+                if (id < 0) throw std::invalid_argument("the autotune sample does not fit the HBM budget");
+                ids.push_back(id);
+            }
+            while (step() > 0) {
+            }
+            best = std::min(best, (now() - t0) * 1e3);
+            for (int id : ids) release(id);
+            rep.runs++;
+        }
+        return best;
+    };
+    Tuning cur = I.tune;
+    run(cur);  // warm-up: kernel loading, graph capture, state allocation
+    double best_ms = run(cur);
+    rep.baseline_ms = best_ms;
+    auto try_values = [&](auto setter, const auto& values) {
+        Tuning keep = cur;
+        for (const auto& v : values) {
+            Tuning t = cur;
+            setter(t, v);
+            double ms = run(t);
+            if (ms < best_ms * 0.98) {  // a change has to win by 2 %
+                best_ms = ms;
+                keep = t;
+            }
+        }
+        cur = keep;
+    };
+    const float scales[] = {0.5f, 0.75f, 1.25f};
+    try_values([](Tuning& t, float v) { t.grid_scale = v; }, scales);
+    const size_t cap = I.eng->capacity();
+    std::vector<size_t> sizes;
+    for (size_t div : {2, 4})
+        if (sample_paths > cap / div) sizes.push_back(cap / div);
+    try_values([](Tuning& t, size_t v) { t.pass_paths = v; }, sizes);
+    if (frames) {
+        const int shifts[] = {3, 5, 6};
+        try_values([](Tuning& t, int v) { t.tile_shift = v; }, shifts);
+    }
+    set_tuning(cur);
+    rep.best = cur;
+    rep.best_ms = best_ms;
+    return rep;
+}
 
 // ------------------------------------------------------------------ serving
 namespace {

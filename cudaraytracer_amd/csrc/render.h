@@ -185,6 +185,23 @@ public:
     size_t bytes_in_use() const;
     void stats(RenderStats& out);
     void reset_stats();
+    // Launch shape and pass size, chosen by autotune() or set directly.
+    struct Tuning {
+        float grid_scale = 1.f;   // persistent stage grids = occupancy x CUs x scale
+        size_t pass_paths = 0;    // paths per pass (0: the engine's capacity)
+        int tile_shift = 4;       // frame kernels on 2^shift x (256 >> shift) tiles
+    };
+    void set_tuning(const Tuning& t);
+    Tuning tuning() const;
+    // Time a sample of the expected jobs under candidate tunings (coordinate
+    // descent: grid scale, then pass size, then tile shape; each candidate
+    // run twice, best time kept) and keep the fastest.  The pool must be idle.
+    struct TuneReport {
+        Tuning best;
+        double baseline_ms = 0, best_ms = 0;
+        int runs = 0;
+    };
+    TuneReport autotune(const std::vector<JobDesc>& sample);
     struct Impl;
 
 private:

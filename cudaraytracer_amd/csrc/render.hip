@@ -441,40 +441,43 @@ __global__ void k_math(int op, const float* a, const float* b, float* out, int n
 }
 
 // ---------------------------------------------------------------- frames
-// Per-pixel frame kernels work on 16x16 tiles (four 16x4 waves), so the
-// neighbourhood taps of the filters stay within a few cache lines per wave.
-__device__ __forceinline__ bool tile_pixel(int w, int h, int& i, int& j) {
-    int tiles_x = (w + 15) >> 4;
+// Per-pixel frame kernels work on tiles of 2^tile_shift x (256 >> tile_shift)
+// pixels (one 256-lane block), so the neighbourhood taps of the filters stay
+// within a few cache lines per wave; the shape is a tuning knob (16 x 16 by default).
+__device__ __forceinline__ bool tile_pixel(int w, int h, int tile_shift, int& i, int& j) {
+    const int tw = 1 << tile_shift, th = kBlock >> tile_shift;
+    int tiles_x = (w + tw - 1) >> tile_shift;
     int t = blockIdx.x;
-    i = (t % tiles_x) * 16 + (threadIdx.x & 15);
-    j = (t / tiles_x) * 16 + (threadIdx.x >> 4);
+    i = (t % tiles_x) * tw + (threadIdx.x & (tw - 1));
+    j = (t / tiles_x) * th + (threadIdx.x >> tile_shift);
     return i < w && j < h;
 }
 
 template <int STACK>
-__global__ __launch_bounds__(kBlock) void k_gbuffer(DevScene sc, GBufferParams g, GBufferView out) {
+__global__ __launch_bounds__(kBlock) void k_gbuffer(DevScene sc, GBufferParams g, GBufferView out, int ts) {
     __shared__ uint32_t stk[STACK * kBlock];
     int i, j;
-    if (!tile_pixel(g.width, g.height, i, j)) return;
+    if (!tile_pixel(g.width, g.height, ts, i, j)) return;
     gbuffer_pixel<STACK>(sc, g, i, j, stk + threadIdx.x, kBlock, out);
 }
 
-__global__ __launch_bounds__(kBlock) void k_temporal(int w, int h, int first, SvgfParams p, TemporalIO io,
+__global__ __launch_bounds__(kBlock) void k_temporal(int w, int h, int ts, int first, SvgfParams p, TemporalIO io,
                                                       GBufferView cur, GBufferView prev) {
     int i, j;
-    if (!tile_pixel(w, h, i, j)) return;
+    if (!tile_pixel(w, h, ts, i, j)) return;
     temporal_pixel(i, j, w, h, first != 0, p, io, cur, prev);
 }
 
-__global__ __launch_bounds__(kBlock) void k_variance(int w, int h, SvgfParams p, FilterIO io, GBufferView g) {
+__global__ __launch_bounds__(kBlock) void k_variance(int w, int h, int ts, SvgfParams p, FilterIO io, GBufferView g) {
     int i, j;
-    if (!tile_pixel(w, h, i, j)) return;
+    if (!tile_pixel(w, h, ts, i, j)) return;
     variance_pixel(i, j, w, h, p, io, g);
 }
 
-__global__ __launch_bounds__(kBlock) void k_atrous(int w, int h, int step, SvgfParams p, FilterIO io, GBufferView g) {
+__global__ __launch_bounds__(kBlock) void k_atrous(int w, int h, int ts, int step, SvgfParams p, FilterIO io,
+                                                    GBufferView g) {
     int i, j;
-    if (!tile_pixel(w, h, i, j)) return;
+    if (!tile_pixel(w, h, ts, i, j)) return;
     atrous_pixel(i, j, w, h, step, p, io, g);
 }
 
@@ -837,6 +840,15 @@ Engine::~Engine() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+void Engine::set_grid_scale(float scale) {
+    if (!(scale > 0.f)) throw std::invalid_argument("grid scale must be positive");
+    if (scale == grid_scale_) return;
+    HIP_OK(hipStreamSynchronize(stream_));
+    for (auto& g : graphs_) (void)hipGraphExecDestroy(g.second);
+    graphs_.clear();
+    grid_scale_ = scale;
+}
+
 void Engine::read_stats(unsigned long long out[kNumStats]) {
     HIP_OK(hipStreamSynchronize(stream_));
     HIP_OK(hipMemcpy(out, stats_.p, sizeof(unsigned long long) * kNumStats, hipMemcpyDeviceToHost));
@@ -950,9 +962,11 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count) {
     unsigned long long* st = stats_.as<unsigned long long>();
     const int cap = (int)cap_;
     const int pi = POOLED ? 1 : 0, si = STACK == 64 ? 1 : 0;
-    const int gext = grid_ext_[pi][si];
-    const int gsh = grid_shadow_[pi][si];
-    const int* gshade = grid_shade_[pi][si];
+    auto scaled = [&](int g) { return std::max(1, (int)((float)g * grid_scale_ + 0.5f)); };
+    const int gext = scaled(grid_ext_[pi][si]);
+    const int gsh = scaled(grid_shadow_[pi][si]);
+    int gshade[kNumClasses];
+    for (int c = 0; c < kNumClasses; ++c) gshade[c] = scaled(grid_shade_[pi][si][c]);
     int q = 0;
     hipEvent_t b = prof.begin(stream_);
     k_raygen<POOLED><<<grid_linear_ * 4, kBlock, 0, stream_>>>(hdr, jobs, qo_[q].as<float4>(), qd_[q].as<float4>(),
@@ -1096,12 +1110,13 @@ void FrameState::begin(const GpuScene& gs, const HostScene& host, const CameraRe
     if (cam.width != w_ || cam.height != h_) throw std::invalid_argument("camera resolution differs from the view's");
     const int c = fcur_;
     const bool first = frames_ == 0;
-    const int tiles = ((w_ + 15) / 16) * ((h_ + 15) / 16);
+    const int ts = tile_shift_, tw = 1 << ts, th = kBlock >> ts;
+    const int tiles = ((w_ + tw - 1) / tw) * ((h_ + th - 1) / th);
     hipEvent_t e = prof.begin(s);
     GBufferParams g = gbuffer_params(host, cam, first ? cam : last_cam_);
     GBufferView v = gview(g_pos_, g_nrm_, g_bary_, g_md_, c);
-    if (gs.stack() == 32) k_gbuffer<32><<<tiles, kBlock, 0, s>>>(gs.dev(), g, v);
-    else k_gbuffer<64><<<tiles, kBlock, 0, s>>>(gs.dev(), g, v);
+    if (gs.stack() == 32) k_gbuffer<32><<<tiles, kBlock, 0, s>>>(gs.dev(), g, v, ts);
+    else k_gbuffer<64><<<tiles, kBlock, 0, s>>>(gs.dev(), g, v, ts);
     HIP_OK(hipGetLastError());
     prof.end(kStageGBuffer, e, s);
     // Fresh samples of this frame only (the reference resets its frame buffer
@@ -1120,7 +1135,8 @@ void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const fl
     started_ = false;
     const int c = fcur_, pv = c ^ 1;
     const bool first = frames_ == 0;
-    const int tiles = ((w_ + 15) / 16) * ((h_ + 15) / 16);
+    const int ts = tile_shift_, tw = 1 << ts, th = kBlock >> ts;
+    const int tiles = ((w_ + tw - 1) / tw) * ((h_ + th - 1) / th);
     GBufferView gc = gview(g_pos_, g_nrm_, g_bary_, g_md_, c), gp = gview(g_pos_, g_nrm_, g_bary_, g_md_, pv);
     hipEvent_t e = prof.begin(s);
     TemporalIO tio;
@@ -1139,7 +1155,7 @@ void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const fl
     tio.len = h_len_[c].as<int>();
     tio.guide_nd = guide_nd_.as<float4>();
     tio.guide_kl = guide_kl_.as<int2>();
-    k_temporal<<<tiles, kBlock, 0, s>>>(w_, h_, first ? 1 : 0, p, tio, gc, gp);
+    k_temporal<<<tiles, kBlock, 0, s>>>(w_, h_, ts, first ? 1 : 0, p, tio, gc, gp);
     HIP_OK(hipGetLastError());
     prof.end(kStageTemporal, e, s);
 
@@ -1154,7 +1170,7 @@ void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const fl
     fio.len = h_len_[c].as<int>();
     fio.guide_nd = guide_nd_.as<float4>();
     fio.guide_kl = guide_kl_.as<int2>();
-    k_variance<<<tiles, kBlock, 0, s>>>(w_, h_, p, fio, gc);
+    k_variance<<<tiles, kBlock, 0, s>>>(w_, h_, ts, p, fio, gc);
     HIP_OK(hipGetLastError());
     prof.end(kStageVariance, e, s);
 
@@ -1169,7 +1185,7 @@ void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const fl
             fio.in_i = k == 0 ? v_i_.as<float4>() : f_i_[o ^ 1].as<float4>();
             fio.out_d = f_d_[o].as<float4>();
             fio.out_i = f_i_[o].as<float4>();
-            k_atrous<<<tiles, kBlock, 0, s>>>(w_, h_, 1 << k, p, fio, gc);
+            k_atrous<<<tiles, kBlock, 0, s>>>(w_, h_, ts, 1 << k, p, fio, gc);
             HIP_OK(hipGetLastError());
             out_sel_ = o;
         }

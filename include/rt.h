@@ -238,6 +238,19 @@ typedef struct rt_serve_report {
     double frame_p50_ms, frame_p99_ms, job_p50_ms, job_p99_ms;
 } rt_serve_report;
 
+/* Launch shape and pass size of a pool. */
+typedef struct rt_pool_tuning {
+    float grid_scale;    /* persistent stage grids = occupancy x CUs x scale (1 = default) */
+    int64_t pass_paths;  /* paths per pass; 0 = the queue capacity */
+    int tile_width;      /* per-pixel frame kernels on tile_width x (256 / tile_width) tiles: 8..128 */
+} rt_pool_tuning;
+
+typedef struct rt_tune_report {
+    rt_pool_tuning best;
+    double baseline_ms, best_ms; /* time of the sample jobs with the starting and the chosen tuning */
+    int runs;
+} rt_tune_report;
+
 int rt_pool_create(const rt_pool_params* p, rt_pool** out);
 void rt_pool_destroy(rt_pool* p);
 /* The scene must outlive the pool. */
@@ -254,6 +267,11 @@ int rt_pool_read_frame(rt_pool* p, int job, int which, void* out, size_t bytes);
 int rt_pool_release(rt_pool* p, int job);
 int rt_pool_stats(rt_pool* p, rt_stats_out* out);
 int rt_pool_reset_stats(rt_pool* p);
+int rt_pool_set_tuning(rt_pool* p, const rt_pool_tuning* t);
+int rt_pool_get_tuning(rt_pool* p, rt_pool_tuning* out);
+/* Autotuner: time the n sample jobs under candidate tunings (grid scale, then pass size, then tile
+   shape; coordinate descent, each candidate run twice) and keep the fastest.  The pool must be idle. */
+int rt_pool_autotune(rt_pool* p, const rt_job_desc* sample, int n, rt_tune_report* out);
 /* Replay an arrival stream: batched = 1 serves it with the pool's continuous batching,
    0 with the plain loop (one job at a time, a fresh single-scene renderer each).  Frame
    latency = completion - (arrival for a job's first frame, the previous frame's completion

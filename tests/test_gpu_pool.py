@@ -155,3 +155,41 @@ def test_pool_serve_batched_and_plain(scene_dir):
         assert (job_ms > 0).all()
         for f, j in zip(frame_ms, job_ms):
             assert abs(f.sum() - j) < 1e-6 * max(1.0, j)
+
+
+def test_pool_tuning_keeps_results_and_autotune_runs(scene_dir):
+    """Launch shape, pass size and tile shape change speed only: results stay
+    bit-identical; the autotuner picks a valid tuning from a sample."""
+    scene = rt.Scene(scene_dir["cornell_spheres_rt"])
+    r = rt.Renderer(scene)
+    frames = _path(r.camera(), 3)
+    alone = []
+    for cam in frames:
+        r.set_camera(cam)
+        r.render_frame(1)
+        alone.append(r.read_frame(rt.RT_FRAME_DENOISED))
+    d_alone, _, _ = _alone_accum(scene, 5)
+    info = scene.info()
+    pool = rt.Pool()
+    slot = pool.add_scene(scene)
+    for scale, paths, tw in ((0.5, 0, 8), (1.25, 20000, 64), (1.0, 0, 32)):
+        pool.set_tuning(scale, paths, tw)
+        t = pool.tuning()
+        assert (t.grid_scale, t.pass_paths, t.tile_width) == (scale, paths, tw)
+        acc = pool.submit(rt.Job(slot, spp=5))
+        rtj = pool.submit(rt.Job(slot, spp=1, frames=3, cameras=frames))
+        f = 0
+        while pool.step() > 0:
+            st = pool.status(rtj)
+            if st.frames_done > f:
+                _bits_equal(pool.read_frame(rtj, rt.RT_FRAME_DENOISED, info.width, info.height), alone[f],
+                            f"tuning {scale, paths, tw} frame {f}")
+                f = st.frames_done
+        assert f == 3
+        _bits_equal(pool.read_accum(acc, rt.RT_ACCUM_DIRECT, info.width, info.height), d_alone, "accumulate")
+        pool.release(acc)
+        pool.release(rtj)
+    rep = pool.autotune([rt.Job(slot, spp=4), rt.Job(slot, spp=1, frames=2, cameras=frames[:2])])
+    print("autotune", rep, flush=True)
+    assert rep["runs"] >= 4 and rep["best_ms"] <= rep["baseline_ms"] + 1e-9
+    assert rep["best"]["tile_width"] in (8, 16, 32, 64)
