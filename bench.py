@@ -8,8 +8,17 @@ Inputs (BVH, triangles, queues) are resident in HBM before timing starts.
 Prints one JSON line.  `value` is Mrays/s over the timed steps (extension +
 shadow rays / GPU wall time).  `roofline` is for the kernel with the largest
 share of GPU time, from per-launch HIP events recorded on the renderer's own
-stream and the algorithmic-bytes model.  `cpu_baseline` times
-the CPU oracle (OpenMP, all host cores) on a bounded sample of the same workload.
+stream and the algorithmic-bytes model below (counted node visits and triangle
+tests times their record sizes); `traffic` is the HBM bytes per launch that
+rocprofv3's FETCH_SIZE / WRITE_SIZE counters measured for the same kernel,
+scaled by what they read for a plain 1 GiB copy (the table under profiles/,
+written by --traffic-from-pmc).  `cpu_baseline` times the CPU oracle (OpenMP,
+all host cores) on a bounded sample of the same workload.
+
+Other modes: --only NAME runs one secondary workload (frames, c3, batched,
+serving) and prints its JSON; --calibrate-copy runs the 1 GiB copy that scales
+the counters; --traffic-from-pmc turns four rocprofv3 --pmc databases into the
+traffic table.
 """
 from __future__ import annotations
 
@@ -32,6 +41,98 @@ HBM_PEAK_GBS = 8000.0
 EXT_RAY_BYTES, SHADOW_RAY_BYTES, NODE_BYTES, TRI_BYTES = 48, 36, 64, 36
 # Shade: 84 B read + 68 B written per path-bounce + 44 B per emitted shadow ray.
 SHADE_PATH_BYTES, SHADE_SHADOW_BYTES = 152, 44
+
+
+TRAFFIC_STAGES = ("k_extend", "k_shade", "k_shadow")
+C2_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_cornell_spheres_512_64spp.txt")
+C3_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_terrain_2M_1080p_32spp.txt")
+
+
+def _stage_of(kernel: str):
+    """Stage of a measured kernel instance: only the plain (uncounted,
+    single-scene) instantiations the timed renders launch."""
+    import re
+
+    m = re.match(r"(?:rt::)?(k_extend|k_shadow)<false, false>$", kernel)
+    if m:
+        return m.group(1)
+    if re.match(r"(?:rt::)?k_shade<\d, false>$", kernel):
+        return "k_shade"
+    return None
+
+
+def traffic_from_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_bytes=1 << 30):
+    """HBM bytes per launch of each stage from rocprofv3 --pmc databases.
+    FETCH_SIZE and WRITE_SIZE are KiB per dispatch; each is scaled by the
+    ratio of the true size of a plain device copy to what the same counter
+    read for it."""
+    import re
+    import sqlite3
+
+    def per_kernel(db):
+        con = sqlite3.connect(db)
+        out = {}
+        for name, disp, val in con.execute("select kernel_name, dispatch_id, value from counters_collection"):
+            k = re.sub(r"\(.*", "", name).replace("void ", "").strip()
+            d = out.setdefault(k, {})
+            d[disp] = d.get(disp, 0.0) + val
+        return out
+
+    def copy_kib(db):
+        rows = [sum(v.values()) / len(v) for k, v in per_kernel(db).items() if "copyBuffer" in k]
+        if not rows:
+            raise SystemExit(f"{db}: no copy kernel found")
+        return max(rows)
+
+    f_scale = copy_bytes / (copy_kib(copy_fetch_db) * 1024)
+    w_scale = copy_bytes / (copy_kib(copy_write_db) * 1024)
+    fetch, write = per_kernel(fetch_db), per_kernel(write_db)
+    lines = [f"fetch_scale {f_scale:.4f}", f"write_scale {w_scale:.4f}", "",
+             "stage      dispatches  fetch_B_per_launch  write_B_per_launch  hbm_B_per_launch"]
+    for st in TRAFFIC_STAGES:
+        fk = [k for k in fetch if _stage_of(k) == st]
+        wk = [k for k in write if _stage_of(k) == st]
+        nf = sum(len(fetch[k]) for k in fk)
+        nw = sum(len(write[k]) for k in wk)
+        if not nf or not nw:
+            continue
+        fb = sum(sum(fetch[k].values()) for k in fk) * 1024 * f_scale / nf
+        wb = sum(sum(write[k].values()) for k in wk) * 1024 * w_scale / nw
+        lines.append(f"{st:10s} {nf:11d} {fb:19.0f} {wb:19.0f} {fb + wb:17.0f}")
+    lines += ["", "instance-level (scaled bytes per dispatch):"]
+    for k in sorted(set(fetch) | set(write)):
+        if _stage_of(k) is None:
+            continue
+        f = fetch.get(k, {})
+        w = write.get(k, {})
+        fb = sum(f.values()) * 1024 * f_scale / max(1, len(f))
+        wb = sum(w.values()) * 1024 * w_scale / max(1, len(w))
+        lines.append(f"  {k:32s} {len(f):5d} {fb:15.0f} {wb:15.0f}")
+    return "\n".join(lines)
+
+
+def load_traffic(path):
+    """Stage -> HBM bytes per launch from a traffic table, {} if absent."""
+    out = {}
+    if not path or not os.path.exists(path):
+        return out
+    for line in open(path):
+        tok = line.split()
+        if len(tok) == 5 and tok[0] in TRAFFIC_STAGES and tok[1].isdigit():
+            out[tok[0]] = int(float(tok[4]))
+    return out
+
+
+def calibrate_copy(nbytes: int = 1 << 30):
+    """A plain device-to-device copy of nbytes, three times: the counter scale."""
+    import torch
+
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "copies": 3}))
 
 
 def _stage_roofline(name, bytes_per_launch, ms_per_launch, traffic=None):
@@ -121,7 +222,7 @@ def c3_bench(env_alias: bool = True):
     info = scene.info()
     spp = info.spp
     r = rt.Renderer(scene, env_alias=env_alias)
-    r.render(2)
+    r.render(spp)  # warm-up at full size, so every launch in the run has one shape
     r.synchronize()
     r.reset()
     r.reset_stats()
@@ -141,8 +242,11 @@ def c3_bench(env_alias: bool = True):
                  + sc.extension_tri_tests * TRI_BYTES)
     sh_bytes = (sc.shadow_rays * SHADOW_RAY_BYTES + sc.shadow_node_visits * NODE_BYTES
                 + sc.shadow_tri_tests * TRI_BYTES)
-    ext = _stage_roofline("k_extend", ext_bytes / sp.stage_launches[1], sp.stage_ms[1] / sp.stage_launches[1])
-    sh = _stage_roofline("k_shadow", sh_bytes / sp.stage_launches[3], sp.stage_ms[3] / sp.stage_launches[3])
+    tr = load_traffic(C3_TRAFFIC)
+    ext = _stage_roofline("k_extend", ext_bytes / sp.stage_launches[1], sp.stage_ms[1] / sp.stage_launches[1],
+                          tr.get("k_extend"))
+    sh = _stage_roofline("k_shadow", sh_bytes / sp.stage_launches[3], sp.stage_ms[3] / sp.stage_launches[3],
+                         tr.get("k_shadow"))
     return {
         "workload": f"terrain_env {info.num_triangles} tris + SSS sphere + 2048x1024 env "
                     f"({'alias tables' if env_alias else 'CDF search'}), {info.width}x{info.height}, {spp} spp, "
@@ -154,8 +258,8 @@ def c3_bench(env_alias: bool = True):
         "nodes_per_ext_ray": round(sc.extension_node_visits / max(1, sc.extension_rays), 2),
         "tris_per_ext_ray": round(sc.extension_tri_tests / max(1, sc.extension_rays), 2),
         "stage_ms": {n: round(sp.stage_ms[i], 3) for i, n in enumerate(rt.STAGES) if sp.stage_launches[i]},
-        "extend_roofline": {k: ext[k] for k in ("bound", "achieved", "peak", "unit", "frac")},
-        "shadow_roofline": {k: sh[k] for k in ("bound", "achieved", "peak", "unit", "frac")},
+        "extend_roofline": {k: ext[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
+        "shadow_roofline": {k: sh[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
     }
 
 
@@ -218,9 +322,12 @@ def batched_bench(n: int = 64, seed: int = 4, queue_gib: int = 16):
     reqs = [(0.0, rt.Job(slot, spp=spp)) for slot, (_, spp) in zip(slots, meta)]
     tris_total = sum(s.info().num_triangles for s in sc)
     paths_total = sum(w * w * spp for w, spp in meta)
-    # Autotune the pool on a sample of the mix (every 8th job, at most 4 spp),
-    # which also warms up kernels and graphs; then one job through the plain loop.
-    sample = [rt.Job(slot, spp=min(spp, 4)) for slot, (_, spp) in list(zip(slots, meta))[::8]]
+    # Autotune the pool on a sample of the mix (every 4th job at its full spp:
+    # passes as full as the real ones -- a low-spp sample makes small passes,
+    # where half-size grids win, while the full mix loses a quarter of its
+    # throughput with them), which also warms up kernels and graphs; then one
+    # job through the plain loop.
+    sample = [rt.Job(slot, spp=spp) for slot, (_, spp) in list(zip(slots, meta))[::4]]
     tune = pool.autotune(sample)
     print(f"[bench] batched autotune: {tune}", file=sys.stderr, flush=True)
     pool.serve([reqs[0]], batched=False)
@@ -327,7 +434,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-spp", type=int, default=64, help="samples per pixel for the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--traffic-json", default=None, help="HBM bytes per launch measured with rocprofv3 --pmc")
+    ap.add_argument("--traffic-file", default=C2_TRAFFIC,
+                    help="HBM bytes per launch measured with rocprofv3 --pmc (table from --traffic-from-pmc)")
+    ap.add_argument("--only", choices=("frames", "c3", "batched", "serving"), default=None,
+                    help="run one secondary workload and print its JSON")
+    ap.add_argument("--calibrate-copy", action="store_true", help="run the 1 GiB copy that scales the counters")
+    ap.add_argument("--traffic-from-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB"),
+                    help="print the traffic table from rocprofv3 --pmc databases")
     ap.add_argument("--no-frames", action="store_true", help="skip the real-time SVGF frame-loop measurement")
     ap.add_argument("--no-c3", action="store_true", help="skip the 2 M-triangle terrain render")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched-vs-plain measurement")
@@ -336,6 +449,16 @@ def main() -> int:
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU is design-only)"}))
         return 2
+    if args.traffic_from_pmc:
+        print(traffic_from_pmc(*args.traffic_from_pmc))
+        return 0
+    if args.calibrate_copy:
+        calibrate_copy()
+        return 0
+    if args.only:
+        fn = {"frames": frames_bench, "c3": c3_bench, "batched": batched_bench, "serving": serving_bench}[args.only]
+        print(json.dumps({args.only: fn()}))
+        return 0
 
     import cudaraytracer_amd as rt
     from cudaraytracer_amd import scenes
@@ -384,9 +507,7 @@ def main() -> int:
     rc = rt.Renderer(scene)
     rc.render(spp, count_traversal=True)
     sc = rc.stats()
-    traffic = None
-    if args.traffic_json and os.path.exists(args.traffic_json):
-        traffic = json.load(open(args.traffic_json))
+    traffic = load_traffic(args.traffic_file)
     ext_bytes = (sc.extension_rays * EXT_RAY_BYTES + sc.extension_node_visits * NODE_BYTES
                  + sc.extension_tri_tests * TRI_BYTES)
     sh_bytes = (sc.shadow_rays * SHADOW_RAY_BYTES + sc.shadow_node_visits * NODE_BYTES
@@ -397,7 +518,7 @@ def main() -> int:
         idx = {"k_extend": 1, "k_shade": 2, "k_shadow": 3}[name]
         launches = sp.stage_launches[idx] / prof_steps
         ms = sp.stage_ms[idx] / prof_steps
-        t = traffic.get(name) if traffic else None
+        t = traffic.get(name)
         stages[name] = _stage_roofline(name, nbytes / launches, ms / launches, t)
         stages[name]["ms_per_step"] = round(ms, 3)
     dominant = max(stages.values(), key=lambda s: s["ms_per_step"])

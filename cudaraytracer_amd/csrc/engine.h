@@ -103,8 +103,15 @@ public:
     float grid_scale() const { return grid_scale_; }
 
 private:
-    template <int STACK, bool POOLED>
-    void run_t(int max_bounces, const bool* classes, bool count);
+    template <bool POOLED>
+    void run_t(int max_bounces, const bool* classes, bool count, int stack_entries);
+    struct Grids {
+        int ext = 0, shadow = 0, shade[kNumClasses] = {};
+    };
+    template <bool POOLED>
+    Grids grids_for(int stack_entries);
+    std::map<uint64_t, Grids> grids_;
+    int cus_ = 0;
     bool use_graphs_ = true;
     float grid_scale_ = 1.f;
     std::map<uint64_t, hipGraphExec_t> graphs_;
@@ -119,11 +126,7 @@ private:
     void* host_tables_ = nullptr;
     hipEvent_t ring_ev_[kRing] = {};
     int ring_next_ = 0;
-    // Persistent grids per [pooled][64-entry stack] kernel variant.
-    int grid_ext_[2][2] = {}, grid_shadow_[2][2] = {}, grid_shade_[2][2][kNumClasses] = {};
     int grid_linear_ = 0;
-    template <bool POOLED, int STACK>
-    void init_grids(int cus);
 };
 
 // G-buffers and SVGF state of one width x height view (reference: the

@@ -57,36 +57,59 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* p, uint32_t v) 
 // on its own slice and moves on (stealing) when a slice is drained.  Each call
 // to next() yields a 64-item chunk (slice, first item) or false when every
 // slice is drained.  All lanes of the wave see the same values.
+//
+// When its slice runs dry the wave looks at every slice at once -- lane k
+// reads slice k's count and fetch counter, one round trip instead of one per
+// slice -- and exits when none is live.  Otherwise it steals from a live
+// slice of its own XCD if there is one: workgroups are dispatched to the 8
+// XCDs round-robin, so with 4 waves per block the waves that start on slice
+// s run on XCD (s >> 2) & 7, and a thief from the same XCD finds the slice's
+// BVH nodes in its own L2.  Among the candidates, a per-wave rotation picks
+// the first live one, which spreads thieves over the live slices.  Counts
+// are fixed for the launch (written by the previous stage) and fetch counters
+// only grow, so a slice seen drained stays drained and every wave's loop ends.
+static_assert(kSlices == kWave, "the drained-slice scan maps one slice per lane");
+static_assert(kBlock / kWave == 4 && kSlices == 64, "slice-to-XCD map assumes 4 waves per block, 64 slices");
 struct SliceCursor {
     int* counters;
     int cnt_kind, fetch_kind;
-    int slice, visited, count;
+    int slice, count, rot;
+    uint64_t own;  // slices whose first waves run on this wave's XCD
     __device__ void init(int* c, int ck, int fk) {
         counters = c;
         cnt_kind = ck;
         fetch_kind = fk;
         int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
         slice = wave % kSlices;
-        visited = 0;
+        rot = (wave / kSlices) % kSlices;
+        const int xcd = (int)(blockIdx.x & 7u);
+        own = (0xFull << (4 * xcd)) | (0xFull << (32 + 4 * xcd));
         count = *ctr_at(counters, cnt_kind, slice);
     }
     __device__ bool next(int& base) {
-        while (visited < kSlices) {
+        const int lane = (int)(threadIdx.x & (kWave - 1));
+        while (true) {
             int* f = ctr_at(counters, fetch_kind, slice);
             int b = count;
-            // Drained slices are skipped with a plain load; only live ones take an atomic.
-            // Lane 0 decides and broadcasts, so the wave stays uniform.
-            if ((threadIdx.x & (kWave - 1)) == 0 && load_relaxed(f) < count) b = atomicAdd(f, kWave);
+            // Lane 0 claims the chunk and broadcasts, so the wave stays uniform.
+            if (lane == 0 && load_relaxed(f) < count) b = atomicAdd(f, kWave);
             b = __shfl(b, 0);
             if (b < count) {
                 base = b;
                 return true;
             }
-            ++visited;
-            slice = slice + 1 == kSlices ? 0 : slice + 1;
-            count = *ctr_at(counters, cnt_kind, slice);
+            // Drained: lane k checks slice k.
+            const int c = *ctr_at(counters, cnt_kind, lane);
+            const bool live = load_relaxed(ctr_at(counters, fetch_kind, lane)) < c;
+            const uint64_t m = __ballot(live);
+            if (m == 0) return false;
+            const uint64_t cand = (m & own) ? (m & own) : m;
+            // First candidate at or after slice `rot` (cyclically).
+            const uint64_t hi = cand & (~0ull << rot);
+            const int k = __ffsll((unsigned long long)(hi ? hi : cand)) - 1;
+            slice = k;
+            count = __shfl(c, k);
         }
-        return false;
     }
 };
 
@@ -169,12 +192,17 @@ __global__ __launch_bounds__(kBlock) void k_raygen(const PassHeader* __restrict_
 // ------------------------------------------------------------------ extend
 // Closest hit for every path of the current queue, then the path's queue
 // index is appended to its slice's list for the shading class of what it hit.
-template <int STACK, bool COUNT, bool POOLED>
+// The traversal stacks live in dynamic LDS: one column of `max stack depth`
+// entries per lane, sized per pass to the deepest tree among its jobs, so a
+// shallow tree leaves room for more resident waves.
+extern __shared__ uint32_t stk_dyn[];
+
+template <bool COUNT, bool POOLED>
 __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jobs, const float4* __restrict__ qo,
                                                     const float4* __restrict__ qd, float4* __restrict__ hits,
                                                     int* __restrict__ lists, int cap, int region, int* counters,
                                                     int qslot, unsigned long long* stats) {
-    __shared__ uint32_t stk[STACK * kBlock];
+    uint32_t* stk = stk_dyn;
     if (blockIdx.x == 0 && threadIdx.x < kSlices) {
         int sl = threadIdx.x;
         *ctr_at(counters, qslot ^ 1, sl) = 0;
@@ -196,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jo
             float4 o = qo[i], d = qd[i];
             const DevScene& sc = pass_job<POOLED>(jobs, (int)((uint32_t)f_as_i(d.w) >> kJobShift)).sc;
             RayInv r = make_ray(v3(o.x, o.y, o.z), v3(d.x, d.y, d.z));
-            Hit h = trace_closest<STACK, COUNT>(sc, r, 0.f, 1e8f, col, kBlock, c);
+            Hit h = trace_closest<0, COUNT>(sc, r, 0.f, 1e8f, col, kBlock, c);
             hits[i] = make_float4(h.t, h.b1, h.b2, i_as_f(h.prim));
             cls = shade_class_of(sc.tri_attr, h.prim);
         }
@@ -219,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jo
 // job's last bounce its survivors are dropped together with everything they
 // gathered (reference: Src/Core/renderer.cuh:1166-1194 never accumulates a
 // path still alive after maxBounces), unless the job flushes survivors.
-template <int CLS, int STACK, bool POOLED>
+template <int CLS, bool POOLED>
 __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ jobs, int bounce,
                                                    const float4* __restrict__ qo, const float4* __restrict__ qd,
                                                    const float4* __restrict__ qw, const float4* __restrict__ hits,
@@ -227,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
                                                    float4* __restrict__ nd, float4* __restrict__ nw,
                                                    float4* __restrict__ shadow, float4* slot_d, float4* slot_i,
                                                    int region, int* counters, int qslot) {
-    __shared__ uint32_t stk[CLS == kClsSubsurface ? STACK * kBlock : 1];
+    uint32_t* stk = stk_dyn;  // sized only for the subsurface class (its probe rays)
     if (blockIdx.x == 0 && threadIdx.x < kSlices) *ctr_at(counters, kFetchExtend, threadIdx.x) = 0;
     uint32_t* col = stk + (CLS == kClsSubsurface ? threadIdx.x : 0);
     SliceCursor cur;
@@ -259,7 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
             const int local = pid - J.path_base;
             uint32_t pixel = (uint32_t)(local % fp.wh);
             uint32_t sample = fp.sample0 + (uint32_t)(local / fp.wh);
-            so = shade_path<CLS, STACK>(J.sc, fp, ps, hits[i], pixel, sample, col, kBlock);
+            so = shade_path<CLS, 0>(J.sc, fp, ps, hits[i], pixel, sample, col, kBlock);
             if (so.has_emit) {
                 float4* s = so.add_direct ? slot_d : slot_i;
                 float4 a = s[pid];
@@ -304,11 +332,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
 // ------------------------------------------------------------------ shadow
 // Closest hit from the light point towards the shading point with tMin 1e-6,
 // then the reference's 0.1 distance tolerances (Src/Core/light.cu:5-27).
-template <int STACK, bool COUNT, bool POOLED>
+template <bool COUNT, bool POOLED>
 __global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jobs, const float4* __restrict__ shadow,
                                                     float4* slot_d, float4* slot_i, int region, int* counters,
                                                     unsigned long long* stats) {
-    __shared__ uint32_t stk[STACK * kBlock];
+    uint32_t* stk = stk_dyn;
     if (blockIdx.x == 0 && threadIdx.x < kSlices) {
         int sl = threadIdx.x;
         int n = *ctr_at(counters, kCntShadow, sl);
@@ -338,7 +366,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jo
             rec.hp = v3(r2.x, r2.y, r2.z);
             vec3 org = rec.lp + 0.01f * rec.dir;
             RayInv ray = make_ray(org, rec.dir);
-            Hit h = trace_closest<STACK, COUNT>(sc, ray, 1e-6f, 1e8f, col, kBlock, c);
+            Hit h = trace_closest<0, COUNT>(sc, ray, 1e-6f, 1e8f, col, kBlock, c);
             bool hit = h.prim >= 0;
             vec3 tp = v3(0.f, 0.f, 0.f);
             if (hit) tp = position_at(sc.tri_attr, h.prim, h.b1, h.b2);
@@ -500,6 +528,7 @@ struct GpuScene::Impl {
         env_alias_marg;
     DevScene dev{};
     int stack = 32;
+    int max_stack = 1;
     bool has_class[kNumClasses] = {true, false, false, false, false};
     int device = 0;
 };
@@ -516,6 +545,7 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
     int n = sc.num_tris();
     if (bvh.max_stack > 64) throw std::runtime_error("BVH deeper than the 64-entry traversal stack");
     I.stack = bvh.max_stack <= 32 ? 32 : 64;
+    I.max_stack = std::max(1, bvh.max_stack);
     // Packed nodes; a leaf-only tree gets one dummy line so the pointer is valid.
     std::vector<PackedNode> nodes = bvh.packed;
     if (nodes.empty()) nodes.resize(1);
@@ -617,6 +647,7 @@ GpuScene::~GpuScene() { delete impl_; }
 
 const DevScene& GpuScene::dev() const { return impl_->dev; }
 int GpuScene::stack() const { return impl_->stack; }
+int GpuScene::max_stack() const { return impl_->max_stack; }
 bool GpuScene::has_class(int c) const { return impl_->has_class[c]; }
 size_t GpuScene::device_bytes() const {
     const Impl& I = *impl_;
@@ -777,9 +808,9 @@ Profiler::~Profiler() {
 
 // ------------------------------------------------------------------ engine
 template <typename K>
-static int persistent_grid(K kernel, int cus) {
+static int persistent_grid(K kernel, int cus, size_t lds = 0) {
     int per_cu = 0;
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, lds));
     if (per_cu < 1) per_cu = 1;
     return per_cu * cus;
 }
@@ -811,24 +842,28 @@ Engine::Engine(size_t capacity_paths, bool profile) {
     for (int k = 0; k < kRing; ++k) HIP_OK(hipEventCreateWithFlags(&ring_ev_[k], hipEventDisableTiming));
     HIP_OK(hipMemset(counters_.p, 0, kCounterInts * sizeof(int)));
     HIP_OK(hipMemset(stats_.p, 0, kNumStats * sizeof(unsigned long long)));
-    init_grids<false, 32>(cus);
-    init_grids<false, 64>(cus);
-    init_grids<true, 32>(cus);
-    init_grids<true, 64>(cus);
+    cus_ = cus;
     grid_linear_ = cus * 8;
     HIP_OK(hipDeviceSynchronize());
 }
 
-template <bool POOLED, int STACK>
-void Engine::init_grids(int cus) {
-    const int p = POOLED ? 1 : 0, s = STACK == 64 ? 1 : 0;
-    grid_ext_[p][s] = persistent_grid(k_extend<STACK, false, POOLED>, cus);
-    grid_shadow_[p][s] = persistent_grid(k_shadow<STACK, false, POOLED>, cus);
-    grid_shade_[p][s][kClsTerminal] = persistent_grid(k_shade<kClsTerminal, STACK, POOLED>, cus);
-    grid_shade_[p][s][kClsDiffuse] = persistent_grid(k_shade<kClsDiffuse, STACK, POOLED>, cus);
-    grid_shade_[p][s][kClsPbr] = persistent_grid(k_shade<kClsPbr, STACK, POOLED>, cus);
-    grid_shade_[p][s][kClsDielectric] = persistent_grid(k_shade<kClsDielectric, STACK, POOLED>, cus);
-    grid_shade_[p][s][kClsSubsurface] = persistent_grid(k_shade<kClsSubsurface, STACK, POOLED>, cus);
+// Persistent grid of a stage kernel for a dynamic LDS size (cached).
+template <bool POOLED>
+Engine::Grids Engine::grids_for(int stack_entries) {
+    const uint64_t key = ((uint64_t)stack_entries << 1) | (POOLED ? 1 : 0);
+    auto it = grids_.find(key);
+    if (it != grids_.end()) return it->second;
+    const size_t lds = (size_t)stack_entries * kBlock * sizeof(uint32_t);
+    Grids g;
+    g.ext = persistent_grid(k_extend<false, POOLED>, cus_, lds);
+    g.shadow = persistent_grid(k_shadow<false, POOLED>, cus_, lds);
+    g.shade[kClsTerminal] = persistent_grid(k_shade<kClsTerminal, POOLED>, cus_);
+    g.shade[kClsDiffuse] = persistent_grid(k_shade<kClsDiffuse, POOLED>, cus_);
+    g.shade[kClsPbr] = persistent_grid(k_shade<kClsPbr, POOLED>, cus_);
+    g.shade[kClsDielectric] = persistent_grid(k_shade<kClsDielectric, POOLED>, cus_);
+    g.shade[kClsSubsurface] = persistent_grid(k_shade<kClsSubsurface, POOLED>, cus_, lds);
+    grids_[key] = g;
+    return g;
 }
 
 Engine::~Engine() {
@@ -848,6 +883,10 @@ void Engine::set_grid_scale(float scale) {
     graphs_.clear();
     grid_scale_ = scale;
 }
+
+// explicit instantiations used by run()
+template Engine::Grids Engine::grids_for<false>(int);
+template Engine::Grids Engine::grids_for<true>(int);
 
 void Engine::read_stats(unsigned long long out[kNumStats]) {
     HIP_OK(hipStreamSynchronize(stream_));
@@ -875,7 +914,7 @@ void Engine::run(const std::vector<PassItem>& items, bool count) {
     PassHeader* hdr = reinterpret_cast<PassHeader*>(host);
     JobDev* jobs = reinterpret_cast<JobDev*>(host + kPassHeaderBytes);
     size_t paths = 0, pixels = 0;
-    int max_bounces = 0, stack = 32;
+    int max_bounces = 0, stack = 1;
     bool classes[kNumClasses] = {true, false, false, false, false};
     for (int k = 0; k < n; ++k) {
         const PassItem& it = items[k];
@@ -895,7 +934,7 @@ void Engine::run(const std::vector<PassItem>& items, bool count) {
         paths += (size_t)J.fp.n_paths;
         pixels += (size_t)it.fp.wh;
         max_bounces = std::max(max_bounces, it.fp.max_bounces);
-        stack = std::max(stack, it.scene->stack());
+        stack = std::max(stack, it.scene->max_stack());
         for (int c = 1; c < kNumClasses; ++c) classes[c] = classes[c] || it.scene->has_class(c);
     }
     if (paths > cap_) throw std::invalid_argument("pass exceeds the engine's path capacity");
@@ -915,7 +954,7 @@ void Engine::run(const std::vector<PassItem>& items, bool count) {
     if (graph) {
         int mask = 0;
         for (int c = 0; c < kNumClasses; ++c) mask |= classes[c] ? (1 << c) : 0;
-        key = ((uint64_t)max_bounces << 16) | ((uint64_t)mask << 8) | ((uint64_t)(stack == 64) << 1) | (pooled ? 1 : 0);
+        key = ((uint64_t)max_bounces << 24) | ((uint64_t)stack << 16) | ((uint64_t)mask << 8) | (pooled ? 1 : 0);
         auto it = graphs_.find(key);
         if (it != graphs_.end()) {
             HIP_OK(hipGraphLaunch(it->second, stream_));
@@ -924,13 +963,8 @@ void Engine::run(const std::vector<PassItem>& items, bool count) {
         HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     }
     try {
-        if (!pooled) {
-            if (stack == 32) run_t<32, false>(max_bounces, classes, count);
-            else run_t<64, false>(max_bounces, classes, count);
-        } else {
-            if (stack == 32) run_t<32, true>(max_bounces, classes, count);
-            else run_t<64, true>(max_bounces, classes, count);
-        }
+        if (!pooled) run_t<false>(max_bounces, classes, count, stack);
+        else run_t<true>(max_bounces, classes, count, stack);
     } catch (...) {
         if (graph) {
             hipGraph_t g = nullptr;
@@ -953,20 +987,21 @@ void Engine::run(const std::vector<PassItem>& items, bool count) {
     }
 }
 
-template <int STACK, bool POOLED>
-void Engine::run_t(int max_bounces, const bool* classes, bool count) {
+template <bool POOLED>
+void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_entries) {
     const char* dev = tables_.as<char>();
     const PassHeader* hdr = reinterpret_cast<const PassHeader*>(dev);
     const JobDev* jobs = reinterpret_cast<const JobDev*>(dev + kPassHeaderBytes);
     int* ctr = counters_.as<int>();
     unsigned long long* st = stats_.as<unsigned long long>();
     const int cap = (int)cap_;
-    const int pi = POOLED ? 1 : 0, si = STACK == 64 ? 1 : 0;
+    const Grids G = grids_for<POOLED>(stack_entries);
+    const size_t lds = (size_t)stack_entries * kBlock * sizeof(uint32_t);
     auto scaled = [&](int g) { return std::max(1, (int)((float)g * grid_scale_ + 0.5f)); };
-    const int gext = scaled(grid_ext_[pi][si]);
-    const int gsh = scaled(grid_shadow_[pi][si]);
+    const int gext = scaled(G.ext);
+    const int gsh = scaled(G.shadow);
     int gshade[kNumClasses];
-    for (int c = 0; c < kNumClasses; ++c) gshade[c] = scaled(grid_shade_[pi][si][c]);
+    for (int c = 0; c < kNumClasses; ++c) gshade[c] = scaled(G.shade[c]);
     int q = 0;
     hipEvent_t b = prof.begin(stream_);
     k_raygen<POOLED><<<grid_linear_ * 4, kBlock, 0, stream_>>>(hdr, jobs, qo_[q].as<float4>(), qd_[q].as<float4>(),
@@ -977,11 +1012,11 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count) {
     for (int bounce = 0; bounce < max_bounces; ++bounce) {
         b = prof.begin(stream_);
         if (count)
-            k_extend<STACK, true, POOLED><<<gext, kBlock, 0, stream_>>>(
+            k_extend<true, POOLED><<<gext, kBlock, lds, stream_>>>(
                 jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), hits_.as<float4>(), lists_.as<int>(), cap, region_,
                 ctr, q, st);
         else
-            k_extend<STACK, false, POOLED><<<gext, kBlock, 0, stream_>>>(
+            k_extend<false, POOLED><<<gext, kBlock, lds, stream_>>>(
                 jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), hits_.as<float4>(), lists_.as<int>(), cap, region_,
                 ctr, q, st);
         HIP_OK(hipGetLastError());
@@ -990,7 +1025,7 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count) {
         // Classes absent from every job of the pass get no launch; their lists stay empty.
         auto shade = [&](auto cls_tag, int grid) {
             constexpr int CLS = decltype(cls_tag)::value;
-            k_shade<CLS, STACK, POOLED><<<grid, kBlock, 0, stream_>>>(
+            k_shade<CLS, POOLED><<<grid, kBlock, CLS == kClsSubsurface ? lds : 0, stream_>>>(
                 jobs, bounce, qo_[q].as<float4>(), qd_[q].as<float4>(), qw_[q].as<float4>(), hits_.as<float4>(),
                 lists_.as<int>() + (size_t)CLS * cap_, qo_[q ^ 1].as<float4>(), qd_[q ^ 1].as<float4>(),
                 qw_[q ^ 1].as<float4>(), shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), region_,
@@ -1005,10 +1040,10 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count) {
         prof.end(kStageShade, b, stream_);
         b = prof.begin(stream_);
         if (count)
-            k_shadow<STACK, true, POOLED><<<gsh, kBlock, 0, stream_>>>(
+            k_shadow<true, POOLED><<<gsh, kBlock, lds, stream_>>>(
                 jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), region_, ctr, st);
         else
-            k_shadow<STACK, false, POOLED><<<gsh, kBlock, 0, stream_>>>(
+            k_shadow<false, POOLED><<<gsh, kBlock, lds, stream_>>>(
                 jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), region_, ctr, st);
         HIP_OK(hipGetLastError());
         prof.end(kStageShadow, b, stream_);
