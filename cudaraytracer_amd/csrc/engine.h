@@ -52,9 +52,10 @@ struct DevBuf {
 constexpr size_t kQueueBytesPerPath = 2 * 3 * 16 + 16 + 4 * kNumClasses + 64 + 32;
 // Device bytes per pixel of a FrameState: two G-buffers (48 B each), two
 // histories (2 x float4), moments (2 x float2) and lengths (int) per frame
-// parity, the variance output, the a-trous ping-pong pair, the denoised colour
-// and the filter guide (decoded normal + depth, key + history length).
-constexpr size_t kFrameStateBytesPerPixel = 2 * 48 + 2 * 32 + 2 * 16 + 2 * 4 + 32 + 2 * 32 + 16 + 24;
+// parity, the variance output, the a-trous ping-pong pair, the denoised colour,
+// the filter guide (decoded normal + depth, key + history length) and the
+// luminance guide of the variance pass's input (float4).
+constexpr size_t kFrameStateBytesPerPixel = 2 * 48 + 2 * 32 + 2 * 16 + 2 * 4 + 32 + 2 * 32 + 16 + 24 + 16;
 
 // Per-stage GPU time from HIP events recorded on the work's own stream.
 struct Profiler {
@@ -169,6 +170,7 @@ private:
     DevBuf h_d_[2], h_i_[2], m_d_[2], m_i_[2], h_len_[2];
     DevBuf v_d_, v_i_, f_d_[2], f_i_[2], denoised_;
     DevBuf guide_nd_, guide_kl_;  // this frame's filter guide
+    DevBuf lum_;                  // luminance guide of the variance pass's input
     CameraRec last_cam_;
     int fcur_ = 0, flast_ = -1, frames_ = 0, out_sel_ = -1;
     int tile_shift_ = 4;

@@ -481,12 +481,11 @@ __device__ __forceinline__ bool tile_pixel(int w, int h, int tile_shift, int& i,
     return i < w && j < h;
 }
 
-template <int STACK>
+// The stack column comes from dynamic LDS sized to the tree's depth, as in extend.
 __global__ __launch_bounds__(kBlock) void k_gbuffer(DevScene sc, GBufferParams g, GBufferView out, int ts) {
-    __shared__ uint32_t stk[STACK * kBlock];
     int i, j;
     if (!tile_pixel(g.width, g.height, ts, i, j)) return;
-    gbuffer_pixel<STACK>(sc, g, i, j, stk + threadIdx.x, kBlock, out);
+    gbuffer_pixel<0>(sc, g, i, j, stk_dyn + threadIdx.x, kBlock, out);
 }
 
 __global__ __launch_bounds__(kBlock) void k_temporal(int w, int h, int ts, int first, SvgfParams p, TemporalIO io,
@@ -1107,6 +1106,7 @@ FrameState::FrameState(int width, int height) : w_(width), h_(height), wh_(width
     denoised_.alloc(wh * 16);
     guide_nd_.alloc(wh * 16);
     guide_kl_.alloc(wh * 8);
+    lum_.alloc(wh * 16);
 }
 
 size_t FrameState::device_bytes() const { return (size_t)wh_ * kFrameStateBytesPerPixel; }
@@ -1150,8 +1150,8 @@ void FrameState::begin(const GpuScene& gs, const HostScene& host, const CameraRe
     hipEvent_t e = prof.begin(s);
     GBufferParams g = gbuffer_params(host, cam, first ? cam : last_cam_);
     GBufferView v = gview(g_pos_, g_nrm_, g_bary_, g_md_, c);
-    if (gs.stack() == 32) k_gbuffer<32><<<tiles, kBlock, 0, s>>>(gs.dev(), g, v, ts);
-    else k_gbuffer<64><<<tiles, kBlock, 0, s>>>(gs.dev(), g, v, ts);
+    const size_t lds = (size_t)gs.max_stack() * kBlock * sizeof(uint32_t);
+    k_gbuffer<<<tiles, kBlock, lds, s>>>(gs.dev(), g, v, ts);
     HIP_OK(hipGetLastError());
     prof.end(kStageGBuffer, e, s);
     // Fresh samples of this frame only (the reference resets its frame buffer
@@ -1190,6 +1190,7 @@ void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const fl
     tio.len = h_len_[c].as<int>();
     tio.guide_nd = guide_nd_.as<float4>();
     tio.guide_kl = guide_kl_.as<int2>();
+    tio.lum = lum_.as<float4>();
     k_temporal<<<tiles, kBlock, 0, s>>>(w_, h_, ts, first ? 1 : 0, p, tio, gc, gp);
     HIP_OK(hipGetLastError());
     prof.end(kStageTemporal, e, s);
@@ -1205,6 +1206,7 @@ void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const fl
     fio.len = h_len_[c].as<int>();
     fio.guide_nd = guide_nd_.as<float4>();
     fio.guide_kl = guide_kl_.as<int2>();
+    fio.lum = lum_.as<float4>();
     k_variance<<<tiles, kBlock, 0, s>>>(w_, h_, ts, p, fio, gc);
     HIP_OK(hipGetLastError());
     prof.end(kStageVariance, e, s);

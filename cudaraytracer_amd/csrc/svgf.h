@@ -134,7 +134,29 @@ struct TemporalIO {
     // depth) and (mesh << 16 | material, history length).
     float4* guide_nd;
     int2* guide_kl;
+    float4* lum;  // luminance guide of cur_d / cur_i (see lum_guide)
 };
+
+// Luminance guide of the temporal pass's colours, read by the variance pass:
+// (lum_d, sqrt(max(lum_d, 1e-4)), lum_i, sqrt(max(lum_i, 1e-4))).  A 7x7 tap
+// then reads its neighbour's luminance instead of recomputing it, and its
+// sqrt(max(lum_c, lum_n, 1e-4)) becomes max(root_c, root_n): a correctly
+// rounded square root is monotone, so the two are the same float (also when
+// either luminance is NaN, which fmaxf drops in both forms).  The a-trous
+// passes compute luminance themselves: one more 16-byte load per tap cost
+// them more than the arithmetic it saves.
+RT_HD float4 lum_guide(float4 d, float4 i) {
+    const float ld = luminance(v3(d.x, d.y, d.z)), li = luminance(v3(i.x, i.y, i.z));
+    return make_float4(ld, sqrtf(fmaxf(ld, 1e-4f)), li, sqrtf(fmaxf(li, 1e-4f)));
+}
+
+// sqrtf(n) for n = 0..18, the squared lengths of the 7x7 and 5x5 filter
+// offsets: the correctly rounded float roots, so a tap reads its distance
+// instead of taking a square root.
+static __constant__ float kTapDist[19] = {
+    0x0p+0f,       0x1p+0f,       0x1.6a09e6p+0f, 0x1.bb67aep+0f, 0x1p+1f,       0x1.1e377ap+1f, 0x1.3988e2p+1f,
+    0x1.52a7fap+1f, 0x1.6a09e6p+1f, 0x1.8p+1f,     0x1.94c584p+1f, 0x1.a8872ap+1f, 0x1.bb67aep+1f, 0x1.cd82b4p+1f,
+    0x1.deeea2p+1f, 0x1.efbdecp+1f, 0x1p+2f,       0x1.07e0f6p+2f, 0x1.0f876cp+2f};
 
 RT_HD int2 guide_key_len(ushort4 nrm_mat, ushort4 bary_mesh, int len) {
     return make_int2((int)(((uint32_t)bary_mesh.w << 16) | (uint32_t)nrm_mat.w), len);
@@ -199,6 +221,7 @@ RT_HD void temporal_pixel(int i, int j, int w, int h, bool first, const SvgfPara
         ci.w = 1.f;
         io.cur_d[idx] = cd;
         io.cur_i[idx] = ci;
+        io.lum[idx] = lum_guide(cd, ci);
         io.len[idx] = 1;
         {
             ushort4 gn0 = cur.nrm_mat[idx];
@@ -306,6 +329,7 @@ RT_HD void temporal_pixel(int i, int j, int w, int h, bool first, const SvgfPara
     }
     io.cur_d[idx] = cd;
     io.cur_i[idx] = ci;
+    io.lum[idx] = lum_guide(cd, ci);
     io.guide_nd[idx] = make_float4(n.x, n.y, n.z, depth);
     io.guide_kl[idx] = guide_key_len(gn, gb, valid ? hl : 1);
 }
@@ -318,6 +342,7 @@ struct FilterIO {
     const int* len;
     const float4* guide_nd;  // decoded normal, linear depth
     const int2* guide_kl;    // mesh << 16 | material, history length
+    const float4* lum;       // variance pass: luminance guide of in_d / in_i
 };
 
 // ComputeVarianceKernel (reference: renderer.cuh:689-805): a 7x7 bilateral
@@ -334,7 +359,8 @@ RT_HD void variance_pixel(int i, int j, int w, int h, const SvgfParams& p, const
         io.out_i[idx] = ci4;
         return;
     }
-    float cld = luminance(v3(cd4.x, cd4.y, cd4.z)), cli = luminance(v3(ci4.x, ci4.y, ci4.z));
+    const float4 cl = io.lum[idx];
+    const float cld = cl.x, cli = cl.z;
     const float4 cg = io.guide_nd[idx];
     const int ckey = io.guide_kl[idx].x;
     float cdepth = cg.w;
@@ -353,13 +379,14 @@ RT_HD void variance_pixel(int i, int j, int w, int h, const SvgfParams& p, const
             float4 ng = io.guide_nd[n];
             float4 nd4 = io.in_d[n], ni4 = io.in_i[n];
             vec3 nd = v3(nd4.x, nd4.y, nd4.z), nic = v3(ni4.x, ni4.y, ni4.z);
-            float nld = luminance(nd), nli = luminance(nic);
+            const float4 nl = io.lum[n];
+            float nld = nl.x, nli = nl.z;
             float2 nmd = io.mom_d[n], nmi = io.mom_i[n];
-            float dist = sqrtf((float)(ki * ki + kj * kj));
+            float dist = kTapDist[ki * ki + kj * kj];
             float wz = -fabsf(cdepth - ng.w) / (sigma_depth * dist + 1e-4f);
             float wn = fm::powf_(fmaxf(0.f, dot(cn, v3(ng.x, ng.y, ng.z))), p.sigma_normal);
-            float dref = fmaxf(fmaxf(cld, nld), 1e-4f);
-            float wdir = -fabsf(cld - nld) / (p.sigma_light / sqrtf(dref));
+            // sqrt(max(cld, nld, 1e-4)) as the larger of the two guide roots
+            float wdir = -fabsf(cld - nld) / (p.sigma_light / fmaxf(cl.y, nl.y));
             float wind = -fabsf(cli - nli) / wind_den;
             float dw = fm::expf_(wz + wdir) * wn;
             float iw = fm::expf_(wz + wind) * wn;
@@ -440,7 +467,7 @@ RT_HD void atrous_pixel(int i, int j, int w, int h, int step, const SvgfParams& 
             float4 nd4 = io.in_d[n], ni4 = io.in_i[n];
             vec3 nd = v3(nd4.x, nd4.y, nd4.z), nic = v3(ni4.x, ni4.y, ni4.z);
             float nvd = fmaxf(0.0f, nd4.w), nvi = fmaxf(0.0f, ni4.w);
-            float dist = sqrtf((float)(ki * ki + kj * kj)) * (float)step;
+            float dist = kTapDist[ki * ki + kj * kj] * (float)step;
             float wz = -fabsf(cdepth - ng.w) / (sigma_depth * dist + 1e-6f);
             float wn = fm::powf_(fmaxf(0.f, dot(cn, v3(ng.x, ng.y, ng.z))), sn);
             float ld = luminance(nd), li = luminance(nic);

@@ -83,17 +83,21 @@ struct TravCount {
 // Address-space views: scene arrays are global memory and the stack column is
 // LDS.  The scene's pointers arrive through a struct, so without these casts
 // the compiler emits flat loads (and flat waits) for nodes, triangles and
-// stack pops.
-using GlobalF4 = const __attribute__((address_space(1))) float4;
+// stack pops.  Loads go through the native vector type, which (unlike
+// float4's class type) copies out of a qualified address space in both the
+// host and the device compilation pass.
+typedef float NativeF4 __attribute__((ext_vector_type(4)));
+using GlobalV4 = const __attribute__((address_space(1))) NativeF4;
 using LdsU32 = __attribute__((address_space(3))) uint32_t;
+__device__ __forceinline__ float4 as_f4(NativeF4 v) { return make_float4(v.x, v.y, v.z, v.w); }
 
 // `stk` points at this lane's stack column in LDS; entries are `stride` words apart.
 template <int STACK, bool COUNT, bool CULL = true>
 __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r, float tmin, float tmax,
                                              uint32_t* stk_generic, int stride, TravCount& cnt) {
     LdsU32* stk = (LdsU32*)stk_generic;
-    GlobalF4* nodes = (GlobalF4*)sc.nodes;
-    GlobalF4* tris = (GlobalF4*)sc.tri_isect;
+    GlobalV4* nodes = (GlobalV4*)sc.nodes;
+    GlobalV4* tris = (GlobalV4*)sc.tri_isect;
     Hit h;
     h.t = tmax;
     h.b1 = 0.f;
@@ -106,14 +110,15 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r
             uint32_t first = (ref & ~kLeafBit) >> 3;
             uint32_t n = ref & 7u;
             for (uint32_t k = 0; k < n; ++k) {
-                GlobalF4* tp = tris + 3 * (size_t)(first + k);
-                float4 a = tp[0], e1 = tp[1], e2 = tp[2];
+                GlobalV4* tp = tris + 3 * (size_t)(first + k);
+                float4 a = as_f4(tp[0]), e1 = as_f4(tp[1]), e2 = as_f4(tp[2]);
                 if (COUNT) cnt.tris++;
                 if (tri_hit<CULL>(r, a, e1, e2, h.t, h.b1, h.b2)) h.prim = (int)(first + k);
             }
         } else {
-            GlobalF4* np = nodes + 4 * (size_t)ref;
-            float4 w0 = np[0], w1 = np[1], w2 = np[2], w3f = np[3];
+            GlobalV4* np = nodes + 4 * (size_t)ref;
+            float4 w0 = as_f4(np[0]), w1 = as_f4(np[1]), w2 = as_f4(np[2]);
+            const NativeF4 w3f = np[3];
             int4 w3 = make_int4(f_as_i(w3f.x), f_as_i(w3f.y), f_as_i(w3f.z), f_as_i(w3f.w));
             if (COUNT) cnt.nodes++;
             float tr, tl;
