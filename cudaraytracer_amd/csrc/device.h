@@ -10,6 +10,21 @@ namespace rt {
 // Scene arrays in HBM.  Triangles are stored twice, as the reference does
 // (once in BVH order for traversal and shading, once in scene order for the
 // light-triangle table used by area-light sampling).
+// Global-memory views of the scene arrays.  Their pointers reach the kernels
+// inside structs, where the compiler cannot tell global memory from LDS and
+// emits flat loads with combined waits; these casts let it issue global
+// loads.  float4 goes through the native vector type, which (unlike the
+// float4 class) copies out of a qualified address space in both the host and
+// the device compilation pass.  Device code only.
+typedef float NativeF4 __attribute__((ext_vector_type(4)));
+using GlobalV4 = const __attribute__((address_space(1))) NativeF4;
+template <typename T>
+using Global = const __attribute__((address_space(1))) T;
+RT_HD float4 as_f4(NativeF4 v) { return make_float4(v.x, v.y, v.z, v.w); }
+RT_HD float4 ldg4(const float4* p) { return as_f4(*(GlobalV4*)p); }
+template <typename T>
+RT_HD Global<T>& gref(const T* p) { return *(Global<T>*)p; }
+
 struct DevScene {
     const PackedNode* nodes;      // interior nodes, 64 B each
     uint32_t root_ref;

@@ -13,7 +13,7 @@ namespace rt {
 // half-integers (the addressing of a linear-filtered, normalised-coordinate
 // texture).  Weights are FP32.
 RT_HD vec3 env_texel(const DevScene& sc, int x, int y) {
-    const float* p = sc.env_rgb + 3 * ((size_t)y * sc.env_w + x);
+    Global<float>* p = &gref(sc.env_rgb + 3 * ((size_t)y * sc.env_w + x));
     return v3(p[0], p[1], p[2]);
 }
 RT_HD vec3 env_bilinear(const DevScene& sc, float u, float v) {
@@ -52,8 +52,9 @@ RT_HD float env_pdf_uv(const DevScene& sc, float u, float v) {
     if (ui > sc.env_w - 1) ui = sc.env_w - 1;
     int vi = (int)(v * (float)sc.env_h);
     if (vi > sc.env_h - 1) vi = sc.env_h - 1;
-    const float* row = sc.env_row_cdf + (size_t)vi * sc.env_w;
-    float p_row = vi == 0 ? sc.env_marg_cdf[0] : sc.env_marg_cdf[vi] - sc.env_marg_cdf[vi - 1];
+    Global<float>* row = &gref(sc.env_row_cdf + (size_t)vi * sc.env_w);
+    Global<float>* marg = &gref(sc.env_marg_cdf);
+    float p_row = vi == 0 ? marg[0] : marg[vi] - marg[vi - 1];
     float p_col = ui == 0 ? row[0] : row[ui] - row[ui - 1];
     float p = p_row * p_col;
     return (p * (float)sc.env_w) * (float)sc.env_h;
@@ -64,11 +65,11 @@ RT_HD void env_sample_uv(const DevScene& sc, float rv, float ru, float& u, float
     int left = 0, right = sc.env_h - 1;
     while (left <= right) {
         int mid = (left + right) / 2;
-        if (sc.env_marg_cdf[mid] < rv) left = mid + 1;
+        if (gref(sc.env_marg_cdf + mid) < rv) left = mid + 1;
         else right = mid - 1;
     }
     int vi = left;
-    const float* row = sc.env_row_cdf + (size_t)(vi < sc.env_h ? vi : sc.env_h - 1) * sc.env_w;
+    Global<float>* row = &gref(sc.env_row_cdf + (size_t)(vi < sc.env_h ? vi : sc.env_h - 1) * sc.env_w);
     left = 0;
     right = sc.env_w - 1;
     while (left <= right) {
@@ -88,8 +89,9 @@ RT_HD int alias_pick(const float2* t, int n, float u) {
     int i = (int)x;
     if (i > n - 1) i = n - 1;
     float f = x - (float)i;
-    float2 e = t[i];
-    return f < e.x ? i : f_as_i(e.y);
+    const float* e = reinterpret_cast<const float*>(t + i);  // (threshold, alias bits)
+    Global<float>* g = &gref(e);
+    return f < g[0] ? i : f_as_i(g[1]);
 }
 
 // Alias-table sampling of the same texel distribution as env_sample_uv (row

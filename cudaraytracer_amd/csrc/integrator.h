@@ -47,7 +47,7 @@ RT_HD bool sample_light(const DevScene& sc, Rng& rng, int& light_id, float& p_se
     if (id == sc.num_lights) id = sc.num_lights - 1;
     p_sel = 1.0f / (float)sc.num_lights;
     light_id = id;
-    const LightRec& L = sc.lights[id];
+    Global<LightRec>& L = gref(sc.lights + id);
     is_env = L.type == kEnvLight;
     if (is_env) {
         float rv = rng.next();
@@ -65,7 +65,7 @@ RT_HD bool sample_light(const DevScene& sc, Rng& rng, int& light_id, float& p_se
         v = 1 - v;
     }
     const float4* tp = sc.light_tris + 4 * (size_t)(L.tri_start + t);
-    float4 a = tp[0], b = tp[1], c = tp[2], n = tp[3];
+    float4 a = ldg4(tp), b = ldg4(tp + 1), c = ldg4(tp + 2), n = ldg4(tp + 3);
     float s0 = (1 - u) - v;
     lp = (s0 * v3(a.x, a.y, a.z) + u * v3(b.x, b.y, b.z)) + v * v3(c.x, c.y, c.z);
     ln = v3(n.x, n.y, n.z);
@@ -73,13 +73,13 @@ RT_HD bool sample_light(const DevScene& sc, Rng& rng, int& light_id, float& p_se
 }
 
 RT_HD vec3 light_emission(const DevScene& sc, int light_id, vec3 wi) {
-    const LightRec& L = sc.lights[light_id];
+    Global<LightRec>& L = gref(sc.lights + light_id);
     if (L.type == kEnvLight) return env_emission(sc, wi);
     return v3(L.emission[0], L.emission[1], L.emission[2]);
 }
 
 RT_HD float light_pdf_sa(const DevScene& sc, int light_id, vec3 origin, vec3 lp, vec3 ln) {
-    const LightRec& L = sc.lights[light_id];
+    Global<LightRec>& L = gref(sc.lights + light_id);
     if (L.type == kEnvLight) return env_pdf_from(sc, origin, lp);
     return area_light_pdf_sa(origin, lp, ln, L.area);
 }

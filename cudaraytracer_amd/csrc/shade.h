@@ -26,7 +26,7 @@ struct DevMaterial {
 };
 
 RT_HD DevMaterial load_material(const MaterialRec* mats, int id) {
-    const MaterialRec& m = mats[id];
+    Global<MaterialRec>& m = gref(mats + id);
     DevMaterial r;
     r.type = m.type;
     r.basecolor = v3(m.basecolor[0], m.basecolor[1], m.basecolor[2]);

@@ -80,23 +80,16 @@ struct TravCount {
     uint32_t nodes, tris;
 };
 
-// Address-space views: scene arrays are global memory and the stack column is
-// LDS.  The scene's pointers arrive through a struct, so without these casts
-// the compiler emits flat loads (and flat waits) for nodes, triangles and
-// stack pops.  Loads go through the native vector type, which (unlike
-// float4's class type) copies out of a qualified address space in both the
-// host and the device compilation pass.
-typedef float NativeF4 __attribute__((ext_vector_type(4)));
-using GlobalV4 = const __attribute__((address_space(1))) NativeF4;
+// The stack column is LDS; node and triangle loads use the global views of
+// device.h, so the loop issues global and LDS instructions, not flat ones.
 using LdsU32 = __attribute__((address_space(3))) uint32_t;
-__device__ __forceinline__ float4 as_f4(NativeF4 v) { return make_float4(v.x, v.y, v.z, v.w); }
 
 // `stk` points at this lane's stack column in LDS; entries are `stride` words apart.
 template <int STACK, bool COUNT, bool CULL = true>
 __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r, float tmin, float tmax,
                                              uint32_t* stk_generic, int stride, TravCount& cnt) {
     LdsU32* stk = (LdsU32*)stk_generic;
-    GlobalV4* nodes = (GlobalV4*)sc.nodes;
+    GlobalV4* nodes = (GlobalV4*)sc.nodes;  // 4 vectors per node
     GlobalV4* tris = (GlobalV4*)sc.tri_isect;
     Hit h;
     h.t = tmax;
@@ -163,7 +156,8 @@ struct SurfaceHit {
 
 RT_HD SurfaceHit surface_at(const float4* attr, int prim, float b1, float b2) {
     const float4* a = attr + kAttrStride * (size_t)prim;
-    float4 a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5], a6 = a[6], a7 = a[7];
+    float4 a0 = ldg4(a), a1 = ldg4(a + 1), a2 = ldg4(a + 2), a3 = ldg4(a + 3), a4 = ldg4(a + 4), a5 = ldg4(a + 5),
+           a6 = ldg4(a + 6), a7 = ldg4(a + 7);
     float b0 = (1.f - b1) - b2;
     SurfaceHit s;
     vec3 v0 = v3(a0.x, a0.y, a0.z), v1 = v3(a1.x, a1.y, a1.z), v2 = v3(a2.x, a2.y, a2.z);
@@ -180,12 +174,12 @@ RT_HD SurfaceHit surface_at(const float4* attr, int prim, float b1, float b2) {
 }
 
 RT_HD int shade_class_of(const float4* attr, int prim) {
-    return prim < 0 ? (int)kClsTerminal : f_as_i(attr[kAttrStride * (size_t)prim + 7].w);
+    return prim < 0 ? (int)kClsTerminal : f_as_i(ldg4(attr + kAttrStride * (size_t)prim + 7).w);
 }
 
 RT_HD vec3 position_at(const float4* attr, int prim, float b1, float b2) {
     const float4* a = attr + kAttrStride * (size_t)prim;
-    float4 a0 = a[0], a1 = a[1], a2 = a[2];
+    float4 a0 = ldg4(a), a1 = ldg4(a + 1), a2 = ldg4(a + 2);
     float b0 = (1.f - b1) - b2;
     vec3 v0 = v3(a0.x, a0.y, a0.z), v1 = v3(a1.x, a1.y, a1.z), v2 = v3(a2.x, a2.y, a2.z);
     return (b0 * v0 + b1 * v1) + b2 * v2;
