@@ -104,6 +104,48 @@ def test_svgf_frames_bit_exact_terrain_env_sss(scene_dir, oracle_lib):
     _run(scene_dir["terrain_small"], spp=2, atrous_iters=3)
 
 
+@pytest.mark.parametrize("w,h", [(37, 23), (2, 3), (1, 1)])
+def test_svgf_ragged_and_tiny_images(tmp_path, oracle_lib, w, h):
+    """Image sizes that no tile shape divides, down to one pixel: every buffer
+    of every frame stays bit-identical to the oracle (tile bounds, the
+    G-buffer's quad partners outside the image, the filters' border taps)."""
+    from cudaraytracer_amd import scenes
+
+    path = scenes.write_scene(scenes.cornell_spheres(width=w, height=h, spp=1, depth=4), str(tmp_path),
+                              f"cs_{w}x{h}")
+    _run(path, spp=1, atrous_iters=3)
+
+
+def test_pool_widest_and_narrowest_tiles_on_ragged_image(tmp_path):
+    """A 37x23 real-time job in the pool with 128x2 and 8x32 tiles (most lanes
+    of a tile fall outside the image) denoises exactly as the job alone."""
+    from cudaraytracer_amd import scenes
+
+    path = scenes.write_scene(scenes.cornell_spheres(width=37, height=23, spp=1, depth=4), str(tmp_path), "cs37")
+    scene = rt.Scene(path)
+    r = rt.Renderer(scene)
+    c0 = r.camera()
+    cams = [rt.Camera(tuple(c0.target), c0.distance, c0.theta_deg, c0.phi_deg + 1.5 * f) for f in range(3)]
+    alone = []
+    for cam in cams:
+        r.set_camera(cam)
+        r.render_frame(1)
+        alone.append(r.read_frame(rt.RT_FRAME_DENOISED))
+    pool = rt.Pool()
+    slot = pool.add_scene(scene)
+    for tw in (128, 8):
+        pool.set_tuning(1.0, 0, tw)
+        job = pool.submit(rt.Job(slot, spp=1, frames=3, cameras=cams))
+        f = 0
+        while pool.step() > 0:
+            st = pool.status(job)
+            if st.frames_done > f:
+                _same(pool.read_frame(job, rt.RT_FRAME_DENOISED, 37, 23), alone[f], f"tile {tw} frame {f}")
+                f = st.frames_done
+        assert f == 3
+        pool.release(job)
+
+
 def test_frame_api_errors_and_accumulation(scene_dir, oracle_lib):
     scene = rt.Scene(scene_dir["cornell_spheres_rt"])
     r = rt.Renderer(scene)
