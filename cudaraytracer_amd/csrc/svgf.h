@@ -419,27 +419,61 @@ RT_HD void variance_pixel(int i, int j, int w, int h, const SvgfParams& p, const
 // AtrousKernel (reference: renderer.cuh:807-954): 3x3-filtered variance,
 // history bootstrap, 5x5 B3 taps at `step`, variance propagated with w^2.
 // Taps read the per-pixel guide; centre-only denominators are formed once.
-RT_HD void atrous_pixel(int i, int j, int w, int h, int step, const SvgfParams& p, const FilterIO& io,
-                        const GBufferView& g) {
-    size_t idx = (size_t)j * w + i;
-    const int2 ckl = io.guide_kl[idx];
+//
+// The pixel's arithmetic is written once (atrous_core) over a tap source:
+// tap (ki, kj) is the pixel at (i + ki * step, j + kj * step) and variance
+// tap (vi, vj) the pixel at (i + vi, j + vj).  key() reports whether the tap
+// lies inside the image and gives its guide key and history length; data()
+// gives its guide normal/depth and the two input colours; var() gives the two
+// input variances of a unit-offset neighbour.  AtrousGlobal reads global
+// memory; the tiled kernel's source reads the LDS tile it staged.
+
+struct AtrousGlobal {
+    const FilterIO& io;
+    int i, j, w, h, step;
+    RT_HD bool key(int ki, int kj, int2& kl) const {
+        const int ni = i + ki * step, nj = j + kj * step;
+        if (!(ni >= 0 && ni < w && nj >= 0 && nj < h)) return false;
+        kl = io.guide_kl[(size_t)nj * w + ni];
+        return true;
+    }
+    RT_HD void data(int ki, int kj, float4& nd, float4& cd, float4& ci) const {
+        const size_t n = (size_t)(j + kj * step) * w + (i + ki * step);
+        nd = io.guide_nd[n];
+        cd = io.in_d[n];
+        ci = io.in_i[n];
+    }
+    RT_HD bool var(int vi, int vj, float& vd, float& vv) const {
+        const int si = i + vi, sj = j + vj;
+        if (si < 0 || si >= w || sj < 0 || sj >= h) return false;
+        const size_t v = (size_t)sj * w + si;
+        vd = io.in_d[v].w;
+        vv = io.in_i[v].w;
+        return true;
+    }
+};
+
+template <class Src>
+RT_HD void atrous_core(size_t idx, int step, const SvgfParams& p, const FilterIO& io, const GBufferView& g,
+                       const Src& src) {
+    int2 ckl;
+    src.key(0, 0, ckl);
     int chl = ckl.y > 1 ? ckl.y : 1;
-    float4 cd4 = io.in_d[idx], ci4 = io.in_i[idx];
+    float4 cg, cd4, ci4;
+    src.data(0, 0, cg, cd4, ci4);
     vec3 cd = v3(cd4.x, cd4.y, cd4.z), ci = v3(ci4.x, ci4.y, ci4.z);
     float cld = luminance(cd), cli = luminance(ci);
-    const float4 cg = io.guide_nd[idx];
     float cdepth = cg.w;
     float dz = fmaxf(g.motion_depth[idx].w, 1e-4f);
     vec3 cn = v3(cg.x, cg.y, cg.z);
     float fvd = 0.f, fvi = 0.f, fvw = 0.f;
     for (int vj = -1; vj <= 1; ++vj)
         for (int vi = -1; vi <= 1; ++vi) {
-            int si = i + vi, sj = j + vj;
-            if (si < 0 || si >= w || sj < 0 || sj >= h) continue;
-            size_t v = (size_t)sj * w + si;
+            float vd, vv;
+            if (!src.var(vi, vj, vd, vv)) continue;
             float vw = (vi == 0 ? 0.5f : 0.25f) * (vj == 0 ? 0.5f : 0.25f);
-            fvd += vw * fmaxf(0.0f, io.in_d[v].w);
-            fvi += vw * fmaxf(0.0f, io.in_i[v].w);
+            fvd += vw * fmaxf(0.0f, vd);
+            fvi += vw * fmaxf(0.0f, vv);
             fvw += vw;
         }
     fvw = fmaxf(fvw, 1e-6f);
@@ -457,14 +491,12 @@ RT_HD void atrous_pixel(int i, int j, int w, int h, int step, const SvgfParams& 
     float twd = 0.f, twi = 0.f, vard = 0.f, vari = 0.f;
     for (int kj = -2; kj <= 2; ++kj)
         for (int ki = -2; ki <= 2; ++ki) {
-            int ni = i + ki * step, nj = j + kj * step;
-            if (!(ni >= 0 && ni < w && nj >= 0 && nj < h)) continue;
-            size_t n = (size_t)nj * w + ni;
-            const int2 nkl = io.guide_kl[n];
+            int2 nkl;
+            if (!src.key(ki, kj, nkl)) continue;
             if (nkl.x != ckl.x) continue;
             int nhl = nkl.y > 1 ? nkl.y : 1;
-            float4 ng = io.guide_nd[n];
-            float4 nd4 = io.in_d[n], ni4 = io.in_i[n];
+            float4 ng, nd4, ni4;
+            src.data(ki, kj, ng, nd4, ni4);
             vec3 nd = v3(nd4.x, nd4.y, nd4.z), nic = v3(ni4.x, ni4.y, ni4.z);
             float nvd = fmaxf(0.0f, nd4.w), nvi = fmaxf(0.0f, ni4.w);
             float dist = kTapDist[ki * ki + kj * kj] * (float)step;
@@ -490,6 +522,11 @@ RT_HD void atrous_pixel(int i, int j, int w, int h, int step, const SvgfParams& 
     else io.out_d[idx] = make_float4(cd.x, cd.y, cd.z, 1.f);
     if (twi > 1e-6f) io.out_i[idx] = make_float4(fi.x / twi, fi.y / twi, fi.z / twi, vari / (twi * twi));
     else io.out_i[idx] = make_float4(ci.x, ci.y, ci.z, 1.f);
+}
+
+RT_HD void atrous_pixel(int i, int j, int w, int h, int step, const SvgfParams& p, const FilterIO& io,
+                        const GBufferView& g) {
+    atrous_core((size_t)j * w + i, step, p, io, g, AtrousGlobal{io, i, j, w, h, step});
 }
 
 }  // namespace rt
