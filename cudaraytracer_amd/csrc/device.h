@@ -110,7 +110,7 @@ struct PassHeader {
     int n_jobs;
     int n_paths;
     int n_pixels;
-    int pad;
+    int region;  // queue entries per slice in this pass: ceil(n_paths / kSlices)
 };
 // Device pass table: header, then the jobs, 256-byte aligned.
 constexpr size_t kPassHeaderBytes = 256;

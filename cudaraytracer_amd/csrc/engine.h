@@ -117,7 +117,6 @@ private:
     float grid_scale_ = 1.f;
     std::map<uint64_t, hipGraphExec_t> graphs_;
     size_t cap_ = 0;
-    int region_ = 0;
     hipStream_t stream_ = nullptr;
     DevBuf qo_[2], qd_[2], qw_[2], hits_, lists_, shadow_, slot_d_, slot_i_, counters_, stats_;
     // Pass tables: one device table, filled by stream-ordered copies from a

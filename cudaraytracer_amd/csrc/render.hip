@@ -152,9 +152,10 @@ __device__ __forceinline__ int job_of_pixel(const JobDev* __restrict__ jobs, int
 template <bool POOLED>
 __global__ __launch_bounds__(kBlock) void k_raygen(const PassHeader* __restrict__ hdr, const JobDev* __restrict__ jobs,
                                                     float4* qo, float4* qd, float4* qw, float4* slot_d, float4* slot_i,
-                                                    int* counters, int qslot, int region) {
+                                                    int* counters, int qslot) {
     const int n_paths = hdr->n_paths;
     const int n_jobs = POOLED ? hdr->n_jobs : 1;
+    const int region = hdr->region;
     int stride = gridDim.x * blockDim.x;
     if (blockIdx.x == 0 && threadIdx.x < kSlices) {
         int sl = threadIdx.x;
@@ -200,8 +201,10 @@ extern __shared__ uint32_t stk_dyn[];
 template <bool COUNT, bool POOLED>
 __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jobs, const float4* __restrict__ qo,
                                                     const float4* __restrict__ qd, float4* __restrict__ hits,
-                                                    int* __restrict__ lists, int cap, int region, int* counters,
-                                                    int qslot, unsigned long long* stats) {
+                                                    int* __restrict__ lists, int cap,
+                                                    const PassHeader* __restrict__ ph, int* counters, int qslot,
+                                                    unsigned long long* stats) {
+    const int region = ph->region;
     uint32_t* stk = stk_dyn;
     if (blockIdx.x == 0 && threadIdx.x < kSlices) {
         int sl = threadIdx.x;
@@ -254,7 +257,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
                                                    const int* __restrict__ list, float4* __restrict__ no,
                                                    float4* __restrict__ nd, float4* __restrict__ nw,
                                                    float4* __restrict__ shadow, float4* slot_d, float4* slot_i,
-                                                   int region, int* counters, int qslot) {
+                                                   const PassHeader* __restrict__ ph, int* counters, int qslot) {
+    const int region = ph->region;
     uint32_t* stk = stk_dyn;  // sized only for the subsurface class (its probe rays)
     if (blockIdx.x == 0 && threadIdx.x < kSlices) *ctr_at(counters, kFetchExtend, threadIdx.x) = 0;
     uint32_t* col = stk + (CLS == kClsSubsurface ? threadIdx.x : 0);
@@ -334,8 +338,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
 // then the reference's 0.1 distance tolerances (Src/Core/light.cu:5-27).
 template <bool COUNT, bool POOLED>
 __global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jobs, const float4* __restrict__ shadow,
-                                                    float4* slot_d, float4* slot_i, int region, int* counters,
-                                                    unsigned long long* stats) {
+                                                    float4* slot_d, float4* slot_i, const PassHeader* __restrict__ ph,
+                                                    int* counters, unsigned long long* stats) {
+    const int region = ph->region;
     uint32_t* stk = stk_dyn;
     if (blockIdx.x == 0 && threadIdx.x < kSlices) {
         int sl = threadIdx.x;
@@ -894,7 +899,6 @@ Engine::Engine(size_t capacity_paths, bool profile) {
     shadow_.alloc(cap_ * 64);
     slot_d_.alloc(cap_ * 16);
     slot_i_.alloc(cap_ * 16);
-    region_ = (int)((cap_ + kSlices - 1) / kSlices);
     counters_.alloc(kCounterInts * sizeof(int));
     stats_.alloc(kNumStats * sizeof(unsigned long long));
     tables_.alloc(kPassTableBytes);
@@ -1002,7 +1006,9 @@ void Engine::run(const std::vector<PassItem>& items, bool count) {
     hdr->n_jobs = n;
     hdr->n_paths = (int)paths;
     hdr->n_pixels = (int)pixels;
-    hdr->pad = 0;
+    // Slices split this pass's paths evenly (not the engine's capacity), so a
+    // pass smaller than the capacity still spreads over all 64 slices and XCDs.
+    hdr->region = (int)((paths + kSlices - 1) / kSlices);
     // One device table: stream order puts this copy after the previous pass's
     // kernels, so every pass (and every replay of a captured pass) reads the
     // same address; the host copy stays untouched until the copy has run.
@@ -1066,18 +1072,18 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
     hipEvent_t b = prof.begin(stream_);
     k_raygen<POOLED><<<grid_linear_ * 4, kBlock, 0, stream_>>>(hdr, jobs, qo_[q].as<float4>(), qd_[q].as<float4>(),
                                                                qw_[q].as<float4>(), slot_d_.as<float4>(),
-                                                               slot_i_.as<float4>(), ctr, q, region_);
+                                                               slot_i_.as<float4>(), ctr, q);
     HIP_OK(hipGetLastError());
     prof.end(kStageRaygen, b, stream_);
     for (int bounce = 0; bounce < max_bounces; ++bounce) {
         b = prof.begin(stream_);
         if (count)
             k_extend<true, POOLED><<<gext, kBlock, lds, stream_>>>(
-                jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), hits_.as<float4>(), lists_.as<int>(), cap, region_,
+                jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), hits_.as<float4>(), lists_.as<int>(), cap, hdr,
                 ctr, q, st);
         else
             k_extend<false, POOLED><<<gext, kBlock, lds, stream_>>>(
-                jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), hits_.as<float4>(), lists_.as<int>(), cap, region_,
+                jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), hits_.as<float4>(), lists_.as<int>(), cap, hdr,
                 ctr, q, st);
         HIP_OK(hipGetLastError());
         prof.end(kStageExtend, b, stream_);
@@ -1088,7 +1094,7 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
             k_shade<CLS, POOLED><<<grid, kBlock, CLS == kClsSubsurface ? lds : 0, stream_>>>(
                 jobs, bounce, qo_[q].as<float4>(), qd_[q].as<float4>(), qw_[q].as<float4>(), hits_.as<float4>(),
                 lists_.as<int>() + (size_t)CLS * cap_, qo_[q ^ 1].as<float4>(), qd_[q ^ 1].as<float4>(),
-                qw_[q ^ 1].as<float4>(), shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), region_,
+                qw_[q ^ 1].as<float4>(), shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), hdr,
                 ctr, q);
             HIP_OK(hipGetLastError());
         };
@@ -1101,10 +1107,10 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
         b = prof.begin(stream_);
         if (count)
             k_shadow<true, POOLED><<<gsh, kBlock, lds, stream_>>>(
-                jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), region_, ctr, st);
+                jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, st);
         else
             k_shadow<false, POOLED><<<gsh, kBlock, lds, stream_>>>(
-                jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), region_, ctr, st);
+                jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, st);
         HIP_OK(hipGetLastError());
         prof.end(kStageShadow, b, stream_);
         q ^= 1;
