@@ -355,7 +355,7 @@ def batched_bench(n: int = 64, seed: int = 4, queue_gib: int = 16):
     }
 
 
-def serving_bench(rates=(5.0, 20.0, 100.0), n_jobs: int = 30, frames: int = 10, width: int = 1280,
+def serving_bench(rates=(5.0, 10.0, 20.0, 50.0, 100.0, 200.0), n_jobs: int = 30, frames: int = 10, width: int = 1280,
                   height: int = 720, seed: int = 5, queue_gib: int = 16):
     """Poisson stream of real-time jobs on terrain + SSS + env scenes (2 M,
     1 M and 0.5 M triangles): each job a `frames`-frame orbit at width x height,
