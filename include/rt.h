@@ -242,7 +242,8 @@ typedef struct rt_serve_report {
 typedef struct rt_pool_tuning {
     float grid_scale;    /* persistent stage grids = occupancy x CUs x scale (1 = default) */
     int64_t pass_paths;  /* paths per pass; 0 = the queue capacity */
-    int tile_width;      /* per-pixel frame kernels on tile_width x (256 / tile_width) tiles: 8..128 */
+    int tile_width;      /* per-pixel frame kernels on tile_width x (256 / tile_width) tiles: 8..128
+                            (the first a-trous pass always runs on 16 x 16 LDS tiles) */
 } rt_pool_tuning;
 
 typedef struct rt_tune_report {
