@@ -119,7 +119,6 @@ RT_HD float burley_pdf(const DevMaterial& m, float r) {
 
 // Subsurface probe: two closest-hit rays along +-n through the disc point
 // (reference: Src/Core/material.cu:405-441).
-template <int STACK>
 __device__ __forceinline__ void sss_probe(const DevScene& sc, vec3 hit_p, vec3 normal, float dist, float angle,
                                          vec3& out_p, vec3& out_n, uint32_t* stk, int stride) {
     vec3 t, b;
@@ -133,8 +132,8 @@ __device__ __forceinline__ void sss_probe(const DevScene& sc, vec3 hit_p, vec3 n
     RayInv fw = make_ray(out_p + pn * ray_off, -pn);
     RayInv bw = make_ray(out_p - pn * ray_off, pn);
     TravCount c{0, 0};
-    Hit hf = trace_closest<STACK, false>(sc, fw, tmin, tmax, stk, stride, c);
-    Hit hb = trace_closest<STACK, false>(sc, bw, tmin, tmax, stk, stride, c);
+    Hit hf = trace_closest<false>(sc, fw, tmin, tmax, stk, stride, c);
+    Hit hb = trace_closest<false>(sc, bw, tmin, tmax, stk, stride, c);
     bool f = hf.prim >= 0, bk = hb.prim >= 0;
     if (f && bk) {
         const Hit& h = hf.t <= hb.t ? hf : hb;
@@ -158,7 +157,7 @@ __device__ __forceinline__ void sss_probe(const DevScene& sc, vec3 hit_p, vec3 n
 // One bounce of one path.  `hit` holds (t, b1, b2, prim) from the extend
 // stage; CLS is the shading class of what was hit, fixed at compile time so
 // each class kernel carries only its own material code.
-template <int CLS, int STACK>
+template <int CLS>
 __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FrameParams& fp, PathState& ps,
                                               float4 hit, uint32_t pixel, uint32_t sample, uint32_t* stk,
                                               int stride) {
@@ -340,7 +339,7 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
         }
         float angle = (float)((double)(rng.next() * 2.f) * kPiD);
         vec3 out_p, out_n;
-        sss_probe<STACK>(sc, hit_p, geom_n, dist, angle, out_p, out_n, stk, stride);
+        sss_probe(sc, hit_p, geom_n, dist, angle, out_p, out_n, stk, stride);
         float sd = length(out_p - hit_p);
         float spatial_pdf = sd < 1e-6f ? 0.f : burley_pdf(mat, sd);
         if ((double)spatial_pdf < kEpsD) return out;

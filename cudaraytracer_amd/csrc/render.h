@@ -81,7 +81,6 @@ public:
     GpuScene(const GpuScene&) = delete;
     GpuScene& operator=(const GpuScene&) = delete;
     const DevScene& dev() const;
-    int stack() const;
     int max_stack() const;  // traversal stack entries the tree needs (depth + 1)
     bool has_class(int shading_class) const;
     size_t device_bytes() const;

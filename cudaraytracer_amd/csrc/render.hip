@@ -227,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jo
             float4 o = qo[i], d = qd[i];
             const DevScene& sc = pass_job<POOLED>(jobs, (int)((uint32_t)f_as_i(d.w) >> kJobShift)).sc;
             RayInv r = make_ray(v3(o.x, o.y, o.z), v3(d.x, d.y, d.z));
-            Hit h = trace_closest<0, COUNT>(sc, r, 0.f, 1e8f, col, kBlock, c);
+            Hit h = trace_closest<COUNT>(sc, r, 0.f, 1e8f, col, kBlock, c);
             hits[i] = make_float4(h.t, h.b1, h.b2, i_as_f(h.prim));
             cls = shade_class_of(sc.tri_attr, h.prim);
         }
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
             const int local = pid - J.path_base;
             uint32_t pixel = (uint32_t)(local % fp.wh);
             uint32_t sample = fp.sample0 + (uint32_t)(local / fp.wh);
-            so = shade_path<CLS, 0>(J.sc, fp, ps, hits[i], pixel, sample, col, kBlock);
+            so = shade_path<CLS>(J.sc, fp, ps, hits[i], pixel, sample, col, kBlock);
             if (so.has_emit) {
                 float4* s = so.add_direct ? slot_d : slot_i;
                 float4 a = s[pid];
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jo
             rec.hp = v3(r2.x, r2.y, r2.z);
             vec3 org = rec.lp + 0.01f * rec.dir;
             RayInv ray = make_ray(org, rec.dir);
-            Hit h = trace_closest<0, COUNT>(sc, ray, 1e-6f, 1e8f, col, kBlock, c);
+            Hit h = trace_closest<COUNT>(sc, ray, 1e-6f, 1e8f, col, kBlock, c);
             bool hit = h.prim >= 0;
             vec3 tp = v3(0.f, 0.f, 0.f);
             if (hit) tp = position_at(sc.tri_attr, h.prim, h.b1, h.b2);
@@ -428,17 +428,16 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
-// Closest hit for an arbitrary ray batch: rays are (o, tmin), (d, tmax).
-template <int STACK>
+// Closest hit for an arbitrary ray batch: rays are (o, tmin), (d, tmax).  The
+// stack column is dynamic LDS sized to the tree, as in extend.
 __global__ __launch_bounds__(kBlock) void k_trace_batch(DevScene sc, const float4* __restrict__ rays, int n,
                                                          float4* __restrict__ out, uint2* __restrict__ counts) {
-    __shared__ uint32_t stk[STACK * kBlock];
-    uint32_t* col = stk + threadIdx.x;
+    uint32_t* col = stk_dyn + threadIdx.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 o = rays[2 * (size_t)i], d = rays[2 * (size_t)i + 1];
         RayInv r = make_ray(v3(o.x, o.y, o.z), v3(d.x, d.y, d.z));
         TravCount c{0, 0};
-        Hit h = trace_closest<STACK, true>(sc, r, o.w, d.w, col, kBlock, c);
+        Hit h = trace_closest<true>(sc, r, o.w, d.w, col, kBlock, c);
         out[i] = make_float4(h.t, h.b1, h.b2, i_as_f(h.prim));
         counts[i] = make_uint2(c.nodes, c.tris);
     }
@@ -490,7 +489,7 @@ __device__ __forceinline__ bool tile_pixel(int w, int h, int tile_shift, int& i,
 __global__ __launch_bounds__(kBlock) void k_gbuffer(DevScene sc, GBufferParams g, GBufferView out, int ts) {
     int i, j;
     if (!tile_pixel(g.width, g.height, ts, i, j)) return;
-    gbuffer_pixel<0>(sc, g, i, j, stk_dyn + threadIdx.x, kBlock, out);
+    gbuffer_pixel(sc, g, i, j, stk_dyn + threadIdx.x, kBlock, out);
 }
 
 __global__ __launch_bounds__(kBlock) void k_temporal(int w, int h, int ts, int first, SvgfParams p, TemporalIO io,
@@ -592,7 +591,6 @@ struct GpuScene::Impl {
     DevBuf nodes, tri_isect, tri_attr, materials, lights, light_tris, env_rgb, env_row, env_marg, env_alias_row,
         env_alias_marg;
     DevScene dev{};
-    int stack = 32;
     int max_stack = 1;
     bool has_class[kNumClasses] = {true, false, false, false, false};
     int device = 0;
@@ -609,7 +607,6 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
     const BvhBuild& bvh = sc.bvh;
     int n = sc.num_tris();
     if (bvh.max_stack > 64) throw std::runtime_error("BVH deeper than the 64-entry traversal stack");
-    I.stack = bvh.max_stack <= 32 ? 32 : 64;
     I.max_stack = std::max(1, bvh.max_stack);
     // Packed nodes; a leaf-only tree gets one dummy line so the pointer is valid.
     std::vector<PackedNode> nodes = bvh.packed;
@@ -711,7 +708,6 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
 GpuScene::~GpuScene() { delete impl_; }
 
 const DevScene& GpuScene::dev() const { return impl_->dev; }
-int GpuScene::stack() const { return impl_->stack; }
 int GpuScene::max_stack() const { return impl_->max_stack; }
 bool GpuScene::has_class(int c) const { return impl_->has_class[c]; }
 size_t GpuScene::device_bytes() const {
@@ -1510,10 +1506,8 @@ void trace_batch(const GpuScene& scene, const float* rays, int n, float* out_hit
     c.alloc((size_t)n * 8);
     HIP_OK(hipMemcpy(r.p, rays, (size_t)n * 32, hipMemcpyHostToDevice));
     int grid = std::max(1, std::min((n + kBlock - 1) / kBlock, 4096));
-    if (scene.stack() == 32)
-        k_trace_batch<32><<<grid, kBlock>>>(scene.dev(), r.as<float4>(), n, o.as<float4>(), c.as<uint2>());
-    else
-        k_trace_batch<64><<<grid, kBlock>>>(scene.dev(), r.as<float4>(), n, o.as<float4>(), c.as<uint2>());
+    const size_t lds = (size_t)scene.max_stack() * kBlock * sizeof(uint32_t);
+    k_trace_batch<<<grid, kBlock, lds>>>(scene.dev(), r.as<float4>(), n, o.as<float4>(), c.as<uint2>());
     HIP_OK(hipGetLastError());
     HIP_OK(hipDeviceSynchronize());
     HIP_OK(hipMemcpy(out_hits, o.p, (size_t)n * 16, hipMemcpyDeviceToHost));

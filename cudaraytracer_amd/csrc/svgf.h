@@ -69,14 +69,13 @@ RT_HD vec3 decode_normal(ushort4 q) {
 // the absolute depth differences to the horizontal and vertical quad partner,
 // measured on the plane of the same triangle (what a rasteriser's helper
 // lanes extrapolate).
-template <int STACK>
 __device__ void gbuffer_pixel(const DevScene& sc, const GBufferParams& g, int i, int j, uint32_t* stk, int stride,
                               GBufferView out) {
     size_t idx = (size_t)j * g.width + i;
     vec3 d = pixel_center_dir(g, i, j);
     RayInv r = make_ray(g.cam_pos, d);
     TravCount c{0, 0};
-    Hit h = trace_closest<STACK, false, false>(sc, r, 0.f, 1e8f, stk, stride, c);
+    Hit h = trace_closest<false, false>(sc, r, 0.f, 1e8f, stk, stride, c);
     if (h.prim < 0) {
         out.pos_tri[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
         out.nrm_mat[idx] = make_ushort4(0, 0, 0, 0);

@@ -85,7 +85,7 @@ struct TravCount {
 using LdsU32 = __attribute__((address_space(3))) uint32_t;
 
 // `stk` points at this lane's stack column in LDS; entries are `stride` words apart.
-template <int STACK, bool COUNT, bool CULL = true>
+template <bool COUNT, bool CULL = true>
 __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r, float tmin, float tmax,
                                              uint32_t* stk_generic, int stride, TravCount& cnt) {
     LdsU32* stk = (LdsU32*)stk_generic;
