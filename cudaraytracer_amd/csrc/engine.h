@@ -135,9 +135,14 @@ class FrameState {
 public:
     FrameState(int width, int height);
     // Start a frame seen from `cam`: G-buffer against the previous frame's
-    // camera, and cleared accumulators for the frame's samples.
+    // camera, and cleared accumulators for the frame's samples.  With a
+    // G-buffer stream the G-buffer runs there, beside the frame's path tracing
+    // on `s` (it needs only the camera); `s` waits for it in end().
     void begin(const GpuScene& gs, const HostScene& host, const CameraRec& cam, float4* acc_d, float4* acc_i,
-               Profiler& prof, hipStream_t s);
+               Profiler& prof, hipStream_t s, hipStream_t gbuf_stream = nullptr);
+    ~FrameState();
+    FrameState(const FrameState&) = delete;
+    FrameState& operator=(const FrameState&) = delete;
     // Finish it: temporal blend of the frame's `spp` samples, variance, a-trous, combine.
     void end(int spp, const SvgfParams& p, const float4* acc_d, const float4* acc_i, Profiler& prof,
              hipStream_t s);
@@ -174,6 +179,8 @@ private:
     int fcur_ = 0, flast_ = -1, frames_ = 0, out_sel_ = -1;
     int tile_shift_ = 4;
     bool started_ = false;
+    hipEvent_t ev_order_ = nullptr, ev_gbuf_ = nullptr;  // G-buffer stream ordering
+    bool gbuf_pending_ = false;                          // end() must wait for ev_gbuf_
 };
 
 // Read accumulators (which: 0 direct sums, 1 indirect sums, 2 (direct + indirect) / spp).

@@ -385,7 +385,7 @@ int Pool::step_serial() {
         if (j->st.start_s < 0) j->st.start_s = t_start;
         if (j->d.frames > 0)
             j->fs->begin(*I.scenes[j->d.scene].gpu, *I.scenes[j->d.scene].host, j->d.cams[j->st.frames_done],
-                         j->acc_d->as<float4>(), j->acc_i->as<float4>(), I.eng->prof, s);
+                         j->acc_d->as<float4>(), j->acc_i->as<float4>(), I.eng->prof, s, I.side);
     }
     I.eng->run(items, false);
     for (size_t k = 0; k < served.size(); ++k) {

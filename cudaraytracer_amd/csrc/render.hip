@@ -1170,6 +1170,13 @@ FrameState::FrameState(int width, int height) : w_(width), h_(height), wh_(width
     guide_nd_.alloc(wh * 16);
     guide_kl_.alloc(wh * 8);
     lum_.alloc(wh * 16);
+    HIP_OK(hipEventCreateWithFlags(&ev_order_, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&ev_gbuf_, hipEventDisableTiming));
+}
+
+FrameState::~FrameState() {
+    if (ev_order_) (void)hipEventDestroy(ev_order_);
+    if (ev_gbuf_) (void)hipEventDestroy(ev_gbuf_);
 }
 
 size_t FrameState::device_bytes() const { return (size_t)wh_ * kFrameStateBytesPerPixel; }
@@ -1204,19 +1211,30 @@ static GBufferParams gbuffer_params(const HostScene& h, const CameraRec& cur, co
 }
 
 void FrameState::begin(const GpuScene& gs, const HostScene& host, const CameraRec& cam, float4* acc_d,
-                       float4* acc_i, Profiler& prof, hipStream_t s) {
+                       float4* acc_i, Profiler& prof, hipStream_t s, hipStream_t gbuf_stream) {
     if (cam.width != w_ || cam.height != h_) throw std::invalid_argument("camera resolution differs from the view's");
     const int c = fcur_;
     const bool first = frames_ == 0;
     const int ts = tile_shift_, tw = 1 << ts, th = kBlock >> ts;
     const int tiles = ((w_ + tw - 1) / tw) * ((h_ + th - 1) / th);
-    hipEvent_t e = prof.begin(s);
+    // Slot c was the previous frame's "previous G-buffer": a G-buffer stream
+    // first waits for everything enqueued on `s` so far.
+    hipStream_t gsm = gbuf_stream ? gbuf_stream : s;
+    if (gbuf_stream) {
+        HIP_OK(hipEventRecord(ev_order_, s));
+        HIP_OK(hipStreamWaitEvent(gsm, ev_order_, 0));
+    }
+    hipEvent_t e = prof.begin(gsm);
     GBufferParams g = gbuffer_params(host, cam, first ? cam : last_cam_);
     GBufferView v = gview(g_pos_, g_nrm_, g_bary_, g_md_, c);
     const size_t lds = (size_t)gs.max_stack() * kBlock * sizeof(uint32_t);
-    k_gbuffer<<<tiles, kBlock, lds, s>>>(gs.dev(), g, v, ts);
+    k_gbuffer<<<tiles, kBlock, lds, gsm>>>(gs.dev(), g, v, ts);
     HIP_OK(hipGetLastError());
-    prof.end(kStageGBuffer, e, s);
+    prof.end(kStageGBuffer, e, gsm);
+    if (gbuf_stream) {
+        HIP_OK(hipEventRecord(ev_gbuf_, gsm));
+        gbuf_pending_ = true;
+    }
     // Fresh samples of this frame only (the reference resets its frame buffer
     // every denoised frame: renderer.cuh:1297-1302, 1121-1124).
     int grid = std::max(1, std::min((wh_ + kBlock - 1) / kBlock, 2048));
@@ -1231,6 +1249,10 @@ void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const fl
                      hipStream_t s) {
     if (!started_) throw std::logic_error("frame end without begin");
     started_ = false;
+    if (gbuf_pending_) {
+        HIP_OK(hipStreamWaitEvent(s, ev_gbuf_, 0));
+        gbuf_pending_ = false;
+    }
     const int c = fcur_, pv = c ^ 1;
     const bool first = frames_ == 0;
     const int ts = tile_shift_, tw = 1 << ts, th = kBlock >> ts;
@@ -1344,6 +1366,7 @@ struct Renderer::Impl {
     uint32_t sample_base = 0;  // sample index of the first sample in the accumulators
     CameraRec cam;             // camera of the following passes
     std::unique_ptr<FrameState> frames;
+    hipStream_t gbuf_stream = nullptr;  // real-time frames: G-buffer beside the path tracing
 };
 
 Renderer::Renderer(const GpuScene& scene, const HostScene& host, const RenderSettings& s) : impl_(new Impl) {
@@ -1374,6 +1397,10 @@ Renderer::Renderer(const GpuScene& scene, const HostScene& host, const RenderSet
 Renderer::~Renderer() {
     if (impl_) {
         if (impl_->eng) (void)hipStreamSynchronize(impl_->eng->stream());
+        if (impl_->gbuf_stream) {
+            (void)hipStreamSynchronize(impl_->gbuf_stream);
+            (void)hipStreamDestroy(impl_->gbuf_stream);
+        }
         delete impl_;
     }
 }
@@ -1484,8 +1511,10 @@ void Renderer::render_frame(int spp, const SvgfParams& p) {
     Impl& I = *impl_;
     if (spp < 1) throw std::invalid_argument("spp must be positive");
     if (!I.frames) I.frames.reset(new FrameState(I.width, I.height));
+    if (!I.gbuf_stream) HIP_OK(hipStreamCreateWithFlags(&I.gbuf_stream, hipStreamNonBlocking));
     hipStream_t s = I.eng->stream();
-    I.frames->begin(*I.scene, *I.host, I.cam, I.acc_d.as<float4>(), I.acc_i.as<float4>(), I.eng->prof, s);
+    I.frames->begin(*I.scene, *I.host, I.cam, I.acc_d.as<float4>(), I.acc_i.as<float4>(), I.eng->prof, s,
+                    I.gbuf_stream);
     I.spp_done = 0;
     I.sample_base = I.frames->next_sample0(spp);
     render(spp);
