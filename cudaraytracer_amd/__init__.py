@@ -61,7 +61,8 @@ EXPORTS = (
     "rt_render_params_default", "rt_scene_load", "rt_scene_info_get", "rt_scene_destroy",
     "rt_renderer_create", "rt_renderer_destroy", "rt_render", "rt_render_counted", "rt_synchronize",
     "rt_reset", "rt_reset_stats", "rt_read_accum", "rt_read_rgba8", "rt_stats", "rt_last_error",
-    "rt_scene_get_triangles", "rt_scene_get_meshes", "rt_scene_get_bvh", "rt_trace_rays", "rt_device_math",
+    "rt_scene_get_triangles", "rt_scene_get_meshes", "rt_scene_get_bvh", "rt_scene_get_texture",
+    "rt_texture_lookup", "rt_trace_rays", "rt_device_math",
     "rt_denoise_params_default", "rt_get_camera", "rt_set_camera", "rt_render_frame", "rt_reset_history",
     "rt_read_frame", "rt_pool_create", "rt_pool_destroy", "rt_pool_add_scene", "rt_pool_submit", "rt_pool_step",
     "rt_pool_job_status", "rt_pool_read_accum", "rt_pool_read_frame", "rt_pool_release", "rt_pool_stats",
@@ -85,7 +86,7 @@ class SceneInfo(ctypes.Structure):
         ("seed", ctypes.c_uint32), ("rr_start_depth", ctypes.c_int),
         ("bounds_lo", ctypes.c_float * 3), ("bounds_hi", ctypes.c_float * 3),
         ("camera_position", ctypes.c_float * 3), ("camera_left_down", ctypes.c_float * 3),
-        ("camera_du", ctypes.c_float * 3), ("camera_dv", ctypes.c_float * 3),
+        ("camera_du", ctypes.c_float * 3), ("camera_dv", ctypes.c_float * 3), ("num_textures", ctypes.c_int),
     ]
 
 
@@ -217,6 +218,9 @@ def lib() -> ctypes.CDLL:
     L.rt_scene_get_triangles.argtypes = [vp, fp, fp, fp, ctypes.POINTER(ctypes.c_int32)]
     L.rt_scene_get_meshes.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), fp]
     L.rt_scene_get_bvh.argtypes = [vp, ip, vp, ctypes.POINTER(ctypes.c_int32)]
+    L.rt_scene_get_texture.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_uint8)]
+    L.rt_texture_lookup.argtypes = [vp, ip, fp, ip, fp]
     L.rt_trace_rays.argtypes = [vp, fp, ip, fp, ctypes.POINTER(ctypes.c_uint32)]
     L.rt_device_math.argtypes = [ip, fp, fp, fp, ip]
     L.rt_denoise_params_default.argtypes = [ctypes.POINTER(DenoiseParams)]
@@ -310,6 +314,22 @@ class Scene:
         order = np.zeros(info.num_triangles, np.int32)
         _check(lib().rt_scene_get_bvh(self._h, int(from_packed), nodes.ctypes.data_as(ctypes.c_void_p), _iptr(order)))
         return nodes, order
+
+    def texture(self, index: int) -> np.ndarray:
+        """Decoded diffuse texture `index`: (h, w, 4) uint8, row 0 = top."""
+        w, h = ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib().rt_scene_get_texture(self._h, index, ctypes.byref(w), ctypes.byref(h), None))
+        out = np.zeros((h.value, w.value, 4), np.uint8)
+        _check(lib().rt_scene_get_texture(self._h, index, ctypes.byref(w), ctypes.byref(h),
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return out
+
+    def texture_lookup(self, index: int, uv: np.ndarray) -> np.ndarray:
+        """Albedo of texture `index` at (n, 2) float32 (u, v) pairs, on the GPU: (n, 3) float32."""
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        out = np.zeros((uv.shape[0], 3), np.float32)
+        _check(lib().rt_texture_lookup(self._h, index, _fptr(uv), uv.shape[0], _fptr(out)))
+        return out
 
     def trace(self, rays: np.ndarray):
         """Closest hits on the GPU. rays: (n, 8) float32 = o.xyz, tmin, d.xyz, tmax."""

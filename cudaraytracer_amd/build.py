@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librtamd.so")
 
-SOURCES = ["json.cpp", "bvh.cpp", "scene.cpp", "capi.cpp", "render.hip", "pool.cpp"]
+SOURCES = ["json.cpp", "image.cpp", "bvh.cpp", "scene.cpp", "capi.cpp", "render.hip", "pool.cpp"]
 
 # -ffp-contract=off keeps host and device FP32 results bit-identical; the
 # target-feature flag keeps the compiler from emitting packed FP32 ops.
@@ -54,7 +54,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-lz", "-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

@@ -187,6 +187,7 @@ int rt_scene_info_get(const rt_scene* s, rt_scene_info* o) {
     o->denoise = h.denoise ? 1 : 0;
     o->seed = h.seed;
     o->rr_start_depth = h.rr_start_depth;
+    o->num_textures = (int)h.textures.size();
     const rt::Box& b = h.bvh.bounds;
     float lo[3] = {b.lo.x, b.lo.y, b.lo.z}, hi[3] = {b.hi.x, b.hi.y, b.hi.z};
     std::memcpy(o->bounds_lo, lo, sizeof(lo));
@@ -615,6 +616,27 @@ int rt_scene_get_bvh(const rt_scene* s, int from_packed, void* nodes_out, int32_
             }
         }
         if (order_out) std::memcpy(order_out, b.order.data(), b.order.size() * 4);
+        return RT_OK;
+    });
+}
+
+int rt_scene_get_texture(const rt_scene* s, int index, int* w, int* h, uint8_t* rgba) {
+    if (!s || !w || !h) return fail(RT_ERR_INVALID_ARG, "null argument");
+    const rt::HostScene& hs = s->host;
+    if (index < 0 || index >= (int)hs.textures.size()) return fail(RT_ERR_INVALID_ARG, "texture index out of range");
+    const rt::LdrImage& t = hs.textures[index];
+    *w = t.w;
+    *h = t.h;
+    if (rgba && !t.rgba.empty()) std::memcpy(rgba, t.rgba.data(), t.rgba.size());
+    return RT_OK;
+}
+
+int rt_texture_lookup(rt_scene* s, int index, const float* uv, int n, float* rgb_out) {
+    if (!s || !uv || !rgb_out || n < 0) return fail(RT_ERR_INVALID_ARG, "invalid argument");
+    if (n == 0) return RT_OK;
+    return guard([&]() {
+        ensure_gpu(s);
+        rt::texture_lookup(*s->gpu, s->host, index, uv, n, rgb_out);
         return RT_OK;
     });
 }

@@ -48,6 +48,10 @@ struct DevScene {
     const float2* env_alias_row;
     const float2* env_alias_marg;
     int env_alias;                // per job: 1 = sample the environment with the alias tables
+    // Diffuse textures: RGBA8 texels of all textures back to back (row 0 =
+    // top), and per texture (first texel, width, height, 0).
+    const uint32_t* tex_texels;
+    const int32_t* tex_info;
 };
 
 // tri_attr layout, 8 float4 = one 128-byte line per triangle:

@@ -9,6 +9,7 @@
 #include "envmap.h"
 #include "rt_math.h"
 #include "shade.h"
+#include "texture.h"
 #include "traverse.h"
 
 namespace rt {
@@ -183,6 +184,10 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
 
     SurfaceHit sh = surface_at(sc.tri_attr, prim, hit.y, hit.z);
     DevMaterial mat = load_material(sc.materials, sh.material);
+    // A diffuse texture replaces the base colour of every lobe that reads it
+    // (all but the dielectric's, material.cu:190, and lights).
+    if (CLS != kClsTerminal && CLS != kClsDielectric && mat.texture >= 0)
+        mat.basecolor = texture_albedo(sc, mat.texture, sh.u, sh.v);
     vec3 hit_p = sh.p;
 
     if (CLS == kClsTerminal) {

@@ -443,6 +443,16 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch(DevScene sc, const float
     }
 }
 
+// Diffuse-texture lookups at given (u, v): the albedo the shading stage uses.
+__global__ __launch_bounds__(kBlock) void k_texture_lookup(DevScene sc, int tex, const float2* __restrict__ uv, int n,
+                                                            float4* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float2 q = uv[i];
+    vec3 c = texture_albedo(sc, tex, q.x, q.y);
+    out[i] = make_float4(c.x, c.y, c.z, 0.f);
+}
+
 // Device math self-test: op 0 div, 1 sqrt, 2 sin, 3 cos, 4 acos, 5 exp, 6 log,
 // 7 atan2, 8 philox (a = counter lane bits), 9 inverse, 10 pow.
 __global__ void k_math(int op, const float* a, const float* b, float* out, int n) {
@@ -589,7 +599,7 @@ __global__ __launch_bounds__(kBlock) void k_combine(const float4* __restrict__ d
 
 struct GpuScene::Impl {
     DevBuf nodes, tri_isect, tri_attr, materials, lights, light_tris, env_rgb, env_row, env_marg, env_alias_row,
-        env_alias_marg;
+        env_alias_marg, tex_texels, tex_info;
     DevScene dev{};
     int max_stack = 1;
     bool has_class[kNumClasses] = {true, false, false, false, false};
@@ -703,6 +713,29 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
         d.env_alias_row = I.env_alias_row.as<float2>();
         d.env_alias_marg = I.env_alias_marg.as<float2>();
     }
+    // Diffuse textures, back to back; an empty slot keeps a (0, 0, 0) entry.
+    d.tex_texels = nullptr;
+    d.tex_info = nullptr;
+    if (!sc.textures.empty()) {
+        std::vector<int32_t> info(4 * sc.textures.size(), 0);
+        size_t texels = 0;
+        for (size_t i = 0; i < sc.textures.size(); ++i) {
+            const LdrImage& t = sc.textures[i];
+            info[4 * i] = (int32_t)texels;
+            info[4 * i + 1] = t.w;
+            info[4 * i + 2] = t.h;
+            texels += (size_t)t.w * t.h;
+        }
+        if (texels > (size_t)INT32_MAX) throw std::runtime_error("textures exceed 2^31 texels");
+        std::vector<uint8_t> all;
+        all.reserve(4 * std::max<size_t>(texels, 1));
+        for (const LdrImage& t : sc.textures) all.insert(all.end(), t.rgba.begin(), t.rgba.end());
+        if (all.empty()) all.resize(4);
+        upload(I.tex_texels, all.data(), all.size());
+        upload(I.tex_info, info.data(), info.size() * sizeof(int32_t));
+        d.tex_texels = I.tex_texels.as<uint32_t>();
+        d.tex_info = I.tex_info.as<int32_t>();
+    }
 }
 
 GpuScene::~GpuScene() { delete impl_; }
@@ -714,7 +747,7 @@ size_t GpuScene::device_bytes() const {
     const Impl& I = *impl_;
     return I.nodes.bytes + I.tri_isect.bytes + I.tri_attr.bytes + I.materials.bytes + I.lights.bytes +
            I.light_tris.bytes + I.env_rgb.bytes + I.env_row.bytes + I.env_marg.bytes + I.env_alias_row.bytes +
-           I.env_alias_marg.bytes;
+           I.env_alias_marg.bytes + I.tex_texels.bytes + I.tex_info.bytes;
 }
 
 // Row-conditional and marginal CDFs weighted by sin(theta) (reference:
@@ -1541,6 +1574,25 @@ void trace_batch(const GpuScene& scene, const float* rays, int n, float* out_hit
     HIP_OK(hipDeviceSynchronize());
     HIP_OK(hipMemcpy(out_hits, o.p, (size_t)n * 16, hipMemcpyDeviceToHost));
     if (out_counts) HIP_OK(hipMemcpy(out_counts, c.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+}
+
+void texture_lookup(const GpuScene& scene, const HostScene& host, int tex, const float* uv, int n, float* rgb_out) {
+    if (tex < 0 || tex >= (int)host.textures.size() || host.textures[tex].w <= 0)
+        throw std::invalid_argument("no decoded texture at that index");
+    DevBuf u, o;
+    u.alloc((size_t)n * 8);
+    o.alloc((size_t)n * 16);
+    HIP_OK(hipMemcpy(u.p, uv, (size_t)n * 8, hipMemcpyHostToDevice));
+    k_texture_lookup<<<(n + kBlock - 1) / kBlock, kBlock>>>(scene.dev(), tex, u.as<float2>(), n, o.as<float4>());
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipDeviceSynchronize());
+    std::vector<float4> res((size_t)n);
+    HIP_OK(hipMemcpy(res.data(), o.p, (size_t)n * 16, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+        rgb_out[3 * i] = res[i].x;
+        rgb_out[3 * i + 1] = res[i].y;
+        rgb_out[3 * i + 2] = res[i].z;
+    }
 }
 
 void math_selftest(int op, const float* a, const float* b, float* out, int n) {

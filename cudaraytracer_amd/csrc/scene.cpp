@@ -482,16 +482,22 @@ void load_scene(const std::string& json_path, HostScene& sc) {
     if (cam.contains("seed")) sc.seed = (uint32_t)cam["seed"].as_number("seed");
     if (cam.contains("rr_start_depth")) sc.rr_start_depth = cam["rr_start_depth"].as_int("rr_start_depth");
 
-    // ---- textures (reference: Src/Core/scene.cpp:618-644); LDR textures are not sampled yet.
+    // ---- textures (reference: Src/Core/scene.cpp:618-644, Texture::Load texture.cu:207-240):
+    // every entry is decoded to RGBA8 at load; an entry without 'path' or
+    // 'name' keeps its slot, empty.  A file that cannot be read or decoded is
+    // an error (the reference prints a message and continues with no image).
     const Json& texj = root["textures"];
     if (!texj.is_array()) throw std::runtime_error("'textures' should be an array");
     std::map<std::string, int> tex_map;
+    sc.textures.assign(texj.size(), LdrImage());
     for (size_t i = 0; i < texj.size(); ++i) {
         if (!texj[i].contains("path") || !texj[i].contains("name")) {
             sc.warnings.push_back("texture missing 'path' or 'name'");
             continue;
         }
-        tex_map[texj[i]["name"].as_string("texture.name")] = (int)i;
+        std::string tname = texj[i]["name"].as_string("texture.name");
+        load_ldr_image(resolve_path(base, texj[i]["path"].as_string("texture.path")), sc.textures[i]);
+        tex_map[tname] = (int)i;
     }
 
     // ---- materials (reference: Src/Core/scene.cpp:254-396)

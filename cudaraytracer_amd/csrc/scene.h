@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "bvh.h"
+#include "image.h"
 #include "rt_math.h"
 
 namespace rt {
@@ -79,6 +80,7 @@ struct HostScene {
     std::vector<LightRec> lights;
     int env_light = -1;
     EnvMap env;
+    std::vector<LdrImage> textures;  // "textures" entries in order, RGBA8 (row 0 = top)
     CameraRec camera;
     int max_bounces = 5;
     float rr = 0.8f;

@@ -10,6 +10,9 @@ same schema the reference parses (Src/Core/scene.cpp:129-644):
   5,120 triangles each, radius U[0.15, 0.25]) with GGX metallic-roughness
   (roughness U[0.05, 0.8], metallic in {0, 1}) and one IOR-1.5 dielectric,
   512x512, 64 spp, depth 12.
+* ``textured_cornell`` -- the shell with PNG diffuse textures on diffuse, GGX,
+  subsurface and dielectric materials (coordinates past [0, 1], 1x1 and 5x3
+  textures, wrapped sphere longitudes), 96x96, 4 spp, depth 6.
 
 Floats are written with 9 significant digits so the OBJ text round-trips
 every float32 exactly.
@@ -38,6 +41,7 @@ class MeshSpec:
     verts: Optional[np.ndarray] = None  # indexed form: (m, 3) float32 shared vertices
     vnorm: Optional[np.ndarray] = None  # (m, 3) float32 shared normals
     faces: Optional[np.ndarray] = None  # (n, 3) int vertex indices
+    uv: Optional[np.ndarray] = None     # (n, 3, 2) float32 texture coordinates (soup form)
 
 
 @dataclass
@@ -47,6 +51,7 @@ class SceneSpec:
     meshes: List[MeshSpec]
     lights: List[dict]
     textures: List[dict] = field(default_factory=list)
+    texture_images: dict = field(default_factory=dict)  # texture file name -> (h, w, 4) uint8, row 0 = top
 
 
 def _quad(p0, p1, p2, p3, normal) -> np.ndarray:
@@ -214,6 +219,11 @@ def write_obj(path: str, mesh: MeshSpec) -> None:
         for k in range(3):
             p = mesh.pos[t, k]
             lines.append(f"v {_fmt(p[0])} {_fmt(p[1])} {_fmt(p[2])}")
+    if mesh.uv is not None:
+        for t in range(n):
+            for k in range(3):
+                q = mesh.uv[t, k]
+                lines.append(f"vt {_fmt(q[0])} {_fmt(q[1])}")
     if mesh.nrm is not None:
         for t in range(n):
             for k in range(3):
@@ -221,7 +231,14 @@ def write_obj(path: str, mesh: MeshSpec) -> None:
                 lines.append(f"vn {_fmt(q[0])} {_fmt(q[1])} {_fmt(q[2])}")
         for t in range(n):
             a, b, c = 3 * t + 1, 3 * t + 2, 3 * t + 3
-            lines.append(f"f {a}//{a} {b}//{b} {c}//{c}")
+            if mesh.uv is not None:
+                lines.append(f"f {a}/{a}/{a} {b}/{b}/{b} {c}/{c}/{c}")
+            else:
+                lines.append(f"f {a}//{a} {b}//{b} {c}//{c}")
+    elif mesh.uv is not None:
+        for t in range(n):
+            a, b, c = 3 * t + 1, 3 * t + 2, 3 * t + 3
+            lines.append(f"f {a}/{a} {b}/{b} {c}/{c}")
     else:
         for t in range(n):
             a, b, c = 3 * t + 1, 3 * t + 2, 3 * t + 3
@@ -242,6 +259,96 @@ def _write_indexed_obj(path: str, mesh: MeshSpec) -> None:
             f.write("\n".join(["f %d//%d %d//%d %d//%d" % (a, a, b, b, c, c) for a, b, c in fi]) + "\n")
         else:
             f.write("\n".join(["f %d %d %d" % (a, b, c) for a, b, c in fi]) + "\n")
+
+
+def write_png_rgba(path: str, rgba: np.ndarray) -> None:
+    """8-bit RGBA PNG, row 0 = top of the image (no row filters, one data chunk)."""
+    import struct
+    import zlib
+
+    h, w, _ = rgba.shape
+    raw = b"".join(b"\x00" + np.ascontiguousarray(rgba[y], np.uint8).tobytes() for y in range(h))
+
+    def chunk(tag: bytes, body: bytes) -> bytes:
+        return struct.pack(">I", len(body)) + tag + body + struct.pack(">I", zlib.crc32(tag + body) & 0xFFFFFFFF)
+
+    with open(path, "wb") as f:
+        f.write(b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0))
+                + chunk(b"IDAT", zlib.compress(raw, 9)) + chunk(b"IEND", b""))
+
+
+def _planar_uv(pos: np.ndarray, lo: float, hi: float) -> np.ndarray:
+    """Texture coordinates of a planar triangle soup (n, 3, 3): the two axes
+    across the dominant normal axis, each mapped from the mesh's extent to [lo, hi]."""
+    p = pos.reshape(-1, 3).astype(np.float64)
+    n = np.abs(np.cross(pos[:, 1] - pos[:, 0], pos[:, 2] - pos[:, 0]).sum(axis=0))
+    axes = [a for a in range(3) if a != int(np.argmax(n))]
+    uv = np.empty((p.shape[0], 2))
+    for k, a in enumerate(axes):
+        span = max(p[:, a].max() - p[:, a].min(), 1e-9)
+        uv[:, k] = lo + (hi - lo) * (p[:, a] - p[:, a].min()) / span
+    return uv.reshape(pos.shape[0], 3, 2).astype(np.float32)
+
+
+def _sphere_uv(dirs: np.ndarray) -> np.ndarray:
+    """Longitude / latitude coordinates of unit directions (n, 3, 3)."""
+    u = 0.5 + np.arctan2(dirs[..., 2], dirs[..., 0]) / (2 * np.pi)
+    v = 0.5 + np.arcsin(np.clip(dirs[..., 1], -1, 1)) / np.pi
+    return np.stack([u, v], axis=-1).astype(np.float32)
+
+
+def _texture_image(h: int, w: int, rng) -> np.ndarray:
+    """Checker pattern with noise: every texel differs from its neighbours."""
+    y, x = np.mgrid[0:h, 0:w]
+    check = ((x // 3 + y // 2) % 2)[..., None] * np.array([150, 60, 20])
+    noise = rng.integers(0, 100, (h, w, 3))
+    rgb = np.clip(40 + check + noise, 0, 255)
+    return np.concatenate([rgb, rng.integers(0, 256, (h, w, 1))], axis=2).astype(np.uint8)
+
+
+def textured_cornell(seed: int = 5, width: int = 96, height: int = 96, spp: int = 4, depth: int = 6) -> SceneSpec:
+    """The Cornell shell with diffuse textures on every lobe that reads an albedo:
+    a diffuse back wall whose coordinates run past [0, 1] (clamped), a GGX floor,
+    a metallic GGX tall box, a 1x1-textured ceiling, a subsurface sphere and a
+    textured dielectric sphere (whose texture the reference ignores).  Sphere
+    longitudes wrap across u = 0."""
+    rng = np.random.default_rng(seed)
+    meshes = cornell_shell()
+    byname = {m.name: m for m in meshes}
+    images = {"wall.png": _texture_image(23, 37, rng), "floor.png": _texture_image(64, 64, rng),
+              "tiny.png": _texture_image(3, 5, rng), "one.png": _texture_image(1, 1, rng),
+              "sphere.png": _texture_image(16, 32, rng)}
+    textures = [{"name": n.split(".")[0], "path": n} for n in images]
+    mats = _base_materials() + [
+        {"name": "wall_tex", "type": "diffuse", "basecolor": [0.5, 0.5, 0.5], "diffuse_texture": "wall"},
+        {"name": "floor_tex", "type": "pbr", "basecolor": [0.5, 0.5, 0.5], "metallic": 0.0, "roughness": 0.45,
+         "diffuse_texture": "floor"},
+        {"name": "box_tex", "type": "pbr", "basecolor": [0.5, 0.5, 0.5], "metallic": 1.0, "roughness": 0.3,
+         "diffuse_texture": "tiny"},
+        {"name": "short_tex", "type": "diffuse", "basecolor": [0.5, 0.5, 0.5], "diffuse_texture": "tiny"},
+        {"name": "ceiling_tex", "type": "diffuse", "basecolor": [0.5, 0.5, 0.5], "diffuse_texture": "one"},
+        {"name": "sss_tex", "type": "subsurface", "basecolor": [0.8, 0.8, 0.8], "scatter_distance": [0.3, 0.15, 0.08],
+         "ior": 1.4, "roughness": 0.3, "diffuse_texture": "sphere"},
+        {"name": "glass_tex", "type": "dielectric", "basecolor": [0.9, 0.95, 1.0], "ior": 1.5,
+         "diffuse_texture": "sphere"},
+    ]
+    for name, mat, lo, hi in (("back", "wall_tex", -0.3, 1.25), ("floor", "floor_tex", 0.0, 1.0),
+                              ("ceiling", "ceiling_tex", 0.0, 1.0)):
+        byname[name].material = mat
+        byname[name].uv = _planar_uv(byname[name].pos, lo, hi)
+    for name, mat in (("tall_box", "box_tex"), ("short_box", "short_tex")):
+        m = byname[name]
+        m.material = mat
+        m.uv = np.concatenate([_planar_uv(m.pos[2 * f:2 * f + 2], 0.0, 1.0) for f in range(m.pos.shape[0] // 2)])
+    V, F = icosphere(3)
+    for name, mat, center, r in (("sss_sphere", "sss_tex", (-0.45, 0.95, -0.3), 0.22),
+                                 ("glass_sphere", "glass_tex", (0.45, 1.45, 0.35), 0.2)):
+        dirs = V[F]
+        pos = (dirs * r + np.array(center)).astype(np.float32)
+        meshes.append(MeshSpec(name, mat, pos, dirs.astype(np.float32), uv=_sphere_uv(dirs)))
+    return SceneSpec(camera=_camera(width, height, spp, depth), materials=mats, meshes=meshes,
+                     lights=[{"name": "light0", "type": "area", "lightMeshName": "light", "material": "light"}],
+                     textures=textures, texture_images=images)
 
 
 def write_pfm(path: str, rgb: np.ndarray) -> None:
@@ -426,6 +533,8 @@ def write_scene(spec: SceneSpec, out_dir: str, name: str = "scene") -> str:
         write_obj(os.path.join(out_dir, fn), m)
         objects.append({"name": m.name, "path": fn, "translate": [0, 0, 0], "scale": [1, 1, 1],
                         "rotate": [{"rotateX": 0}], "material": m.material})
+    for fn, img in spec.texture_images.items():
+        write_png_rgba(os.path.join(out_dir, fn), img)
     env = getattr(spec, "env", None)
     if env is not None:
         for L in spec.lights:
@@ -442,6 +551,7 @@ def write_scene(spec: SceneSpec, out_dir: str, name: str = "scene") -> str:
 WORKLOADS = {
     "cornell_small": cornell_small,
     "cornell_spheres": cornell_spheres,
+    "textured_cornell": textured_cornell,
 }
 # Large scenes are generated on demand (the full terrain is 2M triangles).
 LARGE_WORKLOADS = {

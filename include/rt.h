@@ -108,6 +108,7 @@ typedef struct rt_scene_info {
     int rr_start_depth;
     float bounds_lo[3], bounds_hi[3];
     float camera_position[3], camera_left_down[3], camera_du[3], camera_dv[3];
+    int num_textures;       /* entries of the scene's "textures" array */
 } rt_scene_info;
 
 typedef struct rt_render_params {
@@ -289,6 +290,13 @@ int rt_scene_get_meshes(const rt_scene* s, int32_t* mesh4, float* area);
 /* Reference-layout BVH: 8 words per node as in LinearBVHNode (Src/Core/bvh.h:25-33) and the
    ordered-triangle permutation.  from_packed = 1 rebuilds the array from the device layout. */
 int rt_scene_get_bvh(const rt_scene* s, int from_packed, void* nodes_out, int32_t* order_out);
+/* Decoded diffuse texture `index` (position in the scene's "textures" array) as
+   Texture::Load uploads it (Src/Core/texture.cu:207-240): RGBA8, row 0 = top.  *w, *h
+   receive its size (0 x 0 for an entry without 'path' or 'name'); rgba may be NULL. */
+int rt_scene_get_texture(const rt_scene* s, int index, int* w, int* h, uint8_t* rgba);
+/* Diffuse-texture albedo on the GPU at n (u, v) pairs (2 floats each), as shading
+   reads it (Texture::Sample, Src/Core/texture.cu:242-249): 3 floats per pair. */
+int rt_texture_lookup(rt_scene* s, int index, const float* uv, int n, float* rgb_out);
 /* Closest hit on the GPU for n rays given as 8 floats (o.xyz, tmin, d.xyz, tmax);
    hits_out: (t, b1, b2, prim-as-int-bits) per ray; counts_out: (node visits, tri tests). */
 int rt_trace_rays(rt_scene* s, const float* rays, int n, float* hits_out, uint32_t* counts_out);

@@ -317,6 +317,9 @@ struct ref_scene {
     V3 cam_pos, cam_ld, cam_du, cam_dv;
     V3 cam_lookat, cam_up;
     ref_orbit orbit;  /* target, distance, theta, phi (degrees) of the current camera */
+    int ntex;
+    int32_t* tex_info;  /* (first texel, width, height) per texture */
+    uint8_t* tex_rgba;
 };
 
 static float smin(float a, float b) { return (b < a) ? b : a; } /* std::min */
@@ -802,6 +805,18 @@ ref_scene* ref_scene_create(const ref_scene_desc* d) {
     s->nmat = d->num_materials;
     s->mats = (float*)malloc(sizeof(float) * REF_MAT_STRIDE * (size_t)(s->nmat + 1));
     memcpy(s->mats, d->materials, sizeof(float) * REF_MAT_STRIDE * (size_t)s->nmat);
+    s->ntex = d->num_textures > 0 && d->tex_info && d->tex_rgba ? d->num_textures : 0;
+    if (s->ntex) {
+        size_t texels = 0;
+        s->tex_info = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)s->ntex);
+        memcpy(s->tex_info, d->tex_info, sizeof(int32_t) * 3 * (size_t)s->ntex);
+        for (i = 0; i < s->ntex; ++i) {
+            size_t end = (size_t)s->tex_info[3 * i] + (size_t)s->tex_info[3 * i + 1] * (size_t)s->tex_info[3 * i + 2];
+            if (end > texels) texels = end;
+        }
+        s->tex_rgba = (uint8_t*)malloc(4 * texels + 4);
+        memcpy(s->tex_rgba, d->tex_rgba, 4 * texels);
+    }
     s->nl = d->num_lights;
     s->light_type = (int32_t*)malloc(sizeof(int32_t) * (size_t)(s->nl + 1));
     s->light_mesh = (int32_t*)malloc(sizeof(int32_t) * (size_t)(s->nl + 1));
@@ -850,6 +865,7 @@ void ref_scene_destroy(ref_scene* s) {
     free(s->mesh_material); free(s->mesh_light); free(s->mesh_area); free(s->mesh_start); free(s->mesh_count);
     free(s->mats); free(s->light_type); free(s->light_mesh); free(s->light_em);
     free(s->env); free(s->env_row); free(s->env_marg); free(s->env_alias_row); free(s->env_alias_marg);
+    free(s->tex_info); free(s->tex_rgba);
     free(s);
 }
 
@@ -871,6 +887,50 @@ void ref_env_tables(const ref_scene* s, float* row_cdf, float* marg_cdf) {
 /* ------------------------------------------------------------ path tracing */
 /* FP32 instantiation: the reference's float / double promotion rules. */
 #define SFX _f
+/* Texture::Sample (texture.cu:242-249) with the texture object of
+   Texture::Load (texture.cu:226-237): u, v clamped to [0, 0.999999] (Clamp,
+   mathutil.cu:72-79), v flipped, then the linear filter of a normalised-
+   coordinate texture with U wrapping and V clamping: sample point
+   (u W - 1/2, v' H - 1/2) between the four nearest texel centres, texels read
+   as c / 255.  Weights in FP32 (the hardware rounds them to 8 fractional bits). */
+static V3 tex_texel(const ref_scene* s, int tex, int x, int y) {
+    const uint8_t* t = s->tex_rgba + 4 * ((size_t)s->tex_info[3 * tex] + (size_t)y * s->tex_info[3 * tex + 1] + x);
+    return v3((float)t[0] / 255.f, (float)t[1] / 255.f, (float)t[2] / 255.f);
+}
+
+static V3 tex_albedo(const ref_scene* s, int tex, float u, float v) {
+    int w = s->tex_info[3 * tex + 1], h = s->tex_info[3 * tex + 2];
+    float x, y, ax, ay;
+    int x0, y0, x1, y1;
+    V3 a, b, c, d, top, bot;
+    u = u >= 0.0f ? (u > 0.999999f ? 0.999999f : u) : 0.0f; /* NaN -> 0 */
+    v = v >= 0.0f ? (v > 0.999999f ? 0.999999f : v) : 0.0f;
+    x = u * (float)w - 0.5f;
+    y = (1.0f - v) * (float)h - 0.5f;
+    x0 = (int)floorf(x);
+    y0 = (int)floorf(y);
+    ax = x - (float)x0;
+    ay = y - (float)y0;
+    x1 = (x0 + 1) % w;
+    x0 = ((x0 % w) + w) % w;
+    y1 = y0 + 1 > h - 1 ? h - 1 : y0 + 1;
+    y0 = y0 < 0 ? 0 : (y0 > h - 1 ? h - 1 : y0);
+    a = tex_texel(s, tex, x0, y0);
+    b = tex_texel(s, tex, x1, y0);
+    c = tex_texel(s, tex, x0, y1);
+    d = tex_texel(s, tex, x1, y1);
+    top = v3(a.x * (1.f - ax) + b.x * ax, a.y * (1.f - ax) + b.y * ax, a.z * (1.f - ax) + b.z * ax);
+    bot = v3(c.x * (1.f - ax) + d.x * ax, c.y * (1.f - ax) + d.y * ax, c.z * (1.f - ax) + d.z * ax);
+    return v3(top.x * (1.f - ay) + bot.x * ay, top.y * (1.f - ay) + bot.y * ay, top.z * (1.f - ay) + bot.z * ay);
+}
+
+void ref_texture_albedo(const ref_scene* s, int tex, float u, float v, float* rgb) {
+    V3 c = tex_albedo(s, tex, u, v);
+    rgb[0] = c.x;
+    rgb[1] = c.y;
+    rgb[2] = c.z;
+}
+
 #define REAL float
 #define FP32_MODE 1
 #include "rtref_path.inc"

@@ -24,7 +24,8 @@ extern "C" {
 #define REF_MAT_STRIDE 16
 /* [0] type (0 light, 1 diffuse, 2 pbr, 3 dielectric, 4 subsurface)
    [1..3] basecolor  [4..6] emission  [7] roughness  [8] metallic
-   [9] transmission  [10] ior  [11..13] scatter distance  [14..15] unused */
+   [9] transmission  [10] ior  [11..13] scatter distance
+   [14] diffuse texture index + 1 (0 = none)  [15] unused */
 
 typedef struct ref_scene_desc {
     int num_tris;
@@ -46,6 +47,9 @@ typedef struct ref_scene_desc {
     int width, height;
     float fovy;
     float direction[3];
+    int num_textures;
+    const int32_t* tex_info;  /* (first texel, width, height) per texture */
+    const uint8_t* tex_rgba;  /* RGBA8 texels of all textures, row 0 = top */
 } ref_scene_desc;
 
 typedef struct ref_params {
@@ -146,6 +150,8 @@ void ref_env_sample_alias(const ref_scene* s, float rv, float ru, float* uv_out)
 void ref_alias_build(const double* w, int n, float* prob, int32_t* alias);
 void ref_env_sample(const ref_scene* s, float rv, float ru, float* uv_out);
 float ref_env_pdf_uv(const ref_scene* s, float u, float v);
+/* Texture::Sample (texture.cu:242-249): albedo of texture `tex` at (u, v). */
+void ref_texture_albedo(const ref_scene* s, int tex, float u, float v, float* rgb);
 
 #ifdef __cplusplus
 }

@@ -12,10 +12,12 @@ import ctypes
 import json
 import os
 import subprocess
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from typing import Optional
 
 import numpy as np
+
+from .rtref_image import read_texture
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "librtref.so")
@@ -36,7 +38,8 @@ class Desc(ctypes.Structure):
         ("num_lights", ctypes.c_int), ("light_type", ctypes.c_void_p), ("light_mesh", ctypes.c_void_p),
         ("light_emission", ctypes.c_void_p), ("env_w", ctypes.c_int), ("env_h", ctypes.c_int),
         ("env_rgb", ctypes.c_void_p), ("width", ctypes.c_int), ("height", ctypes.c_int), ("fovy", ctypes.c_float),
-        ("direction", ctypes.c_float * 3),
+        ("direction", ctypes.c_float * 3), ("num_textures", ctypes.c_int), ("tex_info", ctypes.c_void_p),
+        ("tex_rgba", ctypes.c_void_p),
     ]
 
 
@@ -112,6 +115,8 @@ def lib():
         L.ref_env_sample.argtypes = [vp, ctypes.c_float, ctypes.c_float, vp]
         L.ref_env_pdf_uv.argtypes = [vp, ctypes.c_float, ctypes.c_float]
         L.ref_env_pdf_uv.restype = ctypes.c_float
+        L.ref_texture_albedo.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, vp]
+        L.ref_texture_albedo.restype = None
         L.ref_get_orbit.argtypes = [vp, ctypes.POINTER(Orbit)]
         L.ref_set_orbit.argtypes = [vp, ctypes.POINTER(Orbit)]
         L.ref_view_projection.argtypes = [vp, ctypes.POINTER(Orbit), ctypes.c_float, ctypes.c_float, vp]
@@ -294,6 +299,7 @@ class SceneArrays:
     seed: int
     rr_start_depth: int
     num_lights: int = 0
+    textures: list = field(default_factory=list)  # (h, w, 4) uint8 per "textures" entry
 
 
 def parse_scene(json_path: str) -> SceneArrays:
@@ -308,6 +314,16 @@ def parse_scene(json_path: str) -> SceneArrays:
     materials[:, 7] = 0.5
     materials[:, 9] = 1.0
     materials[:, 10] = 1.0
+    # Textures (scene.cpp:618-644): every entry decoded in order; an entry
+    # without 'path' or 'name' keeps an empty slot.
+    textures, tex_index = [], {}
+    for t in doc["textures"]:
+        if "path" not in t or "name" not in t:
+            textures.append(np.zeros((0, 0, 4), np.uint8))
+            continue
+        tp = t["path"] if os.path.isabs(t["path"]) else os.path.join(base, t["path"])
+        tex_index[t["name"]] = len(textures)
+        textures.append(read_texture(tp))
     name_to_mat = {}
     for i, m in enumerate(mats_json):
         if "name" not in m or "type" not in m or m["type"] not in MAT_TYPES:
@@ -337,6 +353,8 @@ def parse_scene(json_path: str) -> SceneArrays:
             r[9] = m.get("transmission", 1.0)
             r[7] = m.get("roughness", 0.0)
             r[8] = 0
+        if m.get("diffuse_texture") in tex_index:
+            r[14] = tex_index[m["diffuse_texture"]] + 1
         name_to_mat[m["name"]] = i
     pos, nrm, uv, mesh, mmat, mnames = [], [], [], [], [], []
     for o in doc["objects"]:
@@ -414,7 +432,7 @@ def parse_scene(json_path: str) -> SceneArrays:
         env_w=env[0], env_h=env[1], env_rgb=env[2], width=int(W), height=int(H), fovy=float(cam.get("fov", 45.0)),
         direction=tuple(cam.get("direction", [0, 0, -1])), max_bounces=int(cam.get("max_bounces", 5)),
         rr=float(cam.get("rr", 0.8)), spp=int(cam.get("spp", 16)), seed=int(cam.get("seed", 1234)),
-        rr_start_depth=int(cam.get("rr_start_depth", 0)), num_lights=len(lj),
+        rr_start_depth=int(cam.get("rr_start_depth", 0)), num_lights=len(lj), textures=textures,
     )
 
 
@@ -436,6 +454,16 @@ class RefScene:
         d.env_rgb = _p(a.env_rgb) if a.env_rgb is not None else None
         d.width, d.height, d.fovy = a.width, a.height, a.fovy
         d.direction[:] = [np.float32(x) for x in a.direction]
+        if a.textures:
+            info, first = [], 0
+            for t in a.textures:
+                info += [first, t.shape[1], t.shape[0]]
+                first += t.shape[0] * t.shape[1]
+            self._tex_info = np.array(info, np.int32)
+            self._tex_rgba = np.ascontiguousarray(
+                np.concatenate([t.reshape(-1) for t in a.textures] + [np.zeros(4, np.uint8)]))
+            d.num_textures = len(a.textures)
+            d.tex_info, d.tex_rgba = _p(self._tex_info), _p(self._tex_rgba)
         self._desc = d
         self._h = lib().ref_scene_create(ctypes.byref(d))
 
@@ -532,6 +560,11 @@ class RefScene:
         uv = np.zeros(2, np.float32)
         lib().ref_env_sample(self._h, rv, ru, _p(uv))
         return uv
+
+    def texture_albedo(self, tex: int, u: float, v: float) -> np.ndarray:
+        out = np.zeros(3, np.float32)
+        lib().ref_texture_albedo(self._h, tex, u, v, _p(out))
+        return out
 
     def env_pdf_uv(self, u: float, v: float) -> float:
         return float(lib().ref_env_pdf_uv(self._h, u, v))

@@ -12,7 +12,8 @@ import cudaraytracer_amd as rt
 
 pytestmark = pytest.mark.gpu
 
-ACCUM = [("cornell_small", 3), ("cornell_spheres_128", 4), ("terrain_small", 5), ("cornell_spheres_rt", 7)]
+ACCUM = [("cornell_small", 3), ("cornell_spheres_128", 4), ("terrain_small", 5), ("cornell_spheres_rt", 7),
+         ("textured_cornell", 3)]
 
 
 def _bits_equal(a, b, what):
@@ -53,7 +54,7 @@ def test_pool_accumulate_jobs_match_alone(scene_dir, queue_mb):
         ext += s_alone.extension_rays
     assert pool.stats().extension_rays == ext
     if queue_mb == 34:
-        assert passes >= 3  # 377 k paths in passes of ~156 k: split and shared
+        assert passes >= 3  # 405 k paths in passes of ~156 k: split and shared
 
 
 def _path(cam, n, dphi=1.5, dtheta=-0.5):
