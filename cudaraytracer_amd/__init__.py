@@ -87,6 +87,7 @@ class SceneInfo(ctypes.Structure):
         ("bounds_lo", ctypes.c_float * 3), ("bounds_hi", ctypes.c_float * 3),
         ("camera_position", ctypes.c_float * 3), ("camera_left_down", ctypes.c_float * 3),
         ("camera_du", ctypes.c_float * 3), ("camera_dv", ctypes.c_float * 3), ("num_textures", ctypes.c_int),
+        ("bvh_build_ms", ctypes.c_float), ("bvh_on_gpu", ctypes.c_int), ("bvh_gpu_phase_ms", ctypes.c_float * 4),
     ]
 
 
@@ -264,12 +265,21 @@ def _iptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
 
 
-class Scene:
-    """A loaded scene: JSON + OBJ (+ environment map), SAH BVH built on the host."""
+class LoadOpts(ctypes.Structure):
+    _fields_ = [("bvh_builder", ctypes.c_int)]
 
-    def __init__(self, json_path: str):
+
+BVH_AUTO, BVH_HOST, BVH_GPU = 0, 1, 2
+
+
+class Scene:
+    """A loaded scene: JSON + OBJ (+ environment map) and its SAH BVH, built on
+    the host or the GPU (`bvh_builder`: BVH_AUTO, BVH_HOST, BVH_GPU; same tree)."""
+
+    def __init__(self, json_path: str, bvh_builder: int = BVH_AUTO):
         self._h = ctypes.c_void_p()
-        _check(lib().rt_scene_load(os.fsencode(json_path), None, ctypes.byref(self._h)))
+        opts = LoadOpts(bvh_builder)
+        _check(lib().rt_scene_load(os.fsencode(json_path), ctypes.byref(opts), ctypes.byref(self._h)))
         self.path = json_path
 
     def close(self) -> None:

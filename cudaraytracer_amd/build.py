@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librtamd.so")
 
-SOURCES = ["json.cpp", "image.cpp", "bvh.cpp", "scene.cpp", "capi.cpp", "render.hip", "pool.cpp"]
+SOURCES = ["json.cpp", "image.cpp", "bvh.cpp", "bvh_gpu.hip", "scene.cpp", "capi.cpp", "render.hip", "pool.cpp"]
 
 # -ffp-contract=off keeps host and device FP32 results bit-identical; the
 # target-feature flag keeps the compiler from emitting packed FP32 ops.

@@ -53,12 +53,16 @@ struct BvhBuild {
     int max_stack = 0;                   // deepest traversal stack the tree can need
     int fallback_splits = 0;             // splits where the reference would not terminate
     Box bounds;
+    // GPU build phases (ms): setup + upload, levels, download, host numbering.
+    double gpu_phase_ms[4] = {0, 0, 0, 0};
 };
 
 // Binned SAH build (12 buckets, leaves of <= 4 triangles) restating
 // reference Src/Core/bvh.cu:44-175 and the flatten at bvh.cu:186-206.
 // `pos` holds 9 floats (v0, v1, v2) per triangle in scene order.
 void build_bvh(const float* pos, int num_tris, BvhBuild& out);
+// The same build on the current GPU (bvh_gpu.hip): an identical BvhBuild.
+void build_bvh_gpu(const float* pos, int num_tris, BvhBuild& out);
 
 // Rebuild the reference LinearNode array from the packed form (test hook).
 void unpack_bvh(const std::vector<PackedNode>& packed, uint32_t root_ref, const Box& root_box,

@@ -158,11 +158,12 @@ const char* rt_last_error(void) { return g_err.c_str(); }
 
 int rt_scene_load(const char* json_path, const rt_load_opts* opts, rt_scene** out) {
     if (!json_path || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
-    if (opts && opts->reserved != 0) return fail(RT_ERR_INVALID_ARG, "rt_load_opts.reserved must be 0");
+    if (opts && (opts->bvh_builder < 0 || opts->bvh_builder > 2))
+        return fail(RT_ERR_INVALID_ARG, "rt_load_opts.bvh_builder must be 0, 1 or 2");
     *out = nullptr;
     return guard([&]() {
         std::unique_ptr<rt_scene> s(new rt_scene);
-        rt::load_scene(json_path, s->host);
+        rt::load_scene(json_path, s->host, opts ? opts->bvh_builder : rt::kBvhAuto);
         *out = s.release();
         return RT_OK;
     });
@@ -188,6 +189,9 @@ int rt_scene_info_get(const rt_scene* s, rt_scene_info* o) {
     o->seed = h.seed;
     o->rr_start_depth = h.rr_start_depth;
     o->num_textures = (int)h.textures.size();
+    o->bvh_build_ms = (float)h.bvh_build_ms;
+    o->bvh_on_gpu = h.bvh_on_gpu ? 1 : 0;
+    for (int k = 0; k < 4; ++k) o->bvh_gpu_phase_ms[k] = (float)h.bvh.gpu_phase_ms[k];
     const rt::Box& b = h.bvh.bounds;
     float lo[3] = {b.lo.x, b.lo.y, b.lo.z}, hi[3] = {b.hi.x, b.hi.y, b.hi.z};
     std::memcpy(o->bounds_lo, lo, sizeof(lo));

@@ -2,6 +2,9 @@
 // Src/Utils/objloader.cpp:66-210), camera set-up and BVH build.
 #include "scene.h"
 
+#include <hip/hip_runtime.h>
+
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -456,7 +459,7 @@ void camera_view_projection(const CameraRec& c, float near_plane, float far_plan
         for (int j = 0; j < 4; ++j) vp[4 * i + j] = m.m[i][j];
 }
 
-void load_scene(const std::string& json_path, HostScene& sc) {
+void load_scene(const std::string& json_path, HostScene& sc, int bvh_builder) {
     sc = HostScene();
     sc.path = json_path;
     std::string base = dir_of(json_path);
@@ -689,7 +692,18 @@ void load_scene(const std::string& json_path, HostScene& sc) {
     }
 
     if (sc.num_tris() == 0) throw std::runtime_error("scene has no triangles");
-    build_bvh(sc.pos.data(), sc.num_tris(), sc.bvh);
+    for (float v : sc.pos)
+        if (!std::isfinite(v)) throw std::runtime_error("non-finite vertex coordinate");
+    bool gpu = bvh_builder == kBvhGpu;
+    if (bvh_builder == kBvhAuto && sc.num_tris() >= kGpuBuildMinTris) {
+        int devices = 0;
+        gpu = hipGetDeviceCount(&devices) == hipSuccess && devices > 0;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    if (gpu) build_bvh_gpu(sc.pos.data(), sc.num_tris(), sc.bvh);
+    else build_bvh(sc.pos.data(), sc.num_tris(), sc.bvh);
+    sc.bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    sc.bvh_on_gpu = gpu;
     setup_camera(sc);
 }
 

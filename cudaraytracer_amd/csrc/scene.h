@@ -89,13 +89,19 @@ struct HostScene {
     uint32_t seed = 1234u;      // reference: Src/Core/scene.cpp:167
     int rr_start_depth = 0;     // 0: roulette from the first bounce, as the reference
     BvhBuild bvh;
+    double bvh_build_ms = 0.0;  // wall time of the BVH build
+    bool bvh_on_gpu = false;
     std::vector<std::string> warnings;
 
     int num_tris() const { return (int)mesh.size(); }
 };
 
+// Where the BVH is built: automatic = on the GPU when one is present and the
+// scene has at least kGpuBuildMinTris triangles.  Both give the same tree.
+enum BvhBuilder : int { kBvhAuto = 0, kBvhHost = 1, kBvhGpu = 2 };
+constexpr int kGpuBuildMinTris = 1 << 16;
 // Parse a scene JSON (+ OBJ files) and build the BVH.  Throws std::runtime_error.
-void load_scene(const std::string& json_path, HostScene& sc);
+void load_scene(const std::string& json_path, HostScene& sc, int bvh_builder = kBvhAuto);
 // Triangle constructor math: geometric normal and area (reference: Src/Core/triangle.cu:3-10).
 void finalize_triangles(HostScene& sc);
 // Camera parameters from the scene bounds (reference: Src/Core/scene.cpp:646-667, camera.h:58-84).

@@ -90,7 +90,8 @@ typedef struct rt_scene rt_scene;
 typedef struct rt_renderer rt_renderer;
 
 typedef struct rt_load_opts {
-    int reserved; /* must be 0 */
+    int bvh_builder; /* 0 = automatic (the GPU for scenes of >= 65536 triangles when one is present),
+                        1 = host, 2 = GPU; every choice builds the same tree */
 } rt_load_opts;
 
 typedef struct rt_scene_info {
@@ -109,6 +110,9 @@ typedef struct rt_scene_info {
     float bounds_lo[3], bounds_hi[3];
     float camera_position[3], camera_left_down[3], camera_du[3], camera_dv[3];
     int num_textures;       /* entries of the scene's "textures" array */
+    float bvh_build_ms;     /* wall time of the BVH build */
+    int bvh_on_gpu;         /* 1 when the GPU built it */
+    float bvh_gpu_phase_ms[4]; /* GPU build: setup + upload, levels, download, host numbering */
 } rt_scene_info;
 
 typedef struct rt_render_params {
