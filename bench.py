@@ -16,7 +16,7 @@ written by --traffic-from-pmc).  `cpu_baseline` times the CPU oracle (OpenMP,
 all host cores) on a bounded sample of the same workload.
 
 Other modes: --only NAME runs one secondary workload (frames, c3, batched,
-serving) and prints its JSON; --calibrate-copy runs the 1 GiB copy that scales
+serving, bvh: the 2 M-triangle SAH build on the host and the GPU) and prints its JSON; --calibrate-copy runs the 1 GiB copy that scales
 the counters; --traffic-from-pmc turns four rocprofv3 --pmc databases into the
 traffic table.
 """
@@ -260,7 +260,39 @@ def c3_bench(env_alias: bool = True):
         "stage_ms": {n: round(sp.stage_ms[i], 3) for i, n in enumerate(rt.STAGES) if sp.stage_launches[i]},
         "extend_roofline": {k: ext[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
         "shadow_roofline": {k: sh[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
+        "bvh_build": bvh_build_bench(path, gpu_scene=scene),
     }
+
+
+def bvh_build_bench(path, gpu_scene=None):
+    """SAH BVH build of a loaded scene on the host and on the GPU: wall time of
+    each (the GPU one warm, after a first build in the process), its phases,
+    and whether the two trees are identical."""
+    import cudaraytracer_amd as rt
+
+    if gpu_scene is None:
+        gpu_scene = rt.Scene(path, bvh_builder=rt.BVH_GPU)
+    warm = rt.Scene(path, bvh_builder=rt.BVH_GPU)
+    host = rt.Scene(path, bvh_builder=rt.BVH_HOST)
+    gi, hi = warm.info(), host.info()
+    gn, go = warm.bvh()
+    hn, ho = host.bvh()
+    return {
+        "triangles": gi.num_triangles,
+        "nodes": gi.num_bvh_nodes,
+        "host_ms": round(hi.bvh_build_ms, 1),
+        "gpu_ms": round(gi.bvh_build_ms, 1),
+        "gpu_phase_ms": {k: round(v, 1) for k, v in zip(("setup_upload", "levels", "numbering_download", "host"),
+                                                        gi.bvh_gpu_phase_ms)},
+        "gpu_mtris_s": round(gi.num_triangles / max(1e-9, gi.bvh_build_ms) / 1e3, 1),
+        "identical": bool(np.array_equal(gn, hn) and np.array_equal(go, ho)),
+    }
+
+
+def bvh_only():
+    tmp = tempfile.mkdtemp(prefix="rtbvh_")
+    path = _generate([("terrain_env", {}, tmp, "terrain_bvh")])[0]
+    return bvh_build_bench(path)
 
 
 def _write_scene_job(job):
@@ -439,7 +471,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-file", default=C2_TRAFFIC,
                     help="HBM bytes per launch measured with rocprofv3 --pmc (table from --traffic-from-pmc)")
-    ap.add_argument("--only", choices=("frames", "c3", "batched", "serving"), default=None,
+    ap.add_argument("--only", choices=("frames", "c3", "batched", "serving", "bvh"), default=None,
                     help="run one secondary workload and print its JSON")
     ap.add_argument("--calibrate-copy", action="store_true", help="run the 1 GiB copy that scales the counters")
     ap.add_argument("--traffic-from-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB"),
@@ -459,7 +491,8 @@ def main() -> int:
         calibrate_copy()
         return 0
     if args.only:
-        fn = {"frames": frames_bench, "c3": c3_bench, "batched": batched_bench, "serving": serving_bench}[args.only]
+        fn = {"frames": frames_bench, "c3": c3_bench, "batched": batched_bench, "serving": serving_bench,
+              "bvh": bvh_only}[args.only]
         print(json.dumps({args.only: fn()}))
         return 0
 
