@@ -88,6 +88,7 @@ class SceneInfo(ctypes.Structure):
         ("camera_position", ctypes.c_float * 3), ("camera_left_down", ctypes.c_float * 3),
         ("camera_du", ctypes.c_float * 3), ("camera_dv", ctypes.c_float * 3), ("num_textures", ctypes.c_int),
         ("bvh_build_ms", ctypes.c_float), ("bvh_on_gpu", ctypes.c_int), ("bvh_gpu_phase_ms", ctypes.c_float * 4),
+        ("load_phase_ms", ctypes.c_float * 3),
     ]
 
 

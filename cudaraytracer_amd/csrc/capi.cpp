@@ -192,6 +192,7 @@ int rt_scene_info_get(const rt_scene* s, rt_scene_info* o) {
     o->bvh_build_ms = (float)h.bvh_build_ms;
     o->bvh_on_gpu = h.bvh_on_gpu ? 1 : 0;
     for (int k = 0; k < 4; ++k) o->bvh_gpu_phase_ms[k] = (float)h.bvh.gpu_phase_ms[k];
+    for (int k = 0; k < 3; ++k) o->load_phase_ms[k] = (float)h.load_phase_ms[k];
     const rt::Box& b = h.bvh.bounds;
     float lo[3] = {b.lo.x, b.lo.y, b.lo.z}, hi[3] = {b.hi.x, b.hi.y, b.hi.z};
     std::memcpy(o->bounds_lo, lo, sizeof(lo));

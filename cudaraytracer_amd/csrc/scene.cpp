@@ -4,8 +4,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <exception>
+#include <mutex>
+#include <thread>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -93,11 +98,23 @@ vec3 xform_normal(const Mat4& inv, vec3 n) {
 }
 
 std::string read_file(const std::string& path) {
-    std::ifstream f(path, std::ios::binary);
-    if (!f.is_open()) throw std::runtime_error("cannot open file: " + path);
-    std::stringstream ss;
-    ss << f.rdbuf();
-    return ss.str();
+    std::FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) throw std::runtime_error("cannot open file: " + path);
+    std::string data;
+    if (std::fseek(f, 0, SEEK_END) == 0) {
+        long size = std::ftell(f);
+        if (size > 0 && std::fseek(f, 0, SEEK_SET) == 0) {
+            data.resize((size_t)size);
+            data.resize(std::fread(&data[0], 1, (size_t)size, f));
+        }
+    }
+    // Files whose size is not known up front (pipes) are read to the end.
+    char buf[1 << 16];
+    for (size_t got; (got = std::fread(buf, 1, sizeof buf, f)) > 0;) data.append(buf, got);
+    bool err = std::ferror(f) != 0;
+    std::fclose(f);
+    if (err) throw std::runtime_error("cannot read file: " + path);
+    return data;
 }
 std::string dir_of(const std::string& p) {
     size_t k = p.find_last_of('/');
@@ -135,137 +152,288 @@ int fix_index(long idx, size_t n) {
 
 // Triangulated OBJ import, one mesh per (shape, material) pair, face normals
 // when a face lacks per-vertex normals (reference: Src/Utils/objloader.cpp:66-210).
-std::vector<LoadedMesh> load_obj(const std::string& path, const Mat4& xf, const Mat4& xf_inv) {
-    std::string text = read_file(path);
+//
+// The text is parsed in line-aligned chunks on host threads: each chunk
+// parses its vertex lines (strtof) and the raw corner indices of its faces
+// (fan-triangulated), noting the chunk's vertex counts at every face line for
+// relative indices.  A sequential pass resolves the indices, replays the
+// object / group / material statements in file order and assigns every
+// triangle its mesh; the triangles are then written in parallel, each into its
+// place in its mesh, with the same per-vertex arithmetic.
+namespace {
+
+struct ObjCorner {
+    int32_t v, vt, vn;  // raw OBJ indices (1-based or negative), 0 = absent
+};
+struct ObjTri {
+    ObjCorner c[3];
+    int32_t nv, nvt, nvn;  // the chunk's vertex counts before the face line
+};
+struct ObjEvent {
+    size_t tri;  // chunk triangles before the statement
+    int kind;    // 0 object / group, 1 material
+    std::string name;
+};
+struct ObjChunk {
     std::vector<float> V, VN, VT;
-    struct Face {
-        ObjIdx a, b, c;
-        int shape;
-        std::string mtl;
-    };
-    std::vector<Face> faces;
-    std::vector<std::string> shape_names;
-    int shape = 0;
-    bool shape_has_faces = false;
-    shape_names.push_back("");
-    std::string mtl;
-    std::istringstream in(text);
+    std::vector<ObjTri> tris;
+    std::vector<ObjEvent> events;
+};
+
+std::string trim_ws(const char* b, const char* e) {
+    while (b < e && (*b == ' ' || *b == '\t')) ++b;
+    while (e > b && (e[-1] == ' ' || e[-1] == '\t')) --e;
+    return std::string(b, e);
+}
+
+// Integer token prefix as strtol reads it (optional sign, decimal digits; 0 if none).
+long parse_long(const char*& t, const char* end) {
+    const char* q = t;
+    bool neg = false;
+    if (q < end && (*q == '+' || *q == '-')) neg = *q++ == '-';
+    if (q >= end || *q < '0' || *q > '9') return 0;  // no conversion: t stays
+    long v = 0;
+    while (q < end && *q >= '0' && *q <= '9') v = v * 10 + (*q++ - '0');
+    t = q;
+    return neg ? -v : v;
+}
+
+void parse_obj_chunk(const char* b, const char* e, ObjChunk& ck) {
     std::string line;
-    while (std::getline(in, line)) {
+    std::vector<ObjCorner> poly;
+    while (b < e) {
+        const char* nl = static_cast<const char*>(std::memchr(b, '\n', (size_t)(e - b)));
+        const char* le = nl ? nl : e;
+        line.assign(b, le);  // NUL-terminated copy for strtof
+        b = nl ? nl + 1 : e;
         if (!line.empty() && line.back() == '\r') line.pop_back();
         size_t s = line.find_first_not_of(" \t");
         if (s == std::string::npos || line[s] == '#') continue;
         const char* c = line.c_str() + s;
+        const char* ce = line.c_str() + line.size();
         if (c[0] == 'v' && (c[1] == ' ' || c[1] == '\t')) {
-            float x = 0, y = 0, z = 0;
-            char* e;
-            x = std::strtof(c + 2, &e);
-            y = std::strtof(e, &e);
-            z = std::strtof(e, &e);
-            V.push_back(x); V.push_back(y); V.push_back(z);
+            char* q;
+            float x = std::strtof(c + 2, &q);
+            float y = std::strtof(q, &q);
+            float z = std::strtof(q, &q);
+            ck.V.push_back(x); ck.V.push_back(y); ck.V.push_back(z);
         } else if (c[0] == 'v' && c[1] == 'n') {
-            char* e;
-            float x = std::strtof(c + 2, &e);
-            float y = std::strtof(e, &e);
-            float z = std::strtof(e, &e);
-            VN.push_back(x); VN.push_back(y); VN.push_back(z);
+            char* q;
+            float x = std::strtof(c + 2, &q);
+            float y = std::strtof(q, &q);
+            float z = std::strtof(q, &q);
+            ck.VN.push_back(x); ck.VN.push_back(y); ck.VN.push_back(z);
         } else if (c[0] == 'v' && c[1] == 't') {
-            char* e;
-            float x = std::strtof(c + 2, &e);
-            float y = std::strtof(e, &e);
-            VT.push_back(x); VT.push_back(y);
+            char* q;
+            float x = std::strtof(c + 2, &q);
+            float y = std::strtof(q, &q);
+            ck.VT.push_back(x); ck.VT.push_back(y);
         } else if (c[0] == 'f' && (c[1] == ' ' || c[1] == '\t')) {
-            std::istringstream fs(std::string(c + 2));
-            std::string tok;
-            std::vector<ObjIdx> poly;
-            while (fs >> tok) {
-                ObjIdx ix;
-                const char* t = tok.c_str();
-                char* e;
-                long a = std::strtol(t, &e, 10);
-                ix.v = fix_index(a, V.size() / 3);
-                if (*e == '/') {
-                    ++e;
-                    if (*e != '/') {
-                        long b = std::strtol(e, &e, 10);
-                        ix.vt = fix_index(b, VT.size() / 2);
-                    }
-                    if (*e == '/') {
-                        ++e;
-                        long n = std::strtol(e, &e, 10);
-                        ix.vn = fix_index(n, VN.size() / 3);
+            poly.clear();
+            const char* t = c + 2;
+            while (true) {
+                while (t < ce && std::isspace((unsigned char)*t)) ++t;
+                if (t >= ce) break;
+                const char* te = t;
+                while (te < ce && !std::isspace((unsigned char)*te)) ++te;
+                ObjCorner k{0, 0, 0};
+                const char* q = t;
+                k.v = (int32_t)parse_long(q, te);
+                if (q < te && *q == '/') {
+                    ++q;
+                    if (q < te && *q != '/') k.vt = (int32_t)parse_long(q, te);
+                    if (q < te && *q == '/') {
+                        ++q;
+                        k.vn = (int32_t)parse_long(q, te);
                     }
                 }
-                poly.push_back(ix);
+                poly.push_back(k);
+                t = te;
             }
-            // Fan triangulation of polygons.
-            for (size_t k = 1; k + 1 < poly.size(); ++k) {
-                faces.push_back(Face{poly[0], poly[k], poly[k + 1], shape, mtl});
-                shape_has_faces = true;
-            }
+            const int32_t nv = (int32_t)(ck.V.size() / 3), nvt = (int32_t)(ck.VT.size() / 2),
+                          nvn = (int32_t)(ck.VN.size() / 3);
+            for (size_t k = 1; k + 1 < poly.size(); ++k) ck.tris.push_back(ObjTri{{poly[0], poly[k], poly[k + 1]}, nv, nvt, nvn});
         } else if ((c[0] == 'o' || c[0] == 'g') && (c[1] == ' ' || c[1] == '\t' || c[1] == 0)) {
-            std::string nm = c[1] ? std::string(c + 2) : std::string();
-            size_t a = nm.find_first_not_of(" \t"), b = nm.find_last_not_of(" \t");
-            nm = a == std::string::npos ? std::string() : nm.substr(a, b - a + 1);
-            if (shape_has_faces) {
-                shape_names.push_back(nm);
-                ++shape;
-                shape_has_faces = false;
-            } else {
-                shape_names[shape] = nm;
-            }
+            ck.events.push_back(ObjEvent{ck.tris.size(), 0, c[1] ? trim_ws(c + 2, ce) : std::string()});
         } else if (std::strncmp(c, "usemtl", 6) == 0) {
-            std::string nm(c + 6);
-            size_t a = nm.find_first_not_of(" \t"), b = nm.find_last_not_of(" \t");
-            mtl = a == std::string::npos ? std::string() : nm.substr(a, b - a + 1);
+            ck.events.push_back(ObjEvent{ck.tris.size(), 1, trim_ws(c + 6, ce)});
         }
+    }
+}
+
+int host_threads() {
+    unsigned h = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(16u, h ? h : 1u));
+}
+
+// Run f(i) for i in [0, n) on up to host_threads() threads.
+template <typename F>
+void parallel_for(size_t n, F&& f) {
+    const int nt = (int)std::min<size_t>((size_t)host_threads(), std::max<size_t>(1, n));
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::exception_ptr err;
+    std::mutex err_mu;
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&]() {
+            try {
+                for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(err_mu);
+                if (!err) err = std::current_exception();
+            }
+        });
+    for (auto& th : pool) th.join();
+    if (err) std::rethrow_exception(err);
+}
+
+}  // namespace
+
+std::vector<LoadedMesh> load_obj(const std::string& path, const Mat4& xf, const Mat4& xf_inv) {
+    const std::string text = read_file(path);
+    // Line-aligned chunks of about 1 MiB.
+    std::vector<size_t> cuts{0};
+    const size_t target = (size_t)1 << 20;
+    while (cuts.back() < text.size()) {
+        size_t c = std::min(text.size(), cuts.back() + target);
+        while (c < text.size() && text[c - 1] != '\n') ++c;
+        cuts.push_back(c);
+    }
+    const size_t nchunks = cuts.size() - 1;
+    std::vector<ObjChunk> chunks(nchunks);
+    parallel_for(nchunks, [&](size_t i) { parse_obj_chunk(text.data() + cuts[i], text.data() + cuts[i + 1], chunks[i]); });
+
+    // Global vertex arrays and the chunks' offsets into them.
+    std::vector<size_t> v0(nchunks + 1, 0), vt0(nchunks + 1, 0), vn0(nchunks + 1, 0), t0(nchunks + 1, 0);
+    for (size_t i = 0; i < nchunks; ++i) {
+        v0[i + 1] = v0[i] + chunks[i].V.size() / 3;
+        vt0[i + 1] = vt0[i] + chunks[i].VT.size() / 2;
+        vn0[i + 1] = vn0[i] + chunks[i].VN.size() / 3;
+        t0[i + 1] = t0[i] + chunks[i].tris.size();
+    }
+    std::vector<float> V(3 * v0[nchunks]), VT(2 * vt0[nchunks]), VN(3 * vn0[nchunks]);
+    parallel_for(nchunks, [&](size_t i) {
+        std::copy(chunks[i].V.begin(), chunks[i].V.end(), V.begin() + 3 * v0[i]);
+        std::copy(chunks[i].VT.begin(), chunks[i].VT.end(), VT.begin() + 2 * vt0[i]);
+        std::copy(chunks[i].VN.begin(), chunks[i].VN.end(), VN.begin() + 3 * vn0[i]);
+    });
+
+    // Statements in file order: shape names, and every triangle's (shape, material).
+    const size_t ntris = t0[nchunks];
+    std::vector<std::string> shape_names{""};
+    std::vector<std::string> materials;
+    std::map<std::string, int> material_id;
+    std::vector<std::pair<int, int>> tri_key(ntris);
+    int shape = 0, mtl = -1;
+    bool shape_has_faces = false;
+    for (size_t i = 0; i < nchunks; ++i) {
+        size_t t = 0;
+        auto emit_until = [&](size_t stop) {
+            if (t < stop) shape_has_faces = true;
+            for (; t < stop; ++t) tri_key[t0[i] + t] = {shape, mtl};
+        };
+        for (const ObjEvent& ev : chunks[i].events) {
+            emit_until(ev.tri);
+            if (ev.kind == 0) {
+                if (shape_has_faces) {
+                    shape_names.push_back(ev.name);
+                    ++shape;
+                    shape_has_faces = false;
+                } else {
+                    shape_names[shape] = ev.name;
+                }
+            } else if (ev.name.empty()) {
+                mtl = -1;  // the same (empty) material as before any usemtl
+            } else {
+                auto it = material_id.find(ev.name);
+                if (it == material_id.end()) {
+                    it = material_id.emplace(ev.name, (int)materials.size()).first;
+                    materials.push_back(ev.name);
+                }
+                mtl = it->second;
+            }
+        }
+        emit_until(chunks[i].tris.size());
     }
 
+    // Meshes in order of first appearance; each triangle's slot in its mesh.
     std::vector<LoadedMesh> meshes;
-    std::map<std::pair<int, std::string>, size_t> lookup;
-    auto vtx = [&](const ObjIdx& ix, vec3& p, vec3& n, float& u, float& v, bool& has_n) {
-        if (ix.v < 0 || (size_t)ix.v * 3 + 2 >= V.size() + 0 || (size_t)(3 * ix.v + 2) >= V.size())
-            throw std::runtime_error("OBJ face references a missing vertex in " + path);
-        p = xform_point(xf, v3(V[3 * ix.v], V[3 * ix.v + 1], V[3 * ix.v + 2]));
-        has_n = ix.vn >= 0 && (size_t)(3 * ix.vn + 2) < VN.size();
-        n = has_n ? xform_normal(xf_inv, v3(VN[3 * ix.vn], VN[3 * ix.vn + 1], VN[3 * ix.vn + 2])) : v3(0.f, 0.f, 0.f);
-        if (ix.vt >= 0 && (size_t)(2 * ix.vt + 1) < VT.size()) { u = VT[2 * ix.vt]; v = VT[2 * ix.vt + 1]; }
-        else { u = 0.f; v = 0.f; }
-    };
-    for (const Face& f : faces) {
-        auto key = std::make_pair(f.shape, f.mtl);
-        size_t mi;
-        auto it = lookup.find(key);
-        if (it == lookup.end()) {
-            LoadedMesh m;
-            const std::string& sn = shape_names[f.shape];
-            std::string base = sn.empty() ? stem_of(path) + "_shape_" + std::to_string(f.shape) : sn;
-            m.name = f.mtl.empty() ? base : base + "::" + f.mtl;
-            m.material = f.mtl;
-            mi = meshes.size();
-            lookup[key] = mi;
-            meshes.push_back(std::move(m));
-        } else {
-            mi = it->second;
+    std::map<std::pair<int, int>, int> lookup;
+    std::vector<int32_t> tri_mesh(ntris), tri_slot(ntris);
+    std::pair<int, int> last_key{-2, -2};
+    int last_mesh = -1;
+    for (size_t t = 0; t < ntris; ++t) {
+        if (tri_key[t] != last_key) {
+            auto it = lookup.find(tri_key[t]);
+            if (it == lookup.end()) {
+                LoadedMesh m;
+                const int sh = tri_key[t].first, mt = tri_key[t].second;
+                const std::string& sn = shape_names[sh];
+                const std::string mname = mt >= 0 ? materials[mt] : std::string();
+                std::string base = sn.empty() ? stem_of(path) + "_shape_" + std::to_string(sh) : sn;
+                m.name = mname.empty() ? base : base + "::" + mname;
+                m.material = mname;
+                it = lookup.emplace(tri_key[t], (int)meshes.size()).first;
+                meshes.push_back(std::move(m));
+            }
+            last_key = tri_key[t];
+            last_mesh = it->second;
         }
-        vec3 p[3], n[3];
-        float u[3], v[3];
-        bool hn[3];
-        vtx(f.a, p[0], n[0], u[0], v[0], hn[0]);
-        vtx(f.b, p[1], n[1], u[1], v[1], hn[1]);
-        vtx(f.c, p[2], n[2], u[2], v[2], hn[2]);
-        if (!(hn[0] && hn[1] && hn[2])) {
-            vec3 fn = normalize(cross(p[1] - p[0], p[2] - p[0]));
-            n[0] = n[1] = n[2] = fn;
-        }
-        LoadedMesh& m = meshes[mi];
-        for (int k = 0; k < 3; ++k) {
-            m.pos.push_back(p[k].x); m.pos.push_back(p[k].y); m.pos.push_back(p[k].z);
-            m.nrm.push_back(n[k].x); m.nrm.push_back(n[k].y); m.nrm.push_back(n[k].z);
-            m.uv.push_back(u[k]); m.uv.push_back(v[k]);
-        }
-        m.count++;
+        tri_mesh[t] = last_mesh;
+        tri_slot[t] = meshes[last_mesh].count++;
     }
+    for (LoadedMesh& m : meshes) {
+        m.pos.resize(9 * (size_t)m.count);
+        m.nrm.resize(9 * (size_t)m.count);
+        m.uv.resize(6 * (size_t)m.count);
+    }
+
+    auto resolve = [](int32_t idx, size_t n) -> int { return fix_index(idx, n); };
+    const size_t block = 4096;
+    parallel_for((ntris + block - 1) / block, [&](size_t bi) {
+        size_t ci = std::upper_bound(t0.begin(), t0.end(), bi * block) - t0.begin() - 1;
+        for (size_t t = bi * block; t < std::min(ntris, (bi + 1) * block); ++t) {
+            while (t >= t0[ci + 1]) ++ci;
+            const ObjTri& tr = chunks[ci].tris[t - t0[ci]];
+            vec3 p[3], n[3];
+            float u[3], v[3];
+            bool hn[3];
+            for (int k = 0; k < 3; ++k) {
+                const ObjCorner& c = tr.c[k];
+                const int iv = resolve(c.v, v0[ci] + (size_t)tr.nv);
+                const int ivt = resolve(c.vt, vt0[ci] + (size_t)tr.nvt);
+                const int ivn = resolve(c.vn, vn0[ci] + (size_t)tr.nvn);
+                if (iv < 0 || (size_t)(3 * (size_t)iv + 2) >= V.size())
+                    throw std::runtime_error("OBJ face references a missing vertex in " + path);
+                p[k] = xform_point(xf, v3(V[3 * (size_t)iv], V[3 * (size_t)iv + 1], V[3 * (size_t)iv + 2]));
+                hn[k] = ivn >= 0 && (size_t)(3 * (size_t)ivn + 2) < VN.size();
+                n[k] = hn[k] ? xform_normal(xf_inv, v3(VN[3 * (size_t)ivn], VN[3 * (size_t)ivn + 1], VN[3 * (size_t)ivn + 2]))
+                             : v3(0.f, 0.f, 0.f);
+                if (ivt >= 0 && (size_t)(2 * (size_t)ivt + 1) < VT.size()) {
+                    u[k] = VT[2 * (size_t)ivt];
+                    v[k] = VT[2 * (size_t)ivt + 1];
+                } else {
+                    u[k] = 0.f;
+                    v[k] = 0.f;
+                }
+            }
+            if (!(hn[0] && hn[1] && hn[2])) {
+                vec3 fn = normalize(cross(p[1] - p[0], p[2] - p[0]));
+                n[0] = n[1] = n[2] = fn;
+            }
+            LoadedMesh& m = meshes[tri_mesh[t]];
+            const size_t o = (size_t)tri_slot[t];
+            for (int k = 0; k < 3; ++k) {
+                m.pos[9 * o + 3 * k] = p[k].x; m.pos[9 * o + 3 * k + 1] = p[k].y; m.pos[9 * o + 3 * k + 2] = p[k].z;
+                m.nrm[9 * o + 3 * k] = n[k].x; m.nrm[9 * o + 3 * k + 1] = n[k].y; m.nrm[9 * o + 3 * k + 2] = n[k].z;
+                m.uv[6 * o + 2 * k] = u[k]; m.uv[6 * o + 2 * k + 1] = v[k];
+            }
+        }
+    });
     return meshes;
 }
 
@@ -569,6 +737,11 @@ void load_scene(const std::string& json_path, HostScene& sc, int bvh_builder) {
     }
 
     // ---- objects (reference: Src/Core/scene.cpp:398-537)
+    using Clock = std::chrono::steady_clock;
+    auto ms_since = [](Clock::time_point t) {
+        return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
+    };
+    auto t_load = Clock::now();
     const Json& objj = root["objects"];
     if (!objj.is_array()) throw std::runtime_error("'objects' should be an array");
     std::unordered_map<std::string, int> mesh_map;
@@ -606,6 +779,12 @@ void load_scene(const std::string& json_path, HostScene& sc, int bvh_builder) {
             continue;
         }
         bool expanded = loaded.size() > 1;
+        size_t more = 0;
+        for (const LoadedMesh& lm : loaded) more += (size_t)lm.count;
+        sc.pos.reserve(sc.pos.size() + 9 * more);
+        sc.nrm.reserve(sc.nrm.size() + 9 * more);
+        sc.uv.reserve(sc.uv.size() + 6 * more);
+        sc.mesh.reserve(sc.mesh.size() + more);
         for (LoadedMesh& lm : loaded) {
             if (lm.count == 0) continue;
             // ResolveMaterialName (reference: Src/Core/scene.cpp:12-36).
@@ -626,15 +805,24 @@ void load_scene(const std::string& json_path, HostScene& sc, int bvh_builder) {
             mr.start = sc.num_tris();
             mr.count = lm.count;
             int mid = (int)sc.meshes.size();
-            sc.pos.insert(sc.pos.end(), lm.pos.begin(), lm.pos.end());
-            sc.nrm.insert(sc.nrm.end(), lm.nrm.begin(), lm.nrm.end());
-            sc.uv.insert(sc.uv.end(), lm.uv.begin(), lm.uv.end());
+            if (sc.pos.empty()) {  // the first mesh: take its arrays
+                sc.pos.swap(lm.pos);
+                sc.nrm.swap(lm.nrm);
+                sc.uv.swap(lm.uv);
+            } else {
+                sc.pos.insert(sc.pos.end(), lm.pos.begin(), lm.pos.end());
+                sc.nrm.insert(sc.nrm.end(), lm.nrm.begin(), lm.nrm.end());
+                sc.uv.insert(sc.uv.end(), lm.uv.begin(), lm.uv.end());
+            }
             sc.mesh.insert(sc.mesh.end(), (size_t)lm.count, mid);
             mesh_map[mr.name] = mid;
             sc.meshes.push_back(mr);
         }
     }
+    sc.load_phase_ms[0] = ms_since(t_load);
+    t_load = Clock::now();
     finalize_triangles(sc);
+    sc.load_phase_ms[1] = ms_since(t_load);
 
     // ---- lights (reference: Src/Core/scene.cpp:539-616)
     const Json& lj = root["lights"];
@@ -675,7 +863,9 @@ void load_scene(const std::string& json_path, HostScene& sc, int bvh_builder) {
         } else if (type == "env") {
             rec.type = kEnvLight;
             if (!l.contains("path")) throw std::runtime_error("env light missing 'path'");
+            auto t_env = Clock::now();
             load_env(resolve_path(base, l["path"].as_string("light.path")), sc.env);
+            sc.load_phase_ms[2] += ms_since(t_env);
             sc.env_light = (int)i;
         } else {
             throw std::runtime_error("unsupported light type " + type);

@@ -90,6 +90,7 @@ struct HostScene {
     int rr_start_depth = 0;     // 0: roulette from the first bounce, as the reference
     BvhBuild bvh;
     double bvh_build_ms = 0.0;  // wall time of the BVH build
+    double load_phase_ms[3] = {0, 0, 0};  // OBJ import, triangle finalize, environment map
     bool bvh_on_gpu = false;
     std::vector<std::string> warnings;
 

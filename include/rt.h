@@ -113,6 +113,7 @@ typedef struct rt_scene_info {
     float bvh_build_ms;     /* wall time of the BVH build */
     int bvh_on_gpu;         /* 1 when the GPU built it */
     float bvh_gpu_phase_ms[4]; /* GPU build: setup + upload, levels, download, host numbering */
+    float load_phase_ms[3];    /* OBJ import (parse, transforms), triangle normals and areas, environment map */
 } rt_scene_info;
 
 typedef struct rt_render_params {
