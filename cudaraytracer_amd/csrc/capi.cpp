@@ -275,24 +275,51 @@ int rt_read_accum(rt_renderer* r, int which, float* rgba) {
     });
 }
 
+// GatherKernel colour (renderer.cuh:969-988): gamma 1/2.2, clamp, truncate to
+// 8 bits, alpha 255, rows flipped to top first.  `img` is RGBA float, row 0 = bottom.
+static void gather_rgba8(const float* img, int w, int h, uint8_t* out) {
+    for (int j = 0; j < h; ++j) {
+        int dst = h - 1 - j;  // top row first
+        for (int i = 0; i < w; ++i) {
+            const float* c = &img[((size_t)j * w + i) * 4];
+            uint8_t* o = &out[((size_t)dst * w + i) * 4];
+            for (int k = 0; k < 3; ++k) {
+                float v = std::pow(c[k], 1.0f / 2.2f) * 255.0f;
+                v = std::fmin(std::fmax(v, 0.0f), 255.0f);
+                o[k] = (uint8_t)v;
+            }
+            o[3] = 255;
+        }
+    }
+}
+
 int rt_read_rgba8(rt_renderer* r, uint8_t* out) {
     if (!r || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
     return guard([&]() {
         int w = r->r->width(), h = r->r->height();
         std::vector<float> img((size_t)w * h * 4);
         r->r->read_accum(RT_ACCUM_MEAN, img.data());
-        for (int j = 0; j < h; ++j) {
-            int dst = h - 1 - j;  // top row first
-            for (int i = 0; i < w; ++i) {
-                const float* c = &img[((size_t)j * w + i) * 4];
-                uint8_t* o = &out[((size_t)dst * w + i) * 4];
-                for (int k = 0; k < 3; ++k) {
-                    float v = std::pow(c[k], 1.0f / 2.2f) * 255.0f;
-                    v = std::fmin(std::fmax(v, 0.0f), 255.0f);
-                    o[k] = (uint8_t)v;
-                }
-                o[3] = 255;
-            }
+        gather_rgba8(img.data(), w, h, out);
+        return RT_OK;
+    });
+}
+
+int rt_save_image(rt_renderer* r, int which, const char* path) {
+    if (!r || !path) return fail(RT_ERR_INVALID_ARG, "null argument");
+    if (which < RT_IMAGE_PNG_COLOR || which > RT_IMAGE_PFM_DENOISED) return fail(RT_ERR_INVALID_ARG, "unknown image kind");
+    return guard([&]() {
+        int w = r->r->width(), h = r->r->height();
+        const bool denoised = which == RT_IMAGE_PNG_DENOISED || which == RT_IMAGE_PFM_DENOISED;
+        if (denoised && r->r->frames() == 0) throw std::invalid_argument("no real-time frame has been rendered");
+        std::vector<float> img((size_t)w * h * 4);
+        if (denoised) r->r->read_frame(RT_FRAME_DENOISED, img.data());
+        else r->r->read_accum(RT_ACCUM_MEAN, img.data());
+        if (which == RT_IMAGE_PNG_COLOR || which == RT_IMAGE_PNG_DENOISED) {
+            std::vector<uint8_t> px((size_t)w * h * 4);
+            gather_rgba8(img.data(), w, h, px.data());
+            rt::write_png_rgba8(path, w, h, px.data());
+        } else {
+            rt::write_pfm_rgb(path, w, h, img.data());
         }
         return RT_OK;
     });

@@ -947,6 +947,55 @@ void decode_ldr_image(const uint8_t* data, size_t size, LdrImage& out) {
     }
 }
 
+void write_png_rgba8(const std::string& path, int w, int h, const uint8_t* rgba) {
+    if (w <= 0 || h <= 0) fail("PNG: empty image");
+    const size_t stride = (size_t)4 * w;
+    std::vector<uint8_t> raw((stride + 1) * h);
+    for (int y = 0; y < h; ++y) {
+        raw[(stride + 1) * y] = 0;  // no row filter
+        std::memcpy(&raw[(stride + 1) * y + 1], rgba + stride * y, stride);
+    }
+    uLongf zlen = compressBound((uLong)raw.size());
+    std::vector<uint8_t> z(zlen);
+    if (compress2(z.data(), &zlen, raw.data(), (uLong)raw.size(), 6) != Z_OK) fail("PNG: zlib compression failed");
+    std::vector<uint8_t> file = {137, 80, 78, 71, 13, 10, 26, 10};
+    auto chunk = [&](const char* tag, const uint8_t* data, size_t len) {
+        const uint8_t hdr[8] = {(uint8_t)(len >> 24), (uint8_t)(len >> 16), (uint8_t)(len >> 8), (uint8_t)len,
+                                (uint8_t)tag[0], (uint8_t)tag[1], (uint8_t)tag[2], (uint8_t)tag[3]};
+        file.insert(file.end(), hdr, hdr + 8);
+        file.insert(file.end(), data, data + len);
+        uLong crc = crc32(0L, hdr + 4, 4);
+        crc = crc32(crc, data, (uInt)len);
+        const uint8_t c[4] = {(uint8_t)(crc >> 24), (uint8_t)(crc >> 16), (uint8_t)(crc >> 8), (uint8_t)crc};
+        file.insert(file.end(), c, c + 4);
+    };
+    const uint8_t ihdr[13] = {(uint8_t)(w >> 24), (uint8_t)(w >> 16), (uint8_t)(w >> 8), (uint8_t)w,
+                              (uint8_t)(h >> 24), (uint8_t)(h >> 16), (uint8_t)(h >> 8), (uint8_t)h,
+                              8, 6, 0, 0, 0};
+    chunk("IHDR", ihdr, 13);
+    chunk("IDAT", z.data(), zlen);
+    chunk("IEND", nullptr, 0);
+    std::FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) fail("cannot write '" + path + "'");
+    bool ok = std::fwrite(file.data(), 1, file.size(), f) == file.size();
+    ok = std::fclose(f) == 0 && ok;
+    if (!ok) fail("cannot write '" + path + "'");
+}
+
+void write_pfm_rgb(const std::string& path, int w, int h, const float* rgba) {
+    if (w <= 0 || h <= 0) fail("PFM: empty image");
+    std::string hdr = "PF\n" + std::to_string(w) + " " + std::to_string(h) + "\n-1.0\n";
+    std::vector<float> rgb((size_t)3 * w * h);
+    for (size_t i = 0; i < (size_t)w * h; ++i)
+        for (int k = 0; k < 3; ++k) rgb[3 * i + k] = rgba[4 * i + k];
+    std::FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) fail("cannot write '" + path + "'");
+    bool ok = std::fwrite(hdr.data(), 1, hdr.size(), f) == hdr.size() &&
+              std::fwrite(rgb.data(), sizeof(float), rgb.size(), f) == rgb.size();  // little-endian host
+    ok = std::fclose(f) == 0 && ok;
+    if (!ok) fail("cannot write '" + path + "'");
+}
+
 void load_ldr_image(const std::string& path, LdrImage& out) {
     std::ifstream f(path, std::ios::binary);
     if (!f) fail("cannot open image '" + path + "'");

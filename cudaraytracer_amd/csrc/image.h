@@ -23,4 +23,12 @@ void decode_ldr_image(const uint8_t* data, size_t size, LdrImage& out);
 // Read and decode a file.  Throws std::runtime_error naming the file.
 void load_ldr_image(const std::string& path, LdrImage& out);
 
+// 8-bit RGBA PNG, row 0 = the top of the image (what stbi_write_png writes for
+// the reference's screenshot, Src/Utils/ui.cpp:370-380; the pixels, not the
+// compressed bytes, are the same).  Throws std::runtime_error.
+void write_png_rgba8(const std::string& path, int w, int h, const uint8_t* rgba);
+// Colour PFM (little-endian float RGB) of an RGBA float image whose row 0 is
+// the bottom row, which is PFM's own row order.  Throws std::runtime_error.
+void write_pfm_rgb(const std::string& path, int w, int h, const float* rgba);
+
 }  // namespace rt

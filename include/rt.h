@@ -59,6 +59,11 @@ extern "C" {
 #define RT_ACCUM_INDIRECT 1
 #define RT_ACCUM_MEAN 2 /* (direct + indirect) / spp */
 
+#define RT_IMAGE_PNG_COLOR 0
+#define RT_IMAGE_PNG_DENOISED 1
+#define RT_IMAGE_PFM_MEAN 2
+#define RT_IMAGE_PFM_DENOISED 3
+
 #define RT_STAGE_RAYGEN 0
 #define RT_STAGE_EXTEND 1
 #define RT_STAGE_SHADE 2
@@ -186,6 +191,11 @@ int rt_reset_stats(rt_renderer* r);
 int rt_read_accum(rt_renderer* r, int which, float* rgba_out);
 /* Gamma-2.2 8-bit colour, top row first (GatherKernel colour mode). */
 int rt_read_rgba8(rt_renderer* r, uint8_t* rgba_out);
+/* Write an image file (UI::SaveScreenshot, Src/Utils/ui.cpp:370-380, writes the gathered colour as
+   an RGBA PNG): RT_IMAGE_PNG_COLOR = rt_read_rgba8's pixels, RT_IMAGE_PNG_DENOISED = the same gather
+   of the last real-time frame's denoised colour (GatherKernel's denoise branch), RT_IMAGE_PFM_MEAN /
+   RT_IMAGE_PFM_DENOISED = the float colour as a PFM. */
+int rt_save_image(rt_renderer* r, int which, const char* path);
 int rt_stats(rt_renderer* r, rt_stats_out* out);
 const char* rt_last_error(void);
 
