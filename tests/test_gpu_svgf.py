@@ -104,6 +104,11 @@ def test_svgf_frames_bit_exact_terrain_env_sss(scene_dir, oracle_lib):
     _run(scene_dir["terrain_small"], spp=2, atrous_iters=3)
 
 
+def test_svgf_frames_bit_exact_textured(scene_dir, oracle_lib):
+    """Diffuse textures on every textured lobe, through the whole frame loop."""
+    _run(scene_dir["textured_cornell"], spp=1)
+
+
 @pytest.mark.parametrize("w,h", [(37, 23), (2, 3), (1, 1)])
 def test_svgf_ragged_and_tiny_images(tmp_path, oracle_lib, w, h):
     """Image sizes that no tile shape divides, down to one pixel: every buffer
