@@ -807,11 +807,39 @@ __global__ void k_bvh_emit(Nodes nodes, int nn, const int32_t* __restrict__ dfs,
     packed[comp[i]] = p;
 }
 
+// Device buffer allocated and freed in stream order on the build's stream
+// (no device-wide synchronisation, unlike hipMalloc / hipFree).
+struct StreamBuf {
+    void* p = nullptr;
+    hipStream_t s = nullptr;
+    explicit StreamBuf(hipStream_t stream) : s(stream) {}
+    StreamBuf(const StreamBuf&) = delete;
+    StreamBuf& operator=(const StreamBuf&) = delete;
+    void alloc(size_t bytes) { HIP_OK(hipMallocAsync(&p, std::max<size_t>(bytes, 256), s)); }
+    template <typename T>
+    T* as() const { return reinterpret_cast<T*>(p); }
+    ~StreamBuf() {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+};
+
 }  // namespace
 
 void build_bvh_gpu(const float* pos, int num_tris, BvhBuild& out) {
     out = BvhBuild();
     if (num_tris <= 0) return;
+    // Its own non-blocking stream: a build does not wait for, or hold up,
+    // rendering on other streams.
+    struct Stream {
+        hipStream_t s = nullptr;
+        Stream() { HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+        ~Stream() { (void)hipStreamDestroy(s); }
+    } stream;
+    hipStream_t st = stream.s;
+    auto copy = [st](void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+        return e == hipSuccess ? hipStreamSynchronize(st) : e;
+    };
     using Clock = std::chrono::steady_clock;
     auto ms_since = [](Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); };
     auto t_phase = Clock::now();
@@ -819,8 +847,9 @@ void build_bvh_gpu(const float* pos, int num_tris, BvhBuild& out) {
     const size_t cap = 2 * n;  // a binary tree over n leaves' ranges has < 2n nodes
     const size_t max_huge = n / kHugeNode + 1;
     const size_t max_tasks = n / kChunk + max_huge + 1;
-    DevBuf dpos, e0a, e0b, e0c, e1a, e1b, e1c, dorder, dscratch, nrange, nkids, nlo, nhi, dcount, dsmall, dmedium,
-        dhuge, dacc, dtasks, dtask_first, dtask_cnt, dtask_off;
+    StreamBuf dpos(st), e0a(st), e0b(st), e0c(st), e1a(st), e1b(st), e1c(st), dorder(st), dscratch(st), nrange(st),
+        nkids(st), nlo(st), nhi(st), dcount(st), dsmall(st), dmedium(st), dhuge(st), dacc(st), dtasks(st),
+        dtask_first(st), dtask_cnt(st), dtask_off(st);
     dpos.alloc(n * 36);
     e0a.alloc(n * 16); e0b.alloc(n * 16); e0c.alloc(n * 8);
     e1a.alloc(n * 16); e1b.alloc(n * 16); e1c.alloc(n * 8);
@@ -839,19 +868,19 @@ void build_bvh_gpu(const float* pos, int num_tris, BvhBuild& out) {
     dtask_first.alloc((max_huge + 1) * 4);
     dtask_cnt.alloc(max_tasks * 8);
     dtask_off.alloc(max_tasks * 8);
-    HIP_OK(hipMemcpy(dpos.p, pos, n * 36, hipMemcpyHostToDevice));
+    HIP_OK(copy(dpos.p, pos, n * 36, hipMemcpyHostToDevice));
     Elems buf[2] = {{e0a.as<float4>(), e0b.as<float4>(), e0c.as<float2>()},
                     {e1a.as<float4>(), e1b.as<float4>(), e1c.as<float2>()}};
     Nodes nodes{nrange.as<int4>(), nkids.as<int2>(), nlo.as<float4>(), nhi.as<float4>()};
-    k_bvh_init<<<(num_tris + 255) / 256, 256>>>(dpos.as<float>(), num_tris, buf[0]);
+    k_bvh_init<<<(num_tris + 255) / 256, 256, 0, st>>>(dpos.as<float>(), num_tris, buf[0]);
     HIP_OK(hipGetLastError());
     int4 root = make_int4(0, num_tris, 0, 0);
-    HIP_OK(hipMemcpy(nrange.p, &root, sizeof root, hipMemcpyHostToDevice));
+    HIP_OK(copy(nrange.p, &root, sizeof root, hipMemcpyHostToDevice));
     // counters: nodes created, error bits, small / medium list lengths, fallback splits, huge list length
     int ctr[6] = {1, 0, 0, 0, 0, 0};
-    HIP_OK(hipMemcpy(dcount.p, ctr, sizeof ctr, hipMemcpyHostToDevice));
+    HIP_OK(copy(dcount.p, ctr, sizeof ctr, hipMemcpyHostToDevice));
     int count = 1;
-    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipStreamSynchronize(st));
     out.gpu_phase_ms[0] = ms_since(t_phase);
     t_phase = Clock::now();
     int level_begin = 0, level_end = 1, cur = 0;
@@ -862,16 +891,16 @@ void build_bvh_gpu(const float* pos, int num_tris, BvhBuild& out) {
     while (level_begin < level_end) {
         levels.push_back(level_begin);
         const int nl = level_end - level_begin;
-        k_bvh_classify<<<(nl + 255) / 256, 256>>>(nodes.range, level_begin, level_end, dsmall.as<int32_t>(),
+        k_bvh_classify<<<(nl + 255) / 256, 256, 0, st>>>(nodes.range, level_begin, level_end, dsmall.as<int32_t>(),
                                                   dmedium.as<int32_t>(), dhuge.as<int4>(), dcount.as<int>());
         HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpy(ctr, dcount.p, sizeof ctr, hipMemcpyDeviceToHost));
+        HIP_OK(copy(ctr, dcount.p, sizeof ctr, hipMemcpyDeviceToHost));
         const int nsmall = ctr[2], nmedium = ctr[3], nhuge = ctr[5];
         const Elems& src = buf[cur];
         const Elems& dst = buf[cur ^ 1];
         if (nhuge > 0) {
             huge.resize(nhuge);
-            HIP_OK(hipMemcpy(huge.data(), dhuge.p, (size_t)nhuge * 16, hipMemcpyDeviceToHost));
+            HIP_OK(copy(huge.data(), dhuge.p, (size_t)nhuge * 16, hipMemcpyDeviceToHost));
             tasks.clear();
             task_first.assign(nhuge + 1, 0);
             for (int j = 0; j < nhuge; ++j) {
@@ -881,38 +910,38 @@ void build_bvh_gpu(const float* pos, int num_tris, BvhBuild& out) {
             }
             task_first[nhuge] = (int)tasks.size();
             const int nt = (int)tasks.size();
-            HIP_OK(hipMemcpy(dtasks.p, tasks.data(), (size_t)nt * sizeof(HugeTask), hipMemcpyHostToDevice));
-            HIP_OK(hipMemcpy(dtask_first.p, task_first.data(), (size_t)(nhuge + 1) * 4, hipMemcpyHostToDevice));
+            HIP_OK(copy(dtasks.p, tasks.data(), (size_t)nt * sizeof(HugeTask), hipMemcpyHostToDevice));
+            HIP_OK(copy(dtask_first.p, task_first.data(), (size_t)(nhuge + 1) * 4, hipMemcpyHostToDevice));
             const HugeTask* tk = dtasks.as<HugeTask>();
             HugeAcc* acc = dacc.as<HugeAcc>();
             const int gs = (nhuge + 63) / 64;
-            k_huge_init<<<gs, 64>>>(acc, nhuge);
-            k_huge_bounds<<<nt, kHugeThreads>>>(src, tk, acc);
-            k_huge_axis<<<gs, 64>>>(src, dhuge.as<int4>(), nhuge, acc, nodes);
-            k_huge_buckets<<<nt, kHugeThreads>>>(src, tk, acc);
-            k_huge_split<<<gs, 64>>>(dhuge.as<int4>(), nhuge, acc, nodes, (int)cap, dcount.as<int>());
-            k_huge_count<<<nt, kHugeThreads>>>(src, tk, acc, dtask_cnt.as<int2>());
-            k_huge_scan<<<gs, 64>>>(dtask_first.as<int>(), nhuge, acc, dtask_cnt.as<int2>(), dtask_off.as<int2>(),
+            k_huge_init<<<gs, 64, 0, st>>>(acc, nhuge);
+            k_huge_bounds<<<nt, kHugeThreads, 0, st>>>(src, tk, acc);
+            k_huge_axis<<<gs, 64, 0, st>>>(src, dhuge.as<int4>(), nhuge, acc, nodes);
+            k_huge_buckets<<<nt, kHugeThreads, 0, st>>>(src, tk, acc);
+            k_huge_split<<<gs, 64, 0, st>>>(dhuge.as<int4>(), nhuge, acc, nodes, (int)cap, dcount.as<int>());
+            k_huge_count<<<nt, kHugeThreads, 0, st>>>(src, tk, acc, dtask_cnt.as<int2>());
+            k_huge_scan<<<gs, 64, 0, st>>>(dtask_first.as<int>(), nhuge, acc, dtask_cnt.as<int2>(), dtask_off.as<int2>(),
                                     dcount.as<int>());
-            k_huge_move<true><<<nt, kHugeThreads>>>(src, dst, dhuge.as<int4>(), tk, acc, dtask_off.as<int2>(),
+            k_huge_move<true><<<nt, kHugeThreads, 0, st>>>(src, dst, dhuge.as<int4>(), tk, acc, dtask_off.as<int2>(),
                                                     dscratch.as<int32_t>());
-            k_huge_move<false><<<nt, kHugeThreads>>>(src, dst, dhuge.as<int4>(), tk, acc, dtask_off.as<int2>(),
+            k_huge_move<false><<<nt, kHugeThreads, 0, st>>>(src, dst, dhuge.as<int4>(), tk, acc, dtask_off.as<int2>(),
                                                      dscratch.as<int32_t>());
             HIP_OK(hipGetLastError());
         }
         if (nmedium > 0)
-            k_bvh_node<kHugeWaves><<<nmedium, kHugeThreads>>>(src, dst, dorder.as<int32_t>(), dscratch.as<int32_t>(),
+            k_bvh_node<kHugeWaves><<<nmedium, kHugeThreads, 0, st>>>(src, dst, dorder.as<int32_t>(), dscratch.as<int32_t>(),
                                                               nodes, dmedium.as<int32_t>(), (int)cap,
                                                               dcount.as<int>());
         if (nsmall > 0)
-            k_bvh_node<1><<<nsmall, kWave>>>(src, dst, dorder.as<int32_t>(), dscratch.as<int32_t>(), nodes,
+            k_bvh_node<1><<<nsmall, kWave, 0, st>>>(src, dst, dorder.as<int32_t>(), dscratch.as<int32_t>(), nodes,
                                              dsmall.as<int32_t>(), (int)cap, dcount.as<int>());
         HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpy(ctr, dcount.p, sizeof ctr, hipMemcpyDeviceToHost));
+        HIP_OK(copy(ctr, dcount.p, sizeof ctr, hipMemcpyDeviceToHost));
         if (ctr[1] != 0 || (size_t)ctr[0] > cap) throw std::runtime_error("GPU BVH build: inconsistent partition");
         count = ctr[0];
         ctr[2] = ctr[3] = ctr[5] = 0;
-        HIP_OK(hipMemcpy(dcount.p, ctr, sizeof ctr, hipMemcpyHostToDevice));
+        HIP_OK(copy(dcount.p, ctr, sizeof ctr, hipMemcpyHostToDevice));
         level_begin = level_end;
         level_end = count;
         cur ^= 1;
@@ -921,7 +950,7 @@ void build_bvh_gpu(const float* pos, int num_tris, BvhBuild& out) {
     t_phase = Clock::now();
     const int nn = count;
     const int depth = (int)levels.size() - 1;
-    DevBuf dsize, dicount, ddfs, dcomp, dlin, dpacked;
+    StreamBuf dsize(st), dicount(st), ddfs(st), dcomp(st), dlin(st), dpacked(st);
     dsize.alloc((size_t)nn * 4);
     dicount.alloc((size_t)nn * 4);
     ddfs.alloc((size_t)nn * 4);
@@ -929,30 +958,30 @@ void build_bvh_gpu(const float* pos, int num_tris, BvhBuild& out) {
     levels.push_back(nn);
     for (int d = depth; d >= 0; --d) {
         int first = levels[d], last = levels[d + 1];
-        k_bvh_sizes<<<(last - first + 255) / 256, 256>>>(nodes.range, nodes.kids, first, last, dsize.as<int32_t>(),
+        k_bvh_sizes<<<(last - first + 255) / 256, 256, 0, st>>>(nodes.range, nodes.kids, first, last, dsize.as<int32_t>(),
                                                          dicount.as<int32_t>());
     }
     for (int d = 0; d <= depth; ++d) {
         int first = levels[d], last = levels[d + 1];
-        k_bvh_index<<<(last - first + 255) / 256, 256>>>(nodes.range, nodes.kids, first, last, dsize.as<int32_t>(),
+        k_bvh_index<<<(last - first + 255) / 256, 256, 0, st>>>(nodes.range, nodes.kids, first, last, dsize.as<int32_t>(),
                                                          dicount.as<int32_t>(), ddfs.as<int32_t>(),
                                                          dcomp.as<int32_t>());
     }
     int interior = 0;
-    HIP_OK(hipMemcpy(&interior, dicount.p, sizeof interior, hipMemcpyDeviceToHost));
+    HIP_OK(copy(&interior, dicount.p, sizeof interior, hipMemcpyDeviceToHost));
     dlin.alloc((size_t)nn * sizeof(LinearNode));
     dpacked.alloc((size_t)std::max(interior, 1) * sizeof(PackedNode));
-    k_bvh_emit<<<(nn + 255) / 256, 256>>>(nodes, nn, ddfs.as<int32_t>(), dcomp.as<int32_t>(), dlin.as<LinearNode>(),
+    k_bvh_emit<<<(nn + 255) / 256, 256, 0, st>>>(nodes, nn, ddfs.as<int32_t>(), dcomp.as<int32_t>(), dlin.as<LinearNode>(),
                                           dpacked.as<PackedNode>(), dcount.as<int>());
     HIP_OK(hipGetLastError());
     out.nodes.resize(nn);
     out.packed.resize(interior);
     out.order.resize(n);
-    HIP_OK(hipMemcpy(out.nodes.data(), dlin.p, (size_t)nn * sizeof(LinearNode), hipMemcpyDeviceToHost));
-    if (interior) HIP_OK(hipMemcpy(out.packed.data(), dpacked.p, (size_t)interior * sizeof(PackedNode),
+    HIP_OK(copy(out.nodes.data(), dlin.p, (size_t)nn * sizeof(LinearNode), hipMemcpyDeviceToHost));
+    if (interior) HIP_OK(copy(out.packed.data(), dpacked.p, (size_t)interior * sizeof(PackedNode),
                                    hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(out.order.data(), dorder.p, n * 4, hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(ctr, dcount.p, sizeof ctr, hipMemcpyDeviceToHost));
+    HIP_OK(copy(out.order.data(), dorder.p, n * 4, hipMemcpyDeviceToHost));
+    HIP_OK(copy(ctr, dcount.p, sizeof ctr, hipMemcpyDeviceToHost));
     out.gpu_phase_ms[2] = ms_since(t_phase);
     t_phase = Clock::now();
     out.fallback_splits = ctr[4];

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <exception>
 #include <mutex>
+#include <system_error>
 #include <thread>
 #include <cstdio>
 #include <cstdlib>
@@ -266,28 +267,47 @@ int host_threads() {
     return (int)std::max(1u, std::min(16u, h ? h : 1u));
 }
 
-// Run f(i) for i in [0, n) on up to host_threads() threads.
+// Run f(i) for i in [0, n) on the calling thread plus helper threads, at most
+// host_threads() helpers across all concurrent calls in the process (scenes
+// may be loaded from several threads at once).
+std::atomic<int> g_helpers{0};
+
 template <typename F>
 void parallel_for(size_t n, F&& f) {
-    const int nt = (int)std::min<size_t>((size_t)host_threads(), std::max<size_t>(1, n));
-    if (nt <= 1) {
-        for (size_t i = 0; i < n; ++i) f(i);
-        return;
+    int want = (int)std::min<size_t>((size_t)host_threads(), n) - 1;
+    int got = 0;
+    while (want > 0) {
+        int cur = g_helpers.load();
+        int take = std::min(want, host_threads() - cur);
+        if (take <= 0) break;
+        if (g_helpers.compare_exchange_weak(cur, cur + take)) {
+            got = take;
+            break;
+        }
     }
     std::atomic<size_t> next{0};
     std::exception_ptr err;
     std::mutex err_mu;
+    auto work = [&]() {
+        try {
+            for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+        } catch (...) {
+            std::lock_guard<std::mutex> lk(err_mu);
+            if (!err) err = std::current_exception();
+            next.store(n);
+        }
+    };
     std::vector<std::thread> pool;
-    for (int t = 0; t < nt; ++t)
-        pool.emplace_back([&]() {
-            try {
-                for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
-            } catch (...) {
-                std::lock_guard<std::mutex> lk(err_mu);
-                if (!err) err = std::current_exception();
-            }
-        });
+    pool.reserve(got);
+    try {
+        for (int t = 0; t < got; ++t) pool.emplace_back(work);
+    } catch (const std::system_error&) {  // fewer helpers if threads cannot be created
+        g_helpers.fetch_sub(got - (int)pool.size());
+        got = (int)pool.size();
+    }
+    work();
     for (auto& th : pool) th.join();
+    g_helpers.fetch_sub(got);
     if (err) std::rethrow_exception(err);
 }
 

@@ -81,3 +81,24 @@ def test_gpu_build_is_the_automatic_choice_for_large_scenes(tmp_path):
     path = _soup_scene(tmp_path, pos, "auto")
     assert rt.Scene(path).info().bvh_on_gpu == 1
     assert rt.Scene(path, bvh_builder=rt.BVH_HOST).info().bvh_on_gpu == 0
+
+
+def test_gpu_build_beside_a_render(scene_dir, tmp_path):
+    """A build runs on its own stream while a render is in flight on another:
+    both results stay what they are alone."""
+    rng = np.random.default_rng(12)
+    n = 100000
+    pos = (rng.uniform(-1, 1, (n, 1, 3)) + rng.normal(0, 0.02, (n, 3, 3))).astype(np.float32)
+    path = _soup_scene(tmp_path, pos, "beside")
+    alone = rt.Renderer(rt.Scene(scene_dir["cornell_spheres_128"]))
+    alone.render(4)
+    want = alone.read_accum(rt.RT_ACCUM_MEAN)
+    r = rt.Renderer(rt.Scene(scene_dir["cornell_spheres_128"]))
+    r.render(4)  # asynchronous on the renderer's stream
+    built = rt.Scene(path, bvh_builder=rt.BVH_GPU)
+    host = rt.Scene(path, bvh_builder=rt.BVH_HOST)
+    r.synchronize()
+    assert np.array_equal(r.read_accum(rt.RT_ACCUM_MEAN).view(np.int32), want.view(np.int32))
+    gn, go = built.bvh()
+    hn, ho = host.bvh()
+    assert np.array_equal(gn, hn) and np.array_equal(go, ho)
