@@ -597,6 +597,37 @@ __global__ __launch_bounds__(kBlock) void k_combine(const float4* __restrict__ d
 // Host side
 // ======================================================================
 
+// Triangle k of the BVH order (original triangle t = order[k]): the traversal
+// record (v0, e1 = v1 - v0, e2 = v2 - v0) and the 128-byte attribute line.
+// mesh_tab holds (first triangle, material, light, shading class) per mesh.
+__global__ void k_pack_triangles(int n, const int32_t* __restrict__ order, const float* __restrict__ pos,
+                                 const float* __restrict__ nrm, const float* __restrict__ uv,
+                                 const float* __restrict__ gn, const int32_t* __restrict__ mesh,
+                                 const int4* __restrict__ mesh_tab, float4* __restrict__ isect,
+                                 float4* __restrict__ attr) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int t = order[k];
+    const float* p = pos + 9 * (size_t)t;
+    const float* nn = nrm + 9 * (size_t)t;
+    const float* w = uv + 6 * (size_t)t;
+    const float* g = gn + 3 * (size_t)t;
+    const int m = mesh[t];
+    const int4 mr = mesh_tab[m];
+    isect[3 * (size_t)k + 0] = make_float4(p[0], p[1], p[2], 0.f);
+    isect[3 * (size_t)k + 1] = make_float4(p[3] - p[0], p[4] - p[1], p[5] - p[2], 0.f);
+    isect[3 * (size_t)k + 2] = make_float4(p[6] - p[0], p[7] - p[1], p[8] - p[2], 0.f);
+    float4* a = attr + kAttrStride * (size_t)k;
+    a[0] = make_float4(p[0], p[1], p[2], i_as_f(t - mr.x));
+    a[1] = make_float4(p[3], p[4], p[5], g[0]);
+    a[2] = make_float4(p[6], p[7], p[8], g[1]);
+    a[3] = make_float4(nn[0], nn[1], nn[2], g[2]);
+    a[4] = make_float4(nn[3], nn[4], nn[5], w[0]);
+    a[5] = make_float4(nn[6], nn[7], nn[8], w[1]);
+    a[6] = make_float4(w[2], w[3], w[4], w[5]);
+    a[7] = make_float4(i_as_f(m), i_as_f(mr.y), i_as_f(mr.z), i_as_f(mr.w));
+}
+
 struct GpuScene::Impl {
     DevBuf nodes, tri_isect, tri_attr, materials, lights, light_tris, env_rgb, env_row, env_marg, env_alias_row,
         env_alias_marg, tex_texels, tex_info;
@@ -622,7 +653,7 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
     std::vector<PackedNode> nodes = bvh.packed;
     if (nodes.empty()) nodes.resize(1);
     upload(I.nodes, nodes.data(), nodes.size() * sizeof(PackedNode));
-    std::vector<float4> isect(3 * (size_t)n), attr(kAttrStride * (size_t)n);
+    // Triangle lines in BVH order, packed on the GPU from the scene-order arrays.
     auto class_of = [&](int material) -> int {
         switch (sc.materials[material].type) {
             case kLight: return kClsTerminal;
@@ -632,32 +663,31 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
             default: return kClsDiffuse;
         }
     };
-    for (int k = 0; k < n; ++k) {
-        int t = bvh.order[k];
-        const float* p = &sc.pos[9 * (size_t)t];
-        const float* nn = &sc.nrm[9 * (size_t)t];
-        const float* uv = &sc.uv[6 * (size_t)t];
-        const float* g = &sc.gnormal[3 * (size_t)t];
-        vec3 v0 = v3(p[0], p[1], p[2]), v1 = v3(p[3], p[4], p[5]), v2 = v3(p[6], p[7], p[8]);
-        vec3 e1 = v1 - v0, e2 = v2 - v0;
-        isect[3 * (size_t)k + 0] = make_float4(v0.x, v0.y, v0.z, 0.f);
-        isect[3 * (size_t)k + 1] = make_float4(e1.x, e1.y, e1.z, 0.f);
-        isect[3 * (size_t)k + 2] = make_float4(e2.x, e2.y, e2.z, 0.f);
-        float4* a = &attr[kAttrStride * (size_t)k];
-        const MeshRec& mr = sc.meshes[sc.mesh[t]];
+    std::vector<int4> mesh_tab(std::max<size_t>(sc.meshes.size(), 1));
+    for (size_t m = 0; m < sc.meshes.size(); ++m) {
+        const MeshRec& mr = sc.meshes[m];
         int cls = class_of(mr.material);
-        I.has_class[cls] = true;
-        a[0] = make_float4(p[0], p[1], p[2], i_as_f(t - mr.start));
-        a[1] = make_float4(p[3], p[4], p[5], g[0]);
-        a[2] = make_float4(p[6], p[7], p[8], g[1]);
-        a[3] = make_float4(nn[0], nn[1], nn[2], g[2]);
-        a[4] = make_float4(nn[3], nn[4], nn[5], uv[0]);
-        a[5] = make_float4(nn[6], nn[7], nn[8], uv[1]);
-        a[6] = make_float4(uv[2], uv[3], uv[4], uv[5]);
-        a[7] = make_float4(i_as_f(sc.mesh[t]), i_as_f(mr.material), i_as_f(mr.light), i_as_f(cls));
+        if (mr.count > 0) I.has_class[cls] = true;
+        mesh_tab[m] = make_int4(mr.start, mr.material, mr.light, cls);
     }
-    upload(I.tri_isect, isect.data(), isect.size() * sizeof(float4));
-    upload(I.tri_attr, attr.data(), attr.size() * sizeof(float4));
+    {
+        DevBuf dpos, dnrm, duv, dgn, dmesh, dorder, dtab;
+        upload(dpos, sc.pos.data(), sc.pos.size() * sizeof(float));
+        upload(dnrm, sc.nrm.data(), sc.nrm.size() * sizeof(float));
+        upload(duv, sc.uv.data(), sc.uv.size() * sizeof(float));
+        upload(dgn, sc.gnormal.data(), sc.gnormal.size() * sizeof(float));
+        upload(dmesh, sc.mesh.data(), sc.mesh.size() * sizeof(int32_t));
+        upload(dorder, bvh.order.data(), bvh.order.size() * sizeof(int32_t));
+        upload(dtab, mesh_tab.data(), mesh_tab.size() * sizeof(int4));
+        I.tri_isect.alloc(3 * (size_t)n * sizeof(float4));
+        I.tri_attr.alloc(kAttrStride * (size_t)n * sizeof(float4));
+        if (n > 0)
+            k_pack_triangles<<<(n + 255) / 256, 256>>>(n, dorder.as<int32_t>(), dpos.as<float>(), dnrm.as<float>(),
+                                                   duv.as<float>(), dgn.as<float>(), dmesh.as<int32_t>(),
+                                                   dtab.as<int4>(), I.tri_isect.as<float4>(), I.tri_attr.as<float4>());
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipDeviceSynchronize());
+    }
     std::vector<MaterialRec> mats = sc.materials;
     if (mats.empty()) mats.resize(1);
     upload(I.materials, mats.data(), mats.size() * sizeof(MaterialRec));
