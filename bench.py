@@ -16,7 +16,8 @@ written by --traffic-from-pmc).  `cpu_baseline` times the CPU oracle (OpenMP,
 all host cores) on a bounded sample of the same workload.
 
 Other modes: --only NAME runs one secondary workload (frames, c3, batched,
-serving, bvh: the 2 M-triangle SAH build on the host and the GPU) and prints its JSON; --calibrate-copy runs the 1 GiB copy that scales
+serving, bvh: the 2 M-triangle SAH build on the host and the GPU, textured: the
+cost of diffuse textures) and prints its JSON; --calibrate-copy runs the 1 GiB copy that scales
 the counters; --traffic-from-pmc turns four rocprofv3 --pmc databases into the
 traffic table.
 """
@@ -289,6 +290,40 @@ def bvh_build_bench(path, gpu_scene=None):
     }
 
 
+def textured_bench():
+    """Cost of diffuse textures: the textured Cornell scene (PNG albedo on diffuse,
+    GGX, subsurface and dielectric materials) at 512x512, 64 spp, depth 12, against
+    the same scene with every diffuse_texture key removed."""
+    import cudaraytracer_amd as rt
+    from cudaraytracer_amd import scenes
+
+    tmp = tempfile.mkdtemp(prefix="rttex_")
+    spec = scenes.textured_cornell(width=512, height=512, spp=64, depth=12)
+    with_tex = scenes.write_scene(spec, os.path.join(tmp, "tex"), "textured")
+    for m in spec.materials:
+        m.pop("diffuse_texture", None)
+    without = scenes.write_scene(spec, os.path.join(tmp, "plain"), "untextured")
+    out = {"workload": "textured_cornell 2596 tris, 5 PNG textures, 512x512, 64 spp, depth 12"}
+    for name, path in (("textured", with_tex), ("untextured", without)):
+        r = rt.Renderer(rt.Scene(path))
+        r.render(64)
+        r.synchronize()
+        best = None
+        for _ in range(3):
+            r.reset()
+            r.reset_stats()
+            t0 = time.perf_counter()
+            r.render(64)
+            r.synchronize()
+            dt = time.perf_counter() - t0
+            st = r.stats()
+            mr = (st.extension_rays + st.shadow_rays) / dt / 1e6
+            best = mr if best is None else max(best, mr)
+        out[name + "_mrays_s"] = round(best, 1)
+    out["textured_over_untextured"] = round(out["textured_mrays_s"] / out["untextured_mrays_s"], 3)
+    return out
+
+
 def bvh_only():
     tmp = tempfile.mkdtemp(prefix="rtbvh_")
     path = _generate([("terrain_env", {}, tmp, "terrain_bvh")])[0]
@@ -471,7 +506,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-file", default=C2_TRAFFIC,
                     help="HBM bytes per launch measured with rocprofv3 --pmc (table from --traffic-from-pmc)")
-    ap.add_argument("--only", choices=("frames", "c3", "batched", "serving", "bvh"), default=None,
+    ap.add_argument("--only", choices=("frames", "c3", "batched", "serving", "bvh", "textured"), default=None,
                     help="run one secondary workload and print its JSON")
     ap.add_argument("--calibrate-copy", action="store_true", help="run the 1 GiB copy that scales the counters")
     ap.add_argument("--traffic-from-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB"),
@@ -492,7 +527,7 @@ def main() -> int:
         return 0
     if args.only:
         fn = {"frames": frames_bench, "c3": c3_bench, "batched": batched_bench, "serving": serving_bench,
-              "bvh": bvh_only}[args.only]
+              "bvh": bvh_only, "textured": textured_bench}[args.only]
         print(json.dumps({args.only: fn()}))
         return 0
 
