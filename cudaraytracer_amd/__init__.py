@@ -33,6 +33,8 @@ RT_OK = 0
 RT_ACCUM_DIRECT = 0
 RT_ACCUM_INDIRECT = 1
 RT_ACCUM_MEAN = 2
+# Gather modes (rt_read_gather).
+RT_GATHER_COLOR, RT_GATHER_DEPTH, RT_GATHER_NORMAL, RT_GATHER_ID, RT_GATHER_POSITION, RT_GATHER_DENOISED = range(6)
 # Image files (rt_save_image).
 RT_IMAGE_PNG_COLOR = 0
 RT_IMAGE_PNG_DENOISED = 1
@@ -67,7 +69,7 @@ EXPORTS = (
     "rt_renderer_create", "rt_renderer_destroy", "rt_render", "rt_render_counted", "rt_synchronize",
     "rt_reset", "rt_reset_stats", "rt_read_accum", "rt_read_rgba8", "rt_stats", "rt_last_error",
     "rt_scene_get_triangles", "rt_scene_get_meshes", "rt_scene_get_bvh", "rt_scene_get_texture",
-    "rt_texture_lookup", "rt_trace_rays", "rt_device_math", "rt_save_image",
+    "rt_texture_lookup", "rt_trace_rays", "rt_device_math", "rt_save_image", "rt_read_gather",
     "rt_denoise_params_default", "rt_get_camera", "rt_set_camera", "rt_render_frame", "rt_reset_history",
     "rt_read_frame", "rt_pool_create", "rt_pool_destroy", "rt_pool_add_scene", "rt_pool_submit", "rt_pool_step",
     "rt_pool_job_status", "rt_pool_read_accum", "rt_pool_read_frame", "rt_pool_release", "rt_pool_stats",
@@ -229,6 +231,7 @@ def lib() -> ctypes.CDLL:
                                        ctypes.POINTER(ctypes.c_uint8)]
     L.rt_texture_lookup.argtypes = [vp, ip, fp, ip, fp]
     L.rt_save_image.argtypes = [vp, ip, ctypes.c_char_p]
+    L.rt_read_gather.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_uint8)]
     L.rt_trace_rays.argtypes = [vp, fp, ip, fp, ctypes.POINTER(ctypes.c_uint32)]
     L.rt_device_math.argtypes = [ip, fp, fp, fp, ip]
     L.rt_denoise_params_default.argtypes = [ctypes.POINTER(DenoiseParams)]
@@ -411,6 +414,12 @@ class Renderer:
     def read_rgba8(self) -> np.ndarray:
         out = np.zeros((self.height, self.width, 4), np.uint8)
         _check(lib().rt_read_rgba8(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return out
+
+    def read_gather(self, mode: int) -> np.ndarray:
+        """GatherKernel view (RT_GATHER_*): (H, W, 4) uint8, top row first."""
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        _check(lib().rt_read_gather(self._h, mode, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
         return out
 
     def save_image(self, which: int, path: str) -> None:

@@ -304,6 +304,57 @@ int rt_read_rgba8(rt_renderer* r, uint8_t* out) {
     });
 }
 
+int rt_read_gather(rt_renderer* r, int mode, uint8_t* out) {
+    if (!r || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
+    if (mode < RT_GATHER_COLOR || mode > RT_GATHER_DENOISED) return fail(RT_ERR_INVALID_ARG, "unknown gather mode");
+    if (mode == RT_GATHER_COLOR) return rt_read_rgba8(r, out);
+    return guard([&]() {
+        const int w = r->r->width(), h = r->r->height();
+        if (r->r->frames() == 0) throw std::invalid_argument("no real-time frame has been rendered");
+        std::vector<float> f4((size_t)w * h * 4);
+        if (mode == RT_GATHER_DENOISED) {
+            r->r->read_frame(RT_FRAME_DENOISED, f4.data());
+            gather_rgba8(f4.data(), w, h, out);
+            return RT_OK;
+        }
+        // GatherKernel's render-target modes 1-4 (renderer.cuh:990-1029) read the
+        // G-buffer row h-1-j into output row j: top row first, like the colour mode.
+        std::vector<uint16_t> u4((size_t)w * h * 4);
+        const rt::Box& b = r->scene->host.bvh.bounds;
+        const rt::CameraRec& cam = r->r->camera();
+        auto u8 = [](float v) { return (uint8_t)std::fmin(std::fmax(v, 0.0f), 255.0f); };
+        if (mode == RT_GATHER_DEPTH) r->r->read_frame(RT_FRAME_G_MOTION_DEPTH, f4.data());
+        else if (mode == RT_GATHER_POSITION) r->r->read_frame(RT_FRAME_G_POSITION_TRI, f4.data());
+        else r->r->read_frame(mode == RT_GATHER_NORMAL ? RT_FRAME_G_NORMAL_MAT : RT_FRAME_G_BARY_MESH, u4.data());
+        // Depth range: near 0.1, far = scene diagonal + camera distance to the scene centre.
+        rt::vec3 ext = b.hi - b.lo, centre = (b.lo + b.hi) * 0.5f, off = cam.position - centre;
+        const float far_plane = rt::length(ext) + rt::length(off);  // Eigen norm order, as everywhere
+        for (int j = 0; j < h; ++j) {
+            const size_t src_row = (size_t)(h - 1 - j) * w;
+            for (int i = 0; i < w; ++i) {
+                const size_t s = src_row + i;
+                uint8_t* o = out + 4 * ((size_t)j * w + i);
+                o[3] = 255;
+                if (mode == RT_GATHER_DEPTH) {
+                    float d = (f4[4 * s + 2] - 0.1f) / (far_plane - 0.1f);
+                    o[0] = o[1] = o[2] = u8(d * 255.0f);
+                } else if (mode == RT_GATHER_NORMAL) {
+                    for (int k = 0; k < 3; ++k) o[k] = u8(((float)u4[4 * s + k] / 65535.0f) * 255.0f);
+                } else if (mode == RT_GATHER_ID) {
+                    const unsigned id = u4[4 * s + 3];
+                    o[0] = (uint8_t)((id * 37) % 256);
+                    o[1] = (uint8_t)((id * 57) % 256);
+                    o[2] = (uint8_t)((id * 97) % 256);
+                } else {
+                    const float lo[3] = {b.lo.x, b.lo.y, b.lo.z}, hi[3] = {b.hi.x, b.hi.y, b.hi.z};
+                    for (int k = 0; k < 3; ++k) o[k] = u8((f4[4 * s + k] - lo[k]) / (hi[k] - lo[k]) * 255.0f);
+                }
+            }
+        }
+        return RT_OK;
+    });
+}
+
 int rt_save_image(rt_renderer* r, int which, const char* path) {
     if (!r || !path) return fail(RT_ERR_INVALID_ARG, "null argument");
     if (which < RT_IMAGE_PNG_COLOR || which > RT_IMAGE_PFM_DENOISED) return fail(RT_ERR_INVALID_ARG, "unknown image kind");

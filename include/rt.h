@@ -59,6 +59,13 @@ extern "C" {
 #define RT_ACCUM_INDIRECT 1
 #define RT_ACCUM_MEAN 2 /* (direct + indirect) / spp */
 
+#define RT_GATHER_COLOR 0     /* renderTargetMode 0, accumulated */
+#define RT_GATHER_DEPTH 1
+#define RT_GATHER_NORMAL 2
+#define RT_GATHER_ID 3
+#define RT_GATHER_POSITION 4
+#define RT_GATHER_DENOISED 5  /* renderTargetMode 0 with denoising on */
+
 #define RT_IMAGE_PNG_COLOR 0
 #define RT_IMAGE_PNG_DENOISED 1
 #define RT_IMAGE_PFM_MEAN 2
@@ -191,6 +198,10 @@ int rt_reset_stats(rt_renderer* r);
 int rt_read_accum(rt_renderer* r, int which, float* rgba_out);
 /* Gamma-2.2 8-bit colour, top row first (GatherKernel colour mode). */
 int rt_read_rgba8(rt_renderer* r, uint8_t* rgba_out);
+/* GatherKernel's render-target modes (renderer.cuh:956-1029) as RGBA8, top row first:
+   RT_GATHER_COLOR = rt_read_rgba8; DEPTH, NORMAL, ID, POSITION = the debug views of the last
+   real-time frame's G-buffer; DENOISED = the colour mode with denoising on (the last frame). */
+int rt_read_gather(rt_renderer* r, int mode, uint8_t* rgba_out);
 /* Write an image file (UI::SaveScreenshot, Src/Utils/ui.cpp:370-380, writes the gathered colour as
    an RGBA PNG): RT_IMAGE_PNG_COLOR = rt_read_rgba8's pixels, RT_IMAGE_PNG_DENOISED = the same gather
    of the last real-time frame's denoised colour (GatherKernel's denoise branch), RT_IMAGE_PFM_MEAN /
