@@ -304,6 +304,18 @@ int rt_read_rgba8(rt_renderer* r, uint8_t* out) {
     });
 }
 
+int rt_read_denoised(rt_renderer* r, float* rgba_out) {
+    if (!r || !rgba_out) return fail(RT_ERR_INVALID_ARG, "null argument");
+    return rt_read_frame(r, RT_FRAME_DENOISED, rgba_out, (size_t)r->r->width() * r->r->height() * 16);
+}
+
+int rt_read_gbuffer(rt_renderer* r, int channel, void* out) {
+    if (!r || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
+    if (channel < 0 || channel > 3) return fail(RT_ERR_INVALID_ARG, "G-buffer channel must be 0..3");
+    const int which = RT_FRAME_G_POSITION_TRI + channel;
+    return rt_read_frame(r, which, out, rt::frame_buffer_bytes(which, r->r->width(), r->r->height()));
+}
+
 int rt_read_gather(rt_renderer* r, int mode, uint8_t* out) {
     if (!r || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
     if (mode < RT_GATHER_COLOR || mode > RT_GATHER_DENOISED) return fail(RT_ERR_INVALID_ARG, "unknown gather mode");

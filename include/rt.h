@@ -224,6 +224,12 @@ int rt_reset_history(rt_renderer* r);
 /* Copy an RT_FRAME_* buffer of the last frame; bytes must be its exact size
    (width*height times 16 for float4, 8 for float2 / uint16x4, 4 for int32). */
 int rt_read_frame(rt_renderer* r, int which, void* out, size_t bytes);
+/* The last frame's denoised colour, float RGBA, width*height*4 (rt_read_frame(RT_FRAME_DENOISED)). */
+int rt_read_denoised(rt_renderer* r, float* rgba_out);
+/* One G-buffer plane of the last frame (channel 0..3 = RT_FRAME_G_POSITION_TRI, _G_NORMAL_MAT,
+   _G_BARY_MESH, _G_MOTION_DEPTH), the reference's four render targets (renderer.cuh:1356-1503),
+   16 or 8 bytes per pixel as listed with those ids. */
+int rt_read_gbuffer(rt_renderer* r, int channel, void* out);
 
 /* ---- pools: many jobs over resident scenes, one wavefront engine ---- */
 typedef struct rt_pool rt_pool;

@@ -91,3 +91,22 @@ def test_gather_modes(scene_dir):
     assert np.abs(r.read_gather(rt.RT_GATHER_DENOISED)[..., :3].astype(int) - want.astype(int)).max() <= 1
     with pytest.raises(rt.RtError):
         r.read_gather(9)
+
+
+def test_read_denoised_and_gbuffer_entry_points(scene_dir):
+    import ctypes
+
+    r = rt.Renderer(rt.Scene(scene_dir["cornell_spheres_rt"]))
+    r.render_frame(1)
+    lib = rt.lib()
+    den = np.zeros((r.height, r.width, 4), np.float32)
+    assert lib.rt_read_denoised(r._h, den.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) == 0
+    assert np.array_equal(den.reshape(-1), r.read_frame(rt.RT_FRAME_DENOISED).reshape(-1))
+    planes = [rt.RT_FRAME_G_POSITION_TRI, rt.RT_FRAME_G_NORMAL_MAT, rt.RT_FRAME_G_BARY_MESH,
+              rt.RT_FRAME_G_MOTION_DEPTH]
+    for c, which in enumerate(planes):
+        want = r.read_frame(which)
+        got = np.zeros_like(want)
+        assert lib.rt_read_gbuffer(r._h, c, got.ctypes.data_as(ctypes.c_void_p)) == 0
+        assert np.array_equal(got, want)
+    assert lib.rt_read_gbuffer(r._h, 4, den.ctypes.data_as(ctypes.c_void_p)) != 0
