@@ -225,6 +225,103 @@ struct BlockOps {
     }
 };
 
+// A node of at most 64 triangles in one wave, one triangle per lane: the
+// same steps as below with every element read once and kept in registers,
+// ranks from ballots and the two-pointer pairing through LDS.
+__device__ void node_in_registers(const Elems& cur, const Elems& nxt, int32_t* __restrict__ order, const Nodes& nodes,
+                                  int node, int4 rg, int node_cap, int* __restrict__ counters, uint32_t* s_cnt,
+                                  uint32_t (*s_lo)[3], uint32_t (*s_hi)[3]) {
+    __shared__ int s_fail[kWave], s_pass[kWave];
+    const int lane = __lane_id();
+    const int start = rg.x, end = rg.y, n = end - start;
+    const int p = start + lane;
+    const bool valid = lane < n;
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+    float2 c = make_float2(0.f, 0.f);
+    if (valid) {
+        a = cur.lo_cx[p];
+        b = cur.hi_cy[p];
+        c = cur.cz_id[p];
+    }
+    const float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z}, ce[3] = {a.w, b.w, c.x};
+    float box[6], cb_lo[3], cb_hi[3];
+    for (int k = 0; k < 3; ++k) {
+        uint64_t kl = valid ? ((uint64_t)ord(lo[k]) << 32 | (uint32_t)p) : ~0ull;
+        uint64_t kh = valid ? ((uint64_t)ord(hi[k]) << 32 | (0xffffffffu - (uint32_t)p)) : 0ull;
+        int wl = (int)(uint32_t)wave_min_u64(kl) - start, wh = (int)(0xffffffffu - (uint32_t)wave_max_u64(kh)) - start;
+        box[k] = __shfl(lo[k], wl, kWave);
+        box[3 + k] = __shfl(hi[k], wh, kWave);
+        cb_lo[k] = wave_min_f(valid ? ce[k] : kFloatMax);
+        cb_hi[k] = wave_max_f(valid ? ce[k] : -kFloatMax);
+    }
+    if (lane == 0) {
+        nodes.box_lo[node] = make_float4(box[0], box[1], box[2], 0.f);
+        nodes.box_hi[node] = make_float4(box[3], box[4], box[5], 0.f);
+    }
+    if (n <= kMaxLeaf) {
+        if (valid) order[p] = __float_as_int(c.y);
+        if (lane == 0) nodes.range[node].w = 1;
+        return;
+    }
+    const float ex = cb_hi[0] - cb_lo[0], ey = cb_hi[1] - cb_lo[1], ez = cb_hi[2] - cb_lo[2];
+    int axis = 0;
+    float m = ex;
+    if (ey > m) { m = ey; axis = 1; }
+    if (ez > m) axis = 2;
+    const int bk = bucket_of(ce[axis], cb_lo[axis], cb_hi[axis]);
+    if (lane < kBuckets) {
+        s_cnt[lane] = 0u;
+        for (int k = 0; k < 3; ++k) {
+            s_lo[lane][k] = ord(kFloatMax);
+            s_hi[lane][k] = ord(-kFloatMax);
+        }
+    }
+    __syncthreads();
+    if (valid) {
+        atomicAdd(&s_cnt[bk], 1u);
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&s_lo[bk][k], ord(lo[k]));
+            atomicMax(&s_hi[bk][k], ord(hi[k]));
+        }
+    }
+    __syncthreads();
+    const int best = choose_split(s_cnt, s_lo, s_hi, box);
+    int L = 0;
+    for (int j = 0; j <= best; ++j) L += (int)s_cnt[j];
+    const int zone = start + L;
+    const bool fallback = best < 0 || zone <= start || zone >= end;
+    const int mid = fallback ? (start + end) / 2 : zone;
+    int dst = p;
+    if (!fallback) {
+        const bool pass = valid && bk <= best;
+        const bool fl = valid && p < zone && !pass, pr = valid && p >= zone && pass;
+        const uint64_t mf = __ballot(fl), mp = __ballot(pr);
+        const int kf = lanes_below(mf);
+        const int kp = __popcll(mp & ~((2ull << lane) - 1ull));  // passing lanes above this one
+        if (fl) s_fail[kf] = p;
+        if (pr) s_pass[kp] = p;
+        __syncthreads();
+        if (fl) dst = s_pass[kf];
+        if (pr) dst = s_fail[kp];
+    }
+    if (valid) {
+        nxt.lo_cx[dst] = a;
+        nxt.hi_cy[dst] = b;
+        nxt.cz_id[dst] = c;
+    }
+    if (lane == 0) {
+        int id = atomicAdd(&counters[0], 2);
+        if (id + 2 > node_cap) {
+            atomicOr(&counters[1], 2);
+            return;
+        }
+        nodes.range[id] = make_int4(start, mid, rg.z + 1, 0);
+        nodes.range[id + 1] = make_int4(mid, end, rg.z + 1, 0);
+        nodes.kids[node] = make_int2(id, id + 1);
+        nodes.range[node].w = fallback ? 2 : 0;
+    }
+}
+
 // One node per workgroup of WAVES waves; `list` names the nodes of this launch.
 template <int WAVES>
 __global__ __launch_bounds__(kWave * WAVES) void k_bvh_node(Elems cur, Elems nxt, int32_t* __restrict__ order,
@@ -242,6 +339,10 @@ __global__ __launch_bounds__(kWave * WAVES) void k_bvh_node(Elems cur, Elems nxt
     const int t = threadIdx.x;
     const int4 rg = nodes.range[node];
     const int start = rg.x, end = rg.y, n = end - start;
+    if (WAVES == 1 && n <= kWave) {
+        node_in_registers(cur, nxt, order, nodes, node, rg, node_cap, counters, s_cnt, s_lo, s_hi);
+        return;
+    }
 
     // Node bounds: first position among equal extremes.
     uint64_t klo[3], khi[3];
