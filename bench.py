@@ -278,6 +278,7 @@ def bvh_build_bench(path, gpu_scene=None):
     gi, hi = warm.info(), host.info()
     gn, go = warm.bvh()
     hn, ho = host.bvh()
+    refit = refit_bench(warm, host)
     return {
         "triangles": gi.num_triangles,
         "nodes": gi.num_bvh_nodes,
@@ -287,6 +288,36 @@ def bvh_build_bench(path, gpu_scene=None):
                                                         gi.bvh_gpu_phase_ms)},
         "gpu_mtris_s": round(gi.num_triangles / max(1e-9, gi.bvh_build_ms) / 1e3, 1),
         "identical": bool(np.array_equal(gn, hn) and np.array_equal(go, ho)),
+        "refit": refit,
+    }
+
+
+def refit_bench(gpu_scene, host_scene, reps=5):
+    """Refit of the same tree to moved vertices (every height scaled by 1.1,
+    alternating with the loaded positions): wall time of rt_scene_refit with
+    the scene resident on the GPU (median of `reps`) and on the host, and
+    whether the two refits give the same nodes."""
+    gpu_scene.trace(np.array([[0, 0, 0, 0, 0, 0, 1, 1e8]], np.float32))  # make it resident
+    pos, _, _, _ = gpu_scene.triangles()
+    moved = pos.copy()
+    moved[:, 1::3] *= np.float32(1.1)
+    gpu_ms, phases = [], []
+    for k in range(reps):
+        gpu_scene.refit(moved if k % 2 == 0 else pos)
+        i = gpu_scene.info()
+        gpu_ms.append(i.refit_ms)
+        phases.append(list(i.refit_phase_ms))
+    gpu_scene.refit(moved)
+    host_scene.refit(moved)
+    hi = host_scene.info()
+    k = int(np.argsort(gpu_ms)[len(gpu_ms) // 2])
+    return {
+        "gpu_ms": round(gpu_ms[k], 2),
+        "gpu_phase_ms": {n: round(v, 2) for n, v in zip(("host_arrays", "uploads", "node_boxes"), phases[k])},
+        "host_ms": round(hi.refit_ms, 2),
+        "host_phase_ms": {n: round(v, 2) for n, v in zip(("host_arrays", "uploads", "node_boxes"),
+                                                          hi.refit_phase_ms)},
+        "identical": bool(np.array_equal(gpu_scene.bvh()[0], host_scene.bvh()[0])),
     }
 
 

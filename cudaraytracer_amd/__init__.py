@@ -68,7 +68,7 @@ EXPORTS = (
     "rt_render_params_default", "rt_scene_load", "rt_scene_info_get", "rt_scene_destroy",
     "rt_renderer_create", "rt_renderer_destroy", "rt_render", "rt_render_counted", "rt_synchronize",
     "rt_reset", "rt_reset_stats", "rt_read_accum", "rt_read_rgba8", "rt_stats", "rt_last_error",
-    "rt_scene_get_triangles", "rt_scene_get_meshes", "rt_scene_get_bvh", "rt_scene_get_texture",
+    "rt_scene_refit", "rt_scene_get_triangles", "rt_scene_get_meshes", "rt_scene_get_bvh", "rt_scene_get_texture",
     "rt_texture_lookup", "rt_trace_rays", "rt_device_math", "rt_save_image", "rt_read_gather", "rt_read_denoised", "rt_read_gbuffer",
     "rt_denoise_params_default", "rt_get_camera", "rt_set_camera", "rt_render_frame", "rt_reset_history",
     "rt_read_frame", "rt_pool_create", "rt_pool_destroy", "rt_pool_add_scene", "rt_pool_submit", "rt_pool_step",
@@ -95,7 +95,8 @@ class SceneInfo(ctypes.Structure):
         ("camera_position", ctypes.c_float * 3), ("camera_left_down", ctypes.c_float * 3),
         ("camera_du", ctypes.c_float * 3), ("camera_dv", ctypes.c_float * 3), ("num_textures", ctypes.c_int),
         ("bvh_build_ms", ctypes.c_float), ("bvh_on_gpu", ctypes.c_int), ("bvh_gpu_phase_ms", ctypes.c_float * 4),
-        ("load_phase_ms", ctypes.c_float * 3),
+        ("load_phase_ms", ctypes.c_float * 3), ("refit_ms", ctypes.c_float), ("refit_on_gpu", ctypes.c_int),
+        ("refit_phase_ms", ctypes.c_float * 3),
     ]
 
 
@@ -224,6 +225,7 @@ def lib() -> ctypes.CDLL:
     L.rt_read_accum.argtypes = [vp, ip, fp]
     L.rt_read_rgba8.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8)]
     L.rt_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.rt_scene_refit.argtypes = [vp, fp, fp, ctypes.c_int64]
     L.rt_scene_get_triangles.argtypes = [vp, fp, fp, fp, ctypes.POINTER(ctypes.c_int32)]
     L.rt_scene_get_meshes.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), fp]
     L.rt_scene_get_bvh.argtypes = [vp, ip, vp, ctypes.POINTER(ctypes.c_int32)]
@@ -322,6 +324,22 @@ class Scene:
         mesh = np.zeros(n, np.int32)
         _check(lib().rt_scene_get_triangles(self._h, _fptr(pos), _fptr(nrm), _fptr(uv), _iptr(mesh)))
         return pos, nrm, uv, mesh
+
+    def refit(self, pos: np.ndarray, nrm: Optional[np.ndarray] = None) -> None:
+        """Move the vertices (pos: (n, 9) float32 in scene order, as triangles()
+        returns them; nrm likewise or None to keep the shading normals) and
+        refit the BVH in place, on the GPU when the scene is resident there."""
+        pos = np.ascontiguousarray(pos, np.float32)
+        n = pos.size // 9
+        if pos.size != 9 * n:
+            raise ValueError("pos must hold 9 floats per triangle")
+        npt = None
+        if nrm is not None:
+            nrm = np.ascontiguousarray(nrm, np.float32)
+            if nrm.size != pos.size:
+                raise ValueError("nrm must match pos")
+            npt = _fptr(nrm)
+        _check(lib().rt_scene_refit(self._h, _fptr(pos), npt, n))
 
     def meshes(self):
         m = self.info().num_meshes

@@ -4,6 +4,7 @@
 #include "bvh.h"
 
 #include <algorithm>
+#include <stdexcept>
 #include <utility>
 
 namespace rt {
@@ -223,6 +224,40 @@ void build_bvh(const float* pos, int num_tris, BvhBuild& out) {
     }
     out.root_ref = ref_of(0);
     out.max_stack = out.max_depth + 1;
+}
+
+void refit_bvh(const float* pos, int num_tris, BvhBuild& b) {
+    const int nn = (int)b.nodes.size();
+    if (nn == 0) return;
+    if ((int)b.order.size() != num_tris) throw std::runtime_error("refit: triangle count differs from the tree's");
+    // Children follow their parent in DFS pre-order, so one reverse sweep
+    // sees every child before its parent.
+    for (int i = nn - 1; i >= 0; --i) {
+        LinearNode& n = b.nodes[i];
+        Box box = empty_box();
+        if (n.tri_index >= 0) {
+            for (int k = 0; k < n.second; ++k) {
+                const float* p = pos + 9 * (size_t)b.order[(size_t)n.tri_index + k];
+                Box t;
+                t.lo = v3(smin(p[0], smin(p[3], p[6])), smin(p[1], smin(p[4], p[7])), smin(p[2], smin(p[5], p[8])));
+                t.hi = v3(smax(p[0], smax(p[3], p[6])), smax(p[1], smax(p[4], p[7])), smax(p[2], smax(p[5], p[8])));
+                box = expand(box, t);
+            }
+        } else {
+            box = expand(expand(box, node_box(b.nodes[i + 1])), node_box(b.nodes[n.second]));
+        }
+        n.lo[0] = box.lo.x; n.lo[1] = box.lo.y; n.lo[2] = box.lo.z;
+        n.hi[0] = box.hi.x; n.hi[1] = box.hi.y; n.hi[2] = box.hi.z;
+    }
+    for (PackedNode& p : b.packed) {
+        const int i = p.w3.z;
+        const LinearNode& L = b.nodes[i + 1];
+        const LinearNode& R = b.nodes[b.nodes[i].second];
+        p.w0 = make_float4(L.lo[0], L.lo[1], L.lo[2], L.hi[0]);
+        p.w1 = make_float4(L.hi[1], L.hi[2], R.lo[0], R.lo[1]);
+        p.w2 = make_float4(R.lo[2], R.hi[0], R.hi[1], R.hi[2]);
+    }
+    b.bounds = node_box(b.nodes[0]);
 }
 
 namespace {

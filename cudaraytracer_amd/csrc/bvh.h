@@ -64,6 +64,15 @@ void build_bvh(const float* pos, int num_tris, BvhBuild& out);
 // The same build on the current GPU (bvh_gpu.hip): an identical BvhBuild.
 void build_bvh_gpu(const float* pos, int num_tris, BvhBuild& out);
 
+// Refit: new vertex positions (9 floats per triangle, scene order) for the
+// same triangles over the same tree.  A leaf's box is the union of its
+// triangles' boxes in leaf order, starting from the empty box (as the build
+// forms a node's box); an interior node's box is its left child's box
+// expanded by its right child's.  Node boxes, packed boxes and `bounds` are
+// rewritten; the topology, the triangle order and the numbering are kept.
+// The reference has no refit: it rebuilds the whole scene.
+void refit_bvh(const float* pos, int num_tris, BvhBuild& b);
+
 // Rebuild the reference LinearNode array from the packed form (test hook).
 void unpack_bvh(const std::vector<PackedNode>& packed, uint32_t root_ref, const Box& root_box,
                 std::vector<LinearNode>& out);

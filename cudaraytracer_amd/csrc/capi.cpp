@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -17,6 +18,10 @@
 struct rt_scene {
     rt::HostScene host;
     std::unique_ptr<rt::GpuScene> gpu;  // created on first GPU use
+    bool host_bvh_stale = false;        // a device refit left the host node boxes behind
+    double refit_ms = 0.0;
+    double refit_phase_ms[3] = {0, 0, 0};
+    bool refit_on_gpu = false;
 };
 
 struct rt_renderer {
@@ -193,6 +198,9 @@ int rt_scene_info_get(const rt_scene* s, rt_scene_info* o) {
     o->bvh_on_gpu = h.bvh_on_gpu ? 1 : 0;
     for (int k = 0; k < 4; ++k) o->bvh_gpu_phase_ms[k] = (float)h.bvh.gpu_phase_ms[k];
     for (int k = 0; k < 3; ++k) o->load_phase_ms[k] = (float)h.load_phase_ms[k];
+    o->refit_ms = (float)s->refit_ms;
+    o->refit_on_gpu = s->refit_on_gpu ? 1 : 0;
+    for (int k = 0; k < 3; ++k) o->refit_phase_ms[k] = (float)s->refit_phase_ms[k];
     const rt::Box& b = h.bvh.bounds;
     float lo[3] = {b.lo.x, b.lo.y, b.lo.z}, hi[3] = {b.hi.x, b.hi.y, b.hi.z};
     std::memcpy(o->bounds_lo, lo, sizeof(lo));
@@ -210,6 +218,30 @@ int rt_scene_info_get(const rt_scene* s, rt_scene_info* o) {
 }
 
 void rt_scene_destroy(rt_scene* s) { delete s; }
+
+int rt_scene_refit(rt_scene* s, const float* pos, const float* nrm, int64_t num_triangles) {
+    if (!s || !pos) return fail(RT_ERR_INVALID_ARG, "null argument");
+    if (num_triangles != (int64_t)s->host.num_tris())
+        return fail(RT_ERR_INVALID_ARG, "rt_scene_refit: " + std::to_string(num_triangles) +
+                                            " triangles given, the scene has " + std::to_string(s->host.num_tris()));
+    return guard([&]() {
+        auto t0 = std::chrono::steady_clock::now();
+        rt::set_scene_geometry(s->host, pos, nrm);
+        auto t1 = std::chrono::steady_clock::now();
+        s->refit_phase_ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        if (s->gpu) {
+            s->gpu->refit(s->host, nrm, s->refit_phase_ms + 1);
+            s->host_bvh_stale = true;
+        } else {
+            rt::refit_bvh(s->host.pos.data(), s->host.num_tris(), s->host.bvh);
+            s->refit_phase_ms[1] = 0.0;
+            s->refit_phase_ms[2] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+        }
+        s->refit_on_gpu = (bool)s->gpu;
+        s->refit_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return RT_OK;
+    });
+}
 
 int rt_renderer_create(rt_scene* s, const rt_render_params* p, rt_renderer** out) {
     if (!s || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
@@ -699,6 +731,11 @@ int rt_scene_get_meshes(const rt_scene* s, int32_t* mesh4, float* area) {
 int rt_scene_get_bvh(const rt_scene* s, int from_packed, void* nodes_out, int32_t* order_out) {
     if (!s) return fail(RT_ERR_INVALID_ARG, "null argument");
     return guard([&]() {
+        if (s->host_bvh_stale) {  // bring the device refit's boxes back to the host tree
+            rt_scene* m = const_cast<rt_scene*>(s);
+            m->gpu->read_bvh(m->host.bvh);
+            m->host_bvh_stale = false;
+        }
         const rt::BvhBuild& b = s->host.bvh;
         if (nodes_out) {
             if (from_packed) {

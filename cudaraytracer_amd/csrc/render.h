@@ -84,6 +84,15 @@ public:
     int max_stack() const;  // traversal stack entries the tree needs (depth + 1)
     bool has_class(int shading_class) const;
     size_t device_bytes() const;
+    // Device side of a refit, after set_scene_geometry has updated `host`:
+    // rewrites the triangle records (shading normals only when `nrm` is not
+    // null), the light tables and every node box in place, then sets the
+    // scene bounds of `host` and of dev() to the new root box.  The host
+    // tree's other boxes are left stale; read_bvh brings them back.  Waits
+    // for all work on the device first.  phase_ms: uploads, kernels.
+    void refit(HostScene& host, const float* nrm, double phase_ms[2]);
+    // Copy the device node boxes into b.packed and b.nodes.
+    void read_bvh(BvhBuild& b) const;
     struct Impl;
 
 private:

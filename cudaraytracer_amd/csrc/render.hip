@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -628,9 +629,124 @@ __global__ void k_pack_triangles(int n, const int32_t* __restrict__ order, const
     a[7] = make_float4(i_as_f(m), i_as_f(mr.y), i_as_f(mr.z), i_as_f(mr.w));
 }
 
+// ------------------------------------------------------------------ refit
+// New positions for the same triangles over the same tree.  Box unions keep
+// std::min / std::max tie rules (the first operand wins a tie, which decides
+// between -0 and +0) so the device refit writes the host refit's bits.
+__device__ __forceinline__ float rf_min(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float rf_max(float a, float b) { return (a < b) ? b : a; }
+
+// Position-dependent words of the triangle records: the intersection record
+// and the vertices and geometric normal of the attribute line (normalised
+// cross product, as the triangle constructor on the host forms it); shading
+// normals too when `nrm` is given.  Indices, uvs and ids are kept.
+__global__ void k_refit_triangles(int n, const int32_t* __restrict__ order, const float* __restrict__ pos,
+                                  const float* __restrict__ nrm, float4* __restrict__ isect,
+                                  float4* __restrict__ attr) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int t = order[k];
+    const float* p = pos + 9 * (size_t)t;
+    const vec3 a = v3(p[0], p[1], p[2]), b = v3(p[3], p[4], p[5]), c = v3(p[6], p[7], p[8]);
+    const vec3 g = normalize(cross(b - a, c - a));
+    isect[3 * (size_t)k + 0] = make_float4(p[0], p[1], p[2], 0.f);
+    isect[3 * (size_t)k + 1] = make_float4(p[3] - p[0], p[4] - p[1], p[5] - p[2], 0.f);
+    isect[3 * (size_t)k + 2] = make_float4(p[6] - p[0], p[7] - p[1], p[8] - p[2], 0.f);
+    float4* A = attr + kAttrStride * (size_t)k;
+    const float4 a0 = A[0], a3 = A[3];
+    A[0] = make_float4(p[0], p[1], p[2], a0.w);
+    A[1] = make_float4(p[3], p[4], p[5], g.x);
+    A[2] = make_float4(p[6], p[7], p[8], g.y);
+    if (nrm) {
+        const float* q = nrm + 9 * (size_t)t;
+        const float4 a4 = A[4], a5 = A[5];
+        A[3] = make_float4(q[0], q[1], q[2], g.z);
+        A[4] = make_float4(q[3], q[4], q[5], a4.w);
+        A[5] = make_float4(q[6], q[7], q[8], a5.w);
+    } else {
+        A[3] = make_float4(a3.x, a3.y, a3.z, g.z);
+    }
+}
+
+// Parent link of every interior node but the root: (parent << 1) | side.
+__global__ void k_refit_parents(int ni, const PackedNode* __restrict__ nodes, int* __restrict__ parent) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= ni) return;
+    const int4 w3 = nodes[p].w3;
+    if (!((uint32_t)w3.x & kLeafBit)) parent[w3.x] = p << 1;
+    if (!((uint32_t)w3.y & kLeafBit)) parent[w3.y] = (p << 1) | 1;
+}
+
+// A child box is 6 consecutive floats of its parent's line: side 0 = words
+// 0..5 (left lo.xyz, hi.xyz), side 1 = words 6..11.  Slots are written and
+// read by different workgroups within one launch, so they go through
+// device-scope atomic loads and stores ordered by fences.
+__device__ __forceinline__ void slot_put(PackedNode* nd, int side, const float* b) {
+    uint32_t* u = reinterpret_cast<uint32_t*>(nd) + 6 * side;
+    for (int i = 0; i < 6; ++i) __hip_atomic_store(u + i, __float_as_uint(b[i]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void slot_get(PackedNode* nd, int side, float* b) {
+    uint32_t* u = reinterpret_cast<uint32_t*>(nd) + 6 * side;
+    for (int i = 0; i < 6; ++i) b[i] = __uint_as_float(__hip_atomic_load(u + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Bottom-up box refit, one thread per interior node.  A thread writes its
+// node's leaf-child boxes (union of the leaf's triangle boxes in leaf order,
+// from the empty box) and adds their number to the node's counter; whoever
+// brings a counter to 2 owns the node: it forms the node's box (left child's
+// box expanded by the right's), writes it into the parent's slot and adds 1
+// to the parent's counter, and so on up to the root.  No thread waits on
+// another, so every thread ends after at most depth steps.
+__global__ void k_refit_nodes(int ni, PackedNode* nodes, const int* __restrict__ parent, int* cnt,
+                              const int32_t* __restrict__ order, const float* __restrict__ pos) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= ni) return;
+    const int4 w3 = nodes[p].w3;  // child references: the refit never writes them
+    const float fmax_ = (float)3.40282e+038;
+    int leaves = 0;
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t ref = (uint32_t)(side ? w3.y : w3.x);
+        if (!(ref & kLeafBit)) continue;
+        const uint32_t first = (ref & ~kLeafBit) >> 3, count = ref & 7u;
+        float b[6] = {fmax_, fmax_, fmax_, -fmax_, -fmax_, -fmax_};
+        for (uint32_t k = 0; k < count; ++k) {
+            const float* q = pos + 9 * (size_t)order[first + k];
+            for (int a = 0; a < 3; ++a) {
+                b[a] = rf_min(b[a], rf_min(q[a], rf_min(q[3 + a], q[6 + a])));
+                b[3 + a] = rf_max(b[3 + a], rf_max(q[a], rf_max(q[3 + a], q[6 + a])));
+            }
+        }
+        slot_put(nodes + p, side, b);
+        ++leaves;
+    }
+    if (leaves == 0) return;
+    __threadfence();
+    if (atomicAdd(&cnt[p], leaves) + leaves < 2) return;
+    int node = p;
+    while (true) {
+        __threadfence();
+        float l[6], r[6], u[6];
+        slot_get(nodes + node, 0, l);
+        slot_get(nodes + node, 1, r);
+        for (int a = 0; a < 3; ++a) {
+            u[a] = rf_min(l[a], r[a]);
+            u[3 + a] = rf_max(l[3 + a], r[3 + a]);
+        }
+        const int pr = parent[node];
+        if (pr < 0) return;  // the root: its box is read by the host from its two slots
+        slot_put(nodes + (pr >> 1), pr & 1, u);
+        __threadfence();
+        if (atomicAdd(&cnt[pr >> 1], 1) == 0) return;  // the sibling is not done yet
+        node = pr >> 1;
+    }
+}
+
 struct GpuScene::Impl {
     DevBuf nodes, tri_isect, tri_attr, materials, lights, light_tris, env_rgb, env_row, env_marg, env_alias_row,
         env_alias_marg, tex_texels, tex_info;
+    DevBuf order;             // BVH slot -> scene triangle, kept for refits
+    DevBuf parent, refit_cnt; // per interior node, allocated by the first refit
+    int num_interior = 0;     // packed nodes of the tree (0: the root is a leaf)
     DevScene dev{};
     int max_stack = 1;
     bool has_class[kNumClasses] = {true, false, false, false, false};
@@ -642,6 +758,27 @@ static void upload(DevBuf& b, const void* src, size_t bytes) {
     if (bytes) HIP_OK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
 }
 
+// Area-light records and their triangles (v0, v1, v2, geometric normal) in
+// light order; one dummy entry each when there are none, so pointers are valid.
+static void light_tables(const HostScene& sc, std::vector<LightRec>& lights, std::vector<float4>& ltris) {
+    lights = sc.lights;
+    ltris.clear();
+    for (const auto& L : sc.lights) {
+        if (L.type != kAreaLight || L.mesh < 0) continue;
+        const MeshRec& m = sc.meshes[L.mesh];
+        for (int t = m.start; t < m.start + m.count; ++t) {
+            const float* p = &sc.pos[9 * (size_t)t];
+            const float* g = &sc.gnormal[3 * (size_t)t];
+            ltris.push_back(make_float4(p[0], p[1], p[2], 0.f));
+            ltris.push_back(make_float4(p[3], p[4], p[5], 0.f));
+            ltris.push_back(make_float4(p[6], p[7], p[8], 0.f));
+            ltris.push_back(make_float4(g[0], g[1], g[2], 0.f));
+        }
+    }
+    if (lights.empty()) lights.resize(1);
+    if (ltris.empty()) ltris.resize(4);
+}
+
 GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
     Impl& I = *impl_;
     HIP_OK(hipGetDevice(&I.device));
@@ -651,6 +788,7 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
     I.max_stack = std::max(1, bvh.max_stack);
     // Packed nodes; a leaf-only tree gets one dummy line so the pointer is valid.
     std::vector<PackedNode> nodes = bvh.packed;
+    I.num_interior = (int)nodes.size();
     if (nodes.empty()) nodes.resize(1);
     upload(I.nodes, nodes.data(), nodes.size() * sizeof(PackedNode));
     // Triangle lines in BVH order, packed on the GPU from the scene-order arrays.
@@ -671,18 +809,18 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
         mesh_tab[m] = make_int4(mr.start, mr.material, mr.light, cls);
     }
     {
-        DevBuf dpos, dnrm, duv, dgn, dmesh, dorder, dtab;
+        DevBuf dpos, dnrm, duv, dgn, dmesh, dtab;
         upload(dpos, sc.pos.data(), sc.pos.size() * sizeof(float));
         upload(dnrm, sc.nrm.data(), sc.nrm.size() * sizeof(float));
         upload(duv, sc.uv.data(), sc.uv.size() * sizeof(float));
         upload(dgn, sc.gnormal.data(), sc.gnormal.size() * sizeof(float));
         upload(dmesh, sc.mesh.data(), sc.mesh.size() * sizeof(int32_t));
-        upload(dorder, bvh.order.data(), bvh.order.size() * sizeof(int32_t));
+        upload(I.order, bvh.order.data(), bvh.order.size() * sizeof(int32_t));
         upload(dtab, mesh_tab.data(), mesh_tab.size() * sizeof(int4));
         I.tri_isect.alloc(3 * (size_t)n * sizeof(float4));
         I.tri_attr.alloc(kAttrStride * (size_t)n * sizeof(float4));
         if (n > 0)
-            k_pack_triangles<<<(n + 255) / 256, 256>>>(n, dorder.as<int32_t>(), dpos.as<float>(), dnrm.as<float>(),
+            k_pack_triangles<<<(n + 255) / 256, 256>>>(n, I.order.as<int32_t>(), dpos.as<float>(), dnrm.as<float>(),
                                                    duv.as<float>(), dgn.as<float>(), dmesh.as<int32_t>(),
                                                    dtab.as<int4>(), I.tri_isect.as<float4>(), I.tri_attr.as<float4>());
         HIP_OK(hipGetLastError());
@@ -691,22 +829,9 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
     std::vector<MaterialRec> mats = sc.materials;
     if (mats.empty()) mats.resize(1);
     upload(I.materials, mats.data(), mats.size() * sizeof(MaterialRec));
-    std::vector<LightRec> lights = sc.lights;
+    std::vector<LightRec> lights;
     std::vector<float4> ltris;
-    for (const auto& L : sc.lights) {
-        if (L.type != kAreaLight || L.mesh < 0) continue;
-        const MeshRec& m = sc.meshes[L.mesh];
-        for (int t = m.start; t < m.start + m.count; ++t) {
-            const float* p = &sc.pos[9 * (size_t)t];
-            const float* g = &sc.gnormal[3 * (size_t)t];
-            ltris.push_back(make_float4(p[0], p[1], p[2], 0.f));
-            ltris.push_back(make_float4(p[3], p[4], p[5], 0.f));
-            ltris.push_back(make_float4(p[6], p[7], p[8], 0.f));
-            ltris.push_back(make_float4(g[0], g[1], g[2], 0.f));
-        }
-    }
-    if (lights.empty()) lights.resize(1);
-    if (ltris.empty()) ltris.resize(4);
+    light_tables(sc, lights, ltris);
     upload(I.lights, lights.data(), lights.size() * sizeof(LightRec));
     upload(I.light_tris, ltris.data(), ltris.size() * sizeof(float4));
 
@@ -770,6 +895,82 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
 
 GpuScene::~GpuScene() { delete impl_; }
 
+// The box a node's two child slots add up to: the left child's box expanded
+// by the right child's, with the device refit's tie rules.
+static Box union_of_slots(const PackedNode& p) {
+    auto mn = [](float a, float b) { return (b < a) ? b : a; };
+    auto mx = [](float a, float b) { return (a < b) ? b : a; };
+    Box b;
+    b.lo = v3(mn(p.w0.x, p.w1.z), mn(p.w0.y, p.w1.w), mn(p.w0.z, p.w2.x));
+    b.hi = v3(mx(p.w0.w, p.w2.y), mx(p.w1.x, p.w2.z), mx(p.w1.y, p.w2.w));
+    return b;
+}
+
+void GpuScene::refit(HostScene& host, const float* nrm, double phase_ms[2]) {
+    Impl& I = *impl_;
+    const int n = host.num_tris();
+    auto t0 = std::chrono::steady_clock::now();
+    auto lap = [&]() {
+        auto t = std::chrono::steady_clock::now();
+        double ms = std::chrono::duration<double, std::milli>(t - t0).count();
+        t0 = t;
+        return ms;
+    };
+    HIP_OK(hipSetDevice(I.device));
+    HIP_OK(hipDeviceSynchronize());  // renders in flight read the arrays rewritten below
+    DevBuf dpos, dnrm;
+    upload(dpos, host.pos.data(), host.pos.size() * sizeof(float));
+    if (nrm) upload(dnrm, host.nrm.data(), host.nrm.size() * sizeof(float));
+    std::vector<LightRec> lights;
+    std::vector<float4> ltris;
+    light_tables(host, lights, ltris);
+    if (lights.size() * sizeof(LightRec) != I.lights.bytes || ltris.size() * sizeof(float4) != I.light_tris.bytes)
+        throw std::runtime_error("refit: light tables changed size");
+    HIP_OK(hipMemcpy(I.lights.p, lights.data(), I.lights.bytes, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(I.light_tris.p, ltris.data(), I.light_tris.bytes, hipMemcpyHostToDevice));
+    phase_ms[0] = lap();
+    if (n > 0)
+        k_refit_triangles<<<(n + 255) / 256, 256>>>(n, I.order.as<int32_t>(), dpos.as<float>(),
+                                                    nrm ? dnrm.as<float>() : nullptr, I.tri_isect.as<float4>(),
+                                                    I.tri_attr.as<float4>());
+    HIP_OK(hipGetLastError());
+    const int ni = I.num_interior;
+    Box root;
+    if (ni > 0) {
+        if (!I.parent.p) {
+            I.parent.alloc((size_t)ni * sizeof(int));
+            I.refit_cnt.alloc((size_t)ni * sizeof(int));
+            HIP_OK(hipMemset(I.parent.p, 0xff, I.parent.bytes));  // the root keeps -1
+            k_refit_parents<<<(ni + 255) / 256, 256>>>(ni, I.nodes.as<PackedNode>(), I.parent.as<int>());
+            HIP_OK(hipGetLastError());
+        }
+        HIP_OK(hipMemset(I.refit_cnt.p, 0, I.refit_cnt.bytes));
+        k_refit_nodes<<<(ni + 255) / 256, 256>>>(ni, I.nodes.as<PackedNode>(), I.parent.as<int>(),
+                                                 I.refit_cnt.as<int>(), I.order.as<int32_t>(), dpos.as<float>());
+        HIP_OK(hipGetLastError());
+        PackedNode top;
+        HIP_OK(hipMemcpy(&top, I.nodes.p, sizeof(PackedNode), hipMemcpyDeviceToHost));  // waits for the kernels
+        root = union_of_slots(top);
+    } else {
+        HIP_OK(hipDeviceSynchronize());
+        refit_bvh(host.pos.data(), n, host.bvh);  // the whole tree is one leaf
+        root = host.bvh.bounds;
+    }
+    host.bvh.bounds = root;
+    I.dev.bounds_lo = root.lo;
+    I.dev.bounds_hi = root.hi;
+    phase_ms[1] = lap();
+}
+
+void GpuScene::read_bvh(BvhBuild& b) const {
+    const Impl& I = *impl_;
+    if (I.num_interior == 0) return;  // a one-leaf tree is refit on the host
+    if ((int)b.packed.size() != I.num_interior) throw std::runtime_error("read_bvh: node count differs");
+    HIP_OK(hipSetDevice(I.device));
+    HIP_OK(hipMemcpy(b.packed.data(), I.nodes.p, b.packed.size() * sizeof(PackedNode), hipMemcpyDeviceToHost));
+    unpack_bvh(b.packed, b.root_ref, b.bounds, b.nodes);
+}
+
 const DevScene& GpuScene::dev() const { return impl_->dev; }
 int GpuScene::max_stack() const { return impl_->max_stack; }
 bool GpuScene::has_class(int c) const { return impl_->has_class[c]; }
@@ -777,7 +978,8 @@ size_t GpuScene::device_bytes() const {
     const Impl& I = *impl_;
     return I.nodes.bytes + I.tri_isect.bytes + I.tri_attr.bytes + I.materials.bytes + I.lights.bytes +
            I.light_tris.bytes + I.env_rgb.bytes + I.env_row.bytes + I.env_marg.bytes + I.env_alias_row.bytes +
-           I.env_alias_marg.bytes + I.tex_texels.bytes + I.tex_info.bytes;
+           I.env_alias_marg.bytes + I.tex_texels.bytes + I.tex_info.bytes + I.order.bytes + I.parent.bytes +
+           I.refit_cnt.bytes;
 }
 
 // Row-conditional and marginal CDFs weighted by sin(theta) (reference:

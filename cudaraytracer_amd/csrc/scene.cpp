@@ -563,23 +563,39 @@ vec3 vec_of(const Json& a, const char* what) {
 }  // namespace
 
 void finalize_triangles(HostScene& sc) {
-    int n = sc.num_tris();
-    sc.gnormal.resize(3 * (size_t)n);
+    const size_t n = (size_t)sc.num_tris();
+    sc.gnormal.resize(3 * n);
     sc.area.resize(n);
-    for (int t = 0; t < n; ++t) {
-        const float* p = &sc.pos[9 * (size_t)t];
-        vec3 a = v3(p[0], p[1], p[2]), b = v3(p[3], p[4], p[5]), c = v3(p[6], p[7], p[8]);
-        vec3 nn = cross(b - a, c - a);
-        sc.area[t] = 0.5f * length(nn);
-        nn = normalize(nn);
-        sc.gnormal[3 * t] = nn.x;
-        sc.gnormal[3 * t + 1] = nn.y;
-        sc.gnormal[3 * t + 2] = nn.z;
-    }
+    // Per triangle independent, so blocks on several threads give the same bits.
+    const size_t block = 1 << 15;
+    parallel_for((n + block - 1) / block, [&](size_t bi) {
+        for (size_t t = bi * block; t < std::min(n, (bi + 1) * block); ++t) {
+            const float* p = &sc.pos[9 * t];
+            vec3 a = v3(p[0], p[1], p[2]), b = v3(p[3], p[4], p[5]), c = v3(p[6], p[7], p[8]);
+            vec3 nn = cross(b - a, c - a);
+            sc.area[t] = 0.5f * length(nn);
+            nn = normalize(nn);
+            sc.gnormal[3 * t] = nn.x;
+            sc.gnormal[3 * t + 1] = nn.y;
+            sc.gnormal[3 * t + 2] = nn.z;
+        }
+    });
+    // Mesh areas are sequential FP32 sums in scene order (reference: Src/Core/scene.cpp:515-519).
     for (auto& m : sc.meshes) {
         m.area = 0.f;
         for (int t = m.start; t < m.start + m.count; ++t) m.area += sc.area[t];
     }
+}
+
+void set_scene_geometry(HostScene& sc, const float* pos, const float* nrm) {
+    const size_t n9 = 9 * (size_t)sc.num_tris();
+    for (size_t i = 0; i < n9; ++i)
+        if (!std::isfinite(pos[i])) throw std::invalid_argument("non-finite vertex coordinate");
+    std::memcpy(sc.pos.data(), pos, n9 * sizeof(float));
+    if (nrm) std::memcpy(sc.nrm.data(), nrm, n9 * sizeof(float));
+    finalize_triangles(sc);
+    for (auto& l : sc.lights)
+        if (l.type == kAreaLight && l.mesh >= 0) l.area = sc.meshes[l.mesh].area;
 }
 
 void compute_camera(CameraRec& c) {

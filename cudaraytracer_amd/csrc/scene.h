@@ -105,6 +105,12 @@ constexpr int kGpuBuildMinTris = 1 << 16;
 void load_scene(const std::string& json_path, HostScene& sc, int bvh_builder = kBvhAuto);
 // Triangle constructor math: geometric normal and area (reference: Src/Core/triangle.cu:3-10).
 void finalize_triangles(HostScene& sc);
+// New vertex positions (9 floats per triangle, scene order) and, unless nrm is
+// null, shading normals for the same triangles: geometric normals, areas and
+// mesh and light areas follow.  The BVH and the camera are not touched (see
+// refit_bvh).  Throws std::invalid_argument for a non-finite coordinate,
+// before anything changes.
+void set_scene_geometry(HostScene& sc, const float* pos, const float* nrm);
 // Camera parameters from the scene bounds (reference: Src/Core/scene.cpp:646-667, camera.h:58-84).
 void setup_camera(HostScene& sc);
 // Pinhole basis and pixel steps from the orbit parameters (target, distance, theta, phi).

@@ -9,6 +9,7 @@
  * entry points below replace that surface one to one:
  *
  *   rt_scene_load        <- Scene::Scene (scene.cpp:50-57): JSON + OBJ, SAH BVH
+ *   rt_scene_refit       <- a new Scene for moved vertices (no refit in the reference)
  *   rt_renderer_create   <- Renderer::Renderer / CudaInit (renderer.cuh:1094-1100, 1309-1355)
  *   rt_render            <- RenderLoop frames without denoising (renderer.cuh:1144-1193),
  *                           one reference frame = one sample per pixel
@@ -126,6 +127,9 @@ typedef struct rt_scene_info {
     int bvh_on_gpu;         /* 1 when the GPU built it */
     float bvh_gpu_phase_ms[4]; /* GPU build: setup + upload, levels, download, host numbering */
     float load_phase_ms[3];    /* OBJ import (parse, transforms), triangle normals and areas, environment map */
+    float refit_ms;            /* wall time of the last rt_scene_refit (0: none yet) */
+    int refit_on_gpu;          /* 1 when the last refit ran on the GPU copy of the scene */
+    float refit_phase_ms[3];   /* host arrays (copy, normals, areas); uploads; node boxes (kernels or host) */
 } rt_scene_info;
 
 typedef struct rt_render_params {
@@ -182,6 +186,21 @@ void rt_denoise_params_default(rt_denoise_params* p);
 int rt_scene_load(const char* json_path, const rt_load_opts* opts, rt_scene** out);
 int rt_scene_info_get(const rt_scene* s, rt_scene_info* out);
 void rt_scene_destroy(rt_scene* s);
+/* Refit: move the scene's vertices without rebuilding its BVH.  No reference
+   counterpart: the reference constructs a new Scene (scene.cpp:50-57) and so
+   rebuilds the tree.  pos holds 9 floats per triangle in scene order (the order
+   rt_scene_get_triangles returns) for all num_triangles triangles of the scene;
+   nrm (same layout) replaces the shading normals, NULL keeps them.  Geometric
+   normals, areas, area-light tables and every node box follow (a leaf's box is
+   the union of its triangles' boxes, an interior node's the union of its
+   children's); topology, triangle order, materials and the camera are kept.
+   When the scene is resident on the GPU (a renderer, pool or GPU query has
+   used it) the refit runs there, in place, after all work on the device has
+   finished; renderers built on the scene see the new geometry from their
+   next pass, and their accumulated samples are not reset (rt_reset).
+   RT_ERR_INVALID_ARG for a wrong count or a non-finite coordinate, before
+   anything changes. */
+int rt_scene_refit(rt_scene* s, const float* pos, const float* nrm, int64_t num_triangles);
 
 /* Renderer::Renderer (Src/Core/renderer.cuh:1094-1100). */
 int rt_renderer_create(rt_scene* s, const rt_render_params* p, rt_renderer** out);

@@ -859,6 +859,59 @@ ref_scene* ref_scene_create(const ref_scene_desc* d) {
     return s;
 }
 
+/* Refit (the reference has none; it reconstructs the Scene, scene.cpp:50-57):
+   new vertex positions, and shading normals unless nrm is NULL, for the same
+   triangles.  The triangle constructor's normal, area and box
+   (triangle.cu:3-10, bounds.h:22-26) and the mesh areas (scene.cpp:515-519)
+   are recomputed; over the unchanged tree a leaf's box is the union of its
+   triangles' boxes in leaf order from the empty box, as the build forms a
+   node's box (bvh.cu:61-64), and an interior node's box is its left child's
+   expanded by its right child's (bounds.h:35-40).  The camera is kept. */
+void ref_scene_refit(ref_scene* s, const float* pos, const float* nrm) {
+    int i, k;
+    for (i = 0; i < s->nt; ++i) {
+        Tri* t = &s->tris[i];
+        V3 nn;
+        for (k = 0; k < 3; ++k) {
+            const float* p = pos + 9 * (size_t)i + 3 * k;
+            t->p[k] = v3(p[0], p[1], p[2]);
+            if (nrm) {
+                const float* q = nrm + 9 * (size_t)i + 3 * k;
+                t->n[k] = v3(q[0], q[1], q[2]);
+            }
+        }
+        nn = vcross(vsub(t->p[1], t->p[0]), vsub(t->p[2], t->p[0]));
+        t->area = 0.5f * vlen(nn);
+        t->gn = vnormalize(nn);
+        for (k = 0; k < 3; ++k) {
+            t->b.lo[k] = smin(vget(t->p[0], k), smin(vget(t->p[1], k), vget(t->p[2], k)));
+            t->b.hi[k] = smax(vget(t->p[0], k), smax(vget(t->p[1], k), vget(t->p[2], k)));
+        }
+    }
+    for (i = 0; i <= s->nm; ++i) s->mesh_area[i] = 0.f;
+    for (i = 0; i < s->nt; ++i) s->mesh_area[s->tris[i].mesh] += s->tris[i].area;
+    for (i = 0; i < s->nt; ++i) s->ordered[i] = s->tris[s->order[i]];
+    /* Children follow their parent in DFS pre-order: one reverse sweep. */
+    for (i = s->nn - 1; i >= 0; --i) {
+        LNode* n = &s->nodes[i];
+        Box b = box_empty();
+        if (n->tri_index >= 0) {
+            for (k = 0; k < n->second; ++k) b = box_union(b, s->ordered[n->tri_index + k].b);
+        } else {
+            Box l, r;
+            memcpy(l.lo, s->nodes[i + 1].lo, 12); memcpy(l.hi, s->nodes[i + 1].hi, 12);
+            memcpy(r.lo, s->nodes[n->second].lo, 12); memcpy(r.hi, s->nodes[n->second].hi, 12);
+            b = box_union(box_union(b, l), r);
+        }
+        memcpy(n->lo, b.lo, 12);
+        memcpy(n->hi, b.hi, 12);
+    }
+    if (s->nn > 0) {
+        memcpy(s->bounds.lo, s->nodes[0].lo, 12);
+        memcpy(s->bounds.hi, s->nodes[0].hi, 12);
+    }
+}
+
 void ref_scene_destroy(ref_scene* s) {
     if (!s) return;
     free(s->tris); free(s->ordered); free(s->order); free(s->nodes);

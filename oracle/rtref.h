@@ -86,6 +86,9 @@ typedef struct ref_svgf_params {
 
 ref_scene* ref_scene_create(const ref_scene_desc* d);
 void ref_scene_destroy(ref_scene* s);
+/* New positions (9 per triangle, scene order) and optionally shading normals
+   (NULL: kept) for the same triangles; node boxes refit over the same tree. */
+void ref_scene_refit(ref_scene* s, const float* pos, const float* nrm);
 
 int ref_bvh_num_nodes(const ref_scene* s);
 int ref_bvh_max_depth(const ref_scene* s);

@@ -84,6 +84,8 @@ def lib():
         L.ref_scene_create.argtypes = [ctypes.POINTER(Desc)]
         L.ref_scene_create.restype = vp
         L.ref_scene_destroy.argtypes = [vp]
+        L.ref_scene_refit.argtypes = [vp, vp, vp]
+        L.ref_scene_refit.restype = None
         L.ref_bvh_num_nodes.argtypes = [vp]
         L.ref_bvh_max_depth.argtypes = [vp]
         L.ref_bvh_nodes.argtypes = [vp, vp]
@@ -493,6 +495,15 @@ class RefScene:
         lib().ref_bvh_nodes(self._h, _p(nodes))
         lib().ref_bvh_order(self._h, _p(order))
         return nodes, order
+
+    def refit(self, pos: np.ndarray, nrm: Optional[np.ndarray] = None) -> None:
+        """New vertex positions ((n, 9) float32, scene order) and optionally shading
+        normals for the same triangles; the tree's boxes are refit (ref_scene_refit)."""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 9)
+        assert pos.shape[0] == self.a.pos.shape[0]
+        self._refit_pos = pos
+        self._refit_nrm = None if nrm is None else np.ascontiguousarray(nrm, np.float32).reshape(-1, 9)
+        lib().ref_scene_refit(self._h, _p(pos), None if nrm is None else _p(self._refit_nrm))
 
     def max_depth(self) -> int:
         return lib().ref_bvh_max_depth(self._h)
