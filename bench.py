@@ -40,6 +40,8 @@ HBM_PEAK_GBS = 8000.0
 # ray; 32 B + 4 B per shadow ray; 64 B per interior node whose child pair is
 # tested; 36 B of vertex data per triangle test.
 EXT_RAY_BYTES, SHADOW_RAY_BYTES, NODE_BYTES, TRI_BYTES = 48, 36, 64, 36
+# With the ray sort on, extend also reads a 4 B permutation entry per ray after the first bounce.
+PERM_BYTES = 4
 # Shade: 84 B read + 68 B written per path-bounce + 44 B per emitted shadow ray.
 SHADE_PATH_BYTES, SHADE_SHADOW_BYTES = 152, 44
 
@@ -535,6 +537,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-spp", type=int, default=64, help="samples per pixel for the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ray-sort", type=int, choices=(0, 1), default=1,
+                    help="reorder extension rays by octant and origin cell before extend (C2 renderer)")
     ap.add_argument("--traffic-file", default=C2_TRAFFIC,
                     help="HBM bytes per launch measured with rocprofv3 --pmc (table from --traffic-from-pmc)")
     ap.add_argument("--only", choices=("frames", "c3", "batched", "serving", "bvh", "textured"), default=None,
@@ -571,7 +575,8 @@ def main() -> int:
     scene = rt.Scene(path)
     info = scene.info()
     spp = info.spp
-    r = rt.Renderer(scene)
+    sort = bool(args.ray_sort)
+    r = rt.Renderer(scene, ray_sort=sort)
     print(f"[bench] scene loaded: {info.num_triangles} tris, {info.width}x{info.height}, {spp} spp, "
           f"depth {info.max_bounces}, batch {r.stats().batch_spp}", file=sys.stderr, flush=True)
 
@@ -596,7 +601,7 @@ def main() -> int:
     print(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, {mrays:.1f} Mrays/s", file=sys.stderr, flush=True)
 
     # Per-kernel timing with HIP events on the renderer's stream, plus traversal counts.
-    rp = rt.Renderer(scene, profile=True)
+    rp = rt.Renderer(scene, profile=True, ray_sort=sort)
     rp.render(spp)
     rp.synchronize()
     rp.reset()
@@ -606,12 +611,14 @@ def main() -> int:
         rp.reset()
         rp.render(spp)
     sp = rp.stats()
-    rc = rt.Renderer(scene)
+    rc = rt.Renderer(scene, ray_sort=sort)
     rc.render(spp, count_traversal=True)
     sc = rc.stats()
     traffic = load_traffic(args.traffic_file)
     ext_bytes = (sc.extension_rays * EXT_RAY_BYTES + sc.extension_node_visits * NODE_BYTES
                  + sc.extension_tri_tests * TRI_BYTES)
+    if sort:
+        ext_bytes += (sc.extension_rays - info.width * info.height * spp) * PERM_BYTES
     sh_bytes = (sc.shadow_rays * SHADOW_RAY_BYTES + sc.shadow_node_visits * NODE_BYTES
                 + sc.shadow_tri_tests * TRI_BYTES)
     shade_bytes = sc.extension_rays * SHADE_PATH_BYTES + sc.shadow_rays * SHADE_SHADOW_BYTES
@@ -623,6 +630,7 @@ def main() -> int:
         t = traffic.get(name)
         stages[name] = _stage_roofline(name, nbytes / launches, ms / launches, t)
         stages[name]["ms_per_step"] = round(ms, 3)
+    sort_idx = rt.STAGES.index("sort")
     dominant = max(stages.values(), key=lambda s: s["ms_per_step"])
 
     out = {
@@ -642,10 +650,11 @@ def main() -> int:
                                "dielectric + MIS, 512x512, 64 spp, depth 12, RR after depth 3",
                    "triangles": info.num_triangles, "width": info.width, "height": info.height, "spp": spp,
                    "max_depth": info.max_bounces, "batch_spp": st.batch_spp,
-                   "rays_per_step": int(rays // args.steps)},
+                   "rays_per_step": int(rays // args.steps), "ray_sort": sort},
         "roofline": {k: dominant[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
         "roofline_kernel": dominant["kernel"],
         "stages": stages,
+        "ray_sort_ms_per_step": round(sp.stage_ms[sort_idx] / prof_steps, 3),
     }
     if not args.no_frames:
         print("[bench] real-time frame loop (SVGF)", file=sys.stderr, flush=True)

@@ -40,7 +40,7 @@ RT_IMAGE_PNG_COLOR = 0
 RT_IMAGE_PNG_DENOISED = 1
 RT_IMAGE_PFM_MEAN = 2
 RT_IMAGE_PFM_DENOISED = 3
-STAGES = ("raygen", "extend", "shade", "shadow", "resolve", "gbuffer", "temporal", "variance", "atrous")
+STAGES = ("raygen", "extend", "shade", "shadow", "resolve", "gbuffer", "temporal", "variance", "atrous", "sort")
 
 # Frame buffers (rt_read_frame): id -> (numpy dtype, channels).
 RT_FRAME_DENOISED = 0
@@ -105,7 +105,7 @@ class RenderParams(ctypes.Structure):
         ("max_bounces", ctypes.c_int), ("rr", ctypes.c_float), ("rr_start_depth", ctypes.c_int),
         ("seed", ctypes.c_int64), ("batch_spp", ctypes.c_int), ("queue_budget_bytes", ctypes.c_int64),
         ("flush_survivors", ctypes.c_int), ("profile", ctypes.c_int), ("env_sampling", ctypes.c_int),
-        ("disable_graphs", ctypes.c_int),
+        ("disable_graphs", ctypes.c_int), ("ray_sort", ctypes.c_int),
     ]
 
 
@@ -388,7 +388,7 @@ class Renderer:
 
     def __init__(self, scene: Scene, max_bounces: int = -1, rr: float = -1.0, rr_start_depth: int = -1,
                  seed: int = -1, batch_spp: int = 0, queue_budget_bytes: int = 0, flush_survivors: bool = False,
-                 profile: bool = False, env_alias: bool = False, graphs: bool = True):
+                 profile: bool = False, env_alias: bool = False, graphs: bool = True, ray_sort: bool = False):
         p = RenderParams()
         lib().rt_render_params_default(ctypes.byref(p))
         p.max_bounces, p.rr, p.rr_start_depth, p.seed = max_bounces, rr, rr_start_depth, seed
@@ -396,6 +396,7 @@ class Renderer:
         p.flush_survivors, p.profile = int(flush_survivors), int(profile)
         p.env_sampling = int(env_alias)
         p.disable_graphs = 0 if graphs else 1
+        p.ray_sort = int(ray_sort)
         self.scene = scene
         self._h = ctypes.c_void_p()
         _check(lib().rt_renderer_create(scene.handle, ctypes.byref(p), ctypes.byref(self._h)))

@@ -15,6 +15,8 @@
 #include "render.h"
 #include "scene.h"
 
+static_assert(RT_NUM_STAGES == rt::kNumStages, "stage ids of rt.h and the engine differ");
+
 struct rt_scene {
     rt::HostScene host;
     std::unique_ptr<rt::GpuScene> gpu;  // created on first GPU use
@@ -89,6 +91,8 @@ rt::RenderSettings settings_from(const rt_render_params* p, const rt::HostScene&
     if (p->env_sampling != 0 && p->env_sampling != 1) throw std::invalid_argument("env_sampling must be 0 or 1");
     set.env_alias = p->env_sampling;
     set.use_graphs = p->disable_graphs ? 0 : 1;
+    if (p->ray_sort != 0 && p->ray_sort != 1) throw std::invalid_argument("ray_sort must be 0 or 1");
+    set.ray_sort = p->ray_sort;
     if (set.max_bounces > 255) throw std::invalid_argument("max_bounces must be <= 255");
     return set;
 }
@@ -146,6 +150,7 @@ void rt_render_params_default(rt_render_params* p) {
     p->profile = 0;
     p->env_sampling = 0;
     p->disable_graphs = 0;
+    p->ray_sort = 0;
 }
 
 void rt_denoise_params_default(rt_denoise_params* p) {

@@ -50,6 +50,8 @@ struct DevBuf {
 // (3 x float4 each), the hit record, one class-list entry per class, the
 // shadow record and the per-sample direct/indirect sums.
 constexpr size_t kQueueBytesPerPath = 2 * 3 * 16 + 16 + 4 * kNumClasses + 64 + 32;
+// With the ray sort on: a 2-byte key and a 4-byte permutation entry per path.
+constexpr size_t kSortBytesPerPath = 6;
 // Device bytes per pixel of a FrameState: two G-buffers (48 B each), two
 // histories (2 x float4), moments (2 x float2) and lengths (int) per frame
 // parity, the variance output, the a-trous ping-pong pair, the denoised colour,
@@ -101,6 +103,9 @@ public:
     void set_use_graphs(bool on) { use_graphs_ = on; }
     // Persistent stage grids = occupancy x CUs x scale (a tuning knob).
     void set_grid_scale(float scale);
+    // Reorder each bounce's extension rays by direction octant and origin
+    // cell before extend (kSortBytesPerPath more device memory per path).
+    void set_ray_sort(bool on);
     float grid_scale() const { return grid_scale_; }
 
 private:
@@ -119,6 +124,8 @@ private:
     size_t cap_ = 0;
     hipStream_t stream_ = nullptr;
     DevBuf qo_[2], qd_[2], qw_[2], hits_, lists_, shadow_, slot_d_, slot_i_, counters_, stats_;
+    bool ray_sort_ = false;
+    DevBuf keys_, perm_, sort_hist_, sort_cur_;  // ray sort: key and permutation per path, per-slice bins
     // Pass tables: one device table, filled by stream-ordered copies from a
     // ring of pinned host tables.
     static constexpr int kRing = 8;

@@ -20,7 +20,8 @@ enum Stage : int {
     kStageTemporal = 6,
     kStageVariance = 7,
     kStageAtrous = 8,
-    kNumStages = 9
+    kStageSort = 9,
+    kNumStages = 10
 };
 
 // Buffers of the last real-time frame, for read-back.
@@ -53,6 +54,7 @@ struct RenderSettings {
     int flush_survivors = 0;         // 1: paths alive after the last bounce keep their radiance
     int env_alias = 0;               // 1: environment sampled with alias tables (not the reference's CDF search)
     int use_graphs = 1;              // replay passes as captured HIP graphs
+    int ray_sort = 0;                // 1: reorder extension rays by direction octant and origin cell
     int profile = 0;                 // per-stage HIP event timing
 };
 

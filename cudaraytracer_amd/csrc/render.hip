@@ -191,6 +191,104 @@ __global__ __launch_bounds__(kBlock) void k_raygen(const PassHeader* __restrict_
     }
 }
 
+// ------------------------------------------------------------------ ray order
+// Optional reordering of each slice's extension rays before extend (render
+// setting ray_sort).  Shade writes a 9-bit key beside every ray it appends:
+// the direction's octant and the origin's cell in a 4x4x4 grid over the job's
+// scene bounds (Morton order).  The keys are counting-sorted per slice into a
+// permutation that extend reads its rays through, so the rays of a wave tend
+// to share their octant and region and take similar paths through the tree.
+// Hits and shading lists keep the queue index: results do not depend on the
+// order.
+constexpr int kSortBins = 512;
+
+__device__ __forceinline__ uint32_t ray_sort_key(vec3 o, vec3 d, const DevScene& sc) {
+    const uint32_t oct = (d.x < 0.f ? 1u : 0u) | (d.y < 0.f ? 2u : 0u) | (d.z < 0.f ? 4u : 0u);
+    auto cell = [](float x, float lo, float hi) {
+        const int c = (int)((x - lo) * (4.f / fmaxf(hi - lo, 1e-30f)));
+        return (uint32_t)min(max(c, 0), 3);
+    };
+    const uint32_t cx = cell(o.x, sc.bounds_lo.x, sc.bounds_hi.x);
+    const uint32_t cy = cell(o.y, sc.bounds_lo.y, sc.bounds_hi.y);
+    const uint32_t cz = cell(o.z, sc.bounds_lo.z, sc.bounds_hi.z);
+    const uint32_t m = (cx & 1u) | ((cy & 1u) << 1) | ((cz & 1u) << 2) | ((cx & 2u) << 2) | ((cy & 2u) << 3) |
+                       ((cz & 2u) << 4);
+    return (oct << 6) | m;
+}
+
+constexpr int kSortItems = 8;  // rays per thread per scatter round
+
+__global__ __launch_bounds__(kBlock) void k_sort_count(const uint16_t* __restrict__ keys,
+                                                        const PassHeader* __restrict__ ph, int* counters, int qslot,
+                                                        int* hist) {
+    __shared__ int h[kSortBins];
+    const int sl = blockIdx.y, region = ph->region;
+    const int n = *ctr_at(counters, qslot, sl);
+    for (int t = threadIdx.x; t < kSortBins; t += kBlock) h[t] = 0;
+    __syncthreads();
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock)
+        atomicAdd(&h[keys[sl * region + j]], 1);
+    __syncthreads();
+    for (int t = threadIdx.x; t < kSortBins; t += kBlock)
+        if (h[t]) atomicAdd(&hist[sl * kSortBins + t], h[t]);
+}
+
+// Exclusive scan of each slice's histogram into the scatter cursors; the
+// histogram is cleared for the next bounce.  One block of kSortBins threads per slice.
+__global__ __launch_bounds__(kSortBins) void k_sort_scan(int* hist, int* cur) {
+    __shared__ int sh[kSortBins];
+    const int sl = blockIdx.x, t = threadIdx.x;
+    const int v = hist[sl * kSortBins + t];
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kSortBins; off <<= 1) {
+        const int a = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+    }
+    cur[sl * kSortBins + t] = sh[t] - v;
+    hist[sl * kSortBins + t] = 0;
+}
+
+// Positions: a block ranks kSortItems * kBlock keys in LDS, reserves each
+// bin's run with one global add, and writes the queue positions into the
+// slice's permutation.
+__global__ __launch_bounds__(kBlock) void k_sort_scatter(const uint16_t* __restrict__ keys,
+                                                          const PassHeader* __restrict__ ph, int* counters, int qslot,
+                                                          int* cur, int* __restrict__ perm) {
+    __shared__ int h[kSortBins], base[kSortBins];
+    const int sl = blockIdx.y, region = ph->region;
+    const int n = *ctr_at(counters, qslot, sl);
+    for (int t = threadIdx.x; t < kSortBins; t += kBlock) h[t] = 0;
+    __syncthreads();
+    constexpr int CHUNK = kSortItems * kBlock;
+    for (int j0 = blockIdx.x * CHUNK; j0 < n; j0 += gridDim.x * CHUNK) {  // uniform per block
+        uint32_t key[kSortItems];
+        int r[kSortItems];
+#pragma unroll
+        for (int k = 0; k < kSortItems; ++k) {
+            const int j = j0 + k * kBlock + (int)threadIdx.x;
+            key[k] = 0;
+            r[k] = -1;
+            if (j < n) {
+                key[k] = keys[sl * region + j];
+                r[k] = atomicAdd(&h[key[k]], 1);
+            }
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < kSortBins; t += kBlock)
+            if (h[t]) base[t] = atomicAdd(&cur[sl * kSortBins + t], h[t]);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kSortItems; ++k)
+            if (r[k] >= 0) perm[sl * region + base[key[k]] + r[k]] = j0 + k * kBlock + (int)threadIdx.x;
+        __syncthreads();
+        for (int t = threadIdx.x; t < kSortBins; t += kBlock) h[t] = 0;
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------ extend
 // Closest hit for every path of the current queue, then the path's queue
 // index is appended to its slice's list for the shading class of what it hit.
@@ -204,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jo
                                                     const float4* __restrict__ qd, float4* __restrict__ hits,
                                                     int* __restrict__ lists, int cap,
                                                     const PassHeader* __restrict__ ph, int* counters, int qslot,
-                                                    unsigned long long* stats) {
+                                                    unsigned long long* stats, const int* __restrict__ perm) {
     const int region = ph->region;
     uint32_t* stk = stk_dyn;
     if (blockIdx.x == 0 && threadIdx.x < kSlices) {
@@ -225,6 +323,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jo
         TravCount c{0, 0};
         int cls = -1;
         if (j < cur.count) {
+            if (perm) i = cur.slice * region + perm[i];
             float4 o = qo[i], d = qd[i];
             const DevScene& sc = pass_job<POOLED>(jobs, (int)((uint32_t)f_as_i(d.w) >> kJobShift)).sc;
             RayInv r = make_ray(v3(o.x, o.y, o.z), v3(d.x, d.y, d.z));
@@ -258,7 +357,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
                                                    const int* __restrict__ list, float4* __restrict__ no,
                                                    float4* __restrict__ nd, float4* __restrict__ nw,
                                                    float4* __restrict__ shadow, float4* slot_d, float4* slot_i,
-                                                   const PassHeader* __restrict__ ph, int* counters, int qslot) {
+                                                   const PassHeader* __restrict__ ph, int* counters, int qslot,
+                                                   uint16_t* __restrict__ nkey) {
     const int region = ph->region;
     uint32_t* stk = stk_dyn;  // sized only for the subsurface class (its probe rays)
     if (blockIdx.x == 0 && threadIdx.x < kSlices) *ctr_at(counters, kFetchExtend, threadIdx.x) = 0;
@@ -275,6 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
         PathState ps;
         int pid = 0;
         uint32_t jobbits = 0;
+        uint32_t key = 0;
         if (j < cur.count) {
             int i = list[(size_t)cur.slice * region + j];
             float4 o = qo[i], d = qd[i], w = qw[i];
@@ -309,6 +410,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
                     so.shadow = false;
                 }
             }
+            if (CLS != kClsTerminal && nkey && so.alive) key = ray_sort_key(ps.o, ps.d, J.sc);
         }
         if (CLS != kClsTerminal) {
             int qpos = wave_append(so.alive, ctr_at(counters, qslot ^ 1, cur.slice));
@@ -317,6 +419,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
                 no[o] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.pdf);
                 nd[o] = make_float4(ps.d.x, ps.d.y, ps.d.z, i_as_f((int)((ps.flags & kFlagMask) | jobbits)));
                 nw[o] = make_float4(ps.w.x, ps.w.y, ps.w.z, i_as_f(pid));
+                if (nkey) nkey[o] = (uint16_t)key;
             }
         }
         if (CLS == kClsDiffuse || CLS == kClsPbr || CLS == kClsSubsurface) {
@@ -1172,6 +1275,18 @@ Engine::Engine(size_t capacity_paths, bool profile) {
     HIP_OK(hipDeviceSynchronize());
 }
 
+void Engine::set_ray_sort(bool on) {
+    if (on && !keys_.p) {
+        keys_.alloc(cap_ * sizeof(uint16_t));
+        perm_.alloc(cap_ * sizeof(int));
+        sort_hist_.alloc((size_t)kSlices * kSortBins * sizeof(int));
+        sort_cur_.alloc((size_t)kSlices * kSortBins * sizeof(int));
+        HIP_OK(hipMemset(sort_hist_.p, 0, sort_hist_.bytes));  // the scan clears it after every use
+        HIP_OK(hipDeviceSynchronize());
+    }
+    ray_sort_ = on;
+}
+
 // Persistent grid of a stage kernel for a dynamic LDS size (cached).
 template <bool POOLED>
 Engine::Grids Engine::grids_for(int stack_entries) {
@@ -1281,7 +1396,8 @@ void Engine::run(const std::vector<PassItem>& items, bool count) {
     if (graph) {
         int mask = 0;
         for (int c = 0; c < kNumClasses; ++c) mask |= classes[c] ? (1 << c) : 0;
-        key = ((uint64_t)max_bounces << 24) | ((uint64_t)stack << 16) | ((uint64_t)mask << 8) | (pooled ? 1 : 0);
+        key = ((uint64_t)max_bounces << 24) | ((uint64_t)stack << 16) | ((uint64_t)mask << 8) | (ray_sort_ ? 2 : 0) |
+              (pooled ? 1 : 0);
         auto it = graphs_.find(key);
         if (it != graphs_.end()) {
             HIP_OK(hipGraphLaunch(it->second, stream_));
@@ -1337,15 +1453,27 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
     HIP_OK(hipGetLastError());
     prof.end(kStageRaygen, b, stream_);
     for (int bounce = 0; bounce < max_bounces; ++bounce) {
+        const int* perm = nullptr;
+        if (ray_sort_ && bounce > 0) {  // primary rays are already in pixel order
+            b = prof.begin(stream_);
+            const dim3 g(std::max(1, grid_linear_ / kSlices), kSlices);
+            k_sort_count<<<g, kBlock, 0, stream_>>>(keys_.as<uint16_t>(), hdr, ctr, q, sort_hist_.as<int>());
+            k_sort_scan<<<kSlices, kSortBins, 0, stream_>>>(sort_hist_.as<int>(), sort_cur_.as<int>());
+            k_sort_scatter<<<g, kBlock, 0, stream_>>>(keys_.as<uint16_t>(), hdr, ctr, q, sort_cur_.as<int>(),
+                                                      perm_.as<int>());
+            HIP_OK(hipGetLastError());
+            prof.end(kStageSort, b, stream_);
+            perm = perm_.as<int>();
+        }
         b = prof.begin(stream_);
         if (count)
             k_extend<true, POOLED><<<gext, kBlock, lds, stream_>>>(
                 jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), hits_.as<float4>(), lists_.as<int>(), cap, hdr,
-                ctr, q, st);
+                ctr, q, st, perm);
         else
             k_extend<false, POOLED><<<gext, kBlock, lds, stream_>>>(
                 jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), hits_.as<float4>(), lists_.as<int>(), cap, hdr,
-                ctr, q, st);
+                ctr, q, st, perm);
         HIP_OK(hipGetLastError());
         prof.end(kStageExtend, b, stream_);
         b = prof.begin(stream_);
@@ -1356,7 +1484,7 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
                 jobs, bounce, qo_[q].as<float4>(), qd_[q].as<float4>(), qw_[q].as<float4>(), hits_.as<float4>(),
                 lists_.as<int>() + (size_t)CLS * cap_, qo_[q ^ 1].as<float4>(), qd_[q ^ 1].as<float4>(),
                 qw_[q ^ 1].as<float4>(), shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), hdr,
-                ctr, q);
+                ctr, q, ray_sort_ ? keys_.as<uint16_t>() : nullptr);
             HIP_OK(hipGetLastError());
         };
         shade(std::integral_constant<int, kClsTerminal>(), gshade[kClsTerminal]);
@@ -1646,12 +1774,14 @@ Renderer::Renderer(const GpuScene& scene, const HostScene& host, const RenderSet
     // Paths per pass: all requested samples of every pixel when they fit the
     // queue budget.
     size_t budget = s.queue_budget_bytes ? s.queue_budget_bytes : ((size_t)8 << 30);
-    int bmax = (int)std::max<size_t>(1, budget / (kQueueBytesPerPath * (size_t)I.wh));
+    const size_t per_path = kQueueBytesPerPath + (s.ray_sort ? kSortBytesPerPath : 0);
+    int bmax = (int)std::max<size_t>(1, budget / (per_path * (size_t)I.wh));
     int want = s.batch_spp > 0 ? s.batch_spp : 64;
     I.batch = std::max(1, std::min(want, bmax));
     if ((size_t)I.wh * I.batch > (size_t)0x7fffffff) I.batch = std::max(1, (int)(0x7fffffff / I.wh));
     I.eng.reset(new Engine((size_t)I.wh * I.batch, s.profile != 0));
     I.eng->set_use_graphs(s.use_graphs != 0);
+    I.eng->set_ray_sort(s.ray_sort != 0);
     I.acc_d.alloc((size_t)I.wh * 16);
     I.acc_i.alloc((size_t)I.wh * 16);
     HIP_OK(hipMemset(I.acc_d.p, 0, (size_t)I.wh * 16));

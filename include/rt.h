@@ -81,7 +81,8 @@ extern "C" {
 #define RT_STAGE_TEMPORAL 6
 #define RT_STAGE_VARIANCE 7
 #define RT_STAGE_ATROUS 8
-#define RT_NUM_STAGES 9
+#define RT_STAGE_SORT 9 /* ray sort before extend (rt_render_params.ray_sort) */
+#define RT_NUM_STAGES 10
 
 /* Buffers of the last rt_render_frame, row 0 = bottom. */
 #define RT_FRAME_DENOISED 0          /* float4: filtered direct + indirect, alpha 1 */
@@ -143,6 +144,9 @@ typedef struct rt_render_params {
     int profile;            /* 1 = per-stage HIP event timing in rt_stats */
     int env_sampling;       /* 0 = CDF binary search (reference, texture.cu:112-153); 1 = alias tables */
     int disable_graphs;     /* 1 = launch every kernel of a pass; 0 = replay passes as captured HIP graphs */
+    int ray_sort;           /* 1 = before each extend, reorder the rays of every queue slice by direction
+                               octant and origin cell (6 more bytes per path; same results, faster
+                               traversal on coherent scenes such as closed rooms) */
 } rt_render_params;
 
 typedef struct rt_stats_out {

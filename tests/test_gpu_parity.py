@@ -232,3 +232,55 @@ def test_converged_image_within_rmse_of_fp64_oracle(scene_dir, oracle_lib, name)
     print(f"{name}: {close.mean():.4f} of pixels close, RMSE {rmse:.3e} vs mean {ref.mean():.3e}", flush=True)
     assert close.mean() > 0.97
     assert rmse < 0.05 * ref.mean()
+
+
+@pytest.mark.parametrize("name", ["cornell_small", "cornell_spheres_128", "terrain_small"])
+def test_ray_sort_keeps_results_bit_exact(scene_dir, oracle_lib, name):
+    """rt_render_params.ray_sort reorders each bounce's extension rays; every
+    path's hits, and so the images and ray counts, stay the oracle's."""
+    spp = 4
+    rs = oracle_lib.RefScene(oracle_lib.parse_scene(scene_dir[name]))
+    s = rt.Scene(scene_dir[name])
+    r = rt.Renderer(s, ray_sort=True)
+    r.render(spp)
+    d = r.read_accum(rt.RT_ACCUM_DIRECT).reshape(-1, 4)
+    i = r.read_accum(rt.RT_ACCUM_INDIRECT).reshape(-1, 4)
+    rd, ri, rst = rs.render(spp, rs.params())
+    assert np.array_equal(_bits(d[:, :3]), _bits(rd[:, :3]))
+    assert np.array_equal(_bits(i[:, :3]), _bits(ri[:, :3]))
+    assert r.stats().extension_rays == rst[0] and r.stats().shadow_rays == rst[1]
+    assert r.stats().stage_launches[rt.STAGES.index("sort")] == 0  # not profiled: no stage events
+
+
+def test_ray_sort_counts_graphs_and_batches(scene_dir):
+    """Counted traversal totals (each ray walks the same nodes in any order),
+    graph replays and pass splits give the unsorted renderer's results."""
+    path = scene_dir["cornell_spheres_128"]
+    base = rt.Renderer(rt.Scene(path))
+    base.render(6, count_traversal=True)
+    want = base.stats()
+    wd = base.read_accum(rt.RT_ACCUM_DIRECT)
+    for kw in ({}, {"graphs": False}, {"batch_spp": 1}, {"batch_spp": 4, "profile": True}):
+        r = rt.Renderer(rt.Scene(path), ray_sort=True, **kw)
+        r.render(6, count_traversal=True)
+        st = r.stats()
+        assert (st.extension_node_visits, st.extension_tri_tests, st.shadow_node_visits) == \
+            (want.extension_node_visits, want.extension_tri_tests, want.shadow_node_visits), kw
+        r2 = rt.Renderer(rt.Scene(path), ray_sort=True, **kw)
+        r2.render(6)
+        assert np.array_equal(_bits(r2.read_accum(rt.RT_ACCUM_DIRECT)), _bits(wd)), kw
+        if kw.get("profile"):
+            assert r2.stats().stage_launches[rt.STAGES.index("sort")] > 0
+
+
+def test_ray_sort_svgf_frames_match_unsorted(scene_dir):
+    path = scene_dir["cornell_spheres_rt"]
+    a, b = rt.Renderer(rt.Scene(path)), rt.Renderer(rt.Scene(path), ray_sort=True)
+    cam = a.camera()
+    for f in range(3):
+        c = rt.Camera(tuple(cam.target), cam.distance, cam.theta_deg, cam.phi_deg + 1.5 * f)
+        a.set_camera(c)
+        b.set_camera(c)
+        a.render_frame(1)
+        b.render_frame(1)
+        assert np.array_equal(_bits(a.read_frame(rt.RT_FRAME_DENOISED)), _bits(b.read_frame(rt.RT_FRAME_DENOISED)))
