@@ -327,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jo
             float4 o = qo[i], d = qd[i];
             const DevScene& sc = pass_job<POOLED>(jobs, (int)((uint32_t)f_as_i(d.w) >> kJobShift)).sc;
             RayInv r = make_ray(v3(o.x, o.y, o.z), v3(d.x, d.y, d.z));
-            Hit h = trace_closest<COUNT>(sc, r, 0.f, 1e8f, col, kBlock, c);
+            Hit h = trace_closest_wave<COUNT>(sc, r, 0.f, 1e8f, col, kBlock, c);
             hits[i] = make_float4(h.t, h.b1, h.b2, i_as_f(h.prim));
             cls = shade_class_of(sc.tri_attr, h.prim);
         }
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jo
             rec.hp = v3(r2.x, r2.y, r2.z);
             vec3 org = rec.lp + 0.01f * rec.dir;
             RayInv ray = make_ray(org, rec.dir);
-            Hit h = trace_closest<COUNT>(sc, ray, 1e-6f, 1e8f, col, kBlock, c);
+            Hit h = trace_closest_wave<COUNT>(sc, ray, 1e-6f, 1e8f, col, kBlock, c);
             bool hit = h.prim >= 0;
             vec3 tp = v3(0.f, 0.f, 0.f);
             if (hit) tp = position_at(sc.tri_attr, h.prim, h.b1, h.b2);
@@ -541,7 +541,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_batch(DevScene sc, const float
         float4 o = rays[2 * (size_t)i], d = rays[2 * (size_t)i + 1];
         RayInv r = make_ray(v3(o.x, o.y, o.z), v3(d.x, d.y, d.z));
         TravCount c{0, 0};
-        Hit h = trace_closest<true>(sc, r, o.w, d.w, col, kBlock, c);
+        Hit h = trace_closest_wave<true>(sc, r, o.w, d.w, col, kBlock, c);
         out[i] = make_float4(h.t, h.b1, h.b2, i_as_f(h.prim));
         counts[i] = make_uint2(c.nodes, c.tris);
     }

@@ -75,7 +75,7 @@ __device__ void gbuffer_pixel(const DevScene& sc, const GBufferParams& g, int i,
     vec3 d = pixel_center_dir(g, i, j);
     RayInv r = make_ray(g.cam_pos, d);
     TravCount c{0, 0};
-    Hit h = trace_closest<false, false>(sc, r, 0.f, 1e8f, stk, stride, c);
+    Hit h = trace_closest_wave<false, false>(sc, r, 0.f, 1e8f, stk, stride, c);
     if (h.prim < 0) {
         out.pos_tri[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
         out.nrm_mat[idx] = make_ushort4(0, 0, 0, 0);

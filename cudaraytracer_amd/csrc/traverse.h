@@ -29,14 +29,21 @@ RT_HD RayInv make_ray(vec3 o, vec3 d) {
 
 // Slab test with the reference's 1e-2 slack: (double)x + 1e-2 >= 0 holds
 // exactly when x >= -0.01f for every float x, so the test stays in FP32.
+// OCT >= 0 fixes the signs of the inverse direction at compile time (bit k
+// set: axis k negative) for rays known to have them; the arithmetic is the
+// same, only the per-axis selects disappear.
+template <int OCT = -1>
 RT_HD bool box_hit(const RayInv& r, float lx, float ly, float lz, float hx, float hy, float hz, float tmin,
                    float tmax, float& t) {
+    const bool nx = OCT < 0 ? r.nx : (OCT & 1) != 0;
+    const bool ny = OCT < 0 ? r.ny : (OCT & 2) != 0;
+    const bool nz = OCT < 0 ? r.nz : (OCT & 4) != 0;
     float ex = (lx - r.o.x) * r.inv.x, xx = (hx - r.o.x) * r.inv.x;
     float ey = (ly - r.o.y) * r.inv.y, xy = (hy - r.o.y) * r.inv.y;
     float ez = (lz - r.o.z) * r.inv.z, xz = (hz - r.o.z) * r.inv.z;
-    float ent_x = r.nx ? xx : ex, ext_x = r.nx ? ex : xx;
-    float ent_y = r.ny ? xy : ey, ext_y = r.ny ? ey : xy;
-    float ent_z = r.nz ? xz : ez, ext_z = r.nz ? ez : xz;
+    float ent_x = nx ? xx : ex, ext_x = nx ? ex : xx;
+    float ent_y = ny ? xy : ey, ext_y = ny ? ey : xy;
+    float ent_z = nz ? xz : ez, ext_z = nz ? ez : xz;
     float exit = fminf(ext_x, fminf(ext_y, ext_z));
     float enter = fmaxf(ent_x, fmaxf(ent_y, ent_z));
     t = enter;
@@ -85,7 +92,8 @@ struct TravCount {
 using LdsU32 = __attribute__((address_space(3))) uint32_t;
 
 // `stk` points at this lane's stack column in LDS; entries are `stride` words apart.
-template <bool COUNT, bool CULL = true>
+// OCT >= 0: the ray's inverse-direction signs (see box_hit), known for the whole wave.
+template <bool COUNT, bool CULL = true, int OCT = -1>
 __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r, float tmin, float tmax,
                                              uint32_t* stk_generic, int stride, TravCount& cnt) {
     LdsU32* stk = (LdsU32*)stk_generic;
@@ -115,8 +123,8 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r
             int4 w3 = make_int4(f_as_i(w3f.x), f_as_i(w3f.y), f_as_i(w3f.z), f_as_i(w3f.w));
             if (COUNT) cnt.nodes++;
             float tr, tl;
-            bool hr = box_hit(r, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w, tmin, h.t, tr);
-            bool hl = box_hit(r, w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, tmin, h.t, tl);
+            bool hr = box_hit<OCT>(r, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w, tmin, h.t, tr);
+            bool hl = box_hit<OCT>(r, w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, tmin, h.t, tl);
             uint32_t rl = (uint32_t)w3.x, rr = (uint32_t)w3.y;
             if (hr && hl) {
                 // Nearer child next; ties visit the left child first.
@@ -142,6 +150,33 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r
         ref = stk[sp * stride];
     }
     return h;
+}
+
+// Octant of a ray as box_hit uses it (bit k: inverse direction negative on axis k).
+RT_HD int ray_octant(const RayInv& r) { return (r.nx ? 1 : 0) | (r.ny ? 2 : 0) | (r.nz ? 4 : 0); }
+
+// trace_closest with the slab selects resolved at compile time when every
+// active lane of the wave has the same octant (after the ray sort most waves
+// do, and so do most camera rays); otherwise the general loop.  Same hits
+// either way.
+template <bool COUNT, bool CULL = true>
+__device__ __forceinline__ Hit trace_closest_wave(const DevScene& sc, const RayInv& r, float tmin, float tmax,
+                                                  uint32_t* stk, int stride, TravCount& cnt) {
+    const int oct = ray_octant(r);
+    const int first = __builtin_amdgcn_readfirstlane(oct);
+    if (__ballot(oct != first) == 0ull) {
+        switch (first) {
+            case 0: return trace_closest<COUNT, CULL, 0>(sc, r, tmin, tmax, stk, stride, cnt);
+            case 1: return trace_closest<COUNT, CULL, 1>(sc, r, tmin, tmax, stk, stride, cnt);
+            case 2: return trace_closest<COUNT, CULL, 2>(sc, r, tmin, tmax, stk, stride, cnt);
+            case 3: return trace_closest<COUNT, CULL, 3>(sc, r, tmin, tmax, stk, stride, cnt);
+            case 4: return trace_closest<COUNT, CULL, 4>(sc, r, tmin, tmax, stk, stride, cnt);
+            case 5: return trace_closest<COUNT, CULL, 5>(sc, r, tmin, tmax, stk, stride, cnt);
+            case 6: return trace_closest<COUNT, CULL, 6>(sc, r, tmin, tmax, stk, stride, cnt);
+            default: return trace_closest<COUNT, CULL, 7>(sc, r, tmin, tmax, stk, stride, cnt);
+        }
+    }
+    return trace_closest<COUNT, CULL>(sc, r, tmin, tmax, stk, stride, cnt);
 }
 
 // Shading data at a hit, reconstructed from barycentrics the way the
