@@ -168,7 +168,8 @@ class JobStatus(ctypes.Structure):
 
 
 class PoolTuning(ctypes.Structure):
-    _fields_ = [("grid_scale", ctypes.c_float), ("pass_paths", ctypes.c_int64), ("tile_width", ctypes.c_int)]
+    _fields_ = [("grid_scale", ctypes.c_float), ("pass_paths", ctypes.c_int64), ("tile_width", ctypes.c_int),
+                ("ray_sort", ctypes.c_int)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -581,8 +582,9 @@ class Pool:
         _check(lib().rt_pool_get_tuning(self._h, ctypes.byref(t)))
         return t
 
-    def set_tuning(self, grid_scale: float = 1.0, pass_paths: int = 0, tile_width: int = 16) -> None:
-        t = PoolTuning(grid_scale, pass_paths, tile_width)
+    def set_tuning(self, grid_scale: float = 1.0, pass_paths: int = 0, tile_width: int = 16,
+                   ray_sort: bool = False) -> None:
+        t = PoolTuning(grid_scale, pass_paths, tile_width, int(ray_sort))
         _check(lib().rt_pool_set_tuning(self._h, ctypes.byref(t)))
 
     def autotune(self, sample_jobs) -> dict:

@@ -644,6 +644,7 @@ void tuning_out(const rt::Pool::Tuning& t, rt_pool_tuning* o) {
     o->grid_scale = t.grid_scale;
     o->pass_paths = (int64_t)t.pass_paths;
     o->tile_width = 1 << t.tile_shift;
+    o->ray_sort = t.ray_sort;
 }
 }  // namespace
 
@@ -655,6 +656,7 @@ int rt_pool_set_tuning(rt_pool* p, const rt_pool_tuning* t) {
         tu.grid_scale = t->grid_scale;
         tu.pass_paths = (size_t)t->pass_paths;
         tu.tile_shift = tile_shift_of(t->tile_width);
+        tu.ray_sort = t->ray_sort;
         p->p->set_tuning(tu);
         return RT_OK;
     });

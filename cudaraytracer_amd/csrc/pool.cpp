@@ -477,7 +477,9 @@ void Pool::reset_stats() { impl_->eng->reset_stats(); }
 void Pool::set_tuning(const Tuning& t) {
     Impl& I = *impl_;
     if (t.tile_shift < 3 || t.tile_shift > 7) throw std::invalid_argument("tile shift must be in 3..7");
+    if (t.ray_sort != 0 && t.ray_sort != 1) throw std::invalid_argument("ray_sort must be 0 or 1");
     I.eng->set_grid_scale(t.grid_scale);
+    I.eng->set_ray_sort(t.ray_sort != 0);
     I.tune = t;
     for (auto& kv : I.jobs)
         if (kv.second->fs) kv.second->fs->set_tile_shift(t.tile_shift);
@@ -546,6 +548,8 @@ Pool::TuneReport Pool::autotune(const std::vector<JobDesc>& sample) {
         const int shifts[] = {3, 5, 6};
         try_values([](Tuning& t, int v) { t.tile_shift = v; }, shifts);
     }
+    const int sorts[] = {1 - cur.ray_sort};
+    try_values([](Tuning& t, int v) { t.ray_sort = v; }, sorts);
     set_tuning(cur);
     rep.best = cur;
     rep.best_ms = best_ms;

@@ -201,11 +201,12 @@ public:
         float grid_scale = 1.f;   // persistent stage grids = occupancy x CUs x scale
         size_t pass_paths = 0;    // paths per pass (0: the engine's capacity)
         int tile_shift = 4;       // frame kernels on 2^shift x (256 >> shift) tiles
+        int ray_sort = 0;         // 1: reorder extension rays before extend (6 B per path beyond the queue budget)
     };
     void set_tuning(const Tuning& t);
     Tuning tuning() const;
     // Time a sample of the expected jobs under candidate tunings (coordinate
-    // descent: grid scale, then pass size, then tile shape; each candidate
+    // descent: grid scale, then pass size, then tile shape, then the ray sort; each candidate
     // run twice, best time kept) and keep the fastest.  The pool must be idle.
     struct TuneReport {
         Tuning best;

@@ -300,6 +300,8 @@ typedef struct rt_pool_tuning {
     int64_t pass_paths;  /* paths per pass; 0 = the queue capacity */
     int tile_width;      /* per-pixel frame kernels on tile_width x (256 / tile_width) tiles: 8..128
                             (the first a-trous pass always runs on 16 x 16 LDS tiles) */
+    int ray_sort;        /* 1 = reorder extension rays before extend (rt_render_params.ray_sort);
+                            6 bytes per path of device memory beyond the queue budget */
 } rt_pool_tuning;
 
 typedef struct rt_tune_report {
@@ -327,7 +329,8 @@ int rt_pool_reset_stats(rt_pool* p);
 int rt_pool_set_tuning(rt_pool* p, const rt_pool_tuning* t);
 int rt_pool_get_tuning(rt_pool* p, rt_pool_tuning* out);
 /* Autotuner: time the n sample jobs under candidate tunings (grid scale, then pass size, then tile
-   shape; coordinate descent, each candidate run twice) and keep the fastest.  The pool must be idle. */
+   shape, then the ray sort; coordinate descent, each candidate run twice) and keep the fastest.
+   The pool must be idle. */
 int rt_pool_autotune(rt_pool* p, const rt_job_desc* sample, int n, rt_tune_report* out);
 /* Replay an arrival stream: batched = 1 serves it with the pool's continuous batching,
    0 with the plain loop (one job at a time, a fresh single-scene renderer each).  Frame

@@ -159,7 +159,7 @@ def test_pool_serve_batched_and_plain(scene_dir):
 
 
 def test_pool_tuning_keeps_results_and_autotune_runs(scene_dir):
-    """Launch shape, pass size and tile shape change speed only: results stay
+    """Launch shape, pass size, tile shape and the ray sort change speed only: results stay
     bit-identical; the autotuner picks a valid tuning from a sample."""
     scene = rt.Scene(scene_dir["cornell_spheres_rt"])
     r = rt.Renderer(scene)
@@ -173,10 +173,10 @@ def test_pool_tuning_keeps_results_and_autotune_runs(scene_dir):
     info = scene.info()
     pool = rt.Pool()
     slot = pool.add_scene(scene)
-    for scale, paths, tw in ((0.5, 0, 8), (1.25, 20000, 64), (1.0, 0, 32)):
-        pool.set_tuning(scale, paths, tw)
+    for scale, paths, tw, sort in ((0.5, 0, 8, False), (1.25, 20000, 64, True), (1.0, 0, 32, True)):
+        pool.set_tuning(scale, paths, tw, sort)
         t = pool.tuning()
-        assert (t.grid_scale, t.pass_paths, t.tile_width) == (scale, paths, tw)
+        assert (t.grid_scale, t.pass_paths, t.tile_width, t.ray_sort) == (scale, paths, tw, int(sort))
         acc = pool.submit(rt.Job(slot, spp=5))
         rtj = pool.submit(rt.Job(slot, spp=1, frames=3, cameras=frames))
         f = 0
@@ -184,7 +184,7 @@ def test_pool_tuning_keeps_results_and_autotune_runs(scene_dir):
             st = pool.status(rtj)
             if st.frames_done > f:
                 _bits_equal(pool.read_frame(rtj, rt.RT_FRAME_DENOISED, info.width, info.height), alone[f],
-                            f"tuning {scale, paths, tw} frame {f}")
+                            f"tuning {scale, paths, tw, sort} frame {f}")
                 f = st.frames_done
         assert f == 3
         _bits_equal(pool.read_accum(acc, rt.RT_ACCUM_DIRECT, info.width, info.height), d_alone, "accumulate")
@@ -193,4 +193,4 @@ def test_pool_tuning_keeps_results_and_autotune_runs(scene_dir):
     rep = pool.autotune([rt.Job(slot, spp=4), rt.Job(slot, spp=1, frames=2, cameras=frames[:2])])
     print("autotune", rep, flush=True)
     assert rep["runs"] >= 4 and rep["best_ms"] <= rep["baseline_ms"] + 1e-9
-    assert rep["best"]["tile_width"] in (8, 16, 32, 64)
+    assert rep["best"]["tile_width"] in (8, 16, 32, 64) and rep["best"]["ray_sort"] in (0, 1)
