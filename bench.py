@@ -40,8 +40,10 @@ HBM_PEAK_GBS = 8000.0
 # ray; 32 B + 4 B per shadow ray; 64 B per interior node whose child pair is
 # tested; 36 B of vertex data per triangle test.
 EXT_RAY_BYTES, SHADOW_RAY_BYTES, NODE_BYTES, TRI_BYTES = 48, 36, 64, 36
-# With the ray sort on, extend also reads a 4 B permutation entry per ray after the first bounce.
-PERM_BYTES = 4
+# With the ray sort on, after the first bounce extend also reads a 4 B permutation
+# entry and writes a 32 B copy of the ray at its sorted position, and shade reads
+# the permutation entry.
+PERM_BYTES, RAY_COPY_BYTES = 4, 32
 # Shade: 84 B read + 68 B written per path-bounce + 44 B per emitted shadow ray.
 SHADE_PATH_BYTES, SHADE_SHADOW_BYTES = 152, 44
 
@@ -623,11 +625,11 @@ def main() -> int:
     traffic = load_traffic(args.traffic_file)
     ext_bytes = (sc.extension_rays * EXT_RAY_BYTES + sc.extension_node_visits * NODE_BYTES
                  + sc.extension_tri_tests * TRI_BYTES)
-    if sort:
-        ext_bytes += (sc.extension_rays - info.width * info.height * spp) * PERM_BYTES
+    sorted_rays = (sc.extension_rays - info.width * info.height * spp) if sort else 0
+    ext_bytes += sorted_rays * (PERM_BYTES + RAY_COPY_BYTES)
     sh_bytes = (sc.shadow_rays * SHADOW_RAY_BYTES + sc.shadow_node_visits * NODE_BYTES
                 + sc.shadow_tri_tests * TRI_BYTES)
-    shade_bytes = sc.extension_rays * SHADE_PATH_BYTES + sc.shadow_rays * SHADE_SHADOW_BYTES
+    shade_bytes = sc.extension_rays * SHADE_PATH_BYTES + sc.shadow_rays * SHADE_SHADOW_BYTES + sorted_rays * PERM_BYTES
     stages = {}
     for k, (name, nbytes) in enumerate([("k_extend", ext_bytes), ("k_shade", shade_bytes), ("k_shadow", sh_bytes)]):
         idx = {"k_extend": 1, "k_shade": 2, "k_shadow": 3}[name]

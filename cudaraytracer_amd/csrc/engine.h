@@ -50,8 +50,9 @@ struct DevBuf {
 // (3 x float4 each), the hit record, one class-list entry per class, the
 // shadow record and the per-sample direct/indirect sums.
 constexpr size_t kQueueBytesPerPath = 2 * 3 * 16 + 16 + 4 * kNumClasses + 64 + 32;
-// With the ray sort on: a 2-byte key and a 4-byte permutation entry per path.
-constexpr size_t kSortBytesPerPath = 6;
+// With the ray sort on: a 2-byte key, a 4-byte permutation entry and a
+// 32-byte copy of the ray at its sorted position per path.
+constexpr size_t kSortBytesPerPath = 38;
 // Device bytes per pixel of a FrameState: two G-buffers (48 B each), two
 // histories (2 x float4), moments (2 x float2) and lengths (int) per frame
 // parity, the variance output, the a-trous ping-pong pair, the denoised colour,
@@ -126,6 +127,7 @@ private:
     DevBuf qo_[2], qd_[2], qw_[2], hits_, lists_, shadow_, slot_d_, slot_i_, counters_, stats_;
     bool ray_sort_ = false;
     DevBuf keys_, perm_, sort_hist_, sort_cur_;  // ray sort: key and permutation per path, per-slice bins
+    DevBuf sorted_o_, sorted_d_;                 // ray sort: extend's copy of each ray at its sorted position
     // Pass tables: one device table, filled by stream-ordered copies from a
     // ring of pinned host tables.
     static constexpr int kRing = 8;

@@ -201,7 +201,7 @@ public:
         float grid_scale = 1.f;   // persistent stage grids = occupancy x CUs x scale
         size_t pass_paths = 0;    // paths per pass (0: the engine's capacity)
         int tile_shift = 4;       // frame kernels on 2^shift x (256 >> shift) tiles
-        int ray_sort = 0;         // 1: reorder extension rays before extend (6 B per path beyond the queue budget)
+        int ray_sort = 0;         // 1: reorder extension rays before extend (38 B per path beyond the queue budget)
     };
     void set_tuning(const Tuning& t);
     Tuning tuning() const;

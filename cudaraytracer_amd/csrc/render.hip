@@ -198,8 +198,9 @@ __global__ __launch_bounds__(kBlock) void k_raygen(const PassHeader* __restrict_
 // scene bounds (Morton order).  The keys are counting-sorted per slice into a
 // permutation that extend reads its rays through, so the rays of a wave tend
 // to share their octant and region and take similar paths through the tree.
-// Hits and shading lists keep the queue index: results do not depend on the
-// order.
+// Extend writes each hit, and a copy of the ray, at the sorted position, so
+// shade reads them in order; only the path weight stays in its queue slot.
+// Every path's arithmetic is the same in any order: results do not change.
 constexpr int kSortBins = 512;
 
 __device__ __forceinline__ uint32_t ray_sort_key(vec3 o, vec3 d, const DevScene& sc) {
@@ -302,7 +303,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jo
                                                     const float4* __restrict__ qd, float4* __restrict__ hits,
                                                     int* __restrict__ lists, int cap,
                                                     const PassHeader* __restrict__ ph, int* counters, int qslot,
-                                                    unsigned long long* stats, const int* __restrict__ perm) {
+                                                    unsigned long long* stats, const int* __restrict__ perm,
+                                                    float4* __restrict__ sorted_o, float4* __restrict__ sorted_d) {
     const int region = ph->region;
     uint32_t* stk = stk_dyn;
     if (blockIdx.x == 0 && threadIdx.x < kSlices) {
@@ -319,23 +321,30 @@ __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jo
     int base;
     while (cur.next(base)) {
         int j = base + (int)(threadIdx.x & (kWave - 1));
-        int i = cur.slice * region + j;
+        // p: the position in this launch's order, where the hit, the class list
+        // entry and (after a ray sort) a copy of the ray go; i: the ray's queue slot.
+        const int p = cur.slice * region + j;
+        int i = p;
         TravCount c{0, 0};
         int cls = -1;
         if (j < cur.count) {
-            if (perm) i = cur.slice * region + perm[i];
+            if (perm) i = cur.slice * region + perm[p];
             float4 o = qo[i], d = qd[i];
             const DevScene& sc = pass_job<POOLED>(jobs, (int)((uint32_t)f_as_i(d.w) >> kJobShift)).sc;
             RayInv r = make_ray(v3(o.x, o.y, o.z), v3(d.x, d.y, d.z));
             Hit h = trace_closest_wave<COUNT>(sc, r, 0.f, 1e8f, col, kBlock, c);
-            hits[i] = make_float4(h.t, h.b1, h.b2, i_as_f(h.prim));
+            hits[p] = make_float4(h.t, h.b1, h.b2, i_as_f(h.prim));
+            if (perm) {
+                sorted_o[p] = o;
+                sorted_d[p] = d;
+            }
             cls = shade_class_of(sc.tri_attr, h.prim);
         }
 #pragma unroll
         for (int k = 0; k < kNumClasses; ++k) {
             if (__ballot(cls == k) == 0ull) continue;
             int pos = wave_append(cls == k, ctr_at(counters, kCntClass0 + k, cur.slice));
-            if (cls == k) lists[(size_t)k * cap + (size_t)cur.slice * region + pos] = i;
+            if (cls == k) lists[(size_t)k * cap + (size_t)cur.slice * region + pos] = p;
         }
         if (COUNT) {
             wave_add_u64(&stats[kStatExtNodes], c.nodes);
@@ -358,7 +367,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
                                                    float4* __restrict__ nd, float4* __restrict__ nw,
                                                    float4* __restrict__ shadow, float4* slot_d, float4* slot_i,
                                                    const PassHeader* __restrict__ ph, int* counters, int qslot,
-                                                   uint16_t* __restrict__ nkey) {
+                                                   uint16_t* __restrict__ nkey, const int* __restrict__ perm,
+                                                   const float4* __restrict__ sorted_o,
+                                                   const float4* __restrict__ sorted_d) {
     const int region = ph->region;
     uint32_t* stk = stk_dyn;  // sized only for the subsurface class (its probe rays)
     if (blockIdx.x == 0 && threadIdx.x < kSlices) *ctr_at(counters, kFetchExtend, threadIdx.x) = 0;
@@ -377,8 +388,20 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
         uint32_t jobbits = 0;
         uint32_t key = 0;
         if (j < cur.count) {
-            int i = list[(size_t)cur.slice * region + j];
-            float4 o = qo[i], d = qd[i], w = qw[i];
+            // List entries are extend's positions: after a ray sort the ray's
+            // origin and direction were copied there and its weight stays in
+            // its queue slot; otherwise position and slot are the same.
+            const int i = list[(size_t)cur.slice * region + j];
+            float4 o, d, w;
+            if (perm) {
+                o = sorted_o[i];
+                d = sorted_d[i];
+                w = qw[cur.slice * region + perm[i]];
+            } else {
+                o = qo[i];
+                d = qd[i];
+                w = qw[i];
+            }
             const uint32_t flags = (uint32_t)f_as_i(d.w);
             jobbits = flags & ~kFlagMask;
             const JobDev& J = pass_job<POOLED>(jobs, (int)(flags >> kJobShift));
@@ -1279,6 +1302,8 @@ void Engine::set_ray_sort(bool on) {
     if (on && !keys_.p) {
         keys_.alloc(cap_ * sizeof(uint16_t));
         perm_.alloc(cap_ * sizeof(int));
+        sorted_o_.alloc(cap_ * sizeof(float4));
+        sorted_d_.alloc(cap_ * sizeof(float4));
         sort_hist_.alloc((size_t)kSlices * kSortBins * sizeof(int));
         sort_cur_.alloc((size_t)kSlices * kSortBins * sizeof(int));
         HIP_OK(hipMemset(sort_hist_.p, 0, sort_hist_.bytes));  // the scan clears it after every use
@@ -1469,11 +1494,11 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
         if (count)
             k_extend<true, POOLED><<<gext, kBlock, lds, stream_>>>(
                 jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), hits_.as<float4>(), lists_.as<int>(), cap, hdr,
-                ctr, q, st, perm);
+                ctr, q, st, perm, perm ? sorted_o_.as<float4>() : nullptr, perm ? sorted_d_.as<float4>() : nullptr);
         else
             k_extend<false, POOLED><<<gext, kBlock, lds, stream_>>>(
                 jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), hits_.as<float4>(), lists_.as<int>(), cap, hdr,
-                ctr, q, st, perm);
+                ctr, q, st, perm, perm ? sorted_o_.as<float4>() : nullptr, perm ? sorted_d_.as<float4>() : nullptr);
         HIP_OK(hipGetLastError());
         prof.end(kStageExtend, b, stream_);
         b = prof.begin(stream_);
@@ -1484,7 +1509,8 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
                 jobs, bounce, qo_[q].as<float4>(), qd_[q].as<float4>(), qw_[q].as<float4>(), hits_.as<float4>(),
                 lists_.as<int>() + (size_t)CLS * cap_, qo_[q ^ 1].as<float4>(), qd_[q ^ 1].as<float4>(),
                 qw_[q ^ 1].as<float4>(), shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), hdr,
-                ctr, q, ray_sort_ ? keys_.as<uint16_t>() : nullptr);
+                ctr, q, ray_sort_ ? keys_.as<uint16_t>() : nullptr, perm, perm ? sorted_o_.as<float4>() : nullptr,
+                perm ? sorted_d_.as<float4>() : nullptr);
             HIP_OK(hipGetLastError());
         };
         shade(std::integral_constant<int, kClsTerminal>(), gshade[kClsTerminal]);

@@ -145,7 +145,7 @@ typedef struct rt_render_params {
     int env_sampling;       /* 0 = CDF binary search (reference, texture.cu:112-153); 1 = alias tables */
     int disable_graphs;     /* 1 = launch every kernel of a pass; 0 = replay passes as captured HIP graphs */
     int ray_sort;           /* 1 = before each extend, reorder the rays of every queue slice by direction
-                               octant and origin cell (6 more bytes per path; same results, faster
+                               octant and origin cell (38 more bytes per path; same results, faster
                                traversal on coherent scenes such as closed rooms) */
 } rt_render_params;
 
@@ -301,7 +301,7 @@ typedef struct rt_pool_tuning {
     int tile_width;      /* per-pixel frame kernels on tile_width x (256 / tile_width) tiles: 8..128
                             (the first a-trous pass always runs on 16 x 16 LDS tiles) */
     int ray_sort;        /* 1 = reorder extension rays before extend (rt_render_params.ray_sort);
-                            6 bytes per path of device memory beyond the queue budget */
+                            38 bytes per path of device memory beyond the queue budget */
 } rt_pool_tuning;
 
 typedef struct rt_tune_report {
