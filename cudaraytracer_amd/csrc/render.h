@@ -50,7 +50,7 @@ struct RenderSettings {
     int rr_start_depth = 0;
     uint32_t seed = 1234u;
     int batch_spp = 0;               // samples per pass (0: as many as the queue budget allows, up to 64)
-    size_t queue_budget_bytes = 0;   // 0: 8 GiB
+    size_t queue_budget_bytes = 0;   // 0: 16 GiB
     int flush_survivors = 0;         // 1: paths alive after the last bounce keep their radiance
     int env_alias = 0;               // 1: environment sampled with alias tables (not the reference's CDF search)
     int use_graphs = 1;              // replay passes as captured HIP graphs

@@ -1799,7 +1799,8 @@ Renderer::Renderer(const GpuScene& scene, const HostScene& host, const RenderSet
     I.cam = host.camera;
     // Paths per pass: all requested samples of every pixel when they fit the
     // queue budget.
-    size_t budget = s.queue_budget_bytes ? s.queue_budget_bytes : ((size_t)8 << 30);
+    // 16 GiB by default: a 1920x1080 job of up to 36 samples per pixel runs as one pass.
+    size_t budget = s.queue_budget_bytes ? s.queue_budget_bytes : ((size_t)16 << 30);
     const size_t per_path = kQueueBytesPerPath + (s.ray_sort ? kSortBytesPerPath : 0);
     int bmax = (int)std::max<size_t>(1, budget / (per_path * (size_t)I.wh));
     int want = s.batch_spp > 0 ? s.batch_spp : 64;

@@ -139,7 +139,7 @@ typedef struct rt_render_params {
     int rr_start_depth;     /* < 0: scene value; 0 = roulette from the first bounce (reference) */
     int64_t seed;           /* < 0: scene value */
     int batch_spp;          /* samples per pass; 0 = automatic */
-    int64_t queue_budget_bytes; /* 0 = 8 GiB */
+    int64_t queue_budget_bytes; /* 0 = 16 GiB (at most 64 samples of every pixel per pass) */
     int flush_survivors;    /* 0 = reference: paths alive after max_bounces contribute nothing */
     int profile;            /* 1 = per-stage HIP event timing in rt_stats */
     int env_sampling;       /* 0 = CDF binary search (reference, texture.cu:112-153); 1 = alias tables */
