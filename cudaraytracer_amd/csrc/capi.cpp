@@ -770,6 +770,15 @@ int rt_scene_get_texture(const rt_scene* s, int index, int* w, int* h, uint8_t* 
     return RT_OK;
 }
 
+int rt_scene_get_envmap(const rt_scene* s, int* w, int* h, float* rgb) {
+    if (!s || !w || !h) return fail(RT_ERR_INVALID_ARG, "null argument");
+    const rt::EnvMap& e = s->host.env;
+    *w = e.width;
+    *h = e.height;
+    if (rgb && !e.rgb.empty()) std::memcpy(rgb, e.rgb.data(), e.rgb.size() * sizeof(float));
+    return RT_OK;
+}
+
 int rt_texture_lookup(rt_scene* s, int index, const float* uv, int n, float* rgb_out) {
     if (!s || !uv || !rgb_out || n < 0) return fail(RT_ERR_INVALID_ARG, "invalid argument");
     if (n == 0) return RT_OK;

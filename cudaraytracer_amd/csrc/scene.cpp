@@ -458,8 +458,133 @@ std::vector<LoadedMesh> load_obj(const std::string& path, const Mat4& xf, const 
 }
 
 // ---------------------------------------------------------------- env ----
-// Environment maps: Radiance RGBE (.hdr) or portable float map (.pfm).
-// Stored with row 0 = top of the image (the orientation stb_image returns).
+// Radiance RGBE (.hdr) as the reference reads it: HDRTexture::Load calls
+// stbi_loadf(path, &w, &h, &c, 4) (Src/Core/texture.cu:17-25), i.e. the
+// vendored stb_image 2.30's Radiance reader (ThirdParty/stb_image/stb_image.h:
+// 7083-7288).  Its rules, kept here:
+//  - the file starts with "#?RADIANCE\n" or "#?RGBE\n"; header lines (at most
+//    1023 characters kept, the rest of a longer line skipped) run to an empty
+//    line and one of them must be "FORMAT=32-bit_rle_rgbe";
+//  - the resolution line is "-Y <h>" then spaces then "+X <w>" (strtol numbers);
+//    sizes above 2^24, negative sizes and w*h*16 bytes above INT_MAX are refused;
+//  - widths below 8 or from 32768 are stored flat (4 bytes per pixel);
+//    otherwise each scanline opens with 2, 2, w >> 8, w & 255 and holds the four
+//    channels one after another as runs (count > 128: count - 128 copies of the
+//    next byte) and literal dumps (count bytes); a zero count, a count past the
+//    end of the row or another length is an error;
+//  - a scanline whose first bytes are not 2, 2 and a length byte below 128 is
+//    not run-length encoded: those four bytes are pixel 0 of ROW 0, and the rest
+//    of the file is read flat from row 0, pixel 1 on, whatever row it occurred
+//    in (stb's goto into its flat loop);
+//  - a texel is byte * 2^(e - 136) per channel when e != 0, else 0 (exact in
+//    float32: a multiple of 2^-149 below 2^128).
+// Deviations, only where stb's result is unspecified or empty: data that ends
+// early is an error (stb reads zeros or leaves texels unset), and so is a zero
+// width or height (stb returns an empty image the reference cannot sample).
+// Row 0 is the first scanline (the orientation stb_image returns).
+void parse_rgbe(const std::string& data, const std::string& path, EnvMap& env) {
+    const uint8_t* d = (const uint8_t*)data.data();
+    const size_t size = data.size();
+    size_t p = 0;
+    auto fail = [&](const char* what) { throw std::runtime_error(std::string(what) + " in RGBE file " + path); };
+    auto get8 = [&]() -> uint8_t {
+        if (p >= size) fail("truncated data");
+        return d[p++];
+    };
+    // stbi__hdr_gettoken: up to '\n' or the end of the file, 1023 characters kept.
+    auto token = [&]() {
+        std::string t;
+        if (p >= size) return t;
+        char c = (char)d[p++];
+        while (p < size && c != '\n') {
+            t.push_back(c);
+            if (t.size() == 1023) {
+                while (p < size && d[p++] != '\n') {
+                }
+                break;
+            }
+            c = (char)d[p++];
+        }
+        return t;
+    };
+    auto starts = [&](const char* sig) { return data.compare(0, std::strlen(sig), sig) == 0; };
+    if (!starts("#?RADIANCE\n") && !starts("#?RGBE\n")) fail("missing #?RADIANCE or #?RGBE signature");
+    token();
+    bool format_ok = false;
+    for (;;) {
+        std::string t = token();
+        if (t.empty()) break;
+        if (t == "FORMAT=32-bit_rle_rgbe") format_ok = true;
+    }
+    if (!format_ok) fail("unsupported format (no FORMAT=32-bit_rle_rgbe line)");
+    std::string res = token();
+    if (res.compare(0, 3, "-Y ") != 0) fail("unsupported data layout");
+    const char* q = res.c_str() + 3;
+    char* after = nullptr;
+    long h = std::strtol(q, &after, 10);
+    q = after;
+    while (*q == ' ') ++q;
+    if (std::strncmp(q, "+X ", 3) != 0) fail("unsupported data layout");
+    long w = std::strtol(q + 3, nullptr, 10);
+    if (h > (1L << 24) || w > (1L << 24) || h < 0 || w < 0 || (w > 0 && h > INT32_MAX / 16 / w))
+        fail("image size out of range");
+    if (w == 0 || h == 0) fail("zero-sized image");
+    const int W = (int)w, H = (int)h;
+    env.width = W;
+    env.height = H;
+    env.rgb.assign((size_t)W * H * 3, 0.f);
+    auto put = [&](size_t texel, const uint8_t* e4) {
+        float* o = &env.rgb[texel * 3];
+        if (e4[3] != 0) {
+            float f = std::ldexp(1.0f, (int)e4[3] - (128 + 8));
+            o[0] = (float)e4[0] * f;
+            o[1] = (float)e4[1] * f;
+            o[2] = (float)e4[2] * f;
+        } else {
+            o[0] = o[1] = o[2] = 0.f;
+        }
+    };
+    auto flat_from = [&](size_t texel) {
+        const size_t n = (size_t)W * H;
+        if (size - p < (n - texel) * 4) fail("truncated data");
+        for (; texel < n; ++texel, p += 4) put(texel, d + p);
+    };
+    if (W < 8 || W >= 32768) {
+        flat_from(0);
+        return;
+    }
+    std::vector<uint8_t> scan((size_t)W * 4);
+    for (int y = 0; y < H; ++y) {
+        uint8_t c1 = get8(), c2 = get8(), len = get8();
+        if (c1 != 2 || c2 != 2 || (len & 0x80)) {
+            uint8_t first[4] = {c1, c2, len, get8()};
+            put(0, first);
+            flat_from(1);
+            return;
+        }
+        int n = (len << 8) | get8();
+        if (n != W) fail("invalid decoded scanline length");
+        for (int c = 0; c < 4; ++c) {
+            int x = 0;
+            while (x < W) {
+                int count = get8();
+                if (count > 128) {
+                    uint8_t value = get8();
+                    count -= 128;
+                    if (count == 0 || count > W - x) fail("bad run-length data");
+                    for (int k = 0; k < count; ++k) scan[(size_t)(x++) * 4 + c] = value;
+                } else {
+                    if (count == 0 || count > W - x) fail("bad run-length data");
+                    for (int k = 0; k < count; ++k) scan[(size_t)(x++) * 4 + c] = get8();
+                }
+            }
+        }
+        for (int x = 0; x < W; ++x) put((size_t)y * W + x, &scan[(size_t)x * 4]);
+    }
+}
+
+// Environment maps: Radiance RGBE (.hdr, parse_rgbe) or portable float map
+// (.pfm, this project's lossless addition).  Stored with row 0 = top.
 void load_env(const std::string& path, EnvMap& env) {
     std::string data = read_file(path);
     if (data.size() >= 2 && data[0] == 'P' && (data[1] == 'F' || data[1] == 'f')) {
@@ -497,62 +622,7 @@ void load_env(const std::string& path, EnvMap& env) {
         }
         return;
     }
-    // Radiance RGBE with new-style RLE scanlines.
-    size_t p = 0;
-    auto getline_ = [&]() {
-        size_t b = p;
-        while (p < data.size() && data[p] != '\n') ++p;
-        std::string l = data.substr(b, p - b);
-        if (p < data.size()) ++p;
-        return l;
-    };
-    std::string l = getline_();
-    if (l.rfind("#?", 0) != 0) throw std::runtime_error("unsupported environment map format: " + path);
-    while (p < data.size()) {
-        l = getline_();
-        if (l.empty()) break;
-    }
-    l = getline_();
-    int w = 0, h = 0;
-    char ya[3] = {0}, xa[3] = {0};
-    if (std::sscanf(l.c_str(), "%2s %d %2s %d", ya, &h, xa, &w) != 4 || std::strcmp(ya, "-Y") != 0 || std::strcmp(xa, "+X") != 0)
-        throw std::runtime_error("unsupported RGBE orientation in " + path);
-    env.width = w;
-    env.height = h;
-    env.rgb.assign((size_t)w * h * 3, 0.f);
-    std::vector<uint8_t> scan((size_t)w * 4);
-    const uint8_t* d = (const uint8_t*)data.data();
-    for (int y = 0; y < h; ++y) {
-        if (p + 4 > data.size()) throw std::runtime_error("truncated RGBE: " + path);
-        if (w >= 8 && w < 32768 && d[p] == 2 && d[p + 1] == 2 && ((d[p + 2] << 8) | d[p + 3]) == w) {
-            p += 4;
-            for (int c = 0; c < 4; ++c) {
-                int x = 0;
-                while (x < w) {
-                    if (p >= data.size()) throw std::runtime_error("truncated RGBE: " + path);
-                    int cnt = d[p++];
-                    if (cnt > 128) {
-                        cnt -= 128;
-                        if (x + cnt > w || p >= data.size()) throw std::runtime_error("bad RGBE run: " + path);
-                        uint8_t val = d[p++];
-                        for (int k = 0; k < cnt; ++k) scan[(size_t)(x++) * 4 + c] = val;
-                    } else {
-                        if (cnt == 0 || x + cnt > w || p + cnt > data.size()) throw std::runtime_error("bad RGBE run: " + path);
-                        for (int k = 0; k < cnt; ++k) scan[(size_t)(x++) * 4 + c] = d[p++];
-                    }
-                }
-            }
-        } else {
-            if (p + (size_t)w * 4 > data.size()) throw std::runtime_error("truncated RGBE: " + path);
-            std::memcpy(scan.data(), d + p, (size_t)w * 4);
-            p += (size_t)w * 4;
-        }
-        for (int x = 0; x < w; ++x) {
-            const uint8_t* px = &scan[(size_t)x * 4];
-            float f = px[3] ? std::ldexp(1.0f, (int)px[3] - (128 + 8)) : 0.f;
-            for (int c = 0; c < 3; ++c) env.rgb[((size_t)y * w + x) * 3 + c] = px[3] ? ((float)px[c] + 0.5f) * f : 0.f;
-        }
-    }
+    parse_rgbe(data, path, env);
 }
 
 vec3 vec_of(const Json& a, const char* what) {

@@ -359,6 +359,58 @@ def write_pfm(path: str, rgb: np.ndarray) -> None:
         f.write(np.ascontiguousarray(rgb[::-1].astype("<f4")).tobytes())
 
 
+def _rle_channel(vals: np.ndarray) -> bytes:
+    """One channel of a new-style RGBE scanline: runs of 3..127 equal bytes as
+    (128 + count, value), everything else as literal dumps of at most 128."""
+    out = bytearray()
+    n = len(vals)
+    # start of each stretch of equal values and its length
+    edges = np.flatnonzero(np.diff(vals.astype(np.int16))) + 1
+    starts = np.concatenate([[0], edges])
+    lens = np.diff(np.concatenate([starts, [n]]))
+    lit = bytearray()
+
+    def flush():
+        for k in range(0, len(lit), 128):
+            chunk = lit[k:k + 128]
+            out.append(len(chunk))
+            out.extend(chunk)
+        lit.clear()
+
+    for st, ln in zip(starts.tolist(), lens.tolist()):
+        v = int(vals[st])
+        while ln >= 3:
+            run = min(ln, 127)
+            flush()
+            out += bytes((128 + run, v))
+            ln -= run
+        lit.extend(bytes([v]) * ln)
+    flush()
+    return bytes(out)
+
+
+def write_hdr(path: str, rgb: np.ndarray) -> None:
+    """Radiance RGBE file (the reference's environment-map format, read with
+    stbi_loadf): Ward's shared-exponent encoding of (h, w, 3) float RGB, row 0 =
+    first scanline, new-style run-length scanlines (widths 8..32767) or flat
+    pixels otherwise."""
+    h, w, _ = rgb.shape
+    v = rgb.max(axis=2).astype(np.float64)
+    m, e = np.frexp(v)
+    ok = v > 1e-32
+    scale = np.where(ok, m * 256.0 / np.where(ok, v, 1.0), 0.0)
+    px = np.zeros((h, w, 4), np.uint8)
+    px[..., :3] = np.clip(np.floor(rgb * scale[..., None]), 0, 255).astype(np.uint8)
+    px[..., 3] = np.where(ok, e + 128, 0).astype(np.uint8)
+    with open(path, "wb") as f:
+        f.write(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n" + f"-Y {h} +X {w}\n".encode())
+        if w < 8 or w >= 32768:
+            f.write(px.tobytes())
+            return
+        for y in range(h):
+            f.write(bytes((2, 2, w >> 8, w & 255)) + b"".join(_rle_channel(px[y, :, c]) for c in range(4)))
+
+
 def sky_env(width: int = 2048, height: int = 1024, sun_dir=(0.35, 0.6, -0.72), sun_peak: float = 1e4,
             sun_radius_deg: float = 0.5) -> np.ndarray:
     """Procedural HDR sky: zenith-to-horizon gradient, dark ground, Gaussian sun
@@ -434,7 +486,10 @@ def burley_from_coefficients(sigma_a, sigma_s):
 
 def terrain_env(seed: int = 3, grid: int = 1001, size: float = 100.0, env_w: int = 2048, env_h: int = 1024,
                 width: int = 1920, height: int = 1080, spp: int = 32, depth: int = 16,
-                sphere_subdiv: int = 4) -> SceneSpec:
+                sphere_subdiv: int = 4, env_file: str = "sky.pfm") -> SceneSpec:
+    """Terrain + subsurface sphere under a procedural HDR sky.  env_file picks
+    the map's format by extension: .pfm keeps FP32 texels, .hdr is the
+    reference's Radiance RGBE."""
     rng = np.random.default_rng(seed)
     terrain = terrain_mesh(grid, size, height=size * 0.06, seed=seed)
     V, F = icosphere(sphere_subdiv)
@@ -454,7 +509,7 @@ def terrain_env(seed: int = 3, grid: int = 1001, size: float = 100.0, env_w: int
     ]
     spec = SceneSpec(camera=dict(_camera(width, height, spp, depth), direction=[0.35, 0.45, -1.0]),
                      materials=mats, meshes=[terrain, sphere],
-                     lights=[{"name": "sky", "type": "env", "path": "sky.pfm"}])
+                     lights=[{"name": "sky", "type": "env", "path": env_file}])
     spec.env = sky_env(env_w, env_h)
     return spec
 
@@ -539,7 +594,8 @@ def write_scene(spec: SceneSpec, out_dir: str, name: str = "scene") -> str:
     if env is not None:
         for L in spec.lights:
             if L.get("type") == "env":
-                write_pfm(os.path.join(out_dir, L["path"]), env)
+                writer = write_hdr if L["path"].lower().endswith(".hdr") else write_pfm
+                writer(os.path.join(out_dir, L["path"]), env)
     doc = {"camera": spec.camera, "textures": spec.textures, "materials": spec.materials, "objects": objects,
            "lights": spec.lights}
     path = os.path.join(out_dir, f"{name}.json")

@@ -352,6 +352,11 @@ int rt_scene_get_bvh(const rt_scene* s, int from_packed, void* nodes_out, int32_
    Texture::Load uploads it (Src/Core/texture.cu:207-240): RGBA8, row 0 = top.  *w, *h
    receive its size (0 x 0 for an entry without 'path' or 'name'); rgba may be NULL. */
 int rt_scene_get_texture(const rt_scene* s, int index, int* w, int* h, uint8_t* rgba);
+/* The scene's decoded environment map as HDRTexture::Load receives it (Src/Core/texture.cu:
+   17-25: stbi_loadf's RGB, row 0 = the file's first scanline; a PFM map is read losslessly):
+   3 floats per texel.  *w, *h receive its size (0 x 0 without an environment light); rgb may
+   be NULL. */
+int rt_scene_get_envmap(const rt_scene* s, int* w, int* h, float* rgb);
 /* Diffuse-texture albedo on the GPU at n (u, v) pairs (2 floats each), as shading
    reads it (Texture::Sample, Src/Core/texture.cu:242-249): 3 floats per pair. */
 int rt_texture_lookup(rt_scene* s, int index, const float* uv, int n, float* rgb_out);

@@ -273,7 +273,124 @@ def _read_env(path: str):
         if ch == 1:
             arr = np.repeat(arr, 3, axis=2)
         return w, h, np.ascontiguousarray(arr)
-    raise ValueError(f"the oracle reads PFM environment maps only: {path}")
+    return _read_rgbe(data, path)
+
+
+def _read_rgbe(data: bytes, path: str):
+    """Radiance RGBE as HDRTexture::Load reads it (Src/Core/texture.cu:17-25,
+    stbi_loadf(..., 4)): the vendored stb_image 2.30 reader
+    (ThirdParty/stb_image/stb_image.h:7083-7288) restated.  Returns (w, h, rgb)
+    with row 0 = the first scanline; raises ValueError where stb fails, and on
+    data that ends early or a zero size (stb's output is unspecified there)."""
+    p = 0
+
+    def get8():  # stbi__get8
+        nonlocal p
+        if p >= len(data):
+            raise ValueError(f"truncated RGBE data: {path}")
+        p += 1
+        return data[p - 1]
+
+    def token():  # stbi__hdr_gettoken (stb_image.h:7108-7128)
+        nonlocal p
+        t = bytearray()
+        if p >= len(data):
+            return bytes(t)
+        c = data[p]
+        p += 1
+        while p < len(data) and c != 10:
+            t.append(c)
+            if len(t) == 1023:
+                while p < len(data):
+                    p += 1
+                    if data[p - 1] == 10:
+                        break
+                break
+            c = data[p]
+            p += 1
+        return bytes(t)
+
+    if not (data.startswith(b"#?RADIANCE\n") or data.startswith(b"#?RGBE\n")):  # stbi__hdr_test
+        raise ValueError(f"not a Radiance RGBE file: {path}")
+    token()
+    valid = False
+    while True:  # header lines up to an empty one (7179-7183)
+        t = token()
+        if not t:
+            break
+        valid |= t == b"FORMAT=32-bit_rle_rgbe"
+    if not valid:
+        raise ValueError(f"unsupported RGBE format: {path}")
+    res = token().decode("latin-1")
+    if not res.startswith("-Y "):  # 7189-7196
+        raise ValueError(f"unsupported RGBE layout: {path}")
+
+    def strtol(s: str):
+        s2 = s.lstrip(" \t\n\v\f\r")
+        k = 0
+        if k < len(s2) and s2[k] in "+-":
+            k += 1
+        while k < len(s2) and s2[k].isdigit():
+            k += 1
+        digits = s2[:k]
+        val = int(digits) if digits not in ("", "+", "-") else 0
+        return val, (s2[k:] if digits not in ("", "+", "-") else s)
+
+    h, rest = strtol(res[3:])
+    rest = rest.lstrip(" ")
+    if not rest.startswith("+X "):
+        raise ValueError(f"unsupported RGBE layout: {path}")
+    w, _ = strtol(rest[3:])
+    if h > (1 << 24) or w > (1 << 24) or h < 0 or w < 0 or (w > 0 and h > (2**31 - 1) // 16 // w):
+        raise ValueError(f"RGBE size out of range: {path}")
+    if w == 0 or h == 0:
+        raise ValueError(f"zero-sized RGBE image: {path}")
+    px = np.zeros((h * w, 4), np.uint8)
+
+    def flat_from(texel):  # the flat loop (7208-7216)
+        nonlocal p
+        n = (h * w - texel) * 4
+        if len(data) - p < n:
+            raise ValueError(f"truncated RGBE data: {path}")
+        px[texel:] = np.frombuffer(data, np.uint8, n, p).reshape(-1, 4)
+        p += n
+
+    if w < 8 or w >= 32768:
+        flat_from(0)
+    else:
+        for y in range(h):  # run-length scanlines (7218-7282)
+            c1, c2, ln = get8(), get8(), get8()
+            if c1 != 2 or c2 != 2 or (ln & 0x80):
+                px[0] = (c1, c2, ln, get8())  # pixel 0 of row 0, then flat from row 0, pixel 1
+                flat_from(1)
+                break
+            if ((ln << 8) | get8()) != w:
+                raise ValueError(f"invalid RGBE scanline length: {path}")
+            row = np.zeros((w, 4), np.uint8)
+            for c in range(4):
+                x = 0
+                while x < w:
+                    count = get8()
+                    if count > 128:
+                        value = get8()
+                        count -= 128
+                        if count == 0 or count > w - x:
+                            raise ValueError(f"bad RGBE run-length data: {path}")
+                        row[x:x + count, c] = value
+                    else:
+                        if count == 0 or count > w - x:
+                            raise ValueError(f"bad RGBE run-length data: {path}")
+                        if len(data) - p < count:
+                            raise ValueError(f"truncated RGBE data: {path}")
+                        row[x:x + count, c] = np.frombuffer(data, np.uint8, count, p)
+                        p += count
+                    x += count
+            px[y * w:(y + 1) * w] = row
+    # stbi__hdr_convert (7131-7155): byte * 2^(e - 136), zero when e == 0.
+    e = px[:, 3].astype(np.int32)
+    f = np.where(e != 0, np.ldexp(np.float32(1.0), e - 136).astype(np.float32), np.float32(0.0))
+    rgb = px[:, :3].astype(np.float32) * f[:, None].astype(np.float32)
+    return w, h, np.ascontiguousarray(rgb.reshape(h, w, 3).astype(np.float32))
 
 
 @dataclass
