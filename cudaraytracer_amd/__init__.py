@@ -69,7 +69,7 @@ EXPORTS = (
     "rt_renderer_create", "rt_renderer_destroy", "rt_render", "rt_render_counted", "rt_synchronize",
     "rt_reset", "rt_reset_stats", "rt_read_accum", "rt_read_rgba8", "rt_stats", "rt_last_error",
     "rt_scene_refit", "rt_scene_get_triangles", "rt_scene_get_meshes", "rt_scene_get_bvh", "rt_scene_get_texture",
-    "rt_scene_get_envmap",
+    "rt_scene_get_envmap", "rt_scene_get_materials",
     "rt_texture_lookup", "rt_trace_rays", "rt_device_math", "rt_save_image", "rt_read_gather", "rt_read_denoised", "rt_read_gbuffer",
     "rt_denoise_params_default", "rt_get_camera", "rt_set_camera", "rt_render_frame", "rt_reset_history",
     "rt_read_frame", "rt_pool_create", "rt_pool_destroy", "rt_pool_add_scene", "rt_pool_submit", "rt_pool_step",
@@ -234,6 +234,7 @@ def lib() -> ctypes.CDLL:
     L.rt_scene_get_texture.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_uint8)]
     L.rt_texture_lookup.argtypes = [vp, ip, fp, ip, fp]
+    L.rt_scene_get_materials.argtypes = [vp, ctypes.POINTER(ctypes.c_int), vp]
     L.rt_scene_get_envmap.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), fp]
     L.rt_save_image.argtypes = [vp, ip, ctypes.c_char_p]
     L.rt_read_gather.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_uint8)]
@@ -365,6 +366,16 @@ class Scene:
         out = np.zeros((h.value, w.value, 4), np.uint8)
         _check(lib().rt_scene_get_texture(self._h, index, ctypes.byref(w), ctypes.byref(h),
                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return out
+
+    def materials(self) -> np.ndarray:
+        """Material records in "materials" order: (n, 16) float32 words (type, basecolor,
+        emission, roughness, metallic, transmission, ior, scatter distance, texture, valid;
+        type, texture and valid are int32 bit patterns)."""
+        n = ctypes.c_int(0)
+        _check(lib().rt_scene_get_materials(self._h, ctypes.byref(n), None))
+        out = np.zeros((n.value, 16), np.float32)
+        _check(lib().rt_scene_get_materials(self._h, ctypes.byref(n), out.ctypes.data_as(ctypes.c_void_p)))
         return out
 
     def envmap(self) -> np.ndarray:

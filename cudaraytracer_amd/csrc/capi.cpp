@@ -770,6 +770,14 @@ int rt_scene_get_texture(const rt_scene* s, int index, int* w, int* h, uint8_t* 
     return RT_OK;
 }
 
+int rt_scene_get_materials(const rt_scene* s, int* n, void* out) {
+    if (!s || !n) return fail(RT_ERR_INVALID_ARG, "null argument");
+    const std::vector<rt::MaterialRec>& m = s->host.materials;
+    *n = (int)m.size();
+    if (out && !m.empty()) std::memcpy(out, m.data(), m.size() * sizeof(rt::MaterialRec));
+    return RT_OK;
+}
+
 int rt_scene_get_envmap(const rt_scene* s, int* w, int* h, float* rgb) {
     if (!s || !w || !h) return fail(RT_ERR_INVALID_ARG, "null argument");
     const rt::EnvMap& e = s->host.env;

@@ -151,6 +151,10 @@ struct Parser {
 
 Json Json::parse(const std::string& text) {
     Parser ps{text};
+    // A leading UTF-8 byte-order mark is skipped, as nlohmann's lexer does
+    // (reference: ThirdParty/Json/json.hpp:8858-8884); some of the reference's
+    // scene files start with one.
+    if (text.compare(0, 3, "\xEF\xBB\xBF") == 0) ps.p = 3;
     Json v = ps.value();
     ps.ws();
     if (ps.p != text.size()) ps.fail("trailing characters");

@@ -423,7 +423,7 @@ class SceneArrays:
 
 def parse_scene(json_path: str) -> SceneArrays:
     base = os.path.dirname(os.path.abspath(json_path))
-    with open(json_path) as f:
+    with open(json_path, encoding="utf-8-sig") as f:  # a leading BOM is skipped, as nlohmann does
         doc = json.load(f)
     cam = doc["camera"]
     W, H = (cam["resolution"] if "resolution" in cam else (800, 600))
