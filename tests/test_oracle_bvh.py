@@ -547,11 +547,11 @@ def test_gpu_build_and_trace_follow_tie_rules(tmp_path, oracle_lib, name):
     rn, ro = rs.bvh()
     assert np.array_equal(nodes, rn) and np.array_equal(order, ro)
     rays = _tie_rays() if name != "clusters" else _rays(4000, 17)
-    hits, counts = s.trace(rays)
+    gt, gb1, gb2, gprim, counts = s.trace(rays)
     t, b1, b2, prim, rc = rs.trace(rays)
-    assert np.array_equal(hits[:, 3].view(np.int32), prim)
-    assert np.array_equal(hits[:, 0].view(np.int32), t.view(np.int32))
-    assert np.array_equal(hits[:, 1:3].view(np.int32), np.stack([b1, b2], 1).view(np.int32))
+    assert np.array_equal(gprim, prim)
+    assert np.array_equal(gt.view(np.int32), t.view(np.int32))
+    assert np.array_equal(gb1.view(np.int32), b1.view(np.int32)) and np.array_equal(gb2.view(np.int32), b2.view(np.int32))
     assert np.array_equal(counts, rc)
     if name == "hit_tie":
         assert (prim >= 0).sum() > 20
