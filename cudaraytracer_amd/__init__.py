@@ -69,7 +69,7 @@ EXPORTS = (
     "rt_renderer_create", "rt_renderer_destroy", "rt_render", "rt_render_counted", "rt_synchronize",
     "rt_reset", "rt_reset_stats", "rt_read_accum", "rt_read_rgba8", "rt_stats", "rt_last_error",
     "rt_scene_refit", "rt_scene_get_triangles", "rt_scene_get_meshes", "rt_scene_get_bvh", "rt_scene_get_texture",
-    "rt_scene_get_envmap", "rt_scene_get_materials",
+    "rt_scene_get_envmap", "rt_scene_get_materials", "rt_trace_rays_noalloc",
     "rt_texture_lookup", "rt_trace_rays", "rt_device_math", "rt_save_image", "rt_read_gather", "rt_read_denoised", "rt_read_gbuffer",
     "rt_denoise_params_default", "rt_get_camera", "rt_set_camera", "rt_render_frame", "rt_reset_history",
     "rt_read_frame", "rt_pool_create", "rt_pool_destroy", "rt_pool_add_scene", "rt_pool_submit", "rt_pool_step",
@@ -241,6 +241,7 @@ def lib() -> ctypes.CDLL:
     L.rt_read_denoised.argtypes = [vp, fp]
     L.rt_read_gbuffer.argtypes = [vp, ip, vp]
     L.rt_trace_rays.argtypes = [vp, fp, ip, fp, ctypes.POINTER(ctypes.c_uint32)]
+    L.rt_trace_rays_noalloc.argtypes = [vp, fp, ip, fp, ctypes.POINTER(ctypes.c_uint32)]
     L.rt_device_math.argtypes = [ip, fp, fp, fp, ip]
     L.rt_denoise_params_default.argtypes = [ctypes.POINTER(DenoiseParams)]
     L.rt_denoise_params_default.restype = None

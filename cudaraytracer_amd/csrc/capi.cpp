@@ -16,6 +16,7 @@
 #include "scene.h"
 
 static_assert(RT_NUM_STAGES == rt::kNumStages, "stage ids of rt.h and the engine differ");
+static_assert(RT_NOALLOC_MAX_RAYS == rt::kNoAllocMaxRays, "rt_trace_rays_noalloc limits of rt.h and the engine differ");
 
 struct rt_scene {
     rt::HostScene host;
@@ -803,6 +804,21 @@ int rt_trace_rays(rt_scene* s, const float* rays, int n, float* hits, uint32_t* 
     return guard([&]() {
         ensure_gpu(s);
         rt::trace_batch(*s->gpu, rays, n, hits, counts);
+        return RT_OK;
+    });
+}
+
+int rt_trace_rays_noalloc(rt_scene* s, const float* rays, int n, float* hits, uint32_t* counts) {
+    if (!s || !rays || !hits || n < 0) return fail(RT_ERR_INVALID_ARG, "invalid argument");
+    if (n > RT_NOALLOC_MAX_RAYS)
+        return fail(RT_ERR_INVALID_ARG, "rt_trace_rays_noalloc takes at most " + std::to_string(RT_NOALLOC_MAX_RAYS) +
+                                            " rays per call, got " + std::to_string(n));
+    if (!s->gpu)
+        return fail(RT_ERR_INVALID_ARG,
+                    "the scene is not resident on the device: create a renderer on it or call rt_trace_rays first");
+    if (n == 0) return RT_OK;
+    return guard([&]() {
+        rt::trace_batch_noalloc(*s->gpu, rays, n, hits, counts);
         return RT_OK;
     });
 }

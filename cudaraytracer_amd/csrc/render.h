@@ -244,6 +244,13 @@ ServeReport serve(Pool& pool, const std::vector<ServeRequest>& reqs, bool batche
 // Closest hit for a batch of rays laid out as (o.xyz, tmin, d.xyz, tmax):
 // out_hits = (t, b1, b2, prim bits) per ray, out_counts = (nodes, tris) per ray.
 void trace_batch(const GpuScene& scene, const float* rays, int n, float* out_hits, uint32_t* out_counts);
+// trace_batch without allocating device memory: rays, hits and counts are
+// staged in a fixed device array of kNoAllocMaxRays rays (56 bytes each) on
+// the current device.  Throws std::invalid_argument for n > kNoAllocMaxRays.
+// Concurrent calls on one device share that array and race.
+constexpr int kNoAllocMaxRays = 1 << 18;
+constexpr int kMaxDevices = 64;
+void trace_batch_noalloc(const GpuScene& scene, const float* rays, int n, float* out_hits, uint32_t* out_counts);
 // Albedo of diffuse texture `tex` at n (u, v) pairs, on the GPU; 3 floats per pair.
 void texture_lookup(const GpuScene& scene, const HostScene& host, int tex, const float* uv, int n, float* rgb_out);
 void math_selftest(int op, const float* a, const float* b, float* out, int n);

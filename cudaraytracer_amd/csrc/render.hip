@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -1963,6 +1964,52 @@ void trace_batch(const GpuScene& scene, const float* rays, int n, float* out_hit
     HIP_OK(hipDeviceSynchronize());
     HIP_OK(hipMemcpy(out_hits, o.p, (size_t)n * 16, hipMemcpyDeviceToHost));
     if (out_counts) HIP_OK(hipMemcpy(out_counts, c.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+}
+
+// Staging of trace_batch_noalloc: the rays (32 B each), then the hits (16 B)
+// and the traversal counts (8 B) of up to kNoAllocMaxRays rays.  Each GPU the
+// library runs on holds its own copy until the process ends.
+static __device__ __align__(256) unsigned char scratch[(size_t)kNoAllocMaxRays * 56];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device
+// (the array's name is only a host-side handle; kernels and copies need this).
+void* scratch_on_current_device() {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw std::runtime_error(std::string("HIP error: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+void trace_batch_noalloc(const GpuScene& scene, const float* rays, int n, float* out_hits, uint32_t* out_counts) {
+    if (n < 0 || n > kNoAllocMaxRays)
+        throw std::invalid_argument("rt_trace_rays_noalloc takes at most " + std::to_string(kNoAllocMaxRays) +
+                                    " rays per call, got " + std::to_string(n));
+    if (n == 0) return;
+    unsigned char* base = (unsigned char*)scratch_on_current_device();
+    float4* r = (float4*)base;                               // 2 float4 per ray
+    float4* o = (float4*)(base + (size_t)n * 32);            // hits
+    uint2* c = (uint2*)(base + (size_t)n * 48);              // counts
+    HIP_OK(hipMemcpy(r, rays, (size_t)n * 32, hipMemcpyHostToDevice));
+    int grid = std::max(1, std::min((n + kBlock - 1) / kBlock, 4096));
+    const size_t lds = (size_t)scene.max_stack() * kBlock * sizeof(uint32_t);
+    k_trace_batch<<<grid, kBlock, lds>>>(scene.dev(), r, n, o, c);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(out_hits, o, (size_t)n * 16, hipMemcpyDeviceToHost));
+    if (out_counts) HIP_OK(hipMemcpy(out_counts, c, (size_t)n * 8, hipMemcpyDeviceToHost));
 }
 
 void texture_lookup(const GpuScene& scene, const HostScene& host, int tex, const float* uv, int n, float* rgb_out) {

@@ -370,6 +370,15 @@ int rt_texture_lookup(rt_scene* s, int index, const float* uv, int n, float* rgb
 /* Closest hit on the GPU for n rays given as 8 floats (o.xyz, tmin, d.xyz, tmax);
    hits_out: (t, b1, b2, prim-as-int-bits) per ray; counts_out: (node visits, tri tests). */
 int rt_trace_rays(rt_scene* s, const float* rays, int n, float* hits_out, uint32_t* counts_out);
+/* rt_trace_rays without allocating device memory.  No counterpart in the reference: for callers
+   that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  Rays, hits and counts are staged in one fixed device array of the library
+   (RT_NOALLOC_MAX_RAYS rays, 56 bytes each, about 14 MiB per GPU, held until the process ends)
+   on the current device; more rays per call is RT_ERR_INVALID_ARG.  The scene must already be
+   resident on the device (a renderer, pool or rt_trace_rays call has put it there), else
+   RT_ERR_INVALID_ARG.  Calls that overlap on one GPU share that array and race. */
+#define RT_NOALLOC_MAX_RAYS 262144
+int rt_trace_rays_noalloc(rt_scene* s, const float* rays, int n, float* hits_out, uint32_t* counts_out);
 /* Device arithmetic check (op: 0 div, 1 sqrt, 2 sin, 3 cos, 4 acos, 5 exp, 6 log, 7 atan2, 8 philox, 9 rcp, 10 pow). */
 int rt_device_math(int op, const float* a, const float* b, float* out, int n);
 
