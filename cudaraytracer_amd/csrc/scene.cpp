@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <iterator>
+#include <limits>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -58,30 +61,59 @@ Mat4 rotate_axis(int axis, float deg) {
     else { r.m[0][0] = c; r.m[0][1] = -s; r.m[1][0] = s; r.m[1][1] = c; }
     return r;
 }
-// General inverse by Gauss-Jordan elimination in double, rounded to float.
+// Matrix4f::inverse as the reference's Eigen 3.4.0 computes it on x86-64
+// (Eigen/src/LU/arch/InverseSize4.h, the SSE path: inverse = adjugate / det
+// formed block-wise from the four 2x2 blocks), written out lane by lane in
+// FP32 with the same products, sums and their order; bit-identical to that
+// code on random and affine matrices (checked against a build of the
+// reference's Eigen in the tests).  The 2x2 blocks are named by their place
+// in the matrix: p = rows 0-1 x cols 0-1, q = rows 2-3 x cols 0-1,
+// r = rows 0-1 x cols 2-3, s = rows 2-3 x cols 2-3, each read column by
+// column (x0 = (0,0), x1 = (1,0), x2 = (0,1), x3 = (1,1) of the block).
 Mat4 inverse(const Mat4& a) {
-    double m[4][8];
-    for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 8; ++j) m[i][j] = j < 4 ? (double)a.m[i][j] : (j - 4 == i ? 1.0 : 0.0);
-    for (int c = 0; c < 4; ++c) {
-        int piv = c;
-        for (int r = c + 1; r < 4; ++r)
-            if (std::fabs(m[r][c]) > std::fabs(m[piv][c])) piv = r;
-        if (m[piv][c] == 0.0) throw std::runtime_error("object transform is singular");
-        if (piv != c)
-            for (int j = 0; j < 8; ++j) std::swap(m[c][j], m[piv][j]);
-        double d = m[c][c];
-        for (int j = 0; j < 8; ++j) m[c][j] /= d;
-        for (int r = 0; r < 4; ++r) {
-            if (r == c) continue;
-            double f = m[r][c];
-            if (f == 0.0) continue;
-            for (int j = 0; j < 8; ++j) m[r][j] -= f * m[c][j];
-        }
+    const float p0 = a.m[0][0], p1 = a.m[1][0], p2 = a.m[0][1], p3 = a.m[1][1];
+    const float q0 = a.m[2][0], q1 = a.m[3][0], q2 = a.m[2][1], q3 = a.m[3][1];
+    const float r0 = a.m[0][2], r1 = a.m[1][2], r2 = a.m[0][3], r3 = a.m[1][3];
+    const float s0 = a.m[2][2], s1 = a.m[3][2], s2 = a.m[2][3], s3 = a.m[3][3];
+    // adj(p) * q and adj(s) * r
+    const float pq[4] = {p3 * q0 - p1 * q2, p3 * q1 - p1 * q3, p0 * q2 - p2 * q0, p0 * q3 - p2 * q1};
+    const float sr[4] = {s3 * r0 - s1 * r2, s3 * r1 - s1 * r3, s0 * r2 - s2 * r0, s0 * r3 - s2 * r1};
+    const float dp = p3 * p0 - p1 * p2, dq = q3 * q0 - q1 * q2, dr = r3 * r0 - r1 * r2, ds = s3 * s0 - s1 * s2;
+    const float tr = (sr[0] * pq[0] + sr[1] * pq[2]) + (sr[2] * pq[1] + sr[3] * pq[3]);
+    const float det = (dp * ds + dq * dr) - tr;
+    if (det == 0.f || !std::isfinite(det)) throw std::runtime_error("object transform is singular");
+    const float rd = 1.0f / det;
+    const float sg[4] = {rd, -rd, -rd, rd};
+    float ip[4], iq[4], ir[4], is[4];
+    ip[0] = p0 * ds - (q0 * sr[0] + q1 * sr[2]);
+    ip[1] = p1 * ds - (q0 * sr[1] + q1 * sr[3]);
+    ip[2] = p2 * ds - (q2 * sr[0] + q3 * sr[2]);
+    ip[3] = p3 * ds - (q2 * sr[1] + q3 * sr[3]);
+    is[0] = s0 * dp - (r0 * pq[0] + r1 * pq[2]);
+    is[1] = s1 * dp - (r0 * pq[1] + r1 * pq[3]);
+    is[2] = s2 * dp - (r2 * pq[0] + r3 * pq[2]);
+    is[3] = s3 * dp - (r2 * pq[1] + r3 * pq[3]);
+    iq[0] = r0 * dq - (s0 * pq[3] - s1 * pq[2]);
+    iq[1] = r1 * dq - (s1 * pq[0] - s0 * pq[1]);
+    iq[2] = r2 * dq - (s2 * pq[3] - s3 * pq[2]);
+    iq[3] = r3 * dq - (s3 * pq[0] - s2 * pq[1]);
+    ir[0] = q0 * dr - (p0 * sr[3] - p1 * sr[2]);
+    ir[1] = q1 * dr - (p1 * sr[0] - p0 * sr[1]);
+    ir[2] = q2 * dr - (p2 * sr[3] - p3 * sr[2]);
+    ir[3] = q3 * dr - (p3 * sr[0] - p2 * sr[1]);
+    for (int k = 0; k < 4; ++k) {
+        ip[k] *= sg[k];
+        iq[k] *= sg[k];
+        ir[k] *= sg[k];
+        is[k] *= sg[k];
     }
     Mat4 r;
-    for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) r.m[i][j] = (float)m[i][j + 4];
+    const float cols[4][4] = {{ip[3], ip[1], iq[3], iq[1]},
+                              {ip[2], ip[0], iq[2], iq[0]},
+                              {ir[3], ir[1], is[3], is[1]},
+                              {ir[2], ir[0], is[2], is[0]}};
+    for (int c = 0; c < 4; ++c)
+        for (int row = 0; row < 4; ++row) r.m[row][c] = cols[c][row];
     return r;
 }
 vec3 xform_point(const Mat4& t, vec3 p) {
@@ -91,10 +123,12 @@ vec3 xform_point(const Mat4& t, vec3 p) {
     return v3(o[0], o[1], o[2]);
 }
 // transformInv.transpose() * (n, 1), then normalized (reference: Src/Utils/objloader.cpp:36-41).
+// Eigen evaluates each coefficient of a transposed (row-major) 4x4 times a
+// vector as one 4-wide product summed pairwise, (x0 + x2) + (x1 + x3).
 vec3 xform_normal(const Mat4& inv, vec3 n) {
     float o[3];
     for (int r = 0; r < 3; ++r)
-        o[r] = ((inv.m[0][r] * n.x + inv.m[1][r] * n.y) + inv.m[2][r] * n.z) + inv.m[3][r] * 1.f;
+        o[r] = (inv.m[0][r] * n.x + inv.m[2][r] * n.z) + (inv.m[1][r] * n.y + inv.m[3][r] * 1.f);
     return normalize(v3(o[0], o[1], o[2]));
 }
 
@@ -135,9 +169,44 @@ std::string resolve_path(const std::string& base_dir, const std::string& p) {
 }
 
 // ---------------------------------------------------------------- OBJ ----
-struct ObjIdx {
-    int v = -1, vt = -1, vn = -1;
-};
+// OBJ import as the reference performs it: OBJLoader::LoadObject
+// (Src/Utils/objloader.cpp:66-210) over its vendored tinyobjloader
+// (ThirdParty/tinyobj/tiny_obj_loader.h: LoadObj with triangulation).  Kept:
+//  - lines end at '\n', '\r' or "\r\n"; leading spaces and tabs are skipped,
+//    '#' starts a comment;
+//  - numbers are read as tinyobj's parseReal / tryParseDouble read them
+//    (:804-953): digits gathered into a double (integer digits times ten,
+//    fraction digits times 10^-k from a table of doubles up to k = 7 and pow
+//    beyond), an exponent applied as ldexp(m * pow(5, e), e), then rounded to
+//    float; an unreadable field keeps its default (0);
+//  - face corners v, v/vt, v//vn, v/vt/vn with tinyobj's index rule (fixIndex,
+//    :738-759): k > 0 is k - 1, k < 0 counts back from the elements read so far;
+//    a 0 or unreadable index makes LoadObj fail, and the object then yields no
+//    meshes (Scene::ParseObjects warns and skips it);
+//  - polygons are triangulated by tinyobj's ear clipping (exportGroupsToShape,
+//    :1330-1560) in FP32, which gives up on the rest of a polygon when it finds
+//    no ear;
+//  - 'g', 'o' and 'usemtl' move faces between shapes exactly as LoadObj does
+//    (:2306-2420, :2575-2585): a 'usemtl' that changes the material closes the
+//    current face group into the current shape, 'g' closes the shape when it
+//    holds triangles, 'o' when its last group held any primitive, end of file
+//    when either holds;
+//  - materials are the 'newmtl' names of the first readable file of each
+//    'mtllib' line (base = the OBJ's directory; first definition of a name
+//    wins); a 'usemtl' name no such file defines is no material;
+//  - one mesh per (shape, material) in order of first appearance, named as
+//    BuildMeshName names it; positions transform * (p, 1), normals
+//    (transformInv^T (n, 1)).normalized(), and the face normal of the
+//    transformed positions where a triangle lacks a normal index.
+// Deviations, only where the reference reads out of bounds: a corner past the
+// last vertex, normal or texture coordinate of the file is an error, and the
+// ear clipping sees the vertices read up to the face's own line (tinyobj: up
+// to the end of its face group; the same for any file whose faces only name
+// vertices defined before them).
+//
+// The text is parsed in line-aligned chunks on host threads; the ear clipping
+// runs per face in parallel; one pass in file order replays the statements;
+// the triangles are written in parallel.
 struct LoadedMesh {
     std::string name;
     std::string material;
@@ -145,121 +214,313 @@ struct LoadedMesh {
     int count = 0;
 };
 
-int fix_index(long idx, size_t n) {
-    if (idx > 0) return (int)(idx - 1);
-    if (idx < 0) return (int)((long)n + idx);
-    return -1;
-}
-
-// Triangulated OBJ import, one mesh per (shape, material) pair, face normals
-// when a face lacks per-vertex normals (reference: Src/Utils/objloader.cpp:66-210).
-//
-// The text is parsed in line-aligned chunks on host threads: each chunk
-// parses its vertex lines (strtof) and the raw corner indices of its faces
-// (fan-triangulated), noting the chunk's vertex counts at every face line for
-// relative indices.  A sequential pass resolves the indices, replays the
-// object / group / material statements in file order and assigns every
-// triangle its mesh; the triangles are then written in parallel, each into its
-// place in its mesh, with the same per-vertex arithmetic.
 namespace {
 
+// tinyobj's tryParseDouble (tiny_obj_loader.h:804-930).
+bool obj_parse_double(const char* s, const char* e, double* out) {
+    static const double kFrac[8] = {1.0, 0.1, 0.01, 0.001, 0.0001, 0.00001, 0.000001, 0.0000001};
+    if (s >= e) return false;
+    double mant = 0.0;
+    int expo = 0, read = 0;
+    char sign = '+', esign = '+';
+    const char* c = s;
+    bool lead_dot = false, more;
+    auto digit = [](char ch) { return ch >= '0' && ch <= '9'; };
+    if (*c == '+' || *c == '-') {
+        sign = *c++;
+        if (c != e && *c == '.') lead_dot = true;
+    } else if (digit(*c)) {
+    } else if (*c == '.') {
+        lead_dot = true;
+    } else {
+        return false;
+    }
+    more = c != e;
+    if (!lead_dot) {
+        while (more && digit(*c)) {
+            mant *= 10;
+            mant += (int)(*c - '0');
+            ++c;
+            ++read;
+            more = c != e;
+        }
+        if (read == 0) return false;
+    }
+    if (!more) goto assemble;
+    if (*c == '.') {
+        ++c;
+        read = 1;
+        more = c != e;
+        while (more && digit(*c)) {
+            mant += (int)(*c - '0') * (read < 8 ? kFrac[read] : std::pow(10.0, -read));
+            ++read;
+            ++c;
+            more = c != e;
+        }
+    } else if (*c == 'e' || *c == 'E') {
+    } else {
+        goto assemble;
+    }
+    if (!more) goto assemble;
+    if (*c == 'e' || *c == 'E') {
+        ++c;
+        more = c != e;
+        if (more && (*c == '+' || *c == '-')) esign = *c++;
+        else if (!digit(*c)) return false;
+        read = 0;
+        more = c != e;
+        while (more && digit(*c)) {
+            expo *= 10;
+            expo += *c - '0';
+            ++c;
+            ++read;
+            more = c != e;
+        }
+        expo *= esign == '+' ? 1 : -1;
+        if (read == 0) return false;
+    }
+assemble:
+    *out = (sign == '+' ? 1 : -1) * (expo ? std::ldexp(mant * std::pow(5.0, expo), expo) : mant);
+    return true;
+}
+
+// parseReal: one whitespace-delimited field, `def` when it does not parse.
+float obj_real(const char*& t, double def = 0.0) {
+    t += std::strspn(t, " \t");
+    const char* end = t + std::strcspn(t, " \t\r");
+    double v = def;
+    obj_parse_double(t, end, &v);
+    t = end;
+    return (float)v;
+}
+
+// A field of tinyobj's parseString: skip spaces and tabs, read to the next blank.
+std::string obj_string(const char*& t) {
+    t += std::strspn(t, " \t");
+    size_t n = std::strcspn(t, " \t\r");
+    std::string s(t, n);
+    t += n;
+    return s;
+}
+
+inline bool obj_space(char c) { return c == ' ' || c == '\t'; }
+inline bool obj_newline(char c) { return c == '\r' || c == '\n' || c == '\0'; }
+
+constexpr int32_t kObjAbsent = INT32_MIN;  // corner field not given
 struct ObjCorner {
-    int32_t v, vt, vn;  // raw OBJ indices (1-based or negative), 0 = absent
+    int32_t v, vt, vn;  // raw OBJ indices (0 = unreadable), kObjAbsent when not given
 };
-struct ObjTri {
-    ObjCorner c[3];
-    int32_t nv, nvt, nvn;  // the chunk's vertex counts before the face line
+struct ObjFace {
+    uint32_t first, count;  // corners in the chunk's corner array
+    int32_t nv, nvt, nvn;   // the chunk's element counts at the face line
 };
+enum ObjEventKind { kObjGroup, kObjObject, kObjUsemtl, kObjMtllib, kObjPrimitive };
 struct ObjEvent {
-    size_t tri;  // chunk triangles before the statement
-    int kind;    // 0 object / group, 1 material
-    std::string name;
+    size_t face;  // chunk faces before the statement
+    int kind;
+    std::string text;
 };
 struct ObjChunk {
     std::vector<float> V, VN, VT;
-    std::vector<ObjTri> tris;
+    std::vector<ObjCorner> corners;
+    std::vector<ObjFace> faces;
     std::vector<ObjEvent> events;
+    // ear clipping: per face its first triangle and count in `tris` (corner positions within the face)
+    std::vector<uint32_t> tri_first, tri_count;
+    std::vector<std::array<uint32_t, 3>> tris;
 };
 
-std::string trim_ws(const char* b, const char* e) {
-    while (b < e && (*b == ' ' || *b == '\t')) ++b;
-    while (e > b && (e[-1] == ' ' || e[-1] == '\t')) --e;
-    return std::string(b, e);
+// parseTriple's index reading: atoi of the field, then the rest of it skipped.
+int32_t obj_index(const char*& t) {
+    int32_t v = (int32_t)std::atoi(t);
+    t += std::strcspn(t, "/ \t\r");
+    return v;
 }
 
-// Integer token prefix as strtol reads it (optional sign, decimal digits; 0 if none).
-long parse_long(const char*& t, const char* end) {
-    const char* q = t;
-    bool neg = false;
-    if (q < end && (*q == '+' || *q == '-')) neg = *q++ == '-';
-    if (q >= end || *q < '0' || *q > '9') return 0;  // no conversion: t stays
-    long v = 0;
-    while (q < end && *q >= '0' && *q <= '9') v = v * 10 + (*q++ - '0');
-    t = q;
-    return neg ? -v : v;
-}
-
-void parse_obj_chunk(const char* b, const char* e, ObjChunk& ck) {
-    std::string line;
-    std::vector<ObjCorner> poly;
-    while (b < e) {
-        const char* nl = static_cast<const char*>(std::memchr(b, '\n', (size_t)(e - b)));
-        const char* le = nl ? nl : e;
-        line.assign(b, le);  // NUL-terminated copy for strtof
-        b = nl ? nl + 1 : e;
-        if (!line.empty() && line.back() == '\r') line.pop_back();
-        size_t s = line.find_first_not_of(" \t");
-        if (s == std::string::npos || line[s] == '#') continue;
-        const char* c = line.c_str() + s;
-        const char* ce = line.c_str() + line.size();
-        if (c[0] == 'v' && (c[1] == ' ' || c[1] == '\t')) {
-            char* q;
-            float x = std::strtof(c + 2, &q);
-            float y = std::strtof(q, &q);
-            float z = std::strtof(q, &q);
-            ck.V.push_back(x); ck.V.push_back(y); ck.V.push_back(z);
-        } else if (c[0] == 'v' && c[1] == 'n') {
-            char* q;
-            float x = std::strtof(c + 2, &q);
-            float y = std::strtof(q, &q);
-            float z = std::strtof(q, &q);
-            ck.VN.push_back(x); ck.VN.push_back(y); ck.VN.push_back(z);
-        } else if (c[0] == 'v' && c[1] == 't') {
-            char* q;
-            float x = std::strtof(c + 2, &q);
-            float y = std::strtof(q, &q);
-            ck.VT.push_back(x); ck.VT.push_back(y);
-        } else if (c[0] == 'f' && (c[1] == ' ' || c[1] == '\t')) {
-            poly.clear();
-            const char* t = c + 2;
-            while (true) {
-                while (t < ce && std::isspace((unsigned char)*t)) ++t;
-                if (t >= ce) break;
-                const char* te = t;
-                while (te < ce && !std::isspace((unsigned char)*te)) ++te;
-                ObjCorner k{0, 0, 0};
-                const char* q = t;
-                k.v = (int32_t)parse_long(q, te);
-                if (q < te && *q == '/') {
-                    ++q;
-                    if (q < te && *q != '/') k.vt = (int32_t)parse_long(q, te);
-                    if (q < te && *q == '/') {
-                        ++q;
-                        k.vn = (int32_t)parse_long(q, te);
+void parse_obj_line(const char* c, ObjChunk& ck) {
+    c += std::strspn(c, " \t");
+    if (c[0] == '\0' || c[0] == '#') return;
+    if (c[0] == 'v' && obj_space(c[1])) {
+        const char* t = c + 2;
+        float x = obj_real(t), y = obj_real(t), z = obj_real(t);
+        ck.V.push_back(x); ck.V.push_back(y); ck.V.push_back(z);
+    } else if (c[0] == 'v' && c[1] == 'n' && obj_space(c[2])) {
+        const char* t = c + 3;
+        float x = obj_real(t), y = obj_real(t), z = obj_real(t);
+        ck.VN.push_back(x); ck.VN.push_back(y); ck.VN.push_back(z);
+    } else if (c[0] == 'v' && c[1] == 't' && obj_space(c[2])) {
+        const char* t = c + 3;
+        float x = obj_real(t), y = obj_real(t);
+        ck.VT.push_back(x); ck.VT.push_back(y);
+    } else if (c[0] == 'f' && obj_space(c[1])) {
+        const char* t = c + 2;
+        t += std::strspn(t, " \t");
+        ObjFace f{(uint32_t)ck.corners.size(), 0, (int32_t)(ck.V.size() / 3), (int32_t)(ck.VT.size() / 2),
+                  (int32_t)(ck.VN.size() / 3)};
+        while (!obj_newline(t[0])) {
+            ObjCorner k{obj_index(t), kObjAbsent, kObjAbsent};
+            if (t[0] == '/') {
+                ++t;
+                if (t[0] == '/') {
+                    ++t;
+                    k.vn = obj_index(t);
+                } else {
+                    k.vt = obj_index(t);
+                    if (t[0] == '/') {
+                        ++t;
+                        k.vn = obj_index(t);
                     }
                 }
-                poly.push_back(k);
-                t = te;
             }
-            const int32_t nv = (int32_t)(ck.V.size() / 3), nvt = (int32_t)(ck.VT.size() / 2),
-                          nvn = (int32_t)(ck.VN.size() / 3);
-            for (size_t k = 1; k + 1 < poly.size(); ++k) ck.tris.push_back(ObjTri{{poly[0], poly[k], poly[k + 1]}, nv, nvt, nvn});
-        } else if ((c[0] == 'o' || c[0] == 'g') && (c[1] == ' ' || c[1] == '\t' || c[1] == 0)) {
-            ck.events.push_back(ObjEvent{ck.tris.size(), 0, c[1] ? trim_ws(c + 2, ce) : std::string()});
-        } else if (std::strncmp(c, "usemtl", 6) == 0) {
-            ck.events.push_back(ObjEvent{ck.tris.size(), 1, trim_ws(c + 6, ce)});
+            ck.corners.push_back(k);
+            ++f.count;
+            t += std::strspn(t, " \t\r");
+        }
+        ck.faces.push_back(f);
+    } else if ((c[0] == 'l' || c[0] == 'p') && obj_space(c[1])) {
+        ck.events.push_back(ObjEvent{ck.faces.size(), kObjPrimitive, std::string()});
+    } else if (c[0] == 'g' && obj_space(c[1])) {
+        const char* t = c;
+        std::vector<std::string> names;
+        while (!obj_newline(t[0])) {
+            names.push_back(obj_string(t));
+            t += std::strspn(t, " \t\r");
+        }
+        std::string joined;
+        for (size_t i = 1; i < names.size(); ++i) joined += (i > 1 ? " " : "") + names[i];
+        ck.events.push_back(ObjEvent{ck.faces.size(), kObjGroup, joined});
+    } else if (c[0] == 'o' && obj_space(c[1])) {
+        ck.events.push_back(ObjEvent{ck.faces.size(), kObjObject, std::string(c + 2)});
+    } else if (std::strncmp(c, "usemtl", 6) == 0 && obj_space(c[6])) {
+        ck.events.push_back(ObjEvent{ck.faces.size(), kObjUsemtl, std::string(c + 7)});
+    } else if (std::strncmp(c, "mtllib", 6) == 0 && obj_space(c[6])) {
+        ck.events.push_back(ObjEvent{ck.faces.size(), kObjMtllib, std::string(c + 7)});
+    }
+}
+
+// Lines of [b, e) end at '\n', '\r' or "\r\n" (tinyobj's safeGetline).
+template <typename F>
+void for_each_line(const char* b, const char* e, F&& f) {
+    std::string line;
+    while (b < e) {
+        const char* q = b;
+        while (q < e && *q != '\n' && *q != '\r') ++q;
+        line.assign(b, q);  // NUL-terminated copy
+        f(line);
+        if (q < e && *q == '\r' && q + 1 < e && q[1] == '\n') ++q;
+        b = q < e ? q + 1 : e;
+    }
+}
+
+// pnpoly (tiny_obj_loader.h:1316-1327): crossings of a ray from (tx, ty).
+bool obj_pnpoly3(const float* vx, const float* vy, float tx, float ty) {
+    bool c = false;
+    for (int i = 0, j = 2; i < 3; j = i++)
+        if (((vy[i] > ty) != (vy[j] > ty)) && (tx < (vx[j] - vx[i]) * (ty - vy[i]) / (vy[j] - vy[i]) + vx[i])) c = !c;
+    return c;
+}
+
+// tinyobj's ear clipping of one face (exportGroupsToShape's triangulate branch,
+// tiny_obj_loader.h:1358-1560), FP32.  vi: the face's resolved vertex indices;
+// V / nv: vertex coordinates and the vertex count the clipping may read.
+// Appends triangles as positions within the face.
+void obj_ear_clip(const int32_t* vi, uint32_t np, const float* V, size_t nv,
+                  std::vector<std::array<uint32_t, 3>>& out) {
+    auto ok = [&](int32_t v) { return v >= 0 && (size_t)v < nv; };
+    size_t ax0 = 1, ax1 = 2;
+    for (uint32_t k = 0; k < np; ++k) {
+        int32_t a = vi[k % np], b = vi[(k + 1) % np], c = vi[(k + 2) % np];
+        if (!ok(a) || !ok(b) || !ok(c)) continue;
+        const float* p0 = V + 3 * (size_t)a;
+        const float* p1 = V + 3 * (size_t)b;
+        const float* p2 = V + 3 * (size_t)c;
+        float e0x = p1[0] - p0[0], e0y = p1[1] - p0[1], e0z = p1[2] - p0[2];
+        float e1x = p2[0] - p1[0], e1y = p2[1] - p1[1], e1z = p2[2] - p1[2];
+        float cx = std::fabs(e0y * e1z - e0z * e1y);
+        float cy = std::fabs(e0z * e1x - e0x * e1z);
+        float cz = std::fabs(e0x * e1y - e0y * e1x);
+        const float eps = std::numeric_limits<float>::epsilon();
+        if (cx > eps || cy > eps || cz > eps) {
+            if (!(cx > cy && cx > cz)) {
+                ax0 = 0;
+                if (cz > cx && cz > cy) ax1 = 1;
+            }
+            break;
         }
     }
+    float area = 0.f;
+    for (uint32_t k = 0; k < np; ++k) {
+        int32_t a = vi[k % np], b = vi[(k + 1) % np];
+        if (!ok(a) || !ok(b)) continue;
+        float x0 = V[3 * (size_t)a + ax0], y0 = V[3 * (size_t)a + ax1];
+        float x1 = V[3 * (size_t)b + ax0], y1 = V[3 * (size_t)b + ax1];
+        area += (x0 * y1 - y0 * x1) * 0.5f;
+    }
+    std::vector<uint32_t> rem(np);
+    for (uint32_t k = 0; k < np; ++k) rem[k] = k;
+    size_t guess = 0, iters = np, prev = np;
+    while (rem.size() > 3 && iters > 0) {
+        const size_t n = rem.size();
+        if (guess >= n) guess -= n;
+        if (prev != n) {
+            prev = n;
+            iters = n;
+        } else {
+            --iters;
+        }
+        uint32_t ind[3];
+        float vx[3], vy[3];
+        for (int k = 0; k < 3; ++k) {
+            ind[k] = rem[(guess + k) % n];
+            int32_t v = vi[ind[k]];
+            vx[k] = ok(v) ? V[3 * (size_t)v + ax0] : 0.f;
+            vy[k] = ok(v) ? V[3 * (size_t)v + ax1] : 0.f;
+        }
+        float e0x = vx[1] - vx[0], e0y = vy[1] - vy[0], e1x = vx[2] - vx[1], e1y = vy[2] - vy[1];
+        float cross = e0x * e1y - e0y * e1x;
+        if (cross * area < 0.f) {
+            guess += 1;
+            continue;
+        }
+        bool overlap = false;
+        for (size_t other = 3; other < n; ++other) {
+            int32_t v = vi[rem[(guess + other) % n]];
+            if (!ok(v)) continue;
+            if (obj_pnpoly3(vx, vy, V[3 * (size_t)v + ax0], V[3 * (size_t)v + ax1])) {
+                overlap = true;
+                break;
+            }
+        }
+        if (overlap) {
+            guess += 1;
+            continue;
+        }
+        out.push_back({ind[0], ind[1], ind[2]});
+        rem.erase(rem.begin() + (long)((guess + 1) % n));
+    }
+    if (rem.size() == 3) out.push_back({rem[0], rem[1], rem[2]});
+}
+
+// The 'newmtl' names of an MTL file in order (LoadMtl, tiny_obj_loader.h:1638-2012):
+// trailing blanks trimmed, the last material flushed even without a name.
+bool obj_read_mtl_names(const std::string& path, std::vector<std::string>& names) {
+    std::ifstream in(path.c_str(), std::ios::binary);
+    if (!in) return false;
+    std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    std::string cur;
+    for_each_line(text.data(), text.data() + text.size(), [&](std::string& line) {
+        size_t last = line.find_last_not_of(" \t");
+        line.erase(last == std::string::npos ? 0 : last + 1);
+        const char* t = line.c_str();
+        t += std::strspn(t, " \t");
+        if (t[0] == '\0' || t[0] == '#') return;
+        if (std::strncmp(t, "newmtl", 6) == 0 && obj_space(t[6])) {
+            if (!cur.empty()) names.push_back(cur);
+            cur = std::string(t + 7);
+        }
+    });
+    names.push_back(cur);
+    return true;
 }
 
 int host_threads() {
@@ -313,6 +574,7 @@ void parallel_for(size_t n, F&& f) {
 
 }  // namespace
 
+// Returns the meshes of `path` (empty when the file does not parse as LoadObj requires).
 std::vector<LoadedMesh> load_obj(const std::string& path, const Mat4& xf, const Mat4& xf_inv) {
     const std::string text = read_file(path);
     // Line-aligned chunks of about 1 MiB.
@@ -320,133 +582,233 @@ std::vector<LoadedMesh> load_obj(const std::string& path, const Mat4& xf, const 
     const size_t target = (size_t)1 << 20;
     while (cuts.back() < text.size()) {
         size_t c = std::min(text.size(), cuts.back() + target);
-        while (c < text.size() && text[c - 1] != '\n') ++c;
+        while (c < text.size() && text[c - 1] != '\n' && text[c - 1] != '\r') ++c;
         cuts.push_back(c);
     }
     const size_t nchunks = cuts.size() - 1;
     std::vector<ObjChunk> chunks(nchunks);
-    parallel_for(nchunks, [&](size_t i) { parse_obj_chunk(text.data() + cuts[i], text.data() + cuts[i + 1], chunks[i]); });
+    parallel_for(nchunks, [&](size_t i) {
+        for_each_line(text.data() + cuts[i], text.data() + cuts[i + 1],
+                      [&](const std::string& line) { parse_obj_line(line.c_str(), chunks[i]); });
+    });
 
-    // Global vertex arrays and the chunks' offsets into them.
-    std::vector<size_t> v0(nchunks + 1, 0), vt0(nchunks + 1, 0), vn0(nchunks + 1, 0), t0(nchunks + 1, 0);
+    // Global element arrays and the chunks' offsets into them.
+    std::vector<size_t> v0(nchunks + 1, 0), vt0(nchunks + 1, 0), vn0(nchunks + 1, 0), f0(nchunks + 1, 0);
     for (size_t i = 0; i < nchunks; ++i) {
         v0[i + 1] = v0[i] + chunks[i].V.size() / 3;
         vt0[i + 1] = vt0[i] + chunks[i].VT.size() / 2;
         vn0[i + 1] = vn0[i] + chunks[i].VN.size() / 3;
-        t0[i + 1] = t0[i] + chunks[i].tris.size();
+        f0[i + 1] = f0[i] + chunks[i].faces.size();
     }
     std::vector<float> V(3 * v0[nchunks]), VT(2 * vt0[nchunks]), VN(3 * vn0[nchunks]);
+    std::atomic<bool> unreadable{false};
     parallel_for(nchunks, [&](size_t i) {
-        std::copy(chunks[i].V.begin(), chunks[i].V.end(), V.begin() + 3 * v0[i]);
-        std::copy(chunks[i].VT.begin(), chunks[i].VT.end(), VT.begin() + 2 * vt0[i]);
-        std::copy(chunks[i].VN.begin(), chunks[i].VN.end(), VN.begin() + 3 * vn0[i]);
+        ObjChunk& ck = chunks[i];
+        std::copy(ck.V.begin(), ck.V.end(), V.begin() + 3 * v0[i]);
+        std::copy(ck.VT.begin(), ck.VT.end(), VT.begin() + 2 * vt0[i]);
+        std::copy(ck.VN.begin(), ck.VN.end(), VN.begin() + 3 * vn0[i]);
+        // fixIndex against the counts at each face line; a 0 fails LoadObj.
+        for (const ObjFace& f : ck.faces)
+            for (uint32_t k = 0; k < f.count; ++k) {
+                ObjCorner& c = ck.corners[f.first + k];
+                auto fix = [&](int32_t& x, size_t n) {
+                    if (x == kObjAbsent) { x = -1; return; }
+                    if (x == 0) { unreadable = true; return; }
+                    x = x > 0 ? x - 1 : (int32_t)((int64_t)n + x);
+                };
+                fix(c.v, v0[i] + (size_t)f.nv);
+                fix(c.vt, vt0[i] + (size_t)f.nvt);
+                fix(c.vn, vn0[i] + (size_t)f.nvn);
+            }
+    });
+    if (unreadable) return {};
+    // Ear clipping per face.
+    parallel_for(nchunks, [&](size_t i) {
+        ObjChunk& ck = chunks[i];
+        ck.tri_first.resize(ck.faces.size());
+        ck.tri_count.resize(ck.faces.size());
+        std::vector<int32_t> vi;
+        for (size_t fi = 0; fi < ck.faces.size(); ++fi) {
+            const ObjFace& f = ck.faces[fi];
+            ck.tri_first[fi] = (uint32_t)ck.tris.size();
+            if (f.count >= 3) {
+                vi.resize(f.count);
+                for (uint32_t k = 0; k < f.count; ++k) vi[k] = ck.corners[f.first + k].v;
+                obj_ear_clip(vi.data(), f.count, V.data(), v0[i] + (size_t)f.nv, ck.tris);
+            }
+            ck.tri_count[fi] = (uint32_t)ck.tris.size() - ck.tri_first[fi];
+        }
     });
 
-    // Statements in file order: shape names, and every triangle's (shape, material).
-    const size_t ntris = t0[nchunks];
-    std::vector<std::string> shape_names{""};
-    std::vector<std::string> materials;
-    std::map<std::string, int> material_id;
-    std::vector<std::pair<int, int>> tri_key(ntris);
-    int shape = 0, mtl = -1;
-    bool shape_has_faces = false;
+    // Per chunk, the triangles of faces [a, b) are tris[tri_prefix[a] .. tri_prefix[b]).
+    std::vector<std::vector<uint32_t>> tri_prefix(nchunks), tri_face(nchunks);
+    parallel_for(nchunks, [&](size_t i) {
+        const ObjChunk& ck = chunks[i];
+        tri_prefix[i].resize(ck.faces.size() + 1);
+        tri_prefix[i][0] = 0;
+        for (size_t fi = 0; fi < ck.faces.size(); ++fi) tri_prefix[i][fi + 1] = tri_prefix[i][fi] + ck.tri_count[fi];
+        tri_face[i].resize(ck.tris.size());
+        for (size_t fi = 0; fi < ck.faces.size(); ++fi)
+            for (uint32_t t = 0; t < ck.tri_count[fi]; ++t) tri_face[i][ck.tri_first[fi] + t] = (uint32_t)fi;
+    });
+
+    // LoadObj's statements in file order; faces move between shapes in runs of
+    // consecutive faces of one chunk.
+    struct ObjRun {
+        uint32_t chunk, a, b;  // faces [a, b) of the chunk
+        int material;
+    };
+    struct ObjShape {
+        std::string name;
+        std::vector<ObjRun> runs;
+        bool has_tris = false;
+    };
+    std::vector<ObjShape> shapes;
+    ObjShape shape;
+    std::string name;
+    int material = -1;
+    std::vector<ObjRun> group;  // faces since the last export
+    bool group_prims = false;   // a line or point statement since the last group was cleared
+    std::map<std::string, int> mat_id;
+    std::vector<std::string> mat_names;
+    const std::string base = dir_of(path);
+    auto export_group = [&]() {
+        if (group.empty() && !group_prims) return false;
+        shape.name = name;
+        for (ObjRun r : group) {
+            r.material = material;
+            shape.runs.push_back(r);
+            if (tri_prefix[r.chunk][r.b] > tri_prefix[r.chunk][r.a]) shape.has_tris = true;
+        }
+        return true;
+    };
     for (size_t i = 0; i < nchunks; ++i) {
-        size_t t = 0;
-        auto emit_until = [&](size_t stop) {
-            if (t < stop) shape_has_faces = true;
-            for (; t < stop; ++t) tri_key[t0[i] + t] = {shape, mtl};
+        uint32_t f = 0;
+        auto take_until = [&](uint32_t stop) {
+            if (stop > f) group.push_back(ObjRun{(uint32_t)i, f, stop, -1});
+            f = stop;
         };
         for (const ObjEvent& ev : chunks[i].events) {
-            emit_until(ev.tri);
-            if (ev.kind == 0) {
-                if (shape_has_faces) {
-                    shape_names.push_back(ev.name);
-                    ++shape;
-                    shape_has_faces = false;
-                } else {
-                    shape_names[shape] = ev.name;
+            take_until((uint32_t)ev.face);
+            switch (ev.kind) {
+                case kObjPrimitive: group_prims = true; break;
+                case kObjUsemtl: {
+                    auto it = mat_id.find(ev.text);
+                    int id = it == mat_id.end() ? -1 : it->second;
+                    if (id != material) {
+                        export_group();
+                        group.clear();
+                        material = id;
+                    }
+                    break;
                 }
-            } else if (ev.name.empty()) {
-                mtl = -1;  // the same (empty) material as before any usemtl
-            } else {
-                auto it = material_id.find(ev.name);
-                if (it == material_id.end()) {
-                    it = material_id.emplace(ev.name, (int)materials.size()).first;
-                    materials.push_back(ev.name);
+                case kObjMtllib: {
+                    std::vector<std::string> files;
+                    std::stringstream ss(ev.text);
+                    for (std::string item; std::getline(ss, item, ' ');) files.push_back(item);
+                    for (const std::string& fn : files) {
+                        std::vector<std::string> names;
+                        if (!obj_read_mtl_names(base + fn, names)) continue;
+                        for (const std::string& n : names) {
+                            mat_id.emplace(n, (int)mat_names.size());
+                            mat_names.push_back(n);
+                        }
+                        break;
+                    }
+                    break;
                 }
-                mtl = it->second;
+                case kObjGroup:
+                    export_group();
+                    if (shape.has_tris) shapes.push_back(shape);
+                    shape = ObjShape();
+                    group.clear();
+                    group_prims = false;
+                    name = ev.text;
+                    break;
+                case kObjObject:
+                    if (export_group()) shapes.push_back(shape);
+                    group.clear();
+                    group_prims = false;
+                    shape = ObjShape();
+                    name = ev.text;
+                    break;
             }
         }
-        emit_until(chunks[i].tris.size());
+        take_until((uint32_t)chunks[i].faces.size());
     }
+    if (export_group() || shape.has_tris) shapes.push_back(shape);
 
-    // Meshes in order of first appearance; each triangle's slot in its mesh.
+    // Meshes per (shape, material) in order of first appearance; each run's
+    // triangles occupy consecutive slots of its mesh.
+    struct Segment {
+        uint32_t chunk, t0, t1, mesh, slot;  // chunk triangles [t0, t1) -> mesh slots from `slot`
+    };
+    std::vector<Segment> segs;
     std::vector<LoadedMesh> meshes;
     std::map<std::pair<int, int>, int> lookup;
-    std::vector<int32_t> tri_mesh(ntris), tri_slot(ntris);
-    std::pair<int, int> last_key{-2, -2};
-    int last_mesh = -1;
-    for (size_t t = 0; t < ntris; ++t) {
-        if (tri_key[t] != last_key) {
-            auto it = lookup.find(tri_key[t]);
+    const std::string file_stem = stem_of(path);
+    for (size_t si = 0; si < shapes.size(); ++si)
+        for (const ObjRun& r : shapes[si].runs) {
+            const uint32_t t0 = tri_prefix[r.chunk][r.a], t1 = tri_prefix[r.chunk][r.b];
+            if (t1 == t0) continue;
+            auto key = std::make_pair((int)si, r.material);
+            auto it = lookup.find(key);
             if (it == lookup.end()) {
                 LoadedMesh m;
-                const int sh = tri_key[t].first, mt = tri_key[t].second;
-                const std::string& sn = shape_names[sh];
-                const std::string mname = mt >= 0 ? materials[mt] : std::string();
-                std::string base = sn.empty() ? stem_of(path) + "_shape_" + std::to_string(sh) : sn;
-                m.name = mname.empty() ? base : base + "::" + mname;
+                const std::string mname =
+                    r.material >= 0 && r.material < (int)mat_names.size() ? mat_names[r.material] : std::string();
+                std::string sn = shapes[si].name.empty() ? file_stem + "_shape_" + std::to_string(si) : shapes[si].name;
+                m.name = mname.empty() ? sn : sn + "::" + mname;
                 m.material = mname;
-                it = lookup.emplace(tri_key[t], (int)meshes.size()).first;
+                it = lookup.emplace(key, (int)meshes.size()).first;
                 meshes.push_back(std::move(m));
             }
-            last_key = tri_key[t];
-            last_mesh = it->second;
+            // ck.tris of a chunk are in face order, so a run's triangles are contiguous
+            const uint32_t first = chunks[r.chunk].tri_first.empty() ? 0 : chunks[r.chunk].tri_first[r.a];
+            LoadedMesh& m = meshes[it->second];
+            const uint32_t block = 4096;
+            for (uint32_t t = 0; t < t1 - t0; t += block)
+                segs.push_back(Segment{r.chunk, first + t, first + std::min(t1 - t0, t + block), (uint32_t)it->second,
+                                       (uint32_t)m.count + t});
+            m.count += (int)(t1 - t0);
         }
-        tri_mesh[t] = last_mesh;
-        tri_slot[t] = meshes[last_mesh].count++;
-    }
     for (LoadedMesh& m : meshes) {
         m.pos.resize(9 * (size_t)m.count);
         m.nrm.resize(9 * (size_t)m.count);
         m.uv.resize(6 * (size_t)m.count);
     }
 
-    auto resolve = [](int32_t idx, size_t n) -> int { return fix_index(idx, n); };
-    const size_t block = 4096;
-    parallel_for((ntris + block - 1) / block, [&](size_t bi) {
-        size_t ci = std::upper_bound(t0.begin(), t0.end(), bi * block) - t0.begin() - 1;
-        for (size_t t = bi * block; t < std::min(ntris, (bi + 1) * block); ++t) {
-            while (t >= t0[ci + 1]) ++ci;
-            const ObjTri& tr = chunks[ci].tris[t - t0[ci]];
+    // The triangles, with BuildVertex's arithmetic (objloader.cpp:10-50, 186-202).
+    std::atomic<bool> out_of_range{false};
+    parallel_for(segs.size(), [&](size_t sgi) {
+        const Segment& sg = segs[sgi];
+        const ObjChunk& ck = chunks[sg.chunk];
+        LoadedMesh& m = meshes[sg.mesh];
+        for (uint32_t tri = sg.t0; tri < sg.t1; ++tri) {
+            const ObjFace& f = ck.faces[tri_face[sg.chunk][tri]];
             vec3 p[3], n[3];
             float u[3], v[3];
             bool hn[3];
             for (int k = 0; k < 3; ++k) {
-                const ObjCorner& c = tr.c[k];
-                const int iv = resolve(c.v, v0[ci] + (size_t)tr.nv);
-                const int ivt = resolve(c.vt, vt0[ci] + (size_t)tr.nvt);
-                const int ivn = resolve(c.vn, vn0[ci] + (size_t)tr.nvn);
-                if (iv < 0 || (size_t)(3 * (size_t)iv + 2) >= V.size())
-                    throw std::runtime_error("OBJ face references a missing vertex in " + path);
-                p[k] = xform_point(xf, v3(V[3 * (size_t)iv], V[3 * (size_t)iv + 1], V[3 * (size_t)iv + 2]));
-                hn[k] = ivn >= 0 && (size_t)(3 * (size_t)ivn + 2) < VN.size();
-                n[k] = hn[k] ? xform_normal(xf_inv, v3(VN[3 * (size_t)ivn], VN[3 * (size_t)ivn + 1], VN[3 * (size_t)ivn + 2]))
-                             : v3(0.f, 0.f, 0.f);
-                if (ivt >= 0 && (size_t)(2 * (size_t)ivt + 1) < VT.size()) {
-                    u[k] = VT[2 * (size_t)ivt];
-                    v[k] = VT[2 * (size_t)ivt + 1];
-                } else {
-                    u[k] = 0.f;
-                    v[k] = 0.f;
+                const ObjCorner& c = ck.corners[f.first + ck.tris[tri][k]];
+                if ((c.v >= 0 && 3 * (size_t)c.v + 2 >= V.size()) || (c.vn >= 0 && 3 * (size_t)c.vn + 2 >= VN.size()) ||
+                    (c.vt >= 0 && 2 * (size_t)c.vt + 1 >= VT.size())) {
+                    out_of_range = true;
+                    return;
                 }
+                p[k] = c.v >= 0 ? xform_point(xf, v3(V[3 * (size_t)c.v], V[3 * (size_t)c.v + 1], V[3 * (size_t)c.v + 2]))
+                                : v3(0.f, 0.f, 0.f);
+                hn[k] = c.vn >= 0;
+                n[k] = hn[k] ? xform_normal(xf_inv, v3(VN[3 * (size_t)c.vn], VN[3 * (size_t)c.vn + 1], VN[3 * (size_t)c.vn + 2]))
+                             : v3(0.f, 0.f, 0.f);
+                u[k] = c.vt >= 0 ? VT[2 * (size_t)c.vt] : 0.f;
+                v[k] = c.vt >= 0 ? VT[2 * (size_t)c.vt + 1] : 0.f;
             }
             if (!(hn[0] && hn[1] && hn[2])) {
                 vec3 fn = normalize(cross(p[1] - p[0], p[2] - p[0]));
                 n[0] = n[1] = n[2] = fn;
             }
-            LoadedMesh& m = meshes[tri_mesh[t]];
-            const size_t o = (size_t)tri_slot[t];
+            const size_t o = sg.slot + (tri - sg.t0);
             for (int k = 0; k < 3; ++k) {
                 m.pos[9 * o + 3 * k] = p[k].x; m.pos[9 * o + 3 * k + 1] = p[k].y; m.pos[9 * o + 3 * k + 2] = p[k].z;
                 m.nrm[9 * o + 3 * k] = n[k].x; m.nrm[9 * o + 3 * k + 1] = n[k].y; m.nrm[9 * o + 3 * k + 2] = n[k].z;
@@ -454,6 +816,7 @@ std::vector<LoadedMesh> load_obj(const std::string& path, const Mat4& xf, const 
             }
         }
     });
+    if (out_of_range) throw std::runtime_error("OBJ face references an element past the end of " + path);
     return meshes;
 }
 

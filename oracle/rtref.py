@@ -3,8 +3,9 @@
 Used by tests/, by ``__graft_entry__.smoke()`` as the checker and by
 ``bench.py``'s cpu_baseline leg.  The scene is read here independently of the
 library's loader: JSON semantics follow the reference's Scene::Parse*
-(Src/Core/scene.cpp:187-616) and OBJ import follows OBJLoader::LoadObject
-(Src/Utils/objloader.cpp:66-210) for triangulated OBJ files.
+(Src/Core/scene.cpp:187-616); OBJ import and object transforms are
+rtref_obj's restatement of OBJLoader::LoadObject (Src/Utils/objloader.cpp:66-210),
+tinyobjloader and Eigen.
 """
 from __future__ import annotations
 
@@ -17,6 +18,7 @@ from typing import Optional
 
 import numpy as np
 
+from . import rtref_obj
 from .rtref_image import read_texture
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -140,116 +142,6 @@ def _p(a: np.ndarray):
 # ------------------------------------------------------------------ scene parse
 def _f32(x) -> np.float32:
     return np.float32(x)
-
-
-def _deg2rad(a: np.float32) -> np.float32:
-    return np.float32(float(np.float32(a) / np.float32(180.0)) * 3.1415926)
-
-
-def _mat4_mul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
-    r = np.zeros((4, 4), np.float32)
-    for i in range(4):
-        for j in range(4):
-            r[i, j] = ((a[i, 0] * b[0, j] + a[i, 1] * b[1, j]) + a[i, 2] * b[2, j]) + a[i, 3] * b[3, j]
-    return r
-
-
-def _xform_points(m: np.ndarray, p: np.ndarray) -> np.ndarray:
-    out = np.empty_like(p)
-    for r in range(3):
-        out[:, r] = ((m[r, 0] * p[:, 0] + m[r, 1] * p[:, 1]) + m[r, 2] * p[:, 2]) + m[r, 3] * np.float32(1)
-    return out
-
-
-def _dot3(a, b):
-    return a[..., 0] * b[..., 0] + (a[..., 1] * b[..., 1] + a[..., 2] * b[..., 2])
-
-
-def _normalize(v: np.ndarray) -> np.ndarray:
-    z = _dot3(v, v)
-    s = np.sqrt(z)
-    out = v.copy()
-    m = z > 0
-    out[m] = v[m] / s[m][:, None]
-    return out
-
-
-def _cross(a, b):
-    return np.stack([a[..., 1] * b[..., 2] - a[..., 2] * b[..., 1],
-                     a[..., 2] * b[..., 0] - a[..., 0] * b[..., 2],
-                     a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]], axis=-1)
-
-
-def _load_obj(path: str, xf: np.ndarray, xf_inv: np.ndarray):
-    """Triangulated OBJ: one mesh per (shape, usemtl) pair, face normals when a
-    face lacks per-vertex normals."""
-    V, VN, VT, faces = [], [], [], []
-    shape, has_faces, names, mtl = 0, False, [""], ""
-    with open(path) as f:
-        for line in f:
-            t = line.split()
-            if not t or t[0].startswith("#"):
-                continue
-            if t[0] == "v":
-                V.append([float(np.float32(x)) for x in t[1:4]])
-            elif t[0] == "vn":
-                VN.append([float(np.float32(x)) for x in t[1:4]])
-            elif t[0] == "vt":
-                VT.append([float(np.float32(x)) for x in t[1:3]])
-            elif t[0] == "f":
-                poly = []
-                for tok in t[1:]:
-                    parts = tok.split("/")
-                    fix = lambda s, n: (int(s) - 1 if int(s) > 0 else n + int(s)) if s else -1
-                    vi = fix(parts[0], len(V))
-                    ti = fix(parts[1], len(VT)) if len(parts) > 1 else -1
-                    ni = fix(parts[2], len(VN)) if len(parts) > 2 else -1
-                    poly.append((vi, ti, ni))
-                for k in range(1, len(poly) - 1):
-                    faces.append((poly[0], poly[k], poly[k + 1], shape, mtl))
-                    has_faces = True
-            elif t[0] in ("o", "g"):
-                nm = " ".join(t[1:])
-                if has_faces:
-                    names.append(nm)
-                    shape += 1
-                    has_faces = False
-                else:
-                    names[shape] = nm
-            elif t[0] == "usemtl":
-                mtl = " ".join(t[1:])
-    V = np.array(V, np.float32).reshape(-1, 3)
-    VN = np.array(VN, np.float32).reshape(-1, 3)
-    VT = np.array(VT, np.float32).reshape(-1, 2)
-    meshes = {}
-    order = []
-    stem = os.path.splitext(os.path.basename(path))[0]
-    for (a, b, c, sh, mt) in faces:
-        key = (sh, mt)
-        if key not in meshes:
-            base = names[sh] if names[sh] else f"{stem}_shape_{sh}"
-            meshes[key] = {"name": base + ("::" + mt if mt else ""), "material": mt, "pos": [], "nrm": [], "uv": []}
-            order.append(key)
-        m = meshes[key]
-        p = _xform_points(xf, V[[a[0], b[0], c[0]]])
-        has_n = all(x[2] >= 0 for x in (a, b, c))
-        if has_n:
-            n = VN[[a[2], b[2], c[2]]]
-            n2 = np.empty_like(n)
-            for r in range(3):
-                n2[:, r] = ((xf_inv[0, r] * n[:, 0] + xf_inv[1, r] * n[:, 1]) + xf_inv[2, r] * n[:, 2]) + xf_inv[3, r] * np.float32(1)
-            n = _normalize(n2)
-        else:
-            fn = _normalize(_cross(p[1] - p[0], p[2] - p[0])[None, :])[0]
-            n = np.stack([fn, fn, fn])
-        uv = np.zeros((3, 2), np.float32)
-        for k, x in enumerate((a, b, c)):
-            if x[1] >= 0:
-                uv[k] = VT[x[1]]
-        m["pos"].append(p.reshape(9))
-        m["nrm"].append(n.reshape(9))
-        m["uv"].append(uv.reshape(6))
-    return [meshes[k] for k in order]
 
 
 def _read_env(path: str):
@@ -479,31 +371,9 @@ def parse_scene(json_path: str) -> SceneArrays:
     for o in doc["objects"]:
         if "name" not in o or "path" not in o:
             continue
-        tr = np.eye(4, dtype=np.float32)
-        rot = np.eye(4, dtype=np.float32)
-        scl = np.eye(4, dtype=np.float32)
-        if "translate" in o:
-            tr[0:3, 3] = o["translate"]
-        for r in o.get("rotate", []):
-            for ax, key in enumerate(("rotateX", "rotateY", "rotateZ")):
-                if key in r:
-                    rad = _deg2rad(np.float32(r[key]))
-                    c, s = np.float32(np.cos(rad)), np.float32(np.sin(rad))
-                    R = np.eye(4, dtype=np.float32)
-                    i0, i1 = [(1, 2), (0, 2), (0, 1)][ax]
-                    R[i0, i0], R[i1, i1] = c, c
-                    if ax == 1:
-                        R[0, 2], R[2, 0] = s, -s
-                    else:
-                        R[i0, i1], R[i1, i0] = -s, s
-                    rot = _mat4_mul(R, rot)
-                    break
-        if "scale" in o:
-            scl[0, 0], scl[1, 1], scl[2, 2] = o["scale"]
-        xf = _mat4_mul(_mat4_mul(tr, rot), scl)
-        xf_inv = np.linalg.inv(xf.astype(np.float64)).astype(np.float32)
+        xf, xf_inv = rtref_obj.object_transform(o)
         path = o["path"] if os.path.isabs(o["path"]) else os.path.join(base, o["path"])
-        loaded = _load_obj(path, xf, xf_inv)
+        loaded = rtref_obj.load_obj(path, xf, xf_inv)
         expanded = len(loaded) > 1
         for lm in loaded:
             if "material" in o:
@@ -519,10 +389,10 @@ def parse_scene(json_path: str) -> SceneArrays:
             mid = len(mmat)
             mmat.append(name_to_mat[mn])
             mnames.append(o["name"] + "::" + lm["name"] if expanded else o["name"])
-            pos += lm["pos"]
-            nrm += lm["nrm"]
-            uv += lm["uv"]
-            mesh += [mid] * len(lm["pos"])
+            pos.append(lm["pos"])
+            nrm.append(lm["nrm"])
+            uv.append(lm["uv"])
+            mesh.append(np.full(len(lm["pos"]), mid, np.int32))
     mesh_light = np.full(max(1, len(mmat)), -1, np.int32)
     lj = doc["lights"]
     lt = np.zeros(max(1, len(lj)), np.int32)
@@ -543,10 +413,10 @@ def parse_scene(json_path: str) -> SceneArrays:
             p = L["path"] if os.path.isabs(L["path"]) else os.path.join(base, L["path"])
             env = _read_env(p)
     return SceneArrays(
-        pos=np.ascontiguousarray(np.array(pos, np.float32).reshape(-1, 9)),
-        nrm=np.ascontiguousarray(np.array(nrm, np.float32).reshape(-1, 9)),
-        uv=np.ascontiguousarray(np.array(uv, np.float32).reshape(-1, 6)),
-        mesh=np.array(mesh, np.int32), mesh_material=np.array(mmat if mmat else [0], np.int32),
+        pos=np.ascontiguousarray(np.concatenate(pos or [np.zeros((0, 9), np.float32)]).astype(np.float32)),
+        nrm=np.ascontiguousarray(np.concatenate(nrm or [np.zeros((0, 9), np.float32)]).astype(np.float32)),
+        uv=np.ascontiguousarray(np.concatenate(uv or [np.zeros((0, 6), np.float32)]).astype(np.float32)),
+        mesh=np.concatenate(mesh or [np.zeros(0, np.int32)]).astype(np.int32), mesh_material=np.array(mmat if mmat else [0], np.int32),
         mesh_light=mesh_light, materials=materials, light_type=lt, light_mesh=lmsh, light_emission=lem,
         env_w=env[0], env_h=env[1], env_rgb=env[2], width=int(W), height=int(H), fovy=float(cam.get("fov", 45.0)),
         direction=tuple(cam.get("direction", [0, 0, -1])), max_bounces=int(cam.get("max_bounces", 5)),
