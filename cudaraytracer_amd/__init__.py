@@ -106,7 +106,7 @@ class RenderParams(ctypes.Structure):
         ("max_bounces", ctypes.c_int), ("rr", ctypes.c_float), ("rr_start_depth", ctypes.c_int),
         ("seed", ctypes.c_int64), ("batch_spp", ctypes.c_int), ("queue_budget_bytes", ctypes.c_int64),
         ("flush_survivors", ctypes.c_int), ("profile", ctypes.c_int), ("env_sampling", ctypes.c_int),
-        ("disable_graphs", ctypes.c_int), ("ray_sort", ctypes.c_int),
+        ("disable_graphs", ctypes.c_int), ("ray_sort", ctypes.c_int), ("shadow_any_hit", ctypes.c_int),
     ]
 
 
@@ -154,7 +154,7 @@ class DenoiseParams(ctypes.Structure):
 
 class PoolParams(ctypes.Structure):
     _fields_ = [("queue_budget_bytes", ctypes.c_int64), ("hbm_budget_bytes", ctypes.c_int64), ("profile", ctypes.c_int),
-                ("cohort_ticks", ctypes.c_int)]
+                ("cohort_ticks", ctypes.c_int), ("max_realtime_jobs", ctypes.c_int)]
 
 
 class JobDescC(ctypes.Structure):
@@ -186,7 +186,7 @@ class TuneReport(ctypes.Structure):
 
 
 class ServeRequestC(ctypes.Structure):
-    _fields_ = [("arrival_s", ctypes.c_double), ("job", JobDescC)]
+    _fields_ = [("arrival_s", ctypes.c_double), ("job", JobDescC), ("result_rgba", ctypes.POINTER(ctypes.c_float))]
 
 
 class ServeReport(ctypes.Structure):
@@ -412,7 +412,8 @@ class Renderer:
 
     def __init__(self, scene: Scene, max_bounces: int = -1, rr: float = -1.0, rr_start_depth: int = -1,
                  seed: int = -1, batch_spp: int = 0, queue_budget_bytes: int = 0, flush_survivors: bool = False,
-                 profile: bool = False, env_alias: bool = False, graphs: bool = True, ray_sort: bool = False):
+                 profile: bool = False, env_alias: bool = False, graphs: bool = True, ray_sort: bool = False,
+                 shadow_any_hit: bool = False):
         p = RenderParams()
         lib().rt_render_params_default(ctypes.byref(p))
         p.max_bounces, p.rr, p.rr_start_depth, p.seed = max_bounces, rr, rr_start_depth, seed
@@ -421,6 +422,7 @@ class Renderer:
         p.env_sampling = int(env_alias)
         p.disable_graphs = 0 if graphs else 1
         p.ray_sort = int(ray_sort)
+        p.shadow_any_hit = int(shadow_any_hit)
         self.scene = scene
         self._h = ctypes.c_void_p()
         _check(lib().rt_renderer_create(scene.handle, ctypes.byref(p), ctypes.byref(self._h)))
@@ -533,8 +535,8 @@ class Pool:
     """Many jobs over resident scenes, served by one wavefront engine."""
 
     def __init__(self, queue_budget_bytes: int = 0, hbm_budget_bytes: int = 0, profile: bool = False,
-                 pipelined: bool = False):
-        p = PoolParams(queue_budget_bytes, hbm_budget_bytes, int(profile), int(pipelined))
+                 pipelined: bool = False, max_realtime_jobs: int = 0):
+        p = PoolParams(queue_budget_bytes, hbm_budget_bytes, int(profile), int(pipelined), int(max_realtime_jobs))
         self._h = ctypes.c_void_p()
         _check(lib().rt_pool_create(ctypes.byref(p), ctypes.byref(self._h)))
         self._scenes = []
@@ -617,9 +619,14 @@ class Pool:
         _check(lib().rt_pool_autotune(self._h, arr, len(sample_jobs), ctypes.byref(rep)))
         return rep.as_dict()
 
-    def serve(self, requests, batched: bool = True):
-        """requests: list of (arrival_s, Job).  Returns (report dict, per-request frame
-        latencies in ms, per-request job latencies in ms)."""
+    def serve(self, requests, batched=True, results=None):
+        """requests: list of (arrival_s, Job).  batched: True (continuous batching), False
+        (the plain loop, a fresh renderer per job) or "resident" (the plain loop on a
+        renderer kept per scene).  results: optional list, one float32 array per request
+        (or None) of the job's image shape (h, w, 4), filled with its result when it
+        completes (accumulate: mean colour; real-time: last denoised frame).  Returns
+        (report dict, per-request frame latencies in ms, per-request job latencies in ms)."""
+        mode = 2 if batched == "resident" else (1 if batched else 0)
         keep: list = []
         n = len(requests)
         arr = (ServeRequestC * n)()
@@ -627,11 +634,15 @@ class Pool:
         for k, (t, job) in enumerate(requests):
             arr[k].arrival_s = t
             arr[k].job = job.to_c(keep)
+            if results is not None and results[k] is not None:
+                buf = results[k]
+                assert buf.dtype == np.float32 and buf.flags.c_contiguous
+                arr[k].result_rgba = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
             nframes.append(max(1, job.frames))
         fl = np.zeros(sum(nframes), np.float64)
         jl = np.zeros(n, np.float64)
         rep = ServeReport()
-        _check(lib().rt_pool_serve(self._h, arr, n, int(batched), ctypes.byref(rep),
+        _check(lib().rt_pool_serve(self._h, arr, n, mode, ctypes.byref(rep),
                                    fl.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                    jl.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         per_req, o = [], 0

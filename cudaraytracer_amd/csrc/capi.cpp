@@ -94,6 +94,8 @@ rt::RenderSettings settings_from(const rt_render_params* p, const rt::HostScene&
     set.use_graphs = p->disable_graphs ? 0 : 1;
     if (p->ray_sort != 0 && p->ray_sort != 1) throw std::invalid_argument("ray_sort must be 0 or 1");
     set.ray_sort = p->ray_sort;
+    if (p->shadow_any_hit != 0 && p->shadow_any_hit != 1) throw std::invalid_argument("shadow_any_hit must be 0 or 1");
+    set.shadow_any_hit = p->shadow_any_hit;
     if (set.max_bounces > 255) throw std::invalid_argument("max_bounces must be <= 255");
     return set;
 }
@@ -152,6 +154,7 @@ void rt_render_params_default(rt_render_params* p) {
     p->env_sampling = 0;
     p->disable_graphs = 0;
     p->ray_sort = 0;
+    p->shadow_any_hit = 0;
 }
 
 void rt_denoise_params_default(rt_denoise_params* p) {
@@ -525,6 +528,8 @@ int rt_pool_create(const rt_pool_params* p, rt_pool** out) {
             s.hbm_budget_bytes = (size_t)p->hbm_budget_bytes;
             s.profile = p->profile;
             s.pipeline = p->cohort_ticks ? 1 : 0;
+            if (p->max_realtime_jobs < 0) throw std::invalid_argument("max_realtime_jobs must be >= 0");
+            s.max_realtime = p->max_realtime_jobs;
         }
         std::unique_ptr<rt_pool> pool(new rt_pool);
         pool->p.reset(new rt::Pool(s));
@@ -692,8 +697,9 @@ int rt_pool_serve(rt_pool* p, const rt_serve_request* reqs, int n, int batched, 
             if (!(reqs[k].arrival_s >= 0.0)) throw std::invalid_argument("arrival times must be >= 0");
             rq[k].arrival_s = reqs[k].arrival_s;
             rq[k].job = job_from(p, &reqs[k].job);
+            rq[k].result = reqs[k].result_rgba;
         }
-        rt::ServeReport rep = rt::serve(*p->p, rq, batched != 0);
+        rt::ServeReport rep = rt::serve(*p->p, rq, batched);
         std::vector<double> frames;
         for (const auto& f : rep.frame_ms) frames.insert(frames.end(), f.begin(), f.end());
         std::memset(out, 0, sizeof(*out));

@@ -107,13 +107,16 @@ public:
     // Reorder each bounce's extension rays by direction octant and origin
     // cell before extend (kSortBytesPerPath more device memory per path).
     void set_ray_sort(bool on);
+    // Shadow rays stop at the first hit clearly in front of the shading point
+    // (the any-hit fast mode) instead of finding the closest hit.
+    void set_shadow_any_hit(bool on) { shadow_any_ = on; }
     float grid_scale() const { return grid_scale_; }
 
 private:
     template <bool POOLED>
     void run_t(int max_bounces, const bool* classes, bool count, int stack_entries);
     struct Grids {
-        int ext = 0, shadow = 0, shade[kNumClasses] = {};
+        int ext = 0, shadow = 0, shadow_any = 0, shade[kNumClasses] = {};
     };
     template <bool POOLED>
     Grids grids_for(int stack_entries);
@@ -126,6 +129,7 @@ private:
     hipStream_t stream_ = nullptr;
     DevBuf qo_[2], qd_[2], qw_[2], hits_, lists_, shadow_, slot_d_, slot_i_, counters_, stats_;
     bool ray_sort_ = false;
+    bool shadow_any_ = false;
     DevBuf keys_, perm_, sort_hist_, sort_cur_;  // ray sort: key and permutation per path, per-slice bins
     DevBuf sorted_o_, sorted_d_;                 // ray sort: extend's copy of each ray at its sorted position
     // Pass tables: one device table, filled by stream-ordered copies from a

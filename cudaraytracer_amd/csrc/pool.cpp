@@ -224,6 +224,8 @@ PassItem pass_item(const GpuScene* gpu, const JobDesc& d, const CameraRec& cam, 
 }
 }  // namespace
 
+int Pool::max_realtime() const { return impl_->set.max_realtime; }
+
 int Pool::step() {
     Impl& I = *impl_;
     if (I.active.empty()) return 0;
@@ -580,9 +582,11 @@ void record(ServeReport& rep, size_t req, double arrival, const std::vector<doub
 
 }  // namespace
 
-ServeReport serve(Pool& pool, const std::vector<ServeRequest>& reqs, bool batched) {
+ServeReport serve(Pool& pool, const std::vector<ServeRequest>& reqs, int mode) {
     ServeReport rep;
     if (reqs.empty()) return rep;
+    if (mode != kServePlain && mode != kServeBatched && mode != kServePlainResident)
+        throw std::invalid_argument("serve mode must be 0 (plain), 1 (batched) or 2 (plain, resident renderers)");
     rep.frame_ms.resize(reqs.size());
     rep.job_ms.assign(reqs.size(), 0.0);
     for (const auto& rq : reqs) pool.check(rq.job);
@@ -592,17 +596,22 @@ ServeReport serve(Pool& pool, const std::vector<ServeRequest>& reqs, bool batche
                      [&](size_t a, size_t b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
     const double base = pool.now();
     double last_done = 0;
-    if (batched) {
+    if (mode == kServeBatched) {
+        const int cap = pool.max_realtime();
         std::deque<size_t> waiting;  // arrived, not yet admitted
         std::map<int, size_t> running;
+        int running_rt = 0;
         size_t next = 0;
         while (next < order.size() || !waiting.empty() || !running.empty()) {
             const double t = pool.now() - base;
             while (next < order.size() && reqs[order[next]].arrival_s <= t) waiting.push_back(order[next++]);
-            while (!waiting.empty()) {
-                int id = pool.submit(reqs[waiting.front()].job);
+            while (!waiting.empty()) {  // admission in arrival order
+                const JobDesc& d = reqs[waiting.front()].job;
+                if (cap > 0 && d.frames > 0 && running_rt >= cap) break;
+                int id = pool.submit(d);
                 if (id < 0) break;  // over the HBM budget: admitted when a running job finishes
                 running[id] = waiting.front();
+                running_rt += d.frames > 0 ? 1 : 0;
                 waiting.pop_front();
             }
             if (running.empty()) {
@@ -626,13 +635,31 @@ ServeReport serve(Pool& pool, const std::vector<ServeRequest>& reqs, bool batche
                 }
                 record(rep, it->second, rq.arrival_s, done);
                 last_done = std::max(last_done, done.back());
+                if (rq.result) {
+                    if (rq.job.frames > 0) pool.read_frame(it->first, 0 /* denoised */, rq.result);
+                    else pool.read_accum(it->first, 2 /* mean */, rq.result);
+                }
+                running_rt -= rq.job.frames > 0 ? 1 : 0;
                 pool.release(it->first);
                 it = running.erase(it);
             }
         }
     } else {
-        // Plain: one job at a time in arrival order, each with a fresh
-        // single-scene renderer whose queues are sized for its own samples.
+        // One job at a time in arrival order: a fresh single-scene renderer whose
+        // queues are sized for its own samples, or (resident) a renderer kept per
+        // scene and settings, its history and accumulation reset for each job.
+        struct Resident {
+            RenderSettings set;
+            std::unique_ptr<Renderer> r;
+        };
+        std::map<int, Resident> resident;
+        auto same = [](const RenderSettings& a, const RenderSettings& b) {
+            return a.max_bounces == b.max_bounces && a.rr == b.rr && a.rr_start_depth == b.rr_start_depth &&
+                   a.seed == b.seed && a.batch_spp == b.batch_spp && a.queue_budget_bytes == b.queue_budget_bytes &&
+                   a.flush_survivors == b.flush_survivors && a.env_alias == b.env_alias &&
+                   a.use_graphs == b.use_graphs && a.ray_sort == b.ray_sort && a.shadow_any_hit == b.shadow_any_hit &&
+                   a.profile == b.profile;
+        };
         for (size_t k : order) {
             const ServeRequest& rq = reqs[k];
             wait_until(pool, base + rq.arrival_s);
@@ -640,22 +667,38 @@ ServeReport serve(Pool& pool, const std::vector<ServeRequest>& reqs, bool batche
             RenderSettings set = d.set;
             set.batch_spp = d.frames > 0 ? d.spp : std::min(d.spp, 64);
             std::vector<double> done;
-            {
-                Renderer r(pool.scene_gpu(d.scene), pool.scene_host(d.scene), set);
-                if (d.frames > 0) {
-                    for (int f = 0; f < d.frames; ++f) {
-                        r.set_camera(d.cams[f]);
-                        r.render_frame(d.spp, d.svgf);
-                        r.synchronize();
-                        done.push_back(pool.now() - base);
-                    }
-                } else {
-                    if (!d.cams.empty()) r.set_camera(d.cams[0]);
-                    r.render(d.spp);
-                    r.synchronize();
+            std::unique_ptr<Renderer> fresh;
+            Renderer* r = nullptr;
+            if (mode == kServePlainResident) {
+                Resident& res = resident[d.scene];
+                if (!res.r || !same(res.set, set)) {
+                    res.r.reset();
+                    res.r.reset(new Renderer(pool.scene_gpu(d.scene), pool.scene_host(d.scene), set));
+                    res.set = set;
+                }
+                r = res.r.get();
+                r->reset_history();
+                r->reset();
+            } else {
+                fresh.reset(new Renderer(pool.scene_gpu(d.scene), pool.scene_host(d.scene), set));
+                r = fresh.get();
+            }
+            if (d.frames > 0) {
+                for (int f = 0; f < d.frames; ++f) {
+                    r->set_camera(d.cams[f]);
+                    r->render_frame(d.spp, d.svgf);
+                    r->synchronize();
                     done.push_back(pool.now() - base);
                 }
+                if (rq.result) r->read_frame(0 /* denoised */, rq.result);
+            } else {
+                if (!d.cams.empty()) r->set_camera(d.cams[0]);
+                r->render(d.spp);
+                r->synchronize();
+                done.push_back(pool.now() - base);
+                if (rq.result) r->read_accum(2 /* mean */, rq.result);
             }
+            fresh.reset();
             record(rep, k, rq.arrival_s, done);
             last_done = std::max(last_done, done.back());
         }

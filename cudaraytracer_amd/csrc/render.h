@@ -55,6 +55,7 @@ struct RenderSettings {
     int env_alias = 0;               // 1: environment sampled with alias tables (not the reference's CDF search)
     int use_graphs = 1;              // replay passes as captured HIP graphs
     int ray_sort = 0;                // 1: reorder extension rays by direction octant and origin cell
+    int shadow_any_hit = 0;          // 1: shadow rays stop at the first hit clearly in front of the shading point
     int profile = 0;                 // per-stage HIP event timing
 };
 
@@ -150,6 +151,8 @@ struct PoolSettings {
     size_t hbm_budget_bytes = 0;    // resident scenes + job state; 0: 64 GiB
     int profile = 0;
     int pipeline = 0;               // 1: two cohorts, one traced while the other is filtered
+    int max_realtime = 0;           // serve's continuous batching admits at most this many real-time
+                                    // jobs at once (0: as many as the HBM budget allows)
 };
 
 struct JobDesc {
@@ -185,6 +188,7 @@ public:
     const GpuScene& scene_gpu(int slot) const;
     // One tick over the active jobs (synchronous); returns the number of jobs served.
     int step();
+    int max_realtime() const;
     bool idle() const;
     const JobStatus& status(int job) const;
     void job_size(int job, int& width, int& height) const;
@@ -232,14 +236,22 @@ private:
 struct ServeRequest {
     double arrival_s = 0;
     JobDesc job;
+    // Optional, width * height * 4 floats: receives the job's result when it completes -- an
+    // accumulate job's (direct + indirect) / spp, a real-time job's last denoised frame.
+    float* result = nullptr;
 };
+// How serve() runs the stream: plain = one job at a time with a fresh
+// single-scene renderer each; batched = the pool's continuous batching;
+// plain_resident = one job at a time on a single-scene renderer kept per scene
+// (history and accumulation reset per job).
+enum ServeMode : int { kServePlain = 0, kServeBatched = 1, kServePlainResident = 2 };
 struct ServeReport {
     int jobs = 0, frames = 0;
     double makespan_s = 0;                        // first arrival to last completion
     std::vector<std::vector<double>> frame_ms;    // per request, per frame
     std::vector<double> job_ms;                   // per request
 };
-ServeReport serve(Pool& pool, const std::vector<ServeRequest>& reqs, bool batched);
+ServeReport serve(Pool& pool, const std::vector<ServeRequest>& reqs, int mode);
 
 // Closest hit for a batch of rays laid out as (o.xyz, tmin, d.xyz, tmax):
 // out_hits = (t, b1, b2, prim bits) per ray, out_counts = (nodes, tris) per ray.

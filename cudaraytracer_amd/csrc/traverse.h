@@ -93,9 +93,15 @@ using LdsU32 = __attribute__((address_space(3))) uint32_t;
 
 // `stk` points at this lane's stack column in LDS; entries are `stride` words apart.
 // OCT >= 0: the ray's inverse-direction signs (see box_hit), known for the whole wave.
-template <bool COUNT, bool CULL = true, int OCT = -1>
+// EARLY: stop at the first accepted triangle whose t is below t_exit and that
+// the ray does not graze (|cos| between ray and triangle normal >= 1e-3); the
+// hit then has prim = kEarlyExit.  Without such a triangle the result is the
+// closest hit, as with EARLY off.
+constexpr int kEarlyExit = -2;
+template <bool COUNT, bool CULL = true, int OCT = -1, bool EARLY = false>
 __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r, float tmin, float tmax,
-                                             uint32_t* stk_generic, int stride, TravCount& cnt) {
+                                             uint32_t* stk_generic, int stride, TravCount& cnt,
+                                             float t_exit = 0.f) {
     LdsU32* stk = (LdsU32*)stk_generic;
     GlobalV4* nodes = (GlobalV4*)sc.nodes;  // 4 vectors per node
     GlobalV4* tris = (GlobalV4*)sc.tri_isect;
@@ -114,7 +120,18 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r
                 GlobalV4* tp = tris + 3 * (size_t)(first + k);
                 float4 a = as_f4(tp[0]), e1 = as_f4(tp[1]), e2 = as_f4(tp[2]);
                 if (COUNT) cnt.tris++;
-                if (tri_hit<CULL>(r, a, e1, e2, h.t, h.b1, h.b2)) h.prim = (int)(first + k);
+                if (tri_hit<CULL>(r, a, e1, e2, h.t, h.b1, h.b2)) {
+                    h.prim = (int)(first + k);
+                    if (EARLY && h.t < t_exit) {
+                        vec3 ve1 = v3(e1.x, e1.y, e1.z), ve2 = v3(e2.x, e2.y, e2.z);
+                        float det = dot(ve1, cross(r.d, ve2));
+                        vec3 n = cross(ve1, ve2);
+                        if (det * det >= 1e-6f * dot(n, n)) {
+                            h.prim = kEarlyExit;
+                            return h;
+                        }
+                    }
+                }
             }
         } else {
             GlobalV4* np = nodes + 4 * (size_t)ref;
@@ -159,24 +176,24 @@ RT_HD int ray_octant(const RayInv& r) { return (r.nx ? 1 : 0) | (r.ny ? 2 : 0) |
 // active lane of the wave has the same octant (after the ray sort most waves
 // do, and so do most camera rays); otherwise the general loop.  Same hits
 // either way.
-template <bool COUNT, bool CULL = true>
+template <bool COUNT, bool CULL = true, bool EARLY = false>
 __device__ __forceinline__ Hit trace_closest_wave(const DevScene& sc, const RayInv& r, float tmin, float tmax,
-                                                  uint32_t* stk, int stride, TravCount& cnt) {
+                                                  uint32_t* stk, int stride, TravCount& cnt, float t_exit = 0.f) {
     const int oct = ray_octant(r);
     const int first = __builtin_amdgcn_readfirstlane(oct);
     if (__ballot(oct != first) == 0ull) {
         switch (first) {
-            case 0: return trace_closest<COUNT, CULL, 0>(sc, r, tmin, tmax, stk, stride, cnt);
-            case 1: return trace_closest<COUNT, CULL, 1>(sc, r, tmin, tmax, stk, stride, cnt);
-            case 2: return trace_closest<COUNT, CULL, 2>(sc, r, tmin, tmax, stk, stride, cnt);
-            case 3: return trace_closest<COUNT, CULL, 3>(sc, r, tmin, tmax, stk, stride, cnt);
-            case 4: return trace_closest<COUNT, CULL, 4>(sc, r, tmin, tmax, stk, stride, cnt);
-            case 5: return trace_closest<COUNT, CULL, 5>(sc, r, tmin, tmax, stk, stride, cnt);
-            case 6: return trace_closest<COUNT, CULL, 6>(sc, r, tmin, tmax, stk, stride, cnt);
-            default: return trace_closest<COUNT, CULL, 7>(sc, r, tmin, tmax, stk, stride, cnt);
+            case 0: return trace_closest<COUNT, CULL, 0, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
+            case 1: return trace_closest<COUNT, CULL, 1, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
+            case 2: return trace_closest<COUNT, CULL, 2, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
+            case 3: return trace_closest<COUNT, CULL, 3, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
+            case 4: return trace_closest<COUNT, CULL, 4, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
+            case 5: return trace_closest<COUNT, CULL, 5, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
+            case 6: return trace_closest<COUNT, CULL, 6, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
+            default: return trace_closest<COUNT, CULL, 7, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
         }
     }
-    return trace_closest<COUNT, CULL>(sc, r, tmin, tmax, stk, stride, cnt);
+    return trace_closest<COUNT, CULL, -1, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
 }
 
 // Shading data at a hit, reconstructed from barycentrics the way the

@@ -147,6 +147,13 @@ typedef struct rt_render_params {
     int ray_sort;           /* 1 = before each extend, reorder the rays of every queue slice by direction
                                octant and origin cell (38 more bytes per path; same results, faster
                                traversal on coherent scenes such as closed rooms) */
+    int shadow_any_hit;     /* 0 = the reference's shadow test (Light::Visible, Src/Core/light.cu:5-27):
+                               the closest hit from the light point, then its 0.1 tolerances;
+                               1 = fast mode: a shadow ray stops at the first hit that lies, by its t,
+                               more than 0.11 (plus a rounding margin) in front of the shading point on a
+                               triangle it does not graze, and counts as occluded -- what the closest-hit
+                               test decides whenever the closest hit lies on the ray where its t says;
+                               rays without such a hit are traced exactly as in mode 0 */
 } rt_render_params;
 
 typedef struct rt_stats_out {
@@ -264,6 +271,9 @@ typedef struct rt_pool_params {
     int cohort_ticks;           /* 0 = a tick traces and filters every real-time job on one stream;
                                    1 = two cohorts alternate, one traced while the other is filtered
                                    on a second stream (a real-time job advances every second tick) */
+    int max_realtime_jobs;      /* rt_pool_serve's continuous batching admits at most this many real-time
+                                   jobs at once (later ones wait in arrival order), which bounds the work of
+                                   a tick and so frame latency; 0 = as many as the HBM budget allows */
 } rt_pool_params;
 
 typedef struct rt_job_desc {
@@ -285,6 +295,9 @@ typedef struct rt_job_status {
 typedef struct rt_serve_request {
     double arrival_s; /* seconds after the start of the run */
     rt_job_desc job;
+    float* result_rgba; /* optional (NULL: none), width*height*4 floats, row 0 = bottom: receives the job's
+                           result when it completes -- an accumulate job's (direct + indirect) / spp
+                           (RT_ACCUM_MEAN), a real-time job's last denoised frame (RT_FRAME_DENOISED) */
 } rt_serve_request;
 
 typedef struct rt_serve_report {
@@ -333,7 +346,9 @@ int rt_pool_get_tuning(rt_pool* p, rt_pool_tuning* out);
    The pool must be idle. */
 int rt_pool_autotune(rt_pool* p, const rt_job_desc* sample, int n, rt_tune_report* out);
 /* Replay an arrival stream: batched = 1 serves it with the pool's continuous batching,
-   0 with the plain loop (one job at a time, a fresh single-scene renderer each).  Frame
+   0 with the plain loop (one job at a time, a fresh single-scene renderer each), 2 with the
+   plain loop on resident renderers (one single-scene renderer kept per scene, its history
+   and accumulation reset for each job).  Frame
    latency = completion - (arrival for a job's first frame, the previous frame's completion
    otherwise); job latency = last completion - arrival.  Optional outputs: frame latencies
    in request order then frame order, job latencies in request order. */

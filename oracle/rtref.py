@@ -313,7 +313,13 @@ class SceneArrays:
     textures: list = field(default_factory=list)  # (h, w, 4) uint8 per "textures" entry
 
 
-def parse_scene(json_path: str) -> SceneArrays:
+def parse_scene(json_path: str, geometry=None) -> SceneArrays:
+    """The scene as Scene::ParseSceneFile reads it.  geometry: optional, one
+    list of meshes per entry of "objects" ({name, material, pos, nrm, uv}, as
+    rtref_obj.load_obj returns them) used instead of reading the OBJ files, for
+    scenes too large for this module's Python OBJ reader (the reader itself is
+    checked against the library's importer and the reference's libraries on
+    smaller files)."""
     base = os.path.dirname(os.path.abspath(json_path))
     with open(json_path, encoding="utf-8-sig") as f:  # a leading BOM is skipped, as nlohmann does
         doc = json.load(f)
@@ -368,12 +374,15 @@ def parse_scene(json_path: str) -> SceneArrays:
             r[14] = tex_index[m["diffuse_texture"]] + 1
         name_to_mat[m["name"]] = i
     pos, nrm, uv, mesh, mmat, mnames = [], [], [], [], [], []
-    for o in doc["objects"]:
+    for oi, o in enumerate(doc["objects"]):
         if "name" not in o or "path" not in o:
             continue
-        xf, xf_inv = rtref_obj.object_transform(o)
-        path = o["path"] if os.path.isabs(o["path"]) else os.path.join(base, o["path"])
-        loaded = rtref_obj.load_obj(path, xf, xf_inv)
+        if geometry is not None:
+            loaded = geometry[oi]
+        else:
+            xf, xf_inv = rtref_obj.object_transform(o)
+            path = o["path"] if os.path.isabs(o["path"]) else os.path.join(base, o["path"])
+            loaded = rtref_obj.load_obj(path, xf, xf_inv)
         expanded = len(loaded) > 1
         for lm in loaded:
             if "material" in o:
@@ -423,6 +432,19 @@ def parse_scene(json_path: str) -> SceneArrays:
         rr=float(cam.get("rr", 0.8)), spp=int(cam.get("spp", 16)), seed=int(cam.get("seed", 1234)),
         rr_start_depth=int(cam.get("rr_start_depth", 0)), num_lights=len(lj), textures=textures,
     )
+
+
+def geometry_from_arrays(doc_objects: list, pos, nrm, uv, mesh_start, mesh_count):
+    """`geometry` for parse_scene from scene-order triangle arrays holding one
+    mesh per object, in object order (a loader's output for scenes whose
+    objects each import as a single mesh)."""
+    if len(mesh_start) != len(doc_objects):
+        raise ValueError("geometry_from_arrays needs exactly one mesh per object")
+    out = []
+    for o, s0, n in zip(doc_objects, mesh_start, mesh_count):
+        sl = slice(int(s0), int(s0) + int(n))
+        out.append([{"name": o.get("name", ""), "material": "", "pos": pos[sl], "nrm": nrm[sl], "uv": uv[sl]}])
+    return out
 
 
 class RefScene:

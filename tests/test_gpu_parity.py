@@ -284,3 +284,23 @@ def test_ray_sort_svgf_frames_match_unsorted(scene_dir):
         a.render_frame(1)
         b.render_frame(1)
         assert np.array_equal(_bits(a.read_frame(rt.RT_FRAME_DENOISED)), _bits(b.read_frame(rt.RT_FRAME_DENOISED)))
+
+
+@pytest.mark.parametrize("name", ["cornell_small", "cornell_spheres_128", "terrain_small", "textured_cornell"])
+def test_shadow_any_hit_mode_matches_closest_hit(scene_dir, name):
+    """The shadow fast mode (rt_render_params.shadow_any_hit) stops occluded
+    shadow rays at the first hit clearly in front of the shading point; on
+    these scenes every visibility decision, and so every pixel, is the
+    closest-hit test's, bit for bit, with fewer node visits."""
+    path = scene_dir[name]
+    s = rt.Scene(path)
+    out = []
+    for any_hit in (False, True):
+        r = rt.Renderer(s, shadow_any_hit=any_hit)
+        r.render(4, count_traversal=True)
+        out.append((r.read_accum(rt.RT_ACCUM_DIRECT), r.read_accum(rt.RT_ACCUM_INDIRECT), r.stats()))
+    (d0, i0, s0), (d1, i1, s1) = out
+    assert np.array_equal(_bits(d0), _bits(d1)) and np.array_equal(_bits(i0), _bits(i1))
+    assert s0.shadow_rays == s1.shadow_rays and s0.extension_node_visits == s1.extension_node_visits
+    assert s1.shadow_node_visits < s0.shadow_node_visits
+    print(f"{name}: shadow node visits {s0.shadow_node_visits} -> {s1.shadow_node_visits}", flush=True)

@@ -101,7 +101,7 @@ def test_svgf_frames_bit_exact_cornell(scene_dir, oracle_lib):
 
 
 def test_svgf_frames_bit_exact_terrain_env_sss(scene_dir, oracle_lib):
-    _run(scene_dir["terrain_small"], spp=2, atrous_iters=3)
+    _run(scene_dir["terrain_small"], spp=2, atrous_iters=5)  # the real-time workload's 5 passes
 
 
 def test_svgf_frames_bit_exact_textured(scene_dir, oracle_lib):
