@@ -1,9 +1,10 @@
 """Benchmark: wavefront path tracing throughput on one MI355X.
 
-Workload (one step): the Cornell shell + 4 icospheres scene (20,516 triangles,
-GGX metallic-roughness, one IOR-1.5 dielectric, area light with MIS), 512x512,
-64 samples per pixel, max depth 12, roulette after depth 3 -- rendered in full.
-Inputs (BVH, triangles, queues) are resident in HBM before timing starts.
+Headline workload (one step): the large-mesh scene -- a 2,000,000-triangle fBm
+terrain plus a subsurface icosphere under a procedural 2048x1024 HDR sky
+(peak 1e4 sun lobe) sampled through alias tables -- at 1920x1080, 32 samples
+per pixel, max depth 16, rendered in full.  Inputs (BVH, triangles, env map,
+queues) are resident in HBM before timing starts.
 
 Prints one JSON line.  `value` is Mrays/s over the timed steps (extension +
 shadow rays / GPU wall time).  `roofline` is for the kernel with the largest
@@ -11,15 +12,18 @@ share of GPU time, from per-launch HIP events recorded on the renderer's own
 stream and the algorithmic-bytes model below (counted node visits and triangle
 tests times their record sizes); `traffic` is the HBM bytes per launch that
 rocprofv3's FETCH_SIZE / WRITE_SIZE counters measured for the same kernel,
-scaled by what they read for a plain 1 GiB copy (the table under profiles/,
-written by --traffic-from-pmc).  `cpu_baseline` times the CPU oracle (OpenMP,
-all host cores) on a bounded sample of the same workload.
+scaled by what they read for a plain 1 GiB copy (the tables under profiles/,
+written by --traffic-from-pmc; the shade stage is counted per bounce, all
+material classes together).  `cpu_baseline` times the CPU oracle (OpenMP, all
+host cores) on a bounded sample of the same workload.  Secondary sections: the
+spheres workload (512x512, 64 spp), the small Cornell box on the GPU and the
+CPU, the real-time SVGF frame loop, the 64-scene batch and the serving stream.
 
-Other modes: --only NAME runs one secondary workload (frames, c3, batched,
+Other modes: --only NAME runs one secondary workload (frames, c1, c2, batched,
 serving, bvh: the 2 M-triangle SAH build on the host and the GPU, textured: the
-cost of diffuse textures) and prints its JSON; --calibrate-copy runs the 1 GiB copy that scales
-the counters; --traffic-from-pmc turns four rocprofv3 --pmc databases into the
-traffic table.
+cost of diffuse textures) and prints its JSON; --calibrate-copy runs the 1 GiB
+copy that scales the counters; --traffic-from-pmc turns four rocprofv3 --pmc
+databases into the traffic table.
 """
 from __future__ import annotations
 
@@ -93,7 +97,9 @@ def traffic_from_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_byte
     w_scale = copy_bytes / (copy_kib(copy_write_db) * 1024)
     fetch, write = per_kernel(fetch_db), per_kernel(write_db)
     lines = [f"fetch_scale {f_scale:.4f}", f"write_scale {w_scale:.4f}", "",
+             "k_shade is per bounce: its class kernels' bytes summed, divided by the k_extend dispatches", "",
              "stage      dispatches  fetch_B_per_launch  write_B_per_launch  hbm_B_per_launch"]
+    n_bounces = sum(len(fetch[k]) for k in fetch if _stage_of(k) == "k_extend")
     for st in TRAFFIC_STAGES:
         fk = [k for k in fetch if _stage_of(k) == st]
         wk = [k for k in write if _stage_of(k) == st]
@@ -101,9 +107,10 @@ def traffic_from_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_byte
         nw = sum(len(write[k]) for k in wk)
         if not nf or not nw:
             continue
-        fb = sum(sum(fetch[k].values()) for k in fk) * 1024 * f_scale / nf
-        wb = sum(sum(write[k].values()) for k in wk) * 1024 * w_scale / nw
-        lines.append(f"{st:10s} {nf:11d} {fb:19.0f} {wb:19.0f} {fb + wb:17.0f}")
+        per = n_bounces if st == "k_shade" and n_bounces else None
+        fb = sum(sum(fetch[k].values()) for k in fk) * 1024 * f_scale / (per or nf)
+        wb = sum(sum(write[k].values()) for k in wk) * 1024 * w_scale / (per or nw)
+        lines.append(f"{st:10s} {per or nf:11d} {fb:19.0f} {wb:19.0f} {fb + wb:17.0f}")
     lines += ["", "instance-level (scaled bytes per dispatch):"]
     for k in sorted(set(fetch) | set(write)):
         if _stage_of(k) is None:
@@ -214,59 +221,119 @@ def frames_bench(frames: int = 10, width: int = 1280, height: int = 720, env_ali
     }
 
 
-def c3_bench(env_alias: bool = True):
-    """The large-mesh workload: 2 M-triangle fBm terrain + random-walk-parameter
-    SSS sphere under a 2048x1024 HDR sky (alias-table sampled), 1920x1080,
-    32 spp, depth 16, one full render; Mrays/s and the extend kernel's
-    algorithmic HBM rate from counted traversal."""
+def measure_render(scene, steps: int, warmup: int, traffic_file=None, **renderer_kw):
+    """Full renders of a loaded scene at its own spp: `warmup` untimed, then
+    `steps` timed (synchronised on both sides); then one profiled step for the
+    per-stage HIP-event times and one counted step for the algorithmic bytes.
+    Returns throughput, per-stage rooflines and traversal statistics."""
     import cudaraytracer_amd as rt
 
-    tmp = tempfile.mkdtemp(prefix="rtc3_")
-    path = _generate([("terrain_env", {}, tmp, "terrain_c3")])[0]
-    scene = rt.Scene(path)
     info = scene.info()
     spp = info.spp
-    r = rt.Renderer(scene, env_alias=env_alias)
-    r.render(spp)  # warm-up at full size, so every launch in the run has one shape
+    sort = bool(renderer_kw.get("ray_sort", False))
+    r = rt.Renderer(scene, **renderer_kw)
+    for _ in range(warmup):
+        r.reset()
+        r.render(spp)
     r.synchronize()
-    r.reset()
     r.reset_stats()
     t0 = time.perf_counter()
-    r.render(spp)
+    for _ in range(steps):
+        r.reset()
+        r.render(spp)
     r.synchronize()
-    dt = time.perf_counter() - t0
+    elapsed = time.perf_counter() - t0
     st = r.stats()
     rays = st.extension_rays + st.shadow_rays
-    rp = rt.Renderer(scene, profile=True, env_alias=env_alias)
+    img = r.read_accum(rt.RT_ACCUM_MEAN)
+    assert np.isfinite(img).all()
+    rp = rt.Renderer(scene, profile=True, **renderer_kw)
     rp.render(spp)
+    rp.synchronize()
+    rp.reset()
+    rp.reset_stats()
+    prof_steps = max(1, min(3, steps))
+    for _ in range(prof_steps):
+        rp.reset()
+        rp.render(spp)
     sp = rp.stats()
-    rc = rt.Renderer(scene, env_alias=env_alias)
+    rc = rt.Renderer(scene, **renderer_kw)
     rc.render(spp, count_traversal=True)
     sc = rc.stats()
+    traffic = load_traffic(traffic_file)
     ext_bytes = (sc.extension_rays * EXT_RAY_BYTES + sc.extension_node_visits * NODE_BYTES
                  + sc.extension_tri_tests * TRI_BYTES)
+    sorted_rays = (sc.extension_rays - info.width * info.height * spp) if sort else 0
+    ext_bytes += sorted_rays * (PERM_BYTES + RAY_COPY_BYTES)
     sh_bytes = (sc.shadow_rays * SHADOW_RAY_BYTES + sc.shadow_node_visits * NODE_BYTES
                 + sc.shadow_tri_tests * TRI_BYTES)
-    tr = load_traffic(C3_TRAFFIC)
-    ext = _stage_roofline("k_extend", ext_bytes / sp.stage_launches[1], sp.stage_ms[1] / sp.stage_launches[1],
-                          tr.get("k_extend"))
-    sh = _stage_roofline("k_shadow", sh_bytes / sp.stage_launches[3], sp.stage_ms[3] / sp.stage_launches[3],
-                         tr.get("k_shadow"))
+    shade_bytes = sc.extension_rays * SHADE_PATH_BYTES + sc.shadow_rays * SHADE_SHADOW_BYTES + sorted_rays * PERM_BYTES
+    stages = {}
+    bounces = sp.stage_launches[1] / prof_steps  # one extend launch per bounce
+    for name, nbytes, idx in (("k_extend", ext_bytes, 1), ("k_shade", shade_bytes, 2), ("k_shadow", sh_bytes, 3)):
+        # shade: one "launch" = one bounce, every material class together
+        launches = bounces if name == "k_shade" else sp.stage_launches[idx] / prof_steps
+        ms = sp.stage_ms[idx] / prof_steps
+        stages[name] = _stage_roofline(name, nbytes / launches, ms / launches, traffic.get(name))
+        stages[name]["ms_per_step"] = round(ms, 3)
     return {
-        "workload": f"terrain_env {info.num_triangles} tris + SSS sphere + 2048x1024 env "
-                    f"({'alias tables' if env_alias else 'CDF search'}), {info.width}x{info.height}, {spp} spp, "
-                    f"depth {info.max_bounces}",
-        "mrays_s": round(rays / dt / 1e6, 1),
-        "ms": round(dt * 1e3, 2),
-        "rays": int(rays),
+        "mrays_s": round(rays / elapsed / 1e6, 2),
+        "ms_per_step": round(elapsed * 1e3 / steps, 3),
+        "rays_per_step": int(rays // steps),
         "batch_spp": st.batch_spp,
         "nodes_per_ext_ray": round(sc.extension_node_visits / max(1, sc.extension_rays), 2),
         "tris_per_ext_ray": round(sc.extension_tri_tests / max(1, sc.extension_rays), 2),
-        "stage_ms": {n: round(sp.stage_ms[i], 3) for i, n in enumerate(rt.STAGES) if sp.stage_launches[i]},
-        "extend_roofline": {k: ext[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
-        "shadow_roofline": {k: sh[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
-        "bvh_build": bvh_build_bench(path, gpu_scene=scene),
+        "stage_ms_per_step": {n: round(sp.stage_ms[i] / prof_steps, 3) for i, n in enumerate(rt.STAGES)
+                              if sp.stage_launches[i]},
+        "stages": stages,
+        "dominant": max(stages.values(), key=lambda x: x["ms_per_step"]),
     }
+
+
+def c3_scene_path():
+    tmp = tempfile.mkdtemp(prefix="rtc3_")
+    return _generate([("terrain_env", {}, tmp, "terrain_c3")])[0]
+
+
+def c2_bench(steps: int = 5, warmup: int = 1, ray_sort: bool = True):
+    """The spheres workload: Cornell shell + 4 icospheres (20,516 triangles, GGX,
+    IOR-1.5 dielectric, MIS), 512x512, 64 spp, depth 12, the ray sort on."""
+    import cudaraytracer_amd as rt
+    from cudaraytracer_amd import scenes
+
+    path = scenes.write_scene(scenes.cornell_spheres(), tempfile.mkdtemp(prefix="rtc2_"), "cornell_spheres")
+    scene = rt.Scene(path)
+    m = measure_render(scene, steps, warmup, C2_TRAFFIC, ray_sort=ray_sort)
+    m["workload"] = ("cornell_spheres: Cornell shell + 4 icospheres (subdiv 4), 20516 tris, GGX + dielectric + MIS, "
+                     f"512x512, 64 spp, depth 12, RR after depth 3, ray sort {'on' if ray_sort else 'off'}")
+    m["roofline"] = {k: m["dominant"][k] for k in ("kernel", "bound", "achieved", "peak", "unit", "frac", "traffic")}
+    del m["dominant"]
+    return m, path
+
+
+def c1_bench(steps: int = 20, warmup: int = 2, cpu: bool = True):
+    """The small Cornell box (36 triangles, 256x256, 16 spp, depth 8, roulette
+    after depth 3) on the GPU and, in full, on the CPU oracle."""
+    import cudaraytracer_amd as rt
+    from cudaraytracer_amd import scenes
+
+    path = scenes.write_scene(scenes.cornell_small(), tempfile.mkdtemp(prefix="rtc1_"), "cornell_small")
+    scene = rt.Scene(path)
+    m = measure_render(scene, steps, warmup)
+    out = {"workload": "cornell_small: 5 wall quads + short and tall box + light quad, 36 tris, 256x256, 16 spp, "
+                       "depth 8, RR after depth 3",
+           "gpu_mrays_s": m["mrays_s"], "gpu_ms_per_render": m["ms_per_step"], "rays_per_render": m["rays_per_step"]}
+    if cpu:
+        out["cpu"] = cpu_baseline(path, scene.info().spp, full=True)
+        out["gpu_over_cpu"] = round(out["gpu_mrays_s"] / out["cpu"]["value"], 1)
+    return out
+
+
+def c3_extra(path):
+    import cudaraytracer_amd as rt
+
+    scene = rt.Scene(path)
+    return {"bvh_build": bvh_build_bench(path, gpu_scene=scene)}
 
 
 def bvh_build_bench(path, gpu_scene=None):
@@ -464,12 +531,16 @@ def batched_bench(n: int = 64, seed: int = 4, queue_gib: int = 16):
 
 
 def serving_bench(rates=(5.0, 10.0, 20.0, 50.0, 100.0, 200.0), n_jobs: int = 30, frames: int = 10, width: int = 1280,
-                  height: int = 720, seed: int = 5, queue_gib: int = 16):
+                  height: int = 720, seed: int = 5, queue_gib: int = 16, caps=(4, 8, 16), cap_rates=(100.0, 200.0)):
     """Poisson stream of real-time jobs on terrain + SSS + env scenes (2 M,
     1 M and 0.5 M triangles): each job a `frames`-frame orbit at width x height,
     1 spp per frame, SVGF with 5 a-trous passes.  For each mean rate the same
-    arrival sequence is served with continuous batching and with the plain
-    loop; p50/p99 of frame and job latency and jobs/s are reported."""
+    arrival sequence is served with continuous batching, with the plain loop
+    (one job at a time on a fresh single-scene renderer) and with the plain
+    loop on resident renderers (one kept per scene, its history reset per
+    job); at `cap_rates` the batched loop also runs with at most `caps`
+    real-time jobs admitted at once.  p50/p99 of frame and job latency and
+    jobs/s are reported."""
     import cudaraytracer_amd as rt
 
     tmp = tempfile.mkdtemp(prefix="rtserve_")
@@ -492,9 +563,12 @@ def serving_bench(rates=(5.0, 10.0, 20.0, 50.0, 100.0, 200.0), n_jobs: int = 30,
         sample.append(rt.Job(slots[k % len(sc)], spp=1, frames=2, cameras=cams, render={"env_sampling": 1}))
     out["autotune"] = pool.autotune(sample)
     print(f"[bench] serving autotune: {out['autotune']}", file=sys.stderr, flush=True)
+    tuning = pool.tuning()
     warm = rt.Job(slots[2], spp=1, frames=2, cameras=[base_cams[2]] * 2)
     pool.serve([(0.0, warm)], batched=False)
-    for rate in rates:
+    pool.serve([(0.0, warm)], batched="resident")
+
+    def stream(rate):
         rng = np.random.default_rng(seed + int(rate))
         arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
         arrivals -= arrivals[0]
@@ -506,36 +580,89 @@ def serving_bench(rates=(5.0, 10.0, 20.0, 50.0, 100.0, 200.0), n_jobs: int = 30,
             cams = [rt.Camera(tuple(c.target), c.distance, c.theta_deg, c.phi_deg + phi0 + 0.5 * f)
                     for f in range(frames)]
             reqs.append((float(t), rt.Job(slots[s], spp=1, frames=frames, cameras=cams, render={"env_sampling": 1})))
+        return reqs
+
+    rnd = lambda rep: {k: round(v, 3) if isinstance(v, float) else v for k, v in rep.items()}
+    for rate in rates:
+        reqs = stream(rate)
         row = {"rate_jobs_s": rate}
-        for mode in ("batched", "plain"):
-            rep, _, _ = pool.serve(reqs, batched=(mode == "batched"))
-            row[mode] = {k: round(v, 3) if isinstance(v, float) else v for k, v in rep.items()}
+        for mode, arg in (("batched", True), ("plain", False), ("plain_resident", "resident")):
+            rep, _, _ = pool.serve(reqs, batched=arg)
+            row[mode] = rnd(rep)
             print(f"[bench] serving rate {rate}: {mode} {row[mode]}", file=sys.stderr, flush=True)
         row["jobs_per_s_ratio"] = round(row["batched"]["jobs_per_s"] / row["plain"]["jobs_per_s"], 3)
+        row["jobs_per_s_ratio_vs_resident"] = round(row["batched"]["jobs_per_s"] / row["plain_resident"]["jobs_per_s"],
+                                                    3)
         out["rates"].append(row)
+    # Capped admission: the same streams with at most `cap` real-time jobs at once.
+    out["capped"] = []
+    for cap in caps:
+        cp = rt.Pool(queue_budget_bytes=queue_gib << 30, max_realtime_jobs=cap)
+        for s in sc:
+            cp.add_scene(s)
+        cp.set_tuning(tuning.grid_scale, tuning.pass_paths, tuning.tile_width, bool(tuning.ray_sort))
+        cp.serve([(0.0, warm)], batched=True)
+        for rate in cap_rates:
+            rep, _, _ = cp.serve(stream(rate), batched=True)
+            row = {"max_realtime_jobs": cap, "rate_jobs_s": rate, "batched": rnd(rep)}
+            print(f"[bench] serving cap {cap} rate {rate}: {row['batched']}", file=sys.stderr, flush=True)
+            out["capped"].append(row)
+        cp.close()
     return out
 
 
-def cpu_baseline(scene_path: str, spp: int):
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene_path: str, spp: int, full: bool = False, pixel_stride: int = 1, scene=None,
+                 env_alias: bool = False, runs: int = 3):
+    """The CPU oracle (the reference integrator restated in C, OpenMP over
+    pixels, every host core) on the same scene: the whole image (`full`) or
+    every `pixel_stride`-th pixel, `spp` samples each; median of `runs`.  With
+    `scene` (a loaded library scene whose objects import as one mesh each) the
+    oracle takes the imported triangles instead of parsing the OBJ files
+    itself (its Python reader is slow at millions of triangles)."""
     from oracle import rtref
 
     rtref.build()
-    a = rtref.parse_scene(scene_path)
+    geometry = None
+    if scene is not None:
+        pos, nrm, uv, _ = scene.triangles()
+        mesh4, _ = scene.meshes()
+        with open(scene_path, encoding="utf-8-sig") as f:
+            objects = json.load(f)["objects"]
+        geometry = rtref.geometry_from_arrays(objects, pos.reshape(-1, 9), nrm.reshape(-1, 9), uv.reshape(-1, 6),
+                                              mesh4[:, 2], mesh4[:, 3])
+    t_setup = time.perf_counter()
+    a = rtref.parse_scene(scene_path, geometry)
     rs = rtref.RefScene(a)
-    p = rs.params()
-    rs.render(1, p, pixels=np.arange(0, a.width * a.height, 64, dtype=np.int32))  # warm-up
+    setup_s = time.perf_counter() - t_setup
+    p = rs.params(env_alias=env_alias)
+    npx = a.width * a.height
+    pixels = None if full or pixel_stride <= 1 else np.arange(0, npx, pixel_stride, dtype=np.int32)
+    rs.render(1, p, pixels=np.arange(0, npx, 997, dtype=np.int32))  # warm-up
     rates, secs = [], []
-    for _ in range(3):
+    for _ in range(runs):
         t0 = time.perf_counter()
-        _, _, st = rs.render(spp, p)
+        _, _, st = rs.render(spp, p, pixels=pixels)
         dt = time.perf_counter() - t0
         rays = int(st[0] + st[1])
         rates.append(rays / dt / 1e6)
         secs.append(dt)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    what = f"{a.width}x{a.height}" if pixels is None else f"every {pixel_stride}th pixel of {a.width}x{a.height}"
     return {"value": round(float(np.median(rates)), 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"median of 3 runs of {a.width}x{a.height} x {spp} spp of the same scene "
-                      f"({sum(secs):.1f} s total, {rays} rays per run)"}
+            "cpu_model": cpu_model(),
+            "sample": f"median of {runs} runs of {what} x {spp} spp of the same scene, environment sampling "
+                      f"{'alias tables' if env_alias else 'CDF search'} ({sum(secs):.1f} s total, {rays} rays per "
+                      f"run; oracle built -O3 -fopenmp -ffp-contract=off; scene set-up {setup_s:.1f} s untimed)"}
 
 
 def main() -> int:
@@ -543,19 +670,18 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-spp", type=int, default=64, help="samples per pixel for the CPU baseline sample")
+    ap.add_argument("--cpu-pixel-stride", type=int, default=4,
+                    help="the CPU baseline renders every Nth pixel of the headline workload at its full spp")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--ray-sort", type=int, choices=(0, 1), default=1,
-                    help="reorder extension rays by octant and origin cell before extend (C2 renderer)")
-    ap.add_argument("--traffic-file", default=C2_TRAFFIC,
+    ap.add_argument("--traffic-file", default=C3_TRAFFIC,
                     help="HBM bytes per launch measured with rocprofv3 --pmc (table from --traffic-from-pmc)")
-    ap.add_argument("--only", choices=("frames", "c3", "batched", "serving", "bvh", "textured"), default=None,
+    ap.add_argument("--only", choices=("frames", "c1", "c2", "batched", "serving", "bvh", "textured"), default=None,
                     help="run one secondary workload and print its JSON")
     ap.add_argument("--calibrate-copy", action="store_true", help="run the 1 GiB copy that scales the counters")
     ap.add_argument("--traffic-from-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB"),
                     help="print the traffic table from rocprofv3 --pmc databases")
     ap.add_argument("--no-frames", action="store_true", help="skip the real-time SVGF frame-loop measurement")
-    ap.add_argument("--no-c3", action="store_true", help="skip the 2 M-triangle terrain render")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the C1 / C2 / BVH-build sections")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched-vs-plain measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the Poisson serving measurement")
     args = ap.parse_args()
@@ -569,108 +695,80 @@ def main() -> int:
         calibrate_copy()
         return 0
     if args.only:
-        fn = {"frames": frames_bench, "c3": c3_bench, "batched": batched_bench, "serving": serving_bench,
-              "bvh": bvh_only, "textured": textured_bench}[args.only]
+        fn = {"frames": frames_bench, "c1": c1_bench, "c2": lambda: c2_bench()[0], "batched": batched_bench,
+              "serving": serving_bench, "bvh": bvh_only, "textured": textured_bench}[args.only]
         print(json.dumps({args.only: fn()}))
         return 0
 
     import cudaraytracer_amd as rt
-    from cudaraytracer_amd import scenes
 
-    tmp = tempfile.mkdtemp(prefix="rtbench_")
-    spec = scenes.cornell_spheres()
-    path = scenes.write_scene(spec, tmp, "cornell_spheres")
+    path = c3_scene_path()
     scene = rt.Scene(path)
     info = scene.info()
-    spp = info.spp
-    sort = bool(args.ray_sort)
-    r = rt.Renderer(scene, ray_sort=sort)
-    print(f"[bench] scene loaded: {info.num_triangles} tris, {info.width}x{info.height}, {spp} spp, "
-          f"depth {info.max_bounces}, batch {r.stats().batch_spp}", file=sys.stderr, flush=True)
-
-    for _ in range(args.warmup):
-        r.reset()
-        r.render(spp)
-    r.synchronize()
-    r.reset_stats()
-    # Timed region: K full renders, synchronised on both sides.
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r.reset()
-        r.render(spp)
-    r.synchronize()
-    elapsed = time.perf_counter() - t0
-    st = r.stats()
-    rays = st.extension_rays + st.shadow_rays
-    mrays = rays / elapsed / 1e6
-    ms_per_step = elapsed * 1e3 / args.steps
-    img = r.read_accum(rt.RT_ACCUM_MEAN)
-    assert np.isfinite(img).all()
-    print(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, {mrays:.1f} Mrays/s", file=sys.stderr, flush=True)
-
-    # Per-kernel timing with HIP events on the renderer's stream, plus traversal counts.
-    rp = rt.Renderer(scene, profile=True, ray_sort=sort)
-    rp.render(spp)
-    rp.synchronize()
-    rp.reset()
-    rp.reset_stats()
-    prof_steps = max(1, min(3, args.steps))
-    for _ in range(prof_steps):
-        rp.reset()
-        rp.render(spp)
-    sp = rp.stats()
-    rc = rt.Renderer(scene, ray_sort=sort)
-    rc.render(spp, count_traversal=True)
-    sc = rc.stats()
-    traffic = load_traffic(args.traffic_file)
-    ext_bytes = (sc.extension_rays * EXT_RAY_BYTES + sc.extension_node_visits * NODE_BYTES
-                 + sc.extension_tri_tests * TRI_BYTES)
-    sorted_rays = (sc.extension_rays - info.width * info.height * spp) if sort else 0
-    ext_bytes += sorted_rays * (PERM_BYTES + RAY_COPY_BYTES)
-    sh_bytes = (sc.shadow_rays * SHADOW_RAY_BYTES + sc.shadow_node_visits * NODE_BYTES
-                + sc.shadow_tri_tests * TRI_BYTES)
-    shade_bytes = sc.extension_rays * SHADE_PATH_BYTES + sc.shadow_rays * SHADE_SHADOW_BYTES + sorted_rays * PERM_BYTES
-    stages = {}
-    for k, (name, nbytes) in enumerate([("k_extend", ext_bytes), ("k_shade", shade_bytes), ("k_shadow", sh_bytes)]):
-        idx = {"k_extend": 1, "k_shade": 2, "k_shadow": 3}[name]
-        launches = sp.stage_launches[idx] / prof_steps
-        ms = sp.stage_ms[idx] / prof_steps
-        t = traffic.get(name)
-        stages[name] = _stage_roofline(name, nbytes / launches, ms / launches, t)
-        stages[name]["ms_per_step"] = round(ms, 3)
-    sort_idx = rt.STAGES.index("sort")
-    dominant = max(stages.values(), key=lambda s: s["ms_per_step"])
-
+    print(f"[bench] scene loaded: {info.num_triangles} tris, {info.width}x{info.height}, {info.spp} spp, "
+          f"depth {info.max_bounces}", file=sys.stderr, flush=True)
+    m = measure_render(scene, args.steps, args.warmup, args.traffic_file, env_alias=True)
+    print(f"[bench] {args.steps} steps: {m['ms_per_step']:.2f} ms/step, {m['mrays_s']:.1f} Mrays/s",
+          file=sys.stderr, flush=True)
+    dom = m.pop("dominant")
     out = {
         "metric": "Mrays/s & % HBM roofline; served jobs/s + p50/p99 latency vs plain",
-        "value": round(mrays, 2),
+        "value": m["mrays_s"],
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (procedural scene generated in the reference's scene-JSON + OBJ format)",
-        "config": {"workload": "cornell_spheres: Cornell shell + 4 icospheres (subdiv 4), 20516 tris, GGX + "
-                               "dielectric + MIS, 512x512, 64 spp, depth 12, RR after depth 3",
-                   "triangles": info.num_triangles, "width": info.width, "height": info.height, "spp": spp,
-                   "max_depth": info.max_bounces, "batch_spp": st.batch_spp,
-                   "rays_per_step": int(rays // args.steps), "ray_sort": sort},
-        "roofline": {k: dominant[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
-        "roofline_kernel": dominant["kernel"],
-        "stages": stages,
-        "ray_sort_ms_per_step": round(sp.stage_ms[sort_idx] / prof_steps, 3),
+        "config": {"workload": "terrain_env: 2.0M-triangle fBm terrain + subsurface icosphere (5120 tris) under a "
+                               "procedural 2048x1024 FP32 HDR sky (Gaussian sun lobe, peak 1e4) sampled by alias "
+                               "tables, 1920x1080, 32 spp, depth 16",
+                   "triangles": info.num_triangles, "width": info.width, "height": info.height, "spp": info.spp,
+                   "max_depth": info.max_bounces, "batch_spp": m["batch_spp"], "rays_per_step": m["rays_per_step"],
+                   "ray_sort": False},
+        "roofline": {k: dom[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
+        "roofline_kernel": dom["kernel"],
+        "stages": m["stages"],
+        "stage_ms_per_step": m["stage_ms_per_step"],
+        "nodes_per_ext_ray": m["nodes_per_ext_ray"],
+        "tris_per_ext_ray": m["tris_per_ext_ray"],
     }
+    # The shadow fast mode on the same workload: throughput, and whether every pixel of a full
+    # render equals the closest-hit (reference) mode's, bit for bit.
+    ma = measure_render(scene, args.steps, args.warmup, args.traffic_file, env_alias=True, shadow_any_hit=True)
+    ma.pop("dominant")
+    acc = []
+    for any_hit in (False, True):
+        rr = rt.Renderer(scene, env_alias=True, shadow_any_hit=any_hit)
+        rr.render(info.spp)
+        acc.append((rr.read_accum(rt.RT_ACCUM_DIRECT), rr.read_accum(rt.RT_ACCUM_INDIRECT)))
+        del rr
+    same = all(np.array_equal(a.view(np.int32), b.view(np.int32)) for a, b in zip(acc[0], acc[1]))
+    out["shadow_any_hit"] = {"mrays_s": ma["mrays_s"], "ms_per_step": ma["ms_per_step"],
+                             "stage_ms_per_step": ma["stage_ms_per_step"],
+                             "shadow": ma["stages"]["k_shadow"], "image_bit_identical_to_closest_hit": bool(same)}
+    print(f"[bench] shadow any-hit: {ma['mrays_s']} Mrays/s, identical {same}", file=sys.stderr, flush=True)
+    del acc
+    if not args.no_cpu_baseline:
+        print("[bench] CPU baseline (oracle on a pixel sample of the same workload)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(path, info.spp, pixel_stride=args.cpu_pixel_stride, scene=scene,
+                                           env_alias=True)
+        print(f"[bench] cpu_baseline: {out['cpu_baseline']}", file=sys.stderr, flush=True)
+    del scene
+    if not args.no_secondary:
+        print("[bench] C2: spheres 512x512 64 spp", file=sys.stderr, flush=True)
+        out["c2"], c2_path = c2_bench()
+        print("[bench] C1: Cornell box 256x256 16 spp, GPU and CPU", file=sys.stderr, flush=True)
+        out["c1"] = c1_bench(cpu=not args.no_cpu_baseline)
+        print("[bench] BVH build and refit of the terrain", file=sys.stderr, flush=True)
+        out["c3_extra"] = c3_extra(path)
     if not args.no_frames:
         print("[bench] real-time frame loop (SVGF)", file=sys.stderr, flush=True)
         out["frames"] = frames_bench()
-    if not args.no_c3:
-        print("[bench] C3: 2 M-triangle terrain, 1920x1080, 32 spp", file=sys.stderr, flush=True)
-        out["c3"] = c3_bench()
-        print(f"[bench] c3: {out['c3']}", file=sys.stderr, flush=True)
     if not args.no_batched:
         print("[bench] batched: 64 uneven scenes, pool vs one at a time", file=sys.stderr, flush=True)
         out["batched"] = batched_bench()
@@ -678,9 +776,6 @@ def main() -> int:
     if not args.no_serving:
         print("[bench] serving: Poisson stream of real-time jobs", file=sys.stderr, flush=True)
         out["serving"] = serving_bench()
-    if not args.no_cpu_baseline:
-        print("[bench] CPU baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(path, args.cpu_spp)
     print(json.dumps(out))
     return 0
 

@@ -1818,13 +1818,15 @@ Renderer::Renderer(const GpuScene& scene, const HostScene& host, const RenderSet
     I.height = host.camera.height;
     I.wh = I.width * I.height;
     I.cam = host.camera;
-    // Paths per pass: all requested samples of every pixel when they fit the
-    // queue budget.
-    // 16 GiB by default: a 1920x1080 job of up to 36 samples per pixel runs as one pass.
+    // Paths per pass: the scene's samples per pixel (its JSON "spp", at most 64)
+    // of every pixel when they fit the queue budget, so a renderer reserves
+    // queues for what its scene asks for (a 1-spp real-time renderer at 1080p:
+    // 0.5 GB).  16 GiB by default: a 1920x1080 job of up to 36 samples per pixel
+    // runs as one pass.  Renders of more samples run in several passes.
     size_t budget = s.queue_budget_bytes ? s.queue_budget_bytes : ((size_t)16 << 30);
     const size_t per_path = kQueueBytesPerPath + (s.ray_sort ? kSortBytesPerPath : 0);
     int bmax = (int)std::max<size_t>(1, budget / (per_path * (size_t)I.wh));
-    int want = s.batch_spp > 0 ? s.batch_spp : 64;
+    int want = s.batch_spp > 0 ? s.batch_spp : std::min(64, std::max(1, host.spp));
     I.batch = std::max(1, std::min(want, bmax));
     if ((size_t)I.wh * I.batch > (size_t)0x7fffffff) I.batch = std::max(1, (int)(0x7fffffff / I.wh));
     I.eng.reset(new Engine((size_t)I.wh * I.batch, s.profile != 0));

@@ -138,8 +138,10 @@ typedef struct rt_render_params {
     float rr;               /* < 0: scene value (continue probability) */
     int rr_start_depth;     /* < 0: scene value; 0 = roulette from the first bounce (reference) */
     int64_t seed;           /* < 0: scene value */
-    int batch_spp;          /* samples per pass; 0 = automatic */
-    int64_t queue_budget_bytes; /* 0 = 16 GiB (at most 64 samples of every pixel per pass) */
+    int batch_spp;          /* samples per pass; 0 = automatic: the scene's spp (at most 64) when the queue
+                               budget holds them (queues are sized for this at creation) */
+    int64_t queue_budget_bytes; /* 0 = 16 GiB; a renderer holds 228 B per path of a pass (batch_spp x pixels)
+                                   plus 32 B per pixel of accumulators */
     int flush_survivors;    /* 0 = reference: paths alive after max_bounces contribute nothing */
     int profile;            /* 1 = per-stage HIP event timing in rt_stats */
     int env_sampling;       /* 0 = CDF binary search (reference, texture.cu:112-153); 1 = alias tables */
