@@ -40,16 +40,28 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes (per unit): ray record 32 B + hit record 16 B per extension
-# ray; 32 B + 4 B per shadow ray; 64 B per interior node whose child pair is
-# tested; 36 B of vertex data per triangle test.
-EXT_RAY_BYTES, SHADOW_RAY_BYTES, NODE_BYTES, TRI_BYTES = 48, 36, 64, 36
+# Algorithmic bytes: the records each stage must move, by unit.
+#   extend: the ray (origin + direction, 32 B) per extension ray; the hit record
+#     (16 B) written per hit; per miss under an environment light, the path
+#     weight (16 B), the four RGB texels of the bilinear lookup (48 B) and the
+#     read-modify-write of the path's sum (32 B); 64 B per interior node whose
+#     child pair is tested; 36 B of vertex data per triangle test.
+#   shade (per hit): the class-list entry (4 B), the ray and weight (48 B), the hit
+#     record (16 B) and the hit triangle's 128 B attribute line; the next bounce's
+#     ray and weight (48 B) per survivor; the 64 B shadow record per shadow ray;
+#     the read-modify-write of the path's sum (32 B) per light hit; per
+#     next-event estimate that samples the environment, the two alias-table
+#     entries (16 B), the four pdf-table floats (16 B) and the bilinear texels (48 B).
+#   shadow: the 64 B record per shadow ray, the nodes and triangles it tests,
+#     and the read-modify-write of the path's sum (32 B) per visible light point.
+EXT_RAY_BYTES, HIT_BYTES, NODE_BYTES, TRI_BYTES = 32, 16, 64, 36
+MISS_ENV_BYTES = 16 + 48 + 32
+SHADE_HIT_BYTES, ATTR_LINE_BYTES, SURVIVOR_BYTES, SHADOW_REC_BYTES, SUM_RMW_BYTES = 68, 128, 48, 64, 32
+ENV_NEE_BYTES = 16 + 16 + 48
 # With the ray sort on, after the first bounce extend also reads a 4 B permutation
-# entry and writes a 32 B copy of the ray at its sorted position, and shade reads
-# the permutation entry.
-PERM_BYTES, RAY_COPY_BYTES = 4, 32
-# Shade: 84 B read + 68 B written per path-bounce + 44 B per emitted shadow ray.
-SHADE_PATH_BYTES, SHADE_SHADOW_BYTES = 152, 44
+# entry and writes a 32 B copy of the ray at its sorted position (hits only), shade
+# reads the permutation entry (for the weight) and writes a 2 B sort key per survivor.
+PERM_BYTES, RAY_COPY_BYTES, SORT_KEY_BYTES = 4, 32, 2
 
 
 TRAFFIC_STAGES = ("k_extend", "k_shade", "k_shadow")
@@ -156,12 +168,14 @@ def _stage_roofline(name, bytes_per_launch, ms_per_launch, traffic=None):
 
 # SVGF, per pixel, each pixel's data read once: temporal reads this frame's sums
 # (32), last frame's history, moments and length (52) and both G-buffers
-# (2 x 48) and writes history, moments and length (52); variance reads colour
-# (32), moments (16), length (4) and the G-buffer planes it uses (32) and writes
-# 32; one a-trous pass reads colour (32), length (4) and G-buffer (32) and
-# writes 32.  The G-buffer kernel writes 48 B per pixel (its traversal is not
-# counted).
-TEMPORAL_PX_BYTES, VARIANCE_PX_BYTES, ATROUS_PX_BYTES, GBUFFER_PX_BYTES = 232, 116, 100, 48
+# (2 x 48) and writes history, moments and length (52), plus the variance
+# pass's output (32) for a pixel whose history reached 4 frames; the variance
+# pass estimates only the other pixels, each reading its list entry (4), colour
+# (32), moments (16), length (4), filter guide (24) and luminance guide (16) and
+# writing 32; one a-trous pass reads colour (32), length (4) and G-buffer (32)
+# and writes 32.  The G-buffer kernel writes 48 B per pixel (its traversal is
+# not counted).
+TEMPORAL_PX_BYTES, TEMPORAL_COPY_BYTES, VARIANCE_PX_BYTES, ATROUS_PX_BYTES, GBUFFER_PX_BYTES = 232, 32, 128, 100, 48
 
 
 def frames_bench(frames: int = 10, width: int = 1280, height: int = 720, env_alias: bool = True):
@@ -195,17 +209,32 @@ def frames_bench(frames: int = 10, width: int = 1280, height: int = 720, env_ali
     rp.render_frame(1)
     rp.synchronize()
     rp.reset_stats()
+    short = 0  # pixels the variance pass estimated (history < 4 frames), over the frames
     for k in range(1, frames + 1):
         rp.set_camera(rt.Camera(tuple(base.target), base.distance, base.theta_deg, base.phi_deg + 0.5 * k))
         rp.render_frame(1)
+        short += int((rp.read_frame(rt.RT_FRAME_HISTORY_LENGTH) < 4).sum())
     st = rp.stats()
     ms = {name: st.stage_ms[i] / frames for i, name in enumerate(rt.STAGES)}
+    # Held camera: after 4 frames every history is long unless the view is disoccluded, and
+    # the variance pass visits only the tiles that still hold a short history.
+    for _ in range(4):
+        rp.render_frame(1)
+    rp.synchronize()
+    rp.reset_stats()
+    held = 5
+    held_short = 0
+    for _ in range(held):
+        rp.render_frame(1)
+        held_short += int((rp.read_frame(rt.RT_FRAME_HISTORY_LENGTH) < 4).sum())
+    sh = rp.stats()
     px = width * height
+    short_pf = short / frames
     atrous_launch_ms = st.stage_ms[8] / frames / 5
     ach = ATROUS_PX_BYTES * px / (atrous_launch_ms * 1e-3) / 1e9
     per_stage_gbs = {
-        "temporal": TEMPORAL_PX_BYTES * px / (ms["temporal"] * 1e-3) / 1e9,
-        "variance": VARIANCE_PX_BYTES * px / (ms["variance"] * 1e-3) / 1e9,
+        "temporal": (TEMPORAL_PX_BYTES * px + TEMPORAL_COPY_BYTES * (px - short_pf)) / (ms["temporal"] * 1e-3) / 1e9,
+        "variance": VARIANCE_PX_BYTES * short_pf / (ms["variance"] * 1e-3) / 1e9,
         "atrous": ach,
     }
     return {
@@ -216,9 +245,36 @@ def frames_bench(frames: int = 10, width: int = 1280, height: int = 720, env_ali
         "frame_ms_max": round(float(np.max(lat)), 3),
         "stage_ms_per_frame": {k: round(v, 4) for k, v in ms.items()},
         "svgf_gbs": {k: round(v, 1) for k, v in per_stage_gbs.items()},
+        "variance_pixels_per_frame": round(short_pf),
+        "held_camera": {"variance_ms_per_frame": round(sh.stage_ms[7] / held, 4),
+                        "temporal_ms_per_frame": round(sh.stage_ms[6] / held, 4),
+                        "variance_pixels_per_frame": round(held_short / held)},
         "atrous_roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": round(ach / HBM_PEAK_GBS, 4)},
     }
+
+
+def stage_bytes(sc, scene, primary: int, sort: bool) -> dict:
+    """Algorithmic bytes of one counted render per stage, from its path counts
+    (rt_stats_out of a count_traversal render)."""
+    info = scene.info()
+    env = scene.envmap().shape[0] > 0
+    E, M = sc.extension_rays, sc.miss_paths
+    hits = E - M
+    cls = sc.as_dict()["class_paths"]
+    nee_paths = cls["diffuse"] + cls["ggx"] + cls["subsurface"]
+    env_nee = nee_paths / max(1, info.num_lights) if env else 0.0
+    survivors = E - primary
+    sorted_hits = (survivors * hits / max(1, E)) if sort else 0.0
+    ext = (E * EXT_RAY_BYTES + hits * HIT_BYTES + (M * MISS_ENV_BYTES if env else 0)
+           + sc.extension_node_visits * NODE_BYTES + sc.extension_tri_tests * TRI_BYTES)
+    ext += (survivors * PERM_BYTES + sorted_hits * RAY_COPY_BYTES) if sort else 0
+    shade = (hits * (SHADE_HIT_BYTES + ATTR_LINE_BYTES) + survivors * SURVIVOR_BYTES
+             + sc.shadow_rays * SHADOW_REC_BYTES + cls["light"] * SUM_RMW_BYTES + env_nee * ENV_NEE_BYTES)
+    shade += (sorted_hits * PERM_BYTES + survivors * SORT_KEY_BYTES) if sort else 0
+    shadow = (sc.shadow_rays * SHADOW_REC_BYTES + sc.shadow_node_visits * NODE_BYTES
+              + sc.shadow_tri_tests * TRI_BYTES + sc.shadow_lit * SUM_RMW_BYTES)
+    return {"k_extend": ext, "k_shade": shade, "k_shadow": shadow}
 
 
 def measure_render(scene, steps: int, warmup: int, traffic_file=None, **renderer_kw):
@@ -261,13 +317,8 @@ def measure_render(scene, steps: int, warmup: int, traffic_file=None, **renderer
     rc.render(spp, count_traversal=True)
     sc = rc.stats()
     traffic = load_traffic(traffic_file)
-    ext_bytes = (sc.extension_rays * EXT_RAY_BYTES + sc.extension_node_visits * NODE_BYTES
-                 + sc.extension_tri_tests * TRI_BYTES)
-    sorted_rays = (sc.extension_rays - info.width * info.height * spp) if sort else 0
-    ext_bytes += sorted_rays * (PERM_BYTES + RAY_COPY_BYTES)
-    sh_bytes = (sc.shadow_rays * SHADOW_RAY_BYTES + sc.shadow_node_visits * NODE_BYTES
-                + sc.shadow_tri_tests * TRI_BYTES)
-    shade_bytes = sc.extension_rays * SHADE_PATH_BYTES + sc.shadow_rays * SHADE_SHADOW_BYTES + sorted_rays * PERM_BYTES
+    model = stage_bytes(sc, scene, info.width * info.height * spp, sort)
+    ext_bytes, shade_bytes, sh_bytes = model["k_extend"], model["k_shade"], model["k_shadow"]
     stages = {}
     bounces = sp.stage_launches[1] / prof_steps  # one extend launch per bounce
     for name, nbytes, idx in (("k_extend", ext_bytes, 1), ("k_shade", shade_bytes, 2), ("k_shadow", sh_bytes, 3)):

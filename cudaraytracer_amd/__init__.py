@@ -41,6 +41,9 @@ RT_IMAGE_PNG_DENOISED = 1
 RT_IMAGE_PFM_MEAN = 2
 RT_IMAGE_PFM_DENOISED = 3
 STAGES = ("raygen", "extend", "shade", "shadow", "resolve", "gbuffer", "temporal", "variance", "atrous", "sort")
+# Shading classes (rt_stats_out.class_paths): one shade kernel each; "light" is a
+# path that hit an emitter and ends there (misses end in extend and are in no class).
+SHADE_CLASSES = ("light", "diffuse", "ggx", "dielectric", "subsurface")
 
 # Frame buffers (rt_read_frame): id -> (numpy dtype, channels).
 RT_FRAME_DENOISED = 0
@@ -118,11 +121,15 @@ class Stats(ctypes.Structure):
         ("probe_rays", ctypes.c_uint64), ("stage_ms", ctypes.c_double * len(STAGES)),
         ("stage_launches", ctypes.c_int * len(STAGES)),
         ("spp_done", ctypes.c_int), ("batch_spp", ctypes.c_int),
+        ("miss_paths", ctypes.c_uint64), ("shadow_lit", ctypes.c_uint64),
+        ("class_paths", ctypes.c_uint64 * len(SHADE_CLASSES)),
     ]
 
     def as_dict(self) -> dict:
-        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("stage_ms", "stage_launches")}
+        d = {f: getattr(self, f) for f, _ in self._fields_
+             if f not in ("stage_ms", "stage_launches", "class_paths")}
         d["stage_ms"] = {s: self.stage_ms[i] for i, s in enumerate(STAGES)}
+        d["class_paths"] = {c: self.class_paths[i] for i, c in enumerate(SHADE_CLASSES)}
         d["stage_launches"] = {s: self.stage_launches[i] for i, s in enumerate(STAGES)}
         return d
 

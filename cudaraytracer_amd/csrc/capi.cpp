@@ -16,6 +16,7 @@
 #include "scene.h"
 
 static_assert(RT_NUM_STAGES == rt::kNumStages, "stage ids of rt.h and the engine differ");
+static_assert(RT_NUM_CLASSES == rt::kNumClasses, "shading classes of rt.h and the engine differ");
 static_assert(RT_NOALLOC_MAX_RAYS == rt::kNoAllocMaxRays, "rt_trace_rays_noalloc limits of rt.h and the engine differ");
 
 struct rt_scene {
@@ -441,6 +442,9 @@ int rt_stats(rt_renderer* r, rt_stats_out* o) {
         o->shadow_node_visits = s.shadow_nodes;
         o->shadow_tri_tests = s.shadow_tris;
         o->probe_rays = s.probe_rays;
+        o->miss_paths = s.miss_paths;
+        o->shadow_lit = s.shadow_lit;
+        for (int k = 0; k < RT_NUM_CLASSES; ++k) o->class_paths[k] = s.class_paths[k];
         for (int k = 0; k < RT_NUM_STAGES; ++k) {
             o->stage_ms[k] = s.stage_ms[k];
             o->stage_launches[k] = s.stage_launches[k];
@@ -624,6 +628,9 @@ int rt_pool_stats(rt_pool* p, rt_stats_out* o) {
         o->shadow_node_visits = s.shadow_nodes;
         o->shadow_tri_tests = s.shadow_tris;
         o->probe_rays = s.probe_rays;
+        o->miss_paths = s.miss_paths;
+        o->shadow_lit = s.shadow_lit;
+        for (int k = 0; k < RT_NUM_CLASSES; ++k) o->class_paths[k] = s.class_paths[k];
         for (int k = 0; k < RT_NUM_STAGES; ++k) {
             o->stage_ms[k] = s.stage_ms[k];
             o->stage_launches[k] = s.stage_launches[k];

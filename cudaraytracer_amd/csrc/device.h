@@ -66,7 +66,7 @@ constexpr int kAttrStride = 8;
 // Shading classes: extend bins every path by the class of what it hit, and
 // one specialised shade kernel per class runs without cross-material divergence.
 enum ShadeClass : int {
-    kClsTerminal = 0,   // miss (environment) or light hit: the path ends
+    kClsTerminal = 0,   // light hit: the path ends (a miss ends in the extend stage)
     kClsDiffuse = 1,
     kClsPbr = 2,
     kClsDielectric = 3,
@@ -149,7 +149,10 @@ enum StatSlot : int {
     kStatShadowNodes = 4,
     kStatShadowTris = 5,
     kStatProbeRays = 6,
-    kNumStats = 8,
+    kStatMisses = 7,       // extension rays that left the scene (counting renders only)
+    kStatShadowLit = 8,    // shadow rays that found their light point visible (counting renders only)
+    kStatClassPaths = 9,   // + class: extension rays binned to each shading class (counting renders only)
+    kNumStats = kStatClassPaths + kNumClasses,
 };
 
 }  // namespace rt

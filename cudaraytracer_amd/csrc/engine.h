@@ -57,7 +57,8 @@ constexpr size_t kSortBytesPerPath = 38;
 // histories (2 x float4), moments (2 x float2) and lengths (int) per frame
 // parity, the variance output, the a-trous ping-pong pair, the denoised colour,
 // the filter guide (decoded normal + depth, key + history length) and the
-// luminance guide of the variance pass's input (float4).
+// luminance guide of the variance pass's input (float4).  (The variance
+// pass's tile list adds under 1/16 B per pixel plus a row of ints.)
 constexpr size_t kFrameStateBytesPerPixel = 2 * 48 + 2 * 32 + 2 * 16 + 2 * 4 + 32 + 2 * 32 + 16 + 24 + 16;
 
 // Per-stage GPU time from HIP events recorded on the work's own stream.
@@ -188,6 +189,8 @@ private:
     DevBuf v_d_, v_i_, f_d_[2], f_i_[2], denoised_;
     DevBuf guide_nd_, guide_kl_;  // this frame's filter guide
     DevBuf lum_;                  // luminance guide of the variance pass's input
+    DevBuf short_tiles_, short_n_;  // tiles holding a short history, and a pair of counts
+    int short_sel_ = 0;             // which count of the pair the next temporal pass fills
     CameraRec last_cam_;
     int fcur_ = 0, flast_ = -1, frames_ = 0, out_sel_ = -1;
     int tile_shift_ = 4;

@@ -171,16 +171,20 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
     const bool first = (ps.flags & kFlagFirst) != 0;
     const bool specular = (ps.flags & kFlagSpecular) != 0;
 
-    if (CLS == kClsTerminal && prim < 0) {
-        // Miss: environment radiance without MIS, then the path ends
-        // (reference: Src/Core/renderer.cuh:142-159).
-        if (sc.env_light >= 0) {
-            vec3 bg = env_emission(sc, ps.d);
-            out.emit = cmul(ps.w, bg);
-            out.has_emit = true;
-        }
-        return out;
-    }
+    // Misses never get here: the extend stage ends them with the environment's
+    // radiance (reference: Src/Core/renderer.cuh:142-159).
+
+    // The light sample of next-event estimation takes the first random numbers
+    // of the bounce and does not depend on the surface, so it is drawn before
+    // the surface is fetched and its table reads overlap that fetch (the
+    // random-number sequence is the reference's either way).
+    Rng rng;
+    rng.init(fp.seed, pixel, sample, (uint32_t)fp.bounce, kStreamPath);
+    int lid = 0;
+    float p_sel = 0.f;
+    vec3 lp = v3(0.f, 0.f, 0.f), ln = v3(0.f, 0.f, 0.f);
+    bool is_env = false, have_light = false;
+    if (CLS == kClsDiffuse || CLS == kClsPbr) have_light = sample_light(sc, rng, lid, p_sel, lp, ln, is_env);
 
     SurfaceHit sh = surface_at(sc.tri_attr, prim, hit.y, hit.z);
     DevMaterial mat = load_material(sc.materials, sh.material);
@@ -212,8 +216,6 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
         return out;
     }
 
-    Rng rng;
-    rng.init(fp.seed, pixel, sample, (uint32_t)fp.bounce, kStreamPath);
     vec3 geom_n = normalize(sh.ng);
     vec3 hit_n = (double)norm2(sh.ns) > kEpsD ? normalize(sh.ns) : geom_n;
     if (dot(hit_n, geom_n) < 0.f) hit_n = -hit_n;
@@ -240,12 +242,8 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
     }
 
     if (CLS == kClsDiffuse || CLS == kClsPbr) {
-        // Next-event estimation (renderer.cuh:242-273).
-        int lid;
-        float p_sel;
-        vec3 lp, ln;
-        bool is_env;
-        if (sample_light(sc, rng, lid, p_sel, lp, ln, is_env)) {
+        // Next-event estimation (renderer.cuh:242-273), with the light sample drawn above.
+        if (have_light) {
             vec3 dir_wi = normalize(lp - hit_p);
             float pdf_light = light_pdf_sa(sc, lid, hit_p, lp, ln) * p_sel;
             float cos_t = dot(hit_n, dir_wi);

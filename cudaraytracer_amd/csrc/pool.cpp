@@ -468,6 +468,9 @@ void Pool::stats(RenderStats& out) {
     out.shadow_nodes = s[kStatShadowNodes];
     out.shadow_tris = s[kStatShadowTris];
     out.probe_rays = s[kStatProbeRays];
+    out.miss_paths = s[kStatMisses];
+    out.shadow_lit = s[kStatShadowLit];
+    for (int k = 0; k < kNumClasses; ++k) out.class_paths[k] = s[kStatClassPaths + k];
     for (int k = 0; k < kNumStages; ++k) {
         out.stage_ms[k] = I.eng->prof.ms[k];
         out.stage_launches[k] = I.eng->prof.launches[k];

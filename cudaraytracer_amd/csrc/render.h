@@ -64,6 +64,8 @@ struct RenderStats {
     double stage_ms[kNumStages];
     int stage_launches[kNumStages];
     int spp_done, batch_spp;
+    uint64_t miss_paths, shadow_lit;  // counting renders only
+    uint64_t class_paths[kNumClasses]; // per shading class, counting renders only
 };
 
 struct EnvTables {

@@ -294,6 +294,10 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const uint16_t* __restr
 // ------------------------------------------------------------------ extend
 // Closest hit for every path of the current queue, then the path's queue
 // index is appended to its slice's list for the shading class of what it hit.
+// A miss ends its path here instead: the environment's radiance times the
+// path weight (no MIS, reference: Src/Core/renderer.cuh:142-159) is added to
+// the path's sum at once, so the shade stage never sees misses, and their
+// texel reads overlap the traversal of the other waves.
 // The traversal stacks live in dynamic LDS: one column of `max stack depth`
 // entries per lane, sized per pass to the deepest tree among its jobs, so a
 // shallow tree leaves room for more resident waves.
@@ -301,8 +305,9 @@ extern __shared__ uint32_t stk_dyn[];
 
 template <bool COUNT, bool POOLED>
 __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jobs, const float4* __restrict__ qo,
-                                                    const float4* __restrict__ qd, float4* __restrict__ hits,
-                                                    int* __restrict__ lists, int cap,
+                                                    const float4* __restrict__ qd, const float4* __restrict__ qw,
+                                                    float4* __restrict__ hits, int* __restrict__ lists, int cap,
+                                                    float4* slot_d, float4* slot_i,
                                                     const PassHeader* __restrict__ ph, int* counters, int qslot,
                                                     unsigned long long* stats, const int* __restrict__ perm,
                                                     float4* __restrict__ sorted_o, float4* __restrict__ sorted_d) {
@@ -328,18 +333,32 @@ __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jo
         int i = p;
         TravCount c{0, 0};
         int cls = -1;
+        bool miss = false;
         if (j < cur.count) {
             if (perm) i = cur.slice * region + perm[p];
             float4 o = qo[i], d = qd[i];
             const DevScene& sc = pass_job<POOLED>(jobs, (int)((uint32_t)f_as_i(d.w) >> kJobShift)).sc;
             RayInv r = make_ray(v3(o.x, o.y, o.z), v3(d.x, d.y, d.z));
             Hit h = trace_closest_wave<COUNT>(sc, r, 0.f, 1e8f, col, kBlock, c);
-            hits[p] = make_float4(h.t, h.b1, h.b2, i_as_f(h.prim));
-            if (perm) {
-                sorted_o[p] = o;
-                sorted_d[p] = d;
+            miss = h.prim < 0;
+            if (!miss) {
+                hits[p] = make_float4(h.t, h.b1, h.b2, i_as_f(h.prim));
+                if (perm) {
+                    sorted_o[p] = o;
+                    sorted_d[p] = d;
+                }
+                cls = shade_class_of(sc.tri_attr, h.prim);
+            } else if (sc.env_light >= 0) {
+                const float4 w = qw[i];
+                const vec3 e = cmul(v3(w.x, w.y, w.z), env_emission(sc, v3(d.x, d.y, d.z)));
+                float4* sum = (f_as_i(d.w) & kFlagFirst) ? slot_d : slot_i;
+                const int pid = f_as_i(w.w);
+                float4 a = sum[pid];
+                a.x += e.x;
+                a.y += e.y;
+                a.z += e.z;
+                sum[pid] = a;
             }
-            cls = shade_class_of(sc.tri_attr, h.prim);
         }
 #pragma unroll
         for (int k = 0; k < kNumClasses; ++k) {
@@ -350,8 +369,26 @@ __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jo
         if (COUNT) {
             wave_add_u64(&stats[kStatExtNodes], c.nodes);
             wave_add_u64(&stats[kStatExtTris], c.tris);
+            const uint64_t mm = __ballot(miss);
+            if ((threadIdx.x & (kWave - 1)) == 0 && mm) atomicAdd(&stats[kStatMisses], (unsigned long long)__popcll(mm));
+#pragma unroll
+            for (int k = 0; k < kNumClasses; ++k) {
+                const uint64_t m = __ballot(cls == k);
+                if ((threadIdx.x & (kWave - 1)) == 0 && m)
+                    atomicAdd(&stats[kStatClassPaths + k], (unsigned long long)__popcll(m));
+            }
         }
     }
+}
+
+// Shadow records are stored in blocks of 64: field f (0..3) of record r sits
+// at shadow[(r / 64) * 256 + f * 64 + r % 64], so a wave's 64 consecutive
+// records write and read each field as one contiguous 1 KiB run (two runs when
+// they straddle a block) instead of 16 bytes at a 64-byte stride.  The buffer
+// holds the engine's capacity rounded up to whole blocks.
+template <typename T>
+__device__ __forceinline__ T* shadow_field(T* base, size_t r, int f) {
+    return base + ((r >> 6) << 8) + ((size_t)f << 6) + (r & 63);
 }
 
 // ------------------------------------------------------------------ shade
@@ -373,7 +410,6 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
                                                    const float4* __restrict__ sorted_d) {
     const int region = ph->region;
     uint32_t* stk = stk_dyn;  // sized only for the subsurface class (its probe rays)
-    if (blockIdx.x == 0 && threadIdx.x < kSlices) *ctr_at(counters, kFetchExtend, threadIdx.x) = 0;
     uint32_t* col = stk + (CLS == kClsSubsurface ? threadIdx.x : 0);
     SliceCursor cur;
     cur.init(counters, kCntClass0 + CLS, kFetchClass0 + CLS);
@@ -451,11 +487,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
             if (so.shadow) {
                 // tag = job << 1 | (sum: 0 direct, 1 indirect)
                 int tag = (int)((jobbits >> kJobShift) << 1) | (so.add_direct ? 0 : 1);
-                float4* r = shadow + 4 * ((size_t)cur.slice * region + spos);
-                r[0] = make_float4(so.rec.lp.x, so.rec.lp.y, so.rec.lp.z, so.rec.sdist);
-                r[1] = make_float4(so.rec.dir.x, so.rec.dir.y, so.rec.dir.z, i_as_f(pid));
-                r[2] = make_float4(so.rec.hp.x, so.rec.hp.y, so.rec.hp.z, i_as_f(tag));
-                r[3] = make_float4(so.rec.contrib.x, so.rec.contrib.y, so.rec.contrib.z, 0.f);
+                const size_t r = (size_t)cur.slice * region + spos;
+                *shadow_field(shadow, r, 0) = make_float4(so.rec.lp.x, so.rec.lp.y, so.rec.lp.z, so.rec.sdist);
+                *shadow_field(shadow, r, 1) = make_float4(so.rec.dir.x, so.rec.dir.y, so.rec.dir.z, i_as_f(pid));
+                *shadow_field(shadow, r, 2) = make_float4(so.rec.hp.x, so.rec.hp.y, so.rec.hp.z, i_as_f(tag));
+                *shadow_field(shadow, r, 3) =
+                    make_float4(so.rec.contrib.x, so.rec.contrib.y, so.rec.contrib.z, 0.f);
             }
         }
     }
@@ -478,7 +515,9 @@ __global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jo
         int sl = threadIdx.x;
         int n = *ctr_at(counters, kCntShadow, sl);
         if (n > 0) atomicAdd(&stats[kStatShadowRays], (unsigned long long)n);
-        // The class lists of this bounce are consumed: reset them for the next extend.
+        // The queue and the class lists of this bounce are consumed: reset their
+        // fetch counters and counts for the next extend.
+        *ctr_at(counters, kFetchExtend, sl) = 0;
         for (int c = 0; c < kNumClasses; ++c) {
             *ctr_at(counters, kCntClass0 + c, sl) = 0;
             *ctr_at(counters, kFetchClass0 + c, sl) = 0;
@@ -491,9 +530,11 @@ __global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jo
     while (cur.next(base)) {
         int j = base + (int)(threadIdx.x & (kWave - 1));
         TravCount c{0, 0};
+        bool lit = false;
         if (j < cur.count) {
-            const float4* r = shadow + 4 * ((size_t)cur.slice * region + j);
-            float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+            const size_t r = (size_t)cur.slice * region + j;
+            const float4 r0 = *shadow_field(shadow, r, 0), r1 = *shadow_field(shadow, r, 1);
+            const float4 r2 = *shadow_field(shadow, r, 2), r3 = *shadow_field(shadow, r, 3);
             const int tag = f_as_i(r2.w);
             const DevScene& sc = pass_job<POOLED>(jobs, tag >> 1).sc;
             ShadowRec rec;
@@ -517,6 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jo
             vec3 tp = v3(0.f, 0.f, 0.f);
             if (hit) tp = position_at(sc.tri_attr, h.prim, h.b1, h.b2);
             if (h.prim != kEarlyExit && shadow_visible(rec, hit, tp)) {
+                lit = true;
                 int pid = f_as_i(r1.w);
                 float4* s = (tag & 1) == 0 ? slot_d : slot_i;
                 float4 a = s[pid];
@@ -529,6 +571,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jo
         if (COUNT) {
             wave_add_u64(&stats[kStatShadowNodes], c.nodes);
             wave_add_u64(&stats[kStatShadowTris], c.tris);
+            const uint64_t m = __ballot(lit);
+            if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(&stats[kStatShadowLit], (unsigned long long)__popcll(m));
         }
     }
 }
@@ -643,17 +687,41 @@ __global__ __launch_bounds__(kBlock) void k_gbuffer(DevScene sc, GBufferParams g
     gbuffer_pixel(sc, g, i, j, stk_dyn + threadIdx.x, kBlock, out);
 }
 
+// The temporal pass also lists the tiles that hold a pixel whose history is
+// shorter than kVarianceHistory, for the variance pass to visit; the pixels
+// with a longer history get the variance pass's output (a copy) from the
+// temporal pass itself.  A listed tile is estimated as a whole workgroup, as
+// an unlisted pass would: the 7x7 gather is latency-bound, and a list of
+// single pixels (measured) leaves too few waves in flight and loses the
+// neighbourhood the lanes of a tile share.  One atomic per workgroup; every
+// lane reaches the barrier, also those outside the image.
 __global__ __launch_bounds__(kBlock) void k_temporal(int w, int h, int ts, int first, SvgfParams p, TemporalIO io,
-                                                      GBufferView cur, GBufferView prev) {
+                                                      GBufferView cur, GBufferView prev, int* __restrict__ tiles_out,
+                                                      int* n_tiles) {
     int i, j;
-    if (!tile_pixel(w, h, ts, i, j)) return;
-    temporal_pixel(i, j, w, h, first != 0, p, io, cur, prev);
+    const bool in = tile_pixel(w, h, ts, i, j);
+    int hl = kVarianceHistory;
+    if (in) hl = temporal_pixel(i, j, w, h, first != 0, p, io, cur, prev);
+    if (__syncthreads_or(in && hl < kVarianceHistory) && threadIdx.x == 0) tiles_out[atomicAdd(n_tiles, 1)] = blockIdx.x;
 }
 
-__global__ __launch_bounds__(kBlock) void k_variance(int w, int h, int ts, SvgfParams p, FilterIO io, GBufferView g) {
-    int i, j;
-    if (!tile_pixel(w, h, ts, i, j)) return;
-    variance_pixel(i, j, w, h, p, io, g);
+// Grid-stride over the temporal pass's tile list, one tile per workgroup
+// iteration; a lane estimates its pixel when that pixel's history is short.
+// The frame after this one lists into the other count of the pair, which is
+// cleared here (the two passes run in stream order, so nothing else touches
+// it meanwhile).
+__global__ __launch_bounds__(kBlock) void k_variance(int w, int h, int ts, SvgfParams p, FilterIO io, GBufferView g,
+                                                      const int* __restrict__ tiles, const int* n_tiles,
+                                                      int* n_tiles_next) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *n_tiles_next = 0;
+    const int n = *n_tiles;
+    const int tw = 1 << ts, th = kBlock >> ts, tiles_x = (w + tw - 1) >> ts;
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        const int t = tiles[k];
+        const int i = (t % tiles_x) * tw + ((int)threadIdx.x & (tw - 1));
+        const int j = (t / tiles_x) * th + ((int)threadIdx.x >> ts);
+        if (i < w && j < h && io.len[(size_t)j * w + i] < kVarianceHistory) variance_estimate(i, j, w, h, p, io, g);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_atrous(int w, int h, int ts, int step, SvgfParams p, FilterIO io,
@@ -889,7 +957,7 @@ struct GpuScene::Impl {
     int num_interior = 0;     // packed nodes of the tree (0: the root is a leaf)
     DevScene dev{};
     int max_stack = 1;
-    bool has_class[kNumClasses] = {true, false, false, false, false};
+    bool has_class[kNumClasses] = {false, false, false, false, false};
     int device = 0;
 };
 
@@ -1297,7 +1365,7 @@ Engine::Engine(size_t capacity_paths, bool profile) {
     }
     hits_.alloc(cap_ * 16);
     lists_.alloc(cap_ * 4 * kNumClasses);
-    shadow_.alloc(cap_ * 64);
+    shadow_.alloc(((cap_ + 63) & ~(size_t)63) * 64);
     slot_d_.alloc(cap_ * 16);
     slot_i_.alloc(cap_ * 16);
     counters_.alloc(kCounterInts * sizeof(int));
@@ -1395,7 +1463,7 @@ void Engine::run(const std::vector<PassItem>& items, bool count) {
     JobDev* jobs = reinterpret_cast<JobDev*>(host + kPassHeaderBytes);
     size_t paths = 0, pixels = 0;
     int max_bounces = 0, stack = 1;
-    bool classes[kNumClasses] = {true, false, false, false, false};
+    bool classes[kNumClasses] = {false, false, false, false, false};
     for (int k = 0; k < n; ++k) {
         const PassItem& it = items[k];
         if (it.spp < 1) throw std::invalid_argument("pass item without samples");
@@ -1415,7 +1483,7 @@ void Engine::run(const std::vector<PassItem>& items, bool count) {
         pixels += (size_t)it.fp.wh;
         max_bounces = std::max(max_bounces, it.fp.max_bounces);
         stack = std::max(stack, it.scene->max_stack());
-        for (int c = 1; c < kNumClasses; ++c) classes[c] = classes[c] || it.scene->has_class(c);
+        for (int c = 0; c < kNumClasses; ++c) classes[c] = classes[c] || it.scene->has_class(c);
     }
     if (paths > cap_) throw std::invalid_argument("pass exceeds the engine's path capacity");
     if (pixels > (size_t)0x7fffffff) throw std::invalid_argument("pass has too many pixels");
@@ -1508,12 +1576,14 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
         b = prof.begin(stream_);
         if (count)
             k_extend<true, POOLED><<<gext, kBlock, lds, stream_>>>(
-                jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), hits_.as<float4>(), lists_.as<int>(), cap, hdr,
-                ctr, q, st, perm, perm ? sorted_o_.as<float4>() : nullptr, perm ? sorted_d_.as<float4>() : nullptr);
+                jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), qw_[q].as<float4>(), hits_.as<float4>(),
+                lists_.as<int>(), cap, slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, q, st, perm,
+                perm ? sorted_o_.as<float4>() : nullptr, perm ? sorted_d_.as<float4>() : nullptr);
         else
             k_extend<false, POOLED><<<gext, kBlock, lds, stream_>>>(
-                jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), hits_.as<float4>(), lists_.as<int>(), cap, hdr,
-                ctr, q, st, perm, perm ? sorted_o_.as<float4>() : nullptr, perm ? sorted_d_.as<float4>() : nullptr);
+                jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), qw_[q].as<float4>(), hits_.as<float4>(),
+                lists_.as<int>(), cap, slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, q, st, perm,
+                perm ? sorted_o_.as<float4>() : nullptr, perm ? sorted_d_.as<float4>() : nullptr);
         HIP_OK(hipGetLastError());
         prof.end(kStageExtend, b, stream_);
         b = prof.begin(stream_);
@@ -1528,7 +1598,7 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
                 perm ? sorted_d_.as<float4>() : nullptr);
             HIP_OK(hipGetLastError());
         };
-        shade(std::integral_constant<int, kClsTerminal>(), gshade[kClsTerminal]);
+        if (classes[kClsTerminal]) shade(std::integral_constant<int, kClsTerminal>(), gshade[kClsTerminal]);
         if (classes[kClsDiffuse]) shade(std::integral_constant<int, kClsDiffuse>(), gshade[kClsDiffuse]);
         if (classes[kClsPbr]) shade(std::integral_constant<int, kClsPbr>(), gshade[kClsPbr]);
         if (classes[kClsDielectric]) shade(std::integral_constant<int, kClsDielectric>(), gshade[kClsDielectric]);
@@ -1610,6 +1680,15 @@ FrameState::FrameState(int width, int height) : w_(width), h_(height), wh_(width
     guide_nd_.alloc(wh * 16);
     guide_kl_.alloc(wh * 8);
     lum_.alloc(wh * 16);
+    // One entry per tile, for the tile shape (set_tile_shift: 3..7) with the most tiles.
+    size_t max_tiles = 0;
+    for (int ts = 3; ts <= 7; ++ts) {
+        const size_t tw = (size_t)1 << ts, th = (size_t)(kBlock >> ts);
+        max_tiles = std::max(max_tiles, ((size_t)w_ + tw - 1) / tw * (((size_t)h_ + th - 1) / th));
+    }
+    short_tiles_.alloc(std::max<size_t>(max_tiles, 1) * sizeof(int));
+    short_n_.alloc(2 * sizeof(int));
+    HIP_OK(hipMemset(short_n_.p, 0, 2 * sizeof(int)));
     HIP_OK(hipEventCreateWithFlags(&ev_order_, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&ev_gbuf_, hipEventDisableTiming));
 }
@@ -1705,6 +1784,8 @@ void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const fl
     tio.inv_spp = 1.f / (float)spp;
     tio.cur_d = h_d_[c].as<float4>();
     tio.cur_i = h_i_[c].as<float4>();
+    tio.var_d = v_d_.as<float4>();
+    tio.var_i = v_i_.as<float4>();
     tio.hist_d = h_d_[pv].as<float4>();
     tio.hist_i = h_i_[pv].as<float4>();
     tio.mom_prev_d = m_d_[pv].as<float2>();
@@ -1716,7 +1797,10 @@ void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const fl
     tio.guide_nd = guide_nd_.as<float4>();
     tio.guide_kl = guide_kl_.as<int2>();
     tio.lum = lum_.as<float4>();
-    k_temporal<<<tiles, kBlock, 0, s>>>(w_, h_, ts, first ? 1 : 0, p, tio, gc, gp);
+    int* n_short = short_n_.as<int>();
+    const int ls = short_sel_;
+    k_temporal<<<tiles, kBlock, 0, s>>>(w_, h_, ts, first ? 1 : 0, p, tio, gc, gp, short_tiles_.as<int>(),
+                                        n_short + ls);
     HIP_OK(hipGetLastError());
     prof.end(kStageTemporal, e, s);
 
@@ -1732,9 +1816,11 @@ void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const fl
     fio.guide_nd = guide_nd_.as<float4>();
     fio.guide_kl = guide_kl_.as<int2>();
     fio.lum = lum_.as<float4>();
-    k_variance<<<tiles, kBlock, 0, s>>>(w_, h_, ts, p, fio, gc);
+    k_variance<<<tiles, kBlock, 0, s>>>(w_, h_, ts, p, fio, gc, short_tiles_.as<int>(), n_short + ls,
+                                        n_short + (ls ^ 1));
     HIP_OK(hipGetLastError());
     prof.end(kStageVariance, e, s);
+    short_sel_ = ls ^ 1;
 
     out_sel_ = -1;
     if (!first) {
@@ -1902,6 +1988,9 @@ void Renderer::stats(RenderStats& out) {
     out.shadow_nodes = s[kStatShadowNodes];
     out.shadow_tris = s[kStatShadowTris];
     out.probe_rays = s[kStatProbeRays];
+    out.miss_paths = s[kStatMisses];
+    out.shadow_lit = s[kStatShadowLit];
+    for (int k = 0; k < kNumClasses; ++k) out.class_paths[k] = s[kStatClassPaths + k];
     for (int k = 0; k < kNumStages; ++k) {
         out.stage_ms[k] = I.eng->prof.ms[k];
         out.stage_launches[k] = I.eng->prof.launches[k];

@@ -118,11 +118,18 @@ __device__ void gbuffer_pixel(const DevScene& sc, const GBufferParams& g, int i,
 // The frame's radiance comes in as per-pixel sample sums (scaled by inv_spp);
 // the blended colour goes to this frame's history buffer, which is also the
 // variance pass's input (the reference blends in place and then copies the
-// result to its history buffer: renderer.cuh:1212-1217).
+// result to its history buffer: renderer.cuh:1212-1217).  A pixel whose
+// history reaches kVarianceHistory frames keeps that colour through the
+// variance pass (renderer.cuh:703, 794-803), so the temporal pass also writes it
+// to the variance pass's output (var_d / var_i) and the variance pass visits
+// only the shorter histories.
+constexpr int kVarianceHistory = 4;
+
 struct TemporalIO {
     const float4 *acc_d, *acc_i;        // this frame's radiance sums
     float inv_spp;
     float4 *cur_d, *cur_i;              // blended colour, .w = variance (this frame's history)
+    float4 *var_d, *var_i;              // variance pass output, written for long histories
     const float4 *hist_d, *hist_i;      // previous frame's history
     const float2 *mom_prev_d, *mom_prev_i;
     const int* len_prev;
@@ -205,8 +212,9 @@ RT_HD void temporal_scale(TapSum& s, float inv) {
     s.len *= inv;
 }
 
-// TemporalDenoising (reference: renderer.cuh:524-677).
-RT_HD void temporal_pixel(int i, int j, int w, int h, bool first, const SvgfParams& p, const TemporalIO& io,
+// TemporalDenoising (reference: renderer.cuh:524-677).  Returns the pixel's
+// history length, which decides whether the variance pass estimates or copies.
+RT_HD int temporal_pixel(int i, int j, int w, int h, bool first, const SvgfParams& p, const TemporalIO& io,
                           const GBufferView& cur, const GBufferView& prev) {
     size_t idx = (size_t)j * w + i;
     float4 cd = io.acc_d[idx], ci = io.acc_i[idx];
@@ -228,7 +236,7 @@ RT_HD void temporal_pixel(int i, int j, int w, int h, bool first, const SvgfPara
             io.guide_nd[idx] = make_float4(n0.x, n0.y, n0.z, cur.motion_depth[idx].z);
             io.guide_kl[idx] = guide_key_len(gn0, cur.bary_mesh[idx], 1);
         }
-        return;
+        return 1;
     }
     float cx = (float)i + 0.5f, cy = (float)j + 0.5f;
     float4 gp = cur.pos_tri[idx];
@@ -331,6 +339,11 @@ RT_HD void temporal_pixel(int i, int j, int w, int h, bool first, const SvgfPara
     io.lum[idx] = lum_guide(cd, ci);
     io.guide_nd[idx] = make_float4(n.x, n.y, n.z, depth);
     io.guide_kl[idx] = guide_key_len(gn, gb, valid ? hl : 1);
+    if (valid && hl >= kVarianceHistory) {
+        io.var_d[idx] = cd;
+        io.var_i[idx] = ci;
+    }
+    return valid ? hl : 1;
 }
 
 // ---------------------------------------------------------------- variance
@@ -345,19 +358,15 @@ struct FilterIO {
 };
 
 // ComputeVarianceKernel (reference: renderer.cuh:689-805): a 7x7 bilateral
-// estimate while the history is shorter than 4 frames, else a copy.  Taps
-// read the per-pixel guide (the same decoded normals and depths the
-// reference's G-buffer fetches give); denominators that depend only on the
-// centre are formed once.
-RT_HD void variance_pixel(int i, int j, int w, int h, const SvgfParams& p, const FilterIO& io, const GBufferView& g) {
+// estimate while the history is shorter than kVarianceHistory frames, else a
+// copy of the input (the temporal pass writes that copy).  This is the
+// estimate for pixel (i, j).  Taps read the per-pixel guide (the same decoded
+// normals and depths the reference's G-buffer fetches give); denominators that
+// depend only on the centre are formed once.
+RT_HD void variance_estimate(int i, int j, int w, int h, const SvgfParams& p, const FilterIO& io,
+                             const GBufferView& g) {
     size_t idx = (size_t)j * w + i;
     int hl = io.len[idx];
-    float4 cd4 = io.in_d[idx], ci4 = io.in_i[idx];
-    if (hl >= 4) {
-        io.out_d[idx] = cd4;
-        io.out_i[idx] = ci4;
-        return;
-    }
     const float4 cl = io.lum[idx];
     const float cld = cl.x, cli = cl.z;
     const float4 cg = io.guide_nd[idx];

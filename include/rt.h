@@ -83,6 +83,7 @@ extern "C" {
 #define RT_STAGE_ATROUS 8
 #define RT_STAGE_SORT 9 /* ray sort before extend (rt_render_params.ray_sort) */
 #define RT_NUM_STAGES 10
+#define RT_NUM_CLASSES 5
 
 /* Buffers of the last rt_render_frame, row 0 = bottom. */
 #define RT_FRAME_DENOISED 0          /* float4: filtered direct + indirect, alpha 1 */
@@ -170,6 +171,13 @@ typedef struct rt_stats_out {
     int stage_launches[RT_NUM_STAGES];
     int spp_done;
     int batch_spp;
+    /* Counting renders only (count_traversal): extension rays that left the scene, shadow
+       rays that found their light point visible, and extension rays per shading class of
+       what they hit (light, diffuse, GGX, dielectric, subsurface; misses are in no class)
+       -- the path counts behind the stages' byte accounting. */
+    uint64_t miss_paths;
+    uint64_t shadow_lit;
+    uint64_t class_paths[RT_NUM_CLASSES];
 } rt_stats_out;
 
 /* Orbit camera (CameraParam, Src/Core/camera.h:11-84): the basis follows from these. */

@@ -64,7 +64,7 @@ def _moves(o0):
     ]
 
 
-def _run(scene_path, spp, atrous_iters=5):
+def _run(scene_path, spp, atrous_iters=5, moves=_moves):
     a = rtref.parse_scene(scene_path)
     ref_scene = rtref.RefScene(a)
     sp = rtref.SvgfParams.default(atrous_iters=atrous_iters)
@@ -76,7 +76,8 @@ def _run(scene_path, spp, atrous_iters=5):
     assert list(cam.target) == list(o0.target)
     assert (cam.distance, cam.theta_deg, cam.phi_deg) == (o0.distance, o0.theta_deg, o0.phi_deg)
     dp = rt.DenoiseParams.default(atrous_iterations=atrous_iters)
-    for f, o in enumerate(_moves(o0)):
+    lengths = []
+    for f, o in enumerate(moves(o0)):
         print(f"frame {f} orbit {None if o is None else (list(o.target), o.distance, o.theta_deg, o.phi_deg)}",
               flush=True)
         if o is not None:
@@ -84,6 +85,8 @@ def _run(scene_path, spp, atrous_iters=5):
         r.render_frame(spp, dp)
         ref = frames.frame(spp, o)
         _check_frame(r, ref, f"frame {f}")
+        lengths.append(r.read_frame(rt.RT_FRAME_HISTORY_LENGTH)[..., 0])
+    r.lengths = lengths
     return r, frames
 
 
@@ -102,6 +105,24 @@ def test_svgf_frames_bit_exact_cornell(scene_dir, oracle_lib):
 
 def test_svgf_frames_bit_exact_terrain_env_sss(scene_dir, oracle_lib):
     _run(scene_dir["terrain_small"], spp=2, atrous_iters=5)  # the real-time workload's 5 passes
+
+
+def test_variance_pass_visits_only_tiles_with_short_histories(scene_dir, oracle_lib):
+    """The variance pass estimates only the tiles that hold a history shorter
+    than 4 frames and takes the temporal pass's copy elsewhere: a held camera
+    until whole 16x16 tiles are long, then a move that shortens some pixels'
+    histories and leaves others long.  Every buffer of every frame matches the
+    oracle, whose variance pass visits every pixel."""
+    def moves(o0):
+        t = tuple(o0.target)
+        return [None] * 6 + [rtref.Orbit(t, o0.distance, o0.theta_deg, o0.phi_deg + 1.0), None]
+
+    r, _ = _run(scene_dir["cornell_spheres_rt"], spp=1, moves=moves)
+    held, moved = r.lengths[5], r.lengths[6]
+    h, w = held.shape
+    tiles = held[: h // 16 * 16, : w // 16 * 16].reshape(h // 16, 16, w // 16, 16).min(axis=(1, 3))
+    assert (tiles >= 4).any(), "no tile without a short history: the skip was not exercised"
+    assert (moved < 4).any() and (moved >= 4).any(), "the move should leave short and long histories"
 
 
 def test_svgf_frames_bit_exact_textured(scene_dir, oracle_lib):
