@@ -97,6 +97,14 @@ using LdsU32 = __attribute__((address_space(3))) uint32_t;
 // the ray does not graze (|cos| between ray and triangle normal >= 1e-3); the
 // hit then has prim = kEarlyExit.  Without such a triangle the result is the
 // closest hit, as with EARLY off.
+//
+// Each loop iteration is one interior node or ONE triangle of the current
+// leaf: a lane in a leaf keeps its next and end triangle in registers and
+// tests them over successive iterations.  Every ray makes the reference's
+// sequence of tests (a leaf's triangles in order, then the pop), so the hit
+// is the same; but a wave no longer runs a whole leaf's triangle loop for the
+// few lanes that reached leaves while the others wait, which made the
+// triangle tests most of the instructions issued.
 constexpr int kEarlyExit = -2;
 template <bool COUNT, bool CULL = true, int OCT = -1, bool EARLY = false>
 __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r, float tmin, float tmax,
@@ -111,17 +119,19 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r
     h.b2 = 0.f;
     h.prim = -1;
     uint32_t ref = sc.root_ref;
+    // In a leaf: triangles tn .. te-1 are still to be tested.
+    bool leaf = (ref & kLeafBit) != 0;
+    uint32_t tn = (ref & ~kLeafBit) >> 3, te = tn + (ref & 7u);
     int sp = 0;
     while (true) {
-        if (ref & kLeafBit) {
-            uint32_t first = (ref & ~kLeafBit) >> 3;
-            uint32_t n = ref & 7u;
-            for (uint32_t k = 0; k < n; ++k) {
-                GlobalV4* tp = tris + 3 * (size_t)(first + k);
+        uint32_t next;
+        if (leaf) {
+            if (tn < te) {
+                GlobalV4* tp = tris + 3 * (size_t)tn;
                 float4 a = as_f4(tp[0]), e1 = as_f4(tp[1]), e2 = as_f4(tp[2]);
                 if (COUNT) cnt.tris++;
                 if (tri_hit<CULL>(r, a, e1, e2, h.t, h.b1, h.b2)) {
-                    h.prim = (int)(first + k);
+                    h.prim = (int)tn;
                     if (EARLY && h.t < t_exit) {
                         vec3 ve1 = v3(e1.x, e1.y, e1.z), ve2 = v3(e2.x, e2.y, e2.z);
                         float det = dot(ve1, cross(r.d, ve2));
@@ -132,39 +142,35 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r
                         }
                     }
                 }
+                ++tn;
             }
+            if (tn < te) continue;
+            if (sp == 0) break;
+            next = stk[--sp * stride];
         } else {
             GlobalV4* np = nodes + 4 * (size_t)ref;
             float4 w0 = as_f4(np[0]), w1 = as_f4(np[1]), w2 = as_f4(np[2]);
             const NativeF4 w3f = np[3];
-            int4 w3 = make_int4(f_as_i(w3f.x), f_as_i(w3f.y), f_as_i(w3f.z), f_as_i(w3f.w));
+            const uint32_t rl = (uint32_t)f_as_i(w3f.x), rr = (uint32_t)f_as_i(w3f.y);
             if (COUNT) cnt.nodes++;
             float tr, tl;
             bool hr = box_hit<OCT>(r, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w, tmin, h.t, tr);
             bool hl = box_hit<OCT>(r, w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, tmin, h.t, tl);
-            uint32_t rl = (uint32_t)w3.x, rr = (uint32_t)w3.y;
             if (hr && hl) {
                 // Nearer child next; ties visit the left child first.
-                if (tr < tl) {
-                    stk[sp * stride] = rl;
-                    ref = rr;
-                } else {
-                    stk[sp * stride] = rr;
-                    ref = rl;
-                }
-                ++sp;
-                continue;
-            } else if (hr) {
-                ref = rr;
-                continue;
-            } else if (hl) {
-                ref = rl;
-                continue;
+                stk[sp++ * stride] = tr < tl ? rl : rr;
+                next = tr < tl ? rr : rl;
+            } else if (hr || hl) {
+                next = hr ? rr : rl;
+            } else {
+                if (sp == 0) break;
+                next = stk[--sp * stride];
             }
         }
-        if (sp == 0) break;
-        --sp;
-        ref = stk[sp * stride];
+        leaf = (next & kLeafBit) != 0;
+        ref = next;
+        tn = (next & ~kLeafBit) >> 3;
+        te = tn + (next & 7u);
     }
     return h;
 }
