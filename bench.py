@@ -493,13 +493,23 @@ def _write_scene_job(job):
 
 
 def _generate(jobs):
-    """Write generated scenes in parallel worker processes (spawned: no GPU state)."""
-    import multiprocessing as mp
-    from concurrent.futures import ProcessPoolExecutor
+    """Write generated scenes in parallel child interpreters (no GPU state), each a
+    `bench.py --write-scenes` run over every workers-th job.  Plain child processes
+    rather than a multiprocessing pool: a spawn-context pool leaves its resource
+    tracker process running after the benchmark ends."""
+    import subprocess
 
     workers = max(1, min(16, os.cpu_count() or 1, len(jobs)))
-    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
-        return list(ex.map(_write_scene_job, jobs))
+    shares = [jobs[k::workers] for k in range(workers)]
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--write-scenes", json.dumps(sh)],
+                              stdout=subprocess.PIPE) for sh in shares]
+    paths = [None] * len(jobs)
+    for k, p in enumerate(procs):
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(f"scene writer {k} exited with status {p.returncode}")
+        paths[k::workers] = json.loads(out)
+    return paths
 
 
 def _load(paths):
@@ -729,6 +739,7 @@ def main() -> int:
     ap.add_argument("--only", choices=("frames", "c1", "c2", "batched", "serving", "bvh", "textured"), default=None,
                     help="run one secondary workload and print its JSON")
     ap.add_argument("--calibrate-copy", action="store_true", help="run the 1 GiB copy that scales the counters")
+    ap.add_argument("--write-scenes", metavar="JOBS_JSON", help=argparse.SUPPRESS)
     ap.add_argument("--traffic-from-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB"),
                     help="print the traffic table from rocprofv3 --pmc databases")
     ap.add_argument("--no-frames", action="store_true", help="skip the real-time SVGF frame-loop measurement")
@@ -739,6 +750,9 @@ def main() -> int:
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU is design-only)"}))
         return 2
+    if args.write_scenes:
+        print(json.dumps([_write_scene_job(tuple(job)) for job in json.loads(args.write_scenes)]))
+        return 0
     if args.traffic_from_pmc:
         print(traffic_from_pmc(*args.traffic_from_pmc))
         return 0
