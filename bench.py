@@ -74,12 +74,80 @@ def _stage_of(kernel: str):
     single-scene) instantiations the timed renders launch."""
     import re
 
-    m = re.match(r"(?:rt::)?(k_extend|k_shadow)<false, false>$", kernel)
+    m = re.match(r"(?:rt::)?(k_extend<false, false|k_shadow<false, false(?:, false)?)>$", kernel)
     if m:
-        return m.group(1)
+        return m.group(1).split("<")[0]
     if re.match(r"(?:rt::)?k_shade<\d, false>$", kernel):
         return "k_shade"
     return None
+
+
+def _kernel_name(name: str) -> str:
+    import re
+
+    return re.sub(r"\(.*", "", name.replace("(anonymous namespace)::", "")).replace("void ", "").strip()
+
+
+def kernel_table(db, top: int = 30) -> str:
+    """Markdown summary of a rocprofv3 --kernel-trace database: calls, total and
+    average duration, share of GPU time and launch resources per kernel."""
+    import sqlite3
+
+    con = sqlite3.connect(db)
+    res = {kid: (vg, ag, sg) for kid, vg, ag, sg in
+           con.execute("select id, arch_vgpr_count, accum_vgpr_count, sgpr_count from rocpd_info_kernel_symbol")}
+    agg, total = {}, 0.0
+    for name, dur, grid, wg, lds, scr, kid in con.execute(
+            "select name, duration, grid_x*grid_y*grid_z, workgroup_x*workgroup_y*workgroup_z, lds_size, "
+            "scratch_size, kernel_id from kernels"):
+        a = agg.setdefault(_kernel_name(name), [0, 0.0, grid, wg, lds, scr, res.get(kid, ("", "", ""))])
+        a[0] += 1
+        a[1] += dur
+        total += dur
+    lines = ["| kernel | calls | total us | avg us | % GPU time | grid | block | LDS B | VGPR | AGPR | SGPR | scratch |",
+             "|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    for n, (cnt, d, grid, wg, lds, scr, (vg, ag, sg)) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+        lines.append(f"| {n} | {cnt} | {d / 1e3:.1f} | {d / 1e3 / cnt:.1f} | {100 * d / total:.2f} | {grid} | {wg} | "
+                     f"{lds} | {vg} | {ag} | {sg} | {scr} |")
+    return "\n".join(lines)
+
+
+def counter_table(kind: str, db, top: int = 12) -> str:
+    """Markdown summary of a rocprofv3 --pmc database, per kernel.  kind "sq": SQ
+    instruction mix per wave; kind "l2": L2 hit rate and VALU lane utilisation."""
+    import sqlite3
+    from collections import defaultdict
+
+    con = sqlite3.connect(db)
+    agg, disp, dur = defaultdict(lambda: defaultdict(float)), defaultdict(set), {}
+    for kn, d, cn, v, du in con.execute(
+            "select kernel_name, dispatch_id, counter_name, value, duration from counters_collection"):
+        n = _kernel_name(kn)
+        agg[n][cn] += v
+        disp[n].add(d)
+        dur[(n, d)] = du
+    tot = defaultdict(float)
+    for (n, _), du in dur.items():
+        tot[n] += du
+    rows = sorted(agg, key=lambda k: -tot[k])[:top]
+    if kind == "l2":
+        lines = ["| kernel | dispatches | L2 hit | VALU lane utilisation |", "|---|---|---|---|"]
+        for n in rows:
+            a = agg[n]
+            hit, miss = a.get("TCC_HIT_sum", 0), a.get("TCC_MISS_sum", 0)
+            util = a.get("SQ_THREAD_CYCLES_VALU", 0) / max(64 * a.get("SQ_ACTIVE_INST_VALU", 0), 1)
+            lines.append(f"| {n} | {len(disp[n])} | {100 * hit / max(hit + miss, 1):.1f} % | {100 * util:.1f} % |")
+        return "\n".join(lines)
+    lines = ["| kernel | dispatches | waves | VALU / wave | VMEM rd / wave | LDS / wave | wait / wave cycles | "
+             "VALU wave-instr / us |", "|---|---|---|---|---|---|---|---|"]
+    for n in rows:
+        a = agg[n]
+        w = a.get("SQ_WAVES", 0) or 1
+        lines.append(f"| {n} | {len(disp[n])} | {int(w)} | {a.get('SQ_INSTS_VALU', 0) / w:.0f} | "
+                     f"{a.get('SQ_INSTS_VMEM_RD', 0) / w:.0f} | {a.get('SQ_INSTS_LDS', 0) / w:.0f} | "
+                     f"{a.get('SQ_WAIT_INST_ANY', 0) / max(a.get('SQ_WAVE_CYCLES', 0), 1):.3f} | "
+                     f"{a.get('SQ_INSTS_VALU', 0) / (tot[n] / 1e3):.0f} |")
+    return "\n".join(lines)
 
 
 def traffic_from_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_bytes=1 << 30):
@@ -94,7 +162,7 @@ def traffic_from_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_byte
         con = sqlite3.connect(db)
         out = {}
         for name, disp, val in con.execute("select kernel_name, dispatch_id, value from counters_collection"):
-            k = re.sub(r"\(.*", "", name).replace("void ", "").strip()
+            k = _kernel_name(name)
             d = out.setdefault(k, {})
             d[disp] = d.get(disp, 0.0) + val
         return out
@@ -109,9 +177,10 @@ def traffic_from_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_byte
     w_scale = copy_bytes / (copy_kib(copy_write_db) * 1024)
     fetch, write = per_kernel(fetch_db), per_kernel(write_db)
     lines = [f"fetch_scale {f_scale:.4f}", f"write_scale {w_scale:.4f}", "",
-             "k_shade is per bounce: its class kernels' bytes summed, divided by the k_extend dispatches", "",
+             "k_shade is per bounce: its class kernels' bytes summed, divided by the bounces (the single-scene",
+             "k_extend dispatches, counting renders included, as their shade launches are the same kernels)", "",
              "stage      dispatches  fetch_B_per_launch  write_B_per_launch  hbm_B_per_launch"]
-    n_bounces = sum(len(fetch[k]) for k in fetch if _stage_of(k) == "k_extend")
+    n_bounces = sum(len(fetch[k]) for k in fetch if re.match(r"(?:rt::)?k_extend<(?:true|false), false>$", k))
     for st in TRAFFIC_STAGES:
         fk = [k for k in fetch if _stage_of(k) == st]
         wk = [k for k in write if _stage_of(k) == st]
@@ -123,15 +192,19 @@ def traffic_from_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_byte
         fb = sum(sum(fetch[k].values()) for k in fk) * 1024 * f_scale / (per or nf)
         wb = sum(sum(write[k].values()) for k in wk) * 1024 * w_scale / (per or nw)
         lines.append(f"{st:10s} {per or nf:11d} {fb:19.0f} {wb:19.0f} {fb + wb:17.0f}")
-    lines += ["", "instance-level (scaled bytes per dispatch):"]
-    for k in sorted(set(fetch) | set(write)):
-        if _stage_of(k) is None:
-            continue
-        f = fetch.get(k, {})
-        w = write.get(k, {})
+    def inst(k):
+        f, w = fetch.get(k, {}), write.get(k, {})
         fb = sum(f.values()) * 1024 * f_scale / max(1, len(f))
         wb = sum(w.values()) * 1024 * w_scale / max(1, len(w))
-        lines.append(f"  {k:32s} {len(f):5d} {fb:15.0f} {wb:15.0f}")
+        return f"  {k:36s} {len(f):5d} {fb:15.0f} {wb:15.0f}"
+
+    names = sorted(set(fetch) | set(write))
+    lines += ["", "instance-level (scaled bytes per dispatch):"]
+    lines += [inst(k) for k in names if _stage_of(k) is not None]
+    others = sorted((k for k in names if _stage_of(k) is None),
+                    key=lambda k: -(sum(fetch.get(k, {}).values()) + sum(write.get(k, {}).values())))
+    lines += ["", "other kernels, by total bytes (scaled bytes per dispatch):"]
+    lines += [inst(k) for k in others[:12]]
     return "\n".join(lines)
 
 
@@ -332,6 +405,9 @@ def measure_render(scene, steps: int, warmup: int, traffic_file=None, **renderer
         "ms_per_step": round(elapsed * 1e3 / steps, 3),
         "rays_per_step": int(rays // steps),
         "batch_spp": st.batch_spp,
+        "path_counts": {"extension_rays": sc.extension_rays, "misses": sc.miss_paths,
+                        "shaded_hits": dict(sc.as_dict()["class_paths"]), "shadow_rays": sc.shadow_rays,
+                        "shadow_lit": sc.shadow_lit, "bounces": int(round(bounces))},
         "nodes_per_ext_ray": round(sc.extension_node_visits / max(1, sc.extension_rays), 2),
         "tris_per_ext_ray": round(sc.extension_tri_tests / max(1, sc.extension_rays), 2),
         "stage_ms_per_step": {n: round(sp.stage_ms[i] / prof_steps, 3) for i, n in enumerate(rt.STAGES)
@@ -740,6 +816,9 @@ def main() -> int:
                     help="run one secondary workload and print its JSON")
     ap.add_argument("--calibrate-copy", action="store_true", help="run the 1 GiB copy that scales the counters")
     ap.add_argument("--write-scenes", metavar="JOBS_JSON", help=argparse.SUPPRESS)
+    ap.add_argument("--kernel-table", metavar="DB", help="print a rocprofv3 --kernel-trace database as a table")
+    ap.add_argument("--counter-table", nargs=2, metavar=("sq|l2", "DB"),
+                    help="print a rocprofv3 --pmc database (SQ or L2/lane counters) as a table")
     ap.add_argument("--traffic-from-pmc", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB"),
                     help="print the traffic table from rocprofv3 --pmc databases")
     ap.add_argument("--no-frames", action="store_true", help="skip the real-time SVGF frame-loop measurement")
@@ -752,6 +831,12 @@ def main() -> int:
         return 2
     if args.write_scenes:
         print(json.dumps([_write_scene_job(tuple(job)) for job in json.loads(args.write_scenes)]))
+        return 0
+    if args.kernel_table:
+        print(kernel_table(args.kernel_table))
+        return 0
+    if args.counter_table:
+        print(counter_table(args.counter_table[0], args.counter_table[1]))
         return 0
     if args.traffic_from_pmc:
         print(traffic_from_pmc(*args.traffic_from_pmc))
@@ -801,6 +886,7 @@ def main() -> int:
         "stage_ms_per_step": m["stage_ms_per_step"],
         "nodes_per_ext_ray": m["nodes_per_ext_ray"],
         "tris_per_ext_ray": m["tris_per_ext_ray"],
+        "path_counts_per_step": m["path_counts"],
     }
     # The shadow fast mode on the same workload: throughput, and whether every pixel of a full
     # render equals the closest-hit (reference) mode's, bit for bit.

@@ -189,8 +189,8 @@ private:
     DevBuf v_d_, v_i_, f_d_[2], f_i_[2], denoised_;
     DevBuf guide_nd_, guide_kl_;  // this frame's filter guide
     DevBuf lum_;                  // luminance guide of the variance pass's input
-    DevBuf short_tiles_, short_n_;  // tiles holding a short history, and a pair of counts
-    int short_sel_ = 0;             // which count of the pair the next temporal pass fills
+    DevBuf short_tiles_, short_n_;  // tiles holding a short history; two sets of per-band counts
+    int short_sel_ = 0;             // which set of counts the next temporal pass fills
     CameraRec last_cam_;
     int fcur_ = 0, flast_ = -1, frames_ = 0, out_sel_ = -1;
     int tile_shift_ = 4;

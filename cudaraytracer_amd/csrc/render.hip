@@ -671,19 +671,48 @@ __global__ void k_math(int op, const float* a, const float* b, float* out, int n
 // Per-pixel frame kernels work on tiles of 2^tile_shift x (256 >> tile_shift)
 // pixels (one 256-lane block), so the neighbourhood taps of the filters stay
 // within a few cache lines per wave; the shape is a tuning knob (16 x 16 by default).
-__device__ __forceinline__ bool tile_pixel(int w, int h, int tile_shift, int& i, int& j) {
+//
+// Tiles go to the XCDs in bands: workgroups are dispatched to the 8 XCDs
+// round-robin (XCD = blockIdx % 8), so workgroup b takes tile
+// (b % 8) * per + b / 8 with per = ceil(tiles / 8), and XCD x works through the
+// x-th eighth of the tiles in raster order -- whole rows of tiles, whose
+// neighbours' pixels the filters' taps find in that XCD's own L2 rather than
+// in another XCD's.  The grid is rounded up to a multiple of 8; the padding
+// workgroups get no tile.
+constexpr int kXcds = 8;
+inline int banded_grid(int tiles) { return kXcds * ((tiles + kXcds - 1) / kXcds); }
+__device__ __forceinline__ int banded_tile(int tiles) {
+    const int per = (tiles + kXcds - 1) / kXcds;
+    const int t = ((int)blockIdx.x % kXcds) * per + (int)blockIdx.x / kXcds;
+    return t < tiles ? t : -1;
+}
+__device__ __forceinline__ bool tile_pixel(int w, int h, int tile_shift, int& i, int& j, int& tile) {
     const int tw = 1 << tile_shift, th = kBlock >> tile_shift;
-    int tiles_x = (w + tw - 1) >> tile_shift;
-    int t = blockIdx.x;
-    i = (t % tiles_x) * tw + (threadIdx.x & (tw - 1));
-    j = (t / tiles_x) * th + (threadIdx.x >> tile_shift);
+    const int tiles_x = (w + tw - 1) >> tile_shift;
+    tile = banded_tile(tiles_x * ((h + th - 1) / th));
+    if (tile < 0) {
+        i = w;
+        j = h;
+        return false;
+    }
+    i = (tile % tiles_x) * tw + (threadIdx.x & (tw - 1));
+    j = (tile / tiles_x) * th + (threadIdx.x >> tile_shift);
     return i < w && j < h;
 }
+__device__ __forceinline__ bool tile_pixel(int w, int h, int tile_shift, int& i, int& j) {
+    int tile;
+    return tile_pixel(w, h, tile_shift, i, j, tile);
+}
 
-// The stack column comes from dynamic LDS sized to the tree's depth, as in extend.
+// The stack column comes from dynamic LDS sized to the tree's depth, as in
+// extend.  Tiles in plain raster order, one per workgroup: the G-buffer's cost
+// follows the scene's depth complexity, and round-robin spreads the costly
+// rows of tiles over all XCDs (in bands, measured 24 % slower).
 __global__ __launch_bounds__(kBlock) void k_gbuffer(DevScene sc, GBufferParams g, GBufferView out, int ts) {
-    int i, j;
-    if (!tile_pixel(g.width, g.height, ts, i, j)) return;
+    const int tw = 1 << ts, th = kBlock >> ts, tiles_x = (g.width + tw - 1) >> ts;
+    const int i = ((int)blockIdx.x % tiles_x) * tw + ((int)threadIdx.x & (tw - 1));
+    const int j = ((int)blockIdx.x / tiles_x) * th + ((int)threadIdx.x >> ts);
+    if (i >= g.width || j >= g.height) return;
     gbuffer_pixel(sc, g, i, j, stk_dyn + threadIdx.x, kBlock, out);
 }
 
@@ -693,31 +722,40 @@ __global__ __launch_bounds__(kBlock) void k_gbuffer(DevScene sc, GBufferParams g
 // temporal pass itself.  A listed tile is estimated as a whole workgroup, as
 // an unlisted pass would: the 7x7 gather is latency-bound, and a list of
 // single pixels (measured) leaves too few waves in flight and loses the
-// neighbourhood the lanes of a tile share.  One atomic per workgroup; every
-// lane reaches the barrier, also those outside the image.
+// neighbourhood the lanes of a tile share.  One list per XCD band (entries
+// band * per + k, per = ceil(tiles / 8)), so the variance pass visits a
+// band's tiles on the same XCD; one atomic per workgroup.  Every lane reaches
+// the barrier, also those outside the image.
 __global__ __launch_bounds__(kBlock) void k_temporal(int w, int h, int ts, int first, SvgfParams p, TemporalIO io,
                                                       GBufferView cur, GBufferView prev, int* __restrict__ tiles_out,
                                                       int* n_tiles) {
-    int i, j;
-    const bool in = tile_pixel(w, h, ts, i, j);
+    int i, j, tile;
+    const bool in = tile_pixel(w, h, ts, i, j, tile);
     int hl = kVarianceHistory;
     if (in) hl = temporal_pixel(i, j, w, h, first != 0, p, io, cur, prev);
-    if (__syncthreads_or(in && hl < kVarianceHistory) && threadIdx.x == 0) tiles_out[atomicAdd(n_tiles, 1)] = blockIdx.x;
+    if (__syncthreads_or(in && hl < kVarianceHistory) && threadIdx.x == 0) {
+        const int tw = 1 << ts, th = kBlock >> ts;
+        const int per = (((w + tw - 1) >> ts) * ((h + th - 1) / th) + kXcds - 1) / kXcds;
+        const int band = (int)blockIdx.x % kXcds;
+        tiles_out[band * per + atomicAdd(&n_tiles[band], 1)] = tile;
+    }
 }
 
-// Grid-stride over the temporal pass's tile list, one tile per workgroup
-// iteration; a lane estimates its pixel when that pixel's history is short.
-// The frame after this one lists into the other count of the pair, which is
-// cleared here (the two passes run in stream order, so nothing else touches
-// it meanwhile).
+// Over the temporal pass's tile lists: workgroup b takes entries b / 8,
+// b / 8 + per, ... of band b % 8's list, on the band's XCD; a lane estimates
+// its pixel when that pixel's history is short.  The frame after this one
+// lists into the other set of counts, which is cleared here (the two passes
+// run in stream order, so nothing else touches it meanwhile).
 __global__ __launch_bounds__(kBlock) void k_variance(int w, int h, int ts, SvgfParams p, FilterIO io, GBufferView g,
                                                       const int* __restrict__ tiles, const int* n_tiles,
                                                       int* n_tiles_next) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *n_tiles_next = 0;
-    const int n = *n_tiles;
+    if (blockIdx.x < kXcds && threadIdx.x == 0) n_tiles_next[blockIdx.x] = 0;
     const int tw = 1 << ts, th = kBlock >> ts, tiles_x = (w + tw - 1) >> ts;
-    for (int k = blockIdx.x; k < n; k += gridDim.x) {
-        const int t = tiles[k];
+    const int per = (tiles_x * ((h + th - 1) / th) + kXcds - 1) / kXcds;
+    const int band = (int)blockIdx.x % kXcds;
+    const int n = n_tiles[band];
+    for (int k = (int)blockIdx.x / kXcds; k < n; k += (int)gridDim.x / kXcds) {
+        const int t = tiles[band * per + k];
         const int i = (t % tiles_x) * tw + ((int)threadIdx.x & (tw - 1));
         const int j = (t / tiles_x) * th + ((int)threadIdx.x >> ts);
         if (i < w && j < h && io.len[(size_t)j * w + i] < kVarianceHistory) variance_estimate(i, j, w, h, p, io, g);
@@ -768,11 +806,13 @@ struct AtrousTile {
     }
 };
 
-__global__ __launch_bounds__(kBlock) void k_atrous_tile1(int w, int h, int tiles_x, SvgfParams p, FilterIO io,
-                                                          GBufferView g) {
+__global__ __launch_bounds__(kBlock) void k_atrous_tile1(int w, int h, int tiles_x, int tiles_y, SvgfParams p,
+                                                          FilterIO io, GBufferView g) {
     __shared__ int2 s_kl[kAtSide * kAtSide];
     __shared__ float4 s_nd[kAtSide * kAtSide], s_cd[kAtSide * kAtSide], s_ci[kAtSide * kAtSide];
-    const int x0 = ((int)blockIdx.x % tiles_x) * kAtTile, y0 = ((int)blockIdx.x / tiles_x) * kAtTile;
+    const int tile = banded_tile(tiles_x * tiles_y);
+    if (tile < 0) return;
+    const int x0 = (tile % tiles_x) * kAtTile, y0 = (tile / tiles_x) * kAtTile;
     for (int k = threadIdx.x; k < kAtSide * kAtSide; k += kBlock) {
         const int x = x0 - kAtApron + k % kAtSide, y = y0 - kAtApron + k / kAtSide;
         if (x >= 0 && x < w && y >= 0 && y < h) {
@@ -1680,15 +1720,16 @@ FrameState::FrameState(int width, int height) : w_(width), h_(height), wh_(width
     guide_nd_.alloc(wh * 16);
     guide_kl_.alloc(wh * 8);
     lum_.alloc(wh * 16);
-    // One entry per tile, for the tile shape (set_tile_shift: 3..7) with the most tiles.
+    // One list of ceil(tiles / 8) entries per XCD band, for the tile shape
+    // (set_tile_shift: 3..7) with the most tiles.
     size_t max_tiles = 0;
     for (int ts = 3; ts <= 7; ++ts) {
         const size_t tw = (size_t)1 << ts, th = (size_t)(kBlock >> ts);
         max_tiles = std::max(max_tiles, ((size_t)w_ + tw - 1) / tw * (((size_t)h_ + th - 1) / th));
     }
-    short_tiles_.alloc(std::max<size_t>(max_tiles, 1) * sizeof(int));
-    short_n_.alloc(2 * sizeof(int));
-    HIP_OK(hipMemset(short_n_.p, 0, 2 * sizeof(int)));
+    short_tiles_.alloc((size_t)banded_grid((int)std::max<size_t>(max_tiles, 1)) * sizeof(int));
+    short_n_.alloc(2 * kXcds * sizeof(int));
+    HIP_OK(hipMemset(short_n_.p, 0, 2 * kXcds * sizeof(int)));
     HIP_OK(hipEventCreateWithFlags(&ev_order_, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&ev_gbuf_, hipEventDisableTiming));
 }
@@ -1799,8 +1840,8 @@ void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const fl
     tio.lum = lum_.as<float4>();
     int* n_short = short_n_.as<int>();
     const int ls = short_sel_;
-    k_temporal<<<tiles, kBlock, 0, s>>>(w_, h_, ts, first ? 1 : 0, p, tio, gc, gp, short_tiles_.as<int>(),
-                                        n_short + ls);
+    k_temporal<<<banded_grid(tiles), kBlock, 0, s>>>(w_, h_, ts, first ? 1 : 0, p, tio, gc, gp,
+                                                     short_tiles_.as<int>(), n_short + kXcds * ls);
     HIP_OK(hipGetLastError());
     prof.end(kStageTemporal, e, s);
 
@@ -1816,8 +1857,8 @@ void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const fl
     fio.guide_nd = guide_nd_.as<float4>();
     fio.guide_kl = guide_kl_.as<int2>();
     fio.lum = lum_.as<float4>();
-    k_variance<<<tiles, kBlock, 0, s>>>(w_, h_, ts, p, fio, gc, short_tiles_.as<int>(), n_short + ls,
-                                        n_short + (ls ^ 1));
+    k_variance<<<banded_grid(tiles), kBlock, 0, s>>>(w_, h_, ts, p, fio, gc, short_tiles_.as<int>(),
+                                                     n_short + kXcds * ls, n_short + kXcds * (ls ^ 1));
     HIP_OK(hipGetLastError());
     prof.end(kStageVariance, e, s);
     short_sel_ = ls ^ 1;
@@ -1835,9 +1876,9 @@ void FrameState::end(int spp, const SvgfParams& p, const float4* acc_d, const fl
             fio.out_i = f_i_[o].as<float4>();
             if (k == 0) {
                 const int tx = (w_ + kAtTile - 1) / kAtTile, ty = (h_ + kAtTile - 1) / kAtTile;
-                k_atrous_tile1<<<tx * ty, kBlock, 0, s>>>(w_, h_, tx, p, fio, gc);
+                k_atrous_tile1<<<banded_grid(tx * ty), kBlock, 0, s>>>(w_, h_, tx, ty, p, fio, gc);
             } else {
-                k_atrous<<<tiles, kBlock, 0, s>>>(w_, h_, ts, 1 << k, p, fio, gc);
+                k_atrous<<<banded_grid(tiles), kBlock, 0, s>>>(w_, h_, ts, 1 << k, p, fio, gc);
             }
             HIP_OK(hipGetLastError());
             out_sel_ = o;
