@@ -77,7 +77,7 @@ def _stage_of(kernel: str):
     m = re.match(r"(?:rt::)?(k_extend<false, false|k_shadow<false, false(?:, false)?)>$", kernel)
     if m:
         return m.group(1).split("<")[0]
-    if re.match(r"(?:rt::)?k_shade<\d, false>$", kernel):
+    if re.match(r"(?:rt::)?k_shade<(?:\d, )?false>$", kernel):
         return "k_shade"
     return None
 

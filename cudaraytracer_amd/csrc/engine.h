@@ -117,7 +117,7 @@ private:
     template <bool POOLED>
     void run_t(int max_bounces, const bool* classes, bool count, int stack_entries);
     struct Grids {
-        int ext = 0, shadow = 0, shadow_any = 0, shade[kNumClasses] = {};
+        int ext = 0, shadow = 0, shadow_any = 0, shade = 0, shade_stack = 0;
     };
     template <bool POOLED>
     Grids grids_for(int stack_entries);

@@ -392,25 +392,27 @@ __device__ __forceinline__ T* shadow_field(T* base, size_t r, int f) {
 }
 
 // ------------------------------------------------------------------ shade
-// One kernel per shading class; each reads the queue indices binned by extend
-// and appends survivors / shadow records to the slice they came from.  At a
-// job's last bounce its survivors are dropped together with everything they
-// gathered (reference: Src/Core/renderer.cuh:1166-1194 never accumulates a
-// path still alive after maxBounces), unless the job flushes survivors.
+// One launch per bounce shades every class present in the pass: its waves
+// drain the class lists (binned by extend) in turn, the subsurface class
+// first -- few paths, each tracing two probe rays, a long latency that the
+// other classes' work then overlaps instead of running as a launch of its own
+// with the GPU nearly idle -- and the diffuse class, the bulk, last.  Each
+// class's code is specialised (shade_class<CLS>), so a wave never mixes
+// materials.  Survivors and shadow records are appended to the slice they
+// came from.  At a job's last bounce its survivors are dropped together with
+// everything they gathered (reference: Src/Core/renderer.cuh:1166-1194 never
+// accumulates a path still alive after maxBounces), unless the job flushes
+// survivors.
 template <int CLS, bool POOLED>
-__global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ jobs, int bounce,
-                                                   const float4* __restrict__ qo, const float4* __restrict__ qd,
-                                                   const float4* __restrict__ qw, const float4* __restrict__ hits,
-                                                   const int* __restrict__ list, float4* __restrict__ no,
-                                                   float4* __restrict__ nd, float4* __restrict__ nw,
-                                                   float4* __restrict__ shadow, float4* slot_d, float4* slot_i,
-                                                   const PassHeader* __restrict__ ph, int* counters, int qslot,
-                                                   uint16_t* __restrict__ nkey, const int* __restrict__ perm,
-                                                   const float4* __restrict__ sorted_o,
-                                                   const float4* __restrict__ sorted_d) {
-    const int region = ph->region;
-    uint32_t* stk = stk_dyn;  // sized only for the subsurface class (its probe rays)
-    uint32_t* col = stk + (CLS == kClsSubsurface ? threadIdx.x : 0);
+__device__ __forceinline__ void shade_class(const JobDev* __restrict__ jobs, int bounce,
+                                            const float4* __restrict__ qo, const float4* __restrict__ qd,
+                                            const float4* __restrict__ qw, const float4* __restrict__ hits,
+                                            const int* __restrict__ list, float4* __restrict__ no,
+                                            float4* __restrict__ nd, float4* __restrict__ nw,
+                                            float4* __restrict__ shadow, float4* slot_d, float4* slot_i,
+                                            int region, int* counters, int qslot, uint16_t* __restrict__ nkey,
+                                            const int* __restrict__ perm, const float4* __restrict__ sorted_o,
+                                            const float4* __restrict__ sorted_d, uint32_t* col) {
     SliceCursor cur;
     cur.init(counters, kCntClass0 + CLS, kFetchClass0 + CLS);
     int base;
@@ -496,6 +498,35 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
             }
         }
     }
+}
+
+// classes: bit c set when class c occurs in the pass.  The stack columns in
+// dynamic LDS are sized only when the subsurface class does (its probe rays).
+template <bool POOLED>
+__global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ jobs, int bounce, int classes,
+                                                   const float4* __restrict__ qo, const float4* __restrict__ qd,
+                                                   const float4* __restrict__ qw, const float4* __restrict__ hits,
+                                                   const int* __restrict__ lists, size_t cap,
+                                                   float4* __restrict__ no, float4* __restrict__ nd,
+                                                   float4* __restrict__ nw, float4* __restrict__ shadow,
+                                                   float4* slot_d, float4* slot_i,
+                                                   const PassHeader* __restrict__ ph, int* counters, int qslot,
+                                                   uint16_t* __restrict__ nkey, const int* __restrict__ perm,
+                                                   const float4* __restrict__ sorted_o,
+                                                   const float4* __restrict__ sorted_d) {
+    const int region = ph->region;
+    uint32_t* col = stk_dyn + threadIdx.x;
+    auto run = [&](auto cls) {
+        constexpr int CLS = decltype(cls)::value;
+        if (classes & (1 << CLS))
+            shade_class<CLS, POOLED>(jobs, bounce, qo, qd, qw, hits, lists + (size_t)CLS * cap, no, nd, nw, shadow,
+                                     slot_d, slot_i, region, counters, qslot, nkey, perm, sorted_o, sorted_d, col);
+    };
+    run(std::integral_constant<int, kClsSubsurface>());
+    run(std::integral_constant<int, kClsDielectric>());
+    run(std::integral_constant<int, kClsPbr>());
+    run(std::integral_constant<int, kClsTerminal>());
+    run(std::integral_constant<int, kClsDiffuse>());
 }
 
 // ------------------------------------------------------------------ shadow
@@ -1445,11 +1476,8 @@ Engine::Grids Engine::grids_for(int stack_entries) {
     g.ext = persistent_grid(k_extend<false, POOLED>, cus_, lds);
     g.shadow = persistent_grid(k_shadow<false, POOLED>, cus_, lds);
     g.shadow_any = persistent_grid(k_shadow<false, POOLED, true>, cus_, lds);
-    g.shade[kClsTerminal] = persistent_grid(k_shade<kClsTerminal, POOLED>, cus_);
-    g.shade[kClsDiffuse] = persistent_grid(k_shade<kClsDiffuse, POOLED>, cus_);
-    g.shade[kClsPbr] = persistent_grid(k_shade<kClsPbr, POOLED>, cus_);
-    g.shade[kClsDielectric] = persistent_grid(k_shade<kClsDielectric, POOLED>, cus_);
-    g.shade[kClsSubsurface] = persistent_grid(k_shade<kClsSubsurface, POOLED>, cus_, lds);
+    g.shade = persistent_grid(k_shade<POOLED>, cus_);
+    g.shade_stack = persistent_grid(k_shade<POOLED>, cus_, lds);
     grids_[key] = g;
     return g;
 }
@@ -1591,8 +1619,11 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
     auto scaled = [&](int g) { return std::max(1, (int)((float)g * grid_scale_ + 0.5f)); };
     const int gext = scaled(G.ext);
     const int gsh = scaled(shadow_any_ ? G.shadow_any : G.shadow);
-    int gshade[kNumClasses];
-    for (int c = 0; c < kNumClasses; ++c) gshade[c] = scaled(G.shade[c]);
+    // The shade launch carries the traversal stacks only when a subsurface class needs them.
+    const int gshade = scaled(classes[kClsSubsurface] ? G.shade_stack : G.shade);
+    const size_t shade_lds = classes[kClsSubsurface] ? lds : 0;
+    int class_bits = 0;
+    for (int c = 0; c < kNumClasses; ++c) class_bits |= classes[c] ? (1 << c) : 0;
     int q = 0;
     hipEvent_t b = prof.begin(stream_);
     k_raygen<POOLED><<<grid_linear_ * 4, kBlock, 0, stream_>>>(hdr, jobs, qo_[q].as<float4>(), qd_[q].as<float4>(),
@@ -1627,22 +1658,14 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
         HIP_OK(hipGetLastError());
         prof.end(kStageExtend, b, stream_);
         b = prof.begin(stream_);
-        // Classes absent from every job of the pass get no launch; their lists stay empty.
-        auto shade = [&](auto cls_tag, int grid) {
-            constexpr int CLS = decltype(cls_tag)::value;
-            k_shade<CLS, POOLED><<<grid, kBlock, CLS == kClsSubsurface ? lds : 0, stream_>>>(
-                jobs, bounce, qo_[q].as<float4>(), qd_[q].as<float4>(), qw_[q].as<float4>(), hits_.as<float4>(),
-                lists_.as<int>() + (size_t)CLS * cap_, qo_[q ^ 1].as<float4>(), qd_[q ^ 1].as<float4>(),
-                qw_[q ^ 1].as<float4>(), shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), hdr,
-                ctr, q, ray_sort_ ? keys_.as<uint16_t>() : nullptr, perm, perm ? sorted_o_.as<float4>() : nullptr,
-                perm ? sorted_d_.as<float4>() : nullptr);
-            HIP_OK(hipGetLastError());
-        };
-        if (classes[kClsTerminal]) shade(std::integral_constant<int, kClsTerminal>(), gshade[kClsTerminal]);
-        if (classes[kClsDiffuse]) shade(std::integral_constant<int, kClsDiffuse>(), gshade[kClsDiffuse]);
-        if (classes[kClsPbr]) shade(std::integral_constant<int, kClsPbr>(), gshade[kClsPbr]);
-        if (classes[kClsDielectric]) shade(std::integral_constant<int, kClsDielectric>(), gshade[kClsDielectric]);
-        if (classes[kClsSubsurface]) shade(std::integral_constant<int, kClsSubsurface>(), gshade[kClsSubsurface]);
+        // Classes absent from every job of the pass are skipped; their lists stay empty.
+        k_shade<POOLED><<<gshade, kBlock, shade_lds, stream_>>>(
+            jobs, bounce, class_bits, qo_[q].as<float4>(), qd_[q].as<float4>(), qw_[q].as<float4>(),
+            hits_.as<float4>(), lists_.as<int>(), cap_, qo_[q ^ 1].as<float4>(), qd_[q ^ 1].as<float4>(),
+            qw_[q ^ 1].as<float4>(), shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, q,
+            ray_sort_ ? keys_.as<uint16_t>() : nullptr, perm, perm ? sorted_o_.as<float4>() : nullptr,
+            perm ? sorted_d_.as<float4>() : nullptr);
+        HIP_OK(hipGetLastError());
         prof.end(kStageShade, b, stream_);
         b = prof.begin(stream_);
         if (count && shadow_any_)
