@@ -888,21 +888,34 @@ def main() -> int:
         "tris_per_ext_ray": m["tris_per_ext_ray"],
         "path_counts_per_step": m["path_counts"],
     }
-    # The shadow fast mode on the same workload: throughput, and whether every pixel of a full
-    # render equals the closest-hit (reference) mode's, bit for bit.
-    ma = measure_render(scene, args.steps, args.warmup, args.traffic_file, env_alias=True, shadow_any_hit=True)
-    ma.pop("dominant")
-    acc = []
-    for any_hit in (False, True):
-        rr = rt.Renderer(scene, env_alias=True, shadow_any_hit=any_hit)
+    # The shadow test's other modes on the same workload (rt_render_params.shadow_any_hit):
+    # 1 = early exit, whose full render must equal mode 0's bit for bit; 2 = the segment test,
+    # an approximation, whose full render is compared with mode 0's.
+    acc = {}
+    for mode in (0, 1, 2):
+        rr = rt.Renderer(scene, env_alias=True, shadow_any_hit=mode)
         rr.render(info.spp)
-        acc.append((rr.read_accum(rt.RT_ACCUM_DIRECT), rr.read_accum(rt.RT_ACCUM_INDIRECT)))
+        acc[mode] = [rr.read_accum(k) for k in (rt.RT_ACCUM_DIRECT, rt.RT_ACCUM_INDIRECT, rt.RT_ACCUM_MEAN)]
         del rr
-    same = all(np.array_equal(a.view(np.int32), b.view(np.int32)) for a, b in zip(acc[0], acc[1]))
-    out["shadow_any_hit"] = {"mrays_s": ma["mrays_s"], "ms_per_step": ma["ms_per_step"],
-                             "stage_ms_per_step": ma["stage_ms_per_step"],
-                             "shadow": ma["stages"]["k_shadow"], "image_bit_identical_to_closest_hit": bool(same)}
-    print(f"[bench] shadow any-hit: {ma['mrays_s']} Mrays/s, identical {same}", file=sys.stderr, flush=True)
+    out["shadow_modes"] = {}
+    for mode, name in ((1, "early_exit"), (2, "segment")):
+        mm = measure_render(scene, args.steps, args.warmup, args.traffic_file, env_alias=True, shadow_any_hit=mode)
+        mm.pop("dominant")
+        shadow = dict(mm["stages"]["k_shadow"], traffic=None)  # the measured traffic is mode 0's
+        rec = {"shadow_any_hit": mode, "mrays_s": mm["mrays_s"], "ms_per_step": mm["ms_per_step"],
+               "stage_ms_per_step": mm["stage_ms_per_step"], "shadow": shadow}
+        if mode == 1:
+            rec["image_bit_identical_to_mode_0"] = all(
+                np.array_equal(x.view(np.int32), y.view(np.int32)) for x, y in zip(acc[0][:2], acc[1][:2]))
+        else:
+            ref, img = acc[0][2][..., :3].astype(np.float64), acc[2][2][..., :3].astype(np.float64)
+            rec["image_vs_mode_0"] = {
+                "mean_rel_diff": round(float((img.mean() - ref.mean()) / ref.mean()), 6),
+                "rmse_over_mean": round(float(np.sqrt(np.mean((img - ref) ** 2)) / ref.mean()), 6),
+                "pixels_differing": round(float(np.mean(np.any(img != ref, axis=-1))), 6)}
+        out["shadow_modes"][name] = rec
+        print(f"[bench] shadow mode {mode}: {mm['mrays_s']} Mrays/s, shadow {mm['stage_ms_per_step']['shadow']} ms "
+              f"{ {k: v for k, v in rec.items() if k.startswith('image')} }", file=sys.stderr, flush=True)
     del acc
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (oracle on a pixel sample of the same workload)", file=sys.stderr, flush=True)

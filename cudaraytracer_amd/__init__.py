@@ -420,7 +420,9 @@ class Renderer:
     def __init__(self, scene: Scene, max_bounces: int = -1, rr: float = -1.0, rr_start_depth: int = -1,
                  seed: int = -1, batch_spp: int = 0, queue_budget_bytes: int = 0, flush_survivors: bool = False,
                  profile: bool = False, env_alias: bool = False, graphs: bool = True, ray_sort: bool = False,
-                 shadow_any_hit: bool = False):
+                 shadow_any_hit: int = 0):
+        """shadow_any_hit: 0 (or False) the reference's shadow test, 1 (or True) its early-exit
+        form (same results), 2 the segment test (fast mode; see rt_render_params)."""
         p = RenderParams()
         lib().rt_render_params_default(ctypes.byref(p))
         p.max_bounces, p.rr, p.rr_start_depth, p.seed = max_bounces, rr, rr_start_depth, seed

@@ -95,7 +95,7 @@ rt::RenderSettings settings_from(const rt_render_params* p, const rt::HostScene&
     set.use_graphs = p->disable_graphs ? 0 : 1;
     if (p->ray_sort != 0 && p->ray_sort != 1) throw std::invalid_argument("ray_sort must be 0 or 1");
     set.ray_sort = p->ray_sort;
-    if (p->shadow_any_hit != 0 && p->shadow_any_hit != 1) throw std::invalid_argument("shadow_any_hit must be 0 or 1");
+    if (p->shadow_any_hit < 0 || p->shadow_any_hit > 2) throw std::invalid_argument("shadow_any_hit must be 0, 1 or 2");
     set.shadow_any_hit = p->shadow_any_hit;
     if (set.max_bounces > 255) throw std::invalid_argument("max_bounces must be <= 255");
     return set;

@@ -110,14 +110,15 @@ public:
     void set_ray_sort(bool on);
     // Shadow rays stop at the first hit clearly in front of the shading point
     // (the any-hit fast mode) instead of finding the closest hit.
-    void set_shadow_any_hit(bool on) { shadow_any_ = on; }
+    // Shadow test: 0 the reference's, 1 its early-exit form, 2 the segment test (k_shadow).
+    void set_shadow_mode(int mode) { shadow_mode_ = mode; }
     float grid_scale() const { return grid_scale_; }
 
 private:
     template <bool POOLED>
     void run_t(int max_bounces, const bool* classes, bool count, int stack_entries);
     struct Grids {
-        int ext = 0, shadow = 0, shadow_any = 0, shade = 0, shade_stack = 0;
+        int ext = 0, shadow = 0, shadow_early = 0, shadow_segment = 0, shade = 0, shade_stack = 0;
     };
     template <bool POOLED>
     Grids grids_for(int stack_entries);
@@ -130,7 +131,7 @@ private:
     hipStream_t stream_ = nullptr;
     DevBuf qo_[2], qd_[2], qw_[2], hits_, lists_, shadow_, slot_d_, slot_i_, counters_, stats_;
     bool ray_sort_ = false;
-    bool shadow_any_ = false;
+    int shadow_mode_ = 0;
     DevBuf keys_, perm_, sort_hist_, sort_cur_;  // ray sort: key and permutation per path, per-slice bins
     DevBuf sorted_o_, sorted_d_;                 // ray sort: extend's copy of each ray at its sorted position
     // Pass tables: one device table, filled by stream-ordered copies from a

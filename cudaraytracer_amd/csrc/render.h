@@ -55,7 +55,7 @@ struct RenderSettings {
     int env_alias = 0;               // 1: environment sampled with alias tables (not the reference's CDF search)
     int use_graphs = 1;              // replay passes as captured HIP graphs
     int ray_sort = 0;                // 1: reorder extension rays by direction octant and origin cell
-    int shadow_any_hit = 0;          // 1: shadow rays stop at the first hit clearly in front of the shading point
+    int shadow_any_hit = 0;          // shadow test: 0 the reference's, 1 early exit (same results), 2 segment test
     int profile = 0;                 // per-stage HIP event timing
 };
 

@@ -530,13 +530,22 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
 }
 
 // ------------------------------------------------------------------ shadow
-// Closest hit from the light point towards the shading point with tMin 1e-6,
-// then the reference's 0.1 distance tolerances (Src/Core/light.cu:5-27).
-// ANY: the fast mode (rt_render_params.shadow_any_hit): stop at the first hit
-// clearly in front of the shading point, which the closest-hit test would
-// call occluded whenever the closest hit lies on the ray where its t says
-// (see trace_closest's EARLY).
-template <bool COUNT, bool POOLED, bool ANY = false>
+// MODE (rt_render_params.shadow_any_hit):
+// 0: the reference's test (Src/Core/light.cu:5-27): the closest hit from the
+//    light point towards the shading point with tMin 1e-6, then its 0.1
+//    distance tolerances.
+// 1: the same ray, stopped at the first hit clearly in front of the shading
+//    point, which the closest-hit test would call occluded whenever the
+//    closest hit lies on the ray where its t says (see trace_closest's EARLY).
+// 2: the segment test: from the shading point towards the light point, any
+//    triangle the reference's ray would meet from the front, between 0.1 past
+//    the shading point and 0.01 short of the light point (the reference's
+//    tolerances as distances along the segment), occludes.  Hits beyond the
+//    shading point and at the shading point's distance from the light, which
+//    the reference's test also calls occluded, are not looked for: an
+//    approximation of mode 0 whose difference the tests bound.
+constexpr int kShadowClosest = 0, kShadowEarly = 1, kShadowSegment = 2;
+template <bool COUNT, bool POOLED, int MODE = kShadowClosest>
 __global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jobs, const float4* __restrict__ shadow,
                                                     float4* slot_d, float4* slot_i, const PassHeader* __restrict__ ph,
                                                     int* counters, unsigned long long* stats) {
@@ -575,20 +584,28 @@ __global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jo
             rec.hp = v3(r2.x, r2.y, r2.z);
             vec3 org = rec.lp + 0.01f * rec.dir;
             RayInv ray = make_ray(org, rec.dir);
-            Hit h;
-            if (ANY) {
-                // test distance ~ 0.01 + t: a hit this far in front of the shading point is more
-                // than 0.1 from it both ways, with room for rounding in the hit point
-                const float scale = fmaxf(fmaxf(fabsf(org.x), fabsf(org.y)), fmaxf(fabsf(org.z), rec.sdist)) + 1.f;
-                const float t_exit = rec.sdist - 0.11f - 1e-4f * scale;
-                h = trace_closest_wave<COUNT, true, true>(sc, ray, 1e-6f, 1e8f, col, kBlock, c, t_exit);
+            bool visible;
+            if (MODE == kShadowSegment) {
+                const RayInv back = make_ray(rec.hp, -rec.dir);
+                visible = !trace_occluded<COUNT>(sc, back, 0.1f, rec.sdist - 0.01f, col, kBlock, c);
             } else {
-                h = trace_closest_wave<COUNT>(sc, ray, 1e-6f, 1e8f, col, kBlock, c);
+                Hit h;
+                if (MODE == kShadowEarly) {
+                    // test distance ~ 0.01 + t: a hit this far in front of the shading point is more
+                    // than 0.1 from it both ways, with room for rounding in the hit point
+                    const float scale =
+                        fmaxf(fmaxf(fabsf(org.x), fabsf(org.y)), fmaxf(fabsf(org.z), rec.sdist)) + 1.f;
+                    const float t_exit = rec.sdist - 0.11f - 1e-4f * scale;
+                    h = trace_closest_wave<COUNT, true, true>(sc, ray, 1e-6f, 1e8f, col, kBlock, c, t_exit);
+                } else {
+                    h = trace_closest_wave<COUNT>(sc, ray, 1e-6f, 1e8f, col, kBlock, c);
+                }
+                const bool hit = h.prim >= 0;
+                vec3 tp = v3(0.f, 0.f, 0.f);
+                if (hit) tp = position_at(sc.tri_attr, h.prim, h.b1, h.b2);
+                visible = h.prim != kEarlyExit && shadow_visible(rec, hit, tp);
             }
-            bool hit = h.prim >= 0;
-            vec3 tp = v3(0.f, 0.f, 0.f);
-            if (hit) tp = position_at(sc.tri_attr, h.prim, h.b1, h.b2);
-            if (h.prim != kEarlyExit && shadow_visible(rec, hit, tp)) {
+            if (visible) {
                 lit = true;
                 int pid = f_as_i(r1.w);
                 float4* s = (tag & 1) == 0 ? slot_d : slot_i;
@@ -1475,7 +1492,8 @@ Engine::Grids Engine::grids_for(int stack_entries) {
     Grids g;
     g.ext = persistent_grid(k_extend<false, POOLED>, cus_, lds);
     g.shadow = persistent_grid(k_shadow<false, POOLED>, cus_, lds);
-    g.shadow_any = persistent_grid(k_shadow<false, POOLED, true>, cus_, lds);
+    g.shadow_early = persistent_grid(k_shadow<false, POOLED, kShadowEarly>, cus_, lds);
+    g.shadow_segment = persistent_grid(k_shadow<false, POOLED, kShadowSegment>, cus_, lds);
     g.shade = persistent_grid(k_shade<POOLED>, cus_);
     g.shade_stack = persistent_grid(k_shade<POOLED>, cus_, lds);
     grids_[key] = g;
@@ -1572,7 +1590,7 @@ void Engine::run(const std::vector<PassItem>& items, bool count) {
     if (graph) {
         int mask = 0;
         for (int c = 0; c < kNumClasses; ++c) mask |= classes[c] ? (1 << c) : 0;
-        key = ((uint64_t)max_bounces << 24) | ((uint64_t)stack << 16) | ((uint64_t)mask << 8) | (shadow_any_ ? 4 : 0) |
+        key = ((uint64_t)max_bounces << 24) | ((uint64_t)stack << 16) | ((uint64_t)mask << 8) | ((uint64_t)shadow_mode_ << 32) |
               (ray_sort_ ? 2 : 0) | (pooled ? 1 : 0);
         auto it = graphs_.find(key);
         if (it != graphs_.end()) {
@@ -1618,7 +1636,9 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
     const size_t lds = (size_t)stack_entries * kBlock * sizeof(uint32_t);
     auto scaled = [&](int g) { return std::max(1, (int)((float)g * grid_scale_ + 0.5f)); };
     const int gext = scaled(G.ext);
-    const int gsh = scaled(shadow_any_ ? G.shadow_any : G.shadow);
+    const int gsh = scaled(shadow_mode_ == kShadowEarly     ? G.shadow_early
+                           : shadow_mode_ == kShadowSegment ? G.shadow_segment
+                                                            : G.shadow);
     // The shade launch carries the traversal stacks only when a subsurface class needs them.
     const int gshade = scaled(classes[kClsSubsurface] ? G.shade_stack : G.shade);
     const size_t shade_lds = classes[kClsSubsurface] ? lds : 0;
@@ -1668,18 +1688,17 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
         HIP_OK(hipGetLastError());
         prof.end(kStageShade, b, stream_);
         b = prof.begin(stream_);
-        if (count && shadow_any_)
-            k_shadow<true, POOLED, true><<<gsh, kBlock, lds, stream_>>>(
+        auto shadow = [&](auto count_tag, auto mode_tag) {
+            k_shadow<decltype(count_tag)::value, POOLED, decltype(mode_tag)::value><<<gsh, kBlock, lds, stream_>>>(
                 jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, st);
-        else if (count)
-            k_shadow<true, POOLED><<<gsh, kBlock, lds, stream_>>>(
-                jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, st);
-        else if (shadow_any_)
-            k_shadow<false, POOLED, true><<<gsh, kBlock, lds, stream_>>>(
-                jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, st);
-        else
-            k_shadow<false, POOLED><<<gsh, kBlock, lds, stream_>>>(
-                jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, st);
+        };
+        auto shadow_mode = [&](auto count_tag) {
+            if (shadow_mode_ == kShadowEarly) shadow(count_tag, std::integral_constant<int, kShadowEarly>());
+            else if (shadow_mode_ == kShadowSegment) shadow(count_tag, std::integral_constant<int, kShadowSegment>());
+            else shadow(count_tag, std::integral_constant<int, kShadowClosest>());
+        };
+        if (count) shadow_mode(std::true_type());
+        else shadow_mode(std::false_type());
         HIP_OK(hipGetLastError());
         prof.end(kStageShadow, b, stream_);
         q ^= 1;
@@ -1982,7 +2001,7 @@ Renderer::Renderer(const GpuScene& scene, const HostScene& host, const RenderSet
     I.eng.reset(new Engine((size_t)I.wh * I.batch, s.profile != 0));
     I.eng->set_use_graphs(s.use_graphs != 0);
     I.eng->set_ray_sort(s.ray_sort != 0);
-    I.eng->set_shadow_any_hit(s.shadow_any_hit != 0);
+    I.eng->set_shadow_mode(s.shadow_any_hit);
     I.acc_d.alloc((size_t)I.wh * 16);
     I.acc_i.alloc((size_t)I.wh * 16);
     HIP_OK(hipMemset(I.acc_d.p, 0, (size_t)I.wh * 16));

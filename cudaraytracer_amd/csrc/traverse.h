@@ -175,6 +175,73 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r
     return h;
 }
 
+// Any-hit test of the open segment (tmin, tmax) along r: true at the first
+// triangle met from its back (det < 0) -- the faces that a ray in the
+// opposite direction, which culls back faces, meets from the front -- with t
+// inside the segment.  The nearer child is still visited first, and each
+// iteration is one node or one triangle as in trace_closest; the first hit
+// ends the search, so the result does not depend on the visiting order.
+template <bool COUNT>
+__device__ __forceinline__ bool trace_occluded(const DevScene& sc, const RayInv& r, float tmin, float tmax,
+                                               uint32_t* stk_generic, int stride, TravCount& cnt) {
+    LdsU32* stk = (LdsU32*)stk_generic;
+    GlobalV4* nodes = (GlobalV4*)sc.nodes;
+    GlobalV4* tris = (GlobalV4*)sc.tri_isect;
+    uint32_t ref = sc.root_ref;
+    bool leaf = (ref & kLeafBit) != 0;
+    uint32_t tn = (ref & ~kLeafBit) >> 3, te = tn + (ref & 7u);
+    int sp = 0;
+    while (true) {
+        uint32_t next;
+        if (leaf) {
+            if (tn < te) {
+                GlobalV4* tp = tris + 3 * (size_t)tn;
+                const float4 a = as_f4(tp[0]), e1f = as_f4(tp[1]), e2f = as_f4(tp[2]);
+                if (COUNT) cnt.tris++;
+                const vec3 e1 = v3(e1f.x, e1f.y, e1f.z), e2 = v3(e2f.x, e2f.y, e2f.z);
+                const vec3 s1 = cross(r.d, e2);
+                const float det = dot(e1, s1);
+                if (det < 0.f) {
+                    const float inv = 1.f / det;
+                    const vec3 so = r.o - v3(a.x, a.y, a.z);
+                    const float b1 = inv * dot(s1, so);
+                    const vec3 s2 = cross(so, e1);
+                    const float b2 = inv * dot(s2, r.d);
+                    const float t = inv * dot(s2, e2);
+                    if (b1 >= 0.f && b2 >= 0.f && b1 + b2 <= 1.f && t > tmin && t < tmax) return true;
+                }
+                ++tn;
+            }
+            if (tn < te) continue;
+            if (sp == 0) break;
+            next = stk[--sp * stride];
+        } else {
+            GlobalV4* np = nodes + 4 * (size_t)ref;
+            float4 w0 = as_f4(np[0]), w1 = as_f4(np[1]), w2 = as_f4(np[2]);
+            const NativeF4 w3f = np[3];
+            const uint32_t rl = (uint32_t)f_as_i(w3f.x), rr = (uint32_t)f_as_i(w3f.y);
+            if (COUNT) cnt.nodes++;
+            float tr, tl;
+            bool hr = box_hit(r, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w, tmin, tmax, tr);
+            bool hl = box_hit(r, w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, tmin, tmax, tl);
+            if (hr && hl) {
+                stk[sp++ * stride] = tr < tl ? rl : rr;
+                next = tr < tl ? rr : rl;
+            } else if (hr || hl) {
+                next = hr ? rr : rl;
+            } else {
+                if (sp == 0) break;
+                next = stk[--sp * stride];
+            }
+        }
+        leaf = (next & kLeafBit) != 0;
+        ref = next;
+        tn = (next & ~kLeafBit) >> 3;
+        te = tn + (next & 7u);
+    }
+    return false;
+}
+
 // Octant of a ray as box_hit uses it (bit k: inverse direction negative on axis k).
 RT_HD int ray_octant(const RayInv& r) { return (r.nx ? 1 : 0) | (r.ny ? 2 : 0) | (r.nz ? 4 : 0); }
 

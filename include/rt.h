@@ -152,11 +152,18 @@ typedef struct rt_render_params {
                                traversal on coherent scenes such as closed rooms) */
     int shadow_any_hit;     /* 0 = the reference's shadow test (Light::Visible, Src/Core/light.cu:5-27):
                                the closest hit from the light point, then its 0.1 tolerances;
-                               1 = fast mode: a shadow ray stops at the first hit that lies, by its t,
-                               more than 0.11 (plus a rounding margin) in front of the shading point on a
-                               triangle it does not graze, and counts as occluded -- what the closest-hit
-                               test decides whenever the closest hit lies on the ray where its t says;
-                               rays without such a hit are traced exactly as in mode 0 */
+                               1 = the same test with an early exit: a shadow ray stops at the first hit
+                               that lies, by its t, more than 0.11 (plus a rounding margin) in front of
+                               the shading point on a triangle it does not graze, and counts as occluded
+                               -- what the closest-hit test decides whenever the closest hit lies on the
+                               ray where its t says; rays without such a hit are traced exactly as in
+                               mode 0;
+                               2 = fast mode, the segment test: any triangle the reference's ray would
+                               meet from the front, between 0.1 past the shading point and 0.01 short
+                               of the light point, occludes; the search starts at the shading point
+                               and stops at the first such hit.  An approximation: the reference also
+                               calls a ray occluded by a hit behind the shading point or at its
+                               distance from the light */
 } rt_render_params;
 
 typedef struct rt_stats_out {

@@ -567,6 +567,81 @@ static Hit bvh_trace(const ref_scene* s, const Ray* r, float tmin, float tmax, u
     return bvh_trace_c(s, r, tmin, tmax, nv, nt, 1);
 }
 
+/* The library's shadow fast mode (rt_render_params.shadow_any_hit = 2; no
+   reference counterpart): the segment from the shading point towards the
+   light point is occluded by any triangle met from its back -- the faces the
+   reference's light-to-point ray meets from the front -- strictly inside
+   (tmin, tmax).  FP32 in the library's operation order, so that decisions
+   at the boundaries are the library's too. */
+static int seg_isect(const Ray* r, const Tri* tr, float tmin, float tmax) {
+    V3 e1 = vsub(tr->p[1], tr->p[0]), e2 = vsub(tr->p[2], tr->p[0]);
+    V3 s1 = vcross(r->d, e2), so, s2;
+    float det = vdot(e1, s1), inv, b1, b2, t;
+    if (!(det < 0.f)) return 0;
+    inv = 1.f / det;
+    so = vsub(r->o, tr->p[0]);
+    b1 = inv * vdot(s1, so);
+    s2 = vcross(so, e1);
+    b2 = inv * vdot(s2, r->d);
+    t = inv * vdot(s2, e2);
+    return b1 >= 0.f && b2 >= 0.f && b1 + b2 <= 1.f && t > tmin && t < tmax;
+}
+
+/* The segment test over the BVH: nodes as bvh_trace_c visits them (the nearer
+   child first), stopping at the first occluding triangle. */
+static int bvh_occluded(const ref_scene* s, const Ray* r, float tmin, float tmax, uint32_t* nv, uint32_t* nt) {
+    int stack[64], sp = 0, cur = 0;
+    for (;;) {
+        const LNode* n = &s->nodes[cur];
+        if (n->tri_index != -1) {
+            int k;
+            for (k = 0; k < n->second; ++k) {
+                if (nt) (*nt)++;
+                if (seg_isect(r, &s->ordered[n->tri_index + k], tmin, tmax)) return 1;
+            }
+        } else {
+            int first = n->second, second = cur + 1;
+            float tf, ts;
+            int hf, hs;
+            if (nv) (*nv)++;
+            hf = slab(r, &s->nodes[first], tmin, tmax, &tf);
+            hs = slab(r, &s->nodes[second], tmin, tmax, &ts);
+            if (hf && hs) {
+                if (tf < ts) { stack[sp++] = second; stack[sp++] = first; }
+                else { stack[sp++] = first; stack[sp++] = second; }
+            } else if (hf) {
+                stack[sp++] = first;
+            } else if (hs) {
+                stack[sp++] = second;
+            }
+        }
+        if (sp == 0) break;
+        cur = stack[--sp];
+    }
+    return 0;
+}
+
+void ref_occluded(const ref_scene* s, const float* rays8, int n, int brute, int32_t* out, uint32_t* counts2) {
+    int i;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (i = 0; i < n; ++i) {
+        const float* q = rays8 + 8 * (size_t)i;
+        Ray r = make_ray(v3(q[0], q[1], q[2]), v3(q[4], q[5], q[6]));
+        uint32_t nv = 0, nt = 0;
+        int hit = 0, k;
+        if (brute) {
+            for (k = 0; k < s->nt && !hit; ++k) hit = seg_isect(&r, &s->ordered[k], q[3], q[7]);
+        } else {
+            hit = bvh_occluded(s, &r, q[3], q[7], &nv, &nt);
+        }
+        out[i] = hit;
+        if (counts2) {
+            counts2[2 * (size_t)i] = nv;
+            counts2[2 * (size_t)i + 1] = nt;
+        }
+    }
+}
+
 void ref_trace(const ref_scene* s, const float* rays8, int n, float* hits4, uint32_t* counts2) {
     int i;
 #pragma omp parallel for schedule(dynamic, 256)

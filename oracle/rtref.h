@@ -59,6 +59,8 @@ typedef struct ref_params {
     uint32_t seed;
     int flush_survivors;
     int env_alias; /* 1: environment sampled with alias tables instead of the CDF search */
+    int shadow_mode; /* 0: the reference's shadow test; 2: the library's segment test (its fast
+                        mode 2; mode 1 gives mode 0's results and is not restated) */
 } ref_params;
 
 typedef struct ref_scene ref_scene;
@@ -99,6 +101,10 @@ void ref_camera(const ref_scene* s, float* out12);       /* position, left_down,
 /* Closest hit for rays (o.xyz, tmin, d.xyz, tmax): hits (t, b1, b2, ordered prim as int bits),
    counts (interior nodes whose children were tested, triangle tests). */
 void ref_trace(const ref_scene* s, const float* rays8, int n, float* hits4, uint32_t* counts2);
+/* The segment test of the library's shadow mode 2 for n rays (o.xyz, tmin, d.xyz, tmax): out[i] = 1 when a
+   triangle met from its back lies strictly inside (tmin, tmax); by the BVH (counts2: node visits and
+   triangle tests, may be NULL) or, brute = 1, by testing every triangle. */
+void ref_occluded(const ref_scene* s, const float* rays8, int n, int brute, int32_t* out, uint32_t* counts2);
 
 /* Accumulate samples [sample0, sample0+spp) for the listed pixels (all when pixels == NULL),
    in sample order; acc_* are npix*4 floats.  stats[0..1] += extension / shadow rays.

@@ -47,7 +47,8 @@ class Desc(ctypes.Structure):
 
 class Params(ctypes.Structure):
     _fields_ = [("max_bounces", ctypes.c_int), ("rr", ctypes.c_float), ("rr_start_depth", ctypes.c_int),
-                ("seed", ctypes.c_uint32), ("flush_survivors", ctypes.c_int), ("env_alias", ctypes.c_int)]
+                ("seed", ctypes.c_uint32), ("flush_survivors", ctypes.c_int), ("env_alias", ctypes.c_int),
+                ("shadow_mode", ctypes.c_int)]
 
 
 class Orbit(ctypes.Structure):
@@ -94,6 +95,7 @@ def lib():
         L.ref_bvh_order.argtypes = [vp, vp]
         L.ref_camera.argtypes = [vp, vp]
         L.ref_trace.argtypes = [vp, vp, ctypes.c_int, vp, vp]
+        L.ref_occluded.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.ref_render.argtypes = [vp, ctypes.POINTER(Params), ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp]
         L.ref_render_f64.argtypes = [vp, ctypes.POINTER(Params), ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp]
         L.ref_philox.argtypes = [vp, vp, vp]
@@ -486,7 +488,7 @@ class RefScene:
             pass
 
     def params(self, max_bounces=None, rr=None, rr_start_depth=None, seed=None, flush_survivors=False,
-               env_alias=False) -> Params:
+               env_alias=False, shadow_mode=0) -> Params:
         a = self.a
         p = Params()
         p.max_bounces = a.max_bounces if max_bounces is None else max_bounces
@@ -495,6 +497,7 @@ class RefScene:
         p.seed = a.seed if seed is None else seed
         p.flush_survivors = int(flush_survivors)
         p.env_alias = int(env_alias)
+        p.shadow_mode = int(shadow_mode)
         return p
 
     def bvh(self):
@@ -529,6 +532,17 @@ class RefScene:
         counts = np.zeros((n, 2), np.uint32)
         lib().ref_trace(self._h, _p(rays), n, _p(hits), _p(counts))
         return hits[:, 0].copy(), hits[:, 1].copy(), hits[:, 2].copy(), hits[:, 3].copy().view(np.int32), counts
+
+    def occluded(self, rays: np.ndarray, brute: bool = False):
+        """The library's shadow segment test (mode 2) for (n, 8) rays o.xyz, tmin, d.xyz, tmax:
+        (occluded bool array, (n, 2) node visits / triangle tests); brute tests every triangle
+        and counts nothing."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        out = np.zeros(n, np.int32)
+        counts = np.zeros((n, 2), np.uint32)
+        lib().ref_occluded(self._h, _p(rays), n, int(brute), _p(out), _p(counts))
+        return out.astype(bool), counts
 
     def render(self, spp: int, params: Params, sample0: int = 0, pixels: Optional[np.ndarray] = None):
         """Returns (direct, indirect) sums, (npix, 4) float32, and [ext, shadow] ray counts."""
