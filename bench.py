@@ -74,7 +74,7 @@ def _stage_of(kernel: str):
     single-scene) instantiations the timed renders launch."""
     import re
 
-    m = re.match(r"(?:rt::)?(k_extend<false, false|k_shadow<false, false(?:, false)?)>$", kernel)
+    m = re.match(r"(?:rt::)?(k_extend<false, false|k_shadow<false, false(?:, (?:false|0))?)>$", kernel)
     if m:
         return m.group(1).split("<")[0]
     if re.match(r"(?:rt::)?k_shade<(?:\d, )?false>$", kernel):
