@@ -29,6 +29,8 @@ RT_HD RayInv make_ray(vec3 o, vec3 d) {
 
 // Slab test with the reference's 1e-2 slack: (double)x + 1e-2 >= 0 holds
 // exactly when x >= -0.01f for every float x, so the test stays in FP32.
+// "exit >= -0.01 and exit >= tmin" is one comparison against the larger of
+// the two (a loop invariant); a NaN tmin keeps both tests false.
 // OCT >= 0 fixes the signs of the inverse direction at compile time (bit k
 // set: axis k negative) for rays known to have them; the arithmetic is the
 // same, only the per-axis selects disappear.
@@ -47,7 +49,8 @@ RT_HD bool box_hit(const RayInv& r, float lx, float ly, float lz, float hx, floa
     float exit = fminf(ext_x, fminf(ext_y, ext_z));
     float enter = fmaxf(ent_x, fmaxf(ent_y, ent_z));
     t = enter;
-    return (exit - enter >= -0.01f) && (exit >= -0.01f) && (enter <= tmax) && (exit >= tmin);
+    const float exit_lo = (tmin >= -0.01f || tmin != tmin) ? tmin : -0.01f;
+    return (exit - enter >= -0.01f) && (exit >= exit_lo) && (enter <= tmax);
 }
 
 // Moller-Trumbore with the triangle's own [0, 1e8] range.  Returns true and
