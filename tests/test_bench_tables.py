@@ -1,0 +1,60 @@
+"""The bench's rocprofv3 table logic on small synthetic databases: counter
+scaling by the copy calibration, the shade stage counted per bounce (all class
+kernels of a bounce summed), and which kernel instances make a stage."""
+import os
+import sqlite3
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+
+def _db(path, rows):
+    con = sqlite3.connect(path)
+    con.execute("create table counters_collection (kernel_name text, dispatch_id integer, value real, "
+                "counter_name text, duration real)")
+    con.executemany("insert into counters_collection values (?, ?, ?, 'X', 1000.0)", rows)
+    con.commit()
+    con.close()
+    return str(path)
+
+
+def test_stage_instances():
+    assert bench._stage_of("rt::k_extend<false, false>") == "k_extend"
+    assert bench._stage_of("rt::k_extend<true, false>") is None       # counting render
+    assert bench._stage_of("rt::k_extend<false, true>") is None       # pooled
+    assert bench._stage_of("rt::k_shadow<false, false, 0>") == "k_shadow"
+    assert bench._stage_of("rt::k_shadow<false, false, 2>") is None   # another shadow test
+    assert bench._stage_of("rt::k_shade<false>") == "k_shade"
+    assert bench._stage_of("rt::k_shade<1, false>") == "k_shade"
+    assert bench._kernel_name("void rt::(anonymous namespace)::k_bvh_node<1>(int, float*)") == "rt::k_bvh_node<1>"
+
+
+def test_traffic_per_bounce_and_scaling(tmp_path):
+    # Two renders of 3 bounces: 6 plain extend dispatches, plus one counting render of
+    # 3 bounces whose shade launches are the same kernel.  Shade has one launch per bounce.
+    ext = [("void rt::k_extend<false, false>(args)", d, 100.0) for d in range(6)]
+    ext += [("void rt::k_extend<true, false>(args)", 10 + d, 500.0) for d in range(3)]
+    shade = [("void rt::k_shade<false>(args)", 20 + d, 40.0) for d in range(9)]
+    shadow = [("void rt::k_shadow<false, false, 0>(args)", 40 + d, 60.0) for d in range(6)]
+    fetch = _db(tmp_path / "f.db", ext + shade + shadow)
+    write = _db(tmp_path / "w.db", [(k, d, v / 10) for k, d, v in ext + shade + shadow])
+    gib = 1 << 30
+    # The copy kernel reads half its bytes on FETCH_SIZE and all of them on WRITE_SIZE.
+    cf = _db(tmp_path / "cf.db", [("__amd_rocclr_copyBuffer", 1, gib / 1024 / 2)])
+    cw = _db(tmp_path / "cw.db", [("__amd_rocclr_copyBuffer", 1, gib / 1024)])
+    out = bench.traffic_from_pmc(fetch, write, cf, cw)
+    rows = {ln.split()[0]: ln.split() for ln in out.splitlines() if ln.startswith("k_")}
+    assert out.splitlines()[0] == "fetch_scale 2.0000"
+    # extend: the plain dispatches only, 100 KiB x 2 fetched and 10 KiB written per launch
+    assert rows["k_extend"][1:] == ["6", str(200 * 1024), str(10 * 1024), str(210 * 1024)]
+    # shade: 9 dispatches over 9 bounces (6 plain + 3 counting)
+    assert rows["k_shade"][1:] == ["9", str(80 * 1024), str(4 * 1024), str(84 * 1024)]
+    assert rows["k_shadow"][1:] == ["6", str(120 * 1024), str(6 * 1024), str(126 * 1024)]
+    (tmp_path / "traffic.txt").write_text("# header\n" + out + "\n")
+    assert bench.load_traffic(str(tmp_path / "traffic.txt")) == {
+        "k_extend": 210 * 1024, "k_shade": 84 * 1024, "k_shadow": 126 * 1024}
+    assert bench.load_traffic(str(tmp_path / "absent.txt")) == {}
