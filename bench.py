@@ -1,9 +1,10 @@
 """Benchmark: wavefront path tracing throughput on one MI355X.
 
 Headline workload (one step): the large-mesh scene -- a 2,000,000-triangle fBm
-terrain plus a subsurface icosphere under a procedural 2048x1024 HDR sky
-(peak 1e4 sun lobe) sampled through alias tables -- at 1920x1080, 32 samples
-per pixel, max depth 16, rendered in full.  Inputs (BVH, triangles, env map,
+terrain plus a random-walk subsurface icosphere under a procedural 2048x1024
+HDR sky (peak 1e4 sun lobe) sampled through alias tables -- at 1920x1080, 32
+samples per pixel, max depth 16, rendered in full.  The same render with the
+reference's Burley subsurface profile is timed beside it (sss_burley).  Inputs (BVH, triangles, env map,
 queues) are resident in HBM before timing starts.
 
 Prints one JSON line.  `value` is Mrays/s over the timed steps (extension +
@@ -420,6 +421,20 @@ def measure_render(scene, steps: int, warmup: int, traffic_file=None, **renderer
 def c3_scene_path():
     tmp = tempfile.mkdtemp(prefix="rtc3_")
     return _generate([("terrain_env", {}, tmp, "terrain_c3")])[0]
+
+
+def burley_variant(path: str) -> str:
+    """A copy of a scene JSON (same directory, same OBJ files) whose
+    random-walk subsurface materials use the Burley profile instead."""
+    with open(path) as f:
+        doc = json.load(f)
+    for m in doc["materials"]:
+        if m.get("subsurface_method") == "random_walk":
+            m["subsurface_method"] = "burley"
+    out = os.path.splitext(path)[0] + "_burley.json"
+    with open(out, "w") as f:
+        json.dump(doc, f)
+    return out
 
 
 def c2_bench(steps: int = 5, warmup: int = 1, ray_sort: bool = True):
@@ -874,9 +889,10 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (procedural scene generated in the reference's scene-JSON + OBJ format)",
-        "config": {"workload": "terrain_env: 2.0M-triangle fBm terrain + subsurface icosphere (5120 tris) under a "
-                               "procedural 2048x1024 FP32 HDR sky (Gaussian sun lobe, peak 1e4) sampled by alias "
-                               "tables, 1920x1080, 32 spp, depth 16",
+        "config": {"workload": "terrain_env: 2.0M-triangle fBm terrain + random-walk subsurface icosphere (5120 "
+                               "tris, sigma_a and sigma_s log-uniform 0.1-10 per m) under a procedural 2048x1024 "
+                               "FP32 HDR sky (Gaussian sun lobe, peak 1e4) sampled by alias tables, 1920x1080, "
+                               "32 spp, depth 16",
                    "triangles": info.num_triangles, "width": info.width, "height": info.height, "spp": info.spp,
                    "max_depth": info.max_bounces, "batch_spp": m["batch_spp"], "rays_per_step": m["rays_per_step"],
                    "ray_sort": False},
@@ -917,6 +933,17 @@ def main() -> int:
         print(f"[bench] shadow mode {mode}: {mm['mrays_s']} Mrays/s, shadow {mm['stage_ms_per_step']['shadow']} ms "
               f"{ {k: v for k, v in rec.items() if k.startswith('image')} }", file=sys.stderr, flush=True)
     del acc
+    # The same workload with the reference's subsurface method (Burley profile and
+    # two probe rays, parameters mapped from the medium's coefficients) instead
+    # of the random walk: the scene JSON rewritten beside the original.
+    bpath = burley_variant(path)
+    mb = measure_render(rt.Scene(bpath), args.steps, args.warmup, None, env_alias=True)
+    mb.pop("dominant")
+    out["sss_burley"] = {"subsurface_method": "burley", "mrays_s": mb["mrays_s"], "ms_per_step": mb["ms_per_step"],
+                         "stage_ms_per_step": mb["stage_ms_per_step"],
+                         "subsurface_paths_per_step": mb["path_counts"]["shaded_hits"]["subsurface"]}
+    print(f"[bench] Burley subsurface variant: {mb['mrays_s']} Mrays/s, {mb['ms_per_step']} ms/step",
+          file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (oracle on a pixel sample of the same workload)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(path, info.spp, pixel_stride=args.cpu_pixel_stride, scene=scene,

@@ -377,12 +377,13 @@ class Scene:
         return out
 
     def materials(self) -> np.ndarray:
-        """Material records in "materials" order: (n, 16) float32 words (type, basecolor,
-        emission, roughness, metallic, transmission, ior, scatter distance, texture, valid;
-        type, texture and valid are int32 bit patterns)."""
+        """Material records in "materials" order: (n, 24) float32 words (type, basecolor,
+        emission, roughness, metallic, transmission, ior, scatter distance, texture, valid,
+        sigma_a, sigma_s, subsurface method, padding; type, texture, valid and the method
+        are int32 bit patterns)."""
         n = ctypes.c_int(0)
         _check(lib().rt_scene_get_materials(self._h, ctypes.byref(n), None))
-        out = np.zeros((n.value, 16), np.float32)
+        out = np.zeros((n.value, 24), np.float32)
         _check(lib().rt_scene_get_materials(self._h, ctypes.byref(n), out.ctypes.data_as(ctypes.c_void_p)))
         return out
 

@@ -33,6 +33,7 @@ struct ShadowRec {
 struct ShadeOut {
     bool alive;
     bool shadow;
+    int sss_rays;       // rays the subsurface event traced (Burley probes or walk segments)
     bool add_direct;    // contribution target: direct (first bounce) or indirect
     vec3 emit;          // radiance added now (light hit / environment)
     bool has_emit;
@@ -155,6 +156,87 @@ __device__ __forceinline__ void sss_probe(const DevScene& sc, vec3 hit_p, vec3 n
     }
 }
 
+// Random-walk subsurface transport, the "random_walk" subsurface method (the
+// reference has only the Burley profile; this follows the walk of Chiang,
+// Kutz and Burley, "Practical and controllable subsurface scattering for
+// production path tracing", SIGGRAPH 2016 talk, with isotropic scattering).
+// The path enters the medium in a cosine-distributed direction about the
+// inward normal (diffuse transmission) and walks.  Each step traces the
+// segment to the first surface of any mesh, both faces counting (the walk sees
+// the boundary from inside), picks one colour channel uniformly and draws a
+// free-flight distance s = -ln(u) / sigma_t of that channel; the pdf is the
+// mean over the channels (one-sample MIS over wavelengths).  Past the surface
+// the walk leaves there with weight T / mean(T), T = exp(-sigma_t t) per
+// channel; otherwise it scatters at s with weight sigma_s T / mean(sigma_t T),
+// turns to a uniformly distributed direction and plays Russian roulette on its
+// largest channel.  Absorbed, escaped (no surface ahead: an open mesh) and
+// over-long walks end the path.  Step k draws from its own coordinate
+// (bounce << 8 | k, kStreamWalk): channel, distance, then two direction
+// numbers and the roulette number when it scatters.
+constexpr int kWalkMaxSteps = 256;
+
+struct WalkExit {
+    vec3 p;     // exit point on the surface
+    vec3 n;     // unit geometric normal there, facing out of the medium
+    vec3 w;     // the walk's weight per channel
+    bool ok;
+    int steps;  // segments traced
+};
+
+__device__ __forceinline__ WalkExit sss_random_walk(const DevScene& sc, const DevMedium& md, vec3 hit_p,
+                                                   vec3 n_out, float e1, float e2, uint32_t seed, uint32_t pixel,
+                                                   uint32_t sample, uint32_t bounce, uint32_t* stk, int stride) {
+    WalkExit ex;
+    ex.ok = false;
+    ex.p = hit_p;
+    ex.n = n_out;
+    ex.w = v3(0.f, 0.f, 0.f);
+    ex.steps = 0;
+    const vec3 st = md.sigma_a + md.sigma_s;
+    vec3 thr = v3(1.f, 1.f, 1.f);
+    vec3 d = sample_diffuse_dir(-n_out, e1, e2);
+    vec3 p = hit_p - n_out * 1e-4f;
+    for (int k = 0; k < kWalkMaxSteps; ++k) {
+        Rng g;
+        g.init(seed, pixel, sample, (bounce << 8) | (uint32_t)k, kStreamWalk);
+        RayInv r = make_ray(p, d);
+        TravCount c{0, 0};
+        Hit h = trace_closest<false, false>(sc, r, 0.f, 1e8f, stk, stride, c);
+        ex.steps = k + 1;
+        int ch = clampi(0, 2, (int)(g.next() * 3.f));
+        float stc = ch == 0 ? st.x : (ch == 1 ? st.y : st.z);
+        float s = -fm::logf_(g.next()) / stc;
+        if (h.prim < 0) return ex;
+        if (s >= h.t) {
+            vec3 T = v3(fm::expf_(-st.x * h.t), fm::expf_(-st.y * h.t), fm::expf_(-st.z * h.t));
+            float pdf = ((T.x + T.y) + T.z) / 3.f;
+            if (!(pdf > 0.f)) return ex;
+            SurfaceHit q = surface_at(sc.tri_attr, h.prim, h.b1, h.b2);
+            vec3 ng = normalize(q.ng);
+            ex.p = q.p;
+            ex.n = dot(ng, d) < 0.f ? -ng : ng;
+            ex.w = cmul(thr, T) / pdf;
+            ex.ok = true;
+            return ex;
+        }
+        vec3 T = v3(fm::expf_(-st.x * s), fm::expf_(-st.y * s), fm::expf_(-st.z * s));
+        float pdf = ((st.x * T.x + st.y * T.y) + st.z * T.z) / 3.f;
+        if (!(pdf > 0.f)) return ex;
+        thr = cmul(thr, cmul(md.sigma_s, T)) / pdf;
+        p = p + d * s;
+        float z = 1.f - 2.f * g.next();
+        float phi = (float)(kTwoPiD * (double)g.next());
+        float sr = sqrtf(fmaxf(0.f, 1.f - z * z));
+        d = v3(sr * fm::cosf_(phi), sr * fm::sinf_(phi), z);
+        float q = fmaxf(thr.x, fmaxf(thr.y, thr.z));
+        if (q < 1.f) {
+            if (g.next() >= q) return ex;
+            thr = thr / q;
+        }
+    }
+    return ex;
+}
+
 // One bounce of one path.  `hit` holds (t, b1, b2, prim) from the extend
 // stage; CLS is the shading class of what was hit, fixed at compile time so
 // each class kernel carries only its own material code.
@@ -166,6 +248,7 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
     out.alive = false;
     out.shadow = false;
     out.has_emit = false;
+    out.sss_rays = 0;
     out.add_direct = (ps.flags & kFlagFirst) != 0;
     int prim = f_as_i(hit.w);
     const bool first = (ps.flags & kFlagFirst) != 0;
@@ -325,30 +408,47 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
             out.alive = true;
             return out;
         }
-        // Burley radius (material.cu:436-456), angle, probe (material.cu:415-424).
-        float dist = 0.f;
-        int ch = (int)(rng.next() * 3.f);
-        ch = clampi(0, 2, ch);
-        float dch = ch == 0 ? mat.scatter.x : (ch == 1 ? mat.scatter.y : mat.scatter.z);
-        if (!(dch < 1e-6f)) {
-            float r1 = rng.next();
-            if (r1 < 0.5f) {
-                r1 = r1 * 2.f;
-                dist = -dch * fm::logf_(1.f - r1);
-            } else {
-                r1 = (r1 - 0.5f) * 2.f;
-                dist = (-3.f * dch) * fm::logf_(1.f - r1);
-            }
-        }
-        float angle = (float)((double)(rng.next() * 2.f) * kPiD);
         vec3 out_p, out_n;
-        sss_probe(sc, hit_p, geom_n, dist, angle, out_p, out_n, stk, stride);
-        float sd = length(out_p - hit_p);
-        float spatial_pdf = sd < 1e-6f ? 0.f : burley_pdf(mat, sd);
-        if ((double)spatial_pdf < kEpsD) return out;
-        vec3 profile = v3(0.f, 0.f, 0.f);
-        if (!(sd < 1e-6f)) profile = cmul(mat.basecolor, v3(burley(mat, sd, 0), burley(mat, sd, 1), burley(mat, sd, 2))) / kPiF;
-        ps.w = cmul(cmul(ps.w, v3(1.f, 1.f, 1.f) - f_in), profile) / (transmit_prob * spatial_pdf);
+        const DevMedium md = load_medium(sc.materials, sh.material);
+        if (md.method == kSssRandomWalk) {
+            // Random walk through the medium (sss_random_walk above).
+            vec3 n_out = dot(geom_n, wo) < 0.f ? -geom_n : geom_n;
+            float e1 = rng.next();
+            float e2 = rng.next();
+            WalkExit ex = sss_random_walk(sc, md, hit_p, n_out, e1, e2, fp.seed, pixel, sample,
+                                          (uint32_t)fp.bounce, stk, stride);
+            out.sss_rays = ex.steps;
+            if (!ex.ok) return out;
+            out_p = ex.p;
+            out_n = ex.n;
+            ps.w = cmul(cmul(ps.w, v3(1.f, 1.f, 1.f) - f_in), ex.w) / transmit_prob;
+        } else {
+            // Burley radius (material.cu:436-456), angle, probe (material.cu:415-424).
+            float dist = 0.f;
+            int ch = (int)(rng.next() * 3.f);
+            ch = clampi(0, 2, ch);
+            float dch = ch == 0 ? mat.scatter.x : (ch == 1 ? mat.scatter.y : mat.scatter.z);
+            if (!(dch < 1e-6f)) {
+                float r1 = rng.next();
+                if (r1 < 0.5f) {
+                    r1 = r1 * 2.f;
+                    dist = -dch * fm::logf_(1.f - r1);
+                } else {
+                    r1 = (r1 - 0.5f) * 2.f;
+                    dist = (-3.f * dch) * fm::logf_(1.f - r1);
+                }
+            }
+            float angle = (float)((double)(rng.next() * 2.f) * kPiD);
+            sss_probe(sc, hit_p, geom_n, dist, angle, out_p, out_n, stk, stride);
+            out.sss_rays = 2;
+            float sd = length(out_p - hit_p);
+            float spatial_pdf = sd < 1e-6f ? 0.f : burley_pdf(mat, sd);
+            if ((double)spatial_pdf < kEpsD) return out;
+            vec3 profile = v3(0.f, 0.f, 0.f);
+            if (!(sd < 1e-6f))
+                profile = cmul(mat.basecolor, v3(burley(mat, sd, 0), burley(mat, sd, 1), burley(mat, sd, 2))) / kPiF;
+            ps.w = cmul(cmul(ps.w, v3(1.f, 1.f, 1.f) - f_in), profile) / (transmit_prob * spatial_pdf);
+        }
 
         int lid;
         float p_sel;

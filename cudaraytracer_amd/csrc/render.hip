@@ -412,7 +412,8 @@ __device__ __forceinline__ void shade_class(const JobDev* __restrict__ jobs, int
                                             float4* __restrict__ shadow, float4* slot_d, float4* slot_i,
                                             int region, int* counters, int qslot, uint16_t* __restrict__ nkey,
                                             const int* __restrict__ perm, const float4* __restrict__ sorted_o,
-                                            const float4* __restrict__ sorted_d, uint32_t* col) {
+                                            const float4* __restrict__ sorted_d, uint32_t* col,
+                                            unsigned long long* stats) {
     SliceCursor cur;
     cur.init(counters, kCntClass0 + CLS, kFetchClass0 + CLS);
     int base;
@@ -422,6 +423,7 @@ __device__ __forceinline__ void shade_class(const JobDev* __restrict__ jobs, int
         so.alive = false;
         so.shadow = false;
         so.add_direct = false;
+        so.sss_rays = 0;
         PathState ps;
         int pid = 0;
         uint32_t jobbits = 0;
@@ -474,6 +476,12 @@ __device__ __forceinline__ void shade_class(const JobDev* __restrict__ jobs, int
             }
             if (CLS != kClsTerminal && nkey && so.alive) key = ray_sort_key(ps.o, ps.d, J.sc);
         }
+        if (CLS == kClsSubsurface) {
+            // Rays traced by the subsurface events (probe rays, walk segments): one add per wave.
+            int n = so.sss_rays;
+            for (int off = kWave / 2; off > 0; off >>= 1) n += __shfl_xor(n, off);
+            if ((threadIdx.x & (kWave - 1)) == 0 && n) atomicAdd(&stats[kStatProbeRays], (unsigned long long)n);
+        }
         if (CLS != kClsTerminal) {
             int qpos = wave_append(so.alive, ctr_at(counters, qslot ^ 1, cur.slice));
             if (so.alive) {
@@ -513,14 +521,16 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
                                                    const PassHeader* __restrict__ ph, int* counters, int qslot,
                                                    uint16_t* __restrict__ nkey, const int* __restrict__ perm,
                                                    const float4* __restrict__ sorted_o,
-                                                   const float4* __restrict__ sorted_d) {
+                                                   const float4* __restrict__ sorted_d,
+                                                   unsigned long long* stats) {
     const int region = ph->region;
     uint32_t* col = stk_dyn + threadIdx.x;
     auto run = [&](auto cls) {
         constexpr int CLS = decltype(cls)::value;
         if (classes & (1 << CLS))
             shade_class<CLS, POOLED>(jobs, bounce, qo, qd, qw, hits, lists + (size_t)CLS * cap, no, nd, nw, shadow,
-                                     slot_d, slot_i, region, counters, qslot, nkey, perm, sorted_o, sorted_d, col);
+                                     slot_d, slot_i, region, counters, qslot, nkey, perm, sorted_o, sorted_d, col,
+                                     stats);
     };
     run(std::integral_constant<int, kClsSubsurface>());
     run(std::integral_constant<int, kClsDielectric>());
@@ -1684,7 +1694,7 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
             hits_.as<float4>(), lists_.as<int>(), cap_, qo_[q ^ 1].as<float4>(), qd_[q ^ 1].as<float4>(),
             qw_[q ^ 1].as<float4>(), shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, q,
             ray_sort_ ? keys_.as<uint16_t>() : nullptr, perm, perm ? sorted_o_.as<float4>() : nullptr,
-            perm ? sorted_d_.as<float4>() : nullptr);
+            perm ? sorted_d_.as<float4>() : nullptr, st);
         HIP_OK(hipGetLastError());
         prof.end(kStageShade, b, stream_);
         b = prof.begin(stream_);

@@ -294,7 +294,9 @@ RT_HD void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
 // the reference's sampler (curand_uniform excludes 0 and includes 1).
 RT_HD float u01(uint32_t x) { return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f); }
 
-enum RngStream : uint32_t { kStreamCamera = 0u, kStreamPath = 1u };
+// kStreamWalk: one coordinate per step of a random walk through a
+// subsurface medium, bounce field (bounce << 8) | step.
+enum RngStream : uint32_t { kStreamCamera = 0u, kStreamPath = 1u, kStreamWalk = 2u };
 
 // Sequential draws within one (pixel, sample, bounce, stream) coordinate.
 // Draw d uses Philox block d/4, lane d%4.

@@ -1186,6 +1186,20 @@ void load_scene(const std::string& json_path, HostScene& sc, int bvh_builder) {
             if (m.contains("ior")) r.ior = m["ior"].as_float("ior");
             if (m.contains("roughness")) r.roughness = m["roughness"].as_float("roughness");
             r.metallic = 0.0f;
+            // Random-walk mode (no counterpart in the reference's schema).
+            if (m.contains("subsurface_method")) {
+                std::string sm = m["subsurface_method"].as_string("subsurface_method");
+                if (sm == "random_walk") r.sss_method = kSssRandomWalk;
+                else if (sm == "burley") r.sss_method = kSssBurley;
+                else throw std::runtime_error("unsupported subsurface_method '" + sm + "' for material '" + name + "'");
+            }
+            set3(r.sigma_a, "sigma_a", 0.f, 0.f, 0.f);
+            set3(r.sigma_s, "sigma_s", 1.f, 1.f, 1.f);
+            for (int c = 0; c < 3; ++c)
+                if (!(r.sigma_a[c] >= 0.f) || !(r.sigma_s[c] >= 0.f) || !std::isfinite(r.sigma_a[c] + r.sigma_s[c]) ||
+                    (r.sss_method == kSssRandomWalk && !(r.sigma_a[c] + r.sigma_s[c] > 0.f)))
+                    throw std::runtime_error("sigma_a and sigma_s of material '" + name +
+                                             "' must be finite, >= 0 and sum to > 0 per channel");
         } else if (r.type == kDielectric) {
             set3(r.basecolor, "basecolor", 1.f, 1.f, 1.f);
             r.ior = m.contains("ior") ? m["ior"].as_float("ior") : 1.5f;

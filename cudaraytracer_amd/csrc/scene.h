@@ -14,9 +14,15 @@ namespace rt {
 // Material kinds in the reference's enum order (reference: Src/Core/material.h:6-12).
 enum MaterialType : int32_t { kLight = 0, kDiffuse = 1, kPbr = 2, kDielectric = 3, kSubsurface = 4 };
 enum LightType : int32_t { kAreaLight = 0, kPointLight = 1, kEnvLight = 2 };
+// Subsurface transport of a "subsurface" material: the reference's Burley
+// diffusion profile with two probe rays (material.cu:326-441), or a random walk
+// through the medium's absorption and scattering coefficients (this project's
+// addition; JSON "subsurface_method": "random_walk").
+enum SssMethod : int32_t { kSssBurley = 0, kSssRandomWalk = 1 };
 
-// Device material record (64 bytes).  Defaults are the reference's Material
-// member initialisers (reference: Src/Core/material.h:57-71).
+// Device material record (96 bytes).  Defaults are the reference's Material
+// member initialisers (reference: Src/Core/material.h:57-71); sigma_a, sigma_s
+// and sss_method are read by the random-walk subsurface mode only.
 struct MaterialRec {
     int32_t type = kDiffuse;
     float basecolor[3] = {0.f, 0.f, 0.f};
@@ -28,8 +34,12 @@ struct MaterialRec {
     float scatter[3] = {0.f, 0.f, 0.f};
     int32_t texture = -1;
     int32_t valid = 0;
+    float sigma_a[3] = {0.f, 0.f, 0.f};  // absorption coefficient per scene unit, RGB
+    float sigma_s[3] = {1.f, 1.f, 1.f};  // scattering coefficient per scene unit, RGB
+    int32_t sss_method = kSssBurley;
+    int32_t pad_ = 0;
 };
-static_assert(sizeof(MaterialRec) == 64, "material record is 64 bytes");
+static_assert(sizeof(MaterialRec) == 96, "material record is 96 bytes");
 
 // Device light record (32 bytes).
 struct LightRec {

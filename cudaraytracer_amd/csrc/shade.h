@@ -25,6 +25,21 @@ struct DevMaterial {
     int32_t texture, valid;
 };
 
+// The random-walk subsurface mode's coefficients, read only by that mode.
+struct DevMedium {
+    vec3 sigma_a, sigma_s;
+    int32_t method;
+};
+
+RT_HD DevMedium load_medium(const MaterialRec* mats, int id) {
+    Global<MaterialRec>& m = gref(mats + id);
+    DevMedium r;
+    r.sigma_a = v3(m.sigma_a[0], m.sigma_a[1], m.sigma_a[2]);
+    r.sigma_s = v3(m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]);
+    r.method = m.sss_method;
+    return r;
+}
+
 RT_HD DevMaterial load_material(const MaterialRec* mats, int id) {
     Global<MaterialRec>& m = gref(mats + id);
     DevMaterial r;

@@ -306,6 +306,27 @@ def _texture_image(h: int, w: int, rng) -> np.ndarray:
     return np.concatenate([rgb, rng.integers(0, 256, (h, w, 1))], axis=2).astype(np.uint8)
 
 
+def cornell_sss_walk(width: int = 96, height: int = 96, spp: int = 4, depth: int = 8) -> SceneSpec:
+    """The Cornell shell with two random-walk subsurface spheres of
+    chromatic media (optical radii from about 1.5 to 8), one of them resting
+    against the short box so that walks also leave through faces of other
+    meshes inside the sphere."""
+    V, F = icosphere(3)
+    meshes = cornell_shell()
+    mats = _base_materials()
+    mats.append({"name": "jade", "type": "subsurface", "basecolor": [1, 1, 1], "ior": 1.45, "roughness": 0.2,
+                 "subsurface_method": "random_walk", "sigma_a": [2.0, 0.3, 1.5], "sigma_s": [6.0, 14.0, 9.0]})
+    mats.append({"name": "milk", "type": "subsurface", "basecolor": [1, 1, 1], "ior": 1.3, "roughness": 0.0,
+                 "subsurface_method": "random_walk", "sigma_a": [0.05, 0.1, 0.4], "sigma_s": [20.0, 22.0, 26.0]})
+    r0, r1 = 0.3, 0.22
+    meshes.append(MeshSpec("jade_ball", "jade", (V[F] * r0 + np.array([0.45, r0 + 1e-3, -0.45])).astype(np.float32),
+                           V[F].astype(np.float32)))
+    meshes.append(MeshSpec("milk_ball", "milk", (V[F] * r1 + np.array([-0.45, 0.6 + r1 * 0.8, -0.3])).astype(np.float32),
+                           V[F].astype(np.float32)))
+    return SceneSpec(camera=_camera(width, height, spp, depth), materials=mats, meshes=meshes,
+                     lights=[{"name": "light0", "type": "area", "lightMeshName": "light", "material": "light"}])
+
+
 def textured_cornell(seed: int = 5, width: int = 96, height: int = 96, spp: int = 4, depth: int = 6) -> SceneSpec:
     """The Cornell shell with diffuse textures on every lobe that reads an albedo:
     a diffuse back wall whose coordinates run past [0, 1] (clamped), a GGX floor,
@@ -486,10 +507,14 @@ def burley_from_coefficients(sigma_a, sigma_s):
 
 def terrain_env(seed: int = 3, grid: int = 1001, size: float = 100.0, env_w: int = 2048, env_h: int = 1024,
                 width: int = 1920, height: int = 1080, spp: int = 32, depth: int = 16,
-                sphere_subdiv: int = 4, env_file: str = "sky.pfm") -> SceneSpec:
+                sphere_subdiv: int = 4, env_file: str = "sky.pfm", sss: str = "random_walk") -> SceneSpec:
     """Terrain + subsurface sphere under a procedural HDR sky.  env_file picks
     the map's format by extension: .pfm keeps FP32 texels, .hdr is the
-    reference's Radiance RGBE."""
+    reference's Radiance RGBE.  The sphere's medium has sigma_a, sigma_s drawn
+    log-uniform in [0.1, 10] per metre (scene unit) per channel; sss picks its
+    transport: "random_walk" through those coefficients, or "burley", the
+    reference's diffusion profile with parameters mapped from them
+    (burley_from_coefficients)."""
     rng = np.random.default_rng(seed)
     terrain = terrain_mesh(grid, size, height=size * 0.06, seed=seed)
     V, F = icosphere(sphere_subdiv)
@@ -502,10 +527,13 @@ def terrain_env(seed: int = 3, grid: int = 1001, size: float = 100.0, env_w: int
     sig_a = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 3))
     sig_s = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 3))
     albedo, dist = burley_from_coefficients(sig_a, sig_s)
+    if sss not in ("random_walk", "burley"):
+        raise ValueError(f"unknown subsurface method {sss!r}")
     mats = [
         {"name": "terrain", "type": "diffuse", "basecolor": [0.35, 0.3, 0.22]},
         {"name": "sss", "type": "subsurface", "basecolor": [float(x) for x in albedo],
-         "scatter_distance": [float(x) for x in dist], "ior": 1.4, "roughness": 0.3},
+         "scatter_distance": [float(x) for x in dist], "ior": 1.4, "roughness": 0.3,
+         "subsurface_method": sss, "sigma_a": [float(x) for x in sig_a], "sigma_s": [float(x) for x in sig_s]},
     ]
     spec = SceneSpec(camera=dict(_camera(width, height, spp, depth), direction=[0.35, 0.45, -1.0]),
                      materials=mats, meshes=[terrain, sphere],
@@ -608,6 +636,7 @@ WORKLOADS = {
     "cornell_small": cornell_small,
     "cornell_spheres": cornell_spheres,
     "textured_cornell": textured_cornell,
+    "cornell_sss_walk": cornell_sss_walk,
 }
 # Large scenes are generated on demand (the full terrain is 2M triangles).
 LARGE_WORKLOADS = {

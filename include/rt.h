@@ -385,11 +385,13 @@ int rt_scene_get_bvh(const rt_scene* s, int from_packed, void* nodes_out, int32_
    receive its size (0 x 0 for an entry without 'path' or 'name'); rgba may be NULL. */
 int rt_scene_get_texture(const rt_scene* s, int index, int* w, int* h, uint8_t* rgba);
 /* The scene's materials in "materials" order, as Scene::ParseMaterials fills them
-   (Src/Core/scene.cpp:254-395, defaults from Src/Core/material.h:57-71): 16 words per
+   (Src/Core/scene.cpp:254-395, defaults from Src/Core/material.h:57-71): 24 words per
    material = int32 type (0 light, 1 diffuse, 2 pbr, 3 dielectric, 4 subsurface), basecolor[3],
    emission[3], roughness, metallic, transmission, ior, scatter_distance[3], int32 texture
-   (-1: none), int32 valid (0: an entry the reference skips).  *n receives the count; out may
-   be NULL. */
+   (-1: none), int32 valid (0: an entry the reference skips), then this library's random-walk
+   subsurface keys: sigma_a[3], sigma_s[3] (per scene unit; defaults 0 and 1), int32 method
+   (0 Burley = the reference's, 1 random walk), one padding word.  *n receives the count; out
+   may be NULL. */
 int rt_scene_get_materials(const rt_scene* s, int* n, void* out);
 /* The scene's decoded environment map as HDRTexture::Load receives it (Src/Core/texture.cu:
    17-25: stbi_loadf's RGB, row 0 = the file's first scanline; a PFM map is read losslessly):

@@ -1078,15 +1078,15 @@ void ref_texture_albedo(const ref_scene* s, int tex, float u, float v, float* rg
 void ref_render(const ref_scene* s, const ref_params* p, int sample0, int spp, const int32_t* pixels, int npix,
                 float* acc_direct, float* acc_indirect, uint64_t* stats) {
     int i;
-    uint64_t ext = 0, sh = 0;
+    uint64_t ext = 0, sh = 0, sss = 0;
     if (!pixels) npix = s->W * s->H;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : ext, sh)
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : ext, sh, sss)
     for (i = 0; i < npix; ++i) {
         uint32_t px = pixels ? (uint32_t)pixels[i] : (uint32_t)i;
         int k;
         for (k = 0; k < spp; ++k) {
             float d[3], in[3];
-            uint64_t st[2] = {0, 0};
+            uint64_t st[3] = {0, 0, 0};
             path_radiance_f(s, p, px, (uint32_t)(sample0 + k), d, in, st);
             acc_direct[4 * (size_t)i] += d[0];
             acc_direct[4 * (size_t)i + 1] += d[1];
@@ -1096,11 +1096,13 @@ void ref_render(const ref_scene* s, const ref_params* p, int sample0, int spp, c
             acc_indirect[4 * (size_t)i + 2] += in[2];
             ext += st[0];
             sh += st[1];
+            sss += st[2];
         }
     }
     if (stats) {
         stats[0] += ext;
         stats[1] += sh;
+        stats[2] += sss;
     }
 }
 
@@ -1114,7 +1116,7 @@ void ref_render_f64(const ref_scene* s, const ref_params* p, int sample0, int sp
         int k;
         for (k = 0; k < spp; ++k) {
             double d[3], in[3];
-            uint64_t st[2] = {0, 0};
+            uint64_t st[3] = {0, 0, 0};
             path_radiance_d(s, p, px, (uint32_t)(sample0 + k), d, in, st);
             acc_direct[4 * (size_t)i] += d[0];
             acc_direct[4 * (size_t)i + 1] += d[1];
@@ -1184,6 +1186,34 @@ void ref_sss_probe(const ref_scene* s, const float* p, const float* n, float dis
     out_n[0] = on.x;
     out_n[1] = on.y;
     out_n[2] = on.z;
+}
+
+int ref_sss_walk(const ref_scene* s, int mat, const float* p, const float* n_out, float e1, float e2, uint32_t seed,
+                 uint32_t pixel, uint32_t sample, uint32_t bounce, float* out_p, float* out_n, float* out_w,
+                 int* steps) {
+    Mat_f m = load_mat_f(s->mats + REF_MAT_STRIDE * (size_t)mat);
+    Vec_f op = mk_f(0, 0, 0), on = mk_f(0, 0, 0), w = mk_f(0, 0, 0);
+    int ok;
+    *steps = 0;
+    ok = rw_walk_f(s, &m, from_f(v3p(p)), from_f(v3p(n_out)), e1, e2, seed, pixel, sample, bounce, &op, &on, &w, steps);
+    out_p[0] = op.x; out_p[1] = op.y; out_p[2] = op.z;
+    out_n[0] = on.x; out_n[1] = on.y; out_n[2] = on.z;
+    out_w[0] = w.x; out_w[1] = w.y; out_w[2] = w.z;
+    return ok;
+}
+
+/* n walks from the same entry point and normal, walk i with pixel i and the
+   entry uniforms e[2i], e[2i+1] (OpenMP over walks). */
+void ref_sss_walk_batch(const ref_scene* s, int mat, const float* p, const float* n_out, int n, const float* e,
+                        uint32_t seed, int32_t* ok, float* out_p, float* out_n, float* out_w, int32_t* steps) {
+    int i;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (i = 0; i < n; ++i) {
+        int st = 0;
+        ok[i] = ref_sss_walk(s, mat, p, n_out, e[2 * i], e[2 * i + 1], seed, (uint32_t)i, 0u, 0u, out_p + 3 * (size_t)i,
+                             out_n + 3 * (size_t)i, out_w + 3 * (size_t)i, &st);
+        steps[i] = st;
+    }
 }
 
 void ref_env_sample(const ref_scene* s, float rv, float ru, float* uv_out) {

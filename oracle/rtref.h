@@ -20,12 +20,14 @@
 extern "C" {
 #endif
 
-/* Material record: 16 floats. */
-#define REF_MAT_STRIDE 16
+/* Material record: 24 floats. */
+#define REF_MAT_STRIDE 24
 /* [0] type (0 light, 1 diffuse, 2 pbr, 3 dielectric, 4 subsurface)
    [1..3] basecolor  [4..6] emission  [7] roughness  [8] metallic
    [9] transmission  [10] ior  [11..13] scatter distance
-   [14] diffuse texture index + 1 (0 = none)  [15] unused */
+   [14] diffuse texture index + 1 (0 = none)  [15] unused
+   [16..18] sigma_a  [19..21] sigma_s  [22] subsurface method (0 Burley, the
+   reference's; 1 random walk, the library's addition)  [23] unused */
 
 typedef struct ref_scene_desc {
     int num_tris;
@@ -107,7 +109,8 @@ void ref_trace(const ref_scene* s, const float* rays8, int n, float* hits4, uint
 void ref_occluded(const ref_scene* s, const float* rays8, int n, int brute, int32_t* out, uint32_t* counts2);
 
 /* Accumulate samples [sample0, sample0+spp) for the listed pixels (all when pixels == NULL),
-   in sample order; acc_* are npix*4 floats.  stats[0..1] += extension / shadow rays.
+   in sample order; acc_* are npix*4 floats.  stats[0..2] += extension / shadow / subsurface
+   rays (subsurface: the two Burley probe rays per event, or the random walk's segments).
    Uses OpenMP threads over pixels. */
 void ref_render(const ref_scene* s, const ref_params* p, int sample0, int spp, const int32_t* pixels, int npix,
                 float* acc_direct, float* acc_indirect, uint64_t* stats);
@@ -149,6 +152,16 @@ float ref_burley(const float* mat, float r, int ch);
 /* SSS: exit distance from one uniform for scatter distance d; the two-sided probe
    from p (normal n) at (dist, angle) returning the exit point and its normal. */
 float ref_sss_dist(float scatter_distance, float u);
+/* Random-walk subsurface transport from surface point p (outward normal
+   n_out) into the medium of material `mat`, entry direction from the uniforms
+   (e1, e2), step numbers from (seed, pixel, sample, bounce).  Returns 1 and the
+   exit point, outward normal and per-channel weight when the walk leaves the
+   medium, 0 when it ends inside; *steps receives the segments traced. */
+int ref_sss_walk(const ref_scene* s, int mat, const float* p, const float* n_out, float e1, float e2, uint32_t seed,
+                 uint32_t pixel, uint32_t sample, uint32_t bounce, float* out_p, float* out_n, float* out_w,
+                 int* steps);
+void ref_sss_walk_batch(const ref_scene* s, int mat, const float* p, const float* n_out, int n, const float* e,
+                        uint32_t seed, int32_t* ok, float* out_p, float* out_n, float* out_w, int32_t* steps);
 void ref_sss_probe(const ref_scene* s, const float* p, const float* n, float dist, float angle, float* out_p,
                    float* out_n);
 void ref_env_tables(const ref_scene* s, float* row_cdf, float* marg_cdf);

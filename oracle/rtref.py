@@ -23,7 +23,7 @@ from .rtref_image import read_texture
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "librtref.so")
-MAT_STRIDE = 16
+MAT_STRIDE = 24
 MAT_TYPES = {"light": 0, "diffuse": 1, "pbr": 2, "dielectric": 3, "glass": 3, "subsurface": 4}
 
 
@@ -114,6 +114,12 @@ def lib():
         L.ref_sss_dist.argtypes = [ctypes.c_float, ctypes.c_float]
         L.ref_sss_dist.restype = ctypes.c_float
         L.ref_sss_probe.argtypes = [vp, vp, vp, ctypes.c_float, ctypes.c_float, vp, vp]
+        L.ref_sss_walk.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
+                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp,
+                                   ctypes.POINTER(ctypes.c_int)]
+        L.ref_sss_walk.restype = ctypes.c_int
+        L.ref_sss_walk_batch.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, vp, ctypes.c_uint32, vp, vp, vp,
+                                         vp, vp]
         L.ref_env_tables.argtypes = [vp, vp, vp]
         L.ref_env_alias_tables.argtypes = [vp, vp, vp]
         L.ref_env_sample_alias.argtypes = [vp, ctypes.c_float, ctypes.c_float, vp]
@@ -333,6 +339,7 @@ def parse_scene(json_path: str, geometry=None) -> SceneArrays:
     materials[:, 7] = 0.5
     materials[:, 9] = 1.0
     materials[:, 10] = 1.0
+    materials[:, 19:22] = 1.0  # sigma_s default
     # Textures (scene.cpp:618-644): every entry decoded in order; an entry
     # without 'path' or 'name' keeps an empty slot.
     textures, tex_index = [], {}
@@ -366,6 +373,13 @@ def parse_scene(json_path: str, geometry=None) -> SceneArrays:
             if "roughness" in m:
                 r[7] = m["roughness"]
             r[8] = 0
+            # The library's random-walk mode (absent from the reference's schema).
+            method = m.get("subsurface_method", "burley")
+            if method not in ("burley", "random_walk"):
+                raise ValueError(f"unsupported subsurface_method {method!r}")
+            r[22] = 1 if method == "random_walk" else 0
+            r[16:19] = m.get("sigma_a", [0, 0, 0])
+            r[19:22] = m.get("sigma_s", [1, 1, 1])
         elif t == 3:
             r[1:4] = m.get("basecolor", [1, 1, 1])
             r[10] = m.get("ior", 1.5)
@@ -545,11 +559,12 @@ class RefScene:
         return out.astype(bool), counts
 
     def render(self, spp: int, params: Params, sample0: int = 0, pixels: Optional[np.ndarray] = None):
-        """Returns (direct, indirect) sums, (npix, 4) float32, and [ext, shadow] ray counts."""
+        """Returns (direct, indirect) sums, (npix, 4) float32, and [ext, shadow, subsurface]
+        ray counts (subsurface: Burley probe rays and random-walk segments)."""
         npix = self.a.width * self.a.height if pixels is None else len(pixels)
         d = np.zeros((npix, 4), np.float32)
         i = np.zeros((npix, 4), np.float32)
-        st = np.zeros(2, np.uint64)
+        st = np.zeros(3, np.uint64)
         px = None if pixels is None else np.ascontiguousarray(pixels, np.int32)
         lib().ref_render(self._h, ctypes.byref(params), sample0, spp, _p(px) if px is not None else None, npix,
                          _p(d), _p(i), _p(st))
@@ -589,6 +604,31 @@ class RefScene:
         on = np.zeros(3, np.float32)
         lib().ref_sss_probe(self._h, _p(pp), _p(nn), dist, angle, _p(op), _p(on))
         return op, on
+
+    def sss_walk(self, mat: int, p, n_out, e1: float, e2: float, seed: int = 1, pixel: int = 0, sample: int = 0,
+                 bounce: int = 0):
+        """One random walk into material `mat`'s medium: (ok, exit point, exit normal,
+        per-channel weight, segments traced)."""
+        pp = np.ascontiguousarray(p, np.float32)
+        nn = np.ascontiguousarray(n_out, np.float32)
+        op, on, w = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(3, np.float32)
+        st = ctypes.c_int(0)
+        ok = lib().ref_sss_walk(self._h, mat, _p(pp), _p(nn), e1, e2, seed, pixel, sample, bounce, _p(op), _p(on),
+                                _p(w), ctypes.byref(st))
+        return bool(ok), op, on, w, st.value
+
+    def sss_walks(self, mat: int, p, n_out, e, seed: int = 1):
+        """len(e) walks from one entry point (walk i: pixel i, entry uniforms e[i]):
+        (ok, exit points, exit normals, weights, segments traced) as arrays."""
+        e = np.ascontiguousarray(e, np.float32).reshape(-1, 2)
+        n = e.shape[0]
+        pp = np.ascontiguousarray(p, np.float32)
+        nn = np.ascontiguousarray(n_out, np.float32)
+        ok, steps = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        op, on, w = (np.zeros((n, 3), np.float32) for _ in range(3))
+        lib().ref_sss_walk_batch(self._h, mat, _p(pp), _p(nn), n, _p(e), seed, _p(ok), _p(op), _p(on), _p(w),
+                                 _p(steps))
+        return ok.astype(bool), op, on, w, steps
 
     def env_sample(self, rv: float, ru: float):
         uv = np.zeros(2, np.float32)

@@ -28,6 +28,10 @@ def scene_dir(tmp_path_factory):
     paths["terrain_small"] = scenes.write_scene(
         scenes.terrain_env(grid=101, env_w=256, env_h=128, width=160, height=90, spp=4),
         str(d / "terrain_small"), "terrain_small")
+    # The same terrain with the reference's Burley subsurface profile.
+    paths["terrain_small_burley"] = scenes.write_scene(
+        scenes.terrain_env(grid=101, env_w=256, env_h=128, width=160, height=90, spp=4, sss="burley"),
+        str(d / "terrain_small_burley"), "terrain_small_burley")
     # Small spheres scene for the real-time frame loop (G-buffer + SVGF).
     paths["cornell_spheres_rt"] = scenes.write_scene(scenes.cornell_spheres(width=96, height=64, spp=1, depth=5),
                                                      str(d / "cornell_spheres_rt"), "cornell_spheres_rt")

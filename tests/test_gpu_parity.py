@@ -102,7 +102,8 @@ def _gpu_render(path, spp, **kw):
     return d, i, st, r
 
 
-@pytest.mark.parametrize("name,spp", [("cornell_small", 4), ("cornell_spheres_128", 4), ("terrain_small", 4)])
+@pytest.mark.parametrize("name,spp", [("cornell_small", 4), ("cornell_spheres_128", 4), ("terrain_small", 4),
+                                      ("terrain_small_burley", 4), ("cornell_sss_walk", 4)])
 def test_render_bit_exact_full_image(scene_dir, oracle_lib, name, spp):
     d, i, st, _ = _gpu_render(scene_dir[name], spp)
     a = oracle_lib.parse_scene(scene_dir[name])
@@ -112,6 +113,9 @@ def test_render_bit_exact_full_image(scene_dir, oracle_lib, name, spp):
     assert np.array_equal(_bits(i[:, :3]), _bits(ri[:, :3]))
     assert st.extension_rays == rst[0]
     assert st.shadow_rays == rst[1]
+    assert st.probe_rays == rst[2]  # Burley probe rays / random-walk segments
+    if name == "cornell_sss_walk":
+        assert rst[2] > 10 * 4 * 96 * 96 // 100  # the walks are exercised
 
 
 def test_render_reference_semantics_variants(scene_dir, oracle_lib):
