@@ -165,14 +165,13 @@ __device__ __forceinline__ void sss_probe(const DevScene& sc, vec3 hit_p, vec3 n
 // segment to the first surface of any mesh, both faces counting (the walk sees
 // the boundary from inside), picks one colour channel uniformly and draws a
 // free-flight distance s = -ln(u) / sigma_t of that channel; the pdf is the
-// mean over the channels (one-sample MIS over wavelengths).  Past the surface
-// the walk leaves there with weight T / mean(T), T = exp(-sigma_t t) per
+// mean over the channels (one-sample MIS over wavelengths).  A surface closer
+// than s ends the walk there with weight T / mean(T), T = exp(-sigma_t t) per
 // channel; otherwise it scatters at s with weight sigma_s T / mean(sigma_t T),
 // turns to a uniformly distributed direction and plays Russian roulette on its
-// largest channel.  Absorbed, escaped (no surface ahead: an open mesh) and
-// over-long walks end the path.  Step k draws from its own coordinate
-// (bounce << 8 | k, kStreamWalk): channel, distance, then two direction
-// numbers and the roulette number when it scatters.
+// largest channel.  Absorbed and over-long walks end the path.  Step k draws
+// from its own coordinate (bounce << 8 | k, kStreamWalk): channel, distance,
+// then two direction numbers and the roulette number when it scatters.
 constexpr int kWalkMaxSteps = 256;
 
 struct WalkExit {
@@ -199,15 +198,16 @@ __device__ __forceinline__ WalkExit sss_random_walk(const DevScene& sc, const De
     for (int k = 0; k < kWalkMaxSteps; ++k) {
         Rng g;
         g.init(seed, pixel, sample, (bounce << 8) | (uint32_t)k, kStreamWalk);
-        RayInv r = make_ray(p, d);
-        TravCount c{0, 0};
-        Hit h = trace_closest<false, false>(sc, r, 0.f, 1e8f, stk, stride, c);
-        ex.steps = k + 1;
         int ch = clampi(0, 2, (int)(g.next() * 3.f));
         float stc = ch == 0 ? st.x : (ch == 1 ? st.y : st.z);
         float s = -fm::logf_(g.next()) / stc;
-        if (h.prim < 0) return ex;
-        if (s >= h.t) {
+        // Only a surface nearer than the sampled distance matters, so the
+        // segment query stops at s and skips every box entered beyond it.
+        RayInv r = make_ray(p, d);
+        TravCount c{0, 0};
+        Hit h = trace_closest<false, false>(sc, r, 0.f, fminf(s, 1e8f), stk, stride, c);
+        ex.steps = k + 1;
+        if (h.prim >= 0) {
             vec3 T = v3(fm::expf_(-st.x * h.t), fm::expf_(-st.y * h.t), fm::expf_(-st.z * h.t));
             float pdf = ((T.x + T.y) + T.z) / 3.f;
             if (!(pdf > 0.f)) return ex;
