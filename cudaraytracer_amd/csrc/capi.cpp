@@ -237,6 +237,7 @@ int rt_scene_refit(rt_scene* s, const float* pos, const float* nrm, int64_t num_
     return guard([&]() {
         auto t0 = std::chrono::steady_clock::now();
         rt::set_scene_geometry(s->host, pos, nrm);
+        rt::refit_media(s->host);
         auto t1 = std::chrono::steady_clock::now();
         s->refit_phase_ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
         if (s->gpu) {

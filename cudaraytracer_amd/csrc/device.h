@@ -52,7 +52,16 @@ struct DevScene {
     // top), and per texture (first texel, width, height, 0).
     const uint32_t* tex_texels;
     const int32_t* tex_info;
+    // Random-walk media: per mesh the root reference of its own tree
+    // (kNoMedium for other meshes), the trees' nodes and triangle records
+    // (all media back to back, references absolute), and per medium triangle
+    // its position in the scene's BVH order (for its attribute line).
+    const uint32_t* media_root;
+    const PackedNode* media_nodes;
+    const float4* media_isect;
+    const int32_t* media_prim;
 };
+constexpr uint32_t kNoMedium = 0xffffffffu;
 
 // tri_attr layout, 8 float4 = one 128-byte line per triangle:
 //   [0] v0.xyz, index within its mesh  [1] v1.xyz, gn.x

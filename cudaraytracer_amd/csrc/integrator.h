@@ -161,10 +161,11 @@ __device__ __forceinline__ void sss_probe(const DevScene& sc, vec3 hit_p, vec3 n
 // Kutz and Burley, "Practical and controllable subsurface scattering for
 // production path tracing", SIGGRAPH 2016 talk, with isotropic scattering).
 // The path enters the medium in a cosine-distributed direction about the
-// inward normal (diffuse transmission) and walks.  Each step traces the
-// segment to the first surface of any mesh, both faces counting (the walk sees
-// the boundary from inside), picks one colour channel uniformly and draws a
-// free-flight distance s = -ln(u) / sigma_t of that channel; the pdf is the
+// inward normal (diffuse transmission) and walks.  Each step picks one colour
+// channel uniformly, draws a free-flight distance s = -ln(u) / sigma_t of that
+// channel and looks for the medium's own boundary along the segment to s: a
+// closest-hit query over the medium's tree (its mesh's triangles only), both
+// faces counting (the walk sees the boundary from inside); the pdf is the
 // mean over the channels (one-sample MIS over wavelengths).  A surface closer
 // than s ends the walk there with weight T / mean(T), T = exp(-sigma_t t) per
 // channel; otherwise it scatters at s with weight sigma_s T / mean(sigma_t T),
@@ -182,9 +183,10 @@ struct WalkExit {
     int steps;  // segments traced
 };
 
-__device__ __forceinline__ WalkExit sss_random_walk(const DevScene& sc, const DevMedium& md, vec3 hit_p,
-                                                   vec3 n_out, float e1, float e2, uint32_t seed, uint32_t pixel,
-                                                   uint32_t sample, uint32_t bounce, uint32_t* stk, int stride) {
+__device__ __forceinline__ WalkExit sss_random_walk(const DevScene& sc, const DevMedium& md, uint32_t root,
+                                                   vec3 hit_p, vec3 n_out, float e1, float e2, uint32_t seed,
+                                                   uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t* stk,
+                                                   int stride) {
     WalkExit ex;
     ex.ok = false;
     ex.p = hit_p;
@@ -205,13 +207,14 @@ __device__ __forceinline__ WalkExit sss_random_walk(const DevScene& sc, const De
         // segment query stops at s and skips every box entered beyond it.
         RayInv r = make_ray(p, d);
         TravCount c{0, 0};
-        Hit h = trace_closest<false, false>(sc, r, 0.f, fminf(s, 1e8f), stk, stride, c);
+        Hit h = trace_closest_tree<false, false>(sc.media_nodes, sc.media_isect, root, r, 0.f, fminf(s, 1e8f), stk,
+                                                 stride, c);
         ex.steps = k + 1;
         if (h.prim >= 0) {
             vec3 T = v3(fm::expf_(-st.x * h.t), fm::expf_(-st.y * h.t), fm::expf_(-st.z * h.t));
             float pdf = ((T.x + T.y) + T.z) / 3.f;
             if (!(pdf > 0.f)) return ex;
-            SurfaceHit q = surface_at(sc.tri_attr, h.prim, h.b1, h.b2);
+            SurfaceHit q = surface_at(sc.tri_attr, gref(sc.media_prim + h.prim), h.b1, h.b2);
             vec3 ng = normalize(q.ng);
             ex.p = q.p;
             ex.n = dot(ng, d) < 0.f ? -ng : ng;
@@ -411,11 +414,12 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
         vec3 out_p, out_n;
         const DevMedium md = load_medium(sc.materials, sh.material);
         if (md.method == kSssRandomWalk) {
-            // Random walk through the medium (sss_random_walk above).
+            // Random walk through the medium (sss_random_walk above), in the tree of the hit's mesh.
             vec3 n_out = dot(geom_n, wo) < 0.f ? -geom_n : geom_n;
             float e1 = rng.next();
             float e2 = rng.next();
-            WalkExit ex = sss_random_walk(sc, md, hit_p, n_out, e1, e2, fp.seed, pixel, sample,
+            const uint32_t root = gref(sc.media_root + sh.mesh);
+            WalkExit ex = sss_random_walk(sc, md, root, hit_p, n_out, e1, e2, fp.seed, pixel, sample,
                                           (uint32_t)fp.bounce, stk, stride);
             out.sss_rays = ex.steps;
             if (!ex.ok) return out;

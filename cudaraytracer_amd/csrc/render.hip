@@ -1052,6 +1052,7 @@ struct GpuScene::Impl {
         env_alias_marg, tex_texels, tex_info;
     DevBuf order;             // BVH slot -> scene triangle, kept for refits
     DevBuf parent, refit_cnt; // per interior node, allocated by the first refit
+    DevBuf media_root, media_nodes, media_isect, media_prim;  // random-walk media trees
     int num_interior = 0;     // packed nodes of the tree (0: the root is a leaf)
     DevScene dev{};
     int max_stack = 1;
@@ -1062,6 +1063,60 @@ struct GpuScene::Impl {
 static void upload(DevBuf& b, const void* src, size_t bytes) {
     b.alloc(bytes);
     if (bytes) HIP_OK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+}
+
+// The media trees packed back to back: node references made absolute (an
+// interior index plus the tree's first node, a leaf's first triangle plus the
+// tree's first triangle), triangle records in each tree's order with the
+// scene's FP32 edge subtraction, and each record's position in the scene's
+// BVH order.  One dummy entry per array when the scene has no medium.
+struct MediaTables {
+    std::vector<uint32_t> root;
+    std::vector<PackedNode> nodes;
+    std::vector<float4> isect;
+    std::vector<int32_t> prim;
+    int max_stack = 1;
+};
+
+static void media_tables(const HostScene& sc, MediaTables& t) {
+    t.root.assign(std::max<size_t>(sc.meshes.size(), 1), kNoMedium);
+    t.nodes.clear();
+    t.isect.clear();
+    t.prim.clear();
+    t.max_stack = 1;
+    std::vector<int32_t> slot;  // scene triangle -> position in the scene's BVH order
+    if (!sc.media.empty()) {
+        slot.resize(sc.bvh.order.size());
+        for (size_t k = 0; k < sc.bvh.order.size(); ++k) slot[sc.bvh.order[k]] = (int32_t)k;
+    }
+    for (const MediumTree& mt : sc.media) {
+        const BvhBuild& b = mt.bvh;
+        const MeshRec& mr = sc.meshes[mt.mesh];
+        const uint32_t node0 = (uint32_t)t.nodes.size(), tri0 = (uint32_t)(t.isect.size() / 3);
+        if ((size_t)tri0 + b.order.size() >= ((size_t)1 << 28)) throw std::runtime_error("media exceed 2^28 triangles");
+        auto shift = [&](uint32_t ref) {
+            if (ref & kLeafBit) return leaf_ref(((ref & ~kLeafBit) >> 3) + tri0, ref & 7u);
+            return ref + node0;
+        };
+        for (PackedNode n : b.packed) {
+            n.w3.x = (int)shift((uint32_t)n.w3.x);
+            n.w3.y = (int)shift((uint32_t)n.w3.y);
+            t.nodes.push_back(n);
+        }
+        for (int32_t local : b.order) {
+            const int tri = mr.start + local;
+            const float* p = sc.pos.data() + 9 * (size_t)tri;
+            t.isect.push_back(make_float4(p[0], p[1], p[2], 0.f));
+            t.isect.push_back(make_float4(p[3] - p[0], p[4] - p[1], p[5] - p[2], 0.f));
+            t.isect.push_back(make_float4(p[6] - p[0], p[7] - p[1], p[8] - p[2], 0.f));
+            t.prim.push_back(slot[tri]);
+        }
+        t.root[mt.mesh] = shift(b.root_ref);
+        t.max_stack = std::max(t.max_stack, b.max_stack);
+    }
+    if (t.nodes.empty()) t.nodes.resize(1);
+    if (t.isect.empty()) t.isect.resize(3);
+    if (t.prim.empty()) t.prim.resize(1);
 }
 
 // Area-light records and their triangles (v0, v1, v2, geometric normal) in
@@ -1135,6 +1190,14 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
     std::vector<MaterialRec> mats = sc.materials;
     if (mats.empty()) mats.resize(1);
     upload(I.materials, mats.data(), mats.size() * sizeof(MaterialRec));
+    MediaTables media;
+    media_tables(sc, media);
+    if (media.max_stack > 64) throw std::runtime_error("medium tree deeper than the 64-entry traversal stack");
+    I.max_stack = std::max(I.max_stack, media.max_stack);
+    upload(I.media_root, media.root.data(), media.root.size() * sizeof(uint32_t));
+    upload(I.media_nodes, media.nodes.data(), media.nodes.size() * sizeof(PackedNode));
+    upload(I.media_isect, media.isect.data(), media.isect.size() * sizeof(float4));
+    upload(I.media_prim, media.prim.data(), media.prim.size() * sizeof(int32_t));
     std::vector<LightRec> lights;
     std::vector<float4> ltris;
     light_tables(sc, lights, ltris);
@@ -1147,6 +1210,10 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
     d.tri_isect = I.tri_isect.as<float4>();
     d.tri_attr = I.tri_attr.as<float4>();
     d.materials = I.materials.as<MaterialRec>();
+    d.media_root = I.media_root.as<uint32_t>();
+    d.media_nodes = I.media_nodes.as<PackedNode>();
+    d.media_isect = I.media_isect.as<float4>();
+    d.media_prim = I.media_prim.as<int32_t>();
     d.lights = I.lights.as<LightRec>();
     d.num_lights = (int)sc.lights.size();
     d.env_light = sc.env_light;
@@ -1265,6 +1332,16 @@ void GpuScene::refit(HostScene& host, const float* nrm, double phase_ms[2]) {
     host.bvh.bounds = root;
     I.dev.bounds_lo = root.lo;
     I.dev.bounds_hi = root.hi;
+    // Media trees: refit on the host (the caller refit them), rewritten in place.
+    if (!host.media.empty()) {
+        MediaTables media;
+        media_tables(host, media);
+        if (media.nodes.size() * sizeof(PackedNode) != I.media_nodes.bytes ||
+            media.isect.size() * sizeof(float4) != I.media_isect.bytes)
+            throw std::runtime_error("refit: media trees changed size");
+        HIP_OK(hipMemcpy(I.media_nodes.p, media.nodes.data(), I.media_nodes.bytes, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(I.media_isect.p, media.isect.data(), I.media_isect.bytes, hipMemcpyHostToDevice));
+    }
     phase_ms[1] = lap();
 }
 
@@ -1285,7 +1362,7 @@ size_t GpuScene::device_bytes() const {
     return I.nodes.bytes + I.tri_isect.bytes + I.tri_attr.bytes + I.materials.bytes + I.lights.bytes +
            I.light_tris.bytes + I.env_rgb.bytes + I.env_row.bytes + I.env_marg.bytes + I.env_alias_row.bytes +
            I.env_alias_marg.bytes + I.tex_texels.bytes + I.tex_info.bytes + I.order.bytes + I.parent.bytes +
-           I.refit_cnt.bytes;
+           I.refit_cnt.bytes + I.media_root.bytes + I.media_nodes.bytes + I.media_isect.bytes + I.media_prim.bytes;
 }
 
 // Row-conditional and marginal CDFs weighted by sin(theta) (reference:

@@ -1031,6 +1031,27 @@ void set_scene_geometry(HostScene& sc, const float* pos, const float* nrm) {
         if (l.type == kAreaLight && l.mesh >= 0) l.area = sc.meshes[l.mesh].area;
 }
 
+void build_media(HostScene& sc) {
+    sc.media.clear();
+    for (size_t m = 0; m < sc.meshes.size(); ++m) {
+        const MeshRec& mr = sc.meshes[m];
+        if (mr.count <= 0 || mr.material < 0) continue;
+        const MaterialRec& mat = sc.materials[mr.material];
+        if (mat.type != kSubsurface || mat.sss_method != kSssRandomWalk) continue;
+        MediumTree t;
+        t.mesh = (int)m;
+        build_bvh(sc.pos.data() + 9 * (size_t)mr.start, mr.count, t.bvh);
+        sc.media.push_back(std::move(t));
+    }
+}
+
+void refit_media(HostScene& sc) {
+    for (MediumTree& t : sc.media) {
+        const MeshRec& mr = sc.meshes[t.mesh];
+        refit_bvh(sc.pos.data() + 9 * (size_t)mr.start, mr.count, t.bvh);
+    }
+}
+
 void compute_camera(CameraRec& c) {
     // Camera::ComputeCameraParam (reference: Src/Core/camera.h:58-84).
     float theta = deg2rad(c.theta), phi = deg2rad(c.phi);
@@ -1377,6 +1398,7 @@ void load_scene(const std::string& json_path, HostScene& sc, int bvh_builder) {
     else build_bvh(sc.pos.data(), sc.num_tris(), sc.bvh);
     sc.bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     sc.bvh_on_gpu = gpu;
+    build_media(sc);
     setup_camera(sc);
 }
 

@@ -75,6 +75,15 @@ struct EnvMap {
     std::vector<float> rgb;  // width*height*3, row 0 = top
 };
 
+// The random walk's own tree: a BVH over the triangles of one mesh whose
+// material is a random-walk subsurface medium (built like the scene's, over
+// that mesh's triangles in scene order), so the walk's segment queries see
+// only the medium's own boundary.
+struct MediumTree {
+    int mesh = -1;
+    BvhBuild bvh;
+};
+
 struct HostScene {
     std::string path;
     // Scene-order triangle soup.
@@ -99,6 +108,7 @@ struct HostScene {
     uint32_t seed = 1234u;      // reference: Src/Core/scene.cpp:167
     int rr_start_depth = 0;     // 0: roulette from the first bounce, as the reference
     BvhBuild bvh;
+    std::vector<MediumTree> media;  // one per mesh of a random-walk subsurface material
     double bvh_build_ms = 0.0;  // wall time of the BVH build
     double load_phase_ms[3] = {0, 0, 0};  // OBJ import, triangle finalize, environment map
     bool bvh_on_gpu = false;
@@ -121,6 +131,10 @@ void finalize_triangles(HostScene& sc);
 // refit_bvh).  Throws std::invalid_argument for a non-finite coordinate,
 // before anything changes.
 void set_scene_geometry(HostScene& sc, const float* pos, const float* nrm);
+// Medium trees: built after the scene's BVH; refit (same topology, new boxes)
+// after set_scene_geometry.
+void build_media(HostScene& sc);
+void refit_media(HostScene& sc);
 // Camera parameters from the scene bounds (reference: Src/Core/scene.cpp:646-667, camera.h:58-84).
 void setup_camera(HostScene& sc);
 // Pinhole basis and pixel steps from the orbit parameters (target, distance, theta, phi).

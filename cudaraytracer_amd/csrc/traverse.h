@@ -109,19 +109,23 @@ using LdsU32 = __attribute__((address_space(3))) uint32_t;
 // few lanes that reached leaves while the others wait, which made the
 // triangle tests most of the instructions issued.
 constexpr int kEarlyExit = -2;
+// The tree is given by its packed nodes, its triangle records and its root
+// reference (the scene's, or a random-walk medium's own; trace_closest below
+// takes the scene's).
 template <bool COUNT, bool CULL = true, int OCT = -1, bool EARLY = false>
-__device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r, float tmin, float tmax,
-                                             uint32_t* stk_generic, int stride, TravCount& cnt,
-                                             float t_exit = 0.f) {
+__device__ __forceinline__ Hit trace_closest_tree(const PackedNode* tree_nodes, const float4* tree_tris,
+                                                  uint32_t root_ref, const RayInv& r, float tmin, float tmax,
+                                                  uint32_t* stk_generic, int stride, TravCount& cnt,
+                                                  float t_exit = 0.f) {
     LdsU32* stk = (LdsU32*)stk_generic;
-    GlobalV4* nodes = (GlobalV4*)sc.nodes;  // 4 vectors per node
-    GlobalV4* tris = (GlobalV4*)sc.tri_isect;
+    GlobalV4* nodes = (GlobalV4*)tree_nodes;  // 4 vectors per node
+    GlobalV4* tris = (GlobalV4*)tree_tris;
     Hit h;
     h.t = tmax;
     h.b1 = 0.f;
     h.b2 = 0.f;
     h.prim = -1;
-    uint32_t ref = sc.root_ref;
+    uint32_t ref = root_ref;
     // In a leaf: triangles tn .. te-1 are still to be tested.
     bool leaf = (ref & kLeafBit) != 0;
     uint32_t tn = (ref & ~kLeafBit) >> 3, te = tn + (ref & 7u);
@@ -176,6 +180,13 @@ __device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r
         te = tn + (next & 7u);
     }
     return h;
+}
+
+template <bool COUNT, bool CULL = true, int OCT = -1, bool EARLY = false>
+__device__ __forceinline__ Hit trace_closest(const DevScene& sc, const RayInv& r, float tmin, float tmax,
+                                             uint32_t* stk, int stride, TravCount& cnt, float t_exit = 0.f) {
+    return trace_closest_tree<COUNT, CULL, OCT, EARLY>(sc.nodes, sc.tri_isect, sc.root_ref, r, tmin, tmax, stk,
+                                                       stride, cnt, t_exit);
 }
 
 // Any-hit test of the open segment (tmin, tmax) along r: true at the first

@@ -320,6 +320,7 @@ struct ref_scene {
     int ntex;
     int32_t* tex_info;  /* (first texel, width, height) per texture */
     uint8_t* tex_rgba;
+    struct ref_scene** medium;  /* per mesh: the random-walk medium's own scene, or NULL */
 };
 
 static float smin(float a, float b) { return (b < a) ? b : a; } /* std::min */
@@ -831,6 +832,8 @@ static void setup_env(ref_scene* s) { /* texture.cu:38-80 */
     }
 }
 
+static void build_media(ref_scene* s);
+
 ref_scene* ref_scene_create(const ref_scene_desc* d) {
     ref_scene* s = (ref_scene*)calloc(1, sizeof(ref_scene));
     int i, k, n_ordered = 0;
@@ -931,7 +934,85 @@ ref_scene* ref_scene_create(const ref_scene_desc* d) {
     } else {
         s->env_light = -1;
     }
+    build_media(s);
     return s;
+}
+
+/* Random-walk media (the library's subsurface mode, no reference
+   counterpart): for each mesh of a random-walk subsurface material, a scene
+   of that mesh's own triangles in scene order, whose BVH the walk's segment
+   queries use.  It carries no materials, lights, textures or environment. */
+static int is_walk_medium(const ref_scene* s, int m) {
+    int mat = s->mesh_material[m];
+    const float* r;
+    if (mat < 0 || mat >= s->nmat || s->mesh_count[m] <= 0) return 0;
+    r = s->mats + REF_MAT_STRIDE * (size_t)mat;
+    return (int)r[0] == 4 && (int)r[22] == 1;
+}
+
+static void medium_arrays(const ref_scene* s, int m, float* pos, float* nrm, float* uv) {
+    int i, k, j = 0;
+    for (i = 0; i < s->nt; ++i) {
+        const Tri* t = &s->tris[i];
+        if (t->mesh != m) continue;
+        for (k = 0; k < 3; ++k) {
+            pos[9 * (size_t)j + 3 * k] = t->p[k].x;
+            pos[9 * (size_t)j + 3 * k + 1] = t->p[k].y;
+            pos[9 * (size_t)j + 3 * k + 2] = t->p[k].z;
+            nrm[9 * (size_t)j + 3 * k] = t->n[k].x;
+            nrm[9 * (size_t)j + 3 * k + 1] = t->n[k].y;
+            nrm[9 * (size_t)j + 3 * k + 2] = t->n[k].z;
+            if (uv) {
+                uv[6 * (size_t)j + 2 * k] = t->uv[k][0];
+                uv[6 * (size_t)j + 2 * k + 1] = t->uv[k][1];
+            }
+        }
+        ++j;
+    }
+}
+
+static void build_media(ref_scene* s) {
+    int m;
+    s->medium = (ref_scene**)calloc((size_t)(s->nm + 1), sizeof(ref_scene*));
+    for (m = 0; m < s->nm; ++m) {
+        ref_scene_desc d;
+        int n = s->mesh_count[m];
+        int32_t zero = 0, none = -1, no_light = 0;
+        float no_f = 0.f;
+        float *pos, *nrm, *uv;
+        int32_t* mesh;
+        if (!is_walk_medium(s, m)) continue;
+        pos = (float*)malloc(sizeof(float) * 9 * (size_t)n);
+        nrm = (float*)malloc(sizeof(float) * 9 * (size_t)n);
+        uv = (float*)malloc(sizeof(float) * 6 * (size_t)n);
+        mesh = (int32_t*)calloc((size_t)n, sizeof(int32_t));
+        medium_arrays(s, m, pos, nrm, uv);
+        memset(&d, 0, sizeof(d));
+        d.num_tris = n;
+        d.pos = pos;
+        d.nrm = nrm;
+        d.uv = uv;
+        d.mesh = mesh;
+        d.num_meshes = 1;
+        d.mesh_material = &none;
+        d.mesh_light = &none;
+        d.num_materials = 0;
+        d.materials = &no_f;
+        d.num_lights = 0;
+        d.light_type = &zero;
+        d.light_mesh = &none;
+        d.light_emission = &no_f;
+        (void)no_light;
+        d.width = 1;
+        d.height = 1;
+        d.fovy = 45.f;
+        d.direction[2] = -1.f;
+        s->medium[m] = ref_scene_create(&d);
+        free(pos);
+        free(nrm);
+        free(uv);
+        free(mesh);
+    }
 }
 
 /* Refit (the reference has none; it reconstructs the Scene, scene.cpp:50-57):
@@ -985,10 +1066,26 @@ void ref_scene_refit(ref_scene* s, const float* pos, const float* nrm) {
         memcpy(s->bounds.lo, s->nodes[0].lo, 12);
         memcpy(s->bounds.hi, s->nodes[0].hi, 12);
     }
+    /* Media trees: refit over their unchanged topology, as the scene's. */
+    for (i = 0; i < s->nm; ++i) {
+        float *mp, *mn;
+        if (!s->medium || !s->medium[i]) continue;
+        mp = (float*)malloc(sizeof(float) * 9 * (size_t)s->mesh_count[i]);
+        mn = (float*)malloc(sizeof(float) * 9 * (size_t)s->mesh_count[i]);
+        medium_arrays(s, i, mp, mn, NULL);
+        ref_scene_refit(s->medium[i], mp, mn);
+        free(mp);
+        free(mn);
+    }
 }
 
 void ref_scene_destroy(ref_scene* s) {
+    int m;
     if (!s) return;
+    if (s->medium) {
+        for (m = 0; m < s->nm; ++m) ref_scene_destroy(s->medium[m]);
+        free(s->medium);
+    }
     free(s->tris); free(s->ordered); free(s->order); free(s->nodes);
     free(s->mesh_material); free(s->mesh_light); free(s->mesh_area); free(s->mesh_start); free(s->mesh_count);
     free(s->mats); free(s->light_type); free(s->light_mesh); free(s->light_em);
@@ -1188,14 +1285,17 @@ void ref_sss_probe(const ref_scene* s, const float* p, const float* n, float dis
     out_n[2] = on.z;
 }
 
-int ref_sss_walk(const ref_scene* s, int mat, const float* p, const float* n_out, float e1, float e2, uint32_t seed,
+int ref_sss_walk(const ref_scene* s, int mesh, const float* p, const float* n_out, float e1, float e2, uint32_t seed,
                  uint32_t pixel, uint32_t sample, uint32_t bounce, float* out_p, float* out_n, float* out_w,
                  int* steps) {
-    Mat_f m = load_mat_f(s->mats + REF_MAT_STRIDE * (size_t)mat);
+    Mat_f m;
     Vec_f op = mk_f(0, 0, 0), on = mk_f(0, 0, 0), w = mk_f(0, 0, 0);
     int ok;
     *steps = 0;
-    ok = rw_walk_f(s, &m, from_f(v3p(p)), from_f(v3p(n_out)), e1, e2, seed, pixel, sample, bounce, &op, &on, &w, steps);
+    if (mesh < 0 || mesh >= s->nm || !s->medium[mesh]) return 0;
+    m = load_mat_f(s->mats + REF_MAT_STRIDE * (size_t)s->mesh_material[mesh]);
+    ok = rw_walk_f(s->medium[mesh], &m, from_f(v3p(p)), from_f(v3p(n_out)), e1, e2, seed, pixel, sample, bounce, &op,
+                   &on, &w, steps);
     out_p[0] = op.x; out_p[1] = op.y; out_p[2] = op.z;
     out_n[0] = on.x; out_n[1] = on.y; out_n[2] = on.z;
     out_w[0] = w.x; out_w[1] = w.y; out_w[2] = w.z;
@@ -1204,13 +1304,13 @@ int ref_sss_walk(const ref_scene* s, int mat, const float* p, const float* n_out
 
 /* n walks from the same entry point and normal, walk i with pixel i and the
    entry uniforms e[2i], e[2i+1] (OpenMP over walks). */
-void ref_sss_walk_batch(const ref_scene* s, int mat, const float* p, const float* n_out, int n, const float* e,
+void ref_sss_walk_batch(const ref_scene* s, int mesh, const float* p, const float* n_out, int n, const float* e,
                         uint32_t seed, int32_t* ok, float* out_p, float* out_n, float* out_w, int32_t* steps) {
     int i;
 #pragma omp parallel for schedule(dynamic, 256)
     for (i = 0; i < n; ++i) {
         int st = 0;
-        ok[i] = ref_sss_walk(s, mat, p, n_out, e[2 * i], e[2 * i + 1], seed, (uint32_t)i, 0u, 0u, out_p + 3 * (size_t)i,
+        ok[i] = ref_sss_walk(s, mesh, p, n_out, e[2 * i], e[2 * i + 1], seed, (uint32_t)i, 0u, 0u, out_p + 3 * (size_t)i,
                              out_n + 3 * (size_t)i, out_w + 3 * (size_t)i, &st);
         steps[i] = st;
     }

@@ -153,14 +153,15 @@ float ref_burley(const float* mat, float r, int ch);
    from p (normal n) at (dist, angle) returning the exit point and its normal. */
 float ref_sss_dist(float scatter_distance, float u);
 /* Random-walk subsurface transport from surface point p (outward normal
-   n_out) into the medium of material `mat`, entry direction from the uniforms
-   (e1, e2), step numbers from (seed, pixel, sample, bounce).  Returns 1 and the
-   exit point, outward normal and per-channel weight when the walk leaves the
-   medium, 0 when it ends inside; *steps receives the segments traced. */
-int ref_sss_walk(const ref_scene* s, int mat, const float* p, const float* n_out, float e1, float e2, uint32_t seed,
+   n_out) into mesh `mesh`, a random-walk medium, entry direction from the
+   uniforms (e1, e2), step numbers from (seed, pixel, sample, bounce).  Returns
+   1 and the exit point, outward normal and per-channel weight when the walk
+   leaves the medium, 0 when it ends inside (or the mesh is no such medium);
+   *steps receives the segments traced. */
+int ref_sss_walk(const ref_scene* s, int mesh, const float* p, const float* n_out, float e1, float e2, uint32_t seed,
                  uint32_t pixel, uint32_t sample, uint32_t bounce, float* out_p, float* out_n, float* out_w,
                  int* steps);
-void ref_sss_walk_batch(const ref_scene* s, int mat, const float* p, const float* n_out, int n, const float* e,
+void ref_sss_walk_batch(const ref_scene* s, int mesh, const float* p, const float* n_out, int n, const float* e,
                         uint32_t seed, int32_t* ok, float* out_p, float* out_n, float* out_w, int32_t* steps);
 void ref_sss_probe(const ref_scene* s, const float* p, const float* n, float dist, float angle, float* out_p,
                    float* out_n);

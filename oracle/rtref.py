@@ -605,19 +605,19 @@ class RefScene:
         lib().ref_sss_probe(self._h, _p(pp), _p(nn), dist, angle, _p(op), _p(on))
         return op, on
 
-    def sss_walk(self, mat: int, p, n_out, e1: float, e2: float, seed: int = 1, pixel: int = 0, sample: int = 0,
+    def sss_walk(self, mesh: int, p, n_out, e1: float, e2: float, seed: int = 1, pixel: int = 0, sample: int = 0,
                  bounce: int = 0):
-        """One random walk into material `mat`'s medium: (ok, exit point, exit normal,
-        per-channel weight, segments traced)."""
+        """One random walk into mesh `mesh` (a random-walk medium): (ok, exit point,
+        exit normal, per-channel weight, segments traced)."""
         pp = np.ascontiguousarray(p, np.float32)
         nn = np.ascontiguousarray(n_out, np.float32)
         op, on, w = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(3, np.float32)
         st = ctypes.c_int(0)
-        ok = lib().ref_sss_walk(self._h, mat, _p(pp), _p(nn), e1, e2, seed, pixel, sample, bounce, _p(op), _p(on),
+        ok = lib().ref_sss_walk(self._h, mesh, _p(pp), _p(nn), e1, e2, seed, pixel, sample, bounce, _p(op), _p(on),
                                 _p(w), ctypes.byref(st))
         return bool(ok), op, on, w, st.value
 
-    def sss_walks(self, mat: int, p, n_out, e, seed: int = 1):
+    def sss_walks(self, mesh: int, p, n_out, e, seed: int = 1):
         """len(e) walks from one entry point (walk i: pixel i, entry uniforms e[i]):
         (ok, exit points, exit normals, weights, segments traced) as arrays."""
         e = np.ascontiguousarray(e, np.float32).reshape(-1, 2)
@@ -626,7 +626,7 @@ class RefScene:
         nn = np.ascontiguousarray(n_out, np.float32)
         ok, steps = np.zeros(n, np.int32), np.zeros(n, np.int32)
         op, on, w = (np.zeros((n, 3), np.float32) for _ in range(3))
-        lib().ref_sss_walk_batch(self._h, mat, _p(pp), _p(nn), n, _p(e), seed, _p(ok), _p(op), _p(on), _p(w),
+        lib().ref_sss_walk_batch(self._h, mesh, _p(pp), _p(nn), n, _p(e), seed, _p(ok), _p(op), _p(on), _p(w),
                                  _p(steps))
         return ok.astype(bool), op, on, w, steps
 
