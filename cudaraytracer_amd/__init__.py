@@ -72,7 +72,7 @@ EXPORTS = (
     "rt_renderer_create", "rt_renderer_destroy", "rt_render", "rt_render_counted", "rt_synchronize",
     "rt_reset", "rt_reset_stats", "rt_read_accum", "rt_read_rgba8", "rt_stats", "rt_last_error",
     "rt_scene_refit", "rt_scene_get_triangles", "rt_scene_get_meshes", "rt_scene_get_bvh", "rt_scene_get_texture",
-    "rt_scene_get_envmap", "rt_scene_get_materials", "rt_trace_rays_noalloc",
+    "rt_scene_get_envmap", "rt_scene_get_materials", "rt_scene_get_medium", "rt_trace_rays_noalloc",
     "rt_texture_lookup", "rt_trace_rays", "rt_device_math", "rt_save_image", "rt_read_gather", "rt_read_denoised", "rt_read_gbuffer",
     "rt_denoise_params_default", "rt_get_camera", "rt_set_camera", "rt_render_frame", "rt_reset_history",
     "rt_read_frame", "rt_pool_create", "rt_pool_destroy", "rt_pool_add_scene", "rt_pool_submit", "rt_pool_step",
@@ -242,6 +242,7 @@ def lib() -> ctypes.CDLL:
                                        ctypes.POINTER(ctypes.c_uint8)]
     L.rt_texture_lookup.argtypes = [vp, ip, fp, ip, fp]
     L.rt_scene_get_materials.argtypes = [vp, ctypes.POINTER(ctypes.c_int), vp]
+    L.rt_scene_get_medium.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), vp, vp, vp]
     L.rt_scene_get_envmap.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), fp]
     L.rt_save_image.argtypes = [vp, ip, ctypes.c_char_p]
     L.rt_read_gather.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_uint8)]
@@ -386,6 +387,19 @@ class Scene:
         out = np.zeros((n.value, 24), np.float32)
         _check(lib().rt_scene_get_materials(self._h, ctypes.byref(n), out.ctypes.data_as(ctypes.c_void_p)))
         return out
+
+    def medium(self, mesh: int):
+        """Clearance grid of a random-walk medium: (lo, inv, cells (n, n, n) indexed
+        [z, y, x]) or None when the mesh is no such medium."""
+        n = ctypes.c_int(0)
+        _check(lib().rt_scene_get_medium(self._h, mesh, ctypes.byref(n), None, None, None))
+        if n.value == 0:
+            return None
+        lo, inv = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        cells = np.zeros((n.value,) * 3, np.float32)
+        _check(lib().rt_scene_get_medium(self._h, mesh, ctypes.byref(n), lo.ctypes.data_as(ctypes.c_void_p),
+                                         inv.ctypes.data_as(ctypes.c_void_p), cells.ctypes.data_as(ctypes.c_void_p)))
+        return lo, inv, cells
 
     def envmap(self) -> np.ndarray:
         """Decoded environment map: (h, w, 3) float32, row 0 = the file's first scanline

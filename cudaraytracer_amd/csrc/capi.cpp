@@ -793,6 +793,19 @@ int rt_scene_get_materials(const rt_scene* s, int* n, void* out) {
     return RT_OK;
 }
 
+int rt_scene_get_medium(const rt_scene* s, int mesh, int* n, float* lo, float* inv, float* cells) {
+    if (!s || !n) return fail(RT_ERR_INVALID_ARG, "null argument");
+    *n = 0;
+    for (const rt::MediumTree& t : s->host.media) {
+        if (t.mesh != mesh) continue;
+        *n = t.grid_n;
+        if (lo) { lo[0] = t.grid_lo.x; lo[1] = t.grid_lo.y; lo[2] = t.grid_lo.z; }
+        if (inv) { inv[0] = t.grid_inv.x; inv[1] = t.grid_inv.y; inv[2] = t.grid_inv.z; }
+        if (cells) std::memcpy(cells, t.clearance.data(), t.clearance.size() * sizeof(float));
+    }
+    return RT_OK;
+}
+
 int rt_scene_get_envmap(const rt_scene* s, int* w, int* h, float* rgb) {
     if (!s || !w || !h) return fail(RT_ERR_INVALID_ARG, "null argument");
     const rt::EnvMap& e = s->host.env;

@@ -60,6 +60,10 @@ struct DevScene {
     const PackedNode* media_nodes;
     const float4* media_isect;
     const int32_t* media_prim;
+    // Per mesh two float4 of its medium's clearance grid: (lo.xyz, n as int
+    // bits), (inv cell size.xyz, first cell as int bits); and the cells.
+    const float4* media_grid;
+    const float* media_clear;
 };
 constexpr uint32_t kNoMedium = 0xffffffffu;
 

@@ -183,7 +183,18 @@ struct WalkExit {
     int steps;  // segments traced
 };
 
-__device__ __forceinline__ WalkExit sss_random_walk(const DevScene& sc, const DevMedium& md, uint32_t root,
+// Clearance of point p in its medium: the grid cell's lower bound of the
+// distance to the mesh, 0 outside the grid.
+__device__ __forceinline__ float medium_clearance(const DevScene& sc, int mesh, vec3 p) {
+    const float4 g0 = ldg4(sc.media_grid + 2 * mesh), g1 = ldg4(sc.media_grid + 2 * mesh + 1);
+    const int n = f_as_i(g0.w);
+    const int x = (int)floorf((p.x - g0.x) * g1.x), y = (int)floorf((p.y - g0.y) * g1.y),
+              z = (int)floorf((p.z - g0.z) * g1.z);
+    if (x < 0 || y < 0 || z < 0 || x >= n || y >= n || z >= n) return 0.f;
+    return gref(sc.media_clear + f_as_i(g1.w) + ((size_t)z * n + y) * n + x);
+}
+
+__device__ __forceinline__ WalkExit sss_random_walk(const DevScene& sc, const DevMedium& md, int mesh, uint32_t root,
                                                    vec3 hit_p, vec3 n_out, float e1, float e2, uint32_t seed,
                                                    uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t* stk,
                                                    int stride) {
@@ -204,11 +215,17 @@ __device__ __forceinline__ WalkExit sss_random_walk(const DevScene& sc, const De
         float stc = ch == 0 ? st.x : (ch == 1 ? st.y : st.z);
         float s = -fm::logf_(g.next()) / stc;
         // Only a surface nearer than the sampled distance matters, so the
-        // segment query stops at s and skips every box entered beyond it.
-        RayInv r = make_ray(p, d);
-        TravCount c{0, 0};
-        Hit h = trace_closest_tree<false, false>(sc.media_nodes, sc.media_isect, root, r, 0.f, fminf(s, 1e8f), stk,
+        // segment query stops at s and skips every box entered beyond it; a
+        // flight shorter than the point's clearance needs no query at all
+        // (no triangle lies within reach: the query would find none).
+        Hit h;
+        h.prim = -1;
+        if (!(s < medium_clearance(sc, mesh, p))) {
+            RayInv r = make_ray(p, d);
+            TravCount c{0, 0};
+            h = trace_closest_tree<false, false>(sc.media_nodes, sc.media_isect, root, r, 0.f, fminf(s, 1e8f), stk,
                                                  stride, c);
+        }
         ex.steps = k + 1;
         if (h.prim >= 0) {
             vec3 T = v3(fm::expf_(-st.x * h.t), fm::expf_(-st.y * h.t), fm::expf_(-st.z * h.t));
@@ -419,7 +436,7 @@ __device__ __forceinline__ ShadeOut shade_path(const DevScene& sc, const FramePa
             float e1 = rng.next();
             float e2 = rng.next();
             const uint32_t root = gref(sc.media_root + sh.mesh);
-            WalkExit ex = sss_random_walk(sc, md, root, hit_p, n_out, e1, e2, fp.seed, pixel, sample,
+            WalkExit ex = sss_random_walk(sc, md, sh.mesh, root, hit_p, n_out, e1, e2, fp.seed, pixel, sample,
                                           (uint32_t)fp.bounce, stk, stride);
             out.sss_rays = ex.steps;
             if (!ex.ok) return out;

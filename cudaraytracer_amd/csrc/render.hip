@@ -1052,7 +1052,7 @@ struct GpuScene::Impl {
         env_alias_marg, tex_texels, tex_info;
     DevBuf order;             // BVH slot -> scene triangle, kept for refits
     DevBuf parent, refit_cnt; // per interior node, allocated by the first refit
-    DevBuf media_root, media_nodes, media_isect, media_prim;  // random-walk media trees
+    DevBuf media_root, media_nodes, media_isect, media_prim, media_grid, media_clear;  // random-walk media
     int num_interior = 0;     // packed nodes of the tree (0: the root is a leaf)
     DevScene dev{};
     int max_stack = 1;
@@ -1075,14 +1075,18 @@ struct MediaTables {
     std::vector<PackedNode> nodes;
     std::vector<float4> isect;
     std::vector<int32_t> prim;
+    std::vector<float4> grid;   // 2 per mesh
+    std::vector<float> clear;
     int max_stack = 1;
 };
 
 static void media_tables(const HostScene& sc, MediaTables& t) {
     t.root.assign(std::max<size_t>(sc.meshes.size(), 1), kNoMedium);
+    t.grid.assign(2 * t.root.size(), make_float4(0.f, 0.f, 0.f, 0.f));
     t.nodes.clear();
     t.isect.clear();
     t.prim.clear();
+    t.clear.clear();
     t.max_stack = 1;
     std::vector<int32_t> slot;  // scene triangle -> position in the scene's BVH order
     if (!sc.media.empty()) {
@@ -1113,7 +1117,11 @@ static void media_tables(const HostScene& sc, MediaTables& t) {
         }
         t.root[mt.mesh] = shift(b.root_ref);
         t.max_stack = std::max(t.max_stack, b.max_stack);
+        t.grid[2 * mt.mesh] = make_float4(mt.grid_lo.x, mt.grid_lo.y, mt.grid_lo.z, i_as_f(mt.grid_n));
+        t.grid[2 * mt.mesh + 1] = make_float4(mt.grid_inv.x, mt.grid_inv.y, mt.grid_inv.z, i_as_f((int)t.clear.size()));
+        t.clear.insert(t.clear.end(), mt.clearance.begin(), mt.clearance.end());
     }
+    if (t.clear.empty()) t.clear.resize(1);
     if (t.nodes.empty()) t.nodes.resize(1);
     if (t.isect.empty()) t.isect.resize(3);
     if (t.prim.empty()) t.prim.resize(1);
@@ -1198,6 +1206,8 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
     upload(I.media_nodes, media.nodes.data(), media.nodes.size() * sizeof(PackedNode));
     upload(I.media_isect, media.isect.data(), media.isect.size() * sizeof(float4));
     upload(I.media_prim, media.prim.data(), media.prim.size() * sizeof(int32_t));
+    upload(I.media_grid, media.grid.data(), media.grid.size() * sizeof(float4));
+    upload(I.media_clear, media.clear.data(), media.clear.size() * sizeof(float));
     std::vector<LightRec> lights;
     std::vector<float4> ltris;
     light_tables(sc, lights, ltris);
@@ -1214,6 +1224,8 @@ GpuScene::GpuScene(const HostScene& sc) : impl_(new Impl) {
     d.media_nodes = I.media_nodes.as<PackedNode>();
     d.media_isect = I.media_isect.as<float4>();
     d.media_prim = I.media_prim.as<int32_t>();
+    d.media_grid = I.media_grid.as<float4>();
+    d.media_clear = I.media_clear.as<float>();
     d.lights = I.lights.as<LightRec>();
     d.num_lights = (int)sc.lights.size();
     d.env_light = sc.env_light;
@@ -1341,6 +1353,8 @@ void GpuScene::refit(HostScene& host, const float* nrm, double phase_ms[2]) {
             throw std::runtime_error("refit: media trees changed size");
         HIP_OK(hipMemcpy(I.media_nodes.p, media.nodes.data(), I.media_nodes.bytes, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(I.media_isect.p, media.isect.data(), I.media_isect.bytes, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(I.media_grid.p, media.grid.data(), I.media_grid.bytes, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(I.media_clear.p, media.clear.data(), I.media_clear.bytes, hipMemcpyHostToDevice));
     }
     phase_ms[1] = lap();
 }
@@ -1362,7 +1376,8 @@ size_t GpuScene::device_bytes() const {
     return I.nodes.bytes + I.tri_isect.bytes + I.tri_attr.bytes + I.materials.bytes + I.lights.bytes +
            I.light_tris.bytes + I.env_rgb.bytes + I.env_row.bytes + I.env_marg.bytes + I.env_alias_row.bytes +
            I.env_alias_marg.bytes + I.tex_texels.bytes + I.tex_info.bytes + I.order.bytes + I.parent.bytes +
-           I.refit_cnt.bytes + I.media_root.bytes + I.media_nodes.bytes + I.media_isect.bytes + I.media_prim.bytes;
+           I.refit_cnt.bytes + I.media_root.bytes + I.media_nodes.bytes + I.media_isect.bytes + I.media_prim.bytes +
+           I.media_grid.bytes + I.media_clear.bytes;
 }
 
 // Row-conditional and marginal CDFs weighted by sin(theta) (reference:

@@ -1031,6 +1031,128 @@ void set_scene_geometry(HostScene& sc, const float* pos, const float* nrm) {
         if (l.type == kAreaLight && l.mesh >= 0) l.area = sc.meshes[l.mesh].area;
 }
 
+namespace {
+
+// Squared distance from point p to triangle (a, b, c), double precision
+// (closest point by Voronoi regions: Ericson, Real-Time Collision Detection, 5.1.5).
+double dist2_point_triangle(const double* p, const double* a, const double* b, const double* c) {
+    auto sub = [](const double* x, const double* y, double* o) { o[0] = x[0] - y[0]; o[1] = x[1] - y[1]; o[2] = x[2] - y[2]; };
+    auto dot = [](const double* x, const double* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+    double ab[3], ac[3], ap[3], bp[3], cp[3], q[3];
+    sub(b, a, ab); sub(c, a, ac); sub(p, a, ap);
+    auto at = [&](const double* o, double u, const double* e1, double v, const double* e2) {
+        for (int k = 0; k < 3; ++k) q[k] = o[k] + u * e1[k] + v * e2[k];
+    };
+    double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) at(a, 0, ab, 0, ac);
+    else {
+        sub(p, b, bp);
+        double d3 = dot(ab, bp), d4 = dot(ac, bp);
+        if (d3 >= 0 && d4 <= d3) at(b, 0, ab, 0, ac);
+        else {
+            double vc = d1 * d4 - d3 * d2;
+            if (vc <= 0 && d1 >= 0 && d3 <= 0) at(a, d1 / (d1 - d3), ab, 0, ac);
+            else {
+                sub(p, c, cp);
+                double d5 = dot(ab, cp), d6 = dot(ac, cp);
+                if (d6 >= 0 && d5 <= d6) at(c, 0, ab, 0, ac);
+                else {
+                    double vb = d5 * d2 - d1 * d6;
+                    if (vb <= 0 && d2 >= 0 && d6 <= 0) at(a, 0, ab, d2 / (d2 - d6), ac);
+                    else {
+                        double va = d3 * d6 - d5 * d4;
+                        if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+                            double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+                            double bc[3];
+                            sub(c, b, bc);
+                            at(b, w, bc, 0, ac);
+                        } else {
+                            double den = 1.0 / (va + vb + vc);
+                            at(a, vb * den, ab, vc * den, ac);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    double dq[3];
+    sub(p, q, dq);
+    return dot(dq, dq);
+}
+
+// Distance from p to the nearest triangle of a tree (branch and bound over its
+// reference-layout nodes, nearer box first).
+double nearest_distance(const BvhBuild& b, const float* pos, const double* p) {
+    double best = std::numeric_limits<double>::infinity();
+    auto box_d2 = [&](const LinearNode& n) {
+        double d2 = 0;
+        for (int k = 0; k < 3; ++k) {
+            double e = std::max({(double)n.lo[k] - p[k], 0.0, p[k] - (double)n.hi[k]});
+            d2 += e * e;
+        }
+        return d2;
+    };
+    std::vector<int> stack{0};
+    while (!stack.empty()) {
+        int i = stack.back();
+        stack.pop_back();
+        const LinearNode& n = b.nodes[i];
+        if (box_d2(n) >= best) continue;
+        if (n.tri_index >= 0) {
+            for (int k = 0; k < n.second; ++k) {
+                const float* t = pos + 9 * (size_t)b.order[(size_t)n.tri_index + k];
+                double a[3] = {t[0], t[1], t[2]}, bb[3] = {t[3], t[4], t[5]}, c[3] = {t[6], t[7], t[8]};
+                best = std::min(best, dist2_point_triangle(p, a, bb, c));
+            }
+        } else {
+            int l = i + 1, r = n.second;
+            if (box_d2(b.nodes[l]) < box_d2(b.nodes[r])) std::swap(l, r);
+            stack.push_back(l);
+            stack.push_back(r);
+        }
+    }
+    return std::sqrt(best);
+}
+
+constexpr int kClearanceGrid = 24;
+
+// Lower bounds of the distance to the mesh per cell: the cell centre's exact
+// distance less half the cell diagonal and a rounding margin well above the
+// FP32 error of a triangle test at the mesh's coordinates (a query of the
+// walk can only report a hit at t when a triangle lies within about t of its
+// origin).  Rounded down to float.
+void clearance_grid(MediumTree& t, const float* pos) {
+    const Box& bx = t.bvh.bounds;
+    const int n = kClearanceGrid;
+    t.grid_n = n;
+    t.grid_lo = bx.lo;
+    double ext[3] = {(double)bx.hi.x - bx.lo.x, (double)bx.hi.y - bx.lo.y, (double)bx.hi.z - bx.lo.z};
+    double cell[3], mag = 0;
+    float inv[3];
+    for (int k = 0; k < 3; ++k) {
+        cell[k] = std::max(ext[k], 1e-6) / n;
+        inv[k] = (float)(1.0 / cell[k]);
+    }
+    for (float v : {bx.lo.x, bx.lo.y, bx.lo.z, bx.hi.x, bx.hi.y, bx.hi.z}) mag = std::max(mag, (double)std::fabs(v));
+    t.grid_inv = v3(inv[0], inv[1], inv[2]);
+    // A point may land one cell over from floor((p - lo) * inv) rounding: allow
+    // the full cell diagonal, not half of it.
+    const double reach = std::sqrt(cell[0] * cell[0] + cell[1] * cell[1] + cell[2] * cell[2]);
+    const double margin = 1e-4 * (1.0 + mag);
+    t.clearance.assign((size_t)n * n * n, 0.f);
+    for (int z = 0; z < n; ++z)
+        for (int y = 0; y < n; ++y)
+            for (int x = 0; x < n; ++x) {
+                double c[3] = {bx.lo.x + (x + 0.5) * cell[0], bx.lo.y + (y + 0.5) * cell[1], bx.lo.z + (z + 0.5) * cell[2]};
+                double d = nearest_distance(t.bvh, pos, c) - reach - margin;
+                float f = d > 0 ? (float)d : 0.f;
+                if ((double)f > d) f = std::nextafter(f, 0.f);
+                t.clearance[((size_t)z * n + y) * n + x] = d > 0 ? f : 0.f;
+            }
+}
+
+}  // namespace
+
 void build_media(HostScene& sc) {
     sc.media.clear();
     for (size_t m = 0; m < sc.meshes.size(); ++m) {
@@ -1041,6 +1163,7 @@ void build_media(HostScene& sc) {
         MediumTree t;
         t.mesh = (int)m;
         build_bvh(sc.pos.data() + 9 * (size_t)mr.start, mr.count, t.bvh);
+        clearance_grid(t, sc.pos.data() + 9 * (size_t)mr.start);
         sc.media.push_back(std::move(t));
     }
 }
@@ -1049,6 +1172,7 @@ void refit_media(HostScene& sc) {
     for (MediumTree& t : sc.media) {
         const MeshRec& mr = sc.meshes[t.mesh];
         refit_bvh(sc.pos.data() + 9 * (size_t)mr.start, mr.count, t.bvh);
+        clearance_grid(t, sc.pos.data() + 9 * (size_t)mr.start);
     }
 }
 

@@ -82,6 +82,13 @@ struct EnvMap {
 struct MediumTree {
     int mesh = -1;
     BvhBuild bvh;
+    // Clearance grid: n^3 cells over the tree's box; cell value = a lower
+    // bound, safe under FP32 rounding, of the distance from any point of the
+    // cell to the mesh (0 near the surface).  A walk step whose free flight is
+    // shorter than its cell's value cannot meet the surface and needs no query.
+    int grid_n = 0;
+    vec3 grid_lo, grid_inv;     // cell index = floor((p - lo) * inv)
+    std::vector<float> clearance;
 };
 
 struct HostScene {

@@ -393,6 +393,12 @@ int rt_scene_get_texture(const rt_scene* s, int index, int* w, int* h, uint8_t* 
    (0 Burley = the reference's, 1 random walk), one padding word.  *n receives the count; out
    may be NULL. */
 int rt_scene_get_materials(const rt_scene* s, int* n, void* out);
+/* The clearance grid of a random-walk medium (no reference counterpart; the
+   random-walk subsurface mode is this library's): *n cells per axis (0 when
+   mesh `mesh` is no such medium), cell index floor((p - lo) * inv) in FP32, and
+   n^3 floats (x fastest), each a lower bound of the distance from any point of
+   the cell to the mesh.  lo, inv and cells may be NULL. */
+int rt_scene_get_medium(const rt_scene* s, int mesh, int* n, float* lo, float* inv, float* cells);
 /* The scene's decoded environment map as HDRTexture::Load receives it (Src/Core/texture.cu:
    17-25: stbi_loadf's RGB, row 0 = the file's first scanline; a PFM map is read losslessly):
    3 floats per texel.  *w, *h receive its size (0 x 0 without an environment light); rgb may
