@@ -619,7 +619,9 @@ def batched_bench(n: int = 64, seed: int = 4, queue_gib: int = 16):
     resolution from {128..1024}, spp 1..64, depth 12), all submitted at t = 0.
     Batched: the pool's continuous batching, every stage launch shared by all
     jobs with work left.  Plain: one job at a time, a fresh single-scene
-    renderer each.  Images are bit-identical between the two (same RNG keys)."""
+    renderer each, all of a job's samples in one pass (and again with the ray
+    sort on, and with one sample per pass as the reference's render loop
+    takes them).  Images are bit-identical between all of them (same RNG keys)."""
     import cudaraytracer_amd as rt
 
     rng = np.random.default_rng(seed)
@@ -662,6 +664,10 @@ def batched_bench(n: int = 64, seed: int = 4, queue_gib: int = 16):
     reqs_s = [(0.0, rt.Job(slot, spp=spp, render={"ray_sort": 1})) for slot, (_, spp) in zip(slots, meta)]
     rep_ps, _, _ = pool.serve(reqs_s, batched=False)
     plain_best = max(rep_p["jobs_per_s"], rep_ps["jobs_per_s"])
+    # And in the reference's RenderLoop structure: one sample of every pixel
+    # per pass (one frame per sample), each job still alone on the GPU.
+    reqs_1 = [(0.0, rt.Job(slot, spp=spp, render={"batch_spp": 1})) for slot, (_, spp) in zip(slots, meta)]
+    rep_p1, _, _ = pool.serve(reqs_1, batched=False)
     return {
         "workload": f"{n} seeded scenes: Cornell shell + GGX/dielectric/diffuse icospheres + heightfield patch, "
                     "triangles log-uniform 1k-500k, resolution from {128,256,512,1024}^2, spp 1..64, depth 12",
@@ -676,7 +682,11 @@ def batched_bench(n: int = 64, seed: int = 4, queue_gib: int = 16):
                   "job_p50_ms": round(rep_p["job_p50_ms"], 2), "job_p99_ms": round(rep_p["job_p99_ms"], 2),
                   "mrays_s": round(rays / rep_p["makespan_s"] / 1e6, 1)},
         "plain_ray_sort": {"jobs_per_s": round(rep_ps["jobs_per_s"], 3), "makespan_s": round(rep_ps["makespan_s"], 4)},
+        "plain_one_sample_per_pass": {"jobs_per_s": round(rep_p1["jobs_per_s"], 3),
+                                      "makespan_s": round(rep_p1["makespan_s"], 4),
+                                      "mrays_s": round(rays / rep_p1["makespan_s"] / 1e6, 1)},
         "speedup": round(rep_b["jobs_per_s"] / plain_best, 3),
+        "speedup_vs_one_sample_per_pass": round(rep_b["jobs_per_s"] / rep_p1["jobs_per_s"], 3),
         "queue_gib": queue_gib,
         "autotune": tune,
     }

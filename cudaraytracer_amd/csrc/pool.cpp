@@ -668,7 +668,10 @@ ServeReport serve(Pool& pool, const std::vector<ServeRequest>& reqs, int mode) {
             wait_until(pool, base + rq.arrival_s);
             const JobDesc& d = rq.job;
             RenderSettings set = d.set;
-            set.batch_spp = d.frames > 0 ? d.spp : std::min(d.spp, 64);
+            // Accumulate jobs run all their samples in one pass (up to 64) unless
+            // the job's own settings ask for smaller passes.
+            set.batch_spp = d.frames > 0 ? d.spp
+                                         : (d.set.batch_spp > 0 ? std::min(d.set.batch_spp, d.spp) : std::min(d.spp, 64));
             std::vector<double> done;
             std::unique_ptr<Renderer> fresh;
             Renderer* r = nullptr;
