@@ -1081,9 +1081,9 @@ double dist2_point_triangle(const double* p, const double* a, const double* b, c
 }
 
 // Distance from p to the nearest triangle of a tree (branch and bound over its
-// reference-layout nodes, nearer box first).
-double nearest_distance(const BvhBuild& b, const float* pos, const double* p) {
-    double best = std::numeric_limits<double>::infinity();
+// reference-layout nodes, nearer box first), given an upper bound of it.
+double nearest_distance(const BvhBuild& b, const float* pos, const double* p, double bound) {
+    double best = bound * bound;
     auto box_d2 = [&](const LinearNode& n) {
         double d2 = 0;
         for (int k = 0; k < 3; ++k) {
@@ -1092,10 +1092,11 @@ double nearest_distance(const BvhBuild& b, const float* pos, const double* p) {
         }
         return d2;
     };
-    std::vector<int> stack{0};
-    while (!stack.empty()) {
-        int i = stack.back();
-        stack.pop_back();
+    int stack[2 * 64 + 2];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        int i = stack[--sp];
         const LinearNode& n = b.nodes[i];
         if (box_d2(n) >= best) continue;
         if (n.tri_index >= 0) {
@@ -1107,8 +1108,8 @@ double nearest_distance(const BvhBuild& b, const float* pos, const double* p) {
         } else {
             int l = i + 1, r = n.second;
             if (box_d2(b.nodes[l]) < box_d2(b.nodes[r])) std::swap(l, r);
-            stack.push_back(l);
-            stack.push_back(r);
+            stack[sp++] = l;
+            stack[sp++] = r;
         }
     }
     return std::sqrt(best);
@@ -1140,15 +1141,33 @@ void clearance_grid(MediumTree& t, const float* pos) {
     const double reach = std::sqrt(cell[0] * cell[0] + cell[1] * cell[1] + cell[2] * cell[2]);
     const double margin = 1e-4 * (1.0 + mag);
     t.clearance.assign((size_t)n * n * n, 0.f);
-    for (int z = 0; z < n; ++z)
-        for (int y = 0; y < n; ++y)
+    // Slabs of z on host threads; along x each cell starts from its
+    // neighbour's distance plus one cell as the upper bound (exact minimum
+    // either way, the bound only prunes).
+    if (t.bvh.max_depth > 64) throw std::runtime_error("medium tree deeper than 64 levels");
+    auto slab = [&](int z) {
+        for (int y = 0; y < n; ++y) {
+            double prev = std::numeric_limits<double>::infinity();
             for (int x = 0; x < n; ++x) {
                 double c[3] = {bx.lo.x + (x + 0.5) * cell[0], bx.lo.y + (y + 0.5) * cell[1], bx.lo.z + (z + 0.5) * cell[2]};
-                double d = nearest_distance(t.bvh, pos, c) - reach - margin;
+                double dist = nearest_distance(t.bvh, pos, c, prev + cell[0]);
+                prev = dist;
+                double d = dist - reach - margin;
                 float f = d > 0 ? (float)d : 0.f;
                 if ((double)f > d) f = std::nextafter(f, 0.f);
                 t.clearance[((size_t)z * n + y) * n + x] = d > 0 ? f : 0.f;
             }
+        }
+    };
+    const int workers = std::max(1, std::min<int>(n, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int w = 1; w < workers; ++w)
+        pool.emplace_back([&]() {
+            for (int z; (z = next.fetch_add(1)) < n;) slab(z);
+        });
+    for (int z; (z = next.fetch_add(1)) < n;) slab(z);
+    for (auto& th : pool) th.join();
 }
 
 }  // namespace
