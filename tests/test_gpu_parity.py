@@ -219,7 +219,7 @@ def test_traversal_counters(scene_dir):
     assert st2.extension_node_visits == 0 and st2.extension_rays == st.extension_rays
 
 
-@pytest.mark.parametrize("name", ["cornell_spheres_128", "textured_cornell"])
+@pytest.mark.parametrize("name", ["cornell_spheres_128", "textured_cornell", "cornell_sss_walk"])
 def test_converged_image_within_rmse_of_fp64_oracle(scene_dir, oracle_lib, name):
     """The GPU image against the oracle's FP64 twin (the same estimator and
     random numbers with FP64 shading): per pixel, 97 % within rtol 1e-3 /
