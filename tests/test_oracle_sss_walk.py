@@ -171,3 +171,41 @@ def test_walk_is_deterministic_per_coordinate(tmp_path, oracle_lib):
     for x, y in zip(a[1:4], b[1:4]):
         assert np.array_equal(x, y)
     assert not (np.array_equal(a[1], c[1]) and a[4] == c[4])
+
+
+@pytest.mark.parametrize("bad,match", [
+    ({"subsurface_method": "diffusion"}, "subsurface_method"),
+    ({"subsurface_method": "random_walk", "sigma_a": [-1, 0, 0]}, "sigma"),
+    ({"subsurface_method": "random_walk", "sigma_a": [0, 0, 0], "sigma_s": [1, 0, 1]}, "sigma"),
+    ({"subsurface_method": "burley", "sigma_s": [1e39, 1, 1]}, "sigma"),  # overflows float
+])
+def test_bad_medium_keys_fail_the_load(tmp_path, bad, match):
+    import cudaraytracer_amd as rt
+
+    mat = {"name": "medium", "type": "subsurface", "basecolor": [1, 1, 1]}
+    mat.update(bad)
+    spec = scenes.SceneSpec(camera=scenes._camera(8, 8, 1, 2), materials=[mat],
+                            meshes=[scenes.MeshSpec("body", "medium", scenes._box((0, 0, 0), (1, 1, 1)))],
+                            lights=[])
+    path = scenes.write_scene(spec, str(tmp_path), "bad")
+    with pytest.raises(rt.RtError, match=match):
+        rt.Scene(path, bvh_builder=rt.BVH_HOST)
+
+
+def test_medium_keys_load_with_defaults(tmp_path):
+    import cudaraytracer_amd as rt
+
+    mats = [{"name": "rw", "type": "subsurface", "basecolor": [1, 1, 1], "subsurface_method": "random_walk"},
+            {"name": "plain", "type": "subsurface", "basecolor": [1, 1, 1]}]
+    spec = scenes.SceneSpec(camera=scenes._camera(8, 8, 1, 2), materials=mats,
+                            meshes=[scenes.MeshSpec("a", "rw", scenes._box((0, 0, 0), (1, 1, 1))),
+                                    scenes.MeshSpec("b", "plain", scenes._box((2, 0, 0), (3, 1, 1)))],
+                            lights=[])
+    path = scenes.write_scene(spec, str(tmp_path), "ok")
+    m = rt.Scene(path, bvh_builder=rt.BVH_HOST).materials()
+    ints = m.view(np.int32)
+    assert ints[0, 22] == 1 and ints[1, 22] == 0  # random walk, Burley
+    assert np.array_equal(m[0, 16:22], np.float32([0, 0, 0, 1, 1, 1]))  # sigma_a, sigma_s defaults
+    a = rtref.parse_scene(path)  # the oracle's own parser reads the same keys
+    assert np.array_equal(a.materials[:, 16:22], m[:, 16:22])
+    assert list(a.materials[:, 22]) == [1.0, 0.0]
