@@ -408,7 +408,8 @@ def measure_render(scene, steps: int, warmup: int, traffic_file=None, **renderer
         "batch_spp": st.batch_spp,
         "path_counts": {"extension_rays": sc.extension_rays, "misses": sc.miss_paths,
                         "shaded_hits": dict(sc.as_dict()["class_paths"]), "shadow_rays": sc.shadow_rays,
-                        "shadow_lit": sc.shadow_lit, "bounces": int(round(bounces))},
+                        "shadow_lit": sc.shadow_lit, "subsurface_rays": sc.probe_rays,
+                        "bounces": int(round(bounces))},
         "nodes_per_ext_ray": round(sc.extension_node_visits / max(1, sc.extension_rays), 2),
         "tris_per_ext_ray": round(sc.extension_tri_tests / max(1, sc.extension_rays), 2),
         "stage_ms_per_step": {n: round(sp.stage_ms[i] / prof_steps, 3) for i, n in enumerate(rt.STAGES)

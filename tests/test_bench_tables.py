@@ -58,3 +58,22 @@ def test_traffic_per_bounce_and_scaling(tmp_path):
     assert bench.load_traffic(str(tmp_path / "traffic.txt")) == {
         "k_extend": 210 * 1024, "k_shade": 84 * 1024, "k_shadow": 126 * 1024}
     assert bench.load_traffic(str(tmp_path / "absent.txt")) == {}
+
+
+def test_burley_variant_rewrites_only_the_method(tmp_path):
+    import json
+
+    from cudaraytracer_amd import scenes
+
+    path = scenes.write_scene(scenes.terrain_env(grid=11, env_w=8, env_h=4, width=8, height=8, spp=1),
+                              str(tmp_path), "t")
+    out = bench.burley_variant(path)
+    a, b = json.load(open(path)), json.load(open(out))
+    assert os.path.dirname(out) == os.path.dirname(path)  # same OBJ and map paths
+    assert [m.get("subsurface_method") for m in a["materials"]] == [None, "random_walk"]
+    assert [m.get("subsurface_method") for m in b["materials"]] == [None, "burley"]
+    for m in b["materials"]:
+        m.pop("subsurface_method", None)
+    for m in a["materials"]:
+        m.pop("subsurface_method", None)
+    assert a == b
