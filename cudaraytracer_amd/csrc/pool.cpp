@@ -55,6 +55,11 @@ struct Pool::Impl {
     std::map<int, std::unique_ptr<Job>> jobs;
     std::deque<int> active;  // submission order
     hipStream_t side = nullptr;  // frame work (G-buffer, SVGF) beside the engine's pass
+    // A third stream for the serial tick's filters: the frames of a tick are
+    // filtered on the engine stream, the side stream and this one in turn, so
+    // one frame's kernels fill the tails of another's.
+    hipStream_t filt = nullptr;
+    hipEvent_t ev_pass = nullptr, ev_side = nullptr, ev_filt = nullptr;
     Pool::Tuning tune;
     long tick = 0;
     int cohort_size[2] = {0, 0};
@@ -76,6 +81,10 @@ Pool::Pool(const PoolSettings& s) : impl_(new Impl) {
     I.eng.reset(new Engine(std::max<size_t>(1, qb / kQueueBytesPerPath), s.profile != 0));
     I.hbm_budget = s.hbm_budget_bytes ? s.hbm_budget_bytes : ((size_t)64 << 30);
     HIP_OK(hipStreamCreateWithFlags(&I.side, hipStreamNonBlocking));
+    HIP_OK(hipStreamCreateWithFlags(&I.filt, hipStreamNonBlocking));
+    HIP_OK(hipEventCreateWithFlags(&I.ev_pass, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&I.ev_side, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&I.ev_filt, hipEventDisableTiming));
     I.t0 = Clock::now();
 }
 
@@ -83,9 +92,14 @@ Pool::~Pool() {
     if (impl_) {
         if (impl_->eng) (void)hipStreamSynchronize(impl_->eng->stream());
         if (impl_->side) (void)hipStreamSynchronize(impl_->side);
-        hipStream_t side = impl_->side;
+        if (impl_->filt) (void)hipStreamSynchronize(impl_->filt);
+        hipStream_t side = impl_->side, filt = impl_->filt;
+        hipEvent_t evs[3] = {impl_->ev_pass, impl_->ev_side, impl_->ev_filt};
         delete impl_;
         if (side) (void)hipStreamDestroy(side);
+        if (filt) (void)hipStreamDestroy(filt);
+        for (hipEvent_t e : evs)
+            if (e) (void)hipEventDestroy(e);
     }
 }
 
@@ -390,10 +404,30 @@ int Pool::step_serial() {
                          j->acc_d->as<float4>(), j->acc_i->as<float4>(), I.eng->prof, s, I.side);
     }
     I.eng->run(items, false);
+    // Each frame's filters (temporal, variance, a-trous) read only its own job's
+    // buffers: the frames go to three streams in turn, after the pass, and the
+    // engine stream waits for all of them (the next tick's begin clears the
+    // accumulators the filters read).
+    hipStream_t fstreams[3] = {s, I.side, I.filt};
+    bool forked = false;
+    int nf = 0;
     for (size_t k = 0; k < served.size(); ++k) {
         Impl::Job* j = served[k];
-        if (j->d.frames > 0)
-            j->fs->end(spp_of[k], j->d.svgf, j->acc_d->as<float4>(), j->acc_i->as<float4>(), I.eng->prof, s);
+        if (j->d.frames <= 0) continue;
+        hipStream_t fs = fstreams[nf++ % 3];
+        if (fs != s && !forked) {
+            HIP_OK(hipEventRecord(I.ev_pass, s));
+            HIP_OK(hipStreamWaitEvent(I.side, I.ev_pass, 0));
+            HIP_OK(hipStreamWaitEvent(I.filt, I.ev_pass, 0));
+            forked = true;
+        }
+        j->fs->end(spp_of[k], j->d.svgf, j->acc_d->as<float4>(), j->acc_i->as<float4>(), I.eng->prof, fs);
+    }
+    if (forked) {
+        HIP_OK(hipEventRecord(I.ev_side, I.side));
+        HIP_OK(hipEventRecord(I.ev_filt, I.filt));
+        HIP_OK(hipStreamWaitEvent(s, I.ev_side, 0));
+        HIP_OK(hipStreamWaitEvent(s, I.ev_filt, 0));
     }
     HIP_OK(hipStreamSynchronize(s));
     const double t_done = now();
