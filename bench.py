@@ -710,7 +710,9 @@ def serving_bench(rates=(5.0, 10.0, 20.0, 50.0, 100.0, 200.0), n_jobs: int = 30,
     specs = [("terrain_env", {"seed": sd, "grid": g, "width": width, "height": height, "spp": 1},
               os.path.join(tmp, f"terrain{sd}"), f"terrain{sd}") for sd, g in ((3, 1001), (5, 709), (7, 501))]
     sc = _load(_generate(specs))
-    pool = rt.Pool(queue_budget_bytes=queue_gib << 30)
+    # Continuous batching with cohort ticks: one cohort's frames traced while the
+    # other's are filtered on two side streams (rt_pool_params.cohort_ticks).
+    pool = rt.Pool(queue_budget_bytes=queue_gib << 30, pipelined=True)
     slots = [pool.add_scene(s) for s in sc]
     base_cams = [rt.Renderer(s, batch_spp=1).camera() for s in sc]
     out = {"workload": f"Poisson arrivals of {n_jobs} jobs per rate; each a {frames}-frame orbit at "
@@ -760,7 +762,7 @@ def serving_bench(rates=(5.0, 10.0, 20.0, 50.0, 100.0, 200.0), n_jobs: int = 30,
     # Capped admission: the same streams with at most `cap` real-time jobs at once.
     out["capped"] = []
     for cap in caps:
-        cp = rt.Pool(queue_budget_bytes=queue_gib << 30, max_realtime_jobs=cap)
+        cp = rt.Pool(queue_budget_bytes=queue_gib << 30, max_realtime_jobs=cap, pipelined=True)
         for s in sc:
             cp.add_scene(s)
         cp.set_tuning(tuning.grid_scale, tuning.pass_paths, tuning.tile_width, bool(tuning.ray_sort))

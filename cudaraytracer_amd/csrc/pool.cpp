@@ -304,22 +304,27 @@ int Pool::step_pipelined() {
         spp_of.push_back(k);
         used += (size_t)j.wh * k;
     }
-    // Side stream: finish traced frames of cohort o, begin the next ones.
+    // Side streams: finish traced frames of cohort o, begin the next ones; a
+    // job's frame work stays on one stream, the jobs alternate between two.
+    hipStream_t sides[2] = {I.side, I.filt};
+    int ns = 0;
     for (Impl::Job* j : side_jobs) {
         const Impl::SceneSlot& sc = I.scenes[j->d.scene];
+        hipStream_t ss = sides[ns++ & 1];
         if (j->phase == 2) {
-            j->fs->end(j->d.spp, j->d.svgf, j->acc_d->as<float4>(), j->acc_i->as<float4>(), I.eng->prof, I.side);
+            j->fs->end(j->d.spp, j->d.svgf, j->acc_d->as<float4>(), j->acc_i->as<float4>(), I.eng->prof, ss);
             if (j->st.frames_done + 1 == j->d.frames) continue;
             j->fs->begin(*sc.gpu, *sc.host, j->d.cams[j->st.frames_done + 1], j->acc_d->as<float4>(),
-                         j->acc_i->as<float4>(), I.eng->prof, I.side);
+                         j->acc_i->as<float4>(), I.eng->prof, ss);
         } else {
             if (j->st.start_s < 0) j->st.start_s = t_start;
             j->fs->begin(*sc.gpu, *sc.host, j->d.cams[j->st.frames_done], j->acc_d->as<float4>(),
-                         j->acc_i->as<float4>(), I.eng->prof, I.side);
+                         j->acc_i->as<float4>(), I.eng->prof, ss);
         }
     }
     if (!items.empty()) I.eng->run(items, false);
     HIP_OK(hipStreamSynchronize(I.side));
+    HIP_OK(hipStreamSynchronize(I.filt));
     HIP_OK(hipStreamSynchronize(s));
     const double t_done = now();
     I.tick++;

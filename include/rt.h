@@ -285,9 +285,10 @@ typedef struct rt_pool_params {
     int64_t queue_budget_bytes; /* path queues of the shared engine; 0 = 8 GiB */
     int64_t hbm_budget_bytes;   /* resident scenes + job state admitted at once; 0 = 64 GiB */
     int profile;                /* 1 = per-stage HIP event timing in rt_pool_stats */
-    int cohort_ticks;           /* 0 = a tick traces and filters every real-time job on one stream;
-                                   1 = two cohorts alternate, one traced while the other is filtered
-                                   on a second stream (a real-time job advances every second tick) */
+    int cohort_ticks;           /* 0 = a tick traces every real-time job's frame, then filters the frames
+                                   on three streams in turn; 1 = two cohorts alternate, one traced while
+                                   the other's frames are filtered on two side streams (a real-time job
+                                   advances every second tick; the faster mode for saturated serving) */
     int max_realtime_jobs;      /* rt_pool_serve's continuous batching admits at most this many real-time
                                    jobs at once (later ones wait in arrival order), which bounds the work of
                                    a tick and so frame latency; 0 = as many as the HBM budget allows */
