@@ -72,10 +72,14 @@ C3_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_terrain_2M_1080p_32spp.txt"
 
 def _stage_of(kernel: str):
     """Stage of a measured kernel instance: only the plain (uncounted,
-    single-scene) instantiations the timed renders launch."""
+    single-scene) instantiations the timed renders launch.  The extend stage
+    is k_extend (with or without straggler suspension) and the continuation
+    launch that finishes its suspended searches."""
     import re
 
-    m = re.match(r"(?:rt::)?(k_extend<false, false|k_shadow<false, false(?:, (?:false|0))?)>$", kernel)
+    if re.match(r"(?:rt::)?(?:k_extend<false, false(?:, \d+)?|k_extend_cont<false, false)>$", kernel):
+        return "k_extend"
+    m = re.match(r"(?:rt::)?(k_shadow<false, false(?:, (?:false|0))?)>$", kernel)
     if m:
         return m.group(1).split("<")[0]
     if re.match(r"(?:rt::)?k_shade<(?:\d, )?false>$", kernel):
@@ -179,9 +183,11 @@ def traffic_from_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_byte
     fetch, write = per_kernel(fetch_db), per_kernel(write_db)
     lines = [f"fetch_scale {f_scale:.4f}", f"write_scale {w_scale:.4f}", "",
              "k_shade is per bounce: its class kernels' bytes summed, divided by the bounces (the single-scene",
-             "k_extend dispatches, counting renders included, as their shade launches are the same kernels)", "",
+             "k_extend dispatches, counting renders included, as their shade launches are the same kernels);",
+             "k_extend is per bounce too: the k_extend launch and its continuation launch summed", "",
              "stage      dispatches  fetch_B_per_launch  write_B_per_launch  hbm_B_per_launch"]
-    n_bounces = sum(len(fetch[k]) for k in fetch if re.match(r"(?:rt::)?k_extend<(?:true|false), false>$", k))
+    n_bounces = sum(len(fetch[k]) for k in fetch if re.match(r"(?:rt::)?k_extend<(?:true|false), false(?:, \d+)?>$", k))
+    n_ext = sum(len(fetch[k]) for k in fetch if re.match(r"(?:rt::)?k_extend<false, false(?:, \d+)?>$", k))
     for st in TRAFFIC_STAGES:
         fk = [k for k in fetch if _stage_of(k) == st]
         wk = [k for k in write if _stage_of(k) == st]
@@ -189,7 +195,8 @@ def traffic_from_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_byte
         nw = sum(len(write[k]) for k in wk)
         if not nf or not nw:
             continue
-        per = n_bounces if st == "k_shade" and n_bounces else None
+        per = (n_bounces if st == "k_shade" and n_bounces else
+               n_ext if st == "k_extend" and n_ext else None)
         fb = sum(sum(fetch[k].values()) for k in fk) * 1024 * f_scale / (per or nf)
         wb = sum(sum(write[k].values()) for k in wk) * 1024 * w_scale / (per or nw)
         lines.append(f"{st:10s} {per or nf:11d} {fb:19.0f} {wb:19.0f} {fb + wb:17.0f}")

@@ -149,7 +149,9 @@ enum CounterKind : int {
     kFetchShadow = 4,
     kCntClass0 = 5,     // kNumClasses per-class list lengths
     kFetchClass0 = 10,  // kNumClasses per-class fetch cursors
-    kNumCounterKinds = 16,
+    kCntCont = 15,      // suspended extension rays stored for the continuation launch
+    kFetchCont = 16,    // and its fetch cursor
+    kNumCounterKinds = 17,
 };
 constexpr size_t kCounterInts = (size_t)kNumCounterKinds * kSlices * kCounterStride;
 

@@ -46,10 +46,16 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// Suspended extension searches: room for one in kContPathsPer paths, each a
+// float4 hit, an int4 state and kContStack stack words.
+constexpr size_t kContPathsPer = 4;
+constexpr size_t kContStack = 16;
+constexpr size_t kContBytes = 16 + 16 + 4 * kContStack;
 // Device bytes per path of an engine's queues: two ping-pong SoA queues
 // (3 x float4 each), the hit record, one class-list entry per class, the
-// shadow record and the per-sample direct/indirect sums.
-constexpr size_t kQueueBytesPerPath = 2 * 3 * 16 + 16 + 4 * kNumClasses + 64 + 32;
+// shadow record, the per-sample direct/indirect sums and the share of the
+// suspended-search records.
+constexpr size_t kQueueBytesPerPath = 2 * 3 * 16 + 16 + 4 * kNumClasses + 64 + 32 + kContBytes / kContPathsPer;
 // With the ray sort on: a 2-byte key, a 4-byte permutation entry and a
 // 32-byte copy of the ray at its sorted position per path.
 constexpr size_t kSortBytesPerPath = 38;
@@ -118,7 +124,7 @@ private:
     template <bool POOLED>
     void run_t(int max_bounces, const bool* classes, bool count, int stack_entries);
     struct Grids {
-        int ext = 0, shadow = 0, shadow_early = 0, shadow_segment = 0, shade = 0, shade_stack = 0;
+        int ext = 0, ext_susp = 0, ext_cont = 0, shadow = 0, shadow_early = 0, shadow_segment = 0, shade = 0, shade_stack = 0;
     };
     template <bool POOLED>
     Grids grids_for(int stack_entries);
@@ -130,6 +136,8 @@ private:
     size_t cap_ = 0;
     hipStream_t stream_ = nullptr;
     DevBuf qo_[2], qd_[2], qw_[2], hits_, lists_, shadow_, slot_d_, slot_i_, counters_, stats_;
+    DevBuf cont_hit_, cont_state_, cont_stack_;  // suspended extension searches (k_extend_cont)
+    int cont_room_ = 0;                          // records per slice
     bool ray_sort_ = false;
     int shadow_mode_ = 0;
     DevBuf keys_, perm_, sort_hist_, sort_cur_;  // ray sort: key and permutation per path, per-slice bins

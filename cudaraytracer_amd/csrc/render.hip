@@ -70,6 +70,8 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* p, uint32_t v) 
 // the first live one, which spreads thieves over the live slices.  Counts
 // are fixed for the launch (written by the previous stage) and fetch counters
 // only grow, so a slice seen drained stays drained and every wave's loop ends.
+// `limit` caps every slice's count (a list whose appends may overrun its
+// per-slice room holds only the first `limit` entries).
 static_assert(kSlices == kWave, "the drained-slice scan maps one slice per lane");
 static_assert(kBlock / kWave == 4 && kSlices == 64, "slice-to-XCD map assumes 4 waves per block, 64 slices");
 struct SliceCursor {
@@ -77,16 +79,18 @@ struct SliceCursor {
     int cnt_kind, fetch_kind;
     int slice, count, rot;
     uint64_t own;  // slices whose first waves run on this wave's XCD
-    __device__ void init(int* c, int ck, int fk) {
+    int limit;
+    __device__ void init(int* c, int ck, int fk, int lim = 0x7fffffff) {
         counters = c;
         cnt_kind = ck;
         fetch_kind = fk;
+        limit = lim;
         int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
         slice = wave % kSlices;
         rot = (wave / kSlices) % kSlices;
         const int xcd = (int)(blockIdx.x & 7u);
         own = (0xFull << (4 * xcd)) | (0xFull << (32 + 4 * xcd));
-        count = *ctr_at(counters, cnt_kind, slice);
+        count = min(*ctr_at(counters, cnt_kind, slice), limit);
     }
     __device__ bool next(int& base) {
         const int lane = (int)(threadIdx.x & (kWave - 1));
@@ -101,7 +105,7 @@ struct SliceCursor {
                 return true;
             }
             // Drained: lane k checks slice k.
-            const int c = *ctr_at(counters, cnt_kind, lane);
+            const int c = min(*ctr_at(counters, cnt_kind, lane), limit);
             const bool live = load_relaxed(ctr_at(counters, fetch_kind, lane)) < c;
             const uint64_t m = __ballot(live);
             if (m == 0) return false;
@@ -164,6 +168,8 @@ __global__ __launch_bounds__(kBlock) void k_raygen(const PassHeader* __restrict_
         int n = n_paths - sl * region;
         *ctr_at(counters, qslot, sl) = n < 0 ? 0 : (n > region ? region : n);
         *ctr_at(counters, kFetchExtend, sl) = 0;
+        *ctr_at(counters, kCntCont, sl) = 0;
+        *ctr_at(counters, kFetchCont, sl) = 0;
         for (int c = 0; c < kNumClasses; ++c) {
             *ctr_at(counters, kCntClass0 + c, sl) = 0;
             *ctr_at(counters, kFetchClass0 + c, sl) = 0;
@@ -303,14 +309,89 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const uint16_t* __restr
 // shallow tree leaves room for more resident waves.
 extern __shared__ uint32_t stk_dyn[];
 
-template <bool COUNT, bool POOLED>
+// Extension rays whose search is suspended (the stragglers of a wave, see
+// trace_closest_run) are stored per slice for k_extend_cont: record k of
+// slice s sits at s * room + k; its hit so far and (launch position, queue
+// slot, next item, stack depth) are one float4 and one int4, its stack entry
+// e is word e * total + s * room + k.  Appends beyond a slice's room are not
+// stored: those lanes finish their search in k_extend.
+struct ContBufs {
+    float4* hit;
+    int4* state;
+    uint32_t* stack;
+    int room;      // records per slice
+    size_t total;  // room * kSlices
+};
+// A wave's searching lanes suspend once fewer than this many are left; passes
+// whose trees need fewer than kSuspendMinStack stack entries (a few thousand
+// triangles at most) do not suspend, their searches are short.
+constexpr int kSuspendLanes = 16;
+constexpr int kSuspendMinStack = 12;
+static_assert(kContStack == (size_t)kSuspendMaxStack, "stored stack depth");
+
+// What extend does with a finished search: the hit record, the ray copied to
+// its sorted position (after a ray sort), the shading class; a miss adds the
+// environment's emission to the path's sums.  cls = -1 when nothing is appended.
+template <bool POOLED>
+__device__ __forceinline__ void extend_finish(const DevScene& sc, const Hit& h, int p, float4 o, float4 d, int i,
+                                              const float4* __restrict__ qw, float4* __restrict__ hits,
+                                              float4* slot_d, float4* slot_i, const int* __restrict__ perm,
+                                              float4* __restrict__ sorted_o, float4* __restrict__ sorted_d,
+                                              int& cls, bool& miss) {
+    miss = h.prim < 0;
+    if (!miss) {
+        hits[p] = make_float4(h.t, h.b1, h.b2, i_as_f(h.prim));
+        if (perm) {
+            sorted_o[p] = o;
+            sorted_d[p] = d;
+        }
+        cls = shade_class_of(sc.tri_attr, h.prim);
+    } else if (sc.env_light >= 0) {
+        const float4 w = qw[i];
+        const vec3 e = cmul(v3(w.x, w.y, w.z), env_emission(sc, v3(d.x, d.y, d.z)));
+        float4* sum = (f_as_i(d.w) & kFlagFirst) ? slot_d : slot_i;
+        const int pid = f_as_i(w.w);
+        float4 a = sum[pid];
+        a.x += e.x;
+        a.y += e.y;
+        a.z += e.z;
+        sum[pid] = a;
+    }
+}
+
+// Class-list appends and counters of one wave's finished rays.
+template <bool COUNT>
+__device__ __forceinline__ void extend_append(int cls, bool miss, int p, int slice, int region, int* lists, int cap,
+                                              int* counters, unsigned long long* stats, const TravCount& c) {
+#pragma unroll
+    for (int k = 0; k < kNumClasses; ++k) {
+        if (__ballot(cls == k) == 0ull) continue;
+        int pos = wave_append(cls == k, ctr_at(counters, kCntClass0 + k, slice));
+        if (cls == k) lists[(size_t)k * cap + (size_t)slice * region + pos] = p;
+    }
+    if (COUNT) {
+        wave_add_u64(&stats[kStatExtNodes], c.nodes);
+        wave_add_u64(&stats[kStatExtTris], c.tris);
+        const uint64_t mm = __ballot(miss);
+        if ((threadIdx.x & (kWave - 1)) == 0 && mm) atomicAdd(&stats[kStatMisses], (unsigned long long)__popcll(mm));
+#pragma unroll
+        for (int k = 0; k < kNumClasses; ++k) {
+            const uint64_t m = __ballot(cls == k);
+            if ((threadIdx.x & (kWave - 1)) == 0 && m)
+                atomicAdd(&stats[kStatClassPaths + k], (unsigned long long)__popcll(m));
+        }
+    }
+}
+
+template <bool COUNT, bool POOLED, int SUSP>
 __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jobs, const float4* __restrict__ qo,
                                                     const float4* __restrict__ qd, const float4* __restrict__ qw,
                                                     float4* __restrict__ hits, int* __restrict__ lists, int cap,
                                                     float4* slot_d, float4* slot_i,
                                                     const PassHeader* __restrict__ ph, int* counters, int qslot,
                                                     unsigned long long* stats, const int* __restrict__ perm,
-                                                    float4* __restrict__ sorted_o, float4* __restrict__ sorted_d) {
+                                                    float4* __restrict__ sorted_o, float4* __restrict__ sorted_d,
+                                                    ContBufs cont) {
     const int region = ph->region;
     uint32_t* stk = stk_dyn;
     if (blockIdx.x == 0 && threadIdx.x < kSlices) {
@@ -333,51 +414,83 @@ __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jo
         int i = p;
         TravCount c{0, 0};
         int cls = -1;
+        bool miss = false, susp = false;
+        const bool live = j < cur.count;
+        float4 o = make_float4(0.f, 0.f, 0.f, 0.f), d = o;
+        if (live) {
+            if (perm) i = cur.slice * region + perm[p];
+            o = qo[i];
+            d = qd[i];
+        }
+        const DevScene& sc = pass_job<POOLED>(jobs, live ? (int)((uint32_t)f_as_i(d.w) >> kJobShift) : 0).sc;
+        const RayInv r = make_ray(v3(o.x, o.y, o.z), v3(d.x, d.y, d.z));
+        TravState st = trav_start(sc.root_ref, 1e8f);
+        if (live) susp = trace_closest_run_wave<COUNT, true, false, SUSP>(sc, r, 0.f, col, kBlock, c, st);
+        if (SUSP > 0 && __ballot(susp) != 0ull) {
+            const int pos = wave_append(susp, ctr_at(counters, kCntCont, cur.slice));
+            if (susp) {
+                if (pos < cont.room) {
+                    const size_t k = (size_t)cur.slice * cont.room + pos;
+                    cont.hit[k] = make_float4(st.h.t, st.h.b1, st.h.b2, i_as_f(st.h.prim));
+                    cont.state[k] = make_int4(p, i, (int)st.ref, st.sp);
+                    for (int e = 0; e < st.sp; ++e) cont.stack[(size_t)e * cont.total + k] = col[e * kBlock];
+                } else {
+                    trace_closest_run<COUNT>(sc.nodes, sc.tri_isect, r, 0.f, col, kBlock, c, st);
+                    susp = false;
+                }
+            }
+        }
+        if (live && !susp) extend_finish<POOLED>(sc, st.h, p, o, d, i, qw, hits, slot_d, slot_i, perm, sorted_o,
+                                                 sorted_d, cls, miss);
+        extend_append<COUNT>(cls, miss, p, cur.slice, region, lists, cap, counters, stats, c);
+    }
+}
+
+// The suspended searches of k_extend, gathered 64 to a wave: each lane loads
+// a record, puts its stack back into its LDS column and resumes the search
+// where it stopped (the same tests in the same order, so the same hit), then
+// finishes the ray as k_extend does, into the same slice's lists.
+template <bool COUNT, bool POOLED>
+__global__ __launch_bounds__(kBlock) void k_extend_cont(const JobDev* __restrict__ jobs,
+                                                         const float4* __restrict__ qo, const float4* __restrict__ qd,
+                                                         const float4* __restrict__ qw, float4* __restrict__ hits,
+                                                         int* __restrict__ lists, int cap, float4* slot_d,
+                                                         float4* slot_i, const PassHeader* __restrict__ ph,
+                                                         int* counters, unsigned long long* stats,
+                                                         const int* __restrict__ perm, float4* __restrict__ sorted_o,
+                                                         float4* __restrict__ sorted_d, ContBufs cont) {
+    const int region = ph->region;
+    uint32_t* col = stk_dyn + threadIdx.x;
+    SliceCursor cur;
+    cur.init(counters, kCntCont, kFetchCont, cont.room);
+    int base;
+    while (cur.next(base)) {
+        const int j = base + (int)(threadIdx.x & (kWave - 1));
+        TravCount c{0, 0};
+        int cls = -1, p = 0;
         bool miss = false;
         if (j < cur.count) {
-            if (perm) i = cur.slice * region + perm[p];
-            float4 o = qo[i], d = qd[i];
+            const size_t k = (size_t)cur.slice * cont.room + j;
+            const int4 s = cont.state[k];
+            const float4 hh = cont.hit[k];
+            p = s.x;
+            const int i = s.y;
+            for (int e = 0; e < s.w; ++e) col[e * kBlock] = cont.stack[(size_t)e * cont.total + k];
+            const float4 o = qo[i], d = qd[i];
             const DevScene& sc = pass_job<POOLED>(jobs, (int)((uint32_t)f_as_i(d.w) >> kJobShift)).sc;
-            RayInv r = make_ray(v3(o.x, o.y, o.z), v3(d.x, d.y, d.z));
-            Hit h = trace_closest_wave<COUNT>(sc, r, 0.f, 1e8f, col, kBlock, c);
-            miss = h.prim < 0;
-            if (!miss) {
-                hits[p] = make_float4(h.t, h.b1, h.b2, i_as_f(h.prim));
-                if (perm) {
-                    sorted_o[p] = o;
-                    sorted_d[p] = d;
-                }
-                cls = shade_class_of(sc.tri_attr, h.prim);
-            } else if (sc.env_light >= 0) {
-                const float4 w = qw[i];
-                const vec3 e = cmul(v3(w.x, w.y, w.z), env_emission(sc, v3(d.x, d.y, d.z)));
-                float4* sum = (f_as_i(d.w) & kFlagFirst) ? slot_d : slot_i;
-                const int pid = f_as_i(w.w);
-                float4 a = sum[pid];
-                a.x += e.x;
-                a.y += e.y;
-                a.z += e.z;
-                sum[pid] = a;
-            }
+            const RayInv r = make_ray(v3(o.x, o.y, o.z), v3(d.x, d.y, d.z));
+            TravState st;
+            st.h.t = hh.x;
+            st.h.b1 = hh.y;
+            st.h.b2 = hh.z;
+            st.h.prim = f_as_i(hh.w);
+            st.ref = (uint32_t)s.z;
+            st.sp = s.w;
+            trace_closest_run_wave<COUNT>(sc, r, 0.f, col, kBlock, c, st);
+            extend_finish<POOLED>(sc, st.h, p, o, d, i, qw, hits, slot_d, slot_i, perm, sorted_o, sorted_d, cls,
+                                  miss);
         }
-#pragma unroll
-        for (int k = 0; k < kNumClasses; ++k) {
-            if (__ballot(cls == k) == 0ull) continue;
-            int pos = wave_append(cls == k, ctr_at(counters, kCntClass0 + k, cur.slice));
-            if (cls == k) lists[(size_t)k * cap + (size_t)cur.slice * region + pos] = p;
-        }
-        if (COUNT) {
-            wave_add_u64(&stats[kStatExtNodes], c.nodes);
-            wave_add_u64(&stats[kStatExtTris], c.tris);
-            const uint64_t mm = __ballot(miss);
-            if ((threadIdx.x & (kWave - 1)) == 0 && mm) atomicAdd(&stats[kStatMisses], (unsigned long long)__popcll(mm));
-#pragma unroll
-            for (int k = 0; k < kNumClasses; ++k) {
-                const uint64_t m = __ballot(cls == k);
-                if ((threadIdx.x & (kWave - 1)) == 0 && m)
-                    atomicAdd(&stats[kStatClassPaths + k], (unsigned long long)__popcll(m));
-            }
-        }
+        extend_append<COUNT>(cls, miss, p, cur.slice, region, lists, cap, counters, stats, c);
     }
 }
 
@@ -568,6 +681,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jo
         // The queue and the class lists of this bounce are consumed: reset their
         // fetch counters and counts for the next extend.
         *ctr_at(counters, kFetchExtend, sl) = 0;
+        *ctr_at(counters, kCntCont, sl) = 0;
+        *ctr_at(counters, kFetchCont, sl) = 0;
         for (int c = 0; c < kNumClasses; ++c) {
             *ctr_at(counters, kCntClass0 + c, sl) = 0;
             *ctr_at(counters, kFetchClass0 + c, sl) = 0;
@@ -1558,6 +1673,11 @@ Engine::Engine(size_t capacity_paths, bool profile) {
     shadow_.alloc(((cap_ + 63) & ~(size_t)63) * 64);
     slot_d_.alloc(cap_ * 16);
     slot_i_.alloc(cap_ * 16);
+    cont_room_ = (int)std::max<size_t>(1, cap_ / (kContPathsPer * kSlices));
+    const size_t cont_total = (size_t)cont_room_ * kSlices;
+    cont_hit_.alloc(cont_total * 16);
+    cont_state_.alloc(cont_total * 16);
+    cont_stack_.alloc(cont_total * 4 * kContStack);
     counters_.alloc(kCounterInts * sizeof(int));
     stats_.alloc(kNumStats * sizeof(unsigned long long));
     tables_.alloc(kPassTableBytes);
@@ -1592,7 +1712,9 @@ Engine::Grids Engine::grids_for(int stack_entries) {
     if (it != grids_.end()) return it->second;
     const size_t lds = (size_t)stack_entries * kBlock * sizeof(uint32_t);
     Grids g;
-    g.ext = persistent_grid(k_extend<false, POOLED>, cus_, lds);
+    g.ext = persistent_grid(k_extend<false, POOLED, 0>, cus_, lds);
+    g.ext_susp = persistent_grid(k_extend<false, POOLED, kSuspendLanes>, cus_, lds);
+    g.ext_cont = persistent_grid(k_extend_cont<false, POOLED>, cus_, lds);
     g.shadow = persistent_grid(k_shadow<false, POOLED>, cus_, lds);
     g.shadow_early = persistent_grid(k_shadow<false, POOLED, kShadowEarly>, cus_, lds);
     g.shadow_segment = persistent_grid(k_shadow<false, POOLED, kShadowSegment>, cus_, lds);
@@ -1737,7 +1859,15 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
     const Grids G = grids_for<POOLED>(stack_entries);
     const size_t lds = (size_t)stack_entries * kBlock * sizeof(uint32_t);
     auto scaled = [&](int g) { return std::max(1, (int)((float)g * grid_scale_ + 0.5f)); };
-    const int gext = scaled(G.ext);
+    const bool suspend = stack_entries >= kSuspendMinStack;
+    const int gext = scaled(suspend ? G.ext_susp : G.ext);
+    const int gext_cont = scaled(G.ext_cont);
+    ContBufs cont;
+    cont.hit = cont_hit_.as<float4>();
+    cont.state = cont_state_.as<int4>();
+    cont.stack = cont_stack_.as<uint32_t>();
+    cont.room = cont_room_;
+    cont.total = (size_t)cont_room_ * kSlices;
     const int gsh = scaled(shadow_mode_ == kShadowEarly     ? G.shadow_early
                            : shadow_mode_ == kShadowSegment ? G.shadow_segment
                                                             : G.shadow);
@@ -1767,16 +1897,21 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
             perm = perm_.as<int>();
         }
         b = prof.begin(stream_);
-        if (count)
-            k_extend<true, POOLED><<<gext, kBlock, lds, stream_>>>(
+        auto extend = [&](auto count_tag) {
+            constexpr bool C = decltype(count_tag)::value;
+            float4* so = perm ? sorted_o_.as<float4>() : nullptr;
+            float4* sd = perm ? sorted_d_.as<float4>() : nullptr;
+            auto ext = suspend ? k_extend<C, POOLED, kSuspendLanes> : k_extend<C, POOLED, 0>;
+            ext<<<gext, kBlock, lds, stream_>>>(jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), qw_[q].as<float4>(),
+                                                hits_.as<float4>(), lists_.as<int>(), cap, slot_d_.as<float4>(),
+                                                slot_i_.as<float4>(), hdr, ctr, q, st, perm, so, sd, cont);
+            if (!suspend) return;
+            k_extend_cont<C, POOLED><<<gext_cont, kBlock, lds, stream_>>>(
                 jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), qw_[q].as<float4>(), hits_.as<float4>(),
-                lists_.as<int>(), cap, slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, q, st, perm,
-                perm ? sorted_o_.as<float4>() : nullptr, perm ? sorted_d_.as<float4>() : nullptr);
-        else
-            k_extend<false, POOLED><<<gext, kBlock, lds, stream_>>>(
-                jobs, qo_[q].as<float4>(), qd_[q].as<float4>(), qw_[q].as<float4>(), hits_.as<float4>(),
-                lists_.as<int>(), cap, slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, q, st, perm,
-                perm ? sorted_o_.as<float4>() : nullptr, perm ? sorted_d_.as<float4>() : nullptr);
+                lists_.as<int>(), cap, slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, st, perm, so, sd, cont);
+        };
+        if (count) extend(std::true_type());
+        else extend(std::false_type());
         HIP_OK(hipGetLastError());
         prof.end(kStageExtend, b, stream_);
         b = prof.begin(stream_);

@@ -109,27 +109,43 @@ using LdsU32 = __attribute__((address_space(3))) uint32_t;
 // few lanes that reached leaves while the others wait, which made the
 // triangle tests most of the instructions issued.
 constexpr int kEarlyExit = -2;
-// The tree is given by its packed nodes, its triangle records and its root
-// reference (the scene's, or a random-walk medium's own; trace_closest below
-// takes the scene's).
-template <bool COUNT, bool CULL = true, int OCT = -1, bool EARLY = false>
-__device__ __forceinline__ Hit trace_closest_tree(const PackedNode* tree_nodes, const float4* tree_tris,
-                                                  uint32_t root_ref, const RayInv& r, float tmin, float tmax,
-                                                  uint32_t* stk_generic, int stride, TravCount& cnt,
-                                                  float t_exit = 0.f) {
+
+// Where a closest-hit search stands between two loop iterations: the hit so
+// far, the next item to visit (an interior node, or a leaf reference whose
+// first triangle and count name the leaf's triangles still to be tested) and
+// the number of entries on the lane's stack.  A search started from this
+// state makes the same tests in the same order as one that never stopped, so
+// a search can be suspended, its state and stack stored, and resumed later on
+// another lane of another launch with the same result.
+struct TravState {
+    Hit h;
+    uint32_t ref;
+    int sp;
+};
+// A suspended search keeps at most this many stack entries (deeper lanes do
+// not suspend).
+constexpr int kSuspendMaxStack = 16;
+
+// The tree is given by its packed nodes, its triangle records and the state
+// to start from (see trace_closest_tree below for a search from the root).
+// SUSPEND > 0: at the end of an iteration, when fewer than SUSPEND lanes of
+// the wave are still searching, every searching lane whose stack holds at
+// most kSuspendMaxStack entries stops, leaves its state in `st` (its stack
+// stays in the column) and the function returns true; the wave's other
+// lanes finished long ago, so the stragglers can be gathered into full waves.
+template <bool COUNT, bool CULL = true, int OCT = -1, bool EARLY = false, int SUSPEND = 0>
+__device__ __forceinline__ bool trace_closest_run(const PackedNode* tree_nodes, const float4* tree_tris,
+                                                  const RayInv& r, float tmin, uint32_t* stk_generic, int stride,
+                                                  TravCount& cnt, TravState& st, float t_exit = 0.f) {
     LdsU32* stk = (LdsU32*)stk_generic;
     GlobalV4* nodes = (GlobalV4*)tree_nodes;  // 4 vectors per node
     GlobalV4* tris = (GlobalV4*)tree_tris;
-    Hit h;
-    h.t = tmax;
-    h.b1 = 0.f;
-    h.b2 = 0.f;
-    h.prim = -1;
-    uint32_t ref = root_ref;
+    Hit h = st.h;
+    uint32_t ref = st.ref;
     // In a leaf: triangles tn .. te-1 are still to be tested.
     bool leaf = (ref & kLeafBit) != 0;
     uint32_t tn = (ref & ~kLeafBit) >> 3, te = tn + (ref & 7u);
-    int sp = 0;
+    int sp = st.sp;
     while (true) {
         uint32_t next;
         if (leaf) {
@@ -145,13 +161,13 @@ __device__ __forceinline__ Hit trace_closest_tree(const PackedNode* tree_nodes, 
                         vec3 n = cross(ve1, ve2);
                         if (det * det >= 1e-6f * dot(n, n)) {
                             h.prim = kEarlyExit;
-                            return h;
+                            break;
                         }
                     }
                 }
                 ++tn;
             }
-            if (tn < te) continue;
+            if (tn < te) goto latch;
             if (sp == 0) break;
             next = stk[--sp * stride];
         } else {
@@ -178,8 +194,44 @@ __device__ __forceinline__ Hit trace_closest_tree(const PackedNode* tree_nodes, 
         ref = next;
         tn = (next & ~kLeafBit) >> 3;
         te = tn + (next & 7u);
+    latch:
+        // The check sits on the loop's back edge, where the state is the next
+        // iteration's, so stopping costs no copies of it.
+        // A wave-uniform test: all searching lanes leave the loop together.
+        if (SUSPEND > 0 && __builtin_amdgcn_readfirstlane(__builtin_popcountll(__ballot(1))) < SUSPEND) {
+            st.h = h;
+            st.ref = leaf ? leaf_ref(tn, te - tn) : ref;
+            st.sp = sp;
+            if (sp <= kSuspendMaxStack) return true;
+            // Too deep a stack to store: finish here.
+            return trace_closest_run<COUNT, CULL, OCT, EARLY, 0>(tree_nodes, tree_tris, r, tmin, stk_generic, stride,
+                                                                 cnt, st, t_exit);
+        }
     }
-    return h;
+    st.h = h;
+    return false;
+}
+
+// A search from the root of a tree (the scene's, or a random-walk medium's
+// own; trace_closest below takes the scene's).
+RT_HD TravState trav_start(uint32_t root_ref, float tmax) {
+    TravState st;
+    st.h.t = tmax;
+    st.h.b1 = 0.f;
+    st.h.b2 = 0.f;
+    st.h.prim = -1;
+    st.ref = root_ref;
+    st.sp = 0;
+    return st;
+}
+
+template <bool COUNT, bool CULL = true, int OCT = -1, bool EARLY = false>
+__device__ __forceinline__ Hit trace_closest_tree(const PackedNode* tree_nodes, const float4* tree_tris,
+                                                  uint32_t root_ref, const RayInv& r, float tmin, float tmax,
+                                                  uint32_t* stk, int stride, TravCount& cnt, float t_exit = 0.f) {
+    TravState st = trav_start(root_ref, tmax);
+    trace_closest_run<COUNT, CULL, OCT, EARLY>(tree_nodes, tree_tris, r, tmin, stk, stride, cnt, st, t_exit);
+    return st.h;
 }
 
 template <bool COUNT, bool CULL = true, int OCT = -1, bool EARLY = false>
@@ -259,28 +311,39 @@ __device__ __forceinline__ bool trace_occluded(const DevScene& sc, const RayInv&
 // Octant of a ray as box_hit uses it (bit k: inverse direction negative on axis k).
 RT_HD int ray_octant(const RayInv& r) { return (r.nx ? 1 : 0) | (r.ny ? 2 : 0) | (r.nz ? 4 : 0); }
 
-// trace_closest with the slab selects resolved at compile time when every
+// trace_closest_run with the slab selects resolved at compile time when every
 // active lane of the wave has the same octant (after the ray sort most waves
 // do, and so do most camera rays); otherwise the general loop.  Same hits
 // either way.
+template <bool COUNT, bool CULL = true, bool EARLY = false, int SUSPEND = 0>
+__device__ __forceinline__ bool trace_closest_run_wave(const DevScene& sc, const RayInv& r, float tmin, uint32_t* stk,
+                                                       int stride, TravCount& cnt, TravState& st, float t_exit = 0.f) {
+    const int oct = ray_octant(r);
+    const int first = __builtin_amdgcn_readfirstlane(oct);
+#define RT_RUN(O) \
+    trace_closest_run<COUNT, CULL, O, EARLY, SUSPEND>(sc.nodes, sc.tri_isect, r, tmin, stk, stride, cnt, st, t_exit)
+    if (__ballot(oct != first) == 0ull) {
+        switch (first) {
+            case 0: return RT_RUN(0);
+            case 1: return RT_RUN(1);
+            case 2: return RT_RUN(2);
+            case 3: return RT_RUN(3);
+            case 4: return RT_RUN(4);
+            case 5: return RT_RUN(5);
+            case 6: return RT_RUN(6);
+            default: return RT_RUN(7);
+        }
+    }
+    return RT_RUN(-1);
+#undef RT_RUN
+}
+
 template <bool COUNT, bool CULL = true, bool EARLY = false>
 __device__ __forceinline__ Hit trace_closest_wave(const DevScene& sc, const RayInv& r, float tmin, float tmax,
                                                   uint32_t* stk, int stride, TravCount& cnt, float t_exit = 0.f) {
-    const int oct = ray_octant(r);
-    const int first = __builtin_amdgcn_readfirstlane(oct);
-    if (__ballot(oct != first) == 0ull) {
-        switch (first) {
-            case 0: return trace_closest<COUNT, CULL, 0, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
-            case 1: return trace_closest<COUNT, CULL, 1, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
-            case 2: return trace_closest<COUNT, CULL, 2, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
-            case 3: return trace_closest<COUNT, CULL, 3, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
-            case 4: return trace_closest<COUNT, CULL, 4, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
-            case 5: return trace_closest<COUNT, CULL, 5, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
-            case 6: return trace_closest<COUNT, CULL, 6, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
-            default: return trace_closest<COUNT, CULL, 7, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
-        }
-    }
-    return trace_closest<COUNT, CULL, -1, EARLY>(sc, r, tmin, tmax, stk, stride, cnt, t_exit);
+    TravState st = trav_start(sc.root_ref, tmax);
+    trace_closest_run_wave<COUNT, CULL, EARLY>(sc, r, tmin, stk, stride, cnt, st, t_exit);
+    return st.h;
 }
 
 // Shading data at a hit, reconstructed from barycentrics the way the

@@ -26,6 +26,9 @@ def test_stage_instances():
     assert bench._stage_of("rt::k_extend<false, false>") == "k_extend"
     assert bench._stage_of("rt::k_extend<true, false>") is None       # counting render
     assert bench._stage_of("rt::k_extend<false, true>") is None       # pooled
+    assert bench._stage_of("rt::k_extend<false, false, 16>") == "k_extend"  # with straggler suspension
+    assert bench._stage_of("rt::k_extend_cont<false, false>") == "k_extend"
+    assert bench._stage_of("rt::k_extend_cont<true, false>") is None
     assert bench._stage_of("rt::k_shadow<false, false, 0>") == "k_shadow"
     assert bench._stage_of("rt::k_shadow<false, false, 2>") is None   # another shadow test
     assert bench._stage_of("rt::k_shade<false>") == "k_shade"
@@ -42,6 +45,13 @@ def test_traffic_per_bounce_and_scaling(tmp_path):
     shadow = [("void rt::k_shadow<false, false, 0>(args)", 40 + d, 60.0) for d in range(6)]
     fetch = _db(tmp_path / "f.db", ext + shade + shadow)
     write = _db(tmp_path / "w.db", [(k, d, v / 10) for k, d, v in ext + shade + shadow])
+    # A render whose extend launches suspend stragglers: each bounce is a k_extend
+    # launch plus its continuation launch, counted together per bounce.
+    ext2 = [("void rt::k_extend<false, false, 16>(args)", d, 80.0) for d in range(6)]
+    ext2 += [("void rt::k_extend_cont<false, false>(args)", 50 + d, 20.0) for d in range(6)]
+    ext2 += [("void rt::k_extend<true, false, 16>(args)", 10 + d, 500.0) for d in range(3)]
+    fetch2 = _db(tmp_path / "f2.db", ext2 + shade + shadow)
+    write2 = _db(tmp_path / "w2.db", [(k, d, v / 10) for k, d, v in ext2 + shade + shadow])
     gib = 1 << 30
     # The copy kernel reads half its bytes on FETCH_SIZE and all of them on WRITE_SIZE.
     cf = _db(tmp_path / "cf.db", [("__amd_rocclr_copyBuffer", 1, gib / 1024 / 2)])
@@ -54,6 +64,10 @@ def test_traffic_per_bounce_and_scaling(tmp_path):
     # shade: 9 dispatches over 9 bounces (6 plain + 3 counting)
     assert rows["k_shade"][1:] == ["9", str(80 * 1024), str(4 * 1024), str(84 * 1024)]
     assert rows["k_shadow"][1:] == ["6", str(120 * 1024), str(6 * 1024), str(126 * 1024)]
+    out2 = bench.traffic_from_pmc(fetch2, write2, cf, cw)
+    rows2 = {ln.split()[0]: ln.split() for ln in out2.splitlines() if ln.startswith("k_")}
+    assert rows2["k_extend"][1:] == rows["k_extend"][1:]
+    assert rows2["k_shade"][1:] == rows["k_shade"][1:]
     (tmp_path / "traffic.txt").write_text("# header\n" + out + "\n")
     assert bench.load_traffic(str(tmp_path / "traffic.txt")) == {
         "k_extend": 210 * 1024, "k_shade": 84 * 1024, "k_shadow": 126 * 1024}
