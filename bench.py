@@ -72,16 +72,15 @@ C3_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_terrain_2M_1080p_32spp.txt"
 
 def _stage_of(kernel: str):
     """Stage of a measured kernel instance: only the plain (uncounted,
-    single-scene) instantiations the timed renders launch.  The extend stage
-    is k_extend (with or without straggler suspension) and the continuation
-    launch that finishes its suspended searches."""
+    single-scene) instantiations the timed renders launch.  The extend and
+    shadow stages are their kernels (with or without straggler suspension)
+    and the continuation launches that finish the suspended searches."""
     import re
 
     if re.match(r"(?:rt::)?(?:k_extend<false, false(?:, \d+)?|k_extend_cont<false, false)>$", kernel):
         return "k_extend"
-    m = re.match(r"(?:rt::)?(k_shadow<false, false(?:, (?:false|0))?)>$", kernel)
-    if m:
-        return m.group(1).split("<")[0]
+    if re.match(r"(?:rt::)?(?:k_shadow<false, false(?:, (?:false|0)(?:, \d+)?)?|k_shadow_cont<false, false)>$", kernel):
+        return "k_shadow"
     if re.match(r"(?:rt::)?k_shade<(?:\d, )?false>$", kernel):
         return "k_shade"
     return None
@@ -184,10 +183,11 @@ def traffic_from_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_byte
     lines = [f"fetch_scale {f_scale:.4f}", f"write_scale {w_scale:.4f}", "",
              "k_shade is per bounce: its class kernels' bytes summed, divided by the bounces (the single-scene",
              "k_extend dispatches, counting renders included, as their shade launches are the same kernels);",
-             "k_extend is per bounce too: the k_extend launch and its continuation launch summed", "",
+             "k_extend and k_shadow are per bounce too: each launch and its continuation launch summed", "",
              "stage      dispatches  fetch_B_per_launch  write_B_per_launch  hbm_B_per_launch"]
     n_bounces = sum(len(fetch[k]) for k in fetch if re.match(r"(?:rt::)?k_extend<(?:true|false), false(?:, \d+)?>$", k))
     n_ext = sum(len(fetch[k]) for k in fetch if re.match(r"(?:rt::)?k_extend<false, false(?:, \d+)?>$", k))
+    n_sh = sum(len(fetch[k]) for k in fetch if _stage_of(k) == "k_shadow" and "_cont" not in k)
     for st in TRAFFIC_STAGES:
         fk = [k for k in fetch if _stage_of(k) == st]
         wk = [k for k in write if _stage_of(k) == st]
@@ -196,7 +196,8 @@ def traffic_from_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_byte
         if not nf or not nw:
             continue
         per = (n_bounces if st == "k_shade" and n_bounces else
-               n_ext if st == "k_extend" and n_ext else None)
+               n_ext if st == "k_extend" and n_ext else
+               n_sh if st == "k_shadow" and n_sh else None)
         fb = sum(sum(fetch[k].values()) for k in fk) * 1024 * f_scale / (per or nf)
         wb = sum(sum(write[k].values()) for k in wk) * 1024 * w_scale / (per or nw)
         lines.append(f"{st:10s} {per or nf:11d} {fb:19.0f} {wb:19.0f} {fb + wb:17.0f}")

@@ -151,7 +151,9 @@ enum CounterKind : int {
     kFetchClass0 = 10,  // kNumClasses per-class fetch cursors
     kCntCont = 15,      // suspended extension rays stored for the continuation launch
     kFetchCont = 16,    // and its fetch cursor
-    kNumCounterKinds = 17,
+    kCntShadowCont = 17,  // suspended shadow rays, and their fetch cursor
+    kFetchShadowCont = 18,
+    kNumCounterKinds = 19,
 };
 constexpr size_t kCounterInts = (size_t)kNumCounterKinds * kSlices * kCounterStride;
 

@@ -122,9 +122,10 @@ public:
 
 private:
     template <bool POOLED>
-    void run_t(int max_bounces, const bool* classes, bool count, int stack_entries);
+    void run_t(int max_bounces, const bool* classes, bool count, int stack_entries, bool suspend,
+               bool suspend_shadow);
     struct Grids {
-        int ext = 0, ext_susp = 0, ext_cont = 0, shadow = 0, shadow_early = 0, shadow_segment = 0, shade = 0, shade_stack = 0;
+        int ext = 0, ext_susp = 0, ext_cont = 0, shadow = 0, shadow_susp = 0, shadow_cont = 0, shadow_early = 0, shadow_segment = 0, shade = 0, shade_stack = 0;
     };
     template <bool POOLED>
     Grids grids_for(int stack_entries);

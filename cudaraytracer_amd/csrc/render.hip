@@ -327,6 +327,15 @@ struct ContBufs {
 // triangles at most) do not suspend, their searches are short.
 constexpr int kSuspendLanes = 16;
 constexpr int kSuspendMinStack = 12;
+// Shadow rays end at the light point, so their searches are shorter: they
+// suspend only in deeper trees (a 20 k-triangle scene of depth 18 loses 12 %
+// of its shadow time to the continuation launch, a 2 M-triangle terrain of
+// depth 26 gains 8 %).
+constexpr int kShadowSuspendMinStack = 24;
+// Each suspending stage adds a launch per bounce, which a small pass (a
+// real-time frame of 0.9 M paths: +0.5 ms per frame for each stage) does not
+// earn back; passes of fewer paths than this do not suspend.
+constexpr size_t kSuspendMinPaths = (size_t)1 << 24;
 static_assert(kContStack == (size_t)kSuspendMaxStack, "stored stack depth");
 
 // What extend does with a finished search: the hit record, the ray copied to
@@ -399,6 +408,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(const JobDev* __restrict__ jo
         *ctr_at(counters, qslot ^ 1, sl) = 0;
         *ctr_at(counters, kCntShadow, sl) = 0;
         *ctr_at(counters, kFetchShadow, sl) = 0;
+        *ctr_at(counters, kCntShadowCont, sl) = 0;
+        *ctr_at(counters, kFetchShadowCont, sl) = 0;
         int n = *ctr_at(counters, qslot, sl);
         if (n > 0) atomicAdd(&stats[kStatExtRays], (unsigned long long)n);
     }
@@ -668,10 +679,37 @@ __global__ __launch_bounds__(kBlock) void k_shade(const JobDev* __restrict__ job
 //    the reference's test also calls occluded, are not looked for: an
 //    approximation of mode 0 whose difference the tests bound.
 constexpr int kShadowClosest = 0, kShadowEarly = 1, kShadowSegment = 2;
-template <bool COUNT, bool POOLED, int MODE = kShadowClosest>
+// The reference's visibility decision from a finished closest-hit search of a
+// shadow record's ray; a visible light point adds the record's contribution
+// to its path's direct or indirect sum.
+__device__ __forceinline__ bool shadow_finish(const DevScene& sc, const float4& r0, const float4& r1, const float4& r2,
+                                              const float4& r3, const Hit& h, float4* slot_d, float4* slot_i) {
+    ShadowRec rec;
+    rec.lp = v3(r0.x, r0.y, r0.z);
+    rec.sdist = r0.w;
+    rec.dir = v3(r1.x, r1.y, r1.z);
+    rec.hp = v3(r2.x, r2.y, r2.z);
+    const bool hit = h.prim >= 0;
+    vec3 tp = v3(0.f, 0.f, 0.f);
+    if (hit) tp = position_at(sc.tri_attr, h.prim, h.b1, h.b2);
+    if (h.prim == kEarlyExit || !shadow_visible(rec, hit, tp)) return false;
+    float4* s = (f_as_i(r2.w) & 1) == 0 ? slot_d : slot_i;
+    const int pid = f_as_i(r1.w);
+    float4 a = s[pid];
+    a.x += r3.x;
+    a.y += r3.y;
+    a.z += r3.z;
+    s[pid] = a;
+    return true;
+}
+
+// SUSP > 0 (closest-hit mode only): stragglers are suspended as in k_extend
+// (their record index instead of the ray) and k_shadow_cont finishes them.
+template <bool COUNT, bool POOLED, int MODE = kShadowClosest, int SUSP = 0>
 __global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jobs, const float4* __restrict__ shadow,
                                                     float4* slot_d, float4* slot_i, const PassHeader* __restrict__ ph,
-                                                    int* counters, unsigned long long* stats) {
+                                                    int* counters, unsigned long long* stats, ContBufs cont) {
+    static_assert(SUSP == 0 || MODE == kShadowClosest, "only the closest-hit test suspends");
     const int region = ph->region;
     uint32_t* stk = stk_dyn;
     if (blockIdx.x == 0 && threadIdx.x < kSlices) {
@@ -696,7 +734,38 @@ __global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jo
         int j = base + (int)(threadIdx.x & (kWave - 1));
         TravCount c{0, 0};
         bool lit = false;
-        if (j < cur.count) {
+        if (SUSP > 0) {
+            const bool live = j < cur.count;
+            const size_t r = (size_t)cur.slice * region + j;
+            float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0, r3 = r0;
+            if (live) {
+                r0 = *shadow_field(shadow, r, 0);
+                r1 = *shadow_field(shadow, r, 1);
+                r2 = *shadow_field(shadow, r, 2);
+                r3 = *shadow_field(shadow, r, 3);
+            }
+            const DevScene& sc = pass_job<POOLED>(jobs, live ? f_as_i(r2.w) >> 1 : 0).sc;
+            const vec3 dir = v3(r1.x, r1.y, r1.z);
+            const RayInv ray = make_ray(v3(r0.x, r0.y, r0.z) + 0.01f * dir, dir);
+            TravState st = trav_start(sc.root_ref, 1e8f);
+            bool susp = false;
+            if (live) susp = trace_closest_run_wave<COUNT, true, false, SUSP>(sc, ray, 1e-6f, col, kBlock, c, st);
+            if (__ballot(susp) != 0ull) {
+                const int pos = wave_append(susp, ctr_at(counters, kCntShadowCont, cur.slice));
+                if (susp) {
+                    if (pos < cont.room) {
+                        const size_t k = (size_t)cur.slice * cont.room + pos;
+                        cont.hit[k] = make_float4(st.h.t, st.h.b1, st.h.b2, i_as_f(st.h.prim));
+                        cont.state[k] = make_int4((int)r, 0, (int)st.ref, st.sp);
+                        for (int e = 0; e < st.sp; ++e) cont.stack[(size_t)e * cont.total + k] = col[e * kBlock];
+                    } else {
+                        trace_closest_run<COUNT>(sc.nodes, sc.tri_isect, ray, 1e-6f, col, kBlock, c, st);
+                        susp = false;
+                    }
+                }
+            }
+            if (live && !susp) lit = shadow_finish(sc, r0, r1, r2, r3, st.h, slot_d, slot_i);
+        } else if (j < cur.count) {
             const size_t r = (size_t)cur.slice * region + j;
             const float4 r0 = *shadow_field(shadow, r, 0), r1 = *shadow_field(shadow, r, 1);
             const float4 r2 = *shadow_field(shadow, r, 2), r3 = *shadow_field(shadow, r, 3);
@@ -740,6 +809,51 @@ __global__ __launch_bounds__(kBlock) void k_shadow(const JobDev* __restrict__ jo
                 a.z += r3.z;
                 s[pid] = a;
             }
+        }
+        if (COUNT) {
+            wave_add_u64(&stats[kStatShadowNodes], c.nodes);
+            wave_add_u64(&stats[kStatShadowTris], c.tris);
+            const uint64_t m = __ballot(lit);
+            if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(&stats[kStatShadowLit], (unsigned long long)__popcll(m));
+        }
+    }
+}
+
+// The suspended shadow searches of k_shadow, 64 to a wave: each lane reloads
+// its shadow record, puts its stack back and resumes where it stopped.
+template <bool COUNT, bool POOLED>
+__global__ __launch_bounds__(kBlock) void k_shadow_cont(const JobDev* __restrict__ jobs,
+                                                         const float4* __restrict__ shadow, float4* slot_d,
+                                                         float4* slot_i, int* counters, unsigned long long* stats,
+                                                         ContBufs cont) {
+    uint32_t* col = stk_dyn + threadIdx.x;
+    SliceCursor cur;
+    cur.init(counters, kCntShadowCont, kFetchShadowCont, cont.room);
+    int base;
+    while (cur.next(base)) {
+        const int j = base + (int)(threadIdx.x & (kWave - 1));
+        TravCount c{0, 0};
+        bool lit = false;
+        if (j < cur.count) {
+            const size_t k = (size_t)cur.slice * cont.room + j;
+            const int4 s = cont.state[k];
+            const float4 hh = cont.hit[k];
+            const size_t r = (size_t)(uint32_t)s.x;
+            for (int e = 0; e < s.w; ++e) col[e * kBlock] = cont.stack[(size_t)e * cont.total + k];
+            const float4 r0 = *shadow_field(shadow, r, 0), r1 = *shadow_field(shadow, r, 1);
+            const float4 r2 = *shadow_field(shadow, r, 2), r3 = *shadow_field(shadow, r, 3);
+            const DevScene& sc = pass_job<POOLED>(jobs, f_as_i(r2.w) >> 1).sc;
+            const vec3 dir = v3(r1.x, r1.y, r1.z);
+            const RayInv ray = make_ray(v3(r0.x, r0.y, r0.z) + 0.01f * dir, dir);
+            TravState st;
+            st.h.t = hh.x;
+            st.h.b1 = hh.y;
+            st.h.b2 = hh.z;
+            st.h.prim = f_as_i(hh.w);
+            st.ref = (uint32_t)s.z;
+            st.sp = s.w;
+            trace_closest_run_wave<COUNT>(sc, ray, 1e-6f, col, kBlock, c, st);
+            lit = shadow_finish(sc, r0, r1, r2, r3, st.h, slot_d, slot_i);
         }
         if (COUNT) {
             wave_add_u64(&stats[kStatShadowNodes], c.nodes);
@@ -1716,6 +1830,8 @@ Engine::Grids Engine::grids_for(int stack_entries) {
     g.ext_susp = persistent_grid(k_extend<false, POOLED, kSuspendLanes>, cus_, lds);
     g.ext_cont = persistent_grid(k_extend_cont<false, POOLED>, cus_, lds);
     g.shadow = persistent_grid(k_shadow<false, POOLED>, cus_, lds);
+    g.shadow_susp = persistent_grid(k_shadow<false, POOLED, kShadowClosest, kSuspendLanes>, cus_, lds);
+    g.shadow_cont = persistent_grid(k_shadow_cont<false, POOLED>, cus_, lds);
     g.shadow_early = persistent_grid(k_shadow<false, POOLED, kShadowEarly>, cus_, lds);
     g.shadow_segment = persistent_grid(k_shadow<false, POOLED, kShadowSegment>, cus_, lds);
     g.shade = persistent_grid(k_shade<POOLED>, cus_);
@@ -1809,13 +1925,16 @@ void Engine::run(const std::vector<PassItem>& items, bool count) {
     HIP_OK(hipMemcpyAsync(tables_.p, host, kPassHeaderBytes + sizeof(JobDev) * n, hipMemcpyHostToDevice, stream_));
     HIP_OK(hipEventRecord(ring_ev_[slot], stream_));
     const bool pooled = n > 1;
+    // Straggler suspension (k_extend / k_shadow) for large passes over deep trees.
+    const bool suspend = stack >= kSuspendMinStack && paths >= kSuspendMinPaths;
+    const bool suspend_shadow = stack >= kShadowSuspendMinStack && paths >= kSuspendMinPaths;
     const bool graph = use_graphs_ && !prof.on && !count;
     uint64_t key = 0;
     if (graph) {
         int mask = 0;
         for (int c = 0; c < kNumClasses; ++c) mask |= classes[c] ? (1 << c) : 0;
         key = ((uint64_t)max_bounces << 24) | ((uint64_t)stack << 16) | ((uint64_t)mask << 8) | ((uint64_t)shadow_mode_ << 32) |
-              (ray_sort_ ? 2 : 0) | (pooled ? 1 : 0);
+              (suspend_shadow ? 8 : 0) | (suspend ? 4 : 0) | (ray_sort_ ? 2 : 0) | (pooled ? 1 : 0);
         auto it = graphs_.find(key);
         if (it != graphs_.end()) {
             HIP_OK(hipGraphLaunch(it->second, stream_));
@@ -1824,8 +1943,8 @@ void Engine::run(const std::vector<PassItem>& items, bool count) {
         HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     }
     try {
-        if (!pooled) run_t<false>(max_bounces, classes, count, stack);
-        else run_t<true>(max_bounces, classes, count, stack);
+        if (!pooled) run_t<false>(max_bounces, classes, count, stack, suspend, suspend_shadow);
+        else run_t<true>(max_bounces, classes, count, stack, suspend, suspend_shadow);
     } catch (...) {
         if (graph) {
             hipGraph_t g = nullptr;
@@ -1849,7 +1968,8 @@ void Engine::run(const std::vector<PassItem>& items, bool count) {
 }
 
 template <bool POOLED>
-void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_entries) {
+void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_entries, bool suspend,
+                   bool suspend_shadow) {
     const char* dev = tables_.as<char>();
     const PassHeader* hdr = reinterpret_cast<const PassHeader*>(dev);
     const JobDev* jobs = reinterpret_cast<const JobDev*>(dev + kPassHeaderBytes);
@@ -1859,9 +1979,11 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
     const Grids G = grids_for<POOLED>(stack_entries);
     const size_t lds = (size_t)stack_entries * kBlock * sizeof(uint32_t);
     auto scaled = [&](int g) { return std::max(1, (int)((float)g * grid_scale_ + 0.5f)); };
-    const bool suspend = stack_entries >= kSuspendMinStack;
+
     const int gext = scaled(suspend ? G.ext_susp : G.ext);
     const int gext_cont = scaled(G.ext_cont);
+    const int gsh_susp = scaled(G.shadow_susp);
+    const int gsh_cont = scaled(G.shadow_cont);
     ContBufs cont;
     cont.hit = cont_hit_.as<float4>();
     cont.state = cont_state_.as<int4>();
@@ -1926,8 +2048,17 @@ void Engine::run_t(int max_bounces, const bool* classes, bool count, int stack_e
         prof.end(kStageShade, b, stream_);
         b = prof.begin(stream_);
         auto shadow = [&](auto count_tag, auto mode_tag) {
-            k_shadow<decltype(count_tag)::value, POOLED, decltype(mode_tag)::value><<<gsh, kBlock, lds, stream_>>>(
-                jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, st);
+            constexpr bool C = decltype(count_tag)::value;
+            constexpr int M = decltype(mode_tag)::value;
+            if (M == kShadowClosest && suspend_shadow) {
+                k_shadow<C, POOLED, kShadowClosest, kSuspendLanes><<<gsh_susp, kBlock, lds, stream_>>>(
+                    jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, st, cont);
+                k_shadow_cont<C, POOLED><<<gsh_cont, kBlock, lds, stream_>>>(
+                    jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), ctr, st, cont);
+            } else {
+                k_shadow<C, POOLED, M><<<gsh, kBlock, lds, stream_>>>(
+                    jobs, shadow_.as<float4>(), slot_d_.as<float4>(), slot_i_.as<float4>(), hdr, ctr, st, cont);
+            }
         };
         auto shadow_mode = [&](auto count_tag) {
             if (shadow_mode_ == kShadowEarly) shadow(count_tag, std::integral_constant<int, kShadowEarly>());

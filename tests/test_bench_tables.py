@@ -29,6 +29,9 @@ def test_stage_instances():
     assert bench._stage_of("rt::k_extend<false, false, 16>") == "k_extend"  # with straggler suspension
     assert bench._stage_of("rt::k_extend_cont<false, false>") == "k_extend"
     assert bench._stage_of("rt::k_extend_cont<true, false>") is None
+    assert bench._stage_of("rt::k_shadow<false, false, 0, 16>") == "k_shadow"
+    assert bench._stage_of("rt::k_shadow_cont<false, false>") == "k_shadow"
+    assert bench._stage_of("rt::k_shadow<false, false, 2, 16>") is None
     assert bench._stage_of("rt::k_shadow<false, false, 0>") == "k_shadow"
     assert bench._stage_of("rt::k_shadow<false, false, 2>") is None   # another shadow test
     assert bench._stage_of("rt::k_shade<false>") == "k_shade"
@@ -50,8 +53,10 @@ def test_traffic_per_bounce_and_scaling(tmp_path):
     ext2 = [("void rt::k_extend<false, false, 16>(args)", d, 80.0) for d in range(6)]
     ext2 += [("void rt::k_extend_cont<false, false>(args)", 50 + d, 20.0) for d in range(6)]
     ext2 += [("void rt::k_extend<true, false, 16>(args)", 10 + d, 500.0) for d in range(3)]
-    fetch2 = _db(tmp_path / "f2.db", ext2 + shade + shadow)
-    write2 = _db(tmp_path / "w2.db", [(k, d, v / 10) for k, d, v in ext2 + shade + shadow])
+    shadow2 = [("void rt::k_shadow<false, false, 0, 16>(args)", 40 + d, 45.0) for d in range(6)]
+    shadow2 += [("void rt::k_shadow_cont<false, false>(args)", 60 + d, 15.0) for d in range(6)]
+    fetch2 = _db(tmp_path / "f2.db", ext2 + shade + shadow2)
+    write2 = _db(tmp_path / "w2.db", [(k, d, v / 10) for k, d, v in ext2 + shade + shadow2])
     gib = 1 << 30
     # The copy kernel reads half its bytes on FETCH_SIZE and all of them on WRITE_SIZE.
     cf = _db(tmp_path / "cf.db", [("__amd_rocclr_copyBuffer", 1, gib / 1024 / 2)])
@@ -68,6 +73,7 @@ def test_traffic_per_bounce_and_scaling(tmp_path):
     rows2 = {ln.split()[0]: ln.split() for ln in out2.splitlines() if ln.startswith("k_")}
     assert rows2["k_extend"][1:] == rows["k_extend"][1:]
     assert rows2["k_shade"][1:] == rows["k_shade"][1:]
+    assert rows2["k_shadow"][1:] == rows["k_shadow"][1:]
     (tmp_path / "traffic.txt").write_text("# header\n" + out + "\n")
     assert bench.load_traffic(str(tmp_path / "traffic.txt")) == {
         "k_extend": 210 * 1024, "k_shade": 84 * 1024, "k_shadow": 126 * 1024}
