@@ -2358,7 +2358,7 @@ Renderer::Renderer(const GpuScene& scene, const HostScene& host, const RenderSet
     // Paths per pass: the scene's samples per pixel (its JSON "spp", at most 64)
     // of every pixel when they fit the queue budget, so a renderer reserves
     // queues for what its scene asks for (a 1-spp real-time renderer at 1080p:
-    // 0.5 GB).  16 GiB by default: a 1920x1080 job of up to 36 samples per pixel
+    // 0.5 GB).  16 GiB by default: a 1920x1080 job of up to 32 samples per pixel
     // runs as one pass.  Renders of more samples run in several passes.
     size_t budget = s.queue_budget_bytes ? s.queue_budget_bytes : ((size_t)16 << 30);
     const size_t per_path = kQueueBytesPerPath + (s.ray_sort ? kSortBytesPerPath : 0);

@@ -141,7 +141,7 @@ typedef struct rt_render_params {
     int64_t seed;           /* < 0: scene value */
     int batch_spp;          /* samples per pass; 0 = automatic: the scene's spp (at most 64) when the queue
                                budget holds them (queues are sized for this at creation) */
-    int64_t queue_budget_bytes; /* 0 = 16 GiB; a renderer holds 228 B per path of a pass (batch_spp x pixels)
+    int64_t queue_budget_bytes; /* 0 = 16 GiB; a renderer holds 252 B per path of a pass (batch_spp x pixels)
                                    plus 32 B per pixel of accumulators */
     int flush_survivors;    /* 0 = reference: paths alive after max_bounces contribute nothing */
     int profile;            /* 1 = per-stage HIP event timing in rt_stats */
