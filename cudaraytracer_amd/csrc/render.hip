@@ -325,7 +325,7 @@ struct ContBufs {
 // A wave's searching lanes suspend once fewer than this many are left; passes
 // whose trees need fewer than kSuspendMinStack stack entries (a few thousand
 // triangles at most) do not suspend, their searches are short.
-constexpr int kSuspendLanes = 16;
+constexpr int kSuspendLanes = 12;
 constexpr int kSuspendMinStack = 12;
 // Shadow rays end at the light point, so their searches are shorter: they
 // suspend only in deeper trees (a 20 k-triangle scene of depth 18 loses 12 %
